@@ -1,0 +1,241 @@
+/*
+ * v2c.h -- C-ABI of the MI355X (gfx950) per-marker hot path of VELO2CYCLEs.
+ *
+ * The reference has no plugin layer: its hot path is a set of `void f(void)`
+ * routines that read and write static global arrays (headevps.c:33-250,
+ * input_RSF.c:3-11).  This header is the drop-in boundary that replaces them.
+ * A C host keeps its call sites and forwards each one to the entry point named
+ * in the comment above it; the reference globals are handed over as plain
+ * pointers and sizes.  No torch or HIP type appears in any signature.
+ *
+ * Conventions (identical to the reference):
+ *   - nodal arrays are FP64, length nodenum = xnumx*ynumy, indexed
+ *     m3 = m1*ynumy + m2 (x-major, y fastest; interpolation.c:267);
+ *   - marker arrays are FP64 structure-of-arrays, marker type is `char`
+ *     (headevps.c:210);
+ *   - times are in seconds, pressures in Pa, temperatures in K.
+ *
+ * Ownership: after v2c_markers_upload the device owns the marker state until
+ * v2c_markers_download.  Nodal arrays live in a device mirror; the host moves
+ * them with v2c_nodal_set / v2c_nodal_get (or the v2c_nodal_io helper).
+ *
+ * Every entry point returns 0 on success or a negative V2C_E* code; it never
+ * calls exit().  A context is not thread-safe: one host thread per context.
+ * Entry points are synchronous on return unless the context was switched to
+ * asynchronous mode with v2c_set_async (then only the calls that return a
+ * value to the host synchronise).
+ */
+#ifndef V2C_H
+#define V2C_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define V2C_ABI_VERSION 1
+#define V2C_MAX_TYPES 256
+
+/* ---- error codes ------------------------------------------------------- */
+#define V2C_OK 0
+#define V2C_EINVAL (-1)      /* bad argument, size or flag value            */
+#define V2C_ENOMEM (-2)      /* device or host allocation failed            */
+#define V2C_EHIP (-3)        /* HIP runtime error (message in last_error)   */
+#define V2C_ECAPACITY (-4)   /* marker count above the context capacity     */
+#define V2C_EUNSUPPORTED (-5)/* mode the build does not implement           */
+#define V2C_ENOGPU (-6)      /* no usable gfx950 device                      */
+
+/* ---- grid (headevps.c:36-40,91) ---------------------------------------- */
+typedef struct v2c_grid {
+    long xnumx, ynumy;      /* number of gridlines in x / y                 */
+    const double *gx;       /* x gridline coordinates [xnumx], increasing   */
+    const double *gy;       /* y gridline coordinates [ynumy], increasing   */
+    double xsize, ysize;    /* model size (m)                               */
+} v2c_grid;
+
+/* ---- run flags and hard-wired constants ----------------------------------
+ * Flags are the reference's mode_*.t3c switches (load_save.c:84-154).
+ * Physical constants that the reference hard-codes are exposed as fields,
+ * each defaulting (v2c_params_default) to the reference value. */
+typedef struct v2c_params {
+    int markmod;            /* 1 Euler, else RK4; 2 = RK4-averaged spin     */
+    int stoksmod;           /* 1 exact rotation, 2 Jaumann                  */
+    int outgrid;            /* 0 move out, 1 stay, 2 orthogonal only        */
+    int densimod;           /* 0 constant density, else P-T dependent       */
+    int interpolationmod;   /* only 2 is implemented                        */
+    int antidiffinter;      /* 1 = antidiffusion correction                  */
+    int rkmod;              /* Picard (0) / RK variants (1..3)               */
+    int RSFmode;            /* 1 rate-and-state, 0 Drucker-Prager           */
+    int power_law_RSF;      /* >0 power-law RSF form                        */
+    int reloadfromDP;
+    int compressiblemode;
+    int stickymod;
+    int inertyn;            /* 0 = radiation damping on                     */
+    int pmode;              /* 0 = constant normal stress nstress           */
+    int tempmod;            /* heat solve active (n2m incoming-marker rule) */
+    int n2m_phantom;        /* 1 = replicate the extra marker that the
+                               reference's `m3<=marknum` loop visits
+                               (interpolation.c:937): a zero marker of type 0
+                               at (0,0) that scatters into the antidiffusion
+                               sums                                          */
+    double nubeg, nuend;    /* global viscosity limits (load_save.c:117-118) */
+    double tdeep, zdeep;    /* deep-marker temperature reset (load_save.c:145-146) */
+    double incoming_T;      /* the reference's eps[2], used as T of newly
+                               incoming markers when tempmod==0
+                               (interpolation.c:979)                         */
+    /* RSF set-up (input_RSF.c:26-59) */
+    double nstress, Dphis;
+    double half_range, start_updip, end_updip, start_downdip, end_downdip;
+    double a_vw, b_vw, L_vw, V0_vw;
+    double a_vs, b_vs, L_vs, V0_vs;
+    double a_hr, b_hr, L_hr, V0_hr;
+    double xstpx;           /* grid step used by calc_frictionpara (calc_RSF.c:243) */
+    /* constants hard-coded in the reference */
+    double faultwidth;      /* 400 m (plastic.c:22,80)                      */
+    double rad_damping;     /* 4.3303e6 Pa s/m (calc_RSF.c:111)             */
+    double bulk_modulus;    /* 50e9 Pa (interpolation.c:226)                */
+    double band_y, band_halfwidth; /* fault band |y-10 km| <= 100 m (calc_RSF.c:226-230) */
+} v2c_params;
+
+/* ---- rock table (headevps.c:231-243) ----------------------------------- */
+typedef struct v2c_materials {
+    int ntypes;                                   /* types 0..ntypes-1     */
+    double nu[V2C_MAX_TYPES], dh[V2C_MAX_TYPES], dv[V2C_MAX_TYPES];
+    double ss[V2C_MAX_TYPES], mm[V2C_MAX_TYPES], gg[V2C_MAX_TYPES];
+    double n0[V2C_MAX_TYPES], n1[V2C_MAX_TYPES];
+    double ro[V2C_MAX_TYPES], bb[V2C_MAX_TYPES], aa[V2C_MAX_TYPES];
+    double cp[V2C_MAX_TYPES], kt[V2C_MAX_TYPES], kf[V2C_MAX_TYPES];
+    double kp[V2C_MAX_TYPES], ht[V2C_MAX_TYPES];
+    int im[V2C_MAX_TYPES];                        /* immobile flag         */
+} v2c_materials;
+
+/* ---- marker SoA (headevps.c:210-246, input_RSF.c:11) --------------------
+ * Field names follow the reference globals with the `mark` prefix dropped.
+ * Every pointer must be non-NULL for upload/download. */
+typedef struct v2c_markers {
+    double *x, *y;          /* markx, marky                                  */
+    double *k;              /* markk   temperature (K)                       */
+    double *p;              /* markp   pressure (Pa)                         */
+    double *xx, *xy;        /* markxx, markxy deviatoric stress              */
+    double *exx, *exy;      /* markexx, markexy strain rate                  */
+    double *e;              /* marke   plastic strain                        */
+    double *phi;            /* markphi RSF state                             */
+    double *vx, *vy;        /* markvx, markvy                                */
+    double *C;              /* markC   cohesion                              */
+    double *d, *w;          /* markd, markw (reset to -1 out of the grid)    */
+    double *a, *b, *L, *L0, *V0, *fric_static, *C0; /* calc_frictionpara outputs */
+    char *t;                /* markt   rock type                             */
+} v2c_markers;
+
+/* ---- nodal fields (reference global names) ------------------------------ */
+enum v2c_nodal_field {
+    /* solution / solver-side fields (inputs of the hot path) */
+    V2C_N_pr = 0, V2C_N_vx, V2C_N_vy, V2C_N_exx, V2C_N_exy, V2C_N_esp,
+    V2C_N_sxx, V2C_N_sxy, V2C_N_sxxs, V2C_N_sppes, V2C_N_exxpl, V2C_N_exxel,
+    /* marker2nodes outputs */
+    V2C_N_bm, V2C_N_ro, V2C_N_et, V2C_N_nu, V2C_N_nd, V2C_N_gg, V2C_N_gd,
+    V2C_N_sppe, V2C_N_phis, V2C_N_phin, V2C_N_sxyen, V2C_N_sxxen, V2C_N_exyen,
+    V2C_N_exxen, V2C_N_sppen, V2C_N_mus_s, V2C_N_as, V2C_N_bs, V2C_N_Ls,
+    V2C_N_V0s, V2C_N_sxxe, V2C_N_sxye, V2C_N_exxe, V2C_N_exye, V2C_N_cohesion,
+    V2C_N_dro, V2C_N_drp, V2C_N_cp, V2C_N_kt, V2C_N_ht, V2C_N_tk, V2C_N_mrx,
+    V2C_N_mry, V2C_N_mvx, V2C_N_mvy, V2C_N_pe, V2C_N_pen, V2C_N_nu_m2c,
+    V2C_N_nd_m2c, V2C_N_mvxn, V2C_N_mvyn,
+    /* nodes2markers antidiffusion outputs and accumulators */
+    V2C_N_phie, V2C_N_sxyee, V2C_N_exyee, V2C_N_pee, V2C_N_sppee, V2C_N_sxxee,
+    V2C_N_exxee, V2C_N_mvxe, V2C_N_mvye,
+    V2C_N_wtphisum, V2C_N_phisum, V2C_N_sxysum, V2C_N_exysum, V2C_N_esum,
+    V2C_N_wtsxxsum, V2C_N_psum, V2C_N_sxxsum, V2C_N_exxsum, V2C_N_wtvxsum,
+    V2C_N_vxsum, V2C_N_wtvysum, V2C_N_vysum, V2C_N_mrx0, V2C_N_mry0,
+    /* plastic / RSF nodal state (plastic.c, calc_RSF.c) */
+    V2C_N_pes, V2C_N_phis_it, V2C_N_syields, V2C_N_nuvp_it, V2C_N_sliprs,
+    V2C_N_RK_sliprs_final, V2C_N_RK_phidts_final, V2C_N_RK_sliprs_solved,
+    V2C_N_RK_phidts_solved, V2C_N_RK_deltas,
+    V2C_N_RK_sliprs0, V2C_N_RK_sliprs1, V2C_N_RK_phidts0, V2C_N_RK_phidts1,
+    V2C_NF
+};
+
+typedef struct v2c_ctx v2c_ctx;
+
+/* Fill *p with the reference defaults (input_RSF.c:13-59 and the shipped
+ * mode_RSF.t3c flags, except markmod=2 which selects RK4). */
+void v2c_params_default(v2c_params *p);
+
+/* Name of a nodal field ("pr", "sxye", ...), NULL if out of range. */
+const char *v2c_nodal_name(int field);
+
+/* Create a context on HIP device `device`.  `capacity` is the largest marker
+ * count the context will hold (device memory is sized for it).
+ * Replaces the static allocation of headevps.c:8-21. */
+int v2c_create(v2c_ctx **ctx, const v2c_grid *grid, const v2c_params *prm,
+               const v2c_materials *mat, long capacity, int device);
+void v2c_destroy(v2c_ctx *ctx);
+const char *v2c_last_error(const v2c_ctx *ctx);
+int v2c_set_async(v2c_ctx *ctx, int async_mode);
+int v2c_synchronize(v2c_ctx *ctx);
+
+/* Mirror of the reference globals timesum / timestep (headevps.c:343-348). */
+int v2c_set_time(v2c_ctx *ctx, double timesum, double timestep);
+
+/* Marker state upload/download (load_model / save_model use it,
+ * load_save.c:164-347,353-527).  Upload also builds the cell ordering. */
+int v2c_markers_upload(v2c_ctx *ctx, const v2c_markers *m, long n);
+int v2c_markers_download(v2c_ctx *ctx, v2c_markers *m, long n);
+long v2c_marker_count(const v2c_ctx *ctx);
+
+/* Nodal mirror transfer, one field at a time (length nodenum). */
+int v2c_nodal_set(v2c_ctx *ctx, int field, const double *host);
+int v2c_nodal_get(v2c_ctx *ctx, int field, double *host);
+
+/* ---- hot-path entry points ---------------------------------------------- */
+
+/* marker2nodes() (interpolation.c:55-916, called from i2evps.c:89):
+ * marker->node scatter with interpolationmod=2 weights, normalisation,
+ * antidiffusion and nodal friction parameters.  The boundary-condition tail
+ * (interpolation.c:918-925) stays on the host. */
+int v2c_marker2nodes(v2c_ctx *ctx);
+
+/* nodes2markers() (interpolation.c:929-1048, called from i2evps.c:108). */
+int v2c_nodes2markers(v2c_ctx *ctx);
+
+/* moveandrotatemarkers() (moveandrotatemarkers.c:1-178, called from
+ * i2evps.c:113).  Marker types >= 100 return V2C_EUNSUPPORTED. */
+int v2c_moveandrotatemarkers(v2c_ctx *ctx);
+
+/* nodes2markers() immediately followed by moveandrotatemarkers() in one pass
+ * over the markers; valid when nothing runs between the two calls (the
+ * reference time loop with tempmod==0 or timesum>timestep, i2evps.c:98-113). */
+int v2c_nodes2markers_advect(v2c_ctx *ctx);
+
+/* compute_before_solve(dt, rk) (plastic.c:2-56, called from
+ * momentum_continuity_iterate.c:69).  *nerr receives the number of nodes that
+ * set the reference's errcheck flag. */
+int v2c_compute_before_solve(v2c_ctx *ctx, double dt, int rk, long *nerr);
+
+/* compute_after_solve(dt, rk) (plastic.c:61-129, called from
+ * momentum_continuity_iterate.c:89).  Returns the RMS yield misfit in
+ * *deltas. */
+int v2c_compute_after_solve(v2c_ctx *ctx, double dt, int rk, double *deltas,
+                            long *nerr);
+
+/* ductilevisc() (density_ductilevisc_calc.c:29-223) evaluated for n
+ * independent inputs; a pure helper that the host may call directly
+ * (headevps.c:556).  Arrays are host arrays of length n; `iters` may be NULL. */
+int v2c_ductilevisc_batch(v2c_ctx *ctx, long n, const char *type,
+                          const double *sxy, const double *sxx,
+                          const double *exy, const double *exx,
+                          const double *tk, const double *pb,
+                          double *nu_out, int *iters);
+
+/* Timing of the last entry point's kernels (HIP events on the context
+ * stream), in milliseconds; `kernel` selects one of V2C_K_* . */
+enum v2c_kernel_id {
+    V2C_K_M2N_CELLS = 0, V2C_K_M2N_NODES, V2C_K_N2M_CELLS, V2C_K_N2M_NODES,
+    V2C_K_ADVECT, V2C_K_SORT, V2C_K_BEFORE, V2C_K_AFTER, V2C_K_COUNT
+};
+int v2c_enable_timing(v2c_ctx *ctx, int on);
+int v2c_kernel_time(v2c_ctx *ctx, int kernel, double *ms, long *launches);
+int v2c_reset_timing(v2c_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* V2C_H */

@@ -1,0 +1,135 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from velo2cycles_amd.abi import (NF, NODAL_FIELDS, Grid, Markers, Materials, Params, _dptr)
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+
+class OrState(C.Structure):
+    _fields_ = [("grid", Grid), ("prm", Params), ("mat", C.POINTER(Materials)),
+                ("timesum", C.c_double), ("timestep", C.c_double), ("m", Markers),
+                ("n", C.c_long), ("nod", C.POINTER(C.c_double) * NF)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "liboracle.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P = C.POINTER
+        L.or_nxsearch.argtypes = [P(Grid), C.c_double]
+        L.or_nxsearch.restype = C.c_long
+        L.or_nysearch.argtypes = [P(Grid), C.c_double]
+        L.or_nysearch.restype = C.c_long
+        L.or_allinterpomp.argtypes = [P(OrState), C.c_double, C.c_double]
+        L.or_allinterpomp.restype = C.c_double
+        L.or_ductilevisc.argtypes = [P(OrState), C.c_int] + [C.c_double] * 6 + [P(C.c_int)]
+        L.or_ductilevisc.restype = C.c_double
+        L.or_ductilevisc_newton.argtypes = ([P(OrState), C.c_int] + [C.c_double] * 6
+                                            + [P(C.c_int), P(C.c_double)])
+        L.or_ductilevisc_newton.restype = C.c_double
+        L.or_marker2nodes.argtypes = [P(OrState)]
+        L.or_marker2nodes.restype = None
+        L.or_nodes2markers.argtypes = [P(OrState)]
+        L.or_nodes2markers.restype = None
+        L.or_moveandrotatemarkers.argtypes = [P(OrState)]
+        L.or_moveandrotatemarkers.restype = C.c_long
+        L.or_compute_before_solve.argtypes = [P(OrState), C.c_double, C.c_int]
+        L.or_compute_before_solve.restype = C.c_long
+        L.or_compute_after_solve.argtypes = [P(OrState), C.c_double, C.c_int, P(C.c_long)]
+        L.or_compute_after_solve.restype = C.c_double
+        L.or_allinteri.argtypes = [P(OrState), C.c_double, C.c_double] + [P(C.c_double)] * 4
+        L.or_allinteri.restype = None
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """Oracle state over caller-visible numpy arrays (a copy of a Config)."""
+
+    def __init__(self, cfg, markers=None, nodal=None):
+        self.cfg = cfg
+        self.grid = cfg.grid
+        self.mats = cfg.mats
+        self.markers = (markers or cfg.markers).copy()
+        self.nodal = {k: np.array(v, dtype=np.float64, copy=True)
+                      for k, v in (nodal or cfg.nodal).items()}
+        for k in NODAL_FIELDS:
+            self.nodal.setdefault(k, np.zeros(cfg.grid.nodenum))
+        self.st = OrState()
+        self.st.grid = cfg.grid.struct
+        self.st.prm = cfg.params
+        self.st.mat = C.pointer(cfg.mats)
+        self.st.timesum = cfg.timesum
+        self.st.timestep = cfg.timestep
+        self.st.m = self.markers.struct
+        self.st.n = self.markers.n
+        for i, k in enumerate(NODAL_FIELDS):
+            self.st.nod[i] = _dptr(self.nodal[k])
+
+    def set_time(self, timesum, timestep):
+        self.st.timesum = timesum
+        self.st.timestep = timestep
+
+    def marker2nodes(self):
+        lib().or_marker2nodes(C.byref(self.st))
+
+    def nodes2markers(self):
+        lib().or_nodes2markers(C.byref(self.st))
+
+    def moveandrotatemarkers(self):
+        return lib().or_moveandrotatemarkers(C.byref(self.st))
+
+    def compute_before_solve(self, dt, rk):
+        return lib().or_compute_before_solve(C.byref(self.st), dt, rk)
+
+    def compute_after_solve(self, dt, rk):
+        nerr = C.c_long(0)
+        d = lib().or_compute_after_solve(C.byref(self.st), dt, rk, C.byref(nerr))
+        return d, nerr.value
+
+    def ductilevisc(self, t, sxy, sxx, tk, pb):
+        it = C.c_int(0)
+        v = lib().or_ductilevisc(C.byref(self.st), int(t), sxy, sxx, 0.0, 0.0, tk, pb, C.byref(it))
+        return v, it.value
+
+    def ductilevisc_newton(self, t, sxy, sxx, tk, pb, guess=0.0, tol=1e-10):
+        it = C.c_int(0)
+        r = C.c_double(0)
+        v = lib().or_ductilevisc_newton(C.byref(self.st), int(t), sxy, sxx, tk, pb, guess, tol,
+                                        C.byref(it), C.byref(r))
+        return v, it.value, r.value
+
+    def allinteri(self, x, y):
+        o = [C.c_double() for _ in range(4)]
+        lib().or_allinteri(C.byref(self.st), x, y, *[C.byref(v) for v in o])
+        return tuple(v.value for v in o)
+
+    def allinterpomp(self, x, y):
+        return lib().or_allinterpomp(C.byref(self.st), x, y)
+
+    def nxsearch(self, x):
+        return lib().or_nxsearch(C.byref(self.grid.struct), x)
+
+    def nysearch(self, y):
+        return lib().or_nysearch(C.byref(self.grid.struct), y)

@@ -1,0 +1,177 @@
+/*
+ * ref_driver.c -- drives the reference's own hot-path routines, compiled in
+ * place from /root/reference/src (the include path is given by
+ * oracle/Makefile; no reference source is copied into this repository).
+ *
+ * TEST INFRASTRUCTURE ONLY: it produces golden vectors (tests/golden/) and
+ * checks the oracle.  It never travels as part of the product.
+ *
+ * Protocol: `ref_driver IN OUT op...`.  IN and OUT are sequences of records
+ *   char name[32]; char kind ('d' double, 'l' long, 'i' int, 'c' char);
+ *   int64 count; payload
+ * IN sets reference globals by name; the ops run in order; OUT receives every
+ * global of the table after the last op.  Ops:
+ *   m2n | n2m | move | before:<dt>:<rk> | after:<dt>:<rk> | ductile | allinteri
+ */
+#ifndef _GNU_SOURCE
+#define _GNU_SOURCE 1
+#endif
+#include <stdio.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+#include <stdbool.h>
+
+/* The reference translation unit (i2evps.c:12-30 order, hot-path files only). */
+#include "headevps.c"
+#include "input_RSF.c"
+#include "calc_RSF.c"
+#include "gausevps.c"
+#include "interpolation.c"
+#include "density_ductilevisc_calc.c"
+#include "moveandrotatemarkers.c"
+#include "temperature_iterate.c"
+#include "momentum_continuity_addmatrix_checkerror.c"
+#include "plastic.c"
+#include "recalc_marker_properties.c"
+
+/* driver-local arrays for the pure-function ops */
+#define DRV_MAX 2000000
+static double dv_sxy[DRV_MAX], dv_sxx[DRV_MAX], dv_tk[DRV_MAX], dv_pb[DRV_MAX], dv_out[DRV_MAX];
+static char dv_type[DRV_MAX];
+static double ai_x[DRV_MAX], ai_y[DRV_MAX], ai_vx[DRV_MAX], ai_vy[DRV_MAX], ai_esp[DRV_MAX], ai_ee[DRV_MAX];
+static long drv_n;
+static double drv_ret[8];
+static double RK_sliprs0[MAXNOD], RK_sliprs1[MAXNOD], RK_phidts0[MAXNOD], RK_phidts1[MAXNOD];
+
+typedef struct { const char *name; char kind; void *ptr; long cap; } entry;
+#define D(a) {#a, 'd', (void *)(a), (long)(sizeof(a) / sizeof(double))}
+#define S(a) {#a, 'd', (void *)&(a), 1}
+#define L(a) {#a, 'l', (void *)&(a), 1}
+#define I(a) {#a, 'i', (void *)&(a), 1}
+static entry table[] = {
+    /* grid and scalars */
+    D(gx), D(gy), L(xnumx), L(ynumy), L(marknum), S(xsize), S(ysize), S(timesum), S(timestep),
+    I(markmod), I(stoksmod), I(outgrid), I(densimod), I(interpolationmod), I(antidiffinter),
+    I(rkmod), I(RSFmode), I(power_law_RSF), I(reloadfromDP), I(compressiblemode), I(stickymod),
+    I(inertyn), I(pmode), I(tempmod), S(nubeg), S(nuend), S(tdeep), S(zdeep), S(nstress), S(Dphis),
+    S(half_range), S(start_updip), S(end_updip), S(start_downdip), S(end_downdip),
+    S(a_vw), S(b_vw), S(L_vw), S(V0_vw), S(a_vs), S(b_vs), S(L_vs), S(V0_vs),
+    S(a_hr), S(b_hr), S(L_hr), S(V0_hr), S(errcheck), D(eps),
+    /* rock table */
+    D(marknu), D(markdh), D(markdv), D(markss), D(markmm), D(markgg), D(markn0), D(markn1),
+    D(markro), D(markbb), D(markaa), D(markcp), D(markkt), D(markkf), D(markkp), D(markht),
+    {"markim", 'i', (void *)markim, MAXTMR},
+    /* markers */
+    D(markx), D(marky), D(markk), D(markp), D(markxx), D(markxy), D(markexx), D(markexy),
+    D(marke), D(markphi), D(markvx), D(markvy), D(markC), D(markd), D(markw), D(mark_a),
+    D(mark_b), D(mark_L), D(mark_L0), D(mark_V0), D(fric_static), D(markC0),
+    {"markt", 'c', (void *)markt, MAXMRK},
+    /* nodal */
+    D(pr), D(vx), D(vy), D(exx), D(exy), D(esp), D(sxx), D(sxy), D(sxxs), D(sppes), D(exxpl), D(exxel),
+    D(bm), D(ro), D(et), D(nu), D(nd), D(gg), D(gd), D(sppe), D(phis), D(phin), D(sxyen), D(sxxen),
+    D(exyen), D(exxen), D(sppen), D(mus_s), D(as), D(bs), D(Ls), D(V0s), D(sxxe), D(sxye), D(exxe),
+    D(exye), D(cohesion), D(dro), D(drp), D(cp), D(kt), D(ht), D(tk), D(mrx), D(mry), D(mvx), D(mvy),
+    D(pe), D(pen), D(nu_m2c), D(nd_m2c), D(mvxn), D(mvyn),
+    D(phie), D(sxyee), D(exyee), D(pee), D(sppee), D(sxxee), D(exxee), D(mvxe), D(mvye),
+    D(wtphisum), D(phisum), D(sxysum), D(exysum), D(esum), D(wtsxxsum), D(psum), D(sxxsum),
+    D(exxsum), D(wtvxsum), D(vxsum), D(wtvysum), D(vysum), D(mrx0), D(mry0),
+    D(pes), D(phis_it), D(syields), D(nuvp_it), D(sliprs), D(RK_sliprs_final), D(RK_phidts_final),
+    D(RK_sliprs_solved), D(RK_phidts_solved), D(RK_deltas),
+    D(RK_sliprs0), D(RK_sliprs1), D(RK_phidts0), D(RK_phidts1),
+    /* pure-function op arrays */
+    D(dv_sxy), D(dv_sxx), D(dv_tk), D(dv_pb), D(dv_out), {"dv_type", 'c', (void *)dv_type, DRV_MAX},
+    D(ai_x), D(ai_y), D(ai_vx), D(ai_vy), D(ai_esp), D(ai_ee), L(drv_n), D(drv_ret),
+};
+#define NTAB ((long)(sizeof table / sizeof table[0]))
+
+static size_t ksize(char k) { return k == 'd' ? 8 : k == 'l' ? 8 : k == 'i' ? 4 : 1; }
+
+static int read_input(const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) { perror(path); return -1; }
+    for (;;) {
+        char name[32]; char kind; long long cnt;
+        if (fread(name, 1, 32, f) != 32) break;
+        if (fread(&kind, 1, 1, f) != 1 || fread(&cnt, 8, 1, f) != 1) { fclose(f); return -1; }
+        name[31] = 0;
+        long i;
+        for (i = 0; i < NTAB; i++) if (!strcmp(table[i].name, name)) break;
+        if (i == NTAB || table[i].kind != kind || cnt > table[i].cap) {
+            fprintf(stderr, "ref_driver: bad record %s kind %c count %lld\n", name, kind, cnt);
+            fclose(f); return -1;
+        }
+        if (fread(table[i].ptr, ksize(kind), (size_t)cnt, f) != (size_t)cnt) { fclose(f); return -1; }
+    }
+    fclose(f);
+    return 0;
+}
+
+static long used_count(const entry *e)
+{
+    long nodenum_ = xnumx * ynumy;
+    if (e->cap == 1) return 1;
+    if (!strcmp(e->name, "gx")) return xnumx;
+    if (!strcmp(e->name, "gy")) return ynumy;
+    if (!strncmp(e->name, "mark", 4) && e->cap == MAXMRK) return marknum + 1;
+    if (e->cap == MAXMRK) return marknum + 1;     /* fric_static */
+    if (!strncmp(e->name, "mark", 4) || !strcmp(e->name, "markim")) return MAXTMR;
+    if (!strncmp(e->name, "dv_", 3) || !strncmp(e->name, "ai_", 3)) return drv_n;
+    if (!strcmp(e->name, "eps") || !strcmp(e->name, "drv_ret")) return e->cap;
+    return nodenum_ + 1;
+}
+
+static int write_output(const char *path)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) { perror(path); return -1; }
+    for (long i = 0; i < NTAB; i++) {
+        char name[32] = {0};
+        strncpy(name, table[i].name, 31);
+        long long cnt = used_count(&table[i]);
+        fwrite(name, 1, 32, f);
+        fwrite(&table[i].kind, 1, 1, f);
+        fwrite(&cnt, 8, 1, f);
+        fwrite(table[i].ptr, ksize(table[i].kind), (size_t)cnt, f);
+    }
+    fclose(f);
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) { fprintf(stderr, "usage: ref_driver IN OUT op...\n"); return 2; }
+    load_extrainput();   /* reference defaults (input_RSF.c:13-63); IN may override */
+    if (read_input(argv[1])) return 3;
+    /* derived grid globals (load_save.c:599-641) */
+    nodenum = xnumx * ynumy; nodenum2 = nodenum * 2; nodenum3 = nodenum * 3;
+    cellnum = (xnumx - 1) * (ynumy - 1);
+    xstpx = xsize / (double)(xnumx - 1); ystpy = ysize / (double)(ynumy - 1);
+    for (long i = 0; i < nodenum; i++) {
+        RK_sliprs[i][0] = RK_sliprs0[i]; RK_sliprs[i][1] = RK_sliprs1[i];
+        RK_phidts[i][0] = RK_phidts0[i]; RK_phidts[i][1] = RK_phidts1[i];
+    }
+    for (int a = 3; a < argc; a++) {
+        const char *op = argv[a];
+        double dt; int rk;
+        if (!strcmp(op, "m2n")) marker2nodes();
+        else if (!strcmp(op, "n2m")) nodes2markers();
+        else if (!strcmp(op, "move")) moveandrotatemarkers();
+        else if (sscanf(op, "before:%lf:%d", &dt, &rk) == 2) drv_ret[0] = compute_before_solve(dt, rk);
+        else if (sscanf(op, "after:%lf:%d", &dt, &rk) == 2) drv_ret[1] = compute_after_solve(dt, rk);
+        else if (!strcmp(op, "ductile")) {
+            for (long i = 0; i < drv_n; i++)
+                dv_out[i] = ductilevisc((int)dv_type[i], dv_sxy[i], dv_sxx[i], 0, 0, dv_tk[i], dv_pb[i]);
+        } else if (!strcmp(op, "allinteri")) {
+            for (long i = 0; i < drv_n; i++)
+                allinteri(ai_x[i], ai_y[i], &ai_vx[i], &ai_vy[i], &ai_esp[i], &ai_ee[i]);
+        } else { fprintf(stderr, "ref_driver: unknown op %s\n", op); return 4; }
+    }
+    for (long i = 0; i < nodenum; i++) {
+        RK_sliprs0[i] = RK_sliprs[i][0]; RK_sliprs1[i] = RK_sliprs[i][1];
+        RK_phidts0[i] = RK_phidts[i][0]; RK_phidts1[i] = RK_phidts[i][1];
+    }
+    return write_output(argv[2]) ? 5 : 0;
+}

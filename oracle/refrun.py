@@ -1,0 +1,121 @@
+"""Run the reference's own hot-path routines (oracle/_ref/ref_driver, compiled
+in place from /root/reference/src by oracle/Makefile) on a Config.
+
+TEST INFRASTRUCTURE ONLY: used in this container to write golden vectors and
+to pin the oracle.  The reference is absent on the GPU box; callers check
+`available()` first.
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(ORACLE_DIR, "_ref", "ref_driver")
+REF_SRC = "/root/reference/src"
+
+# reference capacity (headevps.c:10-20)
+MAXXLN, MAXYLN, MAXNOD, MAXMRK = 300, 120, 36000, 1080000
+
+_MARK_NAME = {f: ("mark" + f) for f in MARKER_FIELDS}
+_MARK_NAME.update(a="mark_a", b="mark_b", L="mark_L", L0="mark_L0", V0="mark_V0",
+                  fric_static="fric_static", C0="markC0", C="markC", t="markt")
+
+_PARAM_INT = ("markmod stoksmod outgrid densimod interpolationmod antidiffinter rkmod RSFmode "
+              "power_law_RSF reloadfromDP compressiblemode stickymod inertyn pmode tempmod").split()
+_PARAM_DBL = ("nubeg nuend tdeep zdeep nstress Dphis half_range start_updip end_updip start_downdip "
+              "end_downdip a_vw b_vw L_vw V0_vw a_vs b_vs L_vs V0_vs a_hr b_hr L_hr V0_hr").split()
+_MAT = "nu dh dv ss mm gg n0 n1 ro bb aa cp kt kf kp ht".split()
+
+
+def available() -> bool:
+    if os.path.exists(REF_SRC) and not os.path.exists(DRIVER):
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=False)
+    return os.path.exists(DRIVER) and os.path.exists(REF_SRC)
+
+
+def legal(cfg) -> bool:
+    g = cfg.grid
+    return g.xnumx <= MAXXLN and g.ynumy <= MAXYLN and g.nodenum <= MAXNOD and cfg.n < MAXMRK
+
+
+def _rec(f, name, kind, arr):
+    arr = np.ascontiguousarray(arr)
+    f.write(name.encode().ljust(32, b"\0"))
+    f.write(kind.encode())
+    f.write(struct.pack("<q", arr.size))
+    f.write(arr.tobytes())
+
+
+def _read(path):
+    out = {}
+    dt = {"d": np.float64, "l": np.int64, "i": np.int32, "c": np.int8}
+    with open(path, "rb") as f:
+        while True:
+            h = f.read(32)
+            if len(h) < 32:
+                break
+            name = h.rstrip(b"\0").decode()
+            kind = f.read(1).decode()
+            (cnt,) = struct.unpack("<q", f.read(8))
+            a = np.frombuffer(f.read(cnt * np.dtype(dt[kind]).itemsize), dtype=dt[kind]).copy()
+            out[name] = a
+    return out
+
+
+def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=None):
+    """Write cfg (+ overrides) into the reference globals, run `ops`, return
+    (markers dict, nodal dict, raw dict).  Markers are returned by the build's
+    field names, nodal arrays by reference names (length nodenum)."""
+    assert available(), "reference driver unavailable"
+    assert legal(cfg), "size exceeds the reference capacity macros"
+    m = markers or cfg.markers
+    nod = nodal or cfg.nodal
+    p = cfg.params
+    g = cfg.grid
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            _rec(f, "gx", "d", g.gx)
+            _rec(f, "gy", "d", g.gy)
+            _rec(f, "xnumx", "l", np.array([g.xnumx], np.int64))
+            _rec(f, "ynumy", "l", np.array([g.ynumy], np.int64))
+            _rec(f, "marknum", "l", np.array([m.n], np.int64))
+            _rec(f, "xsize", "d", np.array([g.xsize]))
+            _rec(f, "ysize", "d", np.array([g.ysize]))
+            _rec(f, "timesum", "d", np.array([cfg.timesum if timesum is None else timesum]))
+            _rec(f, "timestep", "d", np.array([cfg.timestep if timestep is None else timestep]))
+            for k in _PARAM_INT:
+                _rec(f, k, "i", np.array([getattr(p, k)], np.int32))
+            for k in _PARAM_DBL:
+                _rec(f, k, "d", np.array([getattr(p, k)]))
+            eps = np.zeros(100)
+            eps[2] = p.incoming_T
+            _rec(f, "eps", "d", eps)
+            for k in _MAT:
+                _rec(f, "mark" + k, "d", np.array(getattr(cfg.mats, k)[:200]))
+            _rec(f, "markim", "i", np.array(cfg.mats.im[:200], np.int32))
+            for fld in MARKER_FIELDS:
+                _rec(f, _MARK_NAME[fld], "d", m[fld])
+            _rec(f, "markt", "c", m.t)
+            for k in NODAL_FIELDS:
+                _rec(f, k, "d", nod[k])
+            for k, (kind, arr) in (extra or {}).items():
+                _rec(f, k, kind, arr)
+        # one thread: the reference's merge order (interpolation.c:631-674) and
+        # its racy antidiffusion scatter (:942) make multithreaded runs vary
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        with open(os.devnull, "w") as dn:
+            subprocess.run([DRIVER, fin, fout] + list(ops), check=True, stdout=dn, env=env)
+        raw = _read(fout)
+    nn = g.nodenum
+    mk = {fld: raw[_MARK_NAME[fld]][:m.n].copy() for fld in MARKER_FIELDS}
+    mk["t"] = raw["markt"][:m.n].copy()
+    nd = {k: raw[k][:nn].copy() for k in NODAL_FIELDS}
+    return mk, nd, raw

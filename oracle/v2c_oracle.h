@@ -1,0 +1,54 @@
+/*
+ * v2c_oracle.h -- FP64 CPU oracle of the per-marker hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP path is compared
+ * against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load it; the product library (libv2c) never links or calls it.
+ *
+ * It restates the reference algorithm single-threaded, in the reference's own
+ * operation order, so that it reproduces a single-threaded run of the
+ * reference routines bit for bit (the reference is built with gcc, no FMA
+ * contraction; this file is built with -ffp-contract=off).
+ */
+#ifndef V2C_ORACLE_H
+#define V2C_ORACLE_H
+#include "../include/v2c.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_state {
+    v2c_grid grid;
+    v2c_params prm;
+    const v2c_materials *mat;
+    double timesum, timestep;
+    v2c_markers m;          /* caller-owned marker arrays                    */
+    long n;                 /* marker count                                  */
+    double *nod[V2C_NF];    /* caller-owned nodal arrays, length nodenum     */
+} or_state;
+
+long or_nxsearch(const v2c_grid *g, double x);
+long or_nysearch(const v2c_grid *g, double y);
+double or_allinterpomp(const or_state *s, double x, double y);
+double or_ductilevisc(const or_state *s, int type, double sxy, double sxx,
+                      double exy, double exx, double tk, double pb, int *iters);
+/* Build-defined NEWTON mode of the ductile solve: same residual, exact
+ * derivative, stop at |f|/eta <= tol.  `guess` <= 0 selects the reference's
+ * default start.  Returns the clamped effective viscosity. */
+double or_ductilevisc_newton(const or_state *s, int type, double sxy,
+                             double sxx, double tk, double pb, double guess,
+                             double tol, int *iters, double *resid);
+
+void or_marker2nodes(or_state *s);
+void or_nodes2markers(or_state *s);
+long or_moveandrotatemarkers(or_state *s);   /* returns #markers of type >= 100 */
+long or_compute_before_solve(or_state *s, double dt, int rk);   /* returns #err */
+double or_compute_after_solve(or_state *s, double dt, int rk, long *nerr);
+void or_allinteri(const or_state *s, double x, double y, double *vx,
+                  double *vy, double *esp, double *ee);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,326 @@
+"""ctypes mirror of include/v2c.h and the Python handle around libv2c.so.
+
+The structures here are byte-for-byte the C structures of the C-ABI; the same
+definitions serve the HIP library (libv2c.so) and the oracle.  Nodal fields are
+addressed by the reference's global names (headevps.c:89-196); markers by the
+reference's marker array names without the ``mark`` prefix (headevps.c:208-246).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libv2c.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "v2c.h")
+
+V2C_MAX_TYPES = 256
+
+# Order must equal `enum v2c_nodal_field` in include/v2c.h (checked by a test).
+NODAL_FIELDS = (
+    "pr vx vy exx exy esp sxx sxy sxxs sppes exxpl exxel "
+    "bm ro et nu nd gg gd sppe phis phin sxyen sxxen exyen exxen sppen mus_s as bs Ls V0s "
+    "sxxe sxye exxe exye cohesion dro drp cp kt ht tk mrx mry mvx mvy pe pen nu_m2c nd_m2c "
+    "mvxn mvyn "
+    "phie sxyee exyee pee sppee sxxee exxee mvxe mvye "
+    "wtphisum phisum sxysum exysum esum wtsxxsum psum sxxsum exxsum wtvxsum vxsum wtvysum vysum "
+    "mrx0 mry0 "
+    "pes phis_it syields nuvp_it sliprs RK_sliprs_final RK_phidts_final RK_sliprs_solved "
+    "RK_phidts_solved RK_deltas RK_sliprs0 RK_sliprs1 RK_phidts0 RK_phidts1"
+).split()
+NF = len(NODAL_FIELDS)
+FIELD_ID = {n: i for i, n in enumerate(NODAL_FIELDS)}
+
+MARKER_FIELDS = ("x y k p xx xy exx exy e phi vx vy C d w a b L L0 V0 fric_static C0").split()
+
+KERNELS = ("m2n_cells", "m2n_nodes", "n2m_cells", "n2m_nodes", "advect", "sort",
+           "before", "after")
+
+ERRORS = {0: "ok", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECAPACITY",
+          -5: "EUNSUPPORTED", -6: "ENOGPU"}
+
+
+def header_enum_names(path: str = HEADER_PATH) -> list[str]:
+    """Names of `enum v2c_nodal_field`, parsed from the header."""
+    text = open(path).read()
+    body = text[text.index("enum v2c_nodal_field"):]
+    body = body[body.index("{") + 1: body.index("V2C_NF")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return [n[len("V2C_N_"):] for n in re.findall(r"V2C_N_\w+", body)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("xnumx", C.c_long), ("ynumy", C.c_long),
+                ("gx", C.POINTER(C.c_double)), ("gy", C.POINTER(C.c_double)),
+                ("xsize", C.c_double), ("ysize", C.c_double)]
+
+
+_PARAM_INTS = ("markmod stoksmod outgrid densimod interpolationmod antidiffinter rkmod RSFmode "
+               "power_law_RSF reloadfromDP compressiblemode stickymod inertyn pmode tempmod "
+               "n2m_phantom").split()
+_PARAM_DBLS = ("nubeg nuend tdeep zdeep incoming_T nstress Dphis half_range start_updip end_updip "
+               "start_downdip end_downdip a_vw b_vw L_vw V0_vw a_vs b_vs L_vs V0_vs a_hr b_hr "
+               "L_hr V0_hr xstpx faultwidth rad_damping bulk_modulus band_y band_halfwidth").split()
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_int) for n in _PARAM_INTS] + [(n, C.c_double) for n in _PARAM_DBLS]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_MAT_DBLS = "nu dh dv ss mm gg n0 n1 ro bb aa cp kt kf kp ht".split()
+
+
+class Materials(C.Structure):
+    _fields_ = ([("ntypes", C.c_int)] + [(n, C.c_double * V2C_MAX_TYPES) for n in _MAT_DBLS]
+                + [("im", C.c_int * V2C_MAX_TYPES)])
+
+
+class Markers(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in MARKER_FIELDS] + [("t", C.c_char_p)]
+
+
+def default_params(xsize: float = 0.0, xnumx: int = 2) -> Params:
+    """Reference defaults (input_RSF.c:13-59, mode_RSF.t3c flags), markmod=2.
+
+    Kept equal to v2c_params_default() in the library (a test checks both)."""
+    p = Params()
+    vals = dict(markmod=2, stoksmod=1, outgrid=2, densimod=0, interpolationmod=2, antidiffinter=1,
+                rkmod=0, RSFmode=1, power_law_RSF=0, reloadfromDP=0, compressiblemode=0,
+                stickymod=1, inertyn=0, pmode=0, tempmod=0, n2m_phantom=1,
+                nubeg=1e1, nuend=1.667e27, tdeep=1873.0, zdeep=3000000.0, incoming_T=0.0,
+                nstress=10.0e6, Dphis=0.0, half_range=2.0e3,
+                a_vw=0.009, b_vw=0.017, L_vw=0.0100, V0_vw=4.0e-09,
+                a_vs=0.020, b_vs=0.001, L_vs=0.0100, V0_vs=4.0e-09,
+                a_hr=0.009, b_hr=0.017, L_hr=0.0100, V0_hr=4.0e-09,
+                faultwidth=400.0, rad_damping=4.3303e+06, bulk_modulus=50e9,
+                band_y=10.000e3, band_halfwidth=100.0)
+    start_sez, end_sez = 10.0e3, 40.0e3
+    vals["start_updip"] = start_sez - vals["half_range"]
+    vals["end_updip"] = start_sez + vals["half_range"]
+    vals["start_downdip"] = end_sez - vals["half_range"]
+    vals["end_downdip"] = end_sez + vals["half_range"]
+    vals["xstpx"] = xsize / float(xnumx - 1) if xnumx > 1 else 0.0
+    for k, v in vals.items():
+        setattr(p, k, v)
+    return p
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class GridArrays:
+    """Owns the numpy gridline arrays behind a Grid struct."""
+
+    def __init__(self, gx: np.ndarray, gy: np.ndarray, xsize: float | None = None,
+                 ysize: float | None = None):
+        self.gx = np.ascontiguousarray(gx, dtype=np.float64)
+        self.gy = np.ascontiguousarray(gy, dtype=np.float64)
+        self.xsize = float(self.gx[-1]) if xsize is None else float(xsize)
+        self.ysize = float(self.gy[-1]) if ysize is None else float(ysize)
+        self.struct = Grid(len(self.gx), len(self.gy), _dptr(self.gx), _dptr(self.gy),
+                           self.xsize, self.ysize)
+
+    @property
+    def xnumx(self):
+        return len(self.gx)
+
+    @property
+    def ynumy(self):
+        return len(self.gy)
+
+    @property
+    def nodenum(self):
+        return len(self.gx) * len(self.gy)
+
+
+class MarkerArrays:
+    """Numpy SoA of markers plus the Markers pointer struct that views it."""
+
+    def __init__(self, n: int, fields: dict | None = None):
+        self.n = n
+        self.arrays = {}
+        for f in MARKER_FIELDS:
+            src = None if fields is None else fields.get(f)
+            self.arrays[f] = (np.zeros(n) if src is None
+                              else np.ascontiguousarray(src, dtype=np.float64).copy())
+        t = None if fields is None else fields.get("t")
+        self.t = np.zeros(n, dtype=np.int8) if t is None else np.ascontiguousarray(t, np.int8).copy()
+        self._make_struct()
+
+    def _make_struct(self):
+        ptrs = [_dptr(self.arrays[f]) for f in MARKER_FIELDS]
+        self.struct = Markers(*ptrs, C.cast(self.t.ctypes.data, C.c_char_p))
+
+    def __getitem__(self, k):
+        return self.t if k == "t" else self.arrays[k]
+
+    def copy(self) -> "MarkerArrays":
+        d = dict(self.arrays)
+        d["t"] = self.t
+        return MarkerArrays(self.n, d)
+
+
+def load_lib(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise RuntimeError(f"libv2c.so not built at {path}: run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    lib.v2c_params_default.argtypes = [P(Params)]
+    lib.v2c_params_default.restype = None
+    lib.v2c_nodal_name.argtypes = [C.c_int]
+    lib.v2c_nodal_name.restype = C.c_char_p
+    lib.v2c_create.argtypes = [P(C.c_void_p), P(Grid), P(Params), P(Materials), C.c_long, C.c_int]
+    lib.v2c_destroy.argtypes = [C.c_void_p]
+    lib.v2c_destroy.restype = None
+    lib.v2c_last_error.argtypes = [C.c_void_p]
+    lib.v2c_last_error.restype = C.c_char_p
+    for name, args in [("v2c_set_async", [C.c_int]), ("v2c_synchronize", []),
+                       ("v2c_set_time", [C.c_double, C.c_double]),
+                       ("v2c_markers_upload", [P(Markers), C.c_long]),
+                       ("v2c_markers_download", [P(Markers), C.c_long]),
+                       ("v2c_nodal_set", [C.c_int, P(C.c_double)]),
+                       ("v2c_nodal_get", [C.c_int, P(C.c_double)]),
+                       ("v2c_marker2nodes", []), ("v2c_nodes2markers", []),
+                       ("v2c_moveandrotatemarkers", []), ("v2c_nodes2markers_advect", []),
+                       ("v2c_compute_before_solve", [C.c_double, C.c_int, P(C.c_long)]),
+                       ("v2c_compute_after_solve", [C.c_double, C.c_int, P(C.c_double), P(C.c_long)]),
+                       ("v2c_ductilevisc_batch", [C.c_long, C.c_char_p, P(C.c_double), P(C.c_double),
+                                                  P(C.c_double), P(C.c_double), P(C.c_double),
+                                                  P(C.c_double), P(C.c_double), P(C.c_int)]),
+                       ("v2c_enable_timing", [C.c_int]),
+                       ("v2c_kernel_time", [C.c_int, P(C.c_double), P(C.c_long)]),
+                       ("v2c_reset_timing", [])]:
+        fn = getattr(lib, name)
+        fn.argtypes = [C.c_void_p] + args
+        fn.restype = C.c_int
+    lib.v2c_marker_count.argtypes = [C.c_void_p]
+    lib.v2c_marker_count.restype = C.c_long
+    return lib
+
+
+class V2CError(RuntimeError):
+    pass
+
+
+class Context:
+    """One device context of libv2c (one HIP device, one marker cloud)."""
+
+    def __init__(self, grid: GridArrays, params: Params, mats: Materials, capacity: int,
+                 device: int = 0, lib: C.CDLL | None = None):
+        self.lib = lib or load_lib()
+        self.grid = grid
+        self.params = params
+        self.mats = mats
+        self.nodenum = grid.nodenum
+        h = C.c_void_p()
+        rc = self.lib.v2c_create(C.byref(h), C.byref(grid.struct), C.byref(params),
+                                 C.byref(mats), capacity, device)
+        if rc != 0:
+            raise V2CError(f"v2c_create failed: {ERRORS.get(rc, rc)}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.v2c_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.v2c_last_error(self.h)
+            raise V2CError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def set_time(self, timesum, timestep):
+        self._check(self.lib.v2c_set_time(self.h, timesum, timestep), "set_time")
+
+    def set_async(self, on: bool):
+        self._check(self.lib.v2c_set_async(self.h, int(on)), "set_async")
+
+    def synchronize(self):
+        self._check(self.lib.v2c_synchronize(self.h), "synchronize")
+
+    def upload_markers(self, m: MarkerArrays):
+        self._check(self.lib.v2c_markers_upload(self.h, C.byref(m.struct), m.n), "markers_upload")
+
+    def download_markers(self, m: MarkerArrays):
+        self._check(self.lib.v2c_markers_download(self.h, C.byref(m.struct), m.n), "markers_download")
+
+    def set_nodal(self, name: str, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.size == self.nodenum, (name, a.size)
+        self._check(self.lib.v2c_nodal_set(self.h, FIELD_ID[name], _dptr(a)), f"nodal_set {name}")
+
+    def get_nodal(self, name: str) -> np.ndarray:
+        out = np.empty(self.nodenum)
+        self._check(self.lib.v2c_nodal_get(self.h, FIELD_ID[name], _dptr(out)), f"nodal_get {name}")
+        return out
+
+    def set_all_nodal(self, nod: dict):
+        for k, v in nod.items():
+            self.set_nodal(k, v)
+
+    def get_all_nodal(self) -> dict:
+        return {k: self.get_nodal(k) for k in NODAL_FIELDS}
+
+    def marker2nodes(self):
+        self._check(self.lib.v2c_marker2nodes(self.h), "marker2nodes")
+
+    def nodes2markers(self):
+        self._check(self.lib.v2c_nodes2markers(self.h), "nodes2markers")
+
+    def moveandrotatemarkers(self):
+        self._check(self.lib.v2c_moveandrotatemarkers(self.h), "moveandrotatemarkers")
+
+    def nodes2markers_advect(self):
+        self._check(self.lib.v2c_nodes2markers_advect(self.h), "nodes2markers_advect")
+
+    def compute_before_solve(self, dt, rk) -> int:
+        nerr = C.c_long(0)
+        self._check(self.lib.v2c_compute_before_solve(self.h, dt, rk, C.byref(nerr)),
+                    "compute_before_solve")
+        return nerr.value
+
+    def compute_after_solve(self, dt, rk):
+        d = C.c_double(0)
+        nerr = C.c_long(0)
+        self._check(self.lib.v2c_compute_after_solve(self.h, dt, rk, C.byref(d), C.byref(nerr)),
+                    "compute_after_solve")
+        return d.value, nerr.value
+
+    def ductilevisc_batch(self, t, sxy, sxx, exy, exx, tk, pb):
+        n = len(t)
+        t8 = np.ascontiguousarray(t, dtype=np.int8)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (sxy, sxx, exy, exx, tk, pb)]
+        out = np.empty(n)
+        it = np.empty(n, dtype=np.int32)
+        self._check(self.lib.v2c_ductilevisc_batch(
+            self.h, n, C.cast(t8.ctypes.data, C.c_char_p), *[_dptr(a) for a in arrs], _dptr(out),
+            it.ctypes.data_as(C.POINTER(C.c_int))), "ductilevisc_batch")
+        return out, it
+
+    def enable_timing(self, on=True):
+        self._check(self.lib.v2c_enable_timing(self.h, int(on)), "enable_timing")
+
+    def reset_timing(self):
+        self._check(self.lib.v2c_reset_timing(self.h), "reset_timing")
+
+    def kernel_time(self, name: str):
+        ms = C.c_double(0)
+        n = C.c_long(0)
+        self._check(self.lib.v2c_kernel_time(self.h, KERNELS.index(name), C.byref(ms), C.byref(n)),
+                    "kernel_time")
+        return ms.value, n.value

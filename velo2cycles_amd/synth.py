@@ -1,0 +1,402 @@
+"""Seeded synthetic inputs for the per-marker hot path.
+
+The reference builds its initial state in in2evps.c (grid :86-130, markers
+:151-241, rock bodies :926-1093, geotherm :1160-1311) and gets its nodal
+velocities from the Stokes solve.  None of that is on the hot path, so this
+module produces inputs of the same shape directly:
+
+* gridlines: uniform, or refined to a fine spacing in a band with geometric
+  stretching outside (same node count);
+* markers: ``per`` x ``per`` per cell at jittered sub-cell positions, generated
+  cell by cell (x-major, y fastest), which is also the cell-sorted order;
+* eight rock types layered by depth with a 15 degree dipping slab and a weak
+  megathrust layer (the material table is committed in this file);
+* nodal velocities from the analytic arc-corner (wedge) flow of a rigid slab
+  subducting at ``conv`` m/s, strain rates by staggered finite differences,
+  lithostatic pressure, viscous stresses.
+
+Everything is float64; every random draw comes from ``numpy.random.default_rng``
+with the given seed.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .abi import (Context, GridArrays, MarkerArrays, Materials, NODAL_FIELDS, Params,
+                  default_params)
+
+YEAR = 3.15576e7            # the reference's year (load_save.c:91)
+CM_PER_YR = 0.01 / YEAR
+DEFAULT_SEED = 20261203
+
+
+# --------------------------------------------------------------------------- materials
+# Columns: nu  dh(J/mol)  dv(J/mol/bar)  ss(Pa)  n  gg(Pa)  n0  n1  ro  bb  aa  cp  kt  kf  kp  ht
+# Power-law prefactors `nu` are calibrated below so that the dislocation-creep
+# viscosity is eta_ref at (T_ref, 10 MPa).
+_TYPES = {
+    # type: (name, kind, eta_ref, T_ref, dh, dv, ss, n, gg, ro, kt, kf)
+    1: ("sticky air", "newt", 1e18, 0, 0, 0, 0, 1.0, 3e12, 1.0, 20.0, 0.0),
+    2: ("upper crust", "pow", 1e23, 600.0, 154e3, 0.8, 3e4, 2.3, 3e10, 2700.0, 0.64, 807.0),
+    3: ("lower crust", "pow", 1e22, 800.0, 238e3, 0.8, 3e4, 3.2, 3e10, 2900.0, 1.18, 474.0),
+    4: ("oceanic crust", "pow", 1e23, 700.0, 238e3, 0.8, 3e4, 3.2, 3e10, 3000.0, 1.18, 474.0),
+    5: ("lithospheric mantle", "pow", 1e22, 1200.0, 532e3, 0.8, 3e4, 3.5, 6.7e10, 3300.0, 0.73, 1293.0),
+    6: ("asthenosphere", "pow", 1e20, 1500.0, 470e3, 0.8, 3e4, 4.0, 6.7e10, 3300.0, 0.73, 1293.0),
+    7: ("slab mantle", "pow", 1e23, 1000.0, 532e3, 0.8, 3e4, 3.5, 6.7e10, 3350.0, 0.73, 1293.0),
+    8: ("megathrust", "newt", 1e19, 0, 0, 0, 0, 1.0, 3e10, 2800.0, 1.18, 474.0),
+}
+
+
+def material_table(ntypes: int = 9) -> Materials:
+    """Eight-type table (type 0 is a Newtonian placeholder, as the reference's
+    type numbering starts at 0, init_RSF.t3c:19-22)."""
+    m = Materials()
+    m.ntypes = ntypes
+    for t in range(256):
+        m.nu[t] = 1e21; m.dh[t] = 0; m.dv[t] = 0; m.ss[t] = 0; m.mm[t] = 1.0; m.gg[t] = 3e10
+        m.n0[t] = 1e17; m.n1[t] = 1e26; m.ro[t] = 3000.0; m.bb[t] = 3e-5; m.aa[t] = 1e-3
+        m.cp[t] = 1000.0; m.kt[t] = 1.0; m.kf[t] = 0.0; m.kp[t] = 4e-6; m.ht[t] = 1e-6
+        m.im[t] = 0
+    for t, (_, kind, eta, tref, dh, dv, ss, n, gg, ro, kt, kf) in _TYPES.items():
+        m.gg[t] = gg; m.ro[t] = ro; m.kt[t] = kt; m.kf[t] = kf
+        if kind == "newt":
+            m.nu[t] = eta
+        else:
+            sig = 1e7
+            e1 = math.exp(min((dh + dv * 1e3) / (8.314 * tref), 150.0))
+            m.nu[t] = 2.0 * eta * sig ** (n - 1.0) / e1
+            m.dh[t] = dh; m.dv[t] = dv; m.ss[t] = ss; m.mm[t] = n
+    m.ro[1] = 1.0
+    m.ht[1] = 0.0
+    return m
+
+
+# --------------------------------------------------------------------------- grids
+def uniform_axis(n: int, size: float) -> np.ndarray:
+    """Gridlines as the reference builds them: g[i] = g[i-1] + size/(n-1) (in2evps.c:134-136)."""
+    g = np.zeros(n)
+    step = size / float(n - 1)
+    for i in range(1, n):
+        g[i] = g[i - 1] + step
+    return g
+
+
+def refined_axis(n: int, size: float, a: float, b: float, h0: float) -> np.ndarray:
+    """n gridlines over [0, size]: spacing h0 on [a, b], geometric growth outside.
+
+    The growth ratio on each side is solved so the node count and size match."""
+    nf = int(round((b - a) / h0))
+    rest = n - 1 - nf
+    left_len, right_len = a, size - b
+    nl = int(round(rest * left_len / max(left_len + right_len, 1e-30)))
+    nr = rest - nl
+
+    def side(length, k):
+        if k <= 0:
+            return np.zeros(0)
+        if length <= k * h0:
+            return np.full(k, length / k)
+        lo, hi = 1.0, 2.0
+        while h0 * (hi ** (k + 1) - hi) / (hi - 1) < length:
+            hi *= 2
+        for _ in range(200):
+            r = 0.5 * (lo + hi)
+            tot = h0 * (r ** (k + 1) - r) / (r - 1) if r != 1 else h0 * k
+            lo, hi = (r, hi) if tot < length else (lo, r)
+        r = 0.5 * (lo + hi)
+        h = h0 * r ** np.arange(1, k + 1)
+        return h * (length / h.sum())
+
+    hl = side(left_len, nl)[::-1]
+    hm = np.full(nf, (b - a) / nf)
+    hr = side(right_len, nr)
+    h = np.concatenate([hl, hm, hr])
+    g = np.concatenate([[0.0], np.cumsum(h)])
+    g[-1] = size
+    return g
+
+
+# --------------------------------------------------------------------------- geometry
+@dataclasses.dataclass
+class Geometry:
+    xtrench: float
+    ytop: float              # top of solid (air above)
+    dip_deg: float = 15.0
+    conv: float = 5 * CM_PER_YR
+    moho_upper: float = 20e3
+    moho_lower: float = 35e3
+    lab: float = 80e3
+    mega_thick: float = 1.5e3
+    ocrust_thick: float = 8e3
+    slab_thick: float = 60e3
+
+    @property
+    def dip(self):
+        return math.radians(self.dip_deg)
+
+    def slab_depth_below_top(self, x, y):
+        """Perpendicular distance below the slab top (negative above)."""
+        d = self.dip
+        return (y - self.ytop) * math.cos(d) - (x - self.xtrench) * math.sin(d)
+
+    def types(self, x, y):
+        t = np.full(x.shape, 6, dtype=np.int8)
+        t[y < self.lab] = 5
+        t[y < self.moho_lower] = 3
+        t[y < self.moho_upper] = 2
+        dist = np.where(x > self.xtrench, self.slab_depth_below_top(x, y), y - self.ytop)
+        slab = (dist >= 0) & (dist < self.slab_thick) & ((x > self.xtrench) | (y >= self.ytop))
+        t[slab & (dist >= self.ocrust_thick)] = 7
+        t[slab & (dist < self.ocrust_thick)] = 4
+        t[(dist >= 0) & (dist < self.mega_thick) & (x > self.xtrench)] = 8
+        t[y < self.ytop] = 1
+        return t
+
+    def velocity(self, x, y):
+        """Arc-corner flow in the wedge above the slab, rigid slab below it,
+        incoming plate moving at `conv` before the trench, air at rest.
+
+        Wedge: psi = r f(theta), f = A sin + C th sin + D th cos (B = 0),
+        f(0)=f'(0)=0 (rigid overriding base), f(d)=0, f'(d)=U (slab surface):
+        A = U d sin d / (d^2 - sin^2 d), C = -A (sin d - d cos d)/(d sin d), D = -A.
+        """
+        U, d = self.conv, self.dip
+        x = np.asarray(x, dtype=np.float64)
+        y = np.asarray(y, dtype=np.float64)
+        vx = np.zeros_like(x)
+        vy = np.zeros_like(x)
+        dx, dy = x - self.xtrench, y - self.ytop
+        th = np.arctan2(dy, dx)
+        A = U * d * math.sin(d) / (d * d - math.sin(d) ** 2)
+        Cc = -A * (math.sin(d) - d * math.cos(d)) / (d * math.sin(d))
+        D = -A
+        wedge = (dx > 0) & (th >= 0) & (th < d)
+        f = A * np.sin(th) + Cc * th * np.sin(th) + D * th * np.cos(th)
+        fp = Cc * (np.sin(th) + th * np.cos(th)) + A * th * np.sin(th)
+        ur, ut = fp, -f
+        wx = ur * np.cos(th) - ut * np.sin(th)
+        wy = ur * np.sin(th) + ut * np.cos(th)
+        vx = np.where(wedge, wx, vx)
+        vy = np.where(wedge, wy, vy)
+        slab = (dx > 0) & (th >= d)
+        vx = np.where(slab, U * math.cos(d), vx)
+        vy = np.where(slab, U * math.sin(d), vy)
+        incoming = (dx <= 0) & (dy >= 0)
+        vx = np.where(incoming, U, vx)
+        return vx, vy
+
+
+# --------------------------------------------------------------------------- markers
+def place_markers(gx, gy, per_x, per_y, rng, jitter=1.0):
+    """per_x*per_y markers per cell at ((i+0.5+U(-0.5,0.5)*jitter)/per) of the cell,
+    cells in x-major order, sub-cell rows y-fastest."""
+    ncx, ncy = len(gx) - 1, len(gy) - 1
+    ix = (np.arange(per_x) + 0.5)[None, :]
+    iy = (np.arange(per_y) + 0.5)[None, :]
+    fx = np.repeat(ix, per_y, axis=1).ravel()          # sub-cell pattern x (per_x*per_y)
+    fy = np.tile(iy, (1, per_x)).ravel()
+    k = per_x * per_y
+    cx = np.repeat(np.arange(ncx), ncy)
+    cy = np.tile(np.arange(ncy), ncx)
+    n = ncx * ncy * k
+    ux = fx[None, :] + jitter * rng.uniform(-0.5, 0.5, (ncx * ncy, k))
+    uy = fy[None, :] + jitter * rng.uniform(-0.5, 0.5, (ncx * ncy, k))
+    x = gx[cx][:, None] + (gx[cx + 1] - gx[cx])[:, None] * ux / per_x
+    y = gy[cy][:, None] + (gy[cy + 1] - gy[cy])[:, None] * uy / per_y
+    return x.reshape(n), y.reshape(n)
+
+
+# --------------------------------------------------------------------------- nodal fields
+def nodal_fields(gx, gy, geom: Geometry, rng, eta=1e21, timestep=YEAR):
+    nx, ny = len(gx), len(gy)
+    X, Y = np.meshgrid(gx, gy, indexing="ij")
+    gxm = np.concatenate([(gx[:-1] + gx[1:]) / 2, [gx[-1]]])
+    gym = np.concatenate([(gy[:-1] + gy[1:]) / 2, [gy[-1]]])
+    gxc = np.concatenate([[gx[0]], (gx[:-1] + gx[1:]) / 2])   # centre node m1 at mid(m1-1, m1)
+    gyc = np.concatenate([[gy[0]], (gy[:-1] + gy[1:]) / 2])
+    vx, _ = geom.velocity(*np.meshgrid(gx, gym, indexing="ij"))
+    _, vy = geom.velocity(*np.meshgrid(gxm, gy, indexing="ij"))
+    XC, YC = np.meshgrid(gxc, gyc, indexing="ij")
+    rho, g = 3000.0, 9.81
+    pr = 1e5 + rho * g * np.maximum(YC - geom.ytop, 0.0)
+
+    exx = np.zeros((nx, ny))
+    dxc = (gx[1:] - gx[:-1])[:, None]
+    dyc = (gy[1:] - gy[:-1])[None, :]
+    exx[1:, 1:] = 0.5 * ((vx[1:, :-1] - vx[:-1, :-1]) / dxc - (vy[:-1, 1:] - vy[:-1, :-1]) / dyc)
+    exy = np.zeros((nx, ny))
+    esp = np.zeros((nx, ny))
+    dvxdy = (vx[1:-1, 1:-1] - vx[1:-1, :-2]) / ((gy[2:] - gy[:-2]) / 2)[None, :]
+    dvydx = (vy[1:-1, 1:-1] - vy[:-2, 1:-1]) / ((gx[2:] - gx[:-2]) / 2)[:, None]
+    exy[1:-1, 1:-1] = 0.5 * (dvxdy + dvydx)
+    esp[1:-1, 1:-1] = 0.5 * (dvxdy - dvydx)
+    sxx = 2 * eta * exx * (1 + 0.1 * rng.standard_normal((nx, ny)))
+    sxy = 2 * eta * exy * (1 + 0.1 * rng.standard_normal((nx, ny))) + 1e6 * rng.standard_normal((nx, ny))
+
+    def c2b(a):
+        out = np.zeros_like(a)
+        out[1:-1, 1:-1] = 0.25 * (a[1:-1, 1:-1] + a[1:-1, 2:] + a[2:, 1:-1] + a[2:, 2:])
+        return out
+
+    G = 3e10
+    nod = {k: np.zeros(nx * ny) for k in NODAL_FIELDS}
+    nod.update(pr=pr, vx=vx, vy=vy, exx=exx, exy=exy, esp=esp, sxx=sxx, sxy=sxy)
+    nod["sxxs"] = c2b(sxx)
+    nod["sppes"] = c2b(pr)
+    nod["exxpl"] = c2b(exx) - c2b(sxx) / (2 * eta)
+    nod["exxel"] = c2b(2 * G * timestep * exx + sxx)
+    V = 1e-9 * np.exp(rng.uniform(-2.0, 2.0, (nx, ny)))
+    nod["RK_sliprs_final"] = V
+    nod["RK_sliprs_solved"] = V * np.exp(rng.uniform(-0.5, 0.5, (nx, ny)))
+    nod["RK_phidts_final"] = 1e-10 * rng.standard_normal((nx, ny))
+    nod["RK_phidts_solved"] = 1e-10 * rng.standard_normal((nx, ny))
+    # antidiffusion state as a previous step would leave it
+    nod["phis_it"] = 10.0 + 0.1 * rng.standard_normal((nx, ny))
+    nod["pes"] = np.abs(1e-3 * rng.standard_normal((nx, ny)))
+    nod["mvx"] = vx.copy()
+    nod["mvy"] = vy.copy()
+    return {k: np.ascontiguousarray(v, dtype=np.float64).reshape(-1) for k, v in nod.items()}
+
+
+# --------------------------------------------------------------------------- configs
+@dataclasses.dataclass
+class Config:
+    name: str
+    grid: GridArrays
+    params: Params
+    mats: Materials
+    markers: MarkerArrays
+    nodal: dict
+    timestep: float
+    timesum: float
+    geom: Geometry | None = None
+    fault: np.ndarray | None = None     # bool mask of fault-zone markers
+
+    @property
+    def n(self):
+        return self.markers.n
+
+
+def build_config(name, gx, gy, per, geom, seed, timestep=YEAR, timesum=0.0, jitter=1.0,
+                 fault_frac=0.0, params: Params | None = None):
+    rng = np.random.default_rng(seed)
+    grid = GridArrays(gx, gy)
+    prm = params or default_params(grid.xsize, grid.xnumx)
+    mats = material_table()
+    x, y = place_markers(gx, gy, per, per, rng, jitter)
+    n = len(x)
+    t = geom.types(x, y)
+    T = 273.0 + (1600.0 - 273.0) * np.clip(y / grid.ysize, 0, 1)
+    slab = (t == 4) | (t == 7) | (t == 8)
+    T = np.where(slab, np.maximum(T - 300.0, 273.0), T)
+    T = np.where(t == 1, 273.0, T)
+    vx, vy = geom.velocity(x, y)
+    p = 1e5 + 3000.0 * 9.81 * np.maximum(y - geom.ytop, 0.0)
+    fields = dict(
+        x=x, y=y, k=T, p=p,
+        xx=rng.normal(0.0, 1e7, n), xy=rng.normal(0.0, 1e7, n),
+        exx=rng.normal(0.0, 1e-14, n), exy=rng.normal(0.0, 1e-14, n),
+        e=rng.uniform(0.0, 0.1, n), phi=10.0 + 0.1 * rng.standard_normal(n),
+        vx=vx, vy=vy, C=rng.uniform(0.0, 1e7, n), d=np.full(n, 3000.0), w=np.full(n, 1e7),
+        t=t)
+    fault = None
+    if fault_frac > 0:
+        dist = geom.slab_depth_below_top(x, y)
+        cand = np.where((x > geom.xtrench) & (dist >= 0))[0]
+        k = int(fault_frac * n)
+        order = np.argsort(dist[cand], kind="stable")
+        fault = np.zeros(n, dtype=bool)
+        fault[cand[order[:k]]] = True
+        a = rng.uniform(0.008, 0.02, n)
+        L = rng.uniform(0.01, 0.05, n)
+        V = 10.0 ** rng.uniform(-12.0, 0.0, n)
+        theta = L / V * rng.uniform(0.5, 2.0, n)
+        fields["a"] = np.where(fault, a, 0.0)
+        fields["b"] = np.where(fault, 0.015, 0.0)
+        fields["L"] = np.where(fault, L, 0.0)
+        fields["L0"] = fields["L"].copy()
+        fields["V0"] = np.where(fault, prm.V0_vw, 0.0)
+        fields["fric_static"] = np.where(fault, 0.6, 0.0)
+        fields["phi"] = np.where(fault, np.log(prm.V0_vw * theta / L), fields["phi"])
+        fields["_V"] = np.where(fault, V, 0.0)
+    markers = MarkerArrays(n, fields)
+    if fault is not None:
+        markers.rsf_V = fields["_V"]
+    nod = nodal_fields(gx, gy, geom, rng, timestep=timestep)
+    return Config(name, grid, prm, mats, markers, nod, timestep, timesum, geom, fault)
+
+
+def config_small(seed=DEFAULT_SEED, **kw):
+    """401x101 nodes over 400x100 km, uniform, 16 markers per cell (640,000)."""
+    gx = uniform_axis(401, 400e3)
+    gy = uniform_axis(101, 100e3)
+    geom = Geometry(xtrench=150e3, ytop=8e3)
+    return build_config("plain-small-401x101-16pc", gx, gy, 4, geom, seed, **kw)
+
+
+def config_production(seed=DEFAULT_SEED, **kw):
+    """1501x301 nodes over 1500x300 km refined to 500 m near the megathrust,
+    36 markers per cell (~16.2 M), ~2 % fault markers."""
+    gx = refined_axis(1501, 1500e3, 450e3, 800e3, 500.0)
+    gy = refined_axis(301, 300e3, 0.0, 60e3, 500.0)
+    geom = Geometry(xtrench=500e3, ytop=10e3)
+    kw.setdefault("fault_frac", 0.02)
+    return build_config("plain-production-1501x301-36pc", gx, gy, 6, geom, seed, **kw)
+
+
+def config_grid(nx, ny, xsize, ysize, per, seed=DEFAULT_SEED, refine=False, **kw):
+    """Generic case (used for reference-legal parity sizes)."""
+    if refine:
+        gx = refined_axis(nx, xsize, 0.35 * xsize, 0.6 * xsize, 0.6 * xsize / (nx - 1))
+        gy = refined_axis(ny, ysize, 0.0, 0.3 * ysize, 0.6 * ysize / (ny - 1))
+    else:
+        gx = uniform_axis(nx, xsize)
+        gy = uniform_axis(ny, ysize)
+    geom = Geometry(xtrench=0.3 * xsize, ytop=0.08 * ysize)
+    return build_config(f"grid-{nx}x{ny}-{per * per}pc", gx, gy, per, geom, seed, **kw)
+
+
+def config_edge(seed=DEFAULT_SEED):
+    """Tiny refined grid whose marker cloud hits every branch of the path:
+    markers outside the domain, on gridlines and cell midlines, types >= 50,
+    an immobile type, markers with T <= 0 (newly incoming), and every rock type."""
+    cfg = config_grid(13, 9, 24e3, 16e3, 3, seed=seed, refine=True)
+    rng = np.random.default_rng(seed + 1)
+    m = cfg.markers
+    x, y, t, k = m["x"], m["y"], m.t, m["k"]
+    n = m.n
+    gx, gy = cfg.grid.gx, cfg.grid.gy
+    idx = rng.permutation(n)
+    j = 0
+
+    def take(c):
+        nonlocal j
+        s = idx[j:j + c]
+        j += c
+        return s
+
+    s = take(6); x[s] = [-5.0, cfg.grid.xsize + 3.0, 100.0, 200.0, -1.0, cfg.grid.xsize + 1.0]
+    y[s] = [300.0, 400.0, -7.0, cfg.grid.ysize + 2.0, -1.0, cfg.grid.ysize + 1.0]
+    s = take(6); x[s] = gx[rng.integers(1, len(gx) - 1, 6)]          # on x gridlines
+    s = take(6); y[s] = gy[rng.integers(1, len(gy) - 1, 6)]          # on y gridlines
+    s = take(6); c = rng.integers(0, len(gx) - 1, 6); x[s] = (gx[c] + gx[c + 1]) / 2  # cx == 0.5
+    s = take(6); c = rng.integers(0, len(gy) - 1, 6); y[s] = (gy[c] + gy[c + 1]) / 2  # cy == 0.5
+    s = take(4); x[s[:2]] = 0.0; y[s[2:]] = cfg.grid.ysize                            # on the boundary
+    s = take(10); t[s] = rng.integers(0, 9, 10)                                        # every type
+    s = take(4); t[s] = [50, 60, 75, 99]                                               # types >= 50
+    s = take(4); k[s] = [0.0, -1.0, 0.0, -5.0]                                         # incoming
+    s = take(6); y[s] = cfg.params.band_y + rng.uniform(-100.0, 100.0, 6)             # RSF band
+    x[s] = rng.uniform(0.0, cfg.grid.xsize, 6)
+    cfg.mats.im[9] = 1                                                                  # immobile type 9
+    s = take(3); t[s] = 9
+    cfg.name = "edge-13x9-9pc"
+    return cfg
+
+
+def upload(ctx: Context, cfg: Config):
+    ctx.set_time(cfg.timesum, cfg.timestep)
+    ctx.set_all_nodal(cfg.nodal)
+    ctx.upload_markers(cfg.markers)
