@@ -1,0 +1,154 @@
+"""Throughput of the per-marker hot path on one MI355X.
+
+One step = marker2nodes -> compute_before_solve -> compute_after_solve ->
+nodes2markers + moveandrotatemarkers (fused), i.e. the reference time loop
+(i2evps.c:89-113) minus the host Stokes/heat solves, whose nodal outputs are
+synthetic fields already resident in HBM.  Workload: 1501x301 nodes over
+1500x300 km refined to 500 m near the megathrust, 36 markers per cell
+(16.2 M markers, FP64 SoA), dt = 1 yr.
+
+Prints ONE JSON line (rank 0).  The cpu_baseline leg runs the FP64 oracle on
+a bounded sample of the same marker cloud.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+import numpy as np  # noqa: E402
+
+from velo2cycles_amd import synth  # noqa: E402
+from velo2cycles_amd.abi import Context, load_lib  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per marker (FP64 = 8 B, type = 1 B), REFERENCE mode:
+#  marker2nodes reads x y T P sxx sxy exx exy e phi vx vy C (13x8) + type,
+#    writes the 7 calc_frictionpara outputs (7x8)                 -> 105 R + 56 W
+#  nodes2markers+advection reads x y T + type, writes phi exx exy
+#    sxx sxy P vx vy e x y (11x8)                                 ->  25 R + 88 W
+B_M2N_MARKER = 105 + 56
+B_N2M_MARKER = 25 + 88
+# Nodal traffic per node and step (each nodal array touched once): marker2nodes
+# reads 19 fields and writes 50; nodes2markers+advection reads 24 and
+# read-writes 15 sums + writes 9; the two solve-side passes read ~20 and write
+# ~12 fields.
+B_M2N_NODE = (19 + 50) * 8
+B_N2M_NODE = (24 + 15 * 2 + 9) * 8
+B_SOLVE_NODE = 32 * 8
+
+
+def cpu_baseline(cfg, sample: int, steps: int = 2):
+    """FP64 oracle, one thread, full step on the first `sample` markers of the
+    same cloud (x-major cell order) with the full nodal grid."""
+    from oracle.pyoracle import Oracle
+    from velo2cycles_amd.abi import MarkerArrays
+    fields = {f: cfg.markers[f][:sample] for f in cfg.markers.arrays}
+    fields["t"] = cfg.markers.t[:sample]
+    sub = synth.Config(cfg.name, cfg.grid, cfg.params, cfg.mats, MarkerArrays(sample, fields),
+                       cfg.nodal, cfg.timestep, cfg.timesum)
+    o = Oracle(sub)
+    dt = cfg.timestep
+    t0 = time.perf_counter()
+    for s in range(steps):
+        o.set_time(cfg.timesum + s * dt, dt)
+        o.marker2nodes()
+        o.compute_before_solve(dt, 0)
+        o.compute_after_solve(dt, 0)
+        o.nodes2markers()
+        o.moveandrotatemarkers()
+    el = time.perf_counter() - t0
+    return {"value": sample * steps / el, "unit": "marker-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (FP64 C restatement, 1 thread) on the first {sample:,} markers of the "
+                      f"same cloud with the full {cfg.grid.nodenum:,}-node grid, {steps} full steps, "
+                      f"{el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="production", choices=["production", "small"])
+    ap.add_argument("--cpu-sample", type=int, default=3_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
+                                   "(ensemble sharding across GPUs is a design note only)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    cfg = synth.config_production() if args.config == "production" else synth.config_small()
+    lib = load_lib()
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=lib)
+    dt = cfg.timestep
+    ctx.set_time(0.0, dt)
+    ctx.set_all_nodal(cfg.nodal)
+    ctx.upload_markers(cfg.markers)
+    h = ctx.h
+    ctx.set_async(True)
+
+    def step(ts):
+        lib.v2c_set_time(h, ts, dt)
+        for rc in (lib.v2c_marker2nodes(h), lib.v2c_compute_before_solve(h, dt, 0, None),
+                   lib.v2c_compute_after_solve(h, dt, 0, None, None), lib.v2c_nodes2markers_advect(h)):
+            if rc != 0:
+                raise RuntimeError(lib.v2c_last_error(h).decode())
+
+    ts = 0.0
+    for _ in range(args.warmup):
+        step(ts)
+        ts += dt
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(ts)
+        ts += dt
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    ctx.enable_timing(False)
+    kern = {}
+    for k in ("m2n_cells", "m2n_nodes", "n2m_cells", "n2m_nodes", "sort", "before", "after"):
+        ms, nl = ctx.kernel_time(k)
+        kern[k] = {"ms_total": ms, "launches": nl, "ms_per_step": ms / args.steps}
+    n, nn = cfg.n, cfg.grid.nodenum
+    ms_step = el / args.steps * 1e3
+    bytes_step = (n * (B_M2N_MARKER + B_N2M_MARKER) + nn * (B_M2N_NODE + B_N2M_NODE + B_SOLVE_NODE))
+    dom = max(("m2n_cells", "n2m_cells"), key=lambda k: kern[k]["ms_total"])
+    per_launch_bytes = (n * B_M2N_MARKER + nn * 8) if dom == "m2n_cells" else \
+        (n * B_N2M_MARKER + nn * 24 * 8)
+    avg_ms = kern[dom]["ms_total"] / max(kern[dom]["launches"], 1)
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "marker-steps/sec", "value": n * args.steps / el, "unit": "marker-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded marker cloud, analytic corner-flow nodal fields)",
+        "config": {"workload": cfg.name, "markers": n, "nodes": nn, "markers_per_cell": 36
+                   if args.config == "production" else 16, "dt_s": dt, "grid": "1501x301 refined to 500 m"
+                   if args.config == "production" else "401x101 uniform"},
+        "hbm_frac_step": bytes_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms},
+        "kernels_ms_per_step": {k: v["ms_per_step"] for k, v in kern.items()},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, min(args.cpu_sample, cfg.n))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

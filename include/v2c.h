@@ -170,6 +170,11 @@ int v2c_create(v2c_ctx **ctx, const v2c_grid *grid, const v2c_params *prm,
 void v2c_destroy(v2c_ctx *ctx);
 const char *v2c_last_error(const v2c_ctx *ctx);
 int v2c_set_async(v2c_ctx *ctx, int async_mode);
+/* 1 (default): marker2nodes writes the per-marker friction parameters that
+ * calc_frictionpara produces (calc_RSF.c:264-293: fric_static, mark_a/_b/_L/
+ * _L0/_V0, markC0).  The hot path never reads them back (their only reader is
+ * the checkpoint writer, input_RSF.c:106-116); 0 skips these writes. */
+int v2c_set_friction_fields(v2c_ctx *ctx, int on);
 int v2c_synchronize(v2c_ctx *ctx);
 
 /* Mirror of the reference globals timesum / timestep (headevps.c:343-348). */

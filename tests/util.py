@@ -1,0 +1,116 @@
+"""Shared helpers of the parity tests: running a Config through the oracle and
+through libv2c, and the field-by-field comparison bar."""
+from __future__ import annotations
+
+import numpy as np
+
+from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS, Context
+from oracle.pyoracle import Oracle
+
+# FP64 tolerance: |gpu - oracle| <= REL * max(|oracle|, rms(field)) elementwise
+# (the rms floor keeps signed fields with near-zero entries meaningful).
+REL = 1e-12
+# positions and integer state must match bit for bit
+EXACT_MARKER = ("x", "y")
+
+
+def rel_errors(a: np.ndarray, b: np.ndarray) -> tuple[float, float, int]:
+    """(max scaled error, rms scaled error, #nan mismatches) of a against b."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nan_mis = int(np.sum(np.isnan(a) != np.isnan(b)))
+    ok = ~(np.isnan(a) | np.isnan(b))
+    if not ok.any():
+        return 0.0, 0.0, nan_mis
+    a, b = a[ok], b[ok]
+    inf = np.isinf(a) | np.isinf(b)
+    nan_mis += int(np.sum(a[inf] != b[inf]))
+    a, b = a[~inf], b[~inf]
+    if a.size == 0:
+        return 0.0, 0.0, nan_mis
+    rms = np.sqrt(np.mean(b * b))
+    scale = np.maximum(np.abs(b), rms)
+    scale[scale == 0] = 1.0
+    e = np.abs(a - b) / scale
+    return float(e.max()), float(np.sqrt(np.mean(e * e))), nan_mis
+
+
+def compare(gpu_m, ora_m, gpu_n, ora_n, rel=REL, nodal_fields=NODAL_FIELDS, skip=()):
+    """Return a list of failure strings (empty == parity)."""
+    bad = []
+    if not np.array_equal(gpu_m.t, ora_m.t):
+        bad.append("marker types differ")
+    for f in MARKER_FIELDS:
+        if f in skip:
+            continue
+        a, b = gpu_m[f], ora_m[f]
+        if f in EXACT_MARKER:
+            if not np.array_equal(a, b, equal_nan=True):
+                d = np.nanmax(np.abs(a - b))
+                bad.append(f"marker {f}: not bit-exact (max |d| {d:.3e}, {int(np.sum(a != b))} differ)")
+            continue
+        mx, _, nm = rel_errors(a, b)
+        if mx > rel or nm:
+            bad.append(f"marker {f}: max scaled err {mx:.3e}, nan mismatches {nm}")
+    for k in nodal_fields:
+        if k in skip:
+            continue
+        mx, _, nm = rel_errors(gpu_n[k], ora_n[k])
+        if mx > rel or nm:
+            bad.append(f"nodal {k}: max scaled err {mx:.3e}, nan mismatches {nm}")
+    return bad
+
+
+def run_oracle(cfg, ops, markers=None, nodal=None):
+    o = Oracle(cfg, markers, nodal)
+    ret = {}
+    for op in ops:
+        name, *args = op.split(":")
+        if name == "m2n":
+            o.marker2nodes()
+        elif name == "n2m":
+            o.nodes2markers()
+        elif name == "move":
+            o.moveandrotatemarkers()
+        elif name == "n2m_move":
+            o.nodes2markers()
+            o.moveandrotatemarkers()
+        elif name == "before":
+            ret["before"] = o.compute_before_solve(float(args[0]), int(args[1]))
+        elif name == "after":
+            ret["after"] = o.compute_after_solve(float(args[0]), int(args[1]))
+        elif name == "time":
+            o.set_time(float(args[0]), float(args[1]))
+        else:
+            raise ValueError(op)
+    return o.markers, o.nodal, ret
+
+
+def run_gpu(lib, cfg, ops, markers=None, nodal=None, ctx=None):
+    ctx = ctx or Context(cfg.grid, cfg.params, cfg.mats, capacity=max(cfg.n, 1), lib=lib)
+    ctx.set_time(cfg.timesum, cfg.timestep)
+    ctx.set_all_nodal(nodal or cfg.nodal)
+    m = (markers or cfg.markers).copy()
+    ctx.upload_markers(m)
+    ret = {}
+    for op in ops:
+        name, *args = op.split(":")
+        if name == "m2n":
+            ctx.marker2nodes()
+        elif name == "n2m":
+            ctx.nodes2markers()
+        elif name == "move":
+            ctx.moveandrotatemarkers()
+        elif name == "n2m_move":
+            ctx.nodes2markers_advect()
+        elif name == "before":
+            ret["before"] = ctx.compute_before_solve(float(args[0]), int(args[1]))
+        elif name == "after":
+            ret["after"] = ctx.compute_after_solve(float(args[0]), int(args[1]))
+        elif name == "time":
+            ctx.set_time(float(args[0]), float(args[1]))
+        else:
+            raise ValueError(op)
+    ctx.download_markers(m)
+    nod = ctx.get_all_nodal()
+    return m, nod, ret, ctx

@@ -1,0 +1,604 @@
+// v2c_api.hip -- C-ABI of libv2c: context, device memory, ordering and the
+// hot-path entry points declared in include/v2c.h.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#include "v2c_kernels.hip"
+
+using namespace v2c;
+
+struct TimedLaunch { int kernel; hipEvent_t start, stop; };
+
+struct v2c_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int async_mode = 0;
+    int write_friction = 1;
+    v2c_params prm;
+    v2c_materials mat;
+    long xnumx = 0, ynumy = 0, nodenum = 0, ncell = 0;
+    double xsize = 0, ysize = 0;
+    long cap = 0, n = 0;
+    double timesum = 0, timestep = 0;
+    int key_bits = 1;
+    bool order_valid = false;
+    // device memory
+    double *d_gx = nullptr, *d_gy = nullptr;
+    DevMat *d_mat = nullptr;
+    double *d_mk = nullptr;
+    int8_t *d_t = nullptr;
+    uint32_t *d_key = nullptr, *d_skey = nullptr, *d_iota = nullptr, *d_order = nullptr, *d_cs = nullptr;
+    double *d_nod = nullptr, *d_part = nullptr, *d_red = nullptr;
+    int *d_counters = nullptr;
+    void *d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    // timing
+    int timing = 0;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    double ktime[V2C_K_COUNT] = {0};
+    long klaunch[V2C_K_COUNT] = {0};
+    char err[1024] = {0};
+};
+
+static int set_err(v2c_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return set_err((c), V2C_EHIP, "%s failed: %s (%s:%d)", #expr,                 \
+                           hipGetErrorString(e_), __FILE__, __LINE__);                    \
+    } while (0)
+
+static hipEvent_t take_event(v2c_ctx *c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch helper: records events around the launch when timing is on.
+template <typename F>
+static int launch(v2c_ctx *c, int kid, F &&fn)
+{
+    TimedLaunch tl{kid, nullptr, nullptr};
+    if (c->timing) {
+        tl.start = take_event(c);
+        tl.stop = take_event(c);
+        if (tl.start) hipEventRecord(tl.start, c->stream);
+    }
+    fn();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, V2C_EHIP, "kernel %d launch failed: %s", kid, hipGetErrorString(e));
+    if (c->timing && tl.start && tl.stop) {
+        hipEventRecord(tl.stop, c->stream);
+        c->pending.push_back(tl);
+    }
+    return 0;
+}
+
+static int collect_timing(v2c_ctx *c)
+{
+    if (c->pending.empty()) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &tl : c->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, tl.start, tl.stop);
+        c->ktime[tl.kernel] += ms;
+        c->klaunch[tl.kernel] += 1;
+        c->event_pool.push_back(tl.start);
+        c->event_pool.push_back(tl.stop);
+    }
+    c->pending.clear();
+    return 0;
+}
+
+static KArgs make_args(v2c_ctx *c)
+{
+    KArgs a;
+    a.g.gx = c->d_gx; a.g.gy = c->d_gy;
+    a.g.xnumx = (int)c->xnumx; a.g.ynumy = (int)c->ynumy;
+    a.g.xsize = c->xsize; a.g.ysize = c->ysize;
+    a.p = c->prm;
+    a.mat = c->d_mat;
+    a.mk = c->d_mk; a.cap = c->cap; a.t = c->d_t; a.n = c->n;
+    a.order = c->d_order; a.cell_start = c->d_cs; a.key = c->d_key;
+    a.nod = c->d_nod; a.nodenum = c->nodenum; a.ncell = c->ncell;
+    a.part = c->d_part;
+    a.timesum = c->timesum; a.timestep = c->timestep;
+    a.counters = c->d_counters;
+    a.write_friction = c->write_friction;
+    return a;
+}
+
+static int finish(v2c_ctx *c)
+{
+    if (!c->async_mode) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Stable sort of (sub-cell key, marker id) -> order[], then cell_start[].
+static int ensure_order(v2c_ctx *c)
+{
+    if (c->order_valid) return 0;
+    KArgs a = make_args(c);
+    int rc;
+    if (c->n > 0) {
+        rc = launch(c, V2C_K_SORT, [&] {
+            hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp, c->sort_tmp_bytes, c->d_key, c->d_skey,
+                                               c->d_iota, c->d_order, (int)c->n, 0, c->key_bits, c->stream);
+        });
+        if (rc) return rc;
+    }
+    rc = launch(c, V2C_K_SORT, [&] {
+        hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(c->n + 1, 256)), dim3(256), 0, c->stream,
+                           c->d_skey, c->n, c->ncell, c->d_cs);
+    });
+    if (rc) return rc;
+    (void)a;
+    c->order_valid = true;
+    return 0;
+}
+
+extern "C" {
+
+void v2c_params_default(v2c_params *p)
+{
+    memset(p, 0, sizeof *p);
+    p->markmod = 2; p->stoksmod = 1; p->outgrid = 2; p->densimod = 0; p->interpolationmod = 2;
+    p->antidiffinter = 1; p->rkmod = 0; p->RSFmode = 1; p->power_law_RSF = 0; p->reloadfromDP = 0;
+    p->compressiblemode = 0; p->stickymod = 1; p->inertyn = 0; p->pmode = 0; p->tempmod = 0;
+    p->n2m_phantom = 1;
+    p->nubeg = 1e1; p->nuend = 1.667e27; p->tdeep = 1873.0; p->zdeep = 3000000.0; p->incoming_T = 0.0;
+    p->nstress = 10.0e6; p->Dphis = 0.0; p->half_range = 2.0e3;
+    p->start_updip = 10.0e3 - p->half_range; p->end_updip = 10.0e3 + p->half_range;
+    p->start_downdip = 40.0e3 - p->half_range; p->end_downdip = 40.0e3 + p->half_range;
+    p->a_vw = 0.009; p->b_vw = 0.017; p->L_vw = 0.0100; p->V0_vw = 4.0e-09;
+    p->a_vs = 0.020; p->b_vs = 0.001; p->L_vs = 0.0100; p->V0_vs = 4.0e-09;
+    p->a_hr = 0.009; p->b_hr = 0.017; p->L_hr = 0.0100; p->V0_hr = 4.0e-09;
+    p->xstpx = 0.0;
+    p->faultwidth = 400.0; p->rad_damping = 4.3303e+06; p->bulk_modulus = 50e9;
+    p->band_y = 10.000e3; p->band_halfwidth = 100.0;
+}
+
+static const char *const kNodalNames[V2C_NF] = {
+    "pr", "vx", "vy", "exx", "exy", "esp", "sxx", "sxy", "sxxs", "sppes", "exxpl", "exxel",
+    "bm", "ro", "et", "nu", "nd", "gg", "gd", "sppe", "phis", "phin", "sxyen", "sxxen", "exyen",
+    "exxen", "sppen", "mus_s", "as", "bs", "Ls", "V0s", "sxxe", "sxye", "exxe", "exye", "cohesion",
+    "dro", "drp", "cp", "kt", "ht", "tk", "mrx", "mry", "mvx", "mvy", "pe", "pen", "nu_m2c",
+    "nd_m2c", "mvxn", "mvyn",
+    "phie", "sxyee", "exyee", "pee", "sppee", "sxxee", "exxee", "mvxe", "mvye",
+    "wtphisum", "phisum", "sxysum", "exysum", "esum", "wtsxxsum", "psum", "sxxsum", "exxsum",
+    "wtvxsum", "vxsum", "wtvysum", "vysum", "mrx0", "mry0",
+    "pes", "phis_it", "syields", "nuvp_it", "sliprs", "RK_sliprs_final", "RK_phidts_final",
+    "RK_sliprs_solved", "RK_phidts_solved", "RK_deltas", "RK_sliprs0", "RK_sliprs1",
+    "RK_phidts0", "RK_phidts1"};
+
+const char *v2c_nodal_name(int field)
+{
+    return (field >= 0 && field < V2C_NF) ? kNodalNames[field] : nullptr;
+}
+
+const char *v2c_last_error(const v2c_ctx *c) { return c ? c->err : "null context"; }
+
+static void free_all(v2c_ctx *c)
+{
+    void *ptrs[] = {c->d_gx, c->d_gy, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
+                    c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_red, c->d_counters, c->d_sort_tmp};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto &tl : c->pending) { hipEventDestroy(tl.start); hipEventDestroy(tl.stop); }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+
+int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2c_materials *mat,
+               long capacity, int device)
+{
+    if (!out || !g || !prm || !mat) return V2C_EINVAL;
+    *out = nullptr;
+    if (g->xnumx < 4 || g->ynumy < 4 || !g->gx || !g->gy || capacity < 0 || capacity > 0x7fffffffL)
+        return V2C_EINVAL;
+    if (prm->interpolationmod != 2) return V2C_EUNSUPPORTED;
+    if (mat->ntypes < 0 || mat->ntypes > V2C_MAX_TYPES) return V2C_EINVAL;
+    for (long i = 1; i < g->xnumx; i++) if (!(g->gx[i] > g->gx[i - 1])) return V2C_EINVAL;
+    for (long i = 1; i < g->ynumy; i++) if (!(g->gy[i] > g->gy[i - 1])) return V2C_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return V2C_ENOGPU;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return V2C_ENOGPU;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return V2C_ENOGPU;
+
+    v2c_ctx *c = new v2c_ctx();
+    c->device = device;
+    c->prm = *prm;
+    c->mat = *mat;
+    c->xnumx = g->xnumx; c->ynumy = g->ynumy; c->nodenum = g->xnumx * g->ynumy;
+    c->ncell = (g->xnumx - 1) * (g->ynumy - 1);
+    c->xsize = g->xsize; c->ysize = g->ysize;
+    c->cap = capacity > 0 ? capacity : 1;
+    while ((1L << c->key_bits) < 4 * c->ncell) c->key_bits++;
+    if (c->prm.xstpx == 0.0) c->prm.xstpx = g->xsize / (double)(g->xnumx - 1);
+#define CK(expr)                                                                      \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            free_all(c);                                                              \
+            delete c;                                                                 \
+            return e_ == hipErrorOutOfMemory ? V2C_ENOMEM : V2C_EHIP;                 \
+        }                                                                             \
+    } while (0)
+    CK(hipSetDevice(device));
+    CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CK(hipMalloc(&c->d_gx, sizeof(double) * g->xnumx));
+    CK(hipMalloc(&c->d_gy, sizeof(double) * g->ynumy));
+    CK(hipMalloc(&c->d_mat, sizeof(DevMat)));
+    CK(hipMalloc(&c->d_mk, sizeof(double) * NMF * c->cap));
+    CK(hipMalloc(&c->d_t, c->cap));
+    CK(hipMalloc(&c->d_key, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_skey, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_iota, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_order, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_cs, sizeof(uint32_t) * (c->ncell + 1)));
+    CK(hipMalloc(&c->d_nod, sizeof(double) * V2C_NF * c->nodenum));
+    CK(hipMalloc(&c->d_part, sizeof(double) * MP_N * c->ncell));
+    CK(hipMalloc(&c->d_red, sizeof(double) * (2 * blocks_for(c->nodenum, RED_BLOCK) + 8)));
+    CK(hipMalloc(&c->d_counters, sizeof(int) * 8));
+    CK(hipMemsetAsync(c->d_nod, 0, sizeof(double) * V2C_NF * c->nodenum, c->stream));
+    CK(hipMemsetAsync(c->d_mk, 0, sizeof(double) * NMF * c->cap, c->stream));
+    CK(hipMemsetAsync(c->d_t, 0, c->cap, c->stream));
+    CK(hipMemsetAsync(c->d_counters, 0, sizeof(int) * 8, c->stream));
+    CK(hipMemcpyAsync(c->d_gx, g->gx, sizeof(double) * g->xnumx, hipMemcpyHostToDevice, c->stream));
+    CK(hipMemcpyAsync(c->d_gy, g->gy, sizeof(double) * g->ynumy, hipMemcpyHostToDevice, c->stream));
+    {
+        DevMat dm;
+        memcpy(dm.nu, mat->nu, sizeof dm.nu); memcpy(dm.dh, mat->dh, sizeof dm.dh);
+        memcpy(dm.dv, mat->dv, sizeof dm.dv); memcpy(dm.ss, mat->ss, sizeof dm.ss);
+        memcpy(dm.mm, mat->mm, sizeof dm.mm); memcpy(dm.gg, mat->gg, sizeof dm.gg);
+        memcpy(dm.n0, mat->n0, sizeof dm.n0); memcpy(dm.n1, mat->n1, sizeof dm.n1);
+        memcpy(dm.ro, mat->ro, sizeof dm.ro); memcpy(dm.bb, mat->bb, sizeof dm.bb);
+        memcpy(dm.aa, mat->aa, sizeof dm.aa); memcpy(dm.cp, mat->cp, sizeof dm.cp);
+        memcpy(dm.kt, mat->kt, sizeof dm.kt); memcpy(dm.kf, mat->kf, sizeof dm.kf);
+        memcpy(dm.kp, mat->kp, sizeof dm.kp); memcpy(dm.ht, mat->ht, sizeof dm.ht);
+        memcpy(dm.im, mat->im, sizeof dm.im);
+        CK(hipMemcpyAsync(c->d_mat, &dm, sizeof dm, hipMemcpyHostToDevice, c->stream));
+        CK(hipStreamSynchronize(c->stream));
+    }
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(c->cap, 256)), dim3(256), 0, c->stream, c->d_iota, c->cap);
+    CK(hipGetLastError());
+    CK(hipcub::DeviceRadixSort::SortPairs(nullptr, c->sort_tmp_bytes, c->d_key, c->d_skey, c->d_iota,
+                                          c->d_order, (int)c->cap, 0, c->key_bits, c->stream));
+    CK(hipMalloc(&c->d_sort_tmp, c->sort_tmp_bytes > 0 ? c->sort_tmp_bytes : 16));
+    CK(hipStreamSynchronize(c->stream));
+#undef CK
+    *out = c;
+    return 0;
+}
+
+void v2c_destroy(v2c_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+int v2c_set_async(v2c_ctx *c, int on)
+{
+    if (!c) return V2C_EINVAL;
+    c->async_mode = on ? 1 : 0;
+    return 0;
+}
+
+int v2c_set_friction_fields(v2c_ctx *c, int on)
+{
+    if (!c) return V2C_EINVAL;
+    c->write_friction = on ? 1 : 0;
+    return 0;
+}
+
+int v2c_synchronize(v2c_ctx *c)
+{
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_set_time(v2c_ctx *c, double timesum, double timestep)
+{
+    if (!c) return V2C_EINVAL;
+    c->timesum = timesum;
+    c->timestep = timestep;
+    return 0;
+}
+
+long v2c_marker_count(const v2c_ctx *c) { return c ? c->n : -1; }
+
+static double *host_field(const v2c_markers *m, int f)
+{
+    double *const p[NMF] = {m->x, m->y, m->k, m->p, m->xx, m->xy, m->exx, m->exy, m->e, m->phi, m->vx,
+                            m->vy, m->C, m->d, m->w, m->a, m->b, m->L, m->L0, m->V0, m->fric_static, m->C0};
+    return p[f];
+}
+
+int v2c_markers_upload(v2c_ctx *c, const v2c_markers *m, long n)
+{
+    if (!c || !m || n < 0) return V2C_EINVAL;
+    if (n > c->cap) return set_err(c, V2C_ECAPACITY, "marker count %ld above capacity %ld", n, c->cap);
+    for (int f = 0; f < NMF; f++) if (!host_field(m, f)) return set_err(c, V2C_EINVAL, "marker field %d is NULL", f);
+    if (!m->t) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int f = 0; f < NMF; f++)
+        HIPCHK(c, hipMemcpyAsync(c->d_mk + (size_t)f * c->cap, host_field(m, f), sizeof(double) * n,
+                                 hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_t, m->t, n, hipMemcpyHostToDevice, c->stream));
+    c->n = n;
+    KArgs a = make_args(c);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a);
+        HIPCHK(c, hipGetLastError());
+    }
+    c->order_valid = false;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_markers_download(v2c_ctx *c, v2c_markers *m, long n)
+{
+    if (!c || !m || n < 0) return V2C_EINVAL;
+    if (n > c->n) return set_err(c, V2C_EINVAL, "download of %ld markers, %ld resident", n, c->n);
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int f = 0; f < NMF; f++) {
+        double *h = host_field(m, f);
+        if (!h) return V2C_EINVAL;
+        HIPCHK(c, hipMemcpyAsync(h, c->d_mk + (size_t)f * c->cap, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    if (m->t) HIPCHK(c, hipMemcpyAsync(m->t, c->d_t, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_nodal_set(v2c_ctx *c, int field, const double *host)
+{
+    if (!c || !host || field < 0 || field >= V2C_NF) return V2C_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(c->d_nod + (size_t)field * c->nodenum, host, sizeof(double) * c->nodenum,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_nodal_get(v2c_ctx *c, int field, double *host)
+{
+    if (!c || !host || field < 0 || field >= V2C_NF) return V2C_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(host, c->d_nod + (size_t)field * c->nodenum, sizeof(double) * c->nodenum,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_marker2nodes(v2c_ctx *c)
+{
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_order(c);
+    if (rc) return rc;
+    KArgs a = make_args(c);
+    const int nwg = (int)((c->ncell + CPW - 1) / CPW);
+    rc = launch(c, V2C_K_M2N_CELLS, [&] {
+        hipLaunchKernelGGL(k_m2n_cells, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
+    });
+    if (rc) return rc;
+    rc = launch(c, V2C_K_M2N_NODES, [&] {
+        hipLaunchKernelGGL(k_m2n_nodes, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, c->stream, a);
+    });
+    if (rc) return rc;
+    if (c->prm.Dphis > 0) {
+        rc = launch(c, V2C_K_M2N_NODES, [&] {
+            hipLaunchKernelGGL(k_phis_bc, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, c->stream, a);
+        });
+        if (rc) return rc;
+    }
+    return finish(c);
+}
+
+static int read_counter(v2c_ctx *c, int idx, int *val)
+{
+    HIPCHK(c, hipMemcpyAsync(val, c->d_counters + idx, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int n2m_impl(v2c_ctx *c, bool advect)
+{
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_order(c);
+    if (rc) return rc;
+    KArgs a = make_args(c);
+    if (advect) HIPCHK(c, hipMemsetAsync(c->d_counters + 1, 0, sizeof(int), c->stream));
+    if (c->prm.n2m_phantom && c->prm.antidiffinter == 1) {
+        rc = launch(c, V2C_K_N2M_NODES, [&] {
+            hipLaunchKernelGGL(k_n2m_phantom, dim3(1), dim3(1), 0, c->stream, a);
+        });
+        if (rc) return rc;
+    }
+    const int nwg = (int)((c->ncell + CPW - 1) / CPW);
+    rc = launch(c, V2C_K_N2M_CELLS, [&] {
+        if (advect)
+            hipLaunchKernelGGL(k_n2m_cells<true>, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
+        else
+            hipLaunchKernelGGL(k_n2m_cells<false>, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
+    });
+    if (rc) return rc;
+    if (c->prm.antidiffinter == 1) {
+        rc = launch(c, V2C_K_N2M_NODES, [&] {
+            hipLaunchKernelGGL(k_n2m_nodes, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, c->stream, a);
+        });
+        if (rc) return rc;
+    }
+    if (advect) c->order_valid = false;
+    rc = finish(c);
+    if (rc) return rc;
+    if (advect && !c->async_mode) {
+        int uns = 0;
+        rc = read_counter(c, 1, &uns);
+        if (rc) return rc;
+        if (uns) return set_err(c, V2C_EUNSUPPORTED, "%d markers of type >= 100 (immobile spawning is out of scope)", uns);
+    }
+    return 0;
+}
+
+int v2c_nodes2markers(v2c_ctx *c) { return n2m_impl(c, false); }
+int v2c_nodes2markers_advect(v2c_ctx *c) { return n2m_impl(c, true); }
+
+int v2c_moveandrotatemarkers(v2c_ctx *c)
+{
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    KArgs a = make_args(c);
+    HIPCHK(c, hipMemsetAsync(c->d_counters + 1, 0, sizeof(int), c->stream));
+    int rc = 0;
+    if (c->n > 0) {
+        rc = launch(c, V2C_K_ADVECT, [&] {
+            hipLaunchKernelGGL(k_advect, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, a);
+        });
+        if (rc) return rc;
+    }
+    c->order_valid = false;
+    rc = finish(c);
+    if (rc) return rc;
+    if (!c->async_mode) {
+        int uns = 0;
+        rc = read_counter(c, 1, &uns);
+        if (rc) return rc;
+        if (uns) return set_err(c, V2C_EUNSUPPORTED, "%d markers of type >= 100 (immobile spawning is out of scope)", uns);
+    }
+    return 0;
+}
+
+int v2c_compute_before_solve(v2c_ctx *c, double dt, int rk, long *nerr)
+{
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    KArgs a = make_args(c);
+    HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(int), c->stream));
+    int rc = launch(c, V2C_K_BEFORE, [&] {
+        hipLaunchKernelGGL(k_before, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, c->stream, a, dt, rk);
+    });
+    if (rc) return rc;
+    rc = finish(c);
+    if (rc) return rc;
+    if (nerr) {
+        int v = 0;
+        rc = read_counter(c, 0, &v);
+        if (rc) return rc;
+        *nerr = v;
+    }
+    return 0;
+}
+
+int v2c_compute_after_solve(v2c_ctx *c, double dt, int rk, double *deltas, long *nerr)
+{
+    if (!c) return V2C_EINVAL;
+    (void)rk;
+    HIPCHK(c, hipSetDevice(c->device));
+    KArgs a = make_args(c);
+    const int nb = (int)blocks_for(c->nodenum, RED_BLOCK);
+    double *out = c->d_red + 2 * nb;
+    int rc = launch(c, V2C_K_AFTER, [&] {
+        hipLaunchKernelGGL(k_after, dim3(nb), dim3(RED_BLOCK), 0, c->stream, a, dt, c->d_red);
+        hipLaunchKernelGGL(k_after_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, c->d_red, nb, out);
+    });
+    if (rc) return rc;
+    rc = finish(c);
+    if (rc) return rc;
+    if (deltas || nerr) {
+        double h[2];
+        HIPCHK(c, hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (deltas) *deltas = h[0];
+        if (nerr) *nerr = h[0] != h[0] ? 1 : 0;
+    }
+    return 0;
+}
+
+int v2c_ductilevisc_batch(v2c_ctx *c, long n, const char *type, const double *sxy, const double *sxx,
+                          const double *exy, const double *exx, const double *tk, const double *pb,
+                          double *nu_out, int *iters)
+{
+    (void)exy; (void)exx;   // accepted but unused, as in density_ductilevisc_calc.c:114
+    if (!c || n < 0 || !type || !sxy || !sxx || !tk || !pb || !nu_out) return V2C_EINVAL;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    char *buf = nullptr;
+    const size_t bytes = (size_t)n * (5 * sizeof(double) + sizeof(int) + 1);
+    HIPCHK(c, hipMalloc(&buf, bytes));
+    double *d_sxy = (double *)buf, *d_sxx = d_sxy + n, *d_tk = d_sxx + n, *d_pb = d_tk + n, *d_out = d_pb + n;
+    int *d_it = (int *)(d_out + n);
+    int8_t *d_t = (int8_t *)(d_it + n);
+    hipMemcpyAsync(d_sxy, sxy, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(d_sxx, sxx, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(d_tk, tk, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(d_pb, pb, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(d_t, type, n, hipMemcpyHostToDevice, c->stream);
+    KArgs a = make_args(c);
+    hipLaunchKernelGGL(k_ductile_batch, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a, n, d_t, d_sxy,
+                       d_sxx, d_tk, d_pb, d_out, d_it);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(nu_out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && iters) e = hipMemcpyAsync(iters, d_it, n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(buf);
+    if (e != hipSuccess) return set_err(c, V2C_EHIP, "ductilevisc_batch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int v2c_enable_timing(v2c_ctx *c, int on)
+{
+    if (!c) return V2C_EINVAL;
+    c->timing = on ? 1 : 0;
+    return 0;
+}
+
+int v2c_reset_timing(v2c_ctx *c)
+{
+    if (!c) return V2C_EINVAL;
+    int rc = collect_timing(c);
+    for (int k = 0; k < V2C_K_COUNT; k++) { c->ktime[k] = 0; c->klaunch[k] = 0; }
+    return rc;
+}
+
+int v2c_kernel_time(v2c_ctx *c, int kernel, double *ms, long *launches)
+{
+    if (!c || kernel < 0 || kernel >= V2C_K_COUNT) return V2C_EINVAL;
+    int rc = collect_timing(c);
+    if (rc) return rc;
+    if (ms) *ms = c->ktime[kernel];
+    if (launches) *launches = c->klaunch[kernel];
+    return 0;
+}
+
+}  // extern "C"

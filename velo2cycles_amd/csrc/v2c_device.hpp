@@ -1,0 +1,400 @@
+// v2c_device.hpp -- device-side pieces of the per-marker hot path (gfx950).
+//
+// Every routine restates the reference formula named in its comment with the
+// reference's operation order.  The library is compiled with
+// -ffp-contract=off, so the arithmetic that decides cell indices and marker
+// positions (searches, weights, bilinear sums, position updates) rounds exactly
+// like the reference's gcc/x86-64 build; only libm calls (exp, log, pow, sin,
+// ...) may differ, by a few ulp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/v2c.h"
+
+namespace v2c {
+
+// Marker SoA field slots (same order as the v2c_markers struct).
+enum MF {
+    MF_X = 0, MF_Y, MF_K, MF_P, MF_XX, MF_XY, MF_EXX, MF_EXY, MF_E, MF_PHI, MF_VX, MF_VY,
+    MF_C, MF_D, MF_W, MF_A, MF_B, MF_L, MF_L0, MF_V0, MF_FS, MF_C0, NMF
+};
+
+struct DevMat {
+    double nu[V2C_MAX_TYPES], dh[V2C_MAX_TYPES], dv[V2C_MAX_TYPES], ss[V2C_MAX_TYPES];
+    double mm[V2C_MAX_TYPES], gg[V2C_MAX_TYPES], n0[V2C_MAX_TYPES], n1[V2C_MAX_TYPES];
+    double ro[V2C_MAX_TYPES], bb[V2C_MAX_TYPES], aa[V2C_MAX_TYPES], cp[V2C_MAX_TYPES];
+    double kt[V2C_MAX_TYPES], kf[V2C_MAX_TYPES], kp[V2C_MAX_TYPES], ht[V2C_MAX_TYPES];
+    int im[V2C_MAX_TYPES];
+};
+
+struct DevGrid {
+    const double *gx, *gy;
+    int xnumx, ynumy;
+    double xsize, ysize;
+};
+
+// Everything a kernel needs, passed by value.
+struct KArgs {
+    DevGrid g;
+    v2c_params p;
+    const DevMat *mat;
+    double *mk;            // NMF fields x cap
+    long cap;              // field stride
+    int8_t *t;             // marker types
+    long n;                // marker count
+    const uint32_t *order; // sorted position -> marker id
+    const uint32_t *cell_start;
+    uint32_t *key;         // per marker (physical) sub-cell key
+    double *nod;           // V2C_NF fields x nodenum
+    long nodenum;
+    long ncell;
+    double *part;          // per-cell partials, field-major
+    double timesum, timestep;
+    int *counters;         // device counters (errors, unsupported types)
+    int write_friction;    // write the calc_frictionpara marker outputs
+};
+
+__device__ __forceinline__ double *NOD(const KArgs &a, int f) { return a.nod + (long)f * a.nodenum; }
+__device__ __forceinline__ double &MK(const KArgs &a, int f, long i) { return a.mk[(long)f * a.cap + i]; }
+
+// ---- small helpers ---------------------------------------------------------
+__device__ __forceinline__ double ref_abs(double a) { return a >= 0 ? a : -a; }
+__device__ __forceinline__ double ref_minv(double a, double b) { return a <= b ? a : b; }
+__device__ __forceinline__ double clampd(double v, double lo, double hi)
+{
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+// Cell search by bisection (interpolation.c:1536-1573): largest gridline index
+// left of x, clamped to the last cell; NaN lands in the last cell.
+__device__ __forceinline__ int bisect_left(const double *__restrict__ g, int n, double x)
+{
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (g[mid] > x) hi = mid; else lo = mid;
+    }
+    return lo > n - 2 ? n - 2 : lo;
+}
+
+struct Stencil { long m3; double e, n; };
+
+// Pressure-node stencil (interpolation.c:1942-1948, 1216-1224, 1723-1730).
+__device__ __forceinline__ Stencil centre_stencil(const DevGrid &g, int nx, int ny, double x, double y)
+{
+    const double *gx = g.gx, *gy = g.gy;
+    int m10 = nx, m20 = ny;
+    if (x > (gx[m10] + gx[m10 + 1]) / 2.0) m10++;
+    if (y > (gy[m20] + gy[m20 + 1]) / 2.0) m20++;
+    if (m10 < 1) m10 = 1;
+    if (m10 > g.xnumx - 2) m10 = g.xnumx - 2;
+    if (m20 < 1) m20 = 1;
+    if (m20 > g.ynumy - 2) m20 = g.ynumy - 2;
+    Stencil s;
+    s.e = (x - (gx[m10 - 1] + gx[m10]) / 2.0) / ((gx[m10 + 1] - gx[m10 - 1]) / 2.0);
+    s.n = (y - (gy[m20 - 1] + gy[m20]) / 2.0) / ((gy[m20 + 1] - gy[m20 - 1]) / 2.0);
+    s.m3 = (long)m10 * g.ynumy + m20;
+    return s;
+}
+
+// Basic-node stencil without ghost nodes (interpolation.c:1069-1083, 1711-1717).
+__device__ __forceinline__ Stencil basic_stencil(const DevGrid &g, int nx, int ny, double x, double y)
+{
+    int m10 = nx, m20 = ny;
+    if (m10 < 1) m10 = 1;
+    if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
+    if (m20 < 1) m20 = 1;
+    if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
+    Stencil s;
+    s.e = (x - g.gx[m10]) / (g.gx[m10 + 1] - g.gx[m10]);
+    s.n = (y - g.gy[m20]) / (g.gy[m20 + 1] - g.gy[m20]);
+    s.m3 = (long)m10 * g.ynumy + m20;
+    return s;
+}
+
+// vx-node stencil (interpolation.c:1377-1388, 1738-1744).
+__device__ __forceinline__ Stencil vx_stencil(const DevGrid &g, int nx, int ny, double x, double y, bool clampx)
+{
+    const double *gx = g.gx, *gy = g.gy;
+    int m10 = nx, m20 = ny;
+    if (y < (gy[m20] + gy[m20 + 1]) / 2.0) m20 -= 1;
+    if (clampx) {
+        if (m10 < 0) m10 = 0;
+        if (m10 > g.xnumx - 2) m10 = g.xnumx - 2;
+    }
+    if (m20 < 0) m20 = 0;
+    if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
+    Stencil s;
+    s.e = (x - gx[m10]) / (gx[m10 + 1] - gx[m10]);
+    s.n = (y - (gy[m20] + gy[m20 + 1]) / 2.0) / ((gy[m20 + 2] - gy[m20]) / 2.0);
+    s.m3 = (long)m10 * g.ynumy + m20;
+    return s;
+}
+
+// vy-node stencil (interpolation.c:1434-1445, 1749-1755).
+__device__ __forceinline__ Stencil vy_stencil(const DevGrid &g, int nx, int ny, double x, double y, bool clampy)
+{
+    const double *gx = g.gx, *gy = g.gy;
+    int m10 = nx, m20 = ny;
+    if (x < (gx[m10] + gx[m10 + 1]) / 2.0) m10 -= 1;
+    if (m10 < 0) m10 = 0;
+    if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
+    if (clampy) {
+        if (m20 < 0) m20 = 0;
+        if (m20 > g.ynumy - 2) m20 = g.ynumy - 2;
+    }
+    Stencil s;
+    s.e = (x - (gx[m10] + gx[m10 + 1]) / 2.0) / ((gx[m10 + 2] - gx[m10]) / 2.0);
+    s.n = (y - gy[m20]) / (gy[m20 + 1] - gy[m20]);
+    s.m3 = (long)m10 * g.ynumy + m20;
+    return s;
+}
+
+// Bilinear 4-point sum, reference term order.
+struct W4 { double d0, d1, d2, d3; };
+__device__ __forceinline__ W4 wts(const Stencil &s)
+{
+    W4 w;
+    w.d0 = (1.0 - s.e) * (1.0 - s.n);
+    w.d1 = (1.0 - s.e) * s.n;
+    w.d2 = s.e * (1.0 - s.n);
+    w.d3 = s.e * s.n;
+    return w;
+}
+__device__ __forceinline__ double bil(const double *__restrict__ f, long m3, int ny, const W4 &w)
+{
+    return w.d0 * f[m3] + w.d1 * f[m3 + 1] + w.d2 * f[m3 + ny] + w.d3 * f[m3 + ny + 1];
+}
+
+// Sub-cell sort key of a position (after clamping to the domain): cell in
+// x-major order, times 4, plus the quadrant (2*qx + qy) with q = 1 past the
+// cell midline.  All markers of one (cell, quadrant) share their scatter
+// targets under interpolationmod == 2.
+__device__ __forceinline__ uint32_t subcell_key(const DevGrid &g, double x, double y)
+{
+    x = clampd(x, 0, g.xsize);
+    y = clampd(y, 0, g.ysize);
+    int nx = bisect_left(g.gx, g.xnumx, x), ny = bisect_left(g.gy, g.ynumy, y);
+    double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
+    double cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny]);
+    uint32_t q = (cx > 0.5 ? 2u : 0u) | (cy > 0.5 ? 1u : 0u);
+    return (((uint32_t)nx * (uint32_t)(g.ynumy - 1) + (uint32_t)ny) << 2) | q;
+}
+
+// ---- material laws -----------------------------------------------------------
+
+// dencalcomp (density_ductilevisc_calc.c:2-27)
+__device__ __forceinline__ void dencalc(const KArgs &a, double mtk, double mpb, int t,
+                                        double &mro, double &mbb)
+{
+    const DevMat *M = a.mat;
+    mbb = M->bb[t] / (1.0 - M->bb[t] * (mtk - 298.15));
+    if (a.p.densimod == 0) mro = M->ro[t];
+    else mro = M->ro[t] * (1.0 - M->bb[t] * (mtk - 298.15)) * (1.0 + M->aa[t] * (mpb - 1.0) * 1e-3);
+}
+
+// conductivity (recalc_marker_properties.c:51-54)
+__device__ __forceinline__ double conductivity(const DevMat *M, int t, double mtk, double mpb)
+{
+    double k = (M->kt[t] + M->kf[t] / (mtk + 77.0)) * exp(M->kp[t] * mpb);
+    if (k < 0) k = -k / (1.0 + M->kf[t] * (mtk - M->kp[t]) / M->kp[t]);
+    return k;
+}
+
+__device__ __forceinline__ bool power_law_type(const DevMat *M, int t)
+{
+    return M->nu[t] > 0 && (M->dh[t] != 0 || M->dv[t] != 0) && M->ss[t] != 0 && M->mm[t] != 1.0;
+}
+
+__device__ __forceinline__ double clamp_visc(const KArgs &a, int t, double v)
+{
+    if (v < a.p.nubeg) v = a.p.nubeg;
+    if (v > a.p.nuend) v = a.p.nuend;
+    if (v < a.mat->n0[t]) v = a.mat->n0[t];
+    if (v > a.mat->n1[t]) v = a.mat->n1[t];
+    return v;
+}
+
+// ductilevisc (density_ductilevisc_calc.c:29-223): sticky air, Newtonian, or
+// diffusion + dislocation creep with the reference's quasi-Newton iteration
+// (:157-203, stop on |log10(mu_k/mu_{k-1})| < 1e-6 after >3 iterations, at
+// most 50).  The uninitialised branch (:55-65) returns NaN.
+__device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy, double sxx,
+                                              double mtk, double mpb, int *iters)
+{
+    const DevMat *M = a.mat;
+    *iters = 0;
+    if (t == 1) {
+        double v = M->nu[1];
+        if (a.p.stickymod == 1 && a.timesum > 4e7) {
+            double cap = M->gg[2] * a.timestep / 100;
+            if (v > cap) v = cap;
+        }
+        return v;
+    }
+    if (M->dh[t] == 0 && M->dv[t] == 0 && (M->ss[t] == 0 || M->mm[t] == 1.0)) return M->nu[t];
+    if (!power_law_type(M, t)) return __builtin_nan("");
+    const double n = M->mm[t];
+    double sigin = pow(sxx * sxx + sxy * sxy, 0.5);
+    double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * mtk);
+    if (e1 > 150.0) e1 = 150.0;
+    e1 = exp(e1);
+    double k1 = M->nu[t] / pow(M->ss[t], n - 1.0);
+    double inv_newt = 1.0 / (0.5 * k1 * e1);
+    double inv_powl = 0;
+    if (sigin > 0) {
+        double eta_disl = 0.5 * sigin * M->nu[t] * e1 / pow(sigin, n);
+        double eta_new = eta_disl;
+        double eta_diff = 0.5 * k1 * e1;
+        double coeff = pow(eta_disl, 1.0 / n);
+        double gdt = M->gg[t] * a.timestep;
+        double mu_prev = 0, mu;
+        const double ex1 = (n - 1) / n, ex2 = -1 / n, c1 = (n - 1) / (n * eta_diff);
+        int k = 0;
+        for (;;) {
+            if (k >= 50) break;
+            k++;
+            double eta_old = eta_new;
+            mu = 1.0 / (1.0 / eta_diff + 1.0 / eta_new);
+            double pref = (eta_diff + eta_old) / eta_diff;
+            double X = pref * (gdt + mu);
+            double f1 = pow(X, ex1);
+            double f2 = eta_old - coeff * f1;
+            double df = 1 - c1 * coeff * pow(X, ex2) * ((gdt + mu) + pow(eta_diff, 2 / (eta_diff + eta_old)));
+            eta_new = eta_old - f2 / df;
+            if (k > 3 && fabs(log10(mu / mu_prev)) < 1.0e-6) break;
+            mu_prev = mu;
+        }
+        *iters = k;
+        inv_powl = 1 / eta_new;
+    }
+    return clamp_visc(a, t, 1.0 / (inv_newt + inv_powl));
+}
+
+// calc_frictionpara (calc_RSF.c:224-295): marker RSF parameters from the
+// fault-band geometry.  Returns the (possibly reset) state phi; writes the
+// parameter fields when requested.
+__device__ __forceinline__ double friction_marker(const KArgs &a, long m, int t, double x, double y, double phi)
+{
+    const v2c_params &p = a.p;
+    double radius = ref_abs(y - p.band_y);
+    double hr2 = p.half_range;
+    if (t > 0) {
+        if (radius <= p.band_halfwidth) {
+            double A, B, L, V0;
+            if (x > p.end_updip && x < p.start_downdip) {
+                A = p.a_vw; B = p.b_vw; L = p.L_vw; V0 = p.V0_vw;
+            } else if (x > p.start_updip && x < p.end_updip && p.half_range > 0) {
+                double f = (x - (p.start_updip - p.xstpx / 2)) / (2 * hr2);
+                A = p.a_vs + f * (p.a_vw - p.a_vs);
+                B = p.b_vs + f * (p.b_vw - p.b_vs);
+                L = p.L_vs + f * (p.L_vw - p.L_vs);
+                V0 = p.V0_vs + f * (p.V0_vw - p.V0_vs);
+            } else if (x >= p.start_downdip && x < p.end_downdip && p.half_range > 0) {
+                double f = (x - (p.start_downdip - p.xstpx / 2)) / (2 * hr2);
+                A = p.a_vw - f * (p.a_vw - p.a_vs);
+                B = p.b_vw - f * (p.b_vw - p.b_vs);
+                L = p.L_vw - f * (p.L_vw - p.L_vs);
+                V0 = p.V0_vw - f * (p.V0_vw - p.V0_vs);
+            } else {
+                A = p.a_vs; B = p.b_vs; L = p.L_vs; V0 = p.V0_vs;
+            }
+            if (a.write_friction) {
+                MK(a, MF_FS, m) = 0.2;
+                MK(a, MF_A, m) = A; MK(a, MF_B, m) = B; MK(a, MF_L0, m) = L; MK(a, MF_L, m) = L;
+                MK(a, MF_V0, m) = V0; MK(a, MF_C0, m) = 0.0;
+            }
+            if (a.timesum == a.timestep) phi = 0.0;
+        } else {
+            phi = 50.0;
+            if (a.write_friction) {
+                MK(a, MF_FS, m) = 0.6;
+                MK(a, MF_A, m) = p.a_hr; MK(a, MF_B, m) = p.b_hr; MK(a, MF_L0, m) = p.L_hr;
+                MK(a, MF_L, m) = p.L_hr; MK(a, MF_V0, m) = p.V0_hr;
+            }
+        }
+    } else if (a.write_friction) {
+        MK(a, MF_FS, m) = 0.6;
+        MK(a, MF_A, m) = 1.0; MK(a, MF_B, m) = 0.01; MK(a, MF_L0, m) = 0.2; MK(a, MF_V0, m) = p.V0_hr;
+    }
+    return phi;
+}
+
+// calc_frictionpara_nodes (calc_RSF.c:298-350)
+__device__ __forceinline__ void friction_node(const KArgs &a, double x, double y, long m3)
+{
+    const v2c_params &p = a.p;
+    double radius = ref_abs(y - p.band_y);
+    double hr2 = p.half_range;
+    if (radius <= p.band_halfwidth) {
+        double A, B, L, V0;
+        if (x > p.end_updip && x < p.start_downdip) {
+            A = p.a_vw; B = p.b_vw; L = p.L_vw; V0 = p.V0_vw;
+        } else if (x >= p.start_updip && x <= p.end_updip && p.half_range > 0) {
+            double f = (x - (p.start_updip)) / (2 * hr2);
+            A = p.a_vs + f * (p.a_vw - p.a_vs);
+            B = p.b_vs + f * (p.b_vw - p.b_vs);
+            L = p.L_vs + f * (p.L_vw - p.L_vs);
+            V0 = p.V0_vs + f * (p.V0_vw - p.V0_vs);
+        } else if (x >= p.start_downdip && x < p.end_downdip && p.half_range > 0) {
+            double f = (x - (p.start_downdip)) / (2 * hr2);
+            A = p.a_vw - f * (p.a_vw - p.a_vs);
+            B = p.b_vw - f * (p.b_vw - p.b_vs);
+            L = p.L_vw - f * (p.L_vw - p.L_vs);
+            V0 = p.V0_vw - f * (p.V0_vw - p.V0_vs);
+        } else {
+            A = p.a_vs; B = p.b_vs; L = p.L_vs; V0 = p.V0_vs;
+        }
+        NOD(a, V2C_N_mus_s)[m3] = 0.2;
+        NOD(a, V2C_N_as)[m3] = A; NOD(a, V2C_N_bs)[m3] = B; NOD(a, V2C_N_Ls)[m3] = L;
+        NOD(a, V2C_N_V0s)[m3] = V0;
+        NOD(a, V2C_N_cohesion)[m3] = 0.0;
+    } else {
+        NOD(a, V2C_N_mus_s)[m3] = 0.6;
+        NOD(a, V2C_N_as)[m3] = p.a_hr; NOD(a, V2C_N_bs)[m3] = p.b_hr; NOD(a, V2C_N_Ls)[m3] = p.L_hr;
+        NOD(a, V2C_N_V0s)[m3] = p.V0_hr;
+    }
+}
+
+// allinteri (interpolation.c:1688-1758): velocity, spin and strain-rate
+// invariant at a point.
+__device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, double &VX, double &VY,
+                                          double &ESP, double &EE)
+{
+    const DevGrid &g = a.g;
+    const double *gx = g.gx, *gy = g.gy;
+    const int ynumy = g.ynumy;
+    x = clampd(x, 0, g.xsize);
+    y = clampd(y, 0, g.ysize);
+    double xrat = 2.0 / 3.0;
+    if (x < (gx[0] + gx[1]) / 2.0) xrat = 1.0;
+    if (x > (gx[g.xnumx - 2] + gx[g.xnumx - 1]) / 2.0) xrat = 1.0;
+    if (y < (gy[0] + gy[1]) / 2.0) xrat = 1.0;
+    if (y > (gy[ynumy - 2] + gy[ynumy - 1]) / 2.0) xrat = 1.0;
+    int nx = bisect_left(gx, g.xnumx, x), ny = bisect_left(gy, ynumy, y);
+    const double *vx = NOD(a, V2C_N_vx), *vy = NOD(a, V2C_N_vy);
+
+    Stencil s = basic_stencil(g, nx, ny, x, y);
+    W4 w = wts(s);
+    double EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
+    ESP = bil(NOD(a, V2C_N_esp), s.m3, ynumy, w);
+
+    s = centre_stencil(g, nx, ny, x, y);
+    w = wts(s);
+    double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
+    long m3 = s.m3;
+    double vxa = (w.d0 * (vx[m3 - 1] + vx[m3 - ynumy - 1]) + w.d1 * (vx[m3] + vx[m3 - ynumy])
+                  + w.d2 * (vx[m3 + ynumy - 1] + vx[m3 - 1]) + w.d3 * (vx[m3 + ynumy] + vx[m3])) * 0.5 * (1.0 - xrat);
+    double vya = (w.d0 * (vy[m3 - ynumy] + vy[m3 - ynumy - 1]) + w.d1 * (vy[m3 - ynumy + 1] + vy[m3 - ynumy])
+                  + w.d2 * (vy[m3] + vy[m3 - 1]) + w.d3 * (vy[m3 + 1] + vy[m3])) * 0.5 * (1.0 - xrat);
+    EE = pow(EXX * EXX + EXY * EXY, 0.5);
+
+    s = vx_stencil(g, nx, ny, x, y, true);
+    VX = vxa + bil(vx, s.m3, ynumy, wts(s)) * xrat;
+    s = vy_stencil(g, nx, ny, x, y, true);
+    VY = vya + bil(vy, s.m3, ynumy, wts(s)) * xrat;
+}
+
+}  // namespace v2c

@@ -1,0 +1,938 @@
+// v2c_kernels.hip -- HIP kernels of the per-marker hot path for gfx950.
+//
+// Marker state is an FP64 structure-of-arrays resident in HBM.  Markers are
+// ordered by sub-cell key (cell, quadrant) with a stable radix sort of
+// (key, marker id); under interpolationmod == 2 every marker of one
+// (cell, quadrant) scatters to the same basic, vx, vy and centre node, so the
+// marker->node reductions run as deterministic segmented sums:
+//
+//   *_cells kernels  one 64-lane wavefront per group of CPW consecutive cells;
+//                    lanes take markers in sorted order, compute their
+//                    contributions, and the head lane of every (cell, quadrant)
+//                    run sums the run from LDS in lane order into an LDS slot;
+//                    the slots are written as per-cell partial sums;
+//   *_nodes kernels  one thread per node gathers the <= 4 partials of the
+//                    adjacent cells in a fixed order and normalises.
+//
+// No floating-point atomics are used anywhere, so results are bitwise
+// reproducible run to run.
+#include "v2c_device.hpp"
+
+namespace v2c {
+
+constexpr int WAVE = 64;
+constexpr int CPW = 16;          // cells per wavefront group
+
+// Contribution layout of one marker, m2n (interpolation.c:2-52 accumulators):
+//   basic  [0,14): nu, nu/G, e, sxy, exy, C, phi, ro, et, ht, cp, kt, T, w
+//   vx     [14,17): v*ro*w, ro*w, w      vy [17,20): same
+//   centre [20,27): bm, nu, nu/G, sxx, exx, P, w
+constexpr int M_NB = 14, M_NV = 3, M_NC = 7;
+constexpr int M_VX = M_NB, M_VY = M_NB + M_NV, M_C = M_NB + 2 * M_NV, M_NF = M_C + M_NC;
+// per-cell partial layout: basic 4 quadrants x 14, vx 2 halves x 3, vy 2 x 3, centre 7
+constexpr int MP_VX = 4 * M_NB, MP_VY = MP_VX + 2 * M_NV, MP_C = MP_VY + 2 * M_NV, MP_N = MP_C + M_NC;
+
+// n2m antidiffusion sums (interpolation.c:1159-1209, 1360-1374, 1405-1489):
+//   basic [0,5): PHIS*w, SXY*w, EXY*w, ES*w, w
+//   vx [5,8): VX*ro*w, ro*w, w   vy [8,11)   centre [11,15): PR*w, SXX*w, EXX*w, w
+constexpr int N_NB = 5, N_NV = 3, N_NC = 4;
+constexpr int N_VX = N_NB, N_VY = N_NB + N_NV, N_C = N_NB + 2 * N_NV, N_NF = N_C + N_NC;
+constexpr int NP_VX = 4 * N_NB, NP_VY = NP_VX + 2 * N_NV, NP_C = NP_VY + 2 * N_NV, NP_N = NP_C + N_NC;
+
+// Map workgroup ids so each of the 8 XCDs gets a contiguous range of cells
+// (dispatch assigns workgroups round-robin over XCDs; neighbouring cells share
+// nodal data, which then stays in one XCD's L2).
+__device__ __forceinline__ int xcd_remap(int b, int nwg)
+{
+    int full = (nwg / 8) * 8;
+    if (b >= full) return b;
+    return (b % 8) * (full / 8) + b / 8;
+}
+
+// ---------------------------------------------------------------------------
+// Keys, ordering
+// ---------------------------------------------------------------------------
+__global__ void k_keys(KArgs a)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    a.key[i] = subcell_key(a.g, MK(a, MF_X, i), MK(a, MF_Y, i));
+}
+
+__global__ void k_iota(uint32_t *v, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
+// cell_start[c] = first sorted position whose cell is >= c; cell_start[ncell] = n.
+__global__ void k_cell_start(const uint32_t *__restrict__ skey, long n, long ncell, uint32_t *cs)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    long cur = i < n ? (long)(skey[i] >> 2) : ncell;
+    long prev = i > 0 ? (long)(skey[i - 1] >> 2) : -1;
+    for (long c = prev + 1; c <= cur && c <= ncell; c++) cs[c] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// Segmented reduction helper: lanes hold NF contributions in con[f][lane] and
+// a local slot id in skey[lane] (runs of equal slot ids are contiguous); the
+// head lane of each run adds the run total to acc[slot][f].
+// ---------------------------------------------------------------------------
+template <int NF>
+__device__ __forceinline__ void run_reduce(double (*con)[WAVE], const uint32_t *skey, double (*acc)[NF],
+                                           int lane, bool active)
+{
+    uint32_t key = skey[lane];
+    if (active && (lane == 0 || skey[lane - 1] != key)) {
+        int j = lane + 1;
+        while (j < WAVE && skey[j] == key) j++;
+        for (int f = 0; f < NF; f++) {
+            double s = 0.0;
+            for (int r = lane; r < j; r++) s += con[f][r];
+            acc[key][f] += s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// marker2nodes, marker pass (interpolation.c:215-629)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
+{
+    __shared__ double acc[CPW * 4][M_NF];
+    __shared__ double con[M_NF][WAVE];
+    __shared__ uint32_t skey[WAVE];
+    const int lane = threadIdx.x;
+    const int wg = xcd_remap(blockIdx.x, nwg);
+    const long c0 = (long)wg * CPW;
+    const long c1 = c0 + CPW < a.ncell ? c0 + CPW : a.ncell;
+    for (int s = lane; s < CPW * 4 * M_NF; s += WAVE) (&acc[0][0])[s] = 0.0;
+    const uint32_t beg = a.cell_start[c0], end = a.cell_start[c1];
+    const DevMat *M = a.mat;
+    const DevGrid &g = a.g;
+    __syncthreads();
+    for (uint32_t base = beg; base < end; base += WAVE) {
+        const uint32_t i = base + lane;
+        const bool active = i < end;
+        double v[M_NF];
+#pragma unroll
+        for (int f = 0; f < M_NF; f++) v[f] = 0.0;
+        uint32_t slot = 0xffffffffu;
+        if (active) {
+            const long m = a.order[i];
+            slot = a.key[m] - (uint32_t)(c0 << 2);
+            const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
+            if (x > 0 && y > 0 && x < g.xsize && y < g.ysize) {
+                const int nx = bisect_left(g.gx, g.xnumx, x), ny = bisect_left(g.gy, g.ynumy, y);
+                const int t = a.t[m];
+                const double mbm = a.p.bulk_modulus;
+                const double mgg = M->gg[t];
+                const double mtk = MK(a, MF_K, m);
+                Stencil ps = centre_stencil(g, nx, ny, x, y);   // allinterpomp (interpolation.c:1923-1958)
+                const double mpb = 1e-5 * bil(NOD(a, V2C_N_pr), ps.m3, g.ynumy, wts(ps));
+                double mro, mbb;
+                dencalc(a, mtk, mpb, t, mro, mbb);
+                const double mkt = conductivity(M, t, mtk, mpb);
+                const double sxy = MK(a, MF_XY, m), sxx = MK(a, MF_XX, m);
+                int iters;
+                const double mnu = ductilevisc(a, t, sxy, sxx, mtk, mpb, &iters);
+                const double phi0 = MK(a, MF_PHI, m);
+                const double phi = friction_marker(a, m, t, x, y, phi0);
+                if (__double_as_longlong(phi) != __double_as_longlong(phi0)) MK(a, MF_PHI, m) = phi;
+                const double me = MK(a, MF_E, m);
+                const double swt1 = 1.0 / (g.gx[nx + 1] - g.gx[nx]) / (g.gy[ny + 1] - g.gy[ny]);
+                const double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
+                const double cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny]);
+                double bw = 0.0;
+                bool hasb = true;
+                if (cx < 0.5 && cy < 0.5) bw = swt1 * (1.0 - 2.0 * cx) * (1.0 - 2.0 * cy);
+                else if (cx < 0.5 && cy > 0.5) bw = swt1 * (1.0 - 2.0 * cx) * (2.0 * cy - 1.0);
+                else if (cx > 0.5 && cy < 0.5) bw = swt1 * (2.0 * cx - 1.0) * (1.0 - 2.0 * cy);
+                else if (cx > 0.5 && cy > 0.5) bw = swt1 * (2.0 * cx - 1.0) * (2.0 * cy - 1.0);
+                else hasb = false;
+                if (hasb) {
+                    v[0] = mnu * bw; v[1] = mnu / mgg * bw; v[2] = me * bw;
+                    v[3] = sxy * bw; v[4] = MK(a, MF_EXY, m) * bw; v[5] = MK(a, MF_C, m) * bw;
+                    v[6] = phi * bw; v[7] = mro * bw; v[8] = mbb * bw; v[9] = M->ht[t] * bw;
+                    v[10] = M->cp[t] * bw; v[11] = mkt * bw; v[12] = mtk * bw; v[13] = bw;
+                }
+                double vw = 0.0;
+                bool hasv = true;
+                if (cx < 0.5) vw = swt1 * (1.0 - cx) * (1.0 - ref_abs(cy - 0.5));
+                else if (cx > 0.5) vw = swt1 * cx * (1.0 - ref_abs(cy - 0.5));
+                else hasv = false;
+                if (hasv) {
+                    v[M_VX] = MK(a, MF_VX, m) * mro * vw; v[M_VX + 1] = mro * vw; v[M_VX + 2] = vw;
+                }
+                hasv = true;
+                if (cy < 0.5) vw = swt1 * (1.0 - ref_abs(cx - 0.5)) * (1.0 - cy);
+                else if (cy > 0.5) vw = swt1 * (1.0 - ref_abs(cx - 0.5)) * cy;
+                else hasv = false;
+                if (hasv) {
+                    v[M_VY] = MK(a, MF_VY, m) * mro * vw; v[M_VY + 1] = mro * vw; v[M_VY + 2] = vw;
+                }
+                const double cw = swt1 * (1.0 - 2.0 * ref_abs(cx - 0.5)) * (1.0 - 2.0 * ref_abs(cy - 0.5));
+                v[M_C + 0] = mbm * cw; v[M_C + 1] = mnu * cw; v[M_C + 2] = mnu / mgg * cw;
+                v[M_C + 3] = sxx * cw; v[M_C + 4] = MK(a, MF_EXX, m) * cw; v[M_C + 5] = MK(a, MF_P, m) * cw;
+                v[M_C + 6] = cw;
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < M_NF; f++) con[f][lane] = v[f];
+        skey[lane] = slot;
+        __syncthreads();
+        run_reduce<M_NF>(con, skey, acc, lane, active);
+        __syncthreads();
+    }
+    // per-cell partials
+    const long ncell = a.ncell;
+    for (int s = lane; s < CPW * MP_N; s += WAVE) {
+        const int lc = s % CPW, f = s / CPW;
+        const long c = c0 + lc;
+        if (c >= c1) continue;
+        const double(*sl)[M_NF] = &acc[lc * 4];
+        double val;
+        if (f < MP_VX) {
+            val = sl[f / M_NB][f % M_NB];
+        } else if (f < MP_VY) {
+            const int h = (f - MP_VX) / M_NV, k = (f - MP_VX) % M_NV;
+            val = sl[2 * h][M_VX + k] + sl[2 * h + 1][M_VX + k];
+        } else if (f < MP_C) {
+            const int h = (f - MP_VY) / M_NV, k = (f - MP_VY) % M_NV;
+            val = sl[h][M_VY + k] + sl[2 + h][M_VY + k];
+        } else {
+            const int k = f - MP_C;
+            val = sl[0][M_C + k] + sl[1][M_C + k] + sl[2][M_C + k] + sl[3][M_C + k];
+        }
+        a.part[(long)f * ncell + c] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// marker2nodes, node pass: combine cell partials and normalise
+// (interpolation.c:58-153 zeroing, :714-824 normalisation, :827-859 BC copy
+// runs as k_phis_bc)
+// ---------------------------------------------------------------------------
+__global__ void k_m2n_nodes(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const int ynumy = a.g.ynumy, xnumx = a.g.xnumx;
+    const int m22 = (int)(m3 % ynumy), m11 = (int)(m3 / ynumy);
+    const long ncy = ynumy - 1, ncell = a.ncell;
+    const double *P = a.part;
+    auto cell = [&](int cx, int cy) -> long {
+        return (cx >= 0 && cy >= 0 && cx < xnumx - 1 && cy < ynumy - 1) ? (long)cx * ncy + cy : -1;
+    };
+    // basic node: quadrant (1,1) of cell (m11-1,m22-1), (1,0) of (m11-1,m22),
+    // (0,1) of (m11,m22-1), (0,0) of (m11,m22)
+    double B[M_NB];
+#pragma unroll
+    for (int f = 0; f < M_NB; f++) B[f] = 0.0;
+    {
+        const long cs[4] = {cell(m11 - 1, m22 - 1), cell(m11 - 1, m22), cell(m11, m22 - 1), cell(m11, m22)};
+        const int qs[4] = {3, 2, 1, 0};
+        for (int k = 0; k < 4; k++) {
+            if (cs[k] < 0) continue;
+#pragma unroll
+            for (int f = 0; f < M_NB; f++) B[f] += P[(long)(qs[k] * M_NB + f) * ncell + cs[k]];
+        }
+    }
+    double VXs[M_NV] = {0, 0, 0}, VYs[M_NV] = {0, 0, 0}, Cs[M_NC];
+#pragma unroll
+    for (int f = 0; f < M_NC; f++) Cs[f] = 0.0;
+    {
+        long cl = cell(m11 - 1, m22), cr = cell(m11, m22);
+        for (int f = 0; f < M_NV; f++) {
+            if (cl >= 0) VXs[f] += P[(long)(MP_VX + M_NV + f) * ncell + cl];
+            if (cr >= 0) VXs[f] += P[(long)(MP_VX + f) * ncell + cr];
+        }
+        long ct = cell(m11, m22 - 1), cb = cell(m11, m22);
+        for (int f = 0; f < M_NV; f++) {
+            if (ct >= 0) VYs[f] += P[(long)(MP_VY + M_NV + f) * ncell + ct];
+            if (cb >= 0) VYs[f] += P[(long)(MP_VY + f) * ncell + cb];
+        }
+        long cc = cell(m11 - 1, m22 - 1);
+        if (cc >= 0)
+            for (int f = 0; f < M_NC; f++) Cs[f] = P[(long)(MP_C + f) * ncell + cc];
+    }
+
+    // zeroed by the reference's initialisation loop (interpolation.c:58-153)
+    static constexpr int zeroed[] = {
+        V2C_N_bm, V2C_N_ro, V2C_N_et, V2C_N_nu, V2C_N_nd, V2C_N_gg, V2C_N_gd, V2C_N_sppe,
+        V2C_N_phis, V2C_N_phin, V2C_N_sxyen, V2C_N_sxxen, V2C_N_exyen, V2C_N_exxen,
+        V2C_N_sppen, V2C_N_mus_s, V2C_N_as, V2C_N_bs, V2C_N_Ls, V2C_N_V0s, V2C_N_sxxe,
+        V2C_N_sxye, V2C_N_exxe, V2C_N_exye, V2C_N_cohesion, V2C_N_dro, V2C_N_drp,
+        V2C_N_cp, V2C_N_kt, V2C_N_ht, V2C_N_tk, V2C_N_mrx, V2C_N_mry, V2C_N_mvx, V2C_N_mvy,
+        V2C_N_wtphisum, V2C_N_phisum, V2C_N_sxysum, V2C_N_exysum, V2C_N_esum,
+        V2C_N_wtsxxsum, V2C_N_psum, V2C_N_sxxsum, V2C_N_exxsum, V2C_N_wtvxsum,
+        V2C_N_vxsum, V2C_N_wtvysum, V2C_N_vysum, V2C_N_mrx0, V2C_N_mry0};
+    // read the antidiffusion inputs before any output of this node is written
+    const bool anti = a.timesum > 0 && a.p.antidiffinter == 1;
+    double in_phis_it = 0, in_phie = 0, in_sxy = 0, in_sxyee = 0, in_exy = 0, in_exyee = 0,
+           in_pes = 0, in_pee = 0, in_sxx = 0, in_sxxee = 0, in_pr = 0, in_sppee = 0,
+           in_exx = 0, in_exxee = 0, in_vx = 0, in_mvxe = 0, in_vy = 0, in_mvye = 0;
+    if (anti) {
+        in_phis_it = NOD(a, V2C_N_phis_it)[m3]; in_phie = NOD(a, V2C_N_phie)[m3];
+        in_sxy = NOD(a, V2C_N_sxy)[m3]; in_sxyee = NOD(a, V2C_N_sxyee)[m3];
+        in_exy = NOD(a, V2C_N_exy)[m3]; in_exyee = NOD(a, V2C_N_exyee)[m3];
+        in_pes = NOD(a, V2C_N_pes)[m3]; in_pee = NOD(a, V2C_N_pee)[m3];
+        in_sxx = NOD(a, V2C_N_sxx)[m3]; in_sxxee = NOD(a, V2C_N_sxxee)[m3];
+        in_pr = NOD(a, V2C_N_pr)[m3]; in_sppee = NOD(a, V2C_N_sppee)[m3];
+        in_exx = NOD(a, V2C_N_exx)[m3]; in_exxee = NOD(a, V2C_N_exxee)[m3];
+        in_vx = NOD(a, V2C_N_vx)[m3]; in_mvxe = NOD(a, V2C_N_mvxe)[m3];
+        in_vy = NOD(a, V2C_N_vy)[m3]; in_mvye = NOD(a, V2C_N_mvye)[m3];
+    }
+    for (int z = 0; z < (int)(sizeof(zeroed) / sizeof(zeroed[0])); z++) NOD(a, zeroed[z])[m3] = 0.0;
+
+    const double w = B[13];
+    if (w != 0 && m11 >= 1 && m22 >= 1 && m11 < xnumx - 1 && m22 < ynumy - 1) {
+        const double phin = B[6] / w, sxyen = B[3] / w, exyen = B[4] / w, pen = B[2] / w;
+        NOD(a, V2C_N_phin)[m3] = phin; NOD(a, V2C_N_sxyen)[m3] = sxyen;
+        NOD(a, V2C_N_exyen)[m3] = exyen; NOD(a, V2C_N_pen)[m3] = pen;
+        double pe;
+        if (anti) {
+            NOD(a, V2C_N_phis)[m3] = in_phis_it + (phin - in_phie);
+            NOD(a, V2C_N_sxye)[m3] = in_sxy + (sxyen - in_sxyee);
+            NOD(a, V2C_N_exye)[m3] = in_exy + (exyen - in_exyee);
+            pe = in_pes + (pen - in_pee);
+        } else {
+            NOD(a, V2C_N_phis)[m3] = phin; NOD(a, V2C_N_sxye)[m3] = sxyen;
+            NOD(a, V2C_N_exye)[m3] = exyen; pe = pen;
+        }
+        if (pe < 0) pe = 0;
+        NOD(a, V2C_N_pe)[m3] = pe;
+        friction_node(a, a.g.gx[m11], a.g.gy[m22], m3);
+        NOD(a, V2C_N_cohesion)[m3] = B[5] / w;
+        const double nu_m2c = B[0] / w;
+        NOD(a, V2C_N_nu_m2c)[m3] = nu_m2c;
+        NOD(a, V2C_N_gg)[m3] = nu_m2c / (B[1] / w);
+    }
+    const double wc = Cs[6];
+    if (wc != 0 && m11 >= 1 && m22 >= 1) {
+        const double sxxen = Cs[3] / wc, sppen = Cs[5] / wc, exxen = Cs[4] / wc;
+        NOD(a, V2C_N_sxxen)[m3] = sxxen; NOD(a, V2C_N_sppen)[m3] = sppen; NOD(a, V2C_N_exxen)[m3] = exxen;
+        double sppe;
+        if (anti) {
+            NOD(a, V2C_N_sxxe)[m3] = in_sxx + (sxxen - in_sxxee);
+            sppe = in_pr + (sppen - in_sppee);
+            NOD(a, V2C_N_exxe)[m3] = in_exx + (exxen - in_exxee);
+        } else {
+            NOD(a, V2C_N_sxxe)[m3] = sxxen; sppe = sppen; NOD(a, V2C_N_exxe)[m3] = exxen;
+        }
+        if (a.timesum == 0 && a.p.reloadfromDP == 0 && a.p.compressiblemode == 1) sppe = a.p.nstress;
+        NOD(a, V2C_N_sppe)[m3] = sppe;
+        const double nd = Cs[1] / wc;
+        NOD(a, V2C_N_nd_m2c)[m3] = nd;
+        NOD(a, V2C_N_gd)[m3] = nd / (Cs[2] / wc);
+        NOD(a, V2C_N_bm)[m3] = Cs[0] / wc;
+        NOD(a, V2C_N_dro)[m3] = 0 / wc;
+        NOD(a, V2C_N_drp)[m3] = 0 / wc;
+    }
+    if (VXs[2] != 0) {
+        const double mvxn = VXs[0] / VXs[1];
+        NOD(a, V2C_N_mvxn)[m3] = mvxn;
+        NOD(a, V2C_N_mvx)[m3] = anti ? in_vx + (mvxn - in_mvxe) : mvxn;
+        NOD(a, V2C_N_mrx)[m3] = VXs[1] / VXs[2];
+    }
+    if (VYs[2] != 0) {
+        const double mvyn = VYs[0] / VYs[1];
+        NOD(a, V2C_N_mvyn)[m3] = mvyn;
+        NOD(a, V2C_N_mvy)[m3] = anti ? in_vy + (mvyn - in_mvye) : mvyn;
+        NOD(a, V2C_N_mry)[m3] = VYs[1] / VYs[2];
+    }
+    if (w != 0) {
+        NOD(a, V2C_N_ro)[m3] = B[7] / w;
+        NOD(a, V2C_N_et)[m3] = B[8] / w;
+        NOD(a, V2C_N_ht)[m3] = B[9] / w;
+        NOD(a, V2C_N_kt)[m3] = B[11] / w;
+        NOD(a, V2C_N_cp)[m3] = B[10] / w;
+        NOD(a, V2C_N_tk)[m3] = B[12] / w;
+    }
+}
+
+// State-variable boundary copy when its diffusion is on (interpolation.c:827-859).
+// Reads interior values only and writes boundary nodes only, so one pass is race free.
+__global__ void k_phis_bc(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const int ynumy = a.g.ynumy, xnumx = a.g.xnumx;
+    const int m22 = (int)(m3 % ynumy), m11 = (int)(m3 / ynumy);
+    double *phis = NOD(a, V2C_N_phis);
+    if (m11 == 0 && m22 >= 1 && m22 <= ynumy - 2) phis[m3] = phis[m3 + ynumy];
+    else if (m11 == xnumx - 1 && m22 >= 1 && m22 <= ynumy - 2) phis[m3] = phis[m3 - ynumy];
+    else if (m22 == 0 && m11 >= 1 && m11 <= xnumx - 2) phis[m3] = phis[m3 + 1];
+    else if (m22 == ynumy - 1 && m11 > 0 && m11 <= xnumx - 2) phis[m3] = phis[m3 - 1];
+    else if ((m11 == 0 && m22 == 0) || (m11 == 0 && m22 == ynumy - 1) ||
+             (m11 == xnumx - 1 && m22 == 0) || (m11 == xnumx - 1 && m22 == ynumy - 1))
+        phis[m3] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// nodes2markers (interpolation.c:929-1048) and moveandrotatemarkers
+// (moveandrotatemarkers.c:1-138)
+// ---------------------------------------------------------------------------
+struct Mech {
+    double EXY, EXYE, SXY, SXYE, PHIS, PHISE, SXX, SXXE, EXX, EXXE, PR, SPPE, VX, MVX, VY, MVY, ES, ESE;
+};
+
+// interpolate_mechanical gathers (interpolation.c:1060-1092, 1211-1237,
+// 1376-1397, 1433-1454) and the antidiffusion scatter-back contributions.
+__device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, double mro, Mech &o,
+                                            double *v /* N_NF, zeroed */)
+{
+    const DevGrid &g = a.g;
+    const int ynumy = g.ynumy;
+    x = clampd(x, 0, g.xsize);
+    y = clampd(y, 0, g.ysize);
+    const int nx = bisect_left(g.gx, g.xnumx, x), ny = bisect_left(g.gy, ynumy, y);
+    Stencil s = basic_stencil(g, nx, ny, x, y);
+    W4 w = wts(s);
+    o.EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
+    o.EXYE = bil(NOD(a, V2C_N_exye), s.m3, ynumy, w);
+    o.SXY = bil(NOD(a, V2C_N_sxy), s.m3, ynumy, w);
+    o.SXYE = bil(NOD(a, V2C_N_sxye), s.m3, ynumy, w);
+    o.ES = bil(NOD(a, V2C_N_pes), s.m3, ynumy, w);
+    o.ESE = bil(NOD(a, V2C_N_pe), s.m3, ynumy, w);
+    o.PHISE = bil(NOD(a, V2C_N_phis), s.m3, ynumy, w);
+    o.PHIS = bil(NOD(a, V2C_N_phis_it), s.m3, ynumy, w);
+    s = centre_stencil(g, nx, ny, x, y);
+    w = wts(s);
+    o.EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
+    o.EXXE = bil(NOD(a, V2C_N_exxe), s.m3, ynumy, w);
+    o.SXX = bil(NOD(a, V2C_N_sxx), s.m3, ynumy, w);
+    o.SXXE = bil(NOD(a, V2C_N_sxxe), s.m3, ynumy, w);
+    o.PR = bil(NOD(a, V2C_N_pr), s.m3, ynumy, w);
+    o.SPPE = bil(NOD(a, V2C_N_sppe), s.m3, ynumy, w);
+    s = vx_stencil(g, nx, ny, x, y, false);
+    w = wts(s);
+    o.VX = bil(NOD(a, V2C_N_vx), s.m3, ynumy, w);
+    o.MVX = bil(NOD(a, V2C_N_mvx), s.m3, ynumy, w);
+    s = vy_stencil(g, nx, ny, x, y, false);
+    w = wts(s);
+    o.VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, w);
+    o.MVY = bil(NOD(a, V2C_N_mvy), s.m3, ynumy, w);
+    if (a.p.antidiffinter != 1) return;
+    const double swt1 = 1.0 / (g.gx[nx + 1] - g.gx[nx]) / (g.gy[ny + 1] - g.gy[ny]);
+    const double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
+    const double cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny]);
+    double bw = 0.0;
+    bool hasb = true;
+    if (cx < 0.5 && cy < 0.5) bw = swt1 * (1.0 - 2 * cx) * (1.0 - 2 * cy);
+    else if (cx < 0.5 && cy > 0.5) bw = swt1 * (1.0 - 2.0 * cx) * (2.0 * cy - 1.0);
+    else if (cx > 0.5 && cy < 0.5) bw = swt1 * (2.0 * cx - 1.0) * (1.0 - 2.0 * cy);
+    else if (cx > 0.5 && cy > 0.5) bw = swt1 * (2.0 * cx - 1.0) * (2.0 * cy - 1.0);
+    else hasb = false;
+    if (hasb) { v[0] = o.PHIS * bw; v[1] = o.SXY * bw; v[2] = o.EXY * bw; v[3] = o.ES * bw; v[4] = bw; }
+    const double cw = swt1 * (1.0 - 2 * ref_abs(cx - 0.5)) * (1.0 - 2 * ref_abs(cy - 0.5));
+    v[N_C] = o.PR * cw; v[N_C + 1] = o.SXX * cw; v[N_C + 2] = o.EXX * cw; v[N_C + 3] = cw;
+    double vw = 0.0;
+    bool hasv = true;
+    if (cx < 0.5) vw = swt1 * (1.0 - cx) * (1.0 - ref_abs(cy - 0.5));
+    else if (cx > 0.5) vw = swt1 * cx * (1.0 - ref_abs(cy - 0.5));
+    else hasv = false;
+    if (hasv) { v[N_VX] = o.VX * mro * vw; v[N_VX + 1] = mro * vw; v[N_VX + 2] = vw; }
+    hasv = true;
+    if (cy < 0.5) vw = swt1 * (1.0 - ref_abs(cx - 0.5)) * (1.0 - cy);
+    else if (cy > 0.5) vw = swt1 * (1.0 - ref_abs(cx - 0.5)) * cy;
+    else hasv = false;
+    if (hasv) { v[N_VY] = o.VY * mro * vw; v[N_VY + 1] = mro * vw; v[N_VY + 2] = vw; }
+}
+
+// Marker state of one marker, loaded once and written once.
+struct MState {
+    double x, y, k, p, xx, xy, exx, exy, e, phi, vx, vy;
+    bool dirty_state, dirty_k;
+};
+
+// Marker update of nodes2markers (interpolation.c:943-1011).
+__device__ __forceinline__ void n2m_update(const KArgs &a, int t, MState &s, const Mech &v)
+{
+    const DevGrid &g = a.g;
+    const bool anti = a.p.antidiffinter == 1;
+    if (s.x > 0 && s.y > 0 && s.x < g.xsize && s.y < g.ysize && t < 50 && s.k <= 0) {
+        if (anti) {
+            s.xx = v.SXX; s.xy = v.SXY; s.p = v.PR; s.exx = v.EXX; s.exy = v.EXY;
+            s.vx = v.VX; s.vy = v.VY; s.phi = v.PHIS; s.e = v.ES;
+        } else {
+            s.xx = v.SXXE; s.xy = v.SXYE; s.p = v.SPPE; s.exx = v.EXXE; s.exy = v.EXYE;
+            s.vx = v.MVX; s.vy = v.MVY; s.phi = v.PHISE; s.e = v.ESE;
+        }
+        s.dirty_state = true;
+        if (!a.p.tempmod) { s.k = a.p.incoming_T; s.dirty_k = true; }
+    }
+    if (s.x >= 0 && s.y >= 0 && s.x <= g.xsize && s.y <= g.ysize && t < 50 && s.k > 0) {
+        if (anti) {
+            s.phi = v.PHIS; s.exx = v.EXX; s.exy = v.EXY; s.xx = v.SXX; s.xy = v.SXY;
+            s.p = v.PR; s.vx = v.VX; s.vy = v.VY; s.e = v.ES;
+        } else {
+            s.phi += (v.PHIS - v.PHISE); s.exx += (v.EXX - v.EXXE); s.exy += (v.EXY - v.EXYE);
+            s.xx += (v.SXX - v.SXXE); s.xy += (v.SXY - v.SXYE); s.p += (v.PR - v.SPPE);
+            s.vx += (v.VX - v.MVX); s.vy += (v.VY - v.MVY); s.e += (v.ES - v.ESE);
+        }
+        s.dirty_state = true;
+    }
+}
+
+// moveandrotatemarkers body for one marker of type < 100 (moveandrotatemarkers.c:12-88).
+// Returns false (and changes nothing) for types >= 100.
+__device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, bool &moved, bool &reset)
+{
+    const DevGrid &g = a.g;
+    const v2c_params &p = a.p;
+    const double dt = a.timestep;
+    moved = false;
+    reset = false;
+    const bool inside = s.x >= 0 && s.y >= 0 && s.x <= g.xsize && s.y <= g.ysize;
+    if (!((inside || p.outgrid != 1) && !a.mat->im[t])) return true;
+    if (t >= 100) return false;
+    double vx0, vy0, sp0, ee0;
+    if (p.markmod == 1) {
+        allinteri(a, s.x, s.y, vx0, vy0, sp0, ee0);
+    } else {
+        double vx1, vy1, sp1, ee1, vx2, vy2, sp2, ee2, vx3, vy3, sp3, ee3, vx4, vy4, sp4, ee4;
+        allinteri(a, s.x, s.y, vx1, vy1, sp1, ee1);
+        allinteri(a, s.x + vx1 * dt / 2.0, s.y + vy1 * dt / 2.0, vx2, vy2, sp2, ee2);
+        allinteri(a, s.x + vx2 * dt / 2.0, s.y + vy2 * dt / 2.0, vx3, vy3, sp3, ee3);
+        allinteri(a, s.x + vx3 * dt, s.y + vy3 * dt, vx4, vy4, sp4, ee4);
+        vx0 = (vx1 + 2.0 * vx2 + 2.0 * vx3 + vx4) / 6.0;
+        vy0 = (vy1 + 2.0 * vy2 + 2.0 * vy3 + vy4) / 6.0;
+        if (p.markmod == 2) {
+            sp0 = (sp1 + 2.0 * sp2 + 2.0 * sp3 + sp4) / 6.0;
+            ee0 = (ee1 + 2.0 * ee2 + 2.0 * ee3 + ee4) / 6.0;
+        } else {
+            sp0 = sp1; ee0 = ee1;
+        }
+    }
+    (void)ee0;
+    if (p.outgrid == 2) {
+        if (s.x < 0 || s.x > g.xsize) vy0 = 0;
+        if (s.y < 0 || s.y > g.ysize) vx0 = 0;
+    }
+    s.x += dt * vx0;
+    s.y += dt * vy0;
+    moved = true;
+    const double th = sp0 * dt;
+    const double sxx = s.xx, sxy = s.xy;
+    if (p.stoksmod == 1) {
+        const double c = cos(th), sn = sin(th), s2 = sin(2.0 * th), c2 = cos(2.0 * th);
+        s.xx = sxx * c * c - sxx * sn * sn + sxy * s2;
+        s.xy = 0.5 * (-sxx - sxx) * s2 + sxy * c2;
+    } else if (p.stoksmod == 2) {
+        s.xx = sxx + sxy * 2.0 * th;
+        s.xy = sxy + 0.5 * (-sxx - sxx) * 2.0 * th;
+    }
+    if (s.y > p.zdeep && vy0 < 0 && s.k < p.tdeep) { s.k = p.tdeep; s.dirty_k = true; }
+    if (s.x < 0 || s.y < 0 || s.x > g.xsize || s.y > g.ysize) {
+        s.k = 0; s.dirty_k = true; s.e = 0; reset = true;
+    }
+    return true;
+}
+
+// Fused (ADVECT) or plain nodes2markers, marker pass over the sorted order.
+template <bool ADVECT>
+__global__ __launch_bounds__(WAVE) void k_n2m_cells(KArgs a, int nwg)
+{
+    __shared__ double acc[CPW * 4][N_NF];
+    __shared__ double con[N_NF][WAVE];
+    __shared__ uint32_t skey[WAVE];
+    const int lane = threadIdx.x;
+    const int wg = xcd_remap(blockIdx.x, nwg);
+    const long c0 = (long)wg * CPW;
+    const long c1 = c0 + CPW < a.ncell ? c0 + CPW : a.ncell;
+    const bool anti = a.p.antidiffinter == 1;
+    for (int s = lane; s < CPW * 4 * N_NF; s += WAVE) (&acc[0][0])[s] = 0.0;
+    const uint32_t beg = a.cell_start[c0], end = a.cell_start[c1];
+    int unsupported = 0;
+    __syncthreads();
+    for (uint32_t base = beg; base < end; base += WAVE) {
+        const uint32_t i = base + lane;
+        const bool active = i < end;
+        double v[N_NF];
+#pragma unroll
+        for (int f = 0; f < N_NF; f++) v[f] = 0.0;
+        uint32_t slot = 0xffffffffu;
+        if (active) {
+            const long m = a.order[i];
+            slot = a.key[m] - (uint32_t)(c0 << 2);
+            const int t = a.t[m];
+            MState s;
+            s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
+            s.dirty_state = false; s.dirty_k = false;
+            Mech mv;
+            mech_gather(a, s.x, s.y, a.mat->ro[t], mv, v);
+            if (!anti) {
+                s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); s.p = MK(a, MF_P, m);
+                s.exx = MK(a, MF_EXX, m); s.exy = MK(a, MF_EXY, m); s.vx = MK(a, MF_VX, m);
+                s.vy = MK(a, MF_VY, m); s.phi = MK(a, MF_PHI, m); s.e = MK(a, MF_E, m);
+            }
+            n2m_update(a, t, s, mv);
+            bool moved = false, reset = false;
+            if (ADVECT) {
+                if (!s.dirty_state) {   // stress needed for the rotation
+                    s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m);
+                    if (!anti) { /* already loaded */ }
+                }
+                double xx0 = s.xx, xy0 = s.xy;
+                if (!advect_one(a, t, s, moved, reset)) unsupported++;
+                if (moved && !s.dirty_state) {
+                    MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy;
+                }
+                (void)xx0; (void)xy0;
+            }
+            if (s.dirty_state) {
+                MK(a, MF_PHI, m) = s.phi; MK(a, MF_EXX, m) = s.exx; MK(a, MF_EXY, m) = s.exy;
+                MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy; MK(a, MF_P, m) = s.p;
+                MK(a, MF_VX, m) = s.vx; MK(a, MF_VY, m) = s.vy; MK(a, MF_E, m) = s.e;
+            }
+            if (s.dirty_k) MK(a, MF_K, m) = s.k;
+            if (ADVECT) {
+                if (moved) {
+                    MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
+                    a.key[m] = subcell_key(a.g, s.x, s.y);
+                }
+                if (reset) {
+                    MK(a, MF_D, m) = -1.0; MK(a, MF_W, m) = -1.0;
+                    if (!s.dirty_state) MK(a, MF_E, m) = 0;
+                }
+            }
+        }
+        if (anti) {
+#pragma unroll
+            for (int f = 0; f < N_NF; f++) con[f][lane] = v[f];
+            skey[lane] = slot;
+            __syncthreads();
+            run_reduce<N_NF>(con, skey, acc, lane, active);
+            __syncthreads();
+        }
+    }
+    if (ADVECT && unsupported) atomicAdd(&a.counters[1], unsupported);
+    if (!anti) return;
+    const long ncell = a.ncell;
+    for (int s = lane; s < CPW * NP_N; s += WAVE) {
+        const int lc = s % CPW, f = s / CPW;
+        const long c = c0 + lc;
+        if (c >= c1) continue;
+        const double(*sl)[N_NF] = &acc[lc * 4];
+        double val;
+        if (f < NP_VX) {
+            val = sl[f / N_NB][f % N_NB];
+        } else if (f < NP_VY) {
+            const int h = (f - NP_VX) / N_NV, k = (f - NP_VX) % N_NV;
+            val = sl[2 * h][N_VX + k] + sl[2 * h + 1][N_VX + k];
+        } else if (f < NP_C) {
+            const int h = (f - NP_VY) / N_NV, k = (f - NP_VY) % N_NV;
+            val = sl[h][N_VY + k] + sl[2 + h][N_VY + k];
+        } else {
+            const int k = f - NP_C;
+            val = sl[0][N_C + k] + sl[1][N_C + k] + sl[2][N_C + k] + sl[3][N_C + k];
+        }
+        a.part[(long)f * ncell + c] = val;
+    }
+}
+
+// The extra iteration of the reference's `m3<=marknum` loop
+// (interpolation.c:937): a zero marker of type 0 at (0,0).  Only its
+// antidiffusion scatter has an effect; added into the persistent sums.
+__global__ void k_n2m_phantom(KArgs a)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || a.p.antidiffinter != 1) return;
+    double v[N_NF];
+    for (int f = 0; f < N_NF; f++) v[f] = 0.0;
+    Mech mv;
+    mech_gather(a, 0.0, 0.0, a.mat->ro[0], mv, v);
+    const int ynumy = a.g.ynumy;
+    // at (0,0): cx = cy = 0 -> basic node 0, vx node 0, vy node 0, centre (1,1)
+    const long cn = (long)ynumy + 1;
+    NOD(a, V2C_N_phisum)[0] += v[0]; NOD(a, V2C_N_sxysum)[0] += v[1]; NOD(a, V2C_N_exysum)[0] += v[2];
+    NOD(a, V2C_N_esum)[0] += v[3]; NOD(a, V2C_N_wtphisum)[0] += v[4];
+    NOD(a, V2C_N_vxsum)[0] += v[N_VX]; NOD(a, V2C_N_mrx0)[0] += v[N_VX + 1]; NOD(a, V2C_N_wtvxsum)[0] += v[N_VX + 2];
+    NOD(a, V2C_N_vysum)[0] += v[N_VY]; NOD(a, V2C_N_mry0)[0] += v[N_VY + 1]; NOD(a, V2C_N_wtvysum)[0] += v[N_VY + 2];
+    NOD(a, V2C_N_psum)[cn] += v[N_C]; NOD(a, V2C_N_sxxsum)[cn] += v[N_C + 1];
+    NOD(a, V2C_N_exxsum)[cn] += v[N_C + 2]; NOD(a, V2C_N_wtsxxsum)[cn] += v[N_C + 3];
+}
+
+// nodes2markers node pass: add the cell partials to the persistent sums and
+// normalise (interpolation.c:1014-1046).
+__global__ void k_n2m_nodes(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const int ynumy = a.g.ynumy, xnumx = a.g.xnumx;
+    const int m22 = (int)(m3 % ynumy), m11 = (int)(m3 / ynumy);
+    const long ncy = ynumy - 1, ncell = a.ncell;
+    const double *P = a.part;
+    auto cell = [&](int cx, int cy) -> long {
+        return (cx >= 0 && cy >= 0 && cx < xnumx - 1 && cy < ynumy - 1) ? (long)cx * ncy + cy : -1;
+    };
+    double B[N_NB] = {0, 0, 0, 0, 0};
+    {
+        const long cs[4] = {cell(m11 - 1, m22 - 1), cell(m11 - 1, m22), cell(m11, m22 - 1), cell(m11, m22)};
+        const int qs[4] = {3, 2, 1, 0};
+        for (int k = 0; k < 4; k++) {
+            if (cs[k] < 0) continue;
+            for (int f = 0; f < N_NB; f++) B[f] += P[(long)(qs[k] * N_NB + f) * ncell + cs[k]];
+        }
+    }
+    double VXs[N_NV] = {0, 0, 0}, VYs[N_NV] = {0, 0, 0}, Cs[N_NC] = {0, 0, 0, 0};
+    {
+        long cl = cell(m11 - 1, m22), cr = cell(m11, m22);
+        for (int f = 0; f < N_NV; f++) {
+            if (cl >= 0) VXs[f] += P[(long)(NP_VX + N_NV + f) * ncell + cl];
+            if (cr >= 0) VXs[f] += P[(long)(NP_VX + f) * ncell + cr];
+        }
+        long ct = cell(m11, m22 - 1), cb = cell(m11, m22);
+        for (int f = 0; f < N_NV; f++) {
+            if (ct >= 0) VYs[f] += P[(long)(NP_VY + N_NV + f) * ncell + ct];
+            if (cb >= 0) VYs[f] += P[(long)(NP_VY + f) * ncell + cb];
+        }
+        long cc = cell(m11 - 1, m22 - 1);
+        if (cc >= 0)
+            for (int f = 0; f < N_NC; f++) Cs[f] = P[(long)(NP_C + f) * ncell + cc];
+    }
+    double *phisum = NOD(a, V2C_N_phisum), *sxysum = NOD(a, V2C_N_sxysum), *exysum = NOD(a, V2C_N_exysum);
+    double *esum = NOD(a, V2C_N_esum), *wtphisum = NOD(a, V2C_N_wtphisum);
+    double *vxsum = NOD(a, V2C_N_vxsum), *mrx0 = NOD(a, V2C_N_mrx0), *wtvxsum = NOD(a, V2C_N_wtvxsum);
+    double *vysum = NOD(a, V2C_N_vysum), *mry0 = NOD(a, V2C_N_mry0), *wtvysum = NOD(a, V2C_N_wtvysum);
+    double *psum = NOD(a, V2C_N_psum), *sxxsum = NOD(a, V2C_N_sxxsum), *exxsum = NOD(a, V2C_N_exxsum);
+    double *wtsxxsum = NOD(a, V2C_N_wtsxxsum);
+    const double s_phi = phisum[m3] + B[0], s_sxy = sxysum[m3] + B[1], s_exy = exysum[m3] + B[2];
+    const double s_e = esum[m3] + B[3], s_w = wtphisum[m3] + B[4];
+    const double s_vx = vxsum[m3] + VXs[0], s_mrx = mrx0[m3] + VXs[1], s_wvx = wtvxsum[m3] + VXs[2];
+    const double s_vy = vysum[m3] + VYs[0], s_mry = mry0[m3] + VYs[1], s_wvy = wtvysum[m3] + VYs[2];
+    const double s_p = psum[m3] + Cs[0], s_sxx = sxxsum[m3] + Cs[1], s_exx = exxsum[m3] + Cs[2];
+    const double s_wc = wtsxxsum[m3] + Cs[3];
+    phisum[m3] = s_phi; sxysum[m3] = s_sxy; exysum[m3] = s_exy; esum[m3] = s_e; wtphisum[m3] = s_w;
+    vxsum[m3] = s_vx; mrx0[m3] = s_mrx; wtvxsum[m3] = s_wvx;
+    vysum[m3] = s_vy; mry0[m3] = s_mry; wtvysum[m3] = s_wvy;
+    psum[m3] = s_p; sxxsum[m3] = s_sxx; exxsum[m3] = s_exx; wtsxxsum[m3] = s_wc;
+    if (s_w != 0 && m11 >= 1 && m22 >= 1 && m11 < xnumx - 1 && m22 < ynumy - 1) {
+        NOD(a, V2C_N_phie)[m3] = s_phi / s_w;
+        NOD(a, V2C_N_sxyee)[m3] = s_sxy / s_w;
+        NOD(a, V2C_N_exyee)[m3] = s_exy / s_w;
+        NOD(a, V2C_N_pee)[m3] = s_e / s_w;
+    }
+    if (s_wc != 0 && m11 >= 1 && m22 >= 1) {
+        NOD(a, V2C_N_sppee)[m3] = s_p / s_wc;
+        NOD(a, V2C_N_sxxee)[m3] = s_sxx / s_wc;
+        NOD(a, V2C_N_exxee)[m3] = s_exx / s_wc;
+    }
+    if (s_wvx != 0) NOD(a, V2C_N_mvxe)[m3] = s_vx / s_mrx;
+    if (s_wvy != 0) NOD(a, V2C_N_mvye)[m3] = s_vy / s_mry;
+}
+
+// Standalone moveandrotatemarkers over physical marker order.
+__global__ void k_advect(KArgs a)
+{
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int unsupported = 0;
+    if (m < a.n) {
+        const int t = a.t[m];
+        MState s;
+        s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
+        s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m);
+        s.dirty_k = false;
+        bool moved, reset;
+        if (!advect_one(a, t, s, moved, reset)) unsupported = 1;
+        if (moved) {
+            MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
+            MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy;
+            a.key[m] = subcell_key(a.g, s.x, s.y);
+        }
+        if (s.dirty_k) MK(a, MF_K, m) = s.k;
+        if (reset) { MK(a, MF_D, m) = -1.0; MK(a, MF_W, m) = -1.0; MK(a, MF_E, m) = 0; }
+    }
+    if (unsupported) atomicAdd(&a.counters[1], 1);
+}
+
+// ---------------------------------------------------------------------------
+// compute_before_solve / calc_vpvisc (plastic.c:2-56, calc_RSF.c:33-169)
+// ---------------------------------------------------------------------------
+__global__ void k_before(KArgs a, double dt, int rk)
+{
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const int ynumy = a.g.ynumy;
+    const int m2 = (int)(m3 % ynumy), m1 = (int)(m3 / ynumy);
+    if (!(m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ynumy - 1)) return;
+    const v2c_params &p = a.p;
+    const double W = p.faultwidth;
+    const double nps = (p.pmode == 0 || a.timesum == 0) ? p.nstress : NOD(a, V2C_N_sppes)[m3];
+    const double sxxs = NOD(a, V2C_N_sxxs)[m3], sxy = NOD(a, V2C_N_sxy)[m3];
+    const double sigin = pow(sxxs * sxxs + sxy * sxy, 0.5);
+    int err = 0;
+    double nu;
+    if (sigin > 0 || p.RSFmode == 0) {
+        double V = 0;
+        if (rk == 0) {
+            if (a.timesum > 0) V = NOD(a, V2C_N_RK_sliprs_final)[m3];
+            else if (p.RSFmode == 1) V = NOD(a, V2C_N_RK_sliprs_final)[m3];
+        }
+        if ((p.rkmod == 0 && rk > 0) || (rk > 0 && rk < 3)) {
+            if (p.rkmod == 1) {
+                V = NOD(a, V2C_N_RK_sliprs_solved)[m3];
+            } else {
+                if (p.rkmod == 3) {
+                    NOD(a, rk - 1 == 0 ? V2C_N_RK_sliprs0 : V2C_N_RK_sliprs1)[m3] = NOD(a, V2C_N_RK_sliprs_solved)[m3];
+                    if (p.RSFmode == 1)
+                        NOD(a, rk - 1 == 0 ? V2C_N_RK_phidts0 : V2C_N_RK_phidts1)[m3] = NOD(a, V2C_N_RK_phidts_solved)[m3];
+                }
+                V = NOD(a, V2C_N_RK_sliprs_solved)[m3];
+            }
+        }
+        if (rk == 3 && p.rkmod == 3) V = NOD(a, V2C_N_RK_sliprs_solved)[m3];
+        NOD(a, V2C_N_pes)[m3] = NOD(a, V2C_N_pe)[m3] + V / W * dt;
+        double newstate = 0, syield, vp = 0;
+        const double as = NOD(a, V2C_N_as)[m3], bs = NOD(a, V2C_N_bs)[m3], Ls = NOD(a, V2C_N_Ls)[m3];
+        const double V0s = NOD(a, V2C_N_V0s)[m3], mus = NOD(a, V2C_N_mus_s)[m3];
+        const double nu_m2c = NOD(a, V2C_N_nu_m2c)[m3];
+        if (p.RSFmode == 1) {
+            const double phis = NOD(a, V2C_N_phis)[m3];
+            if (V * dt / Ls <= 1e-6) newstate = log(exp(phis) * (1 - V * dt / Ls) + V0s * dt / Ls);
+            else newstate = log(V0s / V + (exp(phis) - V0s / V) * exp(-V * dt / Ls));
+            const double rad = p.inertyn == 0 ? p.rad_damping * V : 0.0;
+            if (p.power_law_RSF > 0) {
+                const double sy0 = nps * mus;
+                syield = sy0 * pow(V / V0s, as / mus) * exp((newstate * bs) / mus) + rad;
+            } else {
+                syield = nps * as * asinh(V / (2 * V0s) * exp((mus + newstate * bs) / as)) + rad;
+            }
+        } else {
+            syield = nps * mus + NOD(a, V2C_N_cohesion)[m3];
+        }
+        if (p.RSFmode == 1) vp = syield > 0 ? nu_m2c * syield / (2 * nu_m2c * V / W + syield) : nu_m2c;
+        else if (p.RSFmode == 0) vp = nu_m2c * syield / (2 * nu_m2c * V / W + syield);
+        NOD(a, V2C_N_phis_it)[m3] = newstate;
+        NOD(a, V2C_N_syields)[m3] = syield;
+        NOD(a, V2C_N_nuvp_it)[m3] = vp;
+        NOD(a, V2C_N_sliprs)[m3] = V;
+        if ((a.timesum > 0 && (isnan(newstate) || isnan(syield) || isnan(vp) || vp == 0 || isnan(V)))
+            || V < 0 || syield < 0)
+            err = 1;
+        nu = vp;
+    } else {
+        nu = NOD(a, V2C_N_nu_m2c)[m3];
+    }
+    NOD(a, V2C_N_nu)[m3] = nu;
+    if (isnan(nu) || nu <= 0) err = 1;
+    if (err) atomicAdd(&a.counters[0], 1);
+}
+
+// compute_after_solve / calc_sliprate_after_solve (plastic.c:61-129,
+// calc_RSF.c:172-221).  Per-block partial sums of the misfit (fixed tree
+// order) go to red[2*block], red[2*block+1].
+constexpr int RED_BLOCK = 256;
+__global__ __launch_bounds__(RED_BLOCK) void k_after(KArgs a, double dt, double *red)
+{
+    __shared__ double ssum[RED_BLOCK];
+    __shared__ double scnt[RED_BLOCK];
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dsq = 0.0, cnt = 0.0;
+    if (m3 < a.nodenum) {
+        const int ynumy = a.g.ynumy;
+        const int m2 = (int)(m3 % ynumy), m1 = (int)(m3 / ynumy);
+        if (m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ynumy - 1) {
+            const v2c_params &p = a.p;
+            const double W = p.faultwidth;
+            const double nP = (p.pmode == 0 || a.timesum == 0) ? p.nstress : NOD(a, V2C_N_sppes)[m3];
+            const double sxxs = NOD(a, V2C_N_sxxs)[m3], sxy = NOD(a, V2C_N_sxy)[m3];
+            const double sigin = pow(sxxs * sxxs + sxy * sxy, 0.5);
+            double *Vs = NOD(a, V2C_N_RK_sliprs_solved);
+            const double as = NOD(a, V2C_N_as)[m3], bs = NOD(a, V2C_N_bs)[m3], Ls = NOD(a, V2C_N_Ls)[m3];
+            const double V0s = NOD(a, V2C_N_V0s)[m3], mus = NOD(a, V2C_N_mus_s)[m3];
+            const double phi = NOD(a, V2C_N_phis_it)[m3];
+            double Vout;
+            if (p.RSFmode == 1) {
+                if (sigin > 0) {
+                    double V;
+                    if (p.power_law_RSF > 0) {
+                        const double sy0 = nP * mus;
+                        V = V0s * pow(sigin / sy0, (mus / as)) * exp((-phi * bs) / as);
+                    } else {
+                        V = 2 * V0s * sinh(sigin / (as * nP)) * exp(-(mus + bs * phi) / as);
+                    }
+                    const double EPXX = NOD(a, V2C_N_exxpl)[m3], EXY = NOD(a, V2C_N_exy)[m3];
+                    const double EII = pow(EPXX * EPXX + EXY * EXY, 0.5);
+                    V = ref_minv(V, EII * W);
+                    if (V < 1e-40) V = 1e-40;
+                    Vout = V;
+                } else {
+                    Vout = 0;
+                }
+                Vs[m3] = Vout;
+                NOD(a, V2C_N_RK_phidts_solved)[m3] = V0s / Ls * exp(-phi) - Vout / Ls;
+            } else {
+                const double G = NOD(a, V2C_N_gg)[m3], NUVP = NOD(a, V2C_N_nu)[m3];
+                const double NU = NOD(a, V2C_N_nu_m2c)[m3];
+                const double E2EL = sigin / (NUVP / (G * dt + NUVP));
+                const double A = NOD(a, V2C_N_syields)[m3] / E2EL;
+                if (A < 1) {
+                    const double nup = G * dt * A / (1.0 - A);
+                    Vout = W * (sigin / 2.0 * (1.0 / (nup) - 1.0 / NU));
+                } else {
+                    Vout = 0;
+                }
+                Vs[m3] = Vout;
+            }
+            if (Vout > 0) {
+                const double d = NOD(a, V2C_N_syields)[m3] - sigin;
+                NOD(a, V2C_N_RK_deltas)[m3] = d;
+                dsq = pow(d, 2);
+                cnt = 1.0;
+            } else {
+                NOD(a, V2C_N_RK_deltas)[m3] = 0;
+            }
+        }
+    }
+    ssum[threadIdx.x] = dsq;
+    scnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = RED_BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+            scnt[threadIdx.x] += scnt[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        red[2 * blockIdx.x] = ssum[0];
+        red[2 * blockIdx.x + 1] = scnt[0];
+    }
+}
+
+// Single-block fixed-order reduction of the per-block partials.
+__global__ __launch_bounds__(RED_BLOCK) void k_after_final(const double *red, int nblocks, double *out)
+{
+    __shared__ double ssum[RED_BLOCK];
+    __shared__ double scnt[RED_BLOCK];
+    double s = 0, c = 0;
+    for (int b = threadIdx.x; b < nblocks; b += RED_BLOCK) { s += red[2 * b]; c += red[2 * b + 1]; }
+    ssum[threadIdx.x] = s;
+    scnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int k = RED_BLOCK / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) { ssum[threadIdx.x] += ssum[threadIdx.x + k]; scnt[threadIdx.x] += scnt[threadIdx.x + k]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double count = scnt[0];
+        out[0] = count > 0 ? pow(ssum[0] / count, 0.5) : 0.0;
+        out[1] = count;
+    }
+}
+
+// ductilevisc on independent inputs (the host-callable helper).
+__global__ void k_ductile_batch(KArgs a, long n, const int8_t *t, const double *sxy, const double *sxx,
+                                const double *tk, const double *pb, double *out, int *iters)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int it;
+    out[i] = ductilevisc(a, t[i], sxy[i], sxx[i], tk[i], pb[i], &it);
+    iters[i] = it;
+}
+
+}  // namespace v2c
