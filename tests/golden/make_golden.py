@@ -1,0 +1,130 @@
+"""Write the golden vectors of tests/golden/ by running the reference's own
+hot-path routines, compiled in place from /root/reference/src
+(oracle/_ref/ref_driver, built by oracle/Makefile), single-threaded.
+
+    python tests/golden/make_golden.py
+
+Each fixture holds its complete input (grid, flags, rock table, markers, nodal
+arrays) and the reference's output after the listed ops, so the tests never
+need the reference or the generator that produced the inputs.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from velo2cycles_amd import synth  # noqa: E402
+from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS, Params  # noqa: E402
+from oracle import refrun  # noqa: E402
+
+DT = synth.YEAR
+MAT = "nu dh dv ss mm gg n0 n1 ro bb aa cp kt kf kp ht".split()
+
+
+def pack(cfg, ops, out_markers, out_nodal, raw, extra=None):
+    d = {"gx": cfg.grid.gx, "gy": cfg.grid.gy,
+         "xsize": np.array(cfg.grid.xsize), "ysize": np.array(cfg.grid.ysize),
+         "timesum": np.array(cfg.timesum), "timestep": np.array(cfg.timestep),
+         "params": np.array(json.dumps(cfg.params.as_dict())),
+         "ops": np.array(json.dumps(list(ops))),
+         "mat_ntypes": np.array(cfg.mats.ntypes),
+         "mat_im": np.array(cfg.mats.im[:], dtype=np.int32)}
+    for k in MAT:
+        d["mat_" + k] = np.array(getattr(cfg.mats, k)[:])
+    for f in MARKER_FIELDS:
+        d["in_m_" + f] = cfg.markers[f]
+        d["out_m_" + f] = out_markers[f]
+    d["in_m_t"] = cfg.markers.t
+    d["out_m_t"] = out_markers["t"]
+    for k in NODAL_FIELDS:
+        d["in_n_" + k] = cfg.nodal[k]
+        d["out_n_" + k] = out_nodal[k]
+    d["ret"] = raw["drv_ret"][:2]
+    d["errcheck"] = raw["errcheck"]
+    if extra:
+        d.update(extra)
+    return d
+
+
+def case(name, cfg, ops):
+    mk, nd, raw = refrun.run(cfg, ops)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **pack(cfg, ops, mk, nd, raw))
+    print("wrote", name, cfg.n, "markers", cfg.grid.nodenum, "nodes", ops)
+
+
+def main():
+    assert refrun.available(), "build oracle/_ref first (make -C oracle)"
+    step = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m", "move"]
+    cfg = synth.config_edge()
+    case("edge_step_t0", cfg, step)
+    cfg = synth.config_edge()
+    cfg.timesum = DT
+    case("edge_step_t1", cfg, step)
+    cfg = synth.config_edge(seed=99)
+    cfg.timesum = DT
+    cfg.params.markmod = 1
+    cfg.params.stoksmod = 2
+    cfg.params.densimod = 1
+    cfg.params.outgrid = 0
+    case("edge_euler_jaumann_densimod", cfg, step)
+    cfg = synth.config_edge(seed=5)
+    cfg.timesum = DT
+    cfg.params.RSFmode = 0
+    cfg.params.antidiffinter = 0
+    cfg.params.tempmod = 1
+    case("edge_dp_noantidiff", cfg, step)
+    cfg = synth.config_edge(seed=6)
+    cfg.timesum = DT
+    cfg.params.power_law_RSF = 1
+    cfg.params.rkmod = 3
+    cfg.params.inertyn = 1
+    cfg.params.pmode = 1
+    cfg.params.Dphis = 1.0
+    cfg.params.compressiblemode = 1
+    case("edge_rk3_powerlaw", cfg, ["m2n", f"before:{DT / 2}:1", f"after:{DT / 2}:1",
+                                    f"before:{DT / 2}:2", f"after:{DT / 2}:2", f"before:{DT}:3"])
+
+    # pure-function vectors: ductilevisc and allinteri
+    cfg = synth.config_edge()
+    rng = np.random.default_rng(2026)
+    n = 3000
+    t = rng.integers(0, 9, n).astype(np.int8)
+    sxy = rng.normal(0, 3e7, n)
+    sxx = rng.normal(0, 3e7, n)
+    sxx[:40] = 0.0
+    sxy[:40] = 0.0
+    tk = rng.uniform(273, 1900, n)
+    pb = rng.uniform(1, 4e4, n)
+    _, _, raw = refrun.run(cfg, ["ductile"], extra={
+        "drv_n": ("l", np.array([n], np.int64)), "dv_type": ("c", t), "dv_sxy": ("d", sxy),
+        "dv_sxx": ("d", sxx), "dv_tk": ("d", tk), "dv_pb": ("d", pb)})
+    base = pack(cfg, [], {f: cfg.markers[f] for f in MARKER_FIELDS} | {"t": cfg.markers.t},
+                cfg.nodal, raw)
+    keep = {k: v for k, v in base.items() if not k.startswith(("in_m_", "out_m_", "out_n_"))}
+    keep.update(dv_type=t, dv_sxy=sxy, dv_sxx=sxx, dv_tk=tk, dv_pb=pb, dv_out=raw["dv_out"][:n])
+    np.savez_compressed(os.path.join(HERE, "ductilevisc.npz"), **keep)
+    print("wrote ductilevisc", n)
+
+    g = cfg.grid
+    x = rng.uniform(-0.05 * g.xsize, 1.05 * g.xsize, n)
+    y = rng.uniform(-0.05 * g.ysize, 1.05 * g.ysize, n)
+    x[:13] = g.gx
+    y[13:22] = g.gy
+    _, _, raw = refrun.run(cfg, ["allinteri"], extra={
+        "drv_n": ("l", np.array([n], np.int64)), "ai_x": ("d", x), "ai_y": ("d", y)})
+    keep = {k: v for k, v in base.items() if not k.startswith(("in_m_", "out_m_", "out_n_"))}
+    keep.update(ai_x=x, ai_y=y, ai_vx=raw["ai_vx"][:n], ai_vy=raw["ai_vy"][:n],
+                ai_esp=raw["ai_esp"][:n], ai_ee=raw["ai_ee"][:n])
+    np.savez_compressed(os.path.join(HERE, "allinteri.npz"), **keep)
+    print("wrote allinteri", n)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,44 @@
+"""The oracle against the reference's own outputs (golden vectors written by
+tests/golden/make_golden.py from the in-place reference build, one thread).
+Bar: bit for bit, every marker and nodal field."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS
+from oracle.pyoracle import Oracle
+from tests.util import load_fixture, run_oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+STEP_FIXTURES = sorted(p for p in glob.glob(os.path.join(GOLDEN, "edge_*.npz")))
+
+
+@pytest.mark.parametrize("path", STEP_FIXTURES, ids=lambda p: os.path.basename(p)[:-4])
+def test_oracle_matches_reference_bitwise(path):
+    cfg, ops, exp, z = load_fixture(path)
+    om, on, ret = run_oracle(cfg, ops)
+    for f in MARKER_FIELDS:
+        assert np.array_equal(om[f], exp["markers"][f], equal_nan=True), f"marker {f}"
+    assert np.array_equal(om.t, exp["markers"].t)
+    for k in NODAL_FIELDS:
+        assert np.array_equal(on[k], exp["nodal"][k], equal_nan=True), f"nodal {k}"
+    if any(o.startswith("after") for o in ops):
+        assert ret["after"][0] == z["ret"][1]
+
+
+def test_ductilevisc_vectors():
+    cfg, _, _, z = load_fixture(os.path.join(GOLDEN, "ductilevisc.npz"))
+    o = Oracle(cfg)
+    out = np.array([o.ductilevisc(t, a, b, c, d)[0] for t, a, b, c, d in
+                    zip(z["dv_type"], z["dv_sxy"], z["dv_sxx"], z["dv_tk"], z["dv_pb"])])
+    assert np.array_equal(out, z["dv_out"], equal_nan=True)
+
+
+def test_allinteri_vectors():
+    cfg, _, _, z = load_fixture(os.path.join(GOLDEN, "allinteri.npz"))
+    o = Oracle(cfg)
+    got = np.array([o.allinteri(x, y) for x, y in zip(z["ai_x"], z["ai_y"])])
+    for j, k in enumerate(("ai_vx", "ai_vy", "ai_esp", "ai_ee")):
+        assert np.array_equal(got[:, j], z[k]), k

@@ -2,6 +2,8 @@
 through libv2c, and the field-by-field comparison bar."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS, Context
@@ -114,3 +116,36 @@ def run_gpu(lib, cfg, ops, markers=None, nodal=None, ctx=None):
     ctx.download_markers(m)
     nod = ctx.get_all_nodal()
     return m, nod, ret, ctx
+
+
+def load_fixture(path):
+    """Rebuild (cfg, ops, expected markers, expected nodal, ret) from a golden .npz."""
+    import json
+    from velo2cycles_amd import synth
+    from velo2cycles_amd.abi import GridArrays, MarkerArrays, Materials, Params
+    z = np.load(path, allow_pickle=False)
+    grid = GridArrays(z["gx"], z["gy"], float(z["xsize"]), float(z["ysize"]))
+    p = Params()
+    for k, v in json.loads(str(z["params"])).items():
+        setattr(p, k, v)
+    mats = Materials()
+    mats.ntypes = int(z["mat_ntypes"])
+    for k in "nu dh dv ss mm gg n0 n1 ro bb aa cp kt kf kp ht".split():
+        getattr(mats, k)[:] = list(z["mat_" + k])
+    mats.im[:] = [int(v) for v in z["mat_im"]]
+    out = {}
+    if "in_m_x" in z:
+        fields = {f: z["in_m_" + f] for f in MARKER_FIELDS}
+        fields["t"] = z["in_m_t"]
+        markers = MarkerArrays(len(z["in_m_x"]), fields)
+        ef = {f: z["out_m_" + f] for f in MARKER_FIELDS}
+        ef["t"] = z["out_m_t"]
+        out["markers"] = MarkerArrays(len(z["in_m_x"]), ef)
+        out["nodal"] = {k: z["out_n_" + k] for k in NODAL_FIELDS}
+    else:
+        markers = MarkerArrays(0)
+    nodal = {k: np.array(z["in_n_" + k]) for k in NODAL_FIELDS}
+    cfg = synth.Config(os.path.basename(path), grid, p, mats, markers, nodal,
+                       float(z["timestep"]), float(z["timesum"]))
+    ops = json.loads(str(z["ops"]))
+    return cfg, ops, out, z
