@@ -17,7 +17,7 @@
  *
  * Ownership: after v2c_markers_upload the device owns the marker state until
  * v2c_markers_download.  Nodal arrays live in a device mirror; the host moves
- * them with v2c_nodal_set / v2c_nodal_get (or the v2c_nodal_io helper).
+ * them with v2c_nodal_set / v2c_nodal_get.
  *
  * Every entry point returns 0 on success or a negative V2C_E* code; it never
  * calls exit().  A context is not thread-safe: one host thread per context.
@@ -186,6 +186,10 @@ int v2c_markers_upload(v2c_ctx *ctx, const v2c_markers *m, long n);
 int v2c_markers_download(v2c_ctx *ctx, v2c_markers *m, long n);
 long v2c_marker_count(const v2c_ctx *ctx);
 
+/* Statistics of the cell ordering: markers whose sub-cell changed at the last
+ * re-ordering, number of full radix sorts and of incremental merges so far. */
+int v2c_order_stats(v2c_ctx *ctx, long *last_movers, long *full_sorts, long *incremental);
+
 /* Nodal mirror transfer, one field at a time (length nodenum). */
 int v2c_nodal_set(v2c_ctx *ctx, int field, const double *host);
 int v2c_nodal_get(v2c_ctx *ctx, int field, double *host);
@@ -230,11 +234,45 @@ int v2c_ductilevisc_batch(v2c_ctx *ctx, long n, const char *type,
                           const double *tk, const double *pb,
                           double *nu_out, int *iters);
 
+/* ---- build-defined per-marker local update (no reference counterpart) ----
+ * The reference evaluates rate-and-state friction per node (plastic.c,
+ * calc_RSF.c:33-221) and the ductile viscosity with a quasi-Newton that stops
+ * on |log10(mu_k/mu_{k-1})| < 1e-6 (density_ductilevisc_calc.c:170-200).  The
+ * local update evaluates both per marker to a stated residual instead:
+ *   ductile: Newton on f(eta) = eta - coeff*X^((n-1)/n) (the reference's own
+ *            residual, density_ductilevisc_calc.c:188-193) to |f|/eta <= tol;
+ *   RSF (fault markers): state aged over dt at the previous slip rate
+ *            (calc_RSF.c:96-103), then the slip rate V solving
+ *            sigII = nP a asinh(V/(2V0) exp((mu0+b phi')/a)) + eta_rad V
+ *            (calc_RSF.c:111,129) by Newton in ln V to |g|/sigII <= tol,
+ *            capped at EII*W and floored at 1e-40 (calc_RSF.c:193-196).
+ * Warm start: the context's MLP proposes eta and V; a start whose residual
+ * grows after one step falls back to the default start. */
+typedef struct v2c_lu_stats {
+    long markers, ductile_markers, ductile_iters, fault_markers, rsf_iters;
+    long max_ductile_iters, max_rsf_iters, fallbacks, nonfinite;
+    long hist_ductile[32], hist_rsf[32];     /* iteration histograms; last bin = overflow */
+} v2c_lu_stats;
+
+/* Per-marker RSF inputs: fault flag, slip rate V, state phi and the marker's
+ * own a, b, L, V0, mu0 (all length n, host arrays). */
+int v2c_rsf_upload(v2c_ctx *ctx, const unsigned char *fault, const double *V, const double *phi,
+                   const double *a, const double *b, const double *L, const double *V0,
+                   const double *mu0, long n);
+/* warm: 0 plain start, 1 MLP start (v2c_mlp_predict must have run). */
+int v2c_local_update(v2c_ctx *ctx, double dt, int warm, double tol, v2c_lu_stats *stats);
+/* Outputs of the last local update (any pointer may be NULL). */
+int v2c_local_download(v2c_ctx *ctx, double *eta, double *V, double *phi, double *eta_vp,
+                       double *syield, long n);
+/* Set the warm-start guesses directly (host arrays; NULL keeps a field). */
+int v2c_local_set_guess(v2c_ctx *ctx, const double *eta, const double *V, long n);
+
 /* Timing of the last entry point's kernels (HIP events on the context
  * stream), in milliseconds; `kernel` selects one of V2C_K_* . */
 enum v2c_kernel_id {
     V2C_K_M2N_CELLS = 0, V2C_K_M2N_NODES, V2C_K_N2M_CELLS, V2C_K_N2M_NODES,
-    V2C_K_ADVECT, V2C_K_SORT, V2C_K_BEFORE, V2C_K_AFTER, V2C_K_COUNT
+    V2C_K_ADVECT, V2C_K_SORT, V2C_K_BEFORE, V2C_K_AFTER, V2C_K_LOCAL, V2C_K_MLP,
+    V2C_K_COUNT
 };
 int v2c_enable_timing(v2c_ctx *ctx, int on);
 int v2c_kernel_time(v2c_ctx *ctx, int kernel, double *ms, long *launches);

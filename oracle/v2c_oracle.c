@@ -258,22 +258,87 @@ double or_ductilevisc_newton(const or_state *s, int t, double sxy, double sxx,
     double coeff = pow(eta_disl0, 1.0 / n);
     double gdt = M->gg[t] * s->timestep;
     double eta = guess > 0 ? guess : eta_disl0;
-    double r = 0;
-    int k = 0;
-    for (; k < 100; k++) {
+    double r = 0, rprev = -1;
+    int k = 0, restarted = guess <= 0;
+    for (; k < 200; k++) {
         double X = (eta_diff + eta) / eta_diff * gdt + eta;
-        double f = eta - coeff * pow(X, (n - 1) / n);
+        double Xp = pow(X, (n - 1) / n);
+        double f = eta - coeff * Xp;
         r = fabs(f) / eta;
         if (r <= tol) break;
-        double df = 1 - (n - 1) / n * coeff * pow(X, -1 / n) * (gdt / eta_diff + 1);
-        double step = f / df;
-        double next = eta - step;
-        while (!(next > 0)) { step *= 0.5; next = eta - step; }
+        /* warm start fallback: the residual grew after the first step */
+        if (!restarted && k == 1 && r > rprev) {
+            eta = eta_disl0;
+            restarted = 1;
+            if (resid) *resid = -1.0;      /* marks the fallback for the caller */
+            continue;
+        }
+        rprev = r;
+        /* X^(-1/n) evaluated as X^((n-1)/n) / X */
+        double df = 1 - (n - 1) / n * coeff * (Xp / X) * (gdt / eta_diff + 1);
+        if (f < 0 && !(df > 0)) { eta *= 2.0; continue; }   /* left of a flat region */
+        double next = eta - f / df;
+        if (!(next > 0)) next = 0.5 * eta;
         eta = next;
     }
     if (iters) *iters = k;
-    if (resid) *resid = r;
+    if (resid && *resid != -1.0) *resid = r;
     return clamp_visc(s, t, 1.0 / (1.0 / eta_diff + 1.0 / eta));
+}
+
+/* Build-defined per-marker RSF update (not reference behaviour).  With the
+ * state advanced by the aging law over dt at the previous slip rate
+ * (calc_RSF.c:96-103), solve the regularised RSF yield condition with
+ * radiation damping for the slip rate,
+ *   g(s) = nP a asinh(e^s K) + eta_rad e^s - sigII = 0,  K = exp((mu0+b phi')/a)/(2 V0),
+ * by Newton in s = ln V (g is increasing and convex in s, so Newton started
+ * right of the root converges monotonically).  Without damping the root is
+ * the closed form of calc_RSF.c:192.  The result is capped at EII*W and
+ * floored at 1e-40 like calc_RSF.c:193-196.  Returns the number of Newton
+ * iterations; *fallback = 1 when a warm start was abandoned. */
+int or_rsf_marker(double sig, double nP, double a, double b, double L, double V0, double mu0,
+                  double phi, double Vprev, double dt, double eta_rad, double EII, double W,
+                  double guessV, double tol, double *V_out, double *phi_out, int *fallback)
+{
+    double phin;
+    if (Vprev * dt / L <= 1e-6) phin = log(exp(phi) * (1 - Vprev * dt / L) + V0 * dt / L);
+    else phin = log(V0 / Vprev + (exp(phi) - V0 / Vprev) * exp(-Vprev * dt / L));
+    *phi_out = phin;
+    *fallback = 0;
+    if (!(sig > 0)) { *V_out = 0; return 0; }
+    double K = exp((mu0 + b * phin) / a) / (2 * V0);
+    double Vu = 2 * V0 * sinh(sig / (a * nP)) * exp(-(mu0 + b * phin) / a);
+    double Vhi = Vu;
+    if (eta_rad > 0 && sig / eta_rad < Vhi) Vhi = sig / eta_rad;
+    double V;
+    int k = 0;
+    if (eta_rad == 0 || !(Vhi > 0) || !isfinite(Vhi)) {
+        V = Vhi;                                     /* closed form (or its overflow) */
+    } else {
+        double shi = log(Vhi);
+        int warm = guessV > 0;
+        double s = warm ? log(guessV) : shi;
+        if (s > shi) s = shi;
+        double gprev = -1;
+        for (; k < 200; k++) {
+            double e = exp(s), z = e * K;
+            double g = nP * a * asinh(z) + eta_rad * e - sig;
+            double ag = fabs(g) / sig;
+            if (ag <= tol) break;
+            if (warm && k == 1 && ag > gprev) { s = shi; warm = 0; *fallback = 1; continue; }
+            gprev = ag;
+            double dg = nP * a * z / sqrt(1 + z * z) + eta_rad * e;
+            double sn = s - g / dg;
+            if (!(sn <= shi)) sn = shi;
+            if (!(sn == sn)) sn = shi;
+            s = sn;
+        }
+        V = exp(s);
+    }
+    V = ref_minv(V, EII * W);
+    if (V < 1e-40) V = 1e-40;
+    *V_out = V;
+    return k;
 }
 
 /* ---- calc_frictionpara (calc_RSF.c:224-295): per-marker RSF parameters from
@@ -968,4 +1033,53 @@ double or_compute_after_solve(or_state *s, double dt, int rk, long *nerr)
     deltas = count > 0 ? pow(deltas / count, 0.5) : 0;
     if (nerr) *nerr = isnan(deltas) ? 1 : 0;
     return deltas;
+}
+
+/* Per-marker local update over all markers (build-defined).  mx holds 12
+ * fields x n in the device layout: V, phi, a, b, L, V0, mu0, eta, eta_vp, sy,
+ * guess_eta, guess_V.  stats: [n, ductile, ductile iters, fault, rsf iters,
+ * max ductile, max rsf, fallbacks, nonfinite]. */
+void or_local_update(or_state *s, double dt, const unsigned char *fault, double *mx, int warm,
+                     double tol, long *stats, int *ditr, int *ritr)
+{
+    const long n = s->n;
+    double *V = mx, *PHI = mx + n, *A = mx + 2 * n, *B = mx + 3 * n, *L = mx + 4 * n, *V0 = mx + 5 * n;
+    double *MU0 = mx + 6 * n, *ETA = mx + 7 * n, *ETAVP = mx + 8 * n, *SY = mx + 9 * n;
+    double *GETA = mx + 10 * n, *GV = mx + 11 * n;
+    for (int k = 0; k < 9; k++) stats[k] = 0;
+    for (long i = 0; i < n; i++) {
+        int t = (int)s->m.t[i];
+        double sxx = s->m.xx[i], sxy = s->m.xy[i], tk = s->m.k[i], p = s->m.p[i];
+        int it = 0;
+        double r = 0;
+        double eta = or_ductilevisc_newton(s, t, sxy, sxx, tk, 1e-5 * p, warm ? GETA[i] : 0.0, tol, &it, &r);
+        int fb = r == -1.0;
+        ETA[i] = eta;
+        stats[0]++;
+        int bad = !isfinite(eta);
+        if (ditr) ditr[i] = it;
+        if (power_law_type(s->mat, t) && t != 1) {
+            stats[1]++; stats[2] += it; if (it > stats[5]) stats[5] = it;
+        }
+        if (ritr) ritr[i] = 0;
+        if (fault[i]) {
+            double exx = s->m.exx[i], exy = s->m.exy[i];
+            double sig = pow(sxx * sxx + sxy * sxy, 0.5);
+            double nP = s->prm.pmode == 0 ? s->prm.nstress : p;
+            double EII = pow(exx * exx + exy * exy, 0.5);
+            double eta_rad = s->prm.inertyn == 0 ? s->prm.rad_damping : 0.0;
+            double W = s->prm.faultwidth, v, phin;
+            int rfb = 0;
+            int rit = or_rsf_marker(sig, nP, A[i], B[i], L[i], V0[i], MU0[i], PHI[i], V[i], dt, eta_rad, EII, W,
+                                    warm ? GV[i] : 0.0, tol, &v, &phin, &rfb);
+            V[i] = v; PHI[i] = phin; SY[i] = sig;
+            ETAVP[i] = sig > 0 ? eta * sig / (2 * eta * v / W + sig) : eta;
+            fb |= rfb;
+            if (!isfinite(v) || !isfinite(phin)) bad = 1;
+            stats[3]++; stats[4] += rit; if (rit > stats[6]) stats[6] = rit;
+            if (ritr) ritr[i] = rit;
+        }
+        stats[7] += fb;
+        stats[8] += bad;
+    }
 }

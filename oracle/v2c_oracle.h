@@ -40,6 +40,13 @@ double or_ductilevisc_newton(const or_state *s, int type, double sxy,
                              double sxx, double tk, double pb, double guess,
                              double tol, int *iters, double *resid);
 
+int or_rsf_marker(double sig, double nP, double a, double b, double L, double V0, double mu0,
+                  double phi, double Vprev, double dt, double eta_rad, double EII, double W,
+                  double guessV, double tol, double *V_out, double *phi_out, int *fallback);
+
+void or_local_update(or_state *s, double dt, const unsigned char *fault, double *mx, int warm,
+                     double tol, long *stats, int *ditr, int *ritr);
+
 void or_marker2nodes(or_state *s);
 void or_nodes2markers(or_state *s);
 long or_moveandrotatemarkers(or_state *s);   /* returns #markers of type >= 100 */

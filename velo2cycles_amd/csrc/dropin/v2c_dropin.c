@@ -1,0 +1,259 @@
+/*
+ * v2c_dropin.c -- reference-side binding of the VELO2CYCLEs hot path to
+ * libv2c.  A maintainer compiles this file into the reference translation unit
+ * in place of the hot-path routines, so every call site stays as it is:
+ *
+ *   marker2nodes()            i2evps.c:89, in2evps.c:1345
+ *   nodes2markers()           i2evps.c:108
+ *   moveandrotatemarkers()    i2evps.c:113
+ *   compute_before_solve()    momentum_continuity_iterate.c:69
+ *   compute_after_solve()     momentum_continuity_iterate.c:89
+ *
+ * The reference globals (headevps.c:33-250, input_RSF.c:3-11) are declared
+ * extern here with the reference's own types; the file owns no state beyond
+ * the device context.  The device owns the marker cloud between
+ * v2c_dropin_attach() (after load_model/extra_init) and
+ * v2c_dropin_markers_to_host() (before save_model and solve_temperature,
+ * which read markers on the host).  Nodal arrays cross PCIe on every call:
+ * inputs are sent, outputs fetched, exactly the arrays each routine reads and
+ * writes.  The boundary-condition tail of marker2nodes
+ * (interpolation.c:918-925) keeps running on the host.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "../../../include/v2c.h"
+
+/* ---- reference globals (declared in headevps.c / input_RSF.c) ------------ */
+extern long int xnumx, ynumy, marknum, nodenum;
+extern double xsize, ysize, xstpx, timesum, timestep, errcheck, nstress, Dphis;
+extern double gx[], gy[];
+extern char markt[];
+extern double markx[], marky[], markk[], markp[], markxx[], markxy[], markexx[], markexy[];
+extern double marke[], markphi[], markvx[], markvy[], markC[], markd[], markw[];
+extern double mark_a[], mark_b[], mark_L[], mark_L0[], mark_V0[], fric_static[], markC0[];
+extern double marknu[], markdh[], markdv[], markss[], markmm[], markgg[], markn0[], markn1[];
+extern double markro[], markbb[], markaa[], markcp[], markkt[], markkf[], markkp[], markht[];
+extern int markim[];
+extern int markmod, stoksmod, outgrid, densimod, interpolationmod, antidiffinter, rkmod, RSFmode;
+extern int power_law_RSF, reloadfromDP, compressiblemode, stickymod, inertyn, pmode, tempmod;
+extern double nubeg, nuend, tdeep, zdeep, eps[];
+extern double half_range, start_updip, end_updip, start_downdip, end_downdip;
+extern double a_vw, b_vw, L_vw, V0_vw, a_vs, b_vs, L_vs, V0_vs, a_hr, b_hr, L_hr, V0_hr;
+extern double RK_sliprs[][2], RK_phidts[][2];
+extern double pr[], vx[], vy[], exx[], exy[], esp[], sxx[], sxy[], sxxs[], sppes[], exxpl[], exxel[];
+extern double bm[], ro[], et[], nu[], nd[], gg[], gd[], sppe[], phis[], phin[], sxyen[], sxxen[];
+extern double exyen[], exxen[], sppen[], mus_s[], as[], bs[], Ls[], V0s[], sxxe[], sxye[], exxe[];
+extern double exye[], cohesion[], dro[], drp[], cp[], kt[], ht[], tk[], mrx[], mry[], mvx[], mvy[];
+extern double pe[], pen[], nu_m2c[], nd_m2c[], mvxn[], mvyn[];
+extern double phie[], sxyee[], exyee[], pee[], sppee[], sxxee[], exxee[], mvxe[], mvye[];
+extern double wtphisum[], phisum[], sxysum[], exysum[], esum[], wtsxxsum[], psum[], sxxsum[];
+extern double exxsum[], wtvxsum[], vxsum[], wtvysum[], vysum[], mrx0[], mry0[];
+extern double pes[], phis_it[], syields[], nuvp_it[], sliprs[], RK_sliprs_final[], RK_phidts_final[];
+extern double RK_sliprs_solved[], RK_phidts_solved[], RK_deltas[];
+void vxvy_boundarycorrection(void);
+void P_boundarycorrection(void);
+void tkrecalc(void);
+
+static v2c_ctx *g_ctx;
+
+static double *nodal_ptr(int f)
+{
+    double *const tab[] = {
+        pr, vx, vy, exx, exy, esp, sxx, sxy, sxxs, sppes, exxpl, exxel,
+        bm, ro, et, nu, nd, gg, gd, sppe, phis, phin, sxyen, sxxen, exyen, exxen, sppen, mus_s, as, bs,
+        Ls, V0s, sxxe, sxye, exxe, exye, cohesion, dro, drp, cp, kt, ht, tk, mrx, mry, mvx, mvy, pe, pen,
+        nu_m2c, nd_m2c, mvxn, mvyn,
+        phie, sxyee, exyee, pee, sppee, sxxee, exxee, mvxe, mvye,
+        wtphisum, phisum, sxysum, exysum, esum, wtsxxsum, psum, sxxsum, exxsum, wtvxsum, vxsum,
+        wtvysum, vysum, mrx0, mry0,
+        pes, phis_it, syields, nuvp_it, sliprs, RK_sliprs_final, RK_phidts_final, RK_sliprs_solved,
+        RK_phidts_solved, RK_deltas, NULL, NULL, NULL, NULL};
+    return f >= 0 && f < V2C_NF ? tab[f] : NULL;
+}
+
+static void die_on(int rc, const char *what)
+{
+    if (rc == 0) return;
+    /* the reference's own failure convention: flag errcheck and report */
+    fprintf(stderr, "libv2c %s failed (%d): %s\n", what, rc, g_ctx ? v2c_last_error(g_ctx) : "");
+    errcheck = 1;
+}
+
+/* RK_sliprs/RK_phidts are [MAXNOD][2] in the reference; the ABI carries the
+ * two columns as separate arrays. */
+static void rk_columns(int to_device)
+{
+    double *tmp = malloc(sizeof(double) * nodenum);
+    const int fid[4] = {V2C_N_RK_sliprs0, V2C_N_RK_sliprs1, V2C_N_RK_phidts0, V2C_N_RK_phidts1};
+    for (int k = 0; k < 4; k++) {
+        double (*src)[2] = k < 2 ? RK_sliprs : RK_phidts;
+        int col = k & 1;
+        if (to_device) {
+            for (long i = 0; i < nodenum; i++) tmp[i] = src[i][col];
+            die_on(v2c_nodal_set(g_ctx, fid[k], tmp), "nodal_set");
+        } else {
+            die_on(v2c_nodal_get(g_ctx, fid[k], tmp), "nodal_get");
+            for (long i = 0; i < nodenum; i++) src[i][col] = tmp[i];
+        }
+    }
+    free(tmp);
+}
+
+static void send(const int *f, int n)
+{
+    for (int i = 0; i < n; i++) die_on(v2c_nodal_set(g_ctx, f[i], nodal_ptr(f[i])), "nodal_set");
+}
+
+static void fetch(const int *f, int n)
+{
+    for (int i = 0; i < n; i++) die_on(v2c_nodal_get(g_ctx, f[i], nodal_ptr(f[i])), "nodal_get");
+}
+
+static v2c_markers marker_view(void)
+{
+    v2c_markers m = {markx, marky, markk, markp, markxx, markxy, markexx, markexy, marke, markphi,
+                     markvx, markvy, markC, markd, markw, mark_a, mark_b, mark_L, mark_L0, mark_V0,
+                     fric_static, markC0, markt};
+    return m;
+}
+
+/* Create the device context from the loaded model and hand it the markers.
+ * Call once after load_model()/extra_init() (i2evps.c:48-50). */
+int v2c_dropin_attach(long capacity, int device)
+{
+    v2c_grid g = {xnumx, ynumy, gx, gy, xsize, ysize};
+    v2c_params p;
+    v2c_params_default(&p);
+    p.markmod = markmod; p.stoksmod = stoksmod; p.outgrid = outgrid; p.densimod = densimod;
+    p.interpolationmod = interpolationmod; p.antidiffinter = antidiffinter; p.rkmod = rkmod;
+    p.RSFmode = RSFmode; p.power_law_RSF = power_law_RSF; p.reloadfromDP = reloadfromDP;
+    p.compressiblemode = compressiblemode; p.stickymod = stickymod; p.inertyn = inertyn;
+    p.pmode = pmode; p.tempmod = tempmod; p.n2m_phantom = 1;
+    p.nubeg = nubeg; p.nuend = nuend; p.tdeep = tdeep; p.zdeep = zdeep; p.incoming_T = eps[2];
+    p.nstress = nstress; p.Dphis = Dphis; p.half_range = half_range;
+    p.start_updip = start_updip; p.end_updip = end_updip;
+    p.start_downdip = start_downdip; p.end_downdip = end_downdip;
+    p.a_vw = a_vw; p.b_vw = b_vw; p.L_vw = L_vw; p.V0_vw = V0_vw;
+    p.a_vs = a_vs; p.b_vs = b_vs; p.L_vs = L_vs; p.V0_vs = V0_vs;
+    p.a_hr = a_hr; p.b_hr = b_hr; p.L_hr = L_hr; p.V0_hr = V0_hr; p.xstpx = xstpx;
+    static v2c_materials mat;
+    mat.ntypes = 200;
+    for (int t = 0; t < 200; t++) {
+        mat.nu[t] = marknu[t]; mat.dh[t] = markdh[t]; mat.dv[t] = markdv[t]; mat.ss[t] = markss[t];
+        mat.mm[t] = markmm[t]; mat.gg[t] = markgg[t]; mat.n0[t] = markn0[t]; mat.n1[t] = markn1[t];
+        mat.ro[t] = markro[t]; mat.bb[t] = markbb[t]; mat.aa[t] = markaa[t]; mat.cp[t] = markcp[t];
+        mat.kt[t] = markkt[t]; mat.kf[t] = markkf[t]; mat.kp[t] = markkp[t]; mat.ht[t] = markht[t];
+        mat.im[t] = markim[t];
+    }
+    int rc = v2c_create(&g_ctx, &g, &p, &mat, capacity, device);
+    if (rc) return rc;
+    v2c_markers m = marker_view();
+    return v2c_markers_upload(g_ctx, &m, marknum);
+}
+
+void v2c_dropin_markers_to_host(void)
+{
+    v2c_markers m = marker_view();
+    die_on(v2c_markers_download(g_ctx, &m, marknum), "markers_download");
+}
+
+void v2c_dropin_markers_to_device(void)
+{
+    v2c_markers m = marker_view();
+    die_on(v2c_markers_upload(g_ctx, &m, marknum), "markers_upload");
+}
+
+void v2c_dropin_detach(void)
+{
+    v2c_destroy(g_ctx);
+    g_ctx = NULL;
+}
+
+/* ---- the reference entry points ---------------------------------------- */
+
+void marker2nodes(void)
+{
+    static const int in[] = {V2C_N_pr, V2C_N_phis_it, V2C_N_sxy, V2C_N_exy, V2C_N_pes, V2C_N_sxx,
+                             V2C_N_exx, V2C_N_vx, V2C_N_vy, V2C_N_phie, V2C_N_sxyee, V2C_N_exyee,
+                             V2C_N_pee, V2C_N_sxxee, V2C_N_sppee, V2C_N_exxee, V2C_N_mvxe, V2C_N_mvye,
+                             /* written only where markers land: keep the host's values elsewhere */
+                             V2C_N_pe, V2C_N_pen, V2C_N_nu_m2c, V2C_N_nd_m2c, V2C_N_mvxn, V2C_N_mvyn};
+    static const int out[] = {V2C_N_bm, V2C_N_ro, V2C_N_et, V2C_N_nu, V2C_N_nd, V2C_N_gg, V2C_N_gd,
+                              V2C_N_sppe, V2C_N_phis, V2C_N_phin, V2C_N_sxyen, V2C_N_sxxen, V2C_N_exyen,
+                              V2C_N_exxen, V2C_N_sppen, V2C_N_mus_s, V2C_N_as, V2C_N_bs, V2C_N_Ls,
+                              V2C_N_V0s, V2C_N_sxxe, V2C_N_sxye, V2C_N_exxe, V2C_N_exye,
+                              V2C_N_cohesion, V2C_N_dro, V2C_N_drp, V2C_N_cp, V2C_N_kt, V2C_N_ht,
+                              V2C_N_tk, V2C_N_mrx, V2C_N_mry, V2C_N_mvx, V2C_N_mvy, V2C_N_pe,
+                              V2C_N_pen, V2C_N_nu_m2c, V2C_N_nd_m2c, V2C_N_mvxn, V2C_N_mvyn,
+                              V2C_N_mrx0, V2C_N_mry0};
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, (int)(sizeof in / sizeof in[0]));
+    die_on(v2c_marker2nodes(g_ctx), "marker2nodes");
+    fetch(out, (int)(sizeof out / sizeof out[0]));
+    /* host tail of the reference routine (interpolation.c:918-925) */
+    vxvy_boundarycorrection();
+    P_boundarycorrection();
+    if (tempmod) tkrecalc();
+}
+
+double nodes2markers(void)
+{
+    static const int in[] = {V2C_N_exy, V2C_N_exye, V2C_N_sxy, V2C_N_sxye, V2C_N_pes, V2C_N_pe,
+                             V2C_N_esp, V2C_N_phis, V2C_N_phis_it, V2C_N_exx, V2C_N_exxe, V2C_N_sxx,
+                             V2C_N_sxxe, V2C_N_pr, V2C_N_sppe, V2C_N_vx, V2C_N_mvx, V2C_N_vy, V2C_N_mvy,
+                             V2C_N_phie, V2C_N_sxyee, V2C_N_exyee, V2C_N_pee, V2C_N_sppee, V2C_N_sxxee,
+                             V2C_N_exxee, V2C_N_mvxe, V2C_N_mvye};
+    static const int out[] = {V2C_N_phie, V2C_N_sxyee, V2C_N_exyee, V2C_N_pee, V2C_N_sppee,
+                              V2C_N_sxxee, V2C_N_exxee, V2C_N_mvxe, V2C_N_mvye};
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, (int)(sizeof in / sizeof in[0]));
+    die_on(v2c_nodes2markers(g_ctx), "nodes2markers");
+    fetch(out, (int)(sizeof out / sizeof out[0]));
+    return 0;
+}
+
+void moveandrotatemarkers(void)
+{
+    static const int in[] = {V2C_N_vx, V2C_N_vy, V2C_N_exx, V2C_N_exy, V2C_N_esp};
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, (int)(sizeof in / sizeof in[0]));
+    die_on(v2c_moveandrotatemarkers(g_ctx), "moveandrotatemarkers");
+}
+
+double compute_before_solve(double dt, int rk)
+{
+    static const int in[] = {V2C_N_sxxs, V2C_N_sxy, V2C_N_sppes, V2C_N_nu_m2c, V2C_N_RK_sliprs_final,
+                             V2C_N_RK_phidts_final, V2C_N_RK_sliprs_solved, V2C_N_RK_phidts_solved,
+                             V2C_N_pe, V2C_N_phis, V2C_N_as, V2C_N_bs, V2C_N_Ls, V2C_N_V0s,
+                             V2C_N_mus_s, V2C_N_cohesion, V2C_N_nu, V2C_N_pes, V2C_N_phis_it,
+                             V2C_N_syields, V2C_N_nuvp_it, V2C_N_sliprs};
+    static const int out[] = {V2C_N_nu, V2C_N_pes, V2C_N_phis_it, V2C_N_syields, V2C_N_nuvp_it,
+                              V2C_N_sliprs};
+    long nerr = 0;
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, (int)(sizeof in / sizeof in[0]));
+    if (rkmod == 3) rk_columns(1);
+    die_on(v2c_compute_before_solve(g_ctx, dt, rk, &nerr), "compute_before_solve");
+    fetch(out, (int)(sizeof out / sizeof out[0]));
+    if (rkmod == 3) rk_columns(0);
+    if (nerr) errcheck = 1;
+    return 0;
+}
+
+double compute_after_solve(double dt, int rk)
+{
+    static const int in[] = {V2C_N_sxxs, V2C_N_sxy, V2C_N_sxye, V2C_N_gg, V2C_N_exy, V2C_N_exxpl,
+                             V2C_N_sppes, V2C_N_nu, V2C_N_nu_m2c, V2C_N_syields, V2C_N_phis_it,
+                             V2C_N_as, V2C_N_bs, V2C_N_Ls, V2C_N_V0s, V2C_N_mus_s,
+                             V2C_N_RK_sliprs_solved, V2C_N_RK_phidts_solved, V2C_N_RK_deltas};
+    static const int out[] = {V2C_N_RK_sliprs_solved, V2C_N_RK_phidts_solved, V2C_N_RK_deltas};
+    double deltas = 0;
+    long nerr = 0;
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, (int)(sizeof in / sizeof in[0]));
+    die_on(v2c_compute_after_solve(g_ctx, dt, rk, &deltas, &nerr), "compute_after_solve");
+    fetch(out, (int)(sizeof out / sizeof out[0]));
+    if (nerr) errcheck = 1;
+    return deltas;
+}

@@ -5,6 +5,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "v2c_kernels.hip"
@@ -26,6 +28,8 @@ struct v2c_ctx {
     double timesum = 0, timestep = 0;
     int key_bits = 1;
     bool order_valid = false;
+    bool ever_sorted = false;
+    long last_movers = 0, n_full_sorts = 0, n_incremental = 0;
     // device memory
     double *d_gx = nullptr, *d_gy = nullptr;
     DevMat *d_mat = nullptr;
@@ -34,6 +38,15 @@ struct v2c_ctx {
     uint32_t *d_key = nullptr, *d_skey = nullptr, *d_iota = nullptr, *d_order = nullptr, *d_cs = nullptr;
     double *d_nod = nullptr, *d_part = nullptr, *d_red = nullptr;
     int *d_counters = nullptr;
+    // incremental re-ordering buffers
+    uint32_t *d_order2 = nullptr, *d_skey2 = nullptr, *d_ids_mv = nullptr, *d_ids_st = nullptr;
+    uint8_t *d_flag_mv = nullptr, *d_flag_st = nullptr;
+    uint64_t *d_k64_in = nullptr, *d_k64_out = nullptr;
+    long *d_nsel = nullptr;
+    // local update
+    double *d_mx = nullptr;
+    uint8_t *d_fault = nullptr;
+    unsigned long long *d_lu = nullptr;
     void *d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     // timing
@@ -84,13 +97,15 @@ static int launch(v2c_ctx *c, int kid, F &&fn)
     if (c->timing) {
         tl.start = take_event(c);
         tl.stop = take_event(c);
-        if (tl.start) hipEventRecord(tl.start, c->stream);
+        if (tl.start) (void)hipEventRecord(tl.start, c->stream);
     }
-    fn();
-    hipError_t e = hipGetLastError();
+    hipError_t e = hipSuccess;
+    if constexpr (std::is_same_v<decltype(fn()), hipError_t>) e = fn();
+    else fn();
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return set_err(c, V2C_EHIP, "kernel %d launch failed: %s", kid, hipGetErrorString(e));
     if (c->timing && tl.start && tl.stop) {
-        hipEventRecord(tl.stop, c->stream);
+        (void)hipEventRecord(tl.stop, c->stream);
         c->pending.push_back(tl);
     }
     return 0;
@@ -102,7 +117,7 @@ static int collect_timing(v2c_ctx *c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (auto &tl : c->pending) {
         float ms = 0;
-        hipEventElapsedTime(&ms, tl.start, tl.stop);
+        (void)hipEventElapsedTime(&ms, tl.start, tl.stop);
         c->ktime[tl.kernel] += ms;
         c->klaunch[tl.kernel] += 1;
         c->event_pool.push_back(tl.start);
@@ -127,6 +142,9 @@ static KArgs make_args(v2c_ctx *c)
     a.timesum = c->timesum; a.timestep = c->timestep;
     a.counters = c->d_counters;
     a.write_friction = c->write_friction;
+    a.mx = c->d_mx;
+    a.fault = c->d_fault;
+    a.lu_stats = c->d_lu;
     return a;
 }
 
@@ -138,25 +156,90 @@ static int finish(v2c_ctx *c)
 
 static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// Stable sort of (sub-cell key, marker id) -> order[], then cell_start[].
-static int ensure_order(v2c_ctx *c)
+static int cell_starts(v2c_ctx *c)
 {
-    if (c->order_valid) return 0;
-    KArgs a = make_args(c);
-    int rc;
-    if (c->n > 0) {
-        rc = launch(c, V2C_K_SORT, [&] {
-            hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp, c->sort_tmp_bytes, c->d_key, c->d_skey,
-                                               c->d_iota, c->d_order, (int)c->n, 0, c->key_bits, c->stream);
-        });
-        if (rc) return rc;
-    }
-    rc = launch(c, V2C_K_SORT, [&] {
+    return launch(c, V2C_K_SORT, [&] {
         hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(c->n + 1, 256)), dim3(256), 0, c->stream,
                            c->d_skey, c->n, c->ncell, c->d_cs);
     });
+}
+
+// Full stable radix sort of (sub-cell key, marker id) -> order[], skey[].
+static int full_sort(v2c_ctx *c)
+{
+    if (c->n > 0) {
+        int rc = launch(c, V2C_K_SORT, [&] {
+            return hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp, c->sort_tmp_bytes, c->d_key, c->d_skey,
+                                                      c->d_iota, c->d_order, (int)c->n, 0, c->key_bits,
+                                                      c->stream);
+        });
+        if (rc) return rc;
+    }
+    c->n_full_sorts++;
+    c->ever_sorted = true;
+    return cell_starts(c);
+}
+
+// Bring order[]/skey[]/cell_start[] up to date with the current keys.  After
+// an advection only markers whose key changed are re-sorted and merged in
+// (see k_mover_flags); the result is the same (key, id) order a full sort
+// gives.
+static int ensure_order(v2c_ctx *c)
+{
+    if (c->order_valid) return 0;
+    int rc;
+    if (!c->ever_sorted || c->n == 0) {
+        rc = full_sort(c);
+        if (rc) return rc;
+        c->last_movers = c->n;
+        c->order_valid = true;
+        return 0;
+    }
+    const int n = (int)c->n;
+    rc = launch(c, V2C_K_SORT, [&] {
+        hipLaunchKernelGGL(k_mover_flags, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->d_order,
+                           c->d_key, c->d_skey, (long)n, c->d_flag_mv, c->d_flag_st);
+        return hipcub::DeviceSelect::Flagged(c->d_sort_tmp, c->sort_tmp_bytes, c->d_order, c->d_flag_mv,
+                                             c->d_ids_mv, c->d_nsel, n, c->stream);
+    });
     if (rc) return rc;
-    (void)a;
+    long nmov = 0;
+    HIPCHK(c, hipMemcpyAsync(&nmov, c->d_nsel, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->last_movers = nmov;
+    if (nmov == 0) {
+        c->order_valid = true;
+        return 0;
+    }
+    if (nmov > n / 4) {
+        rc = full_sort(c);
+        if (rc) return rc;
+        c->order_valid = true;
+        return 0;
+    }
+    const long nstay = n - nmov;
+    rc = launch(c, V2C_K_SORT, [&] {
+        hipError_t e = hipcub::DeviceSelect::Flagged(c->d_sort_tmp, c->sort_tmp_bytes, c->d_order, c->d_flag_st,
+                                                     c->d_ids_st, c->d_nsel + 1, n, c->stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_mover_keys, dim3(blocks_for(nmov, 256)), dim3(256), 0, c->stream, c->d_ids_mv,
+                           c->d_nsel, c->d_key, c->d_k64_in);
+        e = hipcub::DeviceRadixSort::SortKeys(c->d_sort_tmp, c->sort_tmp_bytes, c->d_k64_in, c->d_k64_out,
+                                              (int)nmov, 0, 32 + c->key_bits, c->stream);
+        if (e != hipSuccess) return e;
+        if (nstay > 0)
+            hipLaunchKernelGGL(k_merge_stayers, dim3(blocks_for(nstay, 256)), dim3(256), 0, c->stream,
+                               c->d_ids_st, nstay, c->d_k64_out, nmov, c->d_key, c->d_order2, c->d_skey2);
+        hipLaunchKernelGGL(k_merge_movers, dim3(blocks_for(nmov, 256)), dim3(256), 0, c->stream, c->d_k64_out,
+                           nmov, c->d_ids_st, nstay, c->d_key, c->d_order2, c->d_skey2);
+        return hipSuccess;
+    });
+    if (rc) return rc;
+    std::swap(c->d_order, c->d_order2);
+    std::swap(c->d_skey, c->d_skey2);
+    c->n_incremental++;
+    rc = cell_starts(c);
+    if (rc) return rc;
     c->order_valid = true;
     return 0;
 }
@@ -205,12 +288,14 @@ const char *v2c_last_error(const v2c_ctx *c) { return c ? c->err : "null context
 static void free_all(v2c_ctx *c)
 {
     void *ptrs[] = {c->d_gx, c->d_gy, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
-                    c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_red, c->d_counters, c->d_sort_tmp};
+                    c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_red, c->d_counters, c->d_sort_tmp,
+                    c->d_order2, c->d_skey2, c->d_ids_mv, c->d_ids_st, c->d_flag_mv, c->d_flag_st,
+                    c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu};
     for (void *p : ptrs)
-        if (p) hipFree(p);
-    for (auto &tl : c->pending) { hipEventDestroy(tl.start); hipEventDestroy(tl.stop); }
-    for (auto e : c->event_pool) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
 int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2c_materials *mat,
@@ -265,6 +350,20 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_part, sizeof(double) * MP_N * c->ncell));
     CK(hipMalloc(&c->d_red, sizeof(double) * (2 * blocks_for(c->nodenum, RED_BLOCK) + 8)));
     CK(hipMalloc(&c->d_counters, sizeof(int) * 8));
+    CK(hipMalloc(&c->d_order2, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_skey2, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_ids_mv, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_ids_st, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_flag_mv, c->cap));
+    CK(hipMalloc(&c->d_flag_st, c->cap));
+    CK(hipMalloc(&c->d_k64_in, sizeof(uint64_t) * c->cap));
+    CK(hipMalloc(&c->d_k64_out, sizeof(uint64_t) * c->cap));
+    CK(hipMalloc(&c->d_nsel, sizeof(long) * 2));
+    CK(hipMalloc(&c->d_mx, sizeof(double) * NMX * c->cap));
+    CK(hipMalloc(&c->d_fault, c->cap));
+    CK(hipMalloc(&c->d_lu, sizeof(unsigned long long) * LS_COUNT));
+    CK(hipMemsetAsync(c->d_mx, 0, sizeof(double) * NMX * c->cap, c->stream));
+    CK(hipMemsetAsync(c->d_fault, 0, c->cap, c->stream));
     CK(hipMemsetAsync(c->d_nod, 0, sizeof(double) * V2C_NF * c->nodenum, c->stream));
     CK(hipMemsetAsync(c->d_mk, 0, sizeof(double) * NMF * c->cap, c->stream));
     CK(hipMemsetAsync(c->d_t, 0, c->cap, c->stream));
@@ -287,8 +386,16 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     }
     hipLaunchKernelGGL(k_iota, dim3(blocks_for(c->cap, 256)), dim3(256), 0, c->stream, c->d_iota, c->cap);
     CK(hipGetLastError());
-    CK(hipcub::DeviceRadixSort::SortPairs(nullptr, c->sort_tmp_bytes, c->d_key, c->d_skey, c->d_iota,
-                                          c->d_order, (int)c->cap, 0, c->key_bits, c->stream));
+    {
+        size_t b1 = 0, b2 = 0, b3 = 0;
+        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c->d_key, c->d_skey, c->d_iota, c->d_order,
+                                              (int)c->cap, 0, c->key_bits, c->stream));
+        CK(hipcub::DeviceSelect::Flagged(nullptr, b2, c->d_order, c->d_flag_mv, c->d_ids_mv, c->d_nsel,
+                                         (int)c->cap, c->stream));
+        CK(hipcub::DeviceRadixSort::SortKeys(nullptr, b3, c->d_k64_in, c->d_k64_out, (int)c->cap, 0,
+                                             32 + c->key_bits, c->stream));
+        c->sort_tmp_bytes = std::max(b1, std::max(b2, b3));
+    }
     CK(hipMalloc(&c->d_sort_tmp, c->sort_tmp_bytes > 0 ? c->sort_tmp_bytes : 16));
     CK(hipStreamSynchronize(c->stream));
 #undef CK
@@ -299,8 +406,8 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
 void v2c_destroy(v2c_ctx *c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_all(c);
     delete c;
 }
@@ -361,7 +468,17 @@ int v2c_markers_upload(v2c_ctx *c, const v2c_markers *m, long n)
         HIPCHK(c, hipGetLastError());
     }
     c->order_valid = false;
+    c->ever_sorted = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_order_stats(v2c_ctx *c, long *last_movers, long *full_sorts, long *incremental)
+{
+    if (!c) return V2C_EINVAL;
+    if (last_movers) *last_movers = c->last_movers;
+    if (full_sorts) *full_sorts = c->n_full_sorts;
+    if (incremental) *incremental = c->n_incremental;
     return 0;
 }
 
@@ -559,20 +676,99 @@ int v2c_ductilevisc_batch(v2c_ctx *c, long n, const char *type, const double *sx
     double *d_sxy = (double *)buf, *d_sxx = d_sxy + n, *d_tk = d_sxx + n, *d_pb = d_tk + n, *d_out = d_pb + n;
     int *d_it = (int *)(d_out + n);
     int8_t *d_t = (int8_t *)(d_it + n);
-    hipMemcpyAsync(d_sxy, sxy, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(d_sxx, sxx, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(d_tk, tk, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(d_pb, pb, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(d_t, type, n, hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipMemcpyAsync(d_sxy, sxy, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_sxx, sxx, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tk, tk, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pb, pb, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_t, type, n, hipMemcpyHostToDevice, c->stream);
     KArgs a = make_args(c);
-    hipLaunchKernelGGL(k_ductile_batch, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a, n, d_t, d_sxy,
-                       d_sxx, d_tk, d_pb, d_out, d_it);
-    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ductile_batch, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a, n, d_t, d_sxy,
+                           d_sxx, d_tk, d_pb, d_out, d_it);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(nu_out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && iters) e = hipMemcpyAsync(iters, d_it, n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(buf);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
     if (e != hipSuccess) return set_err(c, V2C_EHIP, "ductilevisc_batch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int v2c_rsf_upload(v2c_ctx *c, const unsigned char *fault, const double *V, const double *phi,
+                   const double *a, const double *b, const double *L, const double *V0,
+                   const double *mu0, long n)
+{
+    if (!c || n < 0 || n > c->cap) return V2C_EINVAL;
+    if (!fault || !V || !phi || !a || !b || !L || !V0 || !mu0) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const double *src[7] = {V, phi, a, b, L, V0, mu0};
+    const int dst[7] = {MX_V, MX_PHI, MX_A, MX_B, MX_L, MX_V0, MX_MU0};
+    for (int k = 0; k < 7; k++)
+        HIPCHK(c, hipMemcpyAsync(c->d_mx + (size_t)dst[k] * c->cap, src[k], sizeof(double) * n,
+                                 hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_fault, fault, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_local_set_guess(v2c_ctx *c, const double *eta, const double *V, long n)
+{
+    if (!c || n < 0 || n > c->cap) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (eta) HIPCHK(c, hipMemcpyAsync(c->d_mx + (size_t)MX_GETA * c->cap, eta, sizeof(double) * n,
+                                      hipMemcpyHostToDevice, c->stream));
+    if (V) HIPCHK(c, hipMemcpyAsync(c->d_mx + (size_t)MX_GV * c->cap, V, sizeof(double) * n,
+                                    hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_local_update(v2c_ctx *c, double dt, int warm, double tol, v2c_lu_stats *st)
+{
+    if (!c || !(tol > 0)) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemsetAsync(c->d_lu, 0, sizeof(unsigned long long) * LS_COUNT, c->stream));
+    KArgs a = make_args(c);
+    int rc = 0;
+    if (c->n > 0) {
+        rc = launch(c, V2C_K_LOCAL, [&] {
+            hipLaunchKernelGGL(k_local_update, dim3(blocks_for(c->n, LU_BLOCK)), dim3(LU_BLOCK), 0, c->stream, a,
+                               dt, warm, tol);
+        });
+        if (rc) return rc;
+    }
+    rc = finish(c);
+    if (rc) return rc;
+    if (st) {
+        unsigned long long h[LS_COUNT];
+        HIPCHK(c, hipMemcpyAsync(h, c->d_lu, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        st->markers = (long)h[LS_N]; st->ductile_markers = (long)h[LS_DUCT];
+        st->ductile_iters = (long)h[LS_DUCT_IT]; st->fault_markers = (long)h[LS_FAULT];
+        st->rsf_iters = (long)h[LS_RSF_IT]; st->max_ductile_iters = (long)h[LS_DUCT_MAX];
+        st->max_rsf_iters = (long)h[LS_RSF_MAX]; st->fallbacks = (long)h[LS_FALLBACK];
+        st->nonfinite = (long)h[LS_NONFINITE];
+        for (int k = 0; k < LU_HIST; k++) {
+            st->hist_ductile[k] = (long)h[LS_HIST_D + k];
+            st->hist_rsf[k] = (long)h[LS_HIST_R + k];
+        }
+    }
+    return 0;
+}
+
+int v2c_local_download(v2c_ctx *c, double *eta, double *V, double *phi, double *eta_vp, double *syield, long n)
+{
+    if (!c || n < 0 || n > c->n) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    double *dst[5] = {eta, V, phi, eta_vp, syield};
+    const int src[5] = {MX_ETA, MX_V, MX_PHI, MX_ETAVP, MX_SY};
+    for (int k = 0; k < 5; k++)
+        if (dst[k])
+            HIPCHK(c, hipMemcpyAsync(dst[k], c->d_mx + (size_t)src[k] * c->cap, sizeof(double) * n,
+                                     hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

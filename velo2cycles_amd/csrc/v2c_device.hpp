@@ -19,6 +19,16 @@ enum MF {
     MF_C, MF_D, MF_W, MF_A, MF_B, MF_L, MF_L0, MF_V0, MF_FS, MF_C0, NMF
 };
 
+// Extension fields of the per-marker local update (not reference state): the
+// fault marker's own RSF parameters and state, the solver outputs and the
+// warm-start guesses.
+enum MX {
+    MX_V = 0, MX_PHI, MX_A, MX_B, MX_L, MX_V0, MX_MU0,       // RSF inputs / state
+    MX_ETA, MX_ETAVP, MX_SY,                                  // outputs
+    MX_GETA, MX_GV,                                           // warm-start guesses
+    NMX
+};
+
 struct DevMat {
     double nu[V2C_MAX_TYPES], dh[V2C_MAX_TYPES], dv[V2C_MAX_TYPES], ss[V2C_MAX_TYPES];
     double mm[V2C_MAX_TYPES], gg[V2C_MAX_TYPES], n0[V2C_MAX_TYPES], n1[V2C_MAX_TYPES];
@@ -52,10 +62,14 @@ struct KArgs {
     double timesum, timestep;
     int *counters;         // device counters (errors, unsupported types)
     int write_friction;    // write the calc_frictionpara marker outputs
+    double *mx;            // NMX extension fields x cap
+    const uint8_t *fault;  // per-marker fault flag
+    unsigned long long *lu_stats;  // local-update statistics
 };
 
 __device__ __forceinline__ double *NOD(const KArgs &a, int f) { return a.nod + (long)f * a.nodenum; }
 __device__ __forceinline__ double &MK(const KArgs &a, int f, long i) { return a.mk[(long)f * a.cap + i]; }
+__device__ __forceinline__ double &MXF(const KArgs &a, int f, long i) { return a.mx[(long)f * a.cap + i]; }
 
 // ---- small helpers ---------------------------------------------------------
 __device__ __forceinline__ double ref_abs(double a) { return a >= 0 ? a : -a; }
@@ -395,6 +409,111 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
     VX = vxa + bil(vx, s.m3, ynumy, wts(s)) * xrat;
     s = vy_stencil(g, nx, ny, x, y, true);
     VY = vya + bil(vy, s.m3, ynumy, wts(s)) * xrat;
+}
+
+// ---- build-defined per-marker local update (not reference behaviour) -------
+
+// Ductile viscosity by Newton on the reference's residual
+// (density_ductilevisc_calc.c:188-193): f(eta) = eta - coeff*X^((n-1)/n),
+// X = (eta_d+eta)/eta_d*G dt + eta, exact f'(eta), to |f|/eta <= tol.
+// guess <= 0 starts from the reference's start eta_disl0; a warm start whose
+// residual grows after its first step restarts from eta_disl0 (*fallback = 1).
+__device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double sxy, double sxx,
+                                                 double mtk, double mpb, double guess, double tol,
+                                                 int *iters, int *fallback)
+{
+    const DevMat *M = a.mat;
+    *iters = 0;
+    *fallback = 0;
+    if (!power_law_type(M, t) || t == 1) {
+        int it;
+        return ductilevisc(a, t, sxy, sxx, mtk, mpb, &it);
+    }
+    const double n = M->mm[t];
+    const double sigin = pow(sxx * sxx + sxy * sxy, 0.5);
+    double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * mtk);
+    if (e1 > 150.0) e1 = 150.0;
+    e1 = exp(e1);
+    const double k1 = M->nu[t] / pow(M->ss[t], n - 1.0);
+    const double eta_diff = 0.5 * k1 * e1;
+    if (!(sigin > 0)) return clamp_visc(a, t, eta_diff);
+    const double eta_disl0 = 0.5 * sigin * M->nu[t] * e1 / pow(sigin, n);
+    const double coeff = pow(eta_disl0, 1.0 / n);
+    const double gdt = M->gg[t] * a.timestep;
+    const double p1 = (n - 1) / n, slope = gdt / eta_diff + 1;
+    double eta = guess > 0 ? guess : eta_disl0;
+    bool restarted = !(guess > 0);
+    double rprev = -1;
+    int k = 0;
+    for (; k < 200; k++) {
+        const double X = (eta_diff + eta) / eta_diff * gdt + eta;
+        const double Xp = pow(X, p1);
+        const double f = eta - coeff * Xp;
+        const double r = fabs(f) / eta;
+        if (r <= tol) break;
+        if (!restarted && k == 1 && r > rprev) {
+            eta = eta_disl0;
+            restarted = true;
+            *fallback = 1;
+            continue;
+        }
+        rprev = r;
+        const double df = 1 - p1 * coeff * (Xp / X) * slope;    // X^(-1/n) = X^((n-1)/n) / X
+        if (f < 0 && !(df > 0)) { eta *= 2.0; continue; }
+        double next = eta - f / df;
+        if (!(next > 0)) next = 0.5 * eta;
+        eta = next;
+    }
+    *iters = k;
+    return clamp_visc(a, t, 1.0 / (1.0 / eta_diff + 1.0 / eta));
+}
+
+// Per-marker RSF slip rate with radiation damping at the aged state
+// (aging law calc_RSF.c:96-103; yield law calc_RSF.c:129 solved for V by
+// Newton in s = ln V; cap and floor calc_RSF.c:193-196).
+__device__ __forceinline__ int rsf_newton(double sig, double nP, double ra, double rb, double rL, double rV0,
+                                          double mu0, double phi, double Vprev, double dt, double eta_rad,
+                                          double EII, double W, double guessV, double tol, double &V_out,
+                                          double &phi_out, int &fallback)
+{
+    double phin;
+    if (Vprev * dt / rL <= 1e-6) phin = log(exp(phi) * (1 - Vprev * dt / rL) + rV0 * dt / rL);
+    else phin = log(rV0 / Vprev + (exp(phi) - rV0 / Vprev) * exp(-Vprev * dt / rL));
+    phi_out = phin;
+    fallback = 0;
+    if (!(sig > 0)) { V_out = 0; return 0; }
+    const double K = exp((mu0 + rb * phin) / ra) / (2 * rV0);
+    const double Vu = 2 * rV0 * sinh(sig / (ra * nP)) * exp(-(mu0 + rb * phin) / ra);
+    double Vhi = Vu;
+    if (eta_rad > 0 && sig / eta_rad < Vhi) Vhi = sig / eta_rad;
+    double V;
+    int k = 0;
+    if (eta_rad == 0 || !(Vhi > 0) || !isfinite(Vhi)) {
+        V = Vhi;
+    } else {
+        const double shi = log(Vhi);
+        bool warm = guessV > 0;
+        double s = warm ? log(guessV) : shi;
+        if (s > shi) s = shi;
+        double gprev = -1;
+        for (; k < 200; k++) {
+            const double e = exp(s), z = e * K;
+            const double g = nP * ra * asinh(z) + eta_rad * e - sig;
+            const double ag = fabs(g) / sig;
+            if (ag <= tol) break;
+            if (warm && k == 1 && ag > gprev) { s = shi; warm = false; fallback = 1; continue; }
+            gprev = ag;
+            const double dg = nP * ra * z / sqrt(1 + z * z) + eta_rad * e;
+            double sn = s - g / dg;
+            if (!(sn <= shi)) sn = shi;
+            s = sn;
+        }
+        V = exp(s);
+    }
+    V = ref_minv(V, EII * W);
+    if (V < 1e-40) V = 1e-40;
+    V_out = V;
+    return k;
 }
 
 }  // namespace v2c

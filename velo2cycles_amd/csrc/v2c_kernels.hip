@@ -76,6 +76,68 @@ __global__ void k_cell_start(const uint32_t *__restrict__ skey, long n, long nce
 }
 
 // ---------------------------------------------------------------------------
+// Incremental re-ordering after advection.  The order is always the (key, id)
+// lexicographic order (a stable radix sort of keys over ids produces exactly
+// that).  Markers whose key did not change keep that order among themselves,
+// so the new order is the merge of the unchanged markers (already sorted) with
+// the few movers sorted by (key, id): every element's output position is its
+// own rank plus the number of elements of the other list that precede it.
+// ---------------------------------------------------------------------------
+__global__ void k_mover_flags(const uint32_t *__restrict__ order, const uint32_t *__restrict__ key,
+                              const uint32_t *__restrict__ skey, long n, uint8_t *mv, uint8_t *stay)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool moved = key[order[i]] != skey[i];
+    mv[i] = moved ? 1 : 0;
+    stay[i] = moved ? 0 : 1;
+}
+
+__device__ __forceinline__ uint64_t kid(uint32_t k, uint32_t id) { return ((uint64_t)k << 32) | id; }
+
+__global__ void k_mover_keys(const uint32_t *__restrict__ ids, const long *__restrict__ nsel,
+                             const uint32_t *__restrict__ key, uint64_t *out)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= *nsel) return;
+    out[r] = kid(key[ids[r]], ids[r]);
+}
+
+__global__ void k_merge_stayers(const uint32_t *__restrict__ stay, long nstay, const uint64_t *__restrict__ mov,
+                                long nmov, const uint32_t *__restrict__ key, uint32_t *order_out,
+                                uint32_t *skey_out)
+{
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nstay) return;
+    const uint32_t id = stay[j], k = key[id];
+    const uint64_t c = kid(k, id);
+    long lo = 0, hi = nmov;
+    while (lo < hi) {
+        long mid = (lo + hi) >> 1;
+        if (mov[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    order_out[j + lo] = id;
+    skey_out[j + lo] = k;
+}
+
+__global__ void k_merge_movers(const uint64_t *__restrict__ mov, long nmov, const uint32_t *__restrict__ stay,
+                               long nstay, const uint32_t *__restrict__ key, uint32_t *order_out,
+                               uint32_t *skey_out)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nmov) return;
+    const uint64_t c = mov[r];
+    long lo = 0, hi = nstay;
+    while (lo < hi) {
+        long mid = (lo + hi) >> 1;
+        const uint32_t sid = stay[mid];
+        if (kid(key[sid], sid) < c) lo = mid + 1; else hi = mid;
+    }
+    order_out[r + lo] = (uint32_t)(c & 0xffffffffu);
+    skey_out[r + lo] = (uint32_t)(c >> 32);
+}
+
+// ---------------------------------------------------------------------------
 // Segmented reduction helper: lanes hold NF contributions in con[f][lane] and
 // a local slot id in skey[lane] (runs of equal slot ids are contiguous); the
 // head lane of each run adds the run total to acc[slot][f].
@@ -933,6 +995,72 @@ __global__ void k_ductile_batch(KArgs a, long n, const int8_t *t, const double *
     int it;
     out[i] = ductilevisc(a, t[i], sxy[i], sxx[i], tk[i], pb[i], &it);
     iters[i] = it;
+}
+
+// ---------------------------------------------------------------------------
+// Build-defined per-marker local update: ductile Newton to |f|/eta <= tol for
+// every marker, RSF slip-rate Newton for fault markers.  Statistics go to
+// lu_stats with integer atomics (order independent).
+// ---------------------------------------------------------------------------
+constexpr int LU_BLOCK = 256;
+constexpr int LU_HIST = 32;
+enum { LS_N = 0, LS_DUCT, LS_DUCT_IT, LS_FAULT, LS_RSF_IT, LS_DUCT_MAX, LS_RSF_MAX, LS_FALLBACK,
+       LS_NONFINITE, LS_HIST_D, LS_HIST_R = LS_HIST_D + LU_HIST, LS_COUNT = LS_HIST_R + LU_HIST };
+
+__global__ __launch_bounds__(LU_BLOCK) void k_local_update(KArgs a, double dt, int warm, double tol)
+{
+    __shared__ unsigned long long sh[LS_COUNT];
+    for (int i = threadIdx.x; i < LS_COUNT; i += LU_BLOCK) sh[i] = 0;
+    __syncthreads();
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < a.n) {
+        const int t = a.t[m];
+        const double sxx = MK(a, MF_XX, m), sxy = MK(a, MF_XY, m);
+        const double tk = MK(a, MF_K, m), p = MK(a, MF_P, m);
+        int it = 0, fb = 0;
+        const double eta = ductile_newton(a, t, sxy, sxx, tk, 1e-5 * p, warm ? MXF(a, MX_GETA, m) : 0.0, tol,
+                                          &it, &fb);
+        MXF(a, MX_ETA, m) = eta;
+        atomicAdd(&sh[LS_N], 1ull);
+        unsigned long long bad = isfinite(eta) ? 0 : 1;
+        if (power_law_type(a.mat, t) && t != 1) {
+            atomicAdd(&sh[LS_DUCT], 1ull);
+            atomicAdd(&sh[LS_DUCT_IT], (unsigned long long)it);
+            atomicMax(&sh[LS_DUCT_MAX], (unsigned long long)it);
+            atomicAdd(&sh[LS_HIST_D + (it < LU_HIST ? it : LU_HIST - 1)], 1ull);
+        }
+        if (a.fault[m]) {
+            const double exx = MK(a, MF_EXX, m), exy = MK(a, MF_EXY, m);
+            const double sig = pow(sxx * sxx + sxy * sxy, 0.5);
+            const double nP = a.p.pmode == 0 ? a.p.nstress : p;
+            const double EII = pow(exx * exx + exy * exy, 0.5);
+            const double eta_rad = a.p.inertyn == 0 ? a.p.rad_damping : 0.0;
+            const double W = a.p.faultwidth;
+            double V, phin;
+            int rfb = 0;
+            const int rit = rsf_newton(sig, nP, MXF(a, MX_A, m), MXF(a, MX_B, m), MXF(a, MX_L, m), MXF(a, MX_V0, m),
+                                       MXF(a, MX_MU0, m), MXF(a, MX_PHI, m), MXF(a, MX_V, m), dt, eta_rad, EII,
+                                       W, warm ? MXF(a, MX_GV, m) : 0.0, tol, V, phin, rfb);
+            MXF(a, MX_V, m) = V;
+            MXF(a, MX_PHI, m) = phin;
+            MXF(a, MX_SY, m) = sig;
+            MXF(a, MX_ETAVP, m) = sig > 0 ? eta * sig / (2 * eta * V / W + sig) : eta;
+            fb |= rfb;
+            if (!isfinite(V) || !isfinite(phin)) bad = 1;
+            atomicAdd(&sh[LS_FAULT], 1ull);
+            atomicAdd(&sh[LS_RSF_IT], (unsigned long long)rit);
+            atomicMax(&sh[LS_RSF_MAX], (unsigned long long)rit);
+            atomicAdd(&sh[LS_HIST_R + (rit < LU_HIST ? rit : LU_HIST - 1)], 1ull);
+        }
+        if (fb) atomicAdd(&sh[LS_FALLBACK], 1ull);
+        if (bad) atomicAdd(&sh[LS_NONFINITE], 1ull);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < LS_COUNT; i += LU_BLOCK) {
+        if (!sh[i]) continue;
+        if (i == LS_DUCT_MAX || i == LS_RSF_MAX) atomicMax(&a.lu_stats[i], sh[i]);
+        else atomicAdd(&a.lu_stats[i], sh[i]);
+    }
 }
 
 }  // namespace v2c
