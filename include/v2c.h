@@ -251,6 +251,7 @@ int v2c_ductilevisc_batch(v2c_ctx *ctx, long n, const char *type,
 typedef struct v2c_lu_stats {
     long markers, ductile_markers, ductile_iters, fault_markers, rsf_iters;
     long max_ductile_iters, max_rsf_iters, fallbacks, nonfinite;
+    long ductile_evals, rsf_evals;           /* residual evaluations (the cost measure) */
     long hist_ductile[32], hist_rsf[32];     /* iteration histograms; last bin = overflow */
 } v2c_lu_stats;
 
@@ -267,6 +268,31 @@ int v2c_local_download(v2c_ctx *ctx, double *eta, double *V, double *phi, double
 /* Set the warm-start guesses directly (host arrays; NULL keeps a field). */
 int v2c_local_set_guess(v2c_ctx *ctx, const double *eta, const double *V, long n);
 
+/* ---- learned warm start (no reference counterpart) ------------------------
+ * 6 -> 64 -> 64 -> 2 tanh MLP, BF16 MFMA with FP32 accumulation.  It predicts
+ * log10(root / default start) of the ductile unknown and of the fault slip
+ * rate from standardised (log10 strain-rate invariant, T, P, log10 stress
+ * invariant, log10 theta, a-b).  params: 4738 FP32 values, weights [out][in]
+ * in the order W1 b1 W2 b2 W3 b3; norm: in_mean[6] in_std[6] out_mean[2]
+ * out_std[2]. */
+int v2c_mlp_set(v2c_ctx *ctx, const float *params, const float *norm);
+int v2c_mlp_get(v2c_ctx *ctx, float *params, float *norm);
+/* Write warm-start guesses for every resident marker (inputs of dt). */
+int v2c_mlp_predict(v2c_ctx *ctx, double dt);
+/* Training samples from one plain local update of the resident markers:
+ * X[n][8] = 6 raw features + 2 labels, mask[n][2] (1 where a label exists).
+ * Runs the plain local update (state advances as in v2c_local_update). */
+int v2c_mlp_samples(v2c_ctx *ctx, double dt, double tol, float *X, float *mask);
+/* Train on standardised Xs[ns][6], Ts[ns][2] with masks Ms[ns][2]; batch is a
+ * multiple of 64; loss_out[epochs] receives each epoch's mean masked MSE. */
+int v2c_mlp_train(v2c_ctx *ctx, const float *Xs, const float *Ts, const float *Ms, long ns, int epochs,
+                  int batch, float lr, unsigned seed, float *loss_out);
+/* Gradient of the masked mean squared error over one batch (no update). */
+int v2c_mlp_grad(v2c_ctx *ctx, const float *Xs, const float *Ts, const float *Ms, long ns, float *grad,
+                 float *loss);
+/* Network outputs on standardised inputs. */
+int v2c_mlp_forward(v2c_ctx *ctx, const float *Xs, long ns, float *Ys);
+
 /* Timing of the last entry point's kernels (HIP events on the context
  * stream), in milliseconds; `kernel` selects one of V2C_K_* . */
 enum v2c_kernel_id {
@@ -275,6 +301,9 @@ enum v2c_kernel_id {
     V2C_K_COUNT
 };
 int v2c_enable_timing(v2c_ctx *ctx, int on);
+/* Measurement aid (no reference counterpart): time a plain device copy of
+ * `bytes` (reads and writes `bytes` each), used to calibrate HBM counters. */
+int v2c_copy_calibration(v2c_ctx *ctx, long bytes, double *ms);
 int v2c_kernel_time(v2c_ctx *ctx, int kernel, double *ms, long *launches);
 int v2c_reset_timing(v2c_ctx *ctx);
 

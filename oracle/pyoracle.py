@@ -60,6 +60,9 @@ def lib():
         L.or_compute_after_solve.restype = C.c_double
         L.or_allinteri.argtypes = [P(OrState), C.c_double, C.c_double] + [P(C.c_double)] * 4
         L.or_allinteri.restype = None
+        L.or_local_update.argtypes = [P(OrState), C.c_double, C.c_char_p, P(C.c_double), C.c_int,
+                                      C.c_double, P(C.c_long), P(C.c_int), P(C.c_int)]
+        L.or_local_update.restype = None
         _lib = L
     return _lib
 
@@ -133,3 +136,29 @@ class Oracle:
 
     def nysearch(self, y):
         return lib().or_nysearch(C.byref(self.grid.struct), y)
+
+
+MX_FIELDS = ("V", "phi", "a", "b", "L", "V0", "mu0", "eta", "eta_vp", "sy", "geta", "gV",
+             "root_eta", "root_V")
+
+
+def local_update(oracle: "Oracle", dt, fault, mx: dict, warm=False, tol=1e-10):
+    """Per-marker local update on the oracle's markers.  mx maps MX_FIELDS to
+    arrays (missing ones are zero); returns (outputs dict, stats, iteration arrays)."""
+    n = oracle.markers.n
+    buf = np.zeros((len(MX_FIELDS), n))
+    for i, k in enumerate(MX_FIELDS):
+        if k in mx:
+            buf[i] = mx[k]
+    buf = np.ascontiguousarray(buf)
+    stats = (C.c_long * 11)()
+    di = np.zeros(n, dtype=np.int32)
+    ri = np.zeros(n, dtype=np.int32)
+    f8 = np.ascontiguousarray(fault, dtype=np.uint8)
+    lib().or_local_update(C.byref(oracle.st), dt, C.cast(f8.ctypes.data, C.c_char_p),
+                          buf.ctypes.data_as(C.POINTER(C.c_double)), int(warm), tol, stats,
+                          di.ctypes.data_as(C.POINTER(C.c_int)), ri.ctypes.data_as(C.POINTER(C.c_int)))
+    out = {k: buf[i].copy() for i, k in enumerate(MX_FIELDS)}
+    names = ("markers ductile_markers ductile_iters fault_markers rsf_iters max_ductile_iters "
+             "max_rsf_iters fallbacks nonfinite ductile_evals rsf_evals").split()
+    return out, dict(zip(names, list(stats))), di, ri

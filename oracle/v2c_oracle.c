@@ -234,16 +234,28 @@ double or_ductilevisc(const or_state *s, int t, double sxy, double sxx,
 
 /* Build-defined NEWTON mode (not reference behaviour): the residual of the
  * reference iteration, f(eta) = eta - coeff*X^((n-1)/n) with
- * X = (eta_d+eta)/eta_d*(G dt + mu(eta)), solved with its exact derivative
+ * X = (eta_d+eta)/eta_d*G dt + eta (density_ductilevisc_calc.c:188-189),
+ * solved with its exact derivative
  * f'(eta) = 1 - (n-1)/n*coeff*X^(-1/n)*(G dt/eta_d + 1) to |f|/eta <= tol.
- * Plain Newton in eta, guarded: a step that would leave eta <= 0 is halved. */
-double or_ductilevisc_newton(const or_state *s, int t, double sxy, double sxx,
-                             double mtk, double mpb, double guess, double tol,
-                             int *iters, double *resid)
+ * Start eta_disl0 (:159); with a guess > 0 the residual is evaluated at both
+ * starts and Newton starts from the smaller; a guessed start whose residual
+ * grows after its first step falls back to eta_disl0.  *iters counts Newton
+ * updates, *evals residual evaluations, *fallback the fall-back; *root is the
+ * converged unknown.  Returns the clamped effective viscosity. */
+static double resid_duct(double eta, double eta_diff, double gdt, double coeff, double p1,
+                         double *f, double *X, double *Xp)
+{
+    *X = (eta_diff + eta) / eta_diff * gdt + eta;
+    *Xp = pow(*X, p1);
+    *f = eta - coeff * *Xp;
+    return fabs(*f) / eta;
+}
+
+double or_ductile_newton_ex(const or_state *s, int t, double sxy, double sxx, double mtk, double mpb,
+                            double guess, double tol, int *iters, int *evals, int *fallback, double *root)
 {
     const v2c_materials *M = s->mat;
-    if (iters) *iters = 0;
-    if (resid) *resid = 0;
+    *iters = 0; *evals = 0; *fallback = 0; *root = 0;
     if (!power_law_type(M, t) || t == 1)
         return or_ductilevisc(s, t, sxy, sxx, 0, 0, mtk, mpb, NULL);
     double n = M->mm[t];
@@ -257,33 +269,63 @@ double or_ductilevisc_newton(const or_state *s, int t, double sxy, double sxx,
     double eta_disl0 = 0.5 * sigin * M->nu[t] * e1 / pow(sigin, n);
     double coeff = pow(eta_disl0, 1.0 / n);
     double gdt = M->gg[t] * s->timestep;
-    double eta = guess > 0 ? guess : eta_disl0;
-    double r = 0, rprev = -1;
-    int k = 0, restarted = guess <= 0;
-    for (; k < 200; k++) {
-        double X = (eta_diff + eta) / eta_diff * gdt + eta;
-        double Xp = pow(X, (n - 1) / n);
-        double f = eta - coeff * Xp;
-        r = fabs(f) / eta;
-        if (r <= tol) break;
-        /* warm start fallback: the residual grew after the first step */
-        if (!restarted && k == 1 && r > rprev) {
-            eta = eta_disl0;
-            restarted = 1;
-            if (resid) *resid = -1.0;      /* marks the fallback for the caller */
-            continue;
-        }
-        rprev = r;
+    double p1 = (n - 1) / n, slope = gdt / eta_diff + 1;
+    double eta = eta_disl0, f, X, Xp;
+    double r = resid_duct(eta, eta_diff, gdt, coeff, p1, &f, &X, &Xp);
+    int ev = 1, k = 0, from_guess = 0;
+    if (r > tol && guess > 0) {
+        double fg, Xg, Xpg;
+        double rg = resid_duct(guess, eta_diff, gdt, coeff, p1, &fg, &Xg, &Xpg);
+        ev++;
+        if (rg < r) { eta = guess; f = fg; X = Xg; Xp = Xpg; r = rg; from_guess = 1; }
+    }
+    double rprev = r;
+    while (r > tol && k < 200) {
         /* X^(-1/n) evaluated as X^((n-1)/n) / X */
-        double df = 1 - (n - 1) / n * coeff * (Xp / X) * (gdt / eta_diff + 1);
-        if (f < 0 && !(df > 0)) { eta *= 2.0; continue; }   /* left of a flat region */
-        double next = eta - f / df;
+        double df = 1 - p1 * coeff * (Xp / X) * slope;
+        double next;
+        if (f < 0 && !(df > 0)) next = 2.0 * eta;
+        else next = eta - f / df;
         if (!(next > 0)) next = 0.5 * eta;
         eta = next;
+        k++;
+        r = resid_duct(eta, eta_diff, gdt, coeff, p1, &f, &X, &Xp);
+        ev++;
+        if (from_guess && k == 1 && r > rprev) {
+            from_guess = 0;
+            *fallback = 1;
+            eta = eta_disl0;
+            r = resid_duct(eta, eta_diff, gdt, coeff, p1, &f, &X, &Xp);
+            ev++;
+        }
+        rprev = r;
     }
-    if (iters) *iters = k;
-    if (resid && *resid != -1.0) *resid = r;
+    *iters = k; *evals = ev; *root = eta;
     return clamp_visc(s, t, 1.0 / (1.0 / eta_diff + 1.0 / eta));
+}
+
+double or_ductilevisc_newton(const or_state *s, int t, double sxy, double sxx, double mtk, double mpb,
+                             double guess, double tol, int *iters, double *resid)
+{
+    int ev, fb;
+    double root;
+    double v = or_ductile_newton_ex(s, t, sxy, sxx, mtk, mpb, guess, tol, iters, &ev, &fb, &root);
+    if (resid) {
+        /* residual at the returned root */
+        *resid = 0;
+        const v2c_materials *M = s->mat;
+        if (root > 0 && power_law_type(M, t)) {
+            double n = M->mm[t], sigin = pow(sxx * sxx + sxy * sxy, 0.5);
+            double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * mtk);
+            if (e1 > 150.0) e1 = 150.0;
+            e1 = exp(e1);
+            double eta_diff = 0.5 * (M->nu[t] / pow(M->ss[t], n - 1.0)) * e1;
+            double coeff = pow(0.5 * sigin * M->nu[t] * e1 / pow(sigin, n), 1.0 / n);
+            double f, X, Xp;
+            *resid = resid_duct(root, eta_diff, M->gg[t] * s->timestep, coeff, (n - 1) / n, &f, &X, &Xp);
+        }
+    }
+    return v;
 }
 
 /* Build-defined per-marker RSF update (not reference behaviour).  With the
@@ -291,20 +333,41 @@ double or_ductilevisc_newton(const or_state *s, int t, double sxy, double sxx,
  * (calc_RSF.c:96-103), solve the regularised RSF yield condition with
  * radiation damping for the slip rate,
  *   g(s) = nP a asinh(e^s K) + eta_rad e^s - sigII = 0,  K = exp((mu0+b phi')/a)/(2 V0),
- * by Newton in s = ln V (g is increasing and convex in s, so Newton started
- * right of the root converges monotonically).  Without damping the root is
- * the closed form of calc_RSF.c:192.  The result is capped at EII*W and
- * floored at 1e-40 like calc_RSF.c:193-196.  Returns the number of Newton
- * iterations; *fallback = 1 when a warm start was abandoned. */
+ * by Newton in s = ln V from s_hi = ln min(closed form of calc_RSF.c:192,
+ * sigII/eta_rad) (g is increasing and convex in s, so Newton from s_hi
+ * converges monotonically); guesses as in or_ductile_newton_ex.  Without
+ * damping the root is the closed form.  The result is capped at EII*W and
+ * floored at 1e-40 like calc_RSF.c:193-196.  Returns the Newton update count. */
+static double gfun_rsf(double s, double K, double nP, double a, double eta_rad, double sig, double *g, double *dg)
+{
+    double e = exp(s), z = e * K;
+    *g = nP * a * asinh(z) + eta_rad * e - sig;
+    *dg = nP * a * z / sqrt(1 + z * z) + eta_rad * e;
+    return fabs(*g) / sig;
+}
+
 int or_rsf_marker(double sig, double nP, double a, double b, double L, double V0, double mu0,
                   double phi, double Vprev, double dt, double eta_rad, double EII, double W,
                   double guessV, double tol, double *V_out, double *phi_out, int *fallback)
+{
+    int evals;
+    double root;
+    return or_rsf_marker_ex(sig, nP, a, b, L, V0, mu0, phi, Vprev, dt, eta_rad, EII, W, guessV, tol,
+                            V_out, phi_out, fallback, &root, &evals);
+}
+
+int or_rsf_marker_ex(double sig, double nP, double a, double b, double L, double V0, double mu0,
+                     double phi, double Vprev, double dt, double eta_rad, double EII, double W,
+                     double guessV, double tol, double *V_out, double *phi_out, int *fallback,
+                     double *root, int *evals)
 {
     double phin;
     if (Vprev * dt / L <= 1e-6) phin = log(exp(phi) * (1 - Vprev * dt / L) + V0 * dt / L);
     else phin = log(V0 / Vprev + (exp(phi) - V0 / Vprev) * exp(-Vprev * dt / L));
     *phi_out = phin;
     *fallback = 0;
+    *root = 0;
+    *evals = 0;
     if (!(sig > 0)) { *V_out = 0; return 0; }
     double K = exp((mu0 + b * phin) / a) / (2 * V0);
     double Vu = 2 * V0 * sinh(sig / (a * nP)) * exp(-(mu0 + b * phin) / a);
@@ -313,28 +376,40 @@ int or_rsf_marker(double sig, double nP, double a, double b, double L, double V0
     double V;
     int k = 0;
     if (eta_rad == 0 || !(Vhi > 0) || !isfinite(Vhi)) {
-        V = Vhi;                                     /* closed form (or its overflow) */
+        V = Vhi;
     } else {
-        double shi = log(Vhi);
-        int warm = guessV > 0;
-        double s = warm ? log(guessV) : shi;
-        if (s > shi) s = shi;
-        double gprev = -1;
-        for (; k < 200; k++) {
-            double e = exp(s), z = e * K;
-            double g = nP * a * asinh(z) + eta_rad * e - sig;
-            double ag = fabs(g) / sig;
-            if (ag <= tol) break;
-            if (warm && k == 1 && ag > gprev) { s = shi; warm = 0; *fallback = 1; continue; }
-            gprev = ag;
-            double dg = nP * a * z / sqrt(1 + z * z) + eta_rad * e;
+        double shi = log(Vhi), s = shi, g, dg;
+        double r = gfun_rsf(s, K, nP, a, eta_rad, sig, &g, &dg);
+        int ev = 1, from_guess = 0;
+        if (r > tol && guessV > 0) {
+            double sg = log(guessV);
+            if (sg > shi) sg = shi;
+            double gg, dgg;
+            double rg = gfun_rsf(sg, K, nP, a, eta_rad, sig, &gg, &dgg);
+            ev++;
+            if (rg < r) { s = sg; g = gg; dg = dgg; r = rg; from_guess = 1; }
+        }
+        double rprev = r;
+        while (r > tol && k < 200) {
             double sn = s - g / dg;
             if (!(sn <= shi)) sn = shi;
-            if (!(sn == sn)) sn = shi;
             s = sn;
+            k++;
+            r = gfun_rsf(s, K, nP, a, eta_rad, sig, &g, &dg);
+            ev++;
+            if (from_guess && k == 1 && r > rprev) {
+                from_guess = 0;
+                *fallback = 1;
+                s = shi;
+                r = gfun_rsf(s, K, nP, a, eta_rad, sig, &g, &dg);
+                ev++;
+            }
+            rprev = r;
         }
+        *evals = ev;
         V = exp(s);
     }
+    *root = V;
     V = ref_minv(V, EII * W);
     if (V < 1e-40) V = 1e-40;
     *V_out = V;
@@ -1035,31 +1110,33 @@ double or_compute_after_solve(or_state *s, double dt, int rk, long *nerr)
     return deltas;
 }
 
-/* Per-marker local update over all markers (build-defined).  mx holds 12
+/* Per-marker local update over all markers (build-defined).  mx holds 14
  * fields x n in the device layout: V, phi, a, b, L, V0, mu0, eta, eta_vp, sy,
- * guess_eta, guess_V.  stats: [n, ductile, ductile iters, fault, rsf iters,
- * max ductile, max rsf, fallbacks, nonfinite]. */
+ * guess_eta, guess_V, root_eta, root_V.  stats: [n, ductile, ductile iters,
+ * fault, rsf iters, max ductile, max rsf, fallbacks, nonfinite, ductile evals,
+ * rsf evals]. */
 void or_local_update(or_state *s, double dt, const unsigned char *fault, double *mx, int warm,
                      double tol, long *stats, int *ditr, int *ritr)
 {
     const long n = s->n;
     double *V = mx, *PHI = mx + n, *A = mx + 2 * n, *B = mx + 3 * n, *L = mx + 4 * n, *V0 = mx + 5 * n;
     double *MU0 = mx + 6 * n, *ETA = mx + 7 * n, *ETAVP = mx + 8 * n, *SY = mx + 9 * n;
-    double *GETA = mx + 10 * n, *GV = mx + 11 * n;
-    for (int k = 0; k < 9; k++) stats[k] = 0;
+    double *GETA = mx + 10 * n, *GV = mx + 11 * n, *RETA = mx + 12 * n, *RV = mx + 13 * n;
+    for (int k = 0; k < 11; k++) stats[k] = 0;
     for (long i = 0; i < n; i++) {
         int t = (int)s->m.t[i];
         double sxx = s->m.xx[i], sxy = s->m.xy[i], tk = s->m.k[i], p = s->m.p[i];
-        int it = 0;
-        double r = 0;
-        double eta = or_ductilevisc_newton(s, t, sxy, sxx, tk, 1e-5 * p, warm ? GETA[i] : 0.0, tol, &it, &r);
-        int fb = r == -1.0;
+        int it = 0, ev = 0, fb = 0;
+        double root;
+        double eta = or_ductile_newton_ex(s, t, sxy, sxx, tk, 1e-5 * p, warm ? GETA[i] : 0.0, tol, &it, &ev,
+                                          &fb, &root);
         ETA[i] = eta;
+        RETA[i] = root;
         stats[0]++;
         int bad = !isfinite(eta);
         if (ditr) ditr[i] = it;
         if (power_law_type(s->mat, t) && t != 1) {
-            stats[1]++; stats[2] += it; if (it > stats[5]) stats[5] = it;
+            stats[1]++; stats[2] += it; stats[9] += ev; if (it > stats[5]) stats[5] = it;
         }
         if (ritr) ritr[i] = 0;
         if (fault[i]) {
@@ -1068,15 +1145,15 @@ void or_local_update(or_state *s, double dt, const unsigned char *fault, double 
             double nP = s->prm.pmode == 0 ? s->prm.nstress : p;
             double EII = pow(exx * exx + exy * exy, 0.5);
             double eta_rad = s->prm.inertyn == 0 ? s->prm.rad_damping : 0.0;
-            double W = s->prm.faultwidth, v, phin;
-            int rfb = 0;
-            int rit = or_rsf_marker(sig, nP, A[i], B[i], L[i], V0[i], MU0[i], PHI[i], V[i], dt, eta_rad, EII, W,
-                                    warm ? GV[i] : 0.0, tol, &v, &phin, &rfb);
-            V[i] = v; PHI[i] = phin; SY[i] = sig;
+            double W = s->prm.faultwidth, v, phin, rroot;
+            int rfb = 0, rev = 0;
+            int rit = or_rsf_marker_ex(sig, nP, A[i], B[i], L[i], V0[i], MU0[i], PHI[i], V[i], dt, eta_rad, EII,
+                                       W, warm ? GV[i] : 0.0, tol, &v, &phin, &rfb, &rroot, &rev);
+            V[i] = v; PHI[i] = phin; SY[i] = sig; RV[i] = rroot;
             ETAVP[i] = sig > 0 ? eta * sig / (2 * eta * v / W + sig) : eta;
             fb |= rfb;
             if (!isfinite(v) || !isfinite(phin)) bad = 1;
-            stats[3]++; stats[4] += rit; if (rit > stats[6]) stats[6] = rit;
+            stats[3]++; stats[4] += rit; stats[10] += rev; if (rit > stats[6]) stats[6] = rit;
             if (ritr) ritr[i] = rit;
         }
         stats[7] += fb;

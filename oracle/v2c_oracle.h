@@ -40,6 +40,12 @@ double or_ductilevisc_newton(const or_state *s, int type, double sxy,
                              double sxx, double tk, double pb, double guess,
                              double tol, int *iters, double *resid);
 
+double or_ductile_newton_ex(const or_state *s, int t, double sxy, double sxx, double mtk, double mpb,
+                            double guess, double tol, int *iters, int *evals, int *fallback, double *root);
+int or_rsf_marker_ex(double sig, double nP, double a, double b, double L, double V0, double mu0,
+                     double phi, double Vprev, double dt, double eta_rad, double EII, double W,
+                     double guessV, double tol, double *V_out, double *phi_out, int *fallback,
+                     double *root, int *evals);
 int or_rsf_marker(double sig, double nP, double a, double b, double L, double V0, double mu0,
                   double phi, double Vprev, double dt, double eta_rad, double EII, double W,
                   double guessV, double tol, double *V_out, double *phi_out, int *fallback);

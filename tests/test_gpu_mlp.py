@@ -1,0 +1,101 @@
+"""Learned warm start: MFMA forward/backward against the FP64 reference,
+deterministic training, and the warm-started local update landing on the same
+roots as the plain one."""
+import numpy as np
+import pytest
+
+from velo2cycles_amd import synth
+from velo2cycles_amd.abi import Context
+from velo2cycles_amd.warmstart import (WarmStart, evaluate, generate, init_params, reference_grad,
+                                       upload_cloud)
+from tests.util import rel_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(lib, cfg, cap=None):
+    return Context(cfg.grid, cfg.params, cfg.mats, capacity=cap or cfg.n, lib=lib)
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    Xs = rng.normal(size=(n, 6)).astype(np.float32)
+    Ts = rng.normal(size=(n, 2)).astype(np.float32)
+    Ms = (rng.uniform(size=(n, 2)) > 0.25).astype(np.float32)
+    return Xs, Ts, Ms
+
+
+def test_mfma_forward_matches_reference(gpu):
+    cfg = synth.config_edge()
+    ctx = _ctx(gpu, cfg, 1)
+    ws = WarmStart(ctx)
+    p = init_params(5) * 1.5
+    ws.load(p, np.concatenate([np.zeros(6), np.ones(6), np.zeros(2), np.ones(2)]).astype(np.float32))
+    Xs, _, _ = _batch(1000, 1)
+    Y = ws.forward(Xs)
+    _, _, Yr = reference_grad(p, Xs, np.zeros((1000, 2)), np.ones((1000, 2)))
+    err = np.abs(Y - Yr).max() / np.abs(Yr).max()
+    assert err < 2e-2, err
+    ctx.close()
+
+
+def test_mfma_gradient_matches_fp64_reference(gpu):
+    cfg = synth.config_edge()
+    ctx = _ctx(gpu, cfg, 1)
+    ws = WarmStart(ctx)
+    p = init_params(9) * 1.2
+    ws.load(p, np.zeros(16, dtype=np.float32) + np.repeat([0, 1, 0, 1], [6, 6, 2, 2]).astype(np.float32))
+    Xs, Ts, Ms = _batch(4096, 2)
+    g, loss = ws.grad(Xs, Ts, Ms)
+    gr, lr_, _ = reference_grad(p, Xs, Ts, Ms, emulate_bf16=True)
+    gx, _, _ = reference_grad(p, Xs, Ts, Ms, emulate_bf16=False)
+    rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    rel_exact = np.linalg.norm(g - gx) / np.linalg.norm(gx)
+    assert rel < 2e-2, rel
+    assert rel_exact < 5e-2, rel_exact
+    assert abs(loss - lr_) <= 1e-2 * lr_
+    # every parameter block carries gradient
+    for a, b in ((0, 384), (384, 448), (448, 4544), (4544, 4608), (4608, 4736), (4736, 4738)):
+        assert np.abs(g[a:b]).max() > 0
+    ctx.close()
+
+
+def test_training_is_deterministic_and_learns(gpu):
+    cfg = synth.config_edge()
+    rng = np.random.default_rng(4)
+    n = 20000
+    Xs = rng.normal(size=(n, 6)).astype(np.float32)
+    Ts = np.stack([np.sin(Xs[:, 0]) + 0.5 * Xs[:, 1], Xs[:, 2] * Xs[:, 3] * 0.3], 1).astype(np.float32)
+    Ms = np.ones((n, 2), dtype=np.float32)
+    norm = np.concatenate([np.zeros(6), np.ones(6), np.zeros(2), np.ones(2)]).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        ctx = _ctx(gpu, cfg, 1)
+        ws = WarmStart(ctx)
+        ws.load(init_params(1), norm)
+        X8 = np.concatenate([Xs, Ts], 1)
+        loss = ws.train(X8, Ms, epochs=6, batch=1024, lr=3e-3, seed=7)
+        runs.append((loss, ws.fetch().copy()))
+        ctx.close()
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][0][-1] < 0.3 * runs[0][0][0]
+
+
+def test_warm_start_same_roots_fewer_rsf_iterations(gpu):
+    dt = synth.YEAR
+    cfg = synth.config_small()
+    ctx = _ctx(gpu, cfg)
+    X, M = generate(ctx, [11], dt)
+    ws = WarmStart(ctx)
+    ws.fit_norm(X, M)
+    ws.load(init_params(0))
+    ws.train(X, M, epochs=4, batch=4096, lr=2e-3)
+    res = evaluate(ctx, cfg, ws, dt, repeats=1)
+    pl, wm = res["plain"], res["warm"]
+    fault = cfg.rsf["fault"].astype(bool)
+    for k, sel in (("eta", slice(None)), ("V", fault), ("phi", fault)):
+        mx, _, nm = rel_errors(wm["out"][k][sel], pl["out"][k][sel])
+        assert nm == 0 and mx <= 1e-9, (k, mx)
+    assert wm["stats"]["rsf_iters"] < pl["stats"]["rsf_iters"]
+    assert wm["stats"]["nonfinite"] == 0
+    ctx.close()

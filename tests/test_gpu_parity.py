@@ -113,6 +113,7 @@ def test_ductilevisc_batch(gpu):
     tk = rng.uniform(273, 1900, n)
     pb = rng.uniform(1, 4e4, n)
     ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=1, lib=gpu)
+    ctx.set_time(cfg.timesum, cfg.timestep)     # G*dt enters the iteration
     nu, it = ctx.ductilevisc_batch(t, sxy, sxx, np.zeros(n), np.zeros(n), tk, pb)
     ctx.close()
     o = Oracle(cfg)

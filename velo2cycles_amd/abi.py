@@ -38,7 +38,7 @@ FIELD_ID = {n: i for i, n in enumerate(NODAL_FIELDS)}
 MARKER_FIELDS = ("x y k p xx xy exx exy e phi vx vy C d w a b L L0 V0 fric_static C0").split()
 
 KERNELS = ("m2n_cells", "m2n_nodes", "n2m_cells", "n2m_nodes", "advect", "sort",
-           "before", "after")
+           "before", "after", "local", "mlp")
 
 ERRORS = {0: "ok", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECAPACITY",
           -5: "EUNSUPPORTED", -6: "ENOGPU"}
@@ -80,6 +80,19 @@ _MAT_DBLS = "nu dh dv ss mm gg n0 n1 ro bb aa cp kt kf kp ht".split()
 class Materials(C.Structure):
     _fields_ = ([("ntypes", C.c_int)] + [(n, C.c_double * V2C_MAX_TYPES) for n in _MAT_DBLS]
                 + [("im", C.c_int * V2C_MAX_TYPES)])
+
+
+class LUStats(C.Structure):
+    _fields_ = ([(n, C.c_long) for n in ("markers ductile_markers ductile_iters fault_markers rsf_iters "
+                                         "max_ductile_iters max_rsf_iters fallbacks nonfinite "
+                                         "ductile_evals rsf_evals").split()]
+                + [("hist_ductile", C.c_long * 32), ("hist_rsf", C.c_long * 32)])
+
+    def as_dict(self) -> dict:
+        d = {n: getattr(self, n) for n, _ in self._fields_[:11]}
+        d["hist_ductile"] = list(self.hist_ductile)
+        d["hist_rsf"] = list(self.hist_rsf)
+        return d
 
 
 class Markers(C.Structure):
@@ -197,6 +210,13 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                                                   P(C.c_double), P(C.c_double), P(C.c_double),
                                                   P(C.c_double), P(C.c_double), P(C.c_int)]),
                        ("v2c_enable_timing", [C.c_int]),
+                       ("v2c_set_friction_fields", [C.c_int]),
+                       ("v2c_copy_calibration", [C.c_long, P(C.c_double)]),
+                       ("v2c_order_stats", [P(C.c_long), P(C.c_long), P(C.c_long)]),
+                       ("v2c_rsf_upload", [C.c_char_p] + [P(C.c_double)] * 7 + [C.c_long]),
+                       ("v2c_local_update", [C.c_double, C.c_int, C.c_double, P(LUStats)]),
+                       ("v2c_local_download", [P(C.c_double)] * 5 + [C.c_long]),
+                       ("v2c_local_set_guess", [P(C.c_double), P(C.c_double), C.c_long]),
                        ("v2c_kernel_time", [C.c_int, P(C.c_double), P(C.c_long)]),
                        ("v2c_reset_timing", [])]:
         fn = getattr(lib, name)
@@ -324,3 +344,48 @@ class Context:
         self._check(self.lib.v2c_kernel_time(self.h, KERNELS.index(name), C.byref(ms), C.byref(n)),
                     "kernel_time")
         return ms.value, n.value
+
+
+def _opt(a):
+    return None if a is None else _dptr(a)
+
+
+def _ctx_extras():
+    def order_stats(self):
+        v = [C.c_long(0) for _ in range(3)]
+        self._check(self.lib.v2c_order_stats(self.h, *[C.byref(x) for x in v]), "order_stats")
+        return {"last_movers": v[0].value, "full_sorts": v[1].value, "incremental": v[2].value}
+
+    def rsf_upload(self, fault, V, phi, a, b, L, V0, mu0):
+        n = len(fault)
+        f8 = np.ascontiguousarray(fault, dtype=np.uint8)
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (V, phi, a, b, L, V0, mu0)]
+        self._check(self.lib.v2c_rsf_upload(self.h, C.cast(f8.ctypes.data, C.c_char_p),
+                                            *[_dptr(x) for x in arrs], n), "rsf_upload")
+
+    def local_update(self, dt, warm=False, tol=1e-10):
+        st = LUStats()
+        self._check(self.lib.v2c_local_update(self.h, dt, int(warm), tol, C.byref(st)), "local_update")
+        return st.as_dict()
+
+    def local_download(self, n):
+        out = {k: np.empty(n) for k in ("eta", "V", "phi", "eta_vp", "sy")}
+        self._check(self.lib.v2c_local_download(self.h, *[_dptr(out[k]) for k in
+                                                           ("eta", "V", "phi", "eta_vp", "sy")], n),
+                    "local_download")
+        return out
+
+    def local_set_guess(self, eta=None, V=None):
+        n = len(eta) if eta is not None else len(V)
+        e = None if eta is None else np.ascontiguousarray(eta, dtype=np.float64)
+        v = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+        self._check(self.lib.v2c_local_set_guess(self.h, _opt(e), _opt(v), n), "local_set_guess")
+
+    def set_friction_fields(self, on):
+        self._check(self.lib.v2c_set_friction_fields(self.h, int(on)), "set_friction_fields")
+
+    for f in (order_stats, rsf_upload, local_update, local_download, local_set_guess, set_friction_fields):
+        setattr(Context, f.__name__, f)
+
+
+_ctx_extras()

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "v2c_kernels.hip"
+#include "v2c_mlp.hip"
 
 using namespace v2c;
 
@@ -47,6 +48,10 @@ struct v2c_ctx {
     double *d_mx = nullptr;
     uint8_t *d_fault = nullptr;
     unsigned long long *d_lu = nullptr;
+    // warm-start MLP
+    float *d_mlp = nullptr, *d_mlp_m = nullptr, *d_mlp_v = nullptr, *d_mlp_norm = nullptr;
+    long mlp_steps = 0;
+    int mlp_ready = 0;
     void *d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     // timing
@@ -290,7 +295,8 @@ static void free_all(v2c_ctx *c)
     void *ptrs[] = {c->d_gx, c->d_gy, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
                     c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_red, c->d_counters, c->d_sort_tmp,
                     c->d_order2, c->d_skey2, c->d_ids_mv, c->d_ids_st, c->d_flag_mv, c->d_flag_st,
-                    c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu};
+                    c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
+                    c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
@@ -364,6 +370,13 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_lu, sizeof(unsigned long long) * LS_COUNT));
     CK(hipMemsetAsync(c->d_mx, 0, sizeof(double) * NMX * c->cap, c->stream));
     CK(hipMemsetAsync(c->d_fault, 0, c->cap, c->stream));
+    CK(hipMalloc(&c->d_mlp, sizeof(float) * MLP_NPARAM));
+    CK(hipMalloc(&c->d_mlp_m, sizeof(float) * MLP_NPARAM));
+    CK(hipMalloc(&c->d_mlp_v, sizeof(float) * MLP_NPARAM));
+    CK(hipMalloc(&c->d_mlp_norm, sizeof(float) * MLP_NNORM));
+    CK(hipMemsetAsync(c->d_mlp, 0, sizeof(float) * MLP_NPARAM, c->stream));
+    CK(hipMemsetAsync(c->d_mlp_m, 0, sizeof(float) * MLP_NPARAM, c->stream));
+    CK(hipMemsetAsync(c->d_mlp_v, 0, sizeof(float) * MLP_NPARAM, c->stream));
     CK(hipMemsetAsync(c->d_nod, 0, sizeof(double) * V2C_NF * c->nodenum, c->stream));
     CK(hipMemsetAsync(c->d_mk, 0, sizeof(double) * NMF * c->cap, c->stream));
     CK(hipMemsetAsync(c->d_t, 0, c->cap, c->stream));
@@ -750,6 +763,8 @@ int v2c_local_update(v2c_ctx *c, double dt, int warm, double tol, v2c_lu_stats *
         st->rsf_iters = (long)h[LS_RSF_IT]; st->max_ductile_iters = (long)h[LS_DUCT_MAX];
         st->max_rsf_iters = (long)h[LS_RSF_MAX]; st->fallbacks = (long)h[LS_FALLBACK];
         st->nonfinite = (long)h[LS_NONFINITE];
+        st->ductile_evals = (long)h[LS_DUCT_EV];
+        st->rsf_evals = (long)h[LS_RSF_EV];
         for (int k = 0; k < LU_HIST; k++) {
             st->hist_ductile[k] = (long)h[LS_HIST_D + k];
             st->hist_rsf[k] = (long)h[LS_HIST_R + k];
@@ -769,6 +784,246 @@ int v2c_local_download(v2c_ctx *c, double *eta, double *V, double *phi, double *
             HIPCHK(c, hipMemcpyAsync(dst[k], c->d_mx + (size_t)src[k] * c->cap, sizeof(double) * n,
                                      hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---- learned warm start --------------------------------------------------
+
+int v2c_mlp_set(v2c_ctx *c, const float *params, const float *norm)
+{
+    if (!c || !params || !norm) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_mlp, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mlp_norm, norm, sizeof(float) * MLP_NNORM, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_mlp_m, 0, sizeof(float) * MLP_NPARAM, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_mlp_v, 0, sizeof(float) * MLP_NPARAM, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->mlp_steps = 0;
+    c->mlp_ready = 1;
+    return 0;
+}
+
+int v2c_mlp_get(v2c_ctx *c, float *params, float *norm)
+{
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (params)
+        HIPCHK(c, hipMemcpyAsync(params, c->d_mlp, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
+    if (norm)
+        HIPCHK(c, hipMemcpyAsync(norm, c->d_mlp_norm, sizeof(float) * MLP_NNORM, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_mlp_predict(v2c_ctx *c, double dt)
+{
+    if (!c) return V2C_EINVAL;
+    if (!c->mlp_ready) return set_err(c, V2C_EINVAL, "no MLP parameters loaded (v2c_mlp_set)");
+    HIPCHK(c, hipSetDevice(c->device));
+    KArgs a = make_args(c);
+    int rc = 0;
+    if (c->n > 0) {
+        rc = launch(c, V2C_K_MLP, [&] {
+            hipLaunchKernelGGL(k_mlp_guess, dim3(blocks_for(c->n, 64)), dim3(64), 0, c->stream, a, dt, c->d_mlp,
+                               c->d_mlp_norm);
+        });
+        if (rc) return rc;
+    }
+    return finish(c);
+}
+
+int v2c_mlp_samples(v2c_ctx *c, double dt, double tol, float *X, float *mask)
+{
+    if (!c || !X || !mask) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long n = c->n;
+    if (n == 0) return 0;
+    float *dX = nullptr, *dM = nullptr;
+    HIPCHK(c, hipMalloc(&dX, sizeof(float) * 8 * n));
+    hipError_t e = hipMalloc(&dM, sizeof(float) * 2 * n);
+    if (e != hipSuccess) { (void)hipFree(dX); return set_err(c, V2C_ENOMEM, "mlp_samples"); }
+    KArgs a = make_args(c);
+    hipLaunchKernelGGL(k_mlp_features, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a, dt, dX, dM);
+    e = hipGetLastError();
+    int rc = 0;
+    if (e == hipSuccess) rc = v2c_local_update(c, dt, 0, tol, nullptr);
+    if (e == hipSuccess && rc == 0) {
+        hipLaunchKernelGGL(k_mlp_labels, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a, dX, dM);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && rc == 0) e = hipMemcpyAsync(X, dX, sizeof(float) * 8 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && rc == 0) e = hipMemcpyAsync(mask, dM, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dX);
+    (void)hipFree(dM);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(c, V2C_EHIP, "mlp_samples: %s", hipGetErrorString(e));
+    return 0;
+}
+
+// Deterministic mini-batch training: a fixed xorshift permutation per epoch
+// (seeded), per-wave gradient partials summed in wave order, Adam on FP32
+// master weights.  Xs/Ts are already standardised.  loss_out[epoch] receives
+// the epoch's mean masked squared error.
+namespace {
+struct TrainBufs {
+    float *X = nullptr, *T = nullptr, *M = nullptr, *G = nullptr, *L = nullptr;
+    uint32_t *idx = nullptr;
+    ~TrainBufs()
+    {
+        void *p[] = {X, T, M, G, L, idx};
+        for (void *q : p)
+            if (q) (void)hipFree(q);
+    }
+};
+}  // namespace
+
+static int mlp_upload(v2c_ctx *c, TrainBufs &b, const float *Xs, const float *Ts, const float *Ms, long ns,
+                      long batch)
+{
+    const long nw = (batch + 63) / 64;
+    HIPCHK(c, hipMalloc(&b.X, sizeof(float) * MLP_IN * ns));
+    HIPCHK(c, hipMalloc(&b.T, sizeof(float) * 2 * ns));
+    HIPCHK(c, hipMalloc(&b.M, sizeof(float) * 2 * ns));
+    HIPCHK(c, hipMalloc(&b.G, sizeof(float) * MLP_NPARAM * nw));
+    HIPCHK(c, hipMalloc(&b.L, sizeof(float) * nw));
+    HIPCHK(c, hipMalloc(&b.idx, sizeof(uint32_t) * ns));
+    HIPCHK(c, hipMemcpyAsync(b.X, Xs, sizeof(float) * MLP_IN * ns, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.T, Ts, sizeof(float) * 2 * ns, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.M, Ms, sizeof(float) * 2 * ns, hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+int v2c_mlp_train(v2c_ctx *c, const float *Xs, const float *Ts, const float *Ms, long ns, int epochs, int batch,
+                  float lr, unsigned seed, float *loss_out)
+{
+    if (!c || !Xs || !Ts || !Ms || ns <= 0 || epochs <= 0 || batch < 64 || batch % 64) return V2C_EINVAL;
+    if (!c->mlp_ready) return set_err(c, V2C_EINVAL, "initialise the MLP with v2c_mlp_set first");
+    HIPCHK(c, hipSetDevice(c->device));
+    TrainBufs b;
+    int rc = mlp_upload(c, b, Xs, Ts, Ms, ns, batch);
+    if (rc) return rc;
+    std::vector<uint32_t> perm(ns);
+    std::vector<float> lhost((batch + 63) / 64);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
+    auto rnd = [&]() {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        return st;
+    };
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < epochs; ep++) {
+        for (long i = 0; i < ns; i++) perm[i] = (uint32_t)i;
+        for (long i = ns - 1; i > 0; i--) std::swap(perm[i], perm[rnd() % (uint64_t)(i + 1)]);
+        HIPCHK(c, hipMemcpyAsync(b.idx, perm.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, c->stream));
+        double ep_loss = 0, ep_cnt = 0;
+        for (long off = 0; off < ns; off += batch) {
+            const long nb = std::min<long>(batch, ns - off);
+            double cnt = 0;
+            for (long i = 0; i < nb; i++) cnt += Ms[2 * perm[off + i]] + Ms[2 * perm[off + i] + 1];
+            if (cnt == 0) continue;
+            const int nw = (int)((nb + 63) / 64);
+            c->mlp_steps++;
+            const float bc1 = 1.f - powf(b1, (float)c->mlp_steps), bc2 = 1.f - powf(b2, (float)c->mlp_steps);
+            rc = launch(c, V2C_K_MLP, [&] {
+                hipLaunchKernelGGL(k_mlp_train, dim3(nw), dim3(64), 0, c->stream, c->d_mlp, b.X, b.T, b.M,
+                                   b.idx + off, nb, (float)(1.0 / cnt), b.G, b.L);
+                hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(MLP_NPARAM, 256)), dim3(256), 0, c->stream, c->d_mlp,
+                                   c->d_mlp_m, c->d_mlp_v, b.G, nw, lr, b1, b2, eps, bc1, bc2);
+            });
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(lhost.data(), b.L, sizeof(float) * nw, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (int w = 0; w < nw; w++) ep_loss += lhost[w];
+            ep_cnt += cnt;
+        }
+        if (loss_out) loss_out[ep] = (float)(ep_cnt > 0 ? ep_loss / ep_cnt : 0.0);
+    }
+    return 0;
+}
+
+int v2c_mlp_grad(v2c_ctx *c, const float *Xs, const float *Ts, const float *Ms, long ns, float *grad, float *loss)
+{
+    if (!c || !Xs || !Ts || !Ms || ns <= 0 || !grad) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long batch = ((ns + 63) / 64) * 64;
+    TrainBufs b;
+    int rc = mlp_upload(c, b, Xs, Ts, Ms, ns, batch);
+    if (rc) return rc;
+    std::vector<uint32_t> idx(ns);
+    double cnt = 0;
+    for (long i = 0; i < ns; i++) { idx[i] = (uint32_t)i; cnt += Ms[2 * i] + Ms[2 * i + 1]; }
+    if (cnt == 0) cnt = 1;
+    HIPCHK(c, hipMemcpyAsync(b.idx, idx.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, c->stream));
+    const int nw = (int)((ns + 63) / 64);
+    float *dgrad = b.T;   // reuse: T is not needed after the kernel reads it? keep separate
+    float *dg = nullptr;
+    HIPCHK(c, hipMalloc(&dg, sizeof(float) * MLP_NPARAM));
+    hipLaunchKernelGGL(k_mlp_train, dim3(nw), dim3(64), 0, c->stream, c->d_mlp, b.X, b.T, b.M, b.idx, ns,
+                       (float)(1.0 / cnt), b.G, b.L);
+    hipLaunchKernelGGL(k_mlp_gradsum, dim3(blocks_for(MLP_NPARAM, 256)), dim3(256), 0, c->stream, b.G, nw, dg);
+    (void)dgrad;
+    hipError_t e = hipGetLastError();
+    std::vector<float> lh(nw);
+    if (e == hipSuccess) e = hipMemcpyAsync(grad, dg, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(lh.data(), b.L, sizeof(float) * nw, hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dg);
+    if (e != hipSuccess) return set_err(c, V2C_EHIP, "mlp_grad: %s", hipGetErrorString(e));
+    if (loss) {
+        double t = 0;
+        for (float v : lh) t += v;
+        *loss = (float)(t / cnt);
+    }
+    return 0;
+}
+
+int v2c_mlp_forward(v2c_ctx *c, const float *Xs, long ns, float *Ys)
+{
+    if (!c || !Xs || !Ys || ns <= 0) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    float *dX = nullptr, *dY = nullptr;
+    HIPCHK(c, hipMalloc(&dX, sizeof(float) * MLP_IN * ns));
+    hipError_t e = hipMalloc(&dY, sizeof(float) * 2 * ns);
+    if (e == hipSuccess) e = hipMemcpyAsync(dX, Xs, sizeof(float) * MLP_IN * ns, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_mlp_forward, dim3(blocks_for(ns, 64)), dim3(64), 0, c->stream, c->d_mlp, dX, ns, dY);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(Ys, dY, sizeof(float) * 2 * ns, hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    if (dX) (void)hipFree(dX);
+    if (dY) (void)hipFree(dY);
+    if (e != hipSuccess) return set_err(c, V2C_EHIP, "mlp_forward: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int v2c_copy_calibration(v2c_ctx *c, long bytes, double *ms)
+{
+    if (!c || bytes <= 0) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long n = bytes / 8;
+    double *src = nullptr, *dst = nullptr;
+    HIPCHK(c, hipMalloc(&src, n * 8));
+    hipError_t e = hipMalloc(&dst, n * 8);
+    if (e == hipSuccess) e = hipMemsetAsync(src, 0, n * 8, c->stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_copy_calib, dim3(8192), dim3(256), 0, c->stream, src, dst, n);   // warm
+        (void)hipEventRecord(e0, c->stream);
+        hipLaunchKernelGGL(k_copy_calib, dim3(8192), dim3(256), 0, c->stream, src, dst, n);
+        (void)hipEventRecord(e1, c->stream);
+        e = hipStreamSynchronize(c->stream);
+    }
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e != hipSuccess) return set_err(c, V2C_EHIP, "copy_calibration: %s", hipGetErrorString(e));
     return 0;
 }
 

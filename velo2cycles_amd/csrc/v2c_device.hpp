@@ -26,6 +26,7 @@ enum MX {
     MX_V = 0, MX_PHI, MX_A, MX_B, MX_L, MX_V0, MX_MU0,       // RSF inputs / state
     MX_ETA, MX_ETAVP, MX_SY,                                  // outputs
     MX_GETA, MX_GV,                                           // warm-start guesses
+    MX_ROOT_ETA, MX_ROOT_V,                                   // raw Newton roots
     NMX
 };
 
@@ -416,15 +417,19 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
 // Ductile viscosity by Newton on the reference's residual
 // (density_ductilevisc_calc.c:188-193): f(eta) = eta - coeff*X^((n-1)/n),
 // X = (eta_d+eta)/eta_d*G dt + eta, exact f'(eta), to |f|/eta <= tol.
-// guess <= 0 starts from the reference's start eta_disl0; a warm start whose
-// residual grows after its first step restarts from eta_disl0 (*fallback = 1).
+// Start: the reference's eta_disl0 (:159).  With a guess (> 0), the residual
+// is evaluated at both starts and Newton starts from the smaller one; a guess
+// whose residual grows after its first step falls back to eta_disl0
+// (*fallback = 1).  *iters counts Newton updates, *evals residual evaluations.
 __device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double sxy, double sxx,
                                                  double mtk, double mpb, double guess, double tol,
-                                                 int *iters, int *fallback)
+                                                 int *iters, int *evals, int *fallback, double *root)
 {
     const DevMat *M = a.mat;
     *iters = 0;
+    *evals = 0;
     *fallback = 0;
+    *root = 0;
     if (!power_law_type(M, t) || t == 1) {
         int it;
         return ductilevisc(a, t, sxy, sxx, mtk, mpb, &it);
@@ -441,46 +446,66 @@ __device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double s
     const double coeff = pow(eta_disl0, 1.0 / n);
     const double gdt = M->gg[t] * a.timestep;
     const double p1 = (n - 1) / n, slope = gdt / eta_diff + 1;
-    double eta = guess > 0 ? guess : eta_disl0;
-    bool restarted = !(guess > 0);
-    double rprev = -1;
-    int k = 0;
-    for (; k < 200; k++) {
-        const double X = (eta_diff + eta) / eta_diff * gdt + eta;
-        const double Xp = pow(X, p1);
-        const double f = eta - coeff * Xp;
-        const double r = fabs(f) / eta;
-        if (r <= tol) break;
-        if (!restarted && k == 1 && r > rprev) {
-            eta = eta_disl0;
-            restarted = true;
-            *fallback = 1;
-            continue;
-        }
-        rprev = r;
-        const double df = 1 - p1 * coeff * (Xp / X) * slope;    // X^(-1/n) = X^((n-1)/n) / X
-        if (f < 0 && !(df > 0)) { eta *= 2.0; continue; }
-        double next = eta - f / df;
+    auto resid = [&](double eta, double &f, double &X, double &Xp) {
+        X = (eta_diff + eta) / eta_diff * gdt + eta;
+        Xp = pow(X, p1);
+        f = eta - coeff * Xp;
+        return fabs(f) / eta;
+    };
+    double eta = eta_disl0, f, X, Xp;
+    double r = resid(eta, f, X, Xp);
+    int ev = 1, k = 0;
+    bool from_guess = false;
+    if (r > tol && guess > 0) {
+        double fg, Xg, Xpg;
+        const double rg = resid(guess, fg, Xg, Xpg);
+        ev++;
+        if (rg < r) { eta = guess; f = fg; X = Xg; Xp = Xpg; r = rg; from_guess = true; }
+    }
+    double rprev = r;
+    while (r > tol && k < 200) {
+        const double df = 1 - p1 * coeff * (Xp / X) * slope;   // X^(-1/n) = X^((n-1)/n) / X
+        double next;
+        if (f < 0 && !(df > 0)) next = 2.0 * eta;               // left of a flat region
+        else next = eta - f / df;
         if (!(next > 0)) next = 0.5 * eta;
         eta = next;
+        k++;
+        r = resid(eta, f, X, Xp);
+        ev++;
+        if (from_guess && k == 1 && r > rprev) {                 // warm start diverging
+            from_guess = false;
+            *fallback = 1;
+            eta = eta_disl0;
+            r = resid(eta, f, X, Xp);
+            ev++;
+        }
+        rprev = r;
     }
     *iters = k;
+    *evals = ev;
+    *root = eta;
     return clamp_visc(a, t, 1.0 / (1.0 / eta_diff + 1.0 / eta));
 }
 
 // Per-marker RSF slip rate with radiation damping at the aged state
 // (aging law calc_RSF.c:96-103; yield law calc_RSF.c:129 solved for V by
-// Newton in s = ln V; cap and floor calc_RSF.c:193-196).
+// Newton in s = ln V; cap and floor calc_RSF.c:193-196).  The default start
+// s_hi = ln min(undamped closed form calc_RSF.c:192, sig/eta_rad) bounds the
+// root from above; g is increasing and convex in s, so Newton from s_hi
+// converges monotonically.  Guess handling as in ductile_newton.
 __device__ __forceinline__ int rsf_newton(double sig, double nP, double ra, double rb, double rL, double rV0,
                                           double mu0, double phi, double Vprev, double dt, double eta_rad,
                                           double EII, double W, double guessV, double tol, double &V_out,
-                                          double &phi_out, int &fallback)
+                                          double &phi_out, int &fallback, double &root, int &evals)
 {
     double phin;
     if (Vprev * dt / rL <= 1e-6) phin = log(exp(phi) * (1 - Vprev * dt / rL) + rV0 * dt / rL);
     else phin = log(rV0 / Vprev + (exp(phi) - rV0 / Vprev) * exp(-Vprev * dt / rL));
     phi_out = phin;
     fallback = 0;
+    root = 0;
+    evals = 0;
     if (!(sig > 0)) { V_out = 0; return 0; }
     const double K = exp((mu0 + rb * phin) / ra) / (2 * rV0);
     const double Vu = 2 * rV0 * sinh(sig / (ra * nP)) * exp(-(mu0 + rb * phin) / ra);
@@ -492,24 +517,45 @@ __device__ __forceinline__ int rsf_newton(double sig, double nP, double ra, doub
         V = Vhi;
     } else {
         const double shi = log(Vhi);
-        bool warm = guessV > 0;
-        double s = warm ? log(guessV) : shi;
-        if (s > shi) s = shi;
-        double gprev = -1;
-        for (; k < 200; k++) {
+        auto gfun = [&](double s, double &g, double &dg) {
             const double e = exp(s), z = e * K;
-            const double g = nP * ra * asinh(z) + eta_rad * e - sig;
-            const double ag = fabs(g) / sig;
-            if (ag <= tol) break;
-            if (warm && k == 1 && ag > gprev) { s = shi; warm = false; fallback = 1; continue; }
-            gprev = ag;
-            const double dg = nP * ra * z / sqrt(1 + z * z) + eta_rad * e;
+            g = nP * ra * asinh(z) + eta_rad * e - sig;
+            dg = nP * ra * z / sqrt(1 + z * z) + eta_rad * e;
+            return fabs(g) / sig;
+        };
+        double s = shi, g, dg;
+        double r = gfun(s, g, dg);
+        int ev = 1;
+        bool from_guess = false;
+        if (r > tol && guessV > 0) {
+            double sg = log(guessV);
+            if (sg > shi) sg = shi;
+            double gg, dgg;
+            const double rg = gfun(sg, gg, dgg);
+            ev++;
+            if (rg < r) { s = sg; g = gg; dg = dgg; r = rg; from_guess = true; }
+        }
+        double rprev = r;
+        while (r > tol && k < 200) {
             double sn = s - g / dg;
             if (!(sn <= shi)) sn = shi;
             s = sn;
+            k++;
+            r = gfun(s, g, dg);
+            ev++;
+            if (from_guess && k == 1 && r > rprev) {
+                from_guess = false;
+                fallback = 1;
+                s = shi;
+                r = gfun(s, g, dg);
+                ev++;
+            }
+            rprev = r;
         }
+        evals = ev;
         V = exp(s);
     }
+    root = V;
     V = ref_minv(V, EII * W);
     if (V < 1e-40) V = 1e-40;
     V_out = V;

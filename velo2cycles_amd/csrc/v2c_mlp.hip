@@ -1,0 +1,365 @@
+// v2c_mlp.hip -- learned warm start of the per-marker local update.
+//
+// A 6 -> 64 -> 64 -> 2 tanh MLP proposes, per marker, the log10 ratio of the
+// Newton root to the analytic default start for (i) the ductile unknown
+// (dislocation viscosity; default eta_disl0 of density_ductilevisc_calc.c:159)
+// and (ii) the fault slip rate (default min(undamped closed form
+// calc_RSF.c:192, sigII/eta_rad)).  Inputs: log10 strain-rate invariant, T, P,
+// log10 stress invariant, log10 theta, a-b, standardised.
+//
+// Forward and backward run on the matrix cores: v_mfma_f32_16x16x16_bf16 with
+// BF16 operands and FP32 accumulation.  One wavefront owns 64 rows (markers or
+// training samples); activations are staged in LDS between layers.  Training
+// writes per-wavefront gradient partials that a second kernel sums in a fixed
+// order before an Adam step on FP32 master weights, so training is
+// deterministic.
+#pragma once
+#include "v2c_device.hpp"
+
+namespace v2c {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MLP_IN = 6, MLP_H = 64, MLP_OUT = 2;
+// flat FP32 parameter layout, weights stored [out][in]
+constexpr int P_W1 = 0, P_B1 = P_W1 + MLP_H * MLP_IN, P_W2 = P_B1 + MLP_H, P_B2 = P_W2 + MLP_H * MLP_H,
+              P_W3 = P_B2 + MLP_H, P_B3 = P_W3 + MLP_OUT * MLP_H, MLP_NPARAM = P_B3 + MLP_OUT;
+static_assert(MLP_NPARAM == 4738, "parameter count");
+// normalisation block: in_mean[6] in_std[6] out_mean[2] out_std[2]
+constexpr int MLP_NNORM = 16;
+
+__device__ __forceinline__ short f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (short)(u >> 16);   // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);                                     // round to nearest even
+    return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
+
+__device__ __forceinline__ float fast_tanh(float x)
+{
+    const float ax = fabsf(x);
+    const float e = __expf(-2.0f * ax);
+    const float t = (1.0f - e) / (1.0f + e);
+    return copysignf(t, x);
+}
+
+// One 16x16 output tile, K = 16*ksteps.  A(i,k) and B(k,n) are fetched by
+// callables; lane l supplies A rows i = l%16 with k = 4*(l/16)+j and B
+// columns n = l%16 with the same k; the result element j of lane l is
+// D(4*(l/16)+j, l%16).
+template <typename FA, typename FB>
+__device__ __forceinline__ f32x4 mfma_tile(int ksteps, FA &&fa, FB &&fb)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kb = 4 * (lane >> 4);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < ksteps; ks++) {
+        bf16x4 av, bv;
+        for (int j = 0; j < 4; j++) {
+            const int k = ks * 16 + kb + j;
+            av[j] = fa(i, k);
+            bv[j] = fb(k, i);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Activation staging of one wavefront (64 rows).
+struct MlpLds {
+    short x[64][16];          // inputs, bf16, padded to K = 16
+    float h1[64][MLP_H];      // tanh(Z1)
+    float h2[64][MLP_H];      // tanh(Z2)
+    float g[64][MLP_H];       // scratch: dZ2 then dZ1 (FP32)
+    float y[64][MLP_OUT];     // outputs / dY
+};
+
+// Forward pass of the wave's 64 rows: x must be filled; fills h1, h2, y.
+__device__ __forceinline__ void mlp_forward(const float *__restrict__ P, MlpLds &s)
+{
+    const int lane = threadIdx.x & 63;
+    const int n = lane & 15, ib = 4 * (lane >> 4);
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = mfma_tile(1, [&](int i, int k) { return s.x[mt * 16 + i][k]; },
+                                [&](int k, int c) { return k < MLP_IN ? f2bf(P[P_W1 + (nt * 16 + c) * MLP_IN + k]) : (short)0; });
+            for (int j = 0; j < 4; j++)
+                s.h1[mt * 16 + ib + j][nt * 16 + n] = fast_tanh(d[j] + P[P_B1 + nt * 16 + n]);
+        }
+    __syncthreads();
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = mfma_tile(4, [&](int i, int k) { return f2bf(s.h1[mt * 16 + i][k]); },
+                                [&](int k, int c) { return f2bf(P[P_W2 + (nt * 16 + c) * MLP_H + k]); });
+            for (int j = 0; j < 4; j++)
+                s.h2[mt * 16 + ib + j][nt * 16 + n] = fast_tanh(d[j] + P[P_B2 + nt * 16 + n]);
+        }
+    __syncthreads();
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 d = mfma_tile(4, [&](int i, int k) { return f2bf(s.h2[mt * 16 + i][k]); },
+                            [&](int k, int c) { return c < MLP_OUT ? f2bf(P[P_W3 + c * MLP_H + k]) : (short)0; });
+        if (n < MLP_OUT)
+            for (int j = 0; j < 4; j++) s.y[mt * 16 + ib + j][n] = d[j] + P[P_B3 + n];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Features and analytic default starts of one marker (shared by inference and
+// training-set extraction).  Returns false when neither solve is active.
+// ---------------------------------------------------------------------------
+struct MlpFeat {
+    float f[MLP_IN];
+    bool duct, rsf;           // which outputs apply
+    double eta0, vhi;         // default starts
+};
+
+__device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, MlpFeat &o)
+{
+    const int t = a.t[m];
+    const DevMat *M = a.mat;
+    const double sxx = MK(a, MF_XX, m), sxy = MK(a, MF_XY, m);
+    const double exx = MK(a, MF_EXX, m), exy = MK(a, MF_EXY, m);
+    const double tk = MK(a, MF_K, m), p = MK(a, MF_P, m);
+    const double sig = pow(sxx * sxx + sxy * sxy, 0.5);
+    const double eii = pow(exx * exx + exy * exy, 0.5);
+    o.duct = false;
+    o.rsf = false;
+    o.eta0 = 0;
+    o.vhi = 0;
+    if (power_law_type(M, t) && t != 1 && sig > 0) {
+        const double n = M->mm[t], mpb = 1e-5 * p;
+        double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * tk);
+        if (e1 > 150.0) e1 = 150.0;
+        e1 = exp(e1);
+        o.eta0 = 0.5 * sig * M->nu[t] * e1 / pow(sig, n);
+        o.duct = o.eta0 > 0 && isfinite(o.eta0);
+    }
+    double theta_l10 = 0.0, amb = 0.0;
+    if (a.fault[m]) {
+        const double ra = MXF(a, MX_A, m), rb = MXF(a, MX_B, m), rL = MXF(a, MX_L, m), rV0 = MXF(a, MX_V0, m);
+        const double mu0 = MXF(a, MX_MU0, m), phi = MXF(a, MX_PHI, m), Vp = MXF(a, MX_V, m);
+        const double nP = a.p.pmode == 0 ? a.p.nstress : p;
+        const double eta_rad = a.p.inertyn == 0 ? a.p.rad_damping : 0.0;
+        double phin;
+        if (Vp * dt / rL <= 1e-6) phin = log(exp(phi) * (1 - Vp * dt / rL) + rV0 * dt / rL);
+        else phin = log(rV0 / Vp + (exp(phi) - rV0 / Vp) * exp(-Vp * dt / rL));
+        if (sig > 0 && eta_rad > 0) {
+            double vhi = 2 * rV0 * sinh(sig / (ra * nP)) * exp(-(mu0 + rb * phin) / ra);
+            if (sig / eta_rad < vhi) vhi = sig / eta_rad;
+            o.vhi = vhi;
+            o.rsf = vhi > 0 && isfinite(vhi);
+        }
+        theta_l10 = (phi + log(rL / rV0)) / 2.302585092994046;   // theta = L e^phi / V0
+        amb = ra - rb;
+    }
+    o.f[0] = (float)log10(eii > 1e-30 ? eii : 1e-30);
+    o.f[1] = (float)tk;
+    o.f[2] = (float)p;
+    o.f[3] = (float)log10(sig > 1e-30 ? sig : 1e-30);
+    o.f[4] = (float)theta_l10;
+    o.f[5] = (float)amb;
+}
+
+// Inference: standardised features -> guesses written to MX_GETA / MX_GV
+// (0 where a solve does not apply, which the local update treats as "no
+// guess").
+__global__ __launch_bounds__(64) void k_mlp_guess(KArgs a, double dt, const float *__restrict__ P,
+                                                  const float *__restrict__ norm)
+{
+    __shared__ MlpLds s;
+    const int lane = threadIdx.x;
+    const long m = (long)blockIdx.x * 64 + lane;
+    MlpFeat ft;
+    ft.duct = ft.rsf = false;
+    if (m < a.n) mlp_features(a, m, dt, ft);
+    for (int k = 0; k < 16; k++) {
+        float v = 0.f;
+        if (k < MLP_IN && m < a.n) v = (ft.f[k] - norm[k]) / norm[MLP_IN + k];
+        s.x[lane][k] = f2bf(v);
+    }
+    __syncthreads();
+    mlp_forward(P, s);
+    if (m < a.n) {
+        const double r0 = s.y[lane][0] * norm[2 * MLP_IN + 2] + norm[2 * MLP_IN];
+        const double r1 = s.y[lane][1] * norm[2 * MLP_IN + 3] + norm[2 * MLP_IN + 1];
+        MXF(a, MX_GETA, m) = ft.duct ? ft.eta0 * pow(10.0, r0) : 0.0;
+        MXF(a, MX_GV, m) = ft.rsf ? ft.vhi * pow(10.0, r1) : 0.0;
+    }
+}
+
+// Training-set extraction.  Before a plain local update: features and the
+// log10 default starts into X[n][8] = f0..f5, log10 eta0, log10 vhi, with the
+// masks of the two outputs in Mask[n][2].  After it: k_mlp_labels turns the
+// last two columns into log10(root / default).
+__global__ void k_mlp_features(KArgs a, double dt, float *X, float *Mask)
+{
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    MlpFeat ft;
+    mlp_features(a, m, dt, ft);
+    for (int k = 0; k < MLP_IN; k++) X[m * 8 + k] = ft.f[k];
+    X[m * 8 + 6] = ft.duct ? (float)log10(ft.eta0) : 0.f;
+    X[m * 8 + 7] = ft.rsf ? (float)log10(ft.vhi) : 0.f;
+    Mask[m * 2 + 0] = ft.duct ? 1.f : 0.f;
+    Mask[m * 2 + 1] = ft.rsf ? 1.f : 0.f;
+}
+
+__global__ void k_mlp_labels(KArgs a, float *X, float *Mask)
+{
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    const double re = MXF(a, MX_ROOT_ETA, m), rv = MXF(a, MX_ROOT_V, m);
+    if (Mask[m * 2 + 0] != 0.f && re > 0 && isfinite(re)) X[m * 8 + 6] = (float)(log10(re) - (double)X[m * 8 + 6]);
+    else { Mask[m * 2 + 0] = 0.f; X[m * 8 + 6] = 0.f; }
+    if (Mask[m * 2 + 1] != 0.f && rv > 0 && isfinite(rv)) X[m * 8 + 7] = (float)(log10(rv) - (double)X[m * 8 + 7]);
+    else { Mask[m * 2 + 1] = 0.f; X[m * 8 + 7] = 0.f; }
+}
+
+// One training step, one wavefront per 64 samples: forward, masked MSE,
+// backward; writes this wave's gradient (sum over its rows) to
+// G[blockIdx.x][MLP_NPARAM] and its loss sum to L[blockIdx.x].
+// Xs: standardised inputs [nb][6]; Ts: standardised targets [nb][2];
+// Ms: masks [nb][2]; nb = rows of this batch; inv_cnt = 1 / (#masked terms).
+__global__ __launch_bounds__(64) void k_mlp_train(const float *__restrict__ P, const float *__restrict__ Xs,
+                                                  const float *__restrict__ Ts, const float *__restrict__ Ms,
+                                                  const uint32_t *__restrict__ idx, long nb, float inv_cnt,
+                                                  float *G, float *L)
+{
+    __shared__ MlpLds s;
+    __shared__ float msk[64][2];
+    const int lane = threadIdx.x;
+    const long rr = (long)blockIdx.x * 64 + lane;
+    const long r = rr < nb ? (long)idx[rr] : 0;
+    const bool live = rr < nb;
+    for (int k = 0; k < 16; k++) s.x[lane][k] = (k < MLP_IN && live) ? f2bf(Xs[r * MLP_IN + k]) : (short)0;
+    __syncthreads();
+    mlp_forward(P, s);
+    // dY = 2 m (y - t) / cnt
+    float loss = 0.f;
+    for (int o = 0; o < MLP_OUT; o++) {
+        const float mk = live ? Ms[r * 2 + o] : 0.f;
+        const float t = live ? Ts[r * 2 + o] : 0.f;
+        const float e = s.y[lane][o] - t;
+        loss += mk * e * e;
+        msk[lane][o] = mk;
+        s.y[lane][o] = 2.f * mk * e * inv_cnt;
+    }
+    __syncthreads();
+    float *g = G + (long)blockIdx.x * MLP_NPARAM;
+    const int n = lane & 15, ib = 4 * (lane >> 4);
+    // dW3[o][k] = sum_r dY[r][o] h2[r][k]  (M = o padded to 16, K = rows)
+    for (int nt = 0; nt < 4; nt++) {
+        f32x4 d = mfma_tile(4, [&](int i, int k) { return i < MLP_OUT ? f2bf(s.y[k][i]) : (short)0; },
+                            [&](int k, int c) { return f2bf(s.h2[k][nt * 16 + c]); });
+        for (int j = 0; j < 4; j++)
+            if (ib + j < MLP_OUT) g[P_W3 + (ib + j) * MLP_H + nt * 16 + n] = d[j];
+    }
+    if (lane < MLP_OUT) {
+        float b = 0.f;
+        for (int q = 0; q < 64; q++) b += s.y[q][lane];
+        g[P_B3 + lane] = b;
+    }
+    // dZ2 = (dY W3) * (1 - h2^2)   (K = outputs padded to 16)
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = mfma_tile(1, [&](int i, int k) { return k < MLP_OUT ? f2bf(s.y[mt * 16 + i][k]) : (short)0; },
+                                [&](int k, int c) { return k < MLP_OUT ? f2bf(P[P_W3 + k * MLP_H + nt * 16 + c]) : (short)0; });
+            for (int j = 0; j < 4; j++) {
+                const float h = s.h2[mt * 16 + ib + j][nt * 16 + n];
+                s.g[mt * 16 + ib + j][nt * 16 + n] = d[j] * (1.f - h * h);
+            }
+        }
+    __syncthreads();
+    // dW2[o][k] = sum_r dZ2[r][o] h1[r][k]
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = mfma_tile(4, [&](int i, int k) { return f2bf(s.g[k][mt * 16 + i]); },
+                                [&](int k, int c) { return f2bf(s.h1[k][nt * 16 + c]); });
+            for (int j = 0; j < 4; j++) g[P_W2 + (mt * 16 + ib + j) * MLP_H + nt * 16 + n] = d[j];
+        }
+    {
+        float b = 0.f;
+        for (int q = 0; q < 64; q++) b += s.g[q][lane];
+        g[P_B2 + lane] = b;
+    }
+    // dZ1 = (dZ2 W2) * (1 - h1^2); reuse h2 as the staging buffer
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = mfma_tile(4, [&](int i, int k) { return f2bf(s.g[mt * 16 + i][k]); },
+                                [&](int k, int c) { return f2bf(P[P_W2 + k * MLP_H + nt * 16 + c]); });
+            for (int j = 0; j < 4; j++) {
+                const float h = s.h1[mt * 16 + ib + j][nt * 16 + n];
+                s.h2[mt * 16 + ib + j][nt * 16 + n] = d[j] * (1.f - h * h);
+            }
+        }
+    __syncthreads();
+    // dW1[o][k] = sum_r dZ1[r][o] x[r][k]   (N = inputs padded to 16)
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 d = mfma_tile(4, [&](int i, int k) { return f2bf(s.h2[k][mt * 16 + i]); },
+                            [&](int k, int c) { return s.x[k][c]; });
+        if (n < MLP_IN)
+            for (int j = 0; j < 4; j++) g[P_W1 + (mt * 16 + ib + j) * MLP_IN + n] = d[j];
+    }
+    {
+        float b = 0.f;
+        for (int q = 0; q < 64; q++) b += s.h2[q][lane];
+        g[P_B1 + lane] = b;
+    }
+    // loss sum of the wave, fixed order
+    __syncthreads();
+    s.g[lane][0] = loss;
+    __syncthreads();
+    if (lane == 0) {
+        float t = 0.f;
+        for (int q = 0; q < 64; q++) t += s.g[q][0];
+        L[blockIdx.x] = t;
+    }
+}
+
+// Sum the per-wave gradients in wave order and take an Adam step on the FP32
+// master weights.  One thread per parameter.
+__global__ void k_mlp_adam(float *P, float *Mom, float *Var, const float *G, int nwaves, float lr, float b1,
+                           float b2, float eps, float bc1, float bc2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= MLP_NPARAM) return;
+    float g = 0.f;
+    for (int w = 0; w < nwaves; w++) g += G[(long)w * MLP_NPARAM + p];
+    const float m = b1 * Mom[p] + (1.f - b1) * g;
+    const float v = b2 * Var[p] + (1.f - b2) * g * g;
+    Mom[p] = m;
+    Var[p] = v;
+    P[p] -= lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+}
+
+// Gradient of one batch without an update (for the gradient check).
+__global__ void k_mlp_gradsum(const float *G, int nwaves, float *out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= MLP_NPARAM) return;
+    float g = 0.f;
+    for (int w = 0; w < nwaves; w++) g += G[(long)w * MLP_NPARAM + p];
+    out[p] = g;
+}
+
+// Forward on standardised inputs Xs[nb][6] -> Ys[nb][2] (for tests/evaluation).
+__global__ __launch_bounds__(64) void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ Xs,
+                                                    long nb, float *Ys)
+{
+    __shared__ MlpLds s;
+    const int lane = threadIdx.x;
+    const long r = (long)blockIdx.x * 64 + lane;
+    for (int k = 0; k < 16; k++) s.x[lane][k] = (k < MLP_IN && r < nb) ? f2bf(Xs[r * MLP_IN + k]) : (short)0;
+    __syncthreads();
+    mlp_forward(P, s);
+    if (r < nb) {
+        Ys[r * 2 + 0] = s.y[lane][0];
+        Ys[r * 2 + 1] = s.y[lane][1];
+    }
+}
+
+}  // namespace v2c

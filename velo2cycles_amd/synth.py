@@ -274,6 +274,7 @@ class Config:
     timesum: float
     geom: Geometry | None = None
     fault: np.ndarray | None = None     # bool mask of fault-zone markers
+    rsf: dict | None = None             # per-marker RSF inputs of the local update
 
     @property
     def n(self):
@@ -302,31 +303,28 @@ def build_config(name, gx, gy, per, geom, seed, timestep=YEAR, timesum=0.0, jitt
         e=rng.uniform(0.0, 0.1, n), phi=10.0 + 0.1 * rng.standard_normal(n),
         vx=vx, vy=vy, C=rng.uniform(0.0, 1e7, n), d=np.full(n, 3000.0), w=np.full(n, 1e7),
         t=t)
-    fault = None
+    fault = np.zeros(n, dtype=bool)
     if fault_frac > 0:
         dist = geom.slab_depth_below_top(x, y)
         cand = np.where((x > geom.xtrench) & (dist >= 0))[0]
         k = int(fault_frac * n)
         order = np.argsort(dist[cand], kind="stable")
-        fault = np.zeros(n, dtype=bool)
         fault[cand[order[:k]]] = True
-        a = rng.uniform(0.008, 0.02, n)
-        L = rng.uniform(0.01, 0.05, n)
-        V = 10.0 ** rng.uniform(-12.0, 0.0, n)
-        theta = L / V * rng.uniform(0.5, 2.0, n)
-        fields["a"] = np.where(fault, a, 0.0)
-        fields["b"] = np.where(fault, 0.015, 0.0)
-        fields["L"] = np.where(fault, L, 0.0)
-        fields["L0"] = fields["L"].copy()
-        fields["V0"] = np.where(fault, prm.V0_vw, 0.0)
-        fields["fric_static"] = np.where(fault, 0.6, 0.0)
-        fields["phi"] = np.where(fault, np.log(prm.V0_vw * theta / L), fields["phi"])
-        fields["_V"] = np.where(fault, V, 0.0)
+    # per-marker RSF state of the fault zone (used by the per-marker local
+    # update): a ~ U(0.008, 0.02), b = 0.015, L ~ U(0.01, 0.05) m, V
+    # log-uniform on [1e-12, 1] m/s, theta = L/V * U(0.5, 2), phi = ln(V0 theta / L)
+    a = rng.uniform(0.008, 0.02, n)
+    L = rng.uniform(0.01, 0.05, n)
+    V = 10.0 ** rng.uniform(-12.0, 0.0, n)
+    theta = L / V * rng.uniform(0.5, 2.0, n)
+    V0 = np.full(n, prm.V0_vw)
+    rsf = {"fault": fault.astype(np.uint8), "V": np.where(fault, V, 0.0),
+           "phi": np.where(fault, np.log(V0 * theta / L), 0.0), "a": np.where(fault, a, 0.0),
+           "b": np.where(fault, 0.015, 0.0), "L": np.where(fault, L, 1.0), "V0": np.where(fault, V0, 1.0),
+           "mu0": np.where(fault, 0.6, 0.0)}
     markers = MarkerArrays(n, fields)
-    if fault is not None:
-        markers.rsf_V = fields["_V"]
     nod = nodal_fields(gx, gy, geom, rng, timestep=timestep)
-    return Config(name, grid, prm, mats, markers, nod, timestep, timesum, geom, fault)
+    return Config(name, grid, prm, mats, markers, nod, timestep, timesum, geom, fault, rsf)
 
 
 def config_small(seed=DEFAULT_SEED, **kw):
@@ -334,6 +332,7 @@ def config_small(seed=DEFAULT_SEED, **kw):
     gx = uniform_axis(401, 400e3)
     gy = uniform_axis(101, 100e3)
     geom = Geometry(xtrench=150e3, ytop=8e3)
+    kw.setdefault("fault_frac", 0.02)
     return build_config("plain-small-401x101-16pc", gx, gy, 4, geom, seed, **kw)
 
 
