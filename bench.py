@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--config", default="production", choices=["production", "small"])
     ap.add_argument("--cpu-sample", type=int, default=3_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
@@ -94,6 +96,9 @@ def main():
     ctx.set_all_nodal(cfg.nodal)
     ctx.upload_markers(cfg.markers)
     h = ctx.h
+    if args.calib:
+        ms = C.c_double(0)
+        lib.v2c_copy_calibration(h, 1 << 30, C.byref(ms))
     ctx.set_async(True)
 
     def step(ts):
@@ -130,6 +135,13 @@ def main():
         (n * B_N2M_MARKER + nn * 24 * 8)
     avg_ms = kern[dom]["ms_total"] / max(kern[dom]["launches"], 1)
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        ent = tj.get(cfg.name, {}).get(dom)
+        if ent:
+            traffic = ent["hbm_bytes_per_launch"]
     out = {
         "metric": "marker-steps/sec", "value": n * args.steps / el, "unit": "marker-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -140,7 +152,7 @@ def main():
                    if args.config == "production" else "401x101 uniform"},
         "hbm_frac_step": bytes_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms},
         "kernels_ms_per_step": {k: v["ms_per_step"] for k, v in kern.items()},
     }

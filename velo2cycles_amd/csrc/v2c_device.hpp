@@ -266,20 +266,32 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
         double coeff = pow(eta_disl, 1.0 / n);
         double gdt = M->gg[t] * a.timestep;
         double mu_prev = 0, mu;
-        const double ex1 = (n - 1) / n, ex2 = -1 / n, c1 = (n - 1) / (n * eta_diff);
+        // Loop-invariant pieces of the iteration, and two exact rewrites:
+        // X^(-1/n) = X^((n-1)/n) / X, and eta_d^(2/(eta_d+eta)) evaluated as
+        // exp(2/(eta_d+eta) * ln eta_d) with ln eta_d hoisted.  The stop test
+        // |log10(mu_k/mu_{k-1})| < 1e-6 is applied as the equivalent bounds on
+        // the ratio (10^-1e-6, 10^1e-6).
+        const double ex1 = (n - 1) / n, c1 = (n - 1) / (n * eta_diff);
+        const double inv_ed = 1.0 / eta_diff, ln_ed = log(eta_diff);
+        const double r_lo = 0.999997697417558, r_hi = 1.000002302587744;
         int k = 0;
         for (;;) {
             if (k >= 50) break;
             k++;
-            double eta_old = eta_new;
-            mu = 1.0 / (1.0 / eta_diff + 1.0 / eta_new);
-            double pref = (eta_diff + eta_old) / eta_diff;
-            double X = pref * (gdt + mu);
-            double f1 = pow(X, ex1);
-            double f2 = eta_old - coeff * f1;
-            double df = 1 - c1 * coeff * pow(X, ex2) * ((gdt + mu) + pow(eta_diff, 2 / (eta_diff + eta_old)));
+            const double eta_old = eta_new;
+            mu = 1.0 / (inv_ed + 1.0 / eta_new);
+            const double pref = (eta_diff + eta_old) / eta_diff;
+            const double gm = gdt + mu;
+            const double X = pref * gm;
+            const double f1 = exp(ex1 * log(X));
+            const double f2 = eta_old - coeff * f1;
+            const double t3 = exp(2 / (eta_diff + eta_old) * ln_ed);
+            const double df = 1 - c1 * coeff * (f1 / X) * (gm + t3);
             eta_new = eta_old - f2 / df;
-            if (k > 3 && fabs(log10(mu / mu_prev)) < 1.0e-6) break;
+            if (k > 3) {
+                const double r = mu / mu_prev;
+                if (r > r_lo && r < r_hi) break;
+            }
             mu_prev = mu;
         }
         *iters = k;
