@@ -71,7 +71,14 @@ def test_local_update_warm_start_and_fallback(gpu):
     fault = cfg.rsf["fault"].astype(bool)
     _cmp(g, ow, fault)
     assert gst["fallbacks"] == owst["fallbacks"]
-    assert gst["rsf_iters"] <= ost["rsf_iters"]
+    # a 1 % perturbation can start on the far side of the root: the oracle
+    # counts 658 fall-backs here; the device must take the same paths
+    for k in ("ductile_iters", "rsf_iters", "ductile_evals", "rsf_evals"):
+        assert abs(gst[k] - owst[k]) <= max(2, 1e-4 * owst[k]), k
+    # the converged roots as guesses: no Newton update is needed anywhere
+    gr, grst = _gpu(gpu, cfg, True, o["root_eta"], o["root_V"])
+    _cmp(gr, o, fault)
+    assert grst["ductile_iters"] == 0 and grst["rsf_iters"] == 0
     bad_eta = np.full(cfg.n, 1e60)
     bad_V = np.full(cfg.n, 1e-300)
     ob, obst, _, _ = _oracle(cfg, True, bad_eta, bad_V)
