@@ -25,24 +25,31 @@ sys.path.insert(0, REPO)
 import numpy as np  # noqa: E402
 
 from velo2cycles_amd import synth  # noqa: E402
-from velo2cycles_amd.abi import Context, load_lib  # noqa: E402
+from velo2cycles_amd.abi import KERNELS, Context, load_lib  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per marker (FP64 = 8 B, type = 1 B), REFERENCE mode:
-#  marker2nodes reads x y T P sxx sxy exx exy e phi vx vy C (13x8) + type,
-#    writes the 7 calc_frictionpara outputs (7x8)                 -> 105 R + 56 W
-#  nodes2markers+advection reads x y T + type, writes phi exx exy
-#    sxx sxy P vx vy e x y (11x8)                                 ->  25 R + 88 W
-B_M2N_MARKER = 105 + 56
-B_N2M_MARKER = 25 + 88
-# Nodal traffic per node and step (each nodal array touched once): marker2nodes
-# reads 19 fields and writes 50; nodes2markers+advection reads 24 and
-# read-writes 15 sums + writes 9; the two solve-side passes read ~20 and write
-# ~12 fields.
-B_M2N_NODE = (19 + 50) * 8
-B_N2M_NODE = (24 + 15 * 2 + 9) * 8
-B_SOLVE_NODE = 32 * 8
+# Algorithmic bytes (FP64 = 8 B, type = 1 B): what each kernel must read and
+# write for its function, excluding the build's own intermediates (sort
+# order, keys, per-cell partials, per-marker property hand-over).
+#   m2n_props   reads x y T sxx sxy phi + type; writes the calc_frictionpara
+#               outputs (6 fields for the markers outside the fault band)
+#               + nodal pr                                   -> 49 R + 48 W
+#   m2n_cells   reads x y T P sxx sxy exx exy e phi vx vy C + type   -> 105 R
+#   n2m_scatter reads x y + type + 9 nodal fields                    ->  17 R
+#   n2m_update  reads x y T + type; writes phi exx exy sxx sxy P vx vy e x y
+#               + 19 nodal fields                            -> 25 R + 88 W
+# Per node: the unique nodal arrays each kernel touches.
+ALG = {
+    "m2n_props": (49 + 48, 1 * 8),
+    "m2n_cells": (105, 0),
+    "n2m_cells": (17, 9 * 8),
+    "n2m_update": (25 + 88, 19 * 8),
+}
+# Whole step: marker2nodes 161 B and nodes2markers + advection 113 B per
+# marker; ~360 B per node over all passes (each nodal array touched once).
+B_STEP_MARKER = 161 + 113
+B_STEP_NODE = 360
 
 
 def cpu_baseline(cfg, sample: int, steps: int = 2):
@@ -124,15 +131,15 @@ def main():
     el = time.perf_counter() - t0
     ctx.enable_timing(False)
     kern = {}
-    for k in ("m2n_cells", "m2n_nodes", "n2m_cells", "n2m_nodes", "sort", "before", "after"):
+    for k in KERNELS:
         ms, nl = ctx.kernel_time(k)
-        kern[k] = {"ms_total": ms, "launches": nl, "ms_per_step": ms / args.steps}
+        if nl:
+            kern[k] = {"ms_total": ms, "launches": nl, "ms_per_step": ms / args.steps}
     n, nn = cfg.n, cfg.grid.nodenum
     ms_step = el / args.steps * 1e3
-    bytes_step = (n * (B_M2N_MARKER + B_N2M_MARKER) + nn * (B_M2N_NODE + B_N2M_NODE + B_SOLVE_NODE))
-    dom = max(("m2n_cells", "n2m_cells"), key=lambda k: kern[k]["ms_total"])
-    per_launch_bytes = (n * B_M2N_MARKER + nn * 8) if dom == "m2n_cells" else \
-        (n * B_N2M_MARKER + nn * 24 * 8)
+    bytes_step = n * B_STEP_MARKER + nn * B_STEP_NODE
+    dom = max((k for k in ALG if k in kern), key=lambda k: kern[k]["ms_total"])
+    per_launch_bytes = n * ALG[dom][0] + nn * ALG[dom][1]
     avg_ms = kern[dom]["ms_total"] / max(kern[dom]["launches"], 1)
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
     traffic = None

@@ -298,6 +298,7 @@ int v2c_mlp_forward(v2c_ctx *ctx, const float *Xs, long ns, float *Ys);
 enum v2c_kernel_id {
     V2C_K_M2N_CELLS = 0, V2C_K_M2N_NODES, V2C_K_N2M_CELLS, V2C_K_N2M_NODES,
     V2C_K_ADVECT, V2C_K_SORT, V2C_K_BEFORE, V2C_K_AFTER, V2C_K_LOCAL, V2C_K_MLP,
+    V2C_K_M2N_PROPS, V2C_K_N2M_UPDATE,
     V2C_K_COUNT
 };
 int v2c_enable_timing(v2c_ctx *ctx, int on);

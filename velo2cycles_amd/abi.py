@@ -38,7 +38,7 @@ FIELD_ID = {n: i for i, n in enumerate(NODAL_FIELDS)}
 MARKER_FIELDS = ("x y k p xx xy exx exy e phi vx vy C d w a b L L0 V0 fric_static C0").split()
 
 KERNELS = ("m2n_cells", "m2n_nodes", "n2m_cells", "n2m_nodes", "advect", "sort",
-           "before", "after", "local", "mlp")
+           "before", "after", "local", "mlp", "m2n_props", "n2m_update")
 
 ERRORS = {0: "ok", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECAPACITY",
           -5: "EUNSUPPORTED", -6: "ENOGPU"}

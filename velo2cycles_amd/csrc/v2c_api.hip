@@ -33,11 +33,14 @@ struct v2c_ctx {
     long last_movers = 0, n_full_sorts = 0, n_incremental = 0;
     // device memory
     double *d_gx = nullptr, *d_gy = nullptr;
+    AxisBin *d_bx = nullptr, *d_by = nullptr;
+    double bx0 = 0, by0 = 0, bxinv = 0, byinv = 0;
+    int nbx = 0, nby = 0;
     DevMat *d_mat = nullptr;
     double *d_mk = nullptr;
     int8_t *d_t = nullptr;
     uint32_t *d_key = nullptr, *d_skey = nullptr, *d_iota = nullptr, *d_order = nullptr, *d_cs = nullptr;
-    double *d_nod = nullptr, *d_part = nullptr, *d_red = nullptr;
+    double *d_nod = nullptr, *d_part = nullptr, *d_red = nullptr, *d_prop = nullptr;
     int *d_counters = nullptr;
     // incremental re-ordering buffers
     uint32_t *d_order2 = nullptr, *d_skey2 = nullptr, *d_ids_mv = nullptr, *d_ids_st = nullptr;
@@ -138,12 +141,16 @@ static KArgs make_args(v2c_ctx *c)
     a.g.gx = c->d_gx; a.g.gy = c->d_gy;
     a.g.xnumx = (int)c->xnumx; a.g.ynumy = (int)c->ynumy;
     a.g.xsize = c->xsize; a.g.ysize = c->ysize;
+    a.g.bx = c->d_bx; a.g.by = c->d_by;
+    a.g.bx0 = c->bx0; a.g.by0 = c->by0; a.g.bxinv = c->bxinv; a.g.byinv = c->byinv;
+    a.g.nbx = c->nbx; a.g.nby = c->nby;
     a.p = c->prm;
     a.mat = c->d_mat;
     a.mk = c->d_mk; a.cap = c->cap; a.t = c->d_t; a.n = c->n;
     a.order = c->d_order; a.cell_start = c->d_cs; a.key = c->d_key;
     a.nod = c->d_nod; a.nodenum = c->nodenum; a.ncell = c->ncell;
     a.part = c->d_part;
+    a.prop = c->d_prop;
     a.timesum = c->timesum; a.timestep = c->timestep;
     a.counters = c->d_counters;
     a.write_friction = c->write_friction;
@@ -292,8 +299,8 @@ const char *v2c_last_error(const v2c_ctx *c) { return c ? c->err : "null context
 
 static void free_all(v2c_ctx *c)
 {
-    void *ptrs[] = {c->d_gx, c->d_gy, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
-                    c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_red, c->d_counters, c->d_sort_tmp,
+    void *ptrs[] = {c->d_gx, c->d_gy, c->d_bx, c->d_by, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
+                    c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_prop, c->d_red, c->d_counters, c->d_sort_tmp,
                     c->d_order2, c->d_skey2, c->d_ids_mv, c->d_ids_st, c->d_flag_mv, c->d_flag_st,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
                     c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm};
@@ -354,6 +361,7 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_cs, sizeof(uint32_t) * (c->ncell + 1)));
     CK(hipMalloc(&c->d_nod, sizeof(double) * V2C_NF * c->nodenum));
     CK(hipMalloc(&c->d_part, sizeof(double) * MP_N * c->ncell));
+    CK(hipMalloc(&c->d_prop, sizeof(double) * NPROP * c->cap));
     CK(hipMalloc(&c->d_red, sizeof(double) * (2 * blocks_for(c->nodenum, RED_BLOCK) + 8)));
     CK(hipMalloc(&c->d_counters, sizeof(int) * 8));
     CK(hipMalloc(&c->d_order2, sizeof(uint32_t) * c->cap));
@@ -383,6 +391,17 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMemsetAsync(c->d_counters, 0, sizeof(int) * 8, c->stream));
     CK(hipMemcpyAsync(c->d_gx, g->gx, sizeof(double) * g->xnumx, hipMemcpyHostToDevice, c->stream));
     CK(hipMemcpyAsync(c->d_gy, g->gy, sizeof(double) * g->ynumy, hipMemcpyHostToDevice, c->stream));
+    {
+        std::vector<AxisBin> bx = make_bins(g->gx, (int)g->xnumx, c->bx0, c->bxinv);
+        std::vector<AxisBin> by = make_bins(g->gy, (int)g->ynumy, c->by0, c->byinv);
+        c->nbx = (int)bx.size();
+        c->nby = (int)by.size();
+        CK(hipMalloc(&c->d_bx, sizeof(AxisBin) * bx.size()));
+        CK(hipMalloc(&c->d_by, sizeof(AxisBin) * by.size()));
+        CK(hipMemcpyAsync(c->d_bx, bx.data(), sizeof(AxisBin) * bx.size(), hipMemcpyHostToDevice, c->stream));
+        CK(hipMemcpyAsync(c->d_by, by.data(), sizeof(AxisBin) * by.size(), hipMemcpyHostToDevice, c->stream));
+        CK(hipStreamSynchronize(c->stream));
+    }
     {
         DevMat dm;
         memcpy(dm.nu, mat->nu, sizeof dm.nu); memcpy(dm.dh, mat->dh, sizeof dm.dh);
@@ -536,6 +555,13 @@ int v2c_marker2nodes(v2c_ctx *c)
     int rc = ensure_order(c);
     if (rc) return rc;
     KArgs a = make_args(c);
+    if (c->n > 0) {
+        const int nb = (int)blocks_for(c->n, 256);
+        rc = launch(c, V2C_K_M2N_PROPS, [&] {
+            hipLaunchKernelGGL(k_m2n_props, dim3(nb), dim3(256), 0, c->stream, a, nb);
+        });
+        if (rc) return rc;
+    }
     const int nwg = (int)((c->ncell + CPW - 1) / CPW);
     rc = launch(c, V2C_K_M2N_CELLS, [&] {
         hipLaunchKernelGGL(k_m2n_cells, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
@@ -565,27 +591,39 @@ static int n2m_impl(v2c_ctx *c, bool advect)
 {
     if (!c) return V2C_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_order(c);
-    if (rc) return rc;
+    const bool anti = c->prm.antidiffinter == 1;
+    int rc = 0;
+    if (anti) {
+        rc = ensure_order(c);
+        if (rc) return rc;
+    }
     KArgs a = make_args(c);
     if (advect) HIPCHK(c, hipMemsetAsync(c->d_counters + 1, 0, sizeof(int), c->stream));
-    if (c->prm.n2m_phantom && c->prm.antidiffinter == 1) {
+    if (c->prm.n2m_phantom && anti) {
         rc = launch(c, V2C_K_N2M_NODES, [&] {
             hipLaunchKernelGGL(k_n2m_phantom, dim3(1), dim3(1), 0, c->stream, a);
         });
         if (rc) return rc;
     }
-    const int nwg = (int)((c->ncell + CPW - 1) / CPW);
-    rc = launch(c, V2C_K_N2M_CELLS, [&] {
-        if (advect)
-            hipLaunchKernelGGL(k_n2m_cells<true>, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
-        else
-            hipLaunchKernelGGL(k_n2m_cells<false>, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
-    });
-    if (rc) return rc;
-    if (c->prm.antidiffinter == 1) {
+    if (anti) {
+        // the scatter reads positions before the update moves any marker
+        const int nwg = (int)((c->ncell + CPW - 1) / CPW);
+        rc = launch(c, V2C_K_N2M_CELLS, [&] {
+            hipLaunchKernelGGL(k_n2m_scatter, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
+        });
+        if (rc) return rc;
         rc = launch(c, V2C_K_N2M_NODES, [&] {
             hipLaunchKernelGGL(k_n2m_nodes, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, c->stream, a);
+        });
+        if (rc) return rc;
+    }
+    if (c->n > 0) {
+        const int nb = (int)blocks_for(c->n, 256);
+        rc = launch(c, V2C_K_N2M_UPDATE, [&] {
+            if (advect)
+                hipLaunchKernelGGL(k_n2m_update<true>, dim3(nb), dim3(256), 0, c->stream, a, nb);
+            else
+                hipLaunchKernelGGL(k_n2m_update<false>, dim3(nb), dim3(256), 0, c->stream, a, nb);
         });
         if (rc) return rc;
     }

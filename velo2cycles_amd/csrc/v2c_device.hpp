@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "../../include/v2c.h"
+#include "v2c_search.hpp"
 
 namespace v2c {
 
@@ -42,6 +43,11 @@ struct DevGrid {
     const double *gx, *gy;
     int xnumx, ynumy;
     double xsize, ysize;
+    // search tables: bins of width h <= the smallest spacing from the first
+    // gridline; bin j = (x - b0) * binv
+    const AxisBin *bx, *by;
+    double bx0, by0, bxinv, byinv;
+    int nbx, nby;
 };
 
 // Everything a kernel needs, passed by value.
@@ -60,6 +66,7 @@ struct KArgs {
     long nodenum;
     long ncell;
     double *part;          // per-cell partials, field-major
+    double *prop;          // per-marker properties of marker2nodes (NPROP x cap)
     double timesum, timestep;
     int *counters;         // device counters (errors, unsupported types)
     int write_friction;    // write the calc_frictionpara marker outputs
@@ -82,16 +89,13 @@ __device__ __forceinline__ double clampd(double v, double lo, double hi)
     return v;
 }
 
-// Cell search by bisection (interpolation.c:1536-1573): largest gridline index
-// left of x, clamped to the last cell; NaN lands in the last cell.
-__device__ __forceinline__ int bisect_left(const double *__restrict__ g, int n, double x)
+__device__ __forceinline__ int find_nx(const DevGrid &g, double x)
 {
-    int lo = 0, hi = n - 1;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (g[mid] > x) hi = mid; else lo = mid;
-    }
-    return lo > n - 2 ? n - 2 : lo;
+    return axis_find(g.gx, g.xnumx, g.bx, g.nbx, g.bx0, g.bxinv, x);
+}
+__device__ __forceinline__ int find_ny(const DevGrid &g, double y)
+{
+    return axis_find(g.gy, g.ynumy, g.by, g.nby, g.by0, g.byinv, y);
 }
 
 struct Stencil { long m3; double e, n; };
@@ -191,7 +195,7 @@ __device__ __forceinline__ uint32_t subcell_key(const DevGrid &g, double x, doub
 {
     x = clampd(x, 0, g.xsize);
     y = clampd(y, 0, g.ysize);
-    int nx = bisect_left(g.gx, g.xnumx, x), ny = bisect_left(g.gy, g.ynumy, y);
+    int nx = find_nx(g, x), ny = find_ny(g, y);
     double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
     double cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny]);
     uint32_t q = (cx > 0.5 ? 2u : 0u) | (cy > 0.5 ? 1u : 0u);
@@ -387,8 +391,9 @@ __device__ __forceinline__ void friction_node(const KArgs &a, double x, double y
 
 // allinteri (interpolation.c:1688-1758): velocity, spin and strain-rate
 // invariant at a point.
+// (hx, hy) >= 0: the cell of the clamped point is already known.
 __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, double &VX, double &VY,
-                                          double &ESP, double &EE)
+                                          double &ESP, double &EE, int hx = -1, int hy = -1)
 {
     const DevGrid &g = a.g;
     const double *gx = g.gx, *gy = g.gy;
@@ -400,7 +405,7 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
     if (x > (gx[g.xnumx - 2] + gx[g.xnumx - 1]) / 2.0) xrat = 1.0;
     if (y < (gy[0] + gy[1]) / 2.0) xrat = 1.0;
     if (y > (gy[ynumy - 2] + gy[ynumy - 1]) / 2.0) xrat = 1.0;
-    int nx = bisect_left(gx, g.xnumx, x), ny = bisect_left(gy, ynumy, y);
+    const int nx = hx >= 0 ? hx : find_nx(g, x), ny = hy >= 0 ? hy : find_ny(g, y);
     const double *vx = NOD(a, V2C_N_vx), *vy = NOD(a, V2C_N_vy);
 
     Stencil s = basic_stencil(g, nx, ny, x, y);

@@ -138,34 +138,89 @@ __global__ void k_merge_movers(const uint64_t *__restrict__ mov, long nmov, cons
 }
 
 // ---------------------------------------------------------------------------
-// Segmented reduction helper: lanes hold NF contributions in con[f][lane] and
-// a local slot id in skey[lane] (runs of equal slot ids are contiguous); the
-// head lane of each run adds the run total to acc[slot][f].
+// Segmented reduction of one batch: lanes [0, nact) hold NF contributions in
+// con[f][lane] and a local slot id in skey[lane], runs of equal slot ids being
+// contiguous.  Work items (run, field) are spread over the lanes; each sums
+// its run serially in lane order and adds the total to acc[slot][f].  A
+// (slot, field) pair occurs once per batch, so no two lanes touch the same
+// accumulator, and the summation order is fixed.
 // ---------------------------------------------------------------------------
 template <int NF>
-__device__ __forceinline__ void run_reduce(double (*con)[WAVE], const uint32_t *skey, double (*acc)[NF],
-                                           int lane, bool active)
+__device__ __forceinline__ void run_reduce(const double (*con)[WAVE], const uint32_t *skey, double (*acc)[NF],
+                                           int *runs, int lane, int nact)
 {
-    uint32_t key = skey[lane];
-    if (active && (lane == 0 || skey[lane - 1] != key)) {
-        int j = lane + 1;
-        while (j < WAVE && skey[j] == key) j++;
-        for (int f = 0; f < NF; f++) {
-            double s = 0.0;
-            for (int r = lane; r < j; r++) s += con[f][r];
-            acc[key][f] += s;
-        }
+    const bool head = lane < nact && (lane == 0 || skey[lane - 1] != skey[lane]);
+    const unsigned long long hm = __ballot(head);
+    const int nr = __popcll(hm);
+    if (head) runs[__popcll(hm & ((1ull << lane) - 1ull))] = lane;
+    if (lane == 0) runs[nr] = nact;
+    __syncthreads();
+    for (int w = lane; w < nr * NF; w += WAVE) {
+        const int f = w / nr, r = w - f * nr;
+        const int b = runs[r], e = runs[r + 1];
+        double sum = 0.0;
+        for (int j = b; j < e; j++) sum += con[f][j];
+        acc[skey[b]][f] += sum;
     }
 }
 
+// Cell indices of a marker inside the grid from its sub-cell key (the key is
+// computed by the same search from the same clamped position).
+__device__ __forceinline__ void key_cell(const KArgs &a, uint32_t key, int &nx, int &ny)
+{
+    const uint32_t cell = key >> 2, ncy = (uint32_t)(a.g.ynumy - 1);
+    nx = (int)(cell / ncy);
+    ny = (int)(cell - (uint32_t)nx * ncy);
+}
+
 // ---------------------------------------------------------------------------
-// marker2nodes, marker pass (interpolation.c:215-629)
+// marker2nodes, per-marker properties (interpolation.c:262-330) in physical
+// marker order: pressure at the marker, density, expansivity, conductivity,
+// viscosity (the FP64 quasi-Newton of ductilevisc) and the friction reset of
+// calc_frictionpara.  Separate from the scatter so the transcendental-heavy
+// part runs at full occupancy with coalesced access.
+// ---------------------------------------------------------------------------
+enum MProp { P_NU = 0, P_RO, P_BB, P_KT, NPROP };
+
+__global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
+{
+    const long m = (long)xcd_remap(blockIdx.x, nwg) * 256 + threadIdx.x;
+    if (m >= a.n) return;
+    const DevGrid &g = a.g;
+    const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
+    if (!(x > 0 && y > 0 && x < g.xsize && y < g.ysize)) return;
+    int nx, ny;
+    key_cell(a, a.key[m], nx, ny);
+    const int t = a.t[m];
+    const double mtk = MK(a, MF_K, m);
+    const Stencil ps = centre_stencil(g, nx, ny, x, y);   // allinterpomp (interpolation.c:1923-1958)
+    const double mpb = 1e-5 * bil(NOD(a, V2C_N_pr), ps.m3, g.ynumy, wts(ps));
+    double mro, mbb;
+    dencalc(a, mtk, mpb, t, mro, mbb);
+    const double mkt = conductivity(a.mat, t, mtk, mpb);
+    int iters;
+    const double mnu = ductilevisc(a, t, MK(a, MF_XY, m), MK(a, MF_XX, m), mtk, mpb, &iters);
+    const double phi0 = MK(a, MF_PHI, m);
+    const double phi = friction_marker(a, m, t, x, y, phi0);
+    if (__double_as_longlong(phi) != __double_as_longlong(phi0)) MK(a, MF_PHI, m) = phi;
+    double *P = a.prop;
+    P[(long)P_NU * a.cap + m] = mnu;
+    P[(long)P_RO * a.cap + m] = mro;
+    P[(long)P_BB * a.cap + m] = mbb;
+    P[(long)P_KT * a.cap + m] = mkt;
+}
+
+// ---------------------------------------------------------------------------
+// marker2nodes, scatter pass (interpolation.c:215-629) over the sorted order:
+// weights and contributions of every marker inside the grid, reduced per
+// (cell, quadrant) and written as per-cell partials.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
 {
     __shared__ double acc[CPW * 4][M_NF];
     __shared__ double con[M_NF][WAVE];
     __shared__ uint32_t skey[WAVE];
+    __shared__ int runs[WAVE + 1];
     const int lane = threadIdx.x;
     const int wg = xcd_remap(blockIdx.x, nwg);
     const long c0 = (long)wg * CPW;
@@ -174,35 +229,32 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
     const uint32_t beg = a.cell_start[c0], end = a.cell_start[c1];
     const DevMat *M = a.mat;
     const DevGrid &g = a.g;
+    const double *P = a.prop;
     __syncthreads();
     for (uint32_t base = beg; base < end; base += WAVE) {
         const uint32_t i = base + lane;
         const bool active = i < end;
+        const int nact = end - base < (uint32_t)WAVE ? (int)(end - base) : WAVE;
         double v[M_NF];
 #pragma unroll
         for (int f = 0; f < M_NF; f++) v[f] = 0.0;
         uint32_t slot = 0xffffffffu;
         if (active) {
             const long m = a.order[i];
-            slot = a.key[m] - (uint32_t)(c0 << 2);
+            const uint32_t key = a.key[m];
+            slot = key - (uint32_t)(c0 << 2);
             const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
             if (x > 0 && y > 0 && x < g.xsize && y < g.ysize) {
-                const int nx = bisect_left(g.gx, g.xnumx, x), ny = bisect_left(g.gy, g.ynumy, y);
+                int nx, ny;
+                key_cell(a, key, nx, ny);
                 const int t = a.t[m];
                 const double mbm = a.p.bulk_modulus;
                 const double mgg = M->gg[t];
                 const double mtk = MK(a, MF_K, m);
-                Stencil ps = centre_stencil(g, nx, ny, x, y);   // allinterpomp (interpolation.c:1923-1958)
-                const double mpb = 1e-5 * bil(NOD(a, V2C_N_pr), ps.m3, g.ynumy, wts(ps));
-                double mro, mbb;
-                dencalc(a, mtk, mpb, t, mro, mbb);
-                const double mkt = conductivity(M, t, mtk, mpb);
+                const double mnu = P[(long)P_NU * a.cap + m], mro = P[(long)P_RO * a.cap + m];
+                const double mbb = P[(long)P_BB * a.cap + m], mkt = P[(long)P_KT * a.cap + m];
                 const double sxy = MK(a, MF_XY, m), sxx = MK(a, MF_XX, m);
-                int iters;
-                const double mnu = ductilevisc(a, t, sxy, sxx, mtk, mpb, &iters);
-                const double phi0 = MK(a, MF_PHI, m);
-                const double phi = friction_marker(a, m, t, x, y, phi0);
-                if (__double_as_longlong(phi) != __double_as_longlong(phi0)) MK(a, MF_PHI, m) = phi;
+                const double phi = MK(a, MF_PHI, m);
                 const double me = MK(a, MF_E, m);
                 const double swt1 = 1.0 / (g.gx[nx + 1] - g.gx[nx]) / (g.gy[ny + 1] - g.gy[ny]);
                 const double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
@@ -245,7 +297,7 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
         for (int f = 0; f < M_NF; f++) con[f][lane] = v[f];
         skey[lane] = slot;
         __syncthreads();
-        run_reduce<M_NF>(con, skey, acc, lane, active);
+        run_reduce<M_NF>(con, skey, acc, runs, lane, nact);
         __syncthreads();
     }
     // per-cell partials
@@ -442,15 +494,11 @@ struct Mech {
 };
 
 // interpolate_mechanical gathers (interpolation.c:1060-1092, 1211-1237,
-// 1376-1397, 1433-1454) and the antidiffusion scatter-back contributions.
-__device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, double mro, Mech &o,
-                                            double *v /* N_NF, zeroed */)
+// 1376-1397, 1433-1454) at a clamped position in cell (nx, ny).
+__device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, int nx, int ny, Mech &o)
 {
     const DevGrid &g = a.g;
     const int ynumy = g.ynumy;
-    x = clampd(x, 0, g.xsize);
-    y = clampd(y, 0, g.ysize);
-    const int nx = bisect_left(g.gx, g.xnumx, x), ny = bisect_left(g.gy, ynumy, y);
     Stencil s = basic_stencil(g, nx, ny, x, y);
     W4 w = wts(s);
     o.EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
@@ -477,7 +525,32 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
     w = wts(s);
     o.VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, w);
     o.MVY = bil(NOD(a, V2C_N_mvy), s.m3, ynumy, w);
-    if (a.p.antidiffinter != 1) return;
+}
+
+// Antidiffusion scatter-back contributions of one marker (interpolation.c:
+// 1159-1209, 1360-1374, 1405-1489): the interpolated PHIS, SXY, EXY, ES, PR,
+// SXX, EXX, VX, VY times the node weights, at the clamped position in cell
+// (nx, ny).  Only these nine interpolants are gathered.
+__device__ __forceinline__ void anti_contrib(const KArgs &a, double x, double y, int nx, int ny, double mro,
+                                             double *v /* N_NF, zeroed */)
+{
+    const DevGrid &g = a.g;
+    const int ynumy = g.ynumy;
+    Stencil s = basic_stencil(g, nx, ny, x, y);
+    W4 w = wts(s);
+    const double EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
+    const double SXY = bil(NOD(a, V2C_N_sxy), s.m3, ynumy, w);
+    const double ES = bil(NOD(a, V2C_N_pes), s.m3, ynumy, w);
+    const double PHIS = bil(NOD(a, V2C_N_phis_it), s.m3, ynumy, w);
+    s = centre_stencil(g, nx, ny, x, y);
+    w = wts(s);
+    const double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
+    const double SXX = bil(NOD(a, V2C_N_sxx), s.m3, ynumy, w);
+    const double PR = bil(NOD(a, V2C_N_pr), s.m3, ynumy, w);
+    s = vx_stencil(g, nx, ny, x, y, false);
+    const double VX = bil(NOD(a, V2C_N_vx), s.m3, ynumy, wts(s));
+    s = vy_stencil(g, nx, ny, x, y, false);
+    const double VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, wts(s));
     const double swt1 = 1.0 / (g.gx[nx + 1] - g.gx[nx]) / (g.gy[ny + 1] - g.gy[ny]);
     const double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
     const double cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny]);
@@ -488,20 +561,20 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
     else if (cx > 0.5 && cy < 0.5) bw = swt1 * (2.0 * cx - 1.0) * (1.0 - 2.0 * cy);
     else if (cx > 0.5 && cy > 0.5) bw = swt1 * (2.0 * cx - 1.0) * (2.0 * cy - 1.0);
     else hasb = false;
-    if (hasb) { v[0] = o.PHIS * bw; v[1] = o.SXY * bw; v[2] = o.EXY * bw; v[3] = o.ES * bw; v[4] = bw; }
+    if (hasb) { v[0] = PHIS * bw; v[1] = SXY * bw; v[2] = EXY * bw; v[3] = ES * bw; v[4] = bw; }
     const double cw = swt1 * (1.0 - 2 * ref_abs(cx - 0.5)) * (1.0 - 2 * ref_abs(cy - 0.5));
-    v[N_C] = o.PR * cw; v[N_C + 1] = o.SXX * cw; v[N_C + 2] = o.EXX * cw; v[N_C + 3] = cw;
+    v[N_C] = PR * cw; v[N_C + 1] = SXX * cw; v[N_C + 2] = EXX * cw; v[N_C + 3] = cw;
     double vw = 0.0;
     bool hasv = true;
     if (cx < 0.5) vw = swt1 * (1.0 - cx) * (1.0 - ref_abs(cy - 0.5));
     else if (cx > 0.5) vw = swt1 * cx * (1.0 - ref_abs(cy - 0.5));
     else hasv = false;
-    if (hasv) { v[N_VX] = o.VX * mro * vw; v[N_VX + 1] = mro * vw; v[N_VX + 2] = vw; }
+    if (hasv) { v[N_VX] = VX * mro * vw; v[N_VX + 1] = mro * vw; v[N_VX + 2] = vw; }
     hasv = true;
     if (cy < 0.5) vw = swt1 * (1.0 - ref_abs(cx - 0.5)) * (1.0 - cy);
     else if (cy > 0.5) vw = swt1 * (1.0 - ref_abs(cx - 0.5)) * cy;
     else hasv = false;
-    if (hasv) { v[N_VY] = o.VY * mro * vw; v[N_VY + 1] = mro * vw; v[N_VY + 2] = vw; }
+    if (hasv) { v[N_VY] = VY * mro * vw; v[N_VY + 1] = mro * vw; v[N_VY + 2] = vw; }
 }
 
 // Marker state of one marker, loaded once and written once.
@@ -541,7 +614,9 @@ __device__ __forceinline__ void n2m_update(const KArgs &a, int t, MState &s, con
 
 // moveandrotatemarkers body for one marker of type < 100 (moveandrotatemarkers.c:12-88).
 // Returns false (and changes nothing) for types >= 100.
-__device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, bool &moved, bool &reset)
+// (hx, hy): the cell of the marker's clamped position, or -1 to search.
+__device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, bool &moved, bool &reset,
+                                           int hx = -1, int hy = -1)
 {
     const DevGrid &g = a.g;
     const v2c_params &p = a.p;
@@ -553,10 +628,10 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
     if (t >= 100) return false;
     double vx0, vy0, sp0, ee0;
     if (p.markmod == 1) {
-        allinteri(a, s.x, s.y, vx0, vy0, sp0, ee0);
+        allinteri(a, s.x, s.y, vx0, vy0, sp0, ee0, hx, hy);
     } else {
         double vx1, vy1, sp1, ee1, vx2, vy2, sp2, ee2, vx3, vy3, sp3, ee3, vx4, vy4, sp4, ee4;
-        allinteri(a, s.x, s.y, vx1, vy1, sp1, ee1);
+        allinteri(a, s.x, s.y, vx1, vy1, sp1, ee1, hx, hy);
         allinteri(a, s.x + vx1 * dt / 2.0, s.y + vy1 * dt / 2.0, vx2, vy2, sp2, ee2);
         allinteri(a, s.x + vx2 * dt / 2.0, s.y + vy2 * dt / 2.0, vx3, vy3, sp3, ee3);
         allinteri(a, s.x + vx3 * dt, s.y + vy3 * dt, vx4, vy4, sp4, ee4);
@@ -580,7 +655,9 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
     const double th = sp0 * dt;
     const double sxx = s.xx, sxy = s.xy;
     if (p.stoksmod == 1) {
-        const double c = cos(th), sn = sin(th), s2 = sin(2.0 * th), c2 = cos(2.0 * th);
+        double c, sn, s2, c2;
+        sincos(th, &sn, &c);
+        sincos(2.0 * th, &s2, &c2);
         s.xx = sxx * c * c - sxx * sn * sn + sxy * s2;
         s.xy = 0.5 * (-sxx - sxx) * s2 + sxy * c2;
     } else if (p.stoksmod == 2) {
@@ -594,85 +671,47 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
     return true;
 }
 
-// Fused (ADVECT) or plain nodes2markers, marker pass over the sorted order.
-template <bool ADVECT>
-__global__ __launch_bounds__(WAVE) void k_n2m_cells(KArgs a, int nwg)
+// nodes2markers antidiffusion scatter (interpolation.c:1159-1209, 1360-1374,
+// 1405-1489) over the sorted order, before any marker moves: contributions of
+// every marker at its clamped position, reduced per (cell, quadrant) into
+// per-cell partials.
+__global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
 {
     __shared__ double acc[CPW * 4][N_NF];
     __shared__ double con[N_NF][WAVE];
     __shared__ uint32_t skey[WAVE];
+    __shared__ int runs[WAVE + 1];
     const int lane = threadIdx.x;
     const int wg = xcd_remap(blockIdx.x, nwg);
     const long c0 = (long)wg * CPW;
     const long c1 = c0 + CPW < a.ncell ? c0 + CPW : a.ncell;
-    const bool anti = a.p.antidiffinter == 1;
+    const DevGrid &g = a.g;
     for (int s = lane; s < CPW * 4 * N_NF; s += WAVE) (&acc[0][0])[s] = 0.0;
     const uint32_t beg = a.cell_start[c0], end = a.cell_start[c1];
-    int unsupported = 0;
     __syncthreads();
     for (uint32_t base = beg; base < end; base += WAVE) {
         const uint32_t i = base + lane;
-        const bool active = i < end;
+        const int nact = end - base < (uint32_t)WAVE ? (int)(end - base) : WAVE;
         double v[N_NF];
 #pragma unroll
         for (int f = 0; f < N_NF; f++) v[f] = 0.0;
         uint32_t slot = 0xffffffffu;
-        if (active) {
+        if (i < end) {
             const long m = a.order[i];
-            slot = a.key[m] - (uint32_t)(c0 << 2);
-            const int t = a.t[m];
-            MState s;
-            s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
-            s.dirty_state = false; s.dirty_k = false;
-            Mech mv;
-            mech_gather(a, s.x, s.y, a.mat->ro[t], mv, v);
-            if (!anti) {
-                s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); s.p = MK(a, MF_P, m);
-                s.exx = MK(a, MF_EXX, m); s.exy = MK(a, MF_EXY, m); s.vx = MK(a, MF_VX, m);
-                s.vy = MK(a, MF_VY, m); s.phi = MK(a, MF_PHI, m); s.e = MK(a, MF_E, m);
-            }
-            n2m_update(a, t, s, mv);
-            bool moved = false, reset = false;
-            if (ADVECT) {
-                if (!s.dirty_state) {   // stress needed for the rotation
-                    s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m);
-                    if (!anti) { /* already loaded */ }
-                }
-                double xx0 = s.xx, xy0 = s.xy;
-                if (!advect_one(a, t, s, moved, reset)) unsupported++;
-                if (moved && !s.dirty_state) {
-                    MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy;
-                }
-                (void)xx0; (void)xy0;
-            }
-            if (s.dirty_state) {
-                MK(a, MF_PHI, m) = s.phi; MK(a, MF_EXX, m) = s.exx; MK(a, MF_EXY, m) = s.exy;
-                MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy; MK(a, MF_P, m) = s.p;
-                MK(a, MF_VX, m) = s.vx; MK(a, MF_VY, m) = s.vy; MK(a, MF_E, m) = s.e;
-            }
-            if (s.dirty_k) MK(a, MF_K, m) = s.k;
-            if (ADVECT) {
-                if (moved) {
-                    MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
-                    a.key[m] = subcell_key(a.g, s.x, s.y);
-                }
-                if (reset) {
-                    MK(a, MF_D, m) = -1.0; MK(a, MF_W, m) = -1.0;
-                    if (!s.dirty_state) MK(a, MF_E, m) = 0;
-                }
-            }
+            const uint32_t key = a.key[m];
+            slot = key - (uint32_t)(c0 << 2);
+            int nx, ny;
+            key_cell(a, key, nx, ny);
+            const double x = clampd(MK(a, MF_X, m), 0, g.xsize), y = clampd(MK(a, MF_Y, m), 0, g.ysize);
+            anti_contrib(a, x, y, nx, ny, a.mat->ro[a.t[m]], v);
         }
-        if (anti) {
 #pragma unroll
-            for (int f = 0; f < N_NF; f++) con[f][lane] = v[f];
-            skey[lane] = slot;
-            __syncthreads();
-            run_reduce<N_NF>(con, skey, acc, lane, active);
-            __syncthreads();
-        }
+        for (int f = 0; f < N_NF; f++) con[f][lane] = v[f];
+        skey[lane] = slot;
+        __syncthreads();
+        run_reduce<N_NF>(con, skey, acc, runs, lane, nact);
+        __syncthreads();
     }
-    if (ADVECT && unsupported) atomicAdd(&a.counters[1], unsupported);
-    if (!anti) return;
     const long ncell = a.ncell;
     for (int s = lane; s < CPW * NP_N; s += WAVE) {
         const int lc = s % CPW, f = s / CPW;
@@ -696,6 +735,58 @@ __global__ __launch_bounds__(WAVE) void k_n2m_cells(KArgs a, int nwg)
     }
 }
 
+// nodes2markers marker update (interpolation.c:943-1011), fused with
+// moveandrotatemarkers (moveandrotatemarkers.c:1-138) when ADVECT, in physical
+// marker order: every marker reads only its own state and nodal fields.
+template <bool ADVECT>
+__global__ __launch_bounds__(256) void k_n2m_update(KArgs a, int nwg)
+{
+    const long m = (long)xcd_remap(blockIdx.x, nwg) * 256 + threadIdx.x;
+    bool unsupported = false;
+    if (m < a.n) {
+        const DevGrid &g = a.g;
+        const bool anti = a.p.antidiffinter == 1;
+        const int t = a.t[m];
+        int nx, ny;
+        key_cell(a, a.key[m], nx, ny);
+        MState s;
+        s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
+        s.dirty_state = false; s.dirty_k = false;
+        if (!anti) {
+            s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); s.p = MK(a, MF_P, m);
+            s.exx = MK(a, MF_EXX, m); s.exy = MK(a, MF_EXY, m); s.vx = MK(a, MF_VX, m);
+            s.vy = MK(a, MF_VY, m); s.phi = MK(a, MF_PHI, m); s.e = MK(a, MF_E, m);
+        }
+        {
+            Mech mv;
+            mech_gather(a, clampd(s.x, 0, g.xsize), clampd(s.y, 0, g.ysize), nx, ny, mv);
+            n2m_update(a, t, s, mv);
+        }
+        if (s.dirty_state) {
+            MK(a, MF_PHI, m) = s.phi; MK(a, MF_EXX, m) = s.exx; MK(a, MF_EXY, m) = s.exy;
+            MK(a, MF_P, m) = s.p; MK(a, MF_VX, m) = s.vx; MK(a, MF_VY, m) = s.vy; MK(a, MF_E, m) = s.e;
+        }
+        bool moved = false, reset = false;
+        if (ADVECT) {
+            if (!s.dirty_state) { s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); }
+            if (!advect_one(a, t, s, moved, reset, nx, ny)) unsupported = true;
+        }
+        if (s.dirty_state || moved) { MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy; }
+        if (s.dirty_k) MK(a, MF_K, m) = s.k;
+        if (ADVECT) {
+            if (moved) {
+                MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
+                a.key[m] = subcell_key(a.g, s.x, s.y);
+            }
+            if (reset) { MK(a, MF_D, m) = -1.0; MK(a, MF_W, m) = -1.0; MK(a, MF_E, m) = 0; }
+        }
+    }
+    if (ADVECT) {
+        const unsigned long long bad = __ballot(unsupported);
+        if (bad && (threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&a.counters[1], (int)__popcll(bad));
+    }
+}
+
 // The extra iteration of the reference's `m3<=marknum` loop
 // (interpolation.c:937): a zero marker of type 0 at (0,0).  Only its
 // antidiffusion scatter has an effect; added into the persistent sums.
@@ -704,8 +795,7 @@ __global__ void k_n2m_phantom(KArgs a)
     if (threadIdx.x != 0 || blockIdx.x != 0 || a.p.antidiffinter != 1) return;
     double v[N_NF];
     for (int f = 0; f < N_NF; f++) v[f] = 0.0;
-    Mech mv;
-    mech_gather(a, 0.0, 0.0, a.mat->ro[0], mv, v);
+    anti_contrib(a, 0.0, 0.0, find_nx(a.g, 0.0), find_ny(a.g, 0.0), a.mat->ro[0], v);
     const int ynumy = a.g.ynumy;
     // at (0,0): cx = cy = 0 -> basic node 0, vx node 0, vy node 0, centre (1,1)
     const long cn = (long)ynumy + 1;
@@ -787,7 +877,7 @@ __global__ void k_n2m_nodes(KArgs a)
 }
 
 // Standalone moveandrotatemarkers over physical marker order.
-__global__ void k_advect(KArgs a)
+__global__ __launch_bounds__(256) void k_advect(KArgs a)
 {
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int unsupported = 0;
@@ -797,8 +887,10 @@ __global__ void k_advect(KArgs a)
         s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
         s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m);
         s.dirty_k = false;
+        int nx, ny;
+        key_cell(a, a.key[m], nx, ny);
         bool moved, reset;
-        if (!advect_one(a, t, s, moved, reset)) unsupported = 1;
+        if (!advect_one(a, t, s, moved, reset, nx, ny)) unsupported = 1;
         if (moved) {
             MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
             MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy;
