@@ -72,6 +72,36 @@ def test_edge_rk_variants(gpu, rkmod, rk):
     _check(gpu, cfg, ["m2n", f"before:{DT / 2}:{rk}", f"after:{DT / 2}:{rk}"])
 
 
+def _clustered():
+    """Edge cloud with sub-cells holding several 64-marker batches (runs that
+    are carried from batch to batch), runs of exactly 64 and 65, and the
+    emptied sub-cells they were taken from."""
+    cfg = synth.config_edge()
+    cfg.timesum = DT
+    g = cfg.grid
+    rng = np.random.default_rng(5)
+    x, y = cfg.markers["x"], cfg.markers["y"]
+    idx = rng.permutation(cfg.n)
+    j = 0
+    for cnt, cx, cy, qx, qy in ((300, 3, 2, 0, 0), (130, 3, 3, 1, 1), (65, 4, 2, 0, 1), (64, 5, 5, 1, 0)):
+        sel = idx[j:j + cnt]
+        j += cnt
+        x[sel] = g.gx[cx] + (g.gx[cx + 1] - g.gx[cx]) * (0.5 * qx + rng.uniform(0.05, 0.45, cnt))
+        y[sel] = g.gy[cy] + (g.gy[cy + 1] - g.gy[cy]) * (0.5 * qy + rng.uniform(0.05, 0.45, cnt))
+    return cfg
+
+
+def test_clustered_runs_full_step(gpu):
+    _check(gpu, _clustered(), STEP)
+
+
+def test_clustered_runs_two_steps(gpu):
+    cfg = _clustered()
+    ops = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m_move", f"time:{2 * DT}:{DT}", "m2n",
+           f"before:{DT}:0", f"after:{DT}:0", "n2m_move"]
+    _check(gpu, cfg, ops)
+
+
 def test_medium_refined_two_steps(gpu):
     cfg = synth.config_grid(201, 101, 200e3, 100e3, 4, refine=True)
     ops = STEP + [f"time:{DT}:{DT}"] + STEP

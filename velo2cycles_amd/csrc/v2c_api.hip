@@ -147,7 +147,7 @@ static KArgs make_args(v2c_ctx *c)
     a.p = c->prm;
     a.mat = c->d_mat;
     a.mk = c->d_mk; a.cap = c->cap; a.t = c->d_t; a.n = c->n;
-    a.order = c->d_order; a.cell_start = c->d_cs; a.key = c->d_key;
+    a.order = c->d_order; a.skey = c->d_skey; a.sub_start = c->d_cs; a.key = c->d_key;
     a.nod = c->d_nod; a.nodenum = c->nodenum; a.ncell = c->ncell;
     a.part = c->d_part;
     a.prop = c->d_prop;
@@ -171,8 +171,8 @@ static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 
 static int cell_starts(v2c_ctx *c)
 {
     return launch(c, V2C_K_SORT, [&] {
-        hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(c->n + 1, 256)), dim3(256), 0, c->stream,
-                           c->d_skey, c->n, c->ncell, c->d_cs);
+        hipLaunchKernelGGL(k_sub_start, dim3(blocks_for(c->n + 1, 256)), dim3(256), 0, c->stream,
+                           c->d_skey, c->n, 4 * c->ncell, c->d_cs);
     });
 }
 
@@ -192,7 +192,7 @@ static int full_sort(v2c_ctx *c)
     return cell_starts(c);
 }
 
-// Bring order[]/skey[]/cell_start[] up to date with the current keys.  After
+// Bring order[]/skey[]/sub_start[] up to date with the current keys.  After
 // an advection only markers whose key changed are re-sorted and merged in
 // (see k_mover_flags); the result is the same (key, id) order a full sort
 // gives.
@@ -358,9 +358,9 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_skey, sizeof(uint32_t) * c->cap));
     CK(hipMalloc(&c->d_iota, sizeof(uint32_t) * c->cap));
     CK(hipMalloc(&c->d_order, sizeof(uint32_t) * c->cap));
-    CK(hipMalloc(&c->d_cs, sizeof(uint32_t) * (c->ncell + 1)));
+    CK(hipMalloc(&c->d_cs, sizeof(uint32_t) * (4 * c->ncell + 1)));
     CK(hipMalloc(&c->d_nod, sizeof(double) * V2C_NF * c->nodenum));
-    CK(hipMalloc(&c->d_part, sizeof(double) * MP_N * c->ncell));
+    CK(hipMalloc(&c->d_part, sizeof(double) * 4 * M_NF * c->ncell));
     CK(hipMalloc(&c->d_prop, sizeof(double) * NPROP * c->cap));
     CK(hipMalloc(&c->d_red, sizeof(double) * (2 * blocks_for(c->nodenum, RED_BLOCK) + 8)));
     CK(hipMalloc(&c->d_counters, sizeof(int) * 8));

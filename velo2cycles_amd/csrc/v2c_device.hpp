@@ -60,12 +60,13 @@ struct KArgs {
     int8_t *t;             // marker types
     long n;                // marker count
     const uint32_t *order; // sorted position -> marker id
-    const uint32_t *cell_start;
+    const uint32_t *skey;  // sorted position -> sub-cell key
+    const uint32_t *sub_start;  // sub-cell key -> first sorted position (4*ncell+1)
     uint32_t *key;         // per marker (physical) sub-cell key
     double *nod;           // V2C_NF fields x nodenum
     long nodenum;
     long ncell;
-    double *part;          // per-cell partials, field-major
+    double *part;          // per-sub-cell sums: [(field*4 + quadrant)*ncell + cell]
     double *prop;          // per-marker properties of marker2nodes (NPROP x cap)
     double timesum, timestep;
     int *counters;         // device counters (errors, unsupported types)

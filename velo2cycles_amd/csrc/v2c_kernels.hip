@@ -4,15 +4,20 @@
 // ordered by sub-cell key (cell, quadrant) with a stable radix sort of
 // (key, marker id); under interpolationmod == 2 every marker of one
 // (cell, quadrant) scatters to the same basic, vx, vy and centre node, so the
-// marker->node reductions run as deterministic segmented sums:
+// marker->node reductions run as deterministic segmented sums.  Each pass is
+// split by access pattern:
 //
-//   *_cells kernels  one 64-lane wavefront per group of CPW consecutive cells;
-//                    lanes take markers in sorted order, compute their
-//                    contributions, and the head lane of every (cell, quadrant)
-//                    run sums the run from LDS in lane order into an LDS slot;
-//                    the slots are written as per-cell partial sums;
-//   *_nodes kernels  one thread per node gathers the <= 4 partials of the
-//                    adjacent cells in a fixed order and normalises.
+//   per-marker kernels  (m2n_props, n2m_update) one thread per marker in
+//                    physical order: material laws, gathers, RK4 advection;
+//                    coalesced, no LDS, occupancy set by registers only;
+//   scatter kernels  (m2n_cells, n2m_scatter) one 64-lane wavefront per group
+//                    of CPW consecutive cells; lanes take 64 markers at a time
+//                    in sorted order and put their contributions in LDS; every
+//                    (cell, quadrant) run is summed in lane order by work items
+//                    (run, field) and written once per sub-cell; a run that
+//                    continues into the next batch is carried in LDS;
+//   node kernels     one thread per node adds the <= 4 sub-cell sums that map
+//                    to it in a fixed order and normalises.
 //
 // No floating-point atomics are used anywhere, so results are bitwise
 // reproducible run to run.
@@ -29,15 +34,12 @@ constexpr int CPW = 16;          // cells per wavefront group
 //   centre [20,27): bm, nu, nu/G, sxx, exx, P, w
 constexpr int M_NB = 14, M_NV = 3, M_NC = 7;
 constexpr int M_VX = M_NB, M_VY = M_NB + M_NV, M_C = M_NB + 2 * M_NV, M_NF = M_C + M_NC;
-// per-cell partial layout: basic 4 quadrants x 14, vx 2 halves x 3, vy 2 x 3, centre 7
-constexpr int MP_VX = 4 * M_NB, MP_VY = MP_VX + 2 * M_NV, MP_C = MP_VY + 2 * M_NV, MP_N = MP_C + M_NC;
 
 // n2m antidiffusion sums (interpolation.c:1159-1209, 1360-1374, 1405-1489):
 //   basic [0,5): PHIS*w, SXY*w, EXY*w, ES*w, w
 //   vx [5,8): VX*ro*w, ro*w, w   vy [8,11)   centre [11,15): PR*w, SXX*w, EXX*w, w
 constexpr int N_NB = 5, N_NV = 3, N_NC = 4;
 constexpr int N_VX = N_NB, N_VY = N_NB + N_NV, N_C = N_NB + 2 * N_NV, N_NF = N_C + N_NC;
-constexpr int NP_VX = 4 * N_NB, NP_VY = NP_VX + 2 * N_NV, NP_C = NP_VY + 2 * N_NV, NP_N = NP_C + N_NC;
 
 // Map workgroup ids so each of the 8 XCDs gets a contiguous range of cells
 // (dispatch assigns workgroups round-robin over XCDs; neighbouring cells share
@@ -65,14 +67,14 @@ __global__ void k_iota(uint32_t *v, long n)
     if (i < n) v[i] = (uint32_t)i;
 }
 
-// cell_start[c] = first sorted position whose cell is >= c; cell_start[ncell] = n.
-__global__ void k_cell_start(const uint32_t *__restrict__ skey, long n, long ncell, uint32_t *cs)
+// ss[k] = first sorted position whose sub-cell key is >= k; ss[nsub] = n.
+__global__ void k_sub_start(const uint32_t *__restrict__ skey, long n, long nsub, uint32_t *ss)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
-    long cur = i < n ? (long)(skey[i] >> 2) : ncell;
-    long prev = i > 0 ? (long)(skey[i - 1] >> 2) : -1;
-    for (long c = prev + 1; c <= cur && c <= ncell; c++) cs[c] = (uint32_t)i;
+    long cur = i < n ? (long)skey[i] : nsub;
+    long prev = i > 0 ? (long)skey[i - 1] : -1;
+    for (long k = prev + 1; k <= cur && k <= nsub; k++) ss[k] = (uint32_t)i;
 }
 
 // ---------------------------------------------------------------------------
@@ -138,16 +140,23 @@ __global__ void k_merge_movers(const uint64_t *__restrict__ mov, long nmov, cons
 }
 
 // ---------------------------------------------------------------------------
-// Segmented reduction of one batch: lanes [0, nact) hold NF contributions in
-// con[f][lane] and a local slot id in skey[lane], runs of equal slot ids being
-// contiguous.  Work items (run, field) are spread over the lanes; each sums
-// its run serially in lane order and adds the total to acc[slot][f].  A
-// (slot, field) pair occurs once per batch, so no two lanes touch the same
-// accumulator, and the summation order is fixed.
+// Segmented sums of one batch of 64 sorted markers.  Lanes [0, nact) hold NF
+// contributions in con[f][lane] and their sub-cell key in skey[lane]; equal
+// keys are contiguous.  Work items (run, field) are spread over the lanes and
+// each sums its run serially in lane order, so the result does not depend on
+// timing.  A sub-cell whose markers continue into the next batch (cont) is
+// carried in LDS (double-buffered by batch parity) and finished there; every
+// other run is complete and written once to out[(f*4 + q)*ncell + cell].
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ long sub_index(int f, uint32_t key, long ncell)
+{
+    return (long)(f * 4 + (int)(key & 3u)) * ncell + (long)(key >> 2);
+}
+
 template <int NF>
-__device__ __forceinline__ void run_reduce(const double (*con)[WAVE], const uint32_t *skey, double (*acc)[NF],
-                                           int *runs, int lane, int nact)
+__device__ __forceinline__ void scatter_batch(const double (*con)[WAVE], const uint32_t *skey, int *runs,
+                                              double (*carry)[NF], int par, bool have_carry, bool cont,
+                                              int lane, int nact, double *__restrict__ out, long ncell)
 {
     const bool head = lane < nact && (lane == 0 || skey[lane - 1] != skey[lane]);
     const unsigned long long hm = __ballot(head);
@@ -158,10 +167,21 @@ __device__ __forceinline__ void run_reduce(const double (*con)[WAVE], const uint
     for (int w = lane; w < nr * NF; w += WAVE) {
         const int f = w / nr, r = w - f * nr;
         const int b = runs[r], e = runs[r + 1];
-        double sum = 0.0;
+        double sum = (r == 0 && have_carry) ? carry[par][f] : 0.0;
         for (int j = b; j < e; j++) sum += con[f][j];
-        acc[skey[b]][f] += sum;
+        if (r == nr - 1 && cont) carry[par ^ 1][f] = sum;
+        else out[sub_index(f, skey[b], ncell)] = sum;
     }
+}
+
+// Zero the sums of the group's empty sub-cells (the node passes read every
+// sub-cell of every cell).
+template <int NF>
+__device__ __forceinline__ void zero_empty(const KArgs &a, long c0, long c1, int lane, double *__restrict__ out)
+{
+    const long k = c0 * 4 + lane;
+    if (k < c1 * 4 && a.sub_start[k] == a.sub_start[k + 1])
+        for (int f = 0; f < NF; f++) out[sub_index(f, (uint32_t)k, a.ncell)] = 0.0;
 }
 
 // Cell indices of a marker inside the grid from its sub-cell key (the key is
@@ -217,32 +237,31 @@ __global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
 {
-    __shared__ double acc[CPW * 4][M_NF];
     __shared__ double con[M_NF][WAVE];
+    __shared__ double carry[2][M_NF];
     __shared__ uint32_t skey[WAVE];
     __shared__ int runs[WAVE + 1];
     const int lane = threadIdx.x;
     const int wg = xcd_remap(blockIdx.x, nwg);
     const long c0 = (long)wg * CPW;
     const long c1 = c0 + CPW < a.ncell ? c0 + CPW : a.ncell;
-    for (int s = lane; s < CPW * 4 * M_NF; s += WAVE) (&acc[0][0])[s] = 0.0;
-    const uint32_t beg = a.cell_start[c0], end = a.cell_start[c1];
+    const uint32_t beg = a.sub_start[c0 * 4], end = a.sub_start[c1 * 4];
     const DevMat *M = a.mat;
     const DevGrid &g = a.g;
     const double *P = a.prop;
-    __syncthreads();
-    for (uint32_t base = beg; base < end; base += WAVE) {
+    bool have_carry = false;
+    int par = 0;
+    for (uint32_t base = beg; base < end; base += WAVE, par ^= 1) {
         const uint32_t i = base + lane;
         const bool active = i < end;
         const int nact = end - base < (uint32_t)WAVE ? (int)(end - base) : WAVE;
         double v[M_NF];
 #pragma unroll
         for (int f = 0; f < M_NF; f++) v[f] = 0.0;
-        uint32_t slot = 0xffffffffu;
+        uint32_t key = 0xffffffffu;
         if (active) {
             const long m = a.order[i];
-            const uint32_t key = a.key[m];
-            slot = key - (uint32_t)(c0 << 2);
+            key = a.skey[i];
             const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
             if (x > 0 && y > 0 && x < g.xsize && y < g.ysize) {
                 int nx, ny;
@@ -295,33 +314,14 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
         }
 #pragma unroll
         for (int f = 0; f < M_NF; f++) con[f][lane] = v[f];
-        skey[lane] = slot;
+        skey[lane] = key;
+        const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
         __syncthreads();
-        run_reduce<M_NF>(con, skey, acc, runs, lane, nact);
+        scatter_batch<M_NF>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a.part, a.ncell);
+        have_carry = cont;
         __syncthreads();
     }
-    // per-cell partials
-    const long ncell = a.ncell;
-    for (int s = lane; s < CPW * MP_N; s += WAVE) {
-        const int lc = s % CPW, f = s / CPW;
-        const long c = c0 + lc;
-        if (c >= c1) continue;
-        const double(*sl)[M_NF] = &acc[lc * 4];
-        double val;
-        if (f < MP_VX) {
-            val = sl[f / M_NB][f % M_NB];
-        } else if (f < MP_VY) {
-            const int h = (f - MP_VX) / M_NV, k = (f - MP_VX) % M_NV;
-            val = sl[2 * h][M_VX + k] + sl[2 * h + 1][M_VX + k];
-        } else if (f < MP_C) {
-            const int h = (f - MP_VY) / M_NV, k = (f - MP_VY) % M_NV;
-            val = sl[h][M_VY + k] + sl[2 + h][M_VY + k];
-        } else {
-            const int k = f - MP_C;
-            val = sl[0][M_C + k] + sl[1][M_C + k] + sl[2][M_C + k] + sl[3][M_C + k];
-        }
-        a.part[(long)f * ncell + c] = val;
-    }
+    zero_empty<M_NF>(a, c0, c1, lane, a.part);
 }
 
 // ---------------------------------------------------------------------------
@@ -340,8 +340,10 @@ __global__ void k_m2n_nodes(KArgs a)
     auto cell = [&](int cx, int cy) -> long {
         return (cx >= 0 && cy >= 0 && cx < xnumx - 1 && cy < ynumy - 1) ? (long)cx * ncy + cy : -1;
     };
-    // basic node: quadrant (1,1) of cell (m11-1,m22-1), (1,0) of (m11-1,m22),
-    // (0,1) of (m11,m22-1), (0,0) of (m11,m22)
+    // Sums of sub-cell (cell, quadrant q = 2*(cx > .5) + (cy > .5)).
+    auto S = [&](int f, int q, long c) -> double { return P[((long)f * 4 + q) * ncell + c]; };
+    // basic node: quadrant 3 of cell (m11-1,m22-1), 2 of (m11-1,m22), 1 of
+    // (m11,m22-1), 0 of (m11,m22)
     double B[M_NB];
 #pragma unroll
     for (int f = 0; f < M_NB; f++) B[f] = 0.0;
@@ -351,26 +353,30 @@ __global__ void k_m2n_nodes(KArgs a)
         for (int k = 0; k < 4; k++) {
             if (cs[k] < 0) continue;
 #pragma unroll
-            for (int f = 0; f < M_NB; f++) B[f] += P[(long)(qs[k] * M_NB + f) * ncell + cs[k]];
+            for (int f = 0; f < M_NB; f++) B[f] += S(f, qs[k], cs[k]);
         }
     }
+    // vx node: right half (q 2, 3) of cell (m11-1,m22), left half (q 0, 1) of
+    // (m11,m22); vy node: upper half (q 1, 3) of (m11,m22-1), lower half
+    // (q 0, 2) of (m11,m22); centre node: all of (m11-1,m22-1)
     double VXs[M_NV] = {0, 0, 0}, VYs[M_NV] = {0, 0, 0}, Cs[M_NC];
 #pragma unroll
     for (int f = 0; f < M_NC; f++) Cs[f] = 0.0;
     {
-        long cl = cell(m11 - 1, m22), cr = cell(m11, m22);
+        const long cl = cell(m11 - 1, m22), cr = cell(m11, m22);
         for (int f = 0; f < M_NV; f++) {
-            if (cl >= 0) VXs[f] += P[(long)(MP_VX + M_NV + f) * ncell + cl];
-            if (cr >= 0) VXs[f] += P[(long)(MP_VX + f) * ncell + cr];
+            if (cl >= 0) VXs[f] += S(M_VX + f, 2, cl) + S(M_VX + f, 3, cl);
+            if (cr >= 0) VXs[f] += S(M_VX + f, 0, cr) + S(M_VX + f, 1, cr);
         }
-        long ct = cell(m11, m22 - 1), cb = cell(m11, m22);
+        const long ct = cell(m11, m22 - 1), cb = cell(m11, m22);
         for (int f = 0; f < M_NV; f++) {
-            if (ct >= 0) VYs[f] += P[(long)(MP_VY + M_NV + f) * ncell + ct];
-            if (cb >= 0) VYs[f] += P[(long)(MP_VY + f) * ncell + cb];
+            if (ct >= 0) VYs[f] += S(M_VY + f, 1, ct) + S(M_VY + f, 3, ct);
+            if (cb >= 0) VYs[f] += S(M_VY + f, 0, cb) + S(M_VY + f, 2, cb);
         }
-        long cc = cell(m11 - 1, m22 - 1);
+        const long cc = cell(m11 - 1, m22 - 1);
         if (cc >= 0)
-            for (int f = 0; f < M_NC; f++) Cs[f] = P[(long)(MP_C + f) * ncell + cc];
+            for (int f = 0; f < M_NC; f++)
+                Cs[f] = S(M_C + f, 0, cc) + S(M_C + f, 1, cc) + S(M_C + f, 2, cc) + S(M_C + f, 3, cc);
     }
 
     // zeroed by the reference's initialisation loop (interpolation.c:58-153)
@@ -677,8 +683,8 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
 // per-cell partials.
 __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
 {
-    __shared__ double acc[CPW * 4][N_NF];
     __shared__ double con[N_NF][WAVE];
+    __shared__ double carry[2][N_NF];
     __shared__ uint32_t skey[WAVE];
     __shared__ int runs[WAVE + 1];
     const int lane = threadIdx.x;
@@ -686,20 +692,19 @@ __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
     const long c0 = (long)wg * CPW;
     const long c1 = c0 + CPW < a.ncell ? c0 + CPW : a.ncell;
     const DevGrid &g = a.g;
-    for (int s = lane; s < CPW * 4 * N_NF; s += WAVE) (&acc[0][0])[s] = 0.0;
-    const uint32_t beg = a.cell_start[c0], end = a.cell_start[c1];
-    __syncthreads();
-    for (uint32_t base = beg; base < end; base += WAVE) {
+    const uint32_t beg = a.sub_start[c0 * 4], end = a.sub_start[c1 * 4];
+    bool have_carry = false;
+    int par = 0;
+    for (uint32_t base = beg; base < end; base += WAVE, par ^= 1) {
         const uint32_t i = base + lane;
         const int nact = end - base < (uint32_t)WAVE ? (int)(end - base) : WAVE;
         double v[N_NF];
 #pragma unroll
         for (int f = 0; f < N_NF; f++) v[f] = 0.0;
-        uint32_t slot = 0xffffffffu;
+        uint32_t key = 0xffffffffu;
         if (i < end) {
             const long m = a.order[i];
-            const uint32_t key = a.key[m];
-            slot = key - (uint32_t)(c0 << 2);
+            key = a.skey[i];
             int nx, ny;
             key_cell(a, key, nx, ny);
             const double x = clampd(MK(a, MF_X, m), 0, g.xsize), y = clampd(MK(a, MF_Y, m), 0, g.ysize);
@@ -707,32 +712,14 @@ __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
         }
 #pragma unroll
         for (int f = 0; f < N_NF; f++) con[f][lane] = v[f];
-        skey[lane] = slot;
+        skey[lane] = key;
+        const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
         __syncthreads();
-        run_reduce<N_NF>(con, skey, acc, runs, lane, nact);
+        scatter_batch<N_NF>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a.part, a.ncell);
+        have_carry = cont;
         __syncthreads();
     }
-    const long ncell = a.ncell;
-    for (int s = lane; s < CPW * NP_N; s += WAVE) {
-        const int lc = s % CPW, f = s / CPW;
-        const long c = c0 + lc;
-        if (c >= c1) continue;
-        const double(*sl)[N_NF] = &acc[lc * 4];
-        double val;
-        if (f < NP_VX) {
-            val = sl[f / N_NB][f % N_NB];
-        } else if (f < NP_VY) {
-            const int h = (f - NP_VX) / N_NV, k = (f - NP_VX) % N_NV;
-            val = sl[2 * h][N_VX + k] + sl[2 * h + 1][N_VX + k];
-        } else if (f < NP_C) {
-            const int h = (f - NP_VY) / N_NV, k = (f - NP_VY) % N_NV;
-            val = sl[h][N_VY + k] + sl[2 + h][N_VY + k];
-        } else {
-            const int k = f - NP_C;
-            val = sl[0][N_C + k] + sl[1][N_C + k] + sl[2][N_C + k] + sl[3][N_C + k];
-        }
-        a.part[(long)f * ncell + c] = val;
-    }
+    zero_empty<N_NF>(a, c0, c1, lane, a.part);
 }
 
 // nodes2markers marker update (interpolation.c:943-1011), fused with
@@ -820,30 +807,32 @@ __global__ void k_n2m_nodes(KArgs a)
     auto cell = [&](int cx, int cy) -> long {
         return (cx >= 0 && cy >= 0 && cx < xnumx - 1 && cy < ynumy - 1) ? (long)cx * ncy + cy : -1;
     };
+    auto S = [&](int f, int q, long c) -> double { return P[((long)f * 4 + q) * ncell + c]; };
     double B[N_NB] = {0, 0, 0, 0, 0};
     {
         const long cs[4] = {cell(m11 - 1, m22 - 1), cell(m11 - 1, m22), cell(m11, m22 - 1), cell(m11, m22)};
         const int qs[4] = {3, 2, 1, 0};
         for (int k = 0; k < 4; k++) {
             if (cs[k] < 0) continue;
-            for (int f = 0; f < N_NB; f++) B[f] += P[(long)(qs[k] * N_NB + f) * ncell + cs[k]];
+            for (int f = 0; f < N_NB; f++) B[f] += S(f, qs[k], cs[k]);
         }
     }
     double VXs[N_NV] = {0, 0, 0}, VYs[N_NV] = {0, 0, 0}, Cs[N_NC] = {0, 0, 0, 0};
     {
-        long cl = cell(m11 - 1, m22), cr = cell(m11, m22);
+        const long cl = cell(m11 - 1, m22), cr = cell(m11, m22);
         for (int f = 0; f < N_NV; f++) {
-            if (cl >= 0) VXs[f] += P[(long)(NP_VX + N_NV + f) * ncell + cl];
-            if (cr >= 0) VXs[f] += P[(long)(NP_VX + f) * ncell + cr];
+            if (cl >= 0) VXs[f] += S(N_VX + f, 2, cl) + S(N_VX + f, 3, cl);
+            if (cr >= 0) VXs[f] += S(N_VX + f, 0, cr) + S(N_VX + f, 1, cr);
         }
-        long ct = cell(m11, m22 - 1), cb = cell(m11, m22);
+        const long ct = cell(m11, m22 - 1), cb = cell(m11, m22);
         for (int f = 0; f < N_NV; f++) {
-            if (ct >= 0) VYs[f] += P[(long)(NP_VY + N_NV + f) * ncell + ct];
-            if (cb >= 0) VYs[f] += P[(long)(NP_VY + f) * ncell + cb];
+            if (ct >= 0) VYs[f] += S(N_VY + f, 1, ct) + S(N_VY + f, 3, ct);
+            if (cb >= 0) VYs[f] += S(N_VY + f, 0, cb) + S(N_VY + f, 2, cb);
         }
-        long cc = cell(m11 - 1, m22 - 1);
+        const long cc = cell(m11 - 1, m22 - 1);
         if (cc >= 0)
-            for (int f = 0; f < N_NC; f++) Cs[f] = P[(long)(NP_C + f) * ncell + cc];
+            for (int f = 0; f < N_NC; f++)
+                Cs[f] = S(N_C + f, 0, cc) + S(N_C + f, 1, cc) + S(N_C + f, 2, cc) + S(N_C + f, 3, cc);
     }
     double *phisum = NOD(a, V2C_N_phisum), *sxysum = NOD(a, V2C_N_sxysum), *exysum = NOD(a, V2C_N_exysum);
     double *esum = NOD(a, V2C_N_esum), *wtphisum = NOD(a, V2C_N_wtphisum);
