@@ -1,5 +1,5 @@
-// Host-side check of the table-driven cell search against the reference
-// bisection.  Reads axes from argv files (raw float64), probes gridlines, their
+// Host-side check of the table-driven cell search (cell index and gridline
+// record) against the reference bisection.  Reads axes from argv files (raw float64), probes gridlines, their
 // neighbouring doubles, midpoints, random points, infinities and NaN; prints
 // "probes mismatches" per axis.
 #include <cmath>
@@ -22,7 +22,8 @@ int main(int argc, char **argv)
         fclose(fp);
         const int n = (int)g.size();
         double b0, binv;
-        std::vector<AxisBin> bins = make_bins(g.data(), n, b0, binv);
+        std::vector<Axis4> bins, cells;
+        make_tables(g.data(), n, bins, cells, b0, binv);
         std::vector<double> probes;
         for (int i = 0; i < n; i++) {
             probes.push_back(g[i]);
@@ -44,9 +45,16 @@ int main(int argc, char **argv)
         for (double s : specials) probes.push_back(s);
         int bad = 0;
         for (double x : probes) {
-            const int a = axis_find(g.data(), n, bins.data(), (int)bins.size(), b0, binv, x);
+            const Axis4 r = axis_find(g.data(), n, bins.data(), cells.data(), (int)bins.size(), b0, binv, x);
+            const int a = r.s;
             const int b = host_bisect(g.data(), n, x);
-            if (a != b) {
+            bool ok = a == b;
+            // the record carries gridlines b-1 .. b+2 (NaN past either end)
+            for (int k = 0; k < 4 && ok; k++) {
+                const int i = b - 1 + k;
+                ok = (i >= 0 && i < n) ? r.g[k] == g[i] : std::isnan(r.g[k]);
+            }
+            if (!ok) {
                 if (bad < 5) printf("mismatch x=%.17g table=%d bisect=%d\n", x, a, b);
                 bad++;
             }

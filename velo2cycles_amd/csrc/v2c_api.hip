@@ -33,8 +33,9 @@ struct v2c_ctx {
     long last_movers = 0, n_full_sorts = 0, n_incremental = 0;
     // device memory
     double *d_gx = nullptr, *d_gy = nullptr;
-    AxisBin *d_bx = nullptr, *d_by = nullptr;
+    Axis4 *d_bx = nullptr, *d_by = nullptr, *d_cellx = nullptr, *d_celly = nullptr;
     double bx0 = 0, by0 = 0, bxinv = 0, byinv = 0;
+    double xr_lo = 0, xr_hi = 0, yr_lo = 0, yr_hi = 0;
     int nbx = 0, nby = 0;
     DevMat *d_mat = nullptr;
     double *d_mk = nullptr;
@@ -141,9 +142,10 @@ static KArgs make_args(v2c_ctx *c)
     a.g.gx = c->d_gx; a.g.gy = c->d_gy;
     a.g.xnumx = (int)c->xnumx; a.g.ynumy = (int)c->ynumy;
     a.g.xsize = c->xsize; a.g.ysize = c->ysize;
-    a.g.bx = c->d_bx; a.g.by = c->d_by;
+    a.g.bx = c->d_bx; a.g.by = c->d_by; a.g.cellx = c->d_cellx; a.g.celly = c->d_celly;
     a.g.bx0 = c->bx0; a.g.by0 = c->by0; a.g.bxinv = c->bxinv; a.g.byinv = c->byinv;
     a.g.nbx = c->nbx; a.g.nby = c->nby;
+    a.g.xr_lo = c->xr_lo; a.g.xr_hi = c->xr_hi; a.g.yr_lo = c->yr_lo; a.g.yr_hi = c->yr_hi;
     a.p = c->prm;
     a.mat = c->d_mat;
     a.mk = c->d_mk; a.cap = c->cap; a.t = c->d_t; a.n = c->n;
@@ -299,7 +301,7 @@ const char *v2c_last_error(const v2c_ctx *c) { return c ? c->err : "null context
 
 static void free_all(v2c_ctx *c)
 {
-    void *ptrs[] = {c->d_gx, c->d_gy, c->d_bx, c->d_by, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
+    void *ptrs[] = {c->d_gx, c->d_gy, c->d_bx, c->d_by, c->d_cellx, c->d_celly, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
                     c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_prop, c->d_red, c->d_counters, c->d_sort_tmp,
                     c->d_order2, c->d_skey2, c->d_ids_mv, c->d_ids_st, c->d_flag_mv, c->d_flag_st,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
@@ -392,14 +394,22 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMemcpyAsync(c->d_gx, g->gx, sizeof(double) * g->xnumx, hipMemcpyHostToDevice, c->stream));
     CK(hipMemcpyAsync(c->d_gy, g->gy, sizeof(double) * g->ynumy, hipMemcpyHostToDevice, c->stream));
     {
-        std::vector<AxisBin> bx = make_bins(g->gx, (int)g->xnumx, c->bx0, c->bxinv);
-        std::vector<AxisBin> by = make_bins(g->gy, (int)g->ynumy, c->by0, c->byinv);
+        std::vector<Axis4> bx, by, cx, cy;
+        make_tables(g->gx, (int)g->xnumx, bx, cx, c->bx0, c->bxinv);
+        make_tables(g->gy, (int)g->ynumy, by, cy, c->by0, c->byinv);
         c->nbx = (int)bx.size();
         c->nby = (int)by.size();
-        CK(hipMalloc(&c->d_bx, sizeof(AxisBin) * bx.size()));
-        CK(hipMalloc(&c->d_by, sizeof(AxisBin) * by.size()));
-        CK(hipMemcpyAsync(c->d_bx, bx.data(), sizeof(AxisBin) * bx.size(), hipMemcpyHostToDevice, c->stream));
-        CK(hipMemcpyAsync(c->d_by, by.data(), sizeof(AxisBin) * by.size(), hipMemcpyHostToDevice, c->stream));
+        const long nx = g->xnumx, ny = g->ynumy;
+        c->xr_lo = (g->gx[0] + g->gx[1]) / 2.0; c->xr_hi = (g->gx[nx - 2] + g->gx[nx - 1]) / 2.0;
+        c->yr_lo = (g->gy[0] + g->gy[1]) / 2.0; c->yr_hi = (g->gy[ny - 2] + g->gy[ny - 1]) / 2.0;
+        CK(hipMalloc(&c->d_bx, sizeof(Axis4) * bx.size()));
+        CK(hipMalloc(&c->d_by, sizeof(Axis4) * by.size()));
+        CK(hipMalloc(&c->d_cellx, sizeof(Axis4) * cx.size()));
+        CK(hipMalloc(&c->d_celly, sizeof(Axis4) * cy.size()));
+        CK(hipMemcpyAsync(c->d_bx, bx.data(), sizeof(Axis4) * bx.size(), hipMemcpyHostToDevice, c->stream));
+        CK(hipMemcpyAsync(c->d_by, by.data(), sizeof(Axis4) * by.size(), hipMemcpyHostToDevice, c->stream));
+        CK(hipMemcpyAsync(c->d_cellx, cx.data(), sizeof(Axis4) * cx.size(), hipMemcpyHostToDevice, c->stream));
+        CK(hipMemcpyAsync(c->d_celly, cy.data(), sizeof(Axis4) * cy.size(), hipMemcpyHostToDevice, c->stream));
         CK(hipStreamSynchronize(c->stream));
     }
     {
@@ -413,6 +423,7 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
         memcpy(dm.kt, mat->kt, sizeof dm.kt); memcpy(dm.kf, mat->kf, sizeof dm.kf);
         memcpy(dm.kp, mat->kp, sizeof dm.kp); memcpy(dm.ht, mat->ht, sizeof dm.ht);
         memcpy(dm.im, mat->im, sizeof dm.im);
+        for (int t = 0; t < V2C_MAX_TYPES; t++) dm.ssn1[t] = pow(mat->ss[t], mat->mm[t] - 1.0);
         CK(hipMemcpyAsync(c->d_mat, &dm, sizeof dm, hipMemcpyHostToDevice, c->stream));
         CK(hipStreamSynchronize(c->stream));
     }

@@ -36,6 +36,7 @@ struct DevMat {
     double mm[V2C_MAX_TYPES], gg[V2C_MAX_TYPES], n0[V2C_MAX_TYPES], n1[V2C_MAX_TYPES];
     double ro[V2C_MAX_TYPES], bb[V2C_MAX_TYPES], aa[V2C_MAX_TYPES], cp[V2C_MAX_TYPES];
     double kt[V2C_MAX_TYPES], kf[V2C_MAX_TYPES], kp[V2C_MAX_TYPES], ht[V2C_MAX_TYPES];
+    double ssn1[V2C_MAX_TYPES];     // ss^(mm-1), computed once on the host
     int im[V2C_MAX_TYPES];
 };
 
@@ -44,10 +45,13 @@ struct DevGrid {
     int xnumx, ynumy;
     double xsize, ysize;
     // search tables: bins of width h <= the smallest spacing from the first
-    // gridline; bin j = (x - b0) * binv
-    const AxisBin *bx, *by;
+    // gridline (bin j = (x - b0) * binv) and per-cell records
+    const Axis4 *bx, *by, *cellx, *celly;
     double bx0, by0, bxinv, byinv;
     int nbx, nby;
+    // allinteri's outer half-cell bounds (interpolation.c:1701-1705):
+    // (gx[0]+gx[1])/2, (gx[n-2]+gx[n-1])/2 and the same in y
+    double xr_lo, xr_hi, yr_lo, yr_hi;
 };
 
 // Everything a kernel needs, passed by value.
@@ -66,7 +70,7 @@ struct KArgs {
     double *nod;           // V2C_NF fields x nodenum
     long nodenum;
     long ncell;
-    double *part;          // per-sub-cell sums: [(field*4 + quadrant)*ncell + cell]
+    double *part;          // per-sub-cell sum records: [(cell*4 + quadrant)*nfields + field]
     double *prop;          // per-marker properties of marker2nodes (NPROP x cap)
     double timesum, timestep;
     int *counters;         // device counters (errors, unsupported types)
@@ -90,56 +94,60 @@ __device__ __forceinline__ double clampd(double v, double lo, double hi)
     return v;
 }
 
-__device__ __forceinline__ int find_nx(const DevGrid &g, double x)
+__device__ __forceinline__ Axis4 find_x(const DevGrid &g, double x)
 {
-    return axis_find(g.gx, g.xnumx, g.bx, g.nbx, g.bx0, g.bxinv, x);
+    return axis_find(g.gx, g.xnumx, g.bx, g.cellx, g.nbx, g.bx0, g.bxinv, x);
 }
-__device__ __forceinline__ int find_ny(const DevGrid &g, double y)
+__device__ __forceinline__ Axis4 find_y(const DevGrid &g, double y)
 {
-    return axis_find(g.gy, g.ynumy, g.by, g.nby, g.by0, g.byinv, y);
+    return axis_find(g.gy, g.ynumy, g.by, g.celly, g.nby, g.by0, g.byinv, y);
 }
 
 struct Stencil { long m3; double e, n; };
 
+// Stencils at (x, y) in cell (X.s, Y.s); every gridline they use lies in
+// X.g / Y.g (see Axis4).
+
 // Pressure-node stencil (interpolation.c:1942-1948, 1216-1224, 1723-1730).
-__device__ __forceinline__ Stencil centre_stencil(const DevGrid &g, int nx, int ny, double x, double y)
+__device__ __forceinline__ Stencil centre_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
+                                                  double x, double y)
 {
-    const double *gx = g.gx, *gy = g.gy;
-    int m10 = nx, m20 = ny;
-    if (x > (gx[m10] + gx[m10 + 1]) / 2.0) m10++;
-    if (y > (gy[m20] + gy[m20 + 1]) / 2.0) m20++;
+    int m10 = X.s, m20 = Y.s;
+    if (x > (pick(X, m10) + pick(X, m10 + 1)) / 2.0) m10++;
+    if (y > (pick(Y, m20) + pick(Y, m20 + 1)) / 2.0) m20++;
     if (m10 < 1) m10 = 1;
     if (m10 > g.xnumx - 2) m10 = g.xnumx - 2;
     if (m20 < 1) m20 = 1;
     if (m20 > g.ynumy - 2) m20 = g.ynumy - 2;
     Stencil s;
-    s.e = (x - (gx[m10 - 1] + gx[m10]) / 2.0) / ((gx[m10 + 1] - gx[m10 - 1]) / 2.0);
-    s.n = (y - (gy[m20 - 1] + gy[m20]) / 2.0) / ((gy[m20 + 1] - gy[m20 - 1]) / 2.0);
+    s.e = (x - (pick(X, m10 - 1) + pick(X, m10)) / 2.0) / ((pick(X, m10 + 1) - pick(X, m10 - 1)) / 2.0);
+    s.n = (y - (pick(Y, m20 - 1) + pick(Y, m20)) / 2.0) / ((pick(Y, m20 + 1) - pick(Y, m20 - 1)) / 2.0);
     s.m3 = (long)m10 * g.ynumy + m20;
     return s;
 }
 
 // Basic-node stencil without ghost nodes (interpolation.c:1069-1083, 1711-1717).
-__device__ __forceinline__ Stencil basic_stencil(const DevGrid &g, int nx, int ny, double x, double y)
+__device__ __forceinline__ Stencil basic_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
+                                                 double x, double y)
 {
-    int m10 = nx, m20 = ny;
+    int m10 = X.s, m20 = Y.s;
     if (m10 < 1) m10 = 1;
     if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
     if (m20 < 1) m20 = 1;
     if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
     Stencil s;
-    s.e = (x - g.gx[m10]) / (g.gx[m10 + 1] - g.gx[m10]);
-    s.n = (y - g.gy[m20]) / (g.gy[m20 + 1] - g.gy[m20]);
+    s.e = (x - pick(X, m10)) / (pick(X, m10 + 1) - pick(X, m10));
+    s.n = (y - pick(Y, m20)) / (pick(Y, m20 + 1) - pick(Y, m20));
     s.m3 = (long)m10 * g.ynumy + m20;
     return s;
 }
 
 // vx-node stencil (interpolation.c:1377-1388, 1738-1744).
-__device__ __forceinline__ Stencil vx_stencil(const DevGrid &g, int nx, int ny, double x, double y, bool clampx)
+__device__ __forceinline__ Stencil vx_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
+                                              double x, double y, bool clampx)
 {
-    const double *gx = g.gx, *gy = g.gy;
-    int m10 = nx, m20 = ny;
-    if (y < (gy[m20] + gy[m20 + 1]) / 2.0) m20 -= 1;
+    int m10 = X.s, m20 = Y.s;
+    if (y < (pick(Y, m20) + pick(Y, m20 + 1)) / 2.0) m20 -= 1;
     if (clampx) {
         if (m10 < 0) m10 = 0;
         if (m10 > g.xnumx - 2) m10 = g.xnumx - 2;
@@ -147,18 +155,18 @@ __device__ __forceinline__ Stencil vx_stencil(const DevGrid &g, int nx, int ny, 
     if (m20 < 0) m20 = 0;
     if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
     Stencil s;
-    s.e = (x - gx[m10]) / (gx[m10 + 1] - gx[m10]);
-    s.n = (y - (gy[m20] + gy[m20 + 1]) / 2.0) / ((gy[m20 + 2] - gy[m20]) / 2.0);
+    s.e = (x - pick(X, m10)) / (pick(X, m10 + 1) - pick(X, m10));
+    s.n = (y - (pick(Y, m20) + pick(Y, m20 + 1)) / 2.0) / ((pick(Y, m20 + 2) - pick(Y, m20)) / 2.0);
     s.m3 = (long)m10 * g.ynumy + m20;
     return s;
 }
 
 // vy-node stencil (interpolation.c:1434-1445, 1749-1755).
-__device__ __forceinline__ Stencil vy_stencil(const DevGrid &g, int nx, int ny, double x, double y, bool clampy)
+__device__ __forceinline__ Stencil vy_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
+                                              double x, double y, bool clampy)
 {
-    const double *gx = g.gx, *gy = g.gy;
-    int m10 = nx, m20 = ny;
-    if (x < (gx[m10] + gx[m10 + 1]) / 2.0) m10 -= 1;
+    int m10 = X.s, m20 = Y.s;
+    if (x < (pick(X, m10) + pick(X, m10 + 1)) / 2.0) m10 -= 1;
     if (m10 < 0) m10 = 0;
     if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
     if (clampy) {
@@ -166,8 +174,8 @@ __device__ __forceinline__ Stencil vy_stencil(const DevGrid &g, int nx, int ny, 
         if (m20 > g.ynumy - 2) m20 = g.ynumy - 2;
     }
     Stencil s;
-    s.e = (x - (gx[m10] + gx[m10 + 1]) / 2.0) / ((gx[m10 + 2] - gx[m10]) / 2.0);
-    s.n = (y - gy[m20]) / (gy[m20 + 1] - gy[m20]);
+    s.e = (x - (pick(X, m10) + pick(X, m10 + 1)) / 2.0) / ((pick(X, m10 + 2) - pick(X, m10)) / 2.0);
+    s.n = (y - pick(Y, m20)) / (pick(Y, m20 + 1) - pick(Y, m20));
     s.m3 = (long)m10 * g.ynumy + m20;
     return s;
 }
@@ -196,11 +204,11 @@ __device__ __forceinline__ uint32_t subcell_key(const DevGrid &g, double x, doub
 {
     x = clampd(x, 0, g.xsize);
     y = clampd(y, 0, g.ysize);
-    int nx = find_nx(g, x), ny = find_ny(g, y);
-    double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
-    double cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny]);
+    const Axis4 X = find_x(g, x), Y = find_y(g, y);
+    double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
     uint32_t q = (cx > 0.5 ? 2u : 0u) | (cy > 0.5 ? 1u : 0u);
-    return (((uint32_t)nx * (uint32_t)(g.ynumy - 1) + (uint32_t)ny) << 2) | q;
+    return (((uint32_t)X.s * (uint32_t)(g.ynumy - 1) + (uint32_t)Y.s) << 2) | q;
 }
 
 // ---- material laws -----------------------------------------------------------
@@ -257,11 +265,12 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
     if (M->dh[t] == 0 && M->dv[t] == 0 && (M->ss[t] == 0 || M->mm[t] == 1.0)) return M->nu[t];
     if (!power_law_type(M, t)) return __builtin_nan("");
     const double n = M->mm[t];
-    double sigin = pow(sxx * sxx + sxy * sxy, 0.5);
+    // pow(s, 0.5) of a non-negative s is sqrt(s) up to pow's last-ulp error
+    double sigin = sqrt(sxx * sxx + sxy * sxy);
     double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * mtk);
     if (e1 > 150.0) e1 = 150.0;
     e1 = exp(e1);
-    double k1 = M->nu[t] / pow(M->ss[t], n - 1.0);
+    double k1 = M->nu[t] / M->ssn1[t];
     double inv_newt = 1.0 / (0.5 * k1 * e1);
     double inv_powl = 0;
     if (sigin > 0) {
@@ -290,7 +299,10 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
             const double X = pref * gm;
             const double f1 = exp(ex1 * log(X));
             const double f2 = eta_old - coeff * f1;
-            const double t3 = exp(2 / (eta_diff + eta_old) * ln_ed);
+            // eta_d^(2/(eta_d+eta)): exactly 1.0 once the exponent's
+            // magnitude is below 2^-54, which holds for any geological eta_d
+            const double sum = eta_diff + eta_old;
+            const double t3 = fabs(ln_ed) < 0x1p-56 * sum ? 1.0 : exp(2 / sum * ln_ed);
             const double df = 1 - c1 * coeff * (f1 / X) * (gm + t3);
             eta_new = eta_old - f2 / df;
             if (k > 3) {
@@ -392,29 +404,29 @@ __device__ __forceinline__ void friction_node(const KArgs &a, double x, double y
 
 // allinteri (interpolation.c:1688-1758): velocity, spin and strain-rate
 // invariant at a point.
-// (hx, hy) >= 0: the cell of the clamped point is already known.
+// hint != nullptr: the cells of the clamped point are already known.
 __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, double &VX, double &VY,
-                                          double &ESP, double &EE, int hx = -1, int hy = -1)
+                                          double &ESP, double &EE, const Axis4 *hx = nullptr,
+                                          const Axis4 *hy = nullptr)
 {
     const DevGrid &g = a.g;
-    const double *gx = g.gx, *gy = g.gy;
     const int ynumy = g.ynumy;
     x = clampd(x, 0, g.xsize);
     y = clampd(y, 0, g.ysize);
     double xrat = 2.0 / 3.0;
-    if (x < (gx[0] + gx[1]) / 2.0) xrat = 1.0;
-    if (x > (gx[g.xnumx - 2] + gx[g.xnumx - 1]) / 2.0) xrat = 1.0;
-    if (y < (gy[0] + gy[1]) / 2.0) xrat = 1.0;
-    if (y > (gy[ynumy - 2] + gy[ynumy - 1]) / 2.0) xrat = 1.0;
-    const int nx = hx >= 0 ? hx : find_nx(g, x), ny = hy >= 0 ? hy : find_ny(g, y);
+    if (x < g.xr_lo) xrat = 1.0;
+    if (x > g.xr_hi) xrat = 1.0;
+    if (y < g.yr_lo) xrat = 1.0;
+    if (y > g.yr_hi) xrat = 1.0;
+    const Axis4 X = hx ? *hx : find_x(g, x), Y = hy ? *hy : find_y(g, y);
     const double *vx = NOD(a, V2C_N_vx), *vy = NOD(a, V2C_N_vy);
 
-    Stencil s = basic_stencil(g, nx, ny, x, y);
+    Stencil s = basic_stencil(g, X, Y, x, y);
     W4 w = wts(s);
     double EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
     ESP = bil(NOD(a, V2C_N_esp), s.m3, ynumy, w);
 
-    s = centre_stencil(g, nx, ny, x, y);
+    s = centre_stencil(g, X, Y, x, y);
     w = wts(s);
     double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
     long m3 = s.m3;
@@ -422,11 +434,11 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
                   + w.d2 * (vx[m3 + ynumy - 1] + vx[m3 - 1]) + w.d3 * (vx[m3 + ynumy] + vx[m3])) * 0.5 * (1.0 - xrat);
     double vya = (w.d0 * (vy[m3 - ynumy] + vy[m3 - ynumy - 1]) + w.d1 * (vy[m3 - ynumy + 1] + vy[m3 - ynumy])
                   + w.d2 * (vy[m3] + vy[m3 - 1]) + w.d3 * (vy[m3 + 1] + vy[m3])) * 0.5 * (1.0 - xrat);
-    EE = pow(EXX * EXX + EXY * EXY, 0.5);
+    EE = sqrt(EXX * EXX + EXY * EXY);
 
-    s = vx_stencil(g, nx, ny, x, y, true);
+    s = vx_stencil(g, X, Y, x, y, true);
     VX = vxa + bil(vx, s.m3, ynumy, wts(s)) * xrat;
-    s = vy_stencil(g, nx, ny, x, y, true);
+    s = vy_stencil(g, X, Y, x, y, true);
     VY = vya + bil(vy, s.m3, ynumy, wts(s)) * xrat;
 }
 
@@ -453,11 +465,11 @@ __device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double s
         return ductilevisc(a, t, sxy, sxx, mtk, mpb, &it);
     }
     const double n = M->mm[t];
-    const double sigin = pow(sxx * sxx + sxy * sxy, 0.5);
+    const double sigin = sqrt(sxx * sxx + sxy * sxy);
     double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * mtk);
     if (e1 > 150.0) e1 = 150.0;
     e1 = exp(e1);
-    const double k1 = M->nu[t] / pow(M->ss[t], n - 1.0);
+    const double k1 = M->nu[t] / M->ssn1[t];
     const double eta_diff = 0.5 * k1 * e1;
     if (!(sigin > 0)) return clamp_visc(a, t, eta_diff);
     const double eta_disl0 = 0.5 * sigin * M->nu[t] * e1 / pow(sigin, n);

@@ -141,22 +141,19 @@ __global__ void k_merge_movers(const uint64_t *__restrict__ mov, long nmov, cons
 
 // ---------------------------------------------------------------------------
 // Segmented sums of one batch of 64 sorted markers.  Lanes [0, nact) hold NF
-// contributions in con[f][lane] and their sub-cell key in skey[lane]; equal
-// keys are contiguous.  Work items (run, field) are spread over the lanes and
-// each sums its run serially in lane order, so the result does not depend on
-// timing.  A sub-cell whose markers continue into the next batch (cont) is
-// carried in LDS (double-buffered by batch parity) and finished there; every
-// other run is complete and written once to out[(f*4 + q)*ncell + cell].
+// contributions in con[lane][f] and their sub-cell key in skey[lane]; equal
+// keys are contiguous.  Work items (run, field), run-major, are spread over
+// the lanes: each sums its run serially in lane order (the result does not
+// depend on timing), consecutive lanes read consecutive LDS words and write
+// consecutive fields of one sub-cell record.  A sub-cell whose markers
+// continue into the next batch (cont) is carried in LDS, double-buffered by
+// batch parity, and finished there; every other run is complete and its
+// record out[key*NF + f] is written once.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ long sub_index(int f, uint32_t key, long ncell)
-{
-    return (long)(f * 4 + (int)(key & 3u)) * ncell + (long)(key >> 2);
-}
-
 template <int NF>
-__device__ __forceinline__ void scatter_batch(const double (*con)[WAVE], const uint32_t *skey, int *runs,
+__device__ __forceinline__ void scatter_batch(const double (*con)[NF], const uint32_t *skey, int *runs,
                                               double (*carry)[NF], int par, bool have_carry, bool cont,
-                                              int lane, int nact, double *__restrict__ out, long ncell)
+                                              int lane, int nact, double *__restrict__ out)
 {
     const bool head = lane < nact && (lane == 0 || skey[lane - 1] != skey[lane]);
     const unsigned long long hm = __ballot(head);
@@ -165,32 +162,33 @@ __device__ __forceinline__ void scatter_batch(const double (*con)[WAVE], const u
     if (lane == 0) runs[nr] = nact;
     __syncthreads();
     for (int w = lane; w < nr * NF; w += WAVE) {
-        const int f = w / nr, r = w - f * nr;
+        const int r = w / NF, f = w - r * NF;
         const int b = runs[r], e = runs[r + 1];
         double sum = (r == 0 && have_carry) ? carry[par][f] : 0.0;
-        for (int j = b; j < e; j++) sum += con[f][j];
+        for (int j = b; j < e; j++) sum += con[j][f];
         if (r == nr - 1 && cont) carry[par ^ 1][f] = sum;
-        else out[sub_index(f, skey[b], ncell)] = sum;
+        else out[(long)skey[b] * NF + f] = sum;
     }
 }
 
-// Zero the sums of the group's empty sub-cells (the node passes read every
+// Zero the records of the group's empty sub-cells (the node passes read every
 // sub-cell of every cell).
 template <int NF>
 __device__ __forceinline__ void zero_empty(const KArgs &a, long c0, long c1, int lane, double *__restrict__ out)
 {
     const long k = c0 * 4 + lane;
     if (k < c1 * 4 && a.sub_start[k] == a.sub_start[k + 1])
-        for (int f = 0; f < NF; f++) out[sub_index(f, (uint32_t)k, a.ncell)] = 0.0;
+        for (int f = 0; f < NF; f++) out[k * NF + f] = 0.0;
 }
 
-// Cell indices of a marker inside the grid from its sub-cell key (the key is
-// computed by the same search from the same clamped position).
-__device__ __forceinline__ void key_cell(const KArgs &a, uint32_t key, int &nx, int &ny)
+// Cells of a marker from its sub-cell key (the key is computed by the same
+// search from the same clamped position), with their gridline records.
+__device__ __forceinline__ void key_axes(const KArgs &a, uint32_t key, Axis4 &X, Axis4 &Y)
 {
     const uint32_t cell = key >> 2, ncy = (uint32_t)(a.g.ynumy - 1);
-    nx = (int)(cell / ncy);
-    ny = (int)(cell - (uint32_t)nx * ncy);
+    const uint32_t nx = cell / ncy, ny = cell - nx * ncy;
+    X = a.g.cellx[nx];
+    Y = a.g.celly[ny];
 }
 
 // ---------------------------------------------------------------------------
@@ -209,11 +207,11 @@ __global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
     const DevGrid &g = a.g;
     const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
     if (!(x > 0 && y > 0 && x < g.xsize && y < g.ysize)) return;
-    int nx, ny;
-    key_cell(a, a.key[m], nx, ny);
+    Axis4 X, Y;
+    key_axes(a, a.key[m], X, Y);
     const int t = a.t[m];
     const double mtk = MK(a, MF_K, m);
-    const Stencil ps = centre_stencil(g, nx, ny, x, y);   // allinterpomp (interpolation.c:1923-1958)
+    const Stencil ps = centre_stencil(g, X, Y, x, y);   // allinterpomp (interpolation.c:1923-1958)
     const double mpb = 1e-5 * bil(NOD(a, V2C_N_pr), ps.m3, g.ynumy, wts(ps));
     double mro, mbb;
     dencalc(a, mtk, mpb, t, mro, mbb);
@@ -237,7 +235,7 @@ __global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
 {
-    __shared__ double con[M_NF][WAVE];
+    __shared__ double con[WAVE][M_NF];
     __shared__ double carry[2][M_NF];
     __shared__ uint32_t skey[WAVE];
     __shared__ int runs[WAVE + 1];
@@ -264,8 +262,8 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
             key = a.skey[i];
             const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
             if (x > 0 && y > 0 && x < g.xsize && y < g.ysize) {
-                int nx, ny;
-                key_cell(a, key, nx, ny);
+                Axis4 X, Y;
+                key_axes(a, key, X, Y);
                 const int t = a.t[m];
                 const double mbm = a.p.bulk_modulus;
                 const double mgg = M->gg[t];
@@ -275,9 +273,9 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
                 const double sxy = MK(a, MF_XY, m), sxx = MK(a, MF_XX, m);
                 const double phi = MK(a, MF_PHI, m);
                 const double me = MK(a, MF_E, m);
-                const double swt1 = 1.0 / (g.gx[nx + 1] - g.gx[nx]) / (g.gy[ny + 1] - g.gy[ny]);
-                const double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
-                const double cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny]);
+                const double swt1 = 1.0 / (X.g[2] - X.g[1]) / (Y.g[2] - Y.g[1]);
+                const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
+                const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
                 double bw = 0.0;
                 bool hasb = true;
                 if (cx < 0.5 && cy < 0.5) bw = swt1 * (1.0 - 2.0 * cx) * (1.0 - 2.0 * cy);
@@ -313,11 +311,11 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
             }
         }
 #pragma unroll
-        for (int f = 0; f < M_NF; f++) con[f][lane] = v[f];
+        for (int f = 0; f < M_NF; f++) con[lane][f] = v[f];
         skey[lane] = key;
         const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
         __syncthreads();
-        scatter_batch<M_NF>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a.part, a.ncell);
+        scatter_batch<M_NF>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a.part);
         have_carry = cont;
         __syncthreads();
     }
@@ -340,8 +338,9 @@ __global__ void k_m2n_nodes(KArgs a)
     auto cell = [&](int cx, int cy) -> long {
         return (cx >= 0 && cy >= 0 && cx < xnumx - 1 && cy < ynumy - 1) ? (long)cx * ncy + cy : -1;
     };
-    // Sums of sub-cell (cell, quadrant q = 2*(cx > .5) + (cy > .5)).
-    auto S = [&](int f, int q, long c) -> double { return P[((long)f * 4 + q) * ncell + c]; };
+    // Sums of sub-cell (cell, quadrant q = 2*(cx > .5) + (cy > .5)): one
+    // record of M_NF doubles per sub-cell key cell*4 + q.
+    auto S = [&](int f, int q, long c) -> double { return P[(c * 4 + q) * M_NF + f]; };
     // basic node: quadrant 3 of cell (m11-1,m22-1), 2 of (m11-1,m22), 1 of
     // (m11,m22-1), 0 of (m11,m22)
     double B[M_NB];
@@ -500,12 +499,13 @@ struct Mech {
 };
 
 // interpolate_mechanical gathers (interpolation.c:1060-1092, 1211-1237,
-// 1376-1397, 1433-1454) at a clamped position in cell (nx, ny).
-__device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, int nx, int ny, Mech &o)
+// 1376-1397, 1433-1454) at a clamped position in cell (X.s, Y.s).
+__device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, const Axis4 &X, const Axis4 &Y,
+                                            Mech &o)
 {
     const DevGrid &g = a.g;
     const int ynumy = g.ynumy;
-    Stencil s = basic_stencil(g, nx, ny, x, y);
+    Stencil s = basic_stencil(g, X, Y, x, y);
     W4 w = wts(s);
     o.EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
     o.EXYE = bil(NOD(a, V2C_N_exye), s.m3, ynumy, w);
@@ -515,7 +515,7 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
     o.ESE = bil(NOD(a, V2C_N_pe), s.m3, ynumy, w);
     o.PHISE = bil(NOD(a, V2C_N_phis), s.m3, ynumy, w);
     o.PHIS = bil(NOD(a, V2C_N_phis_it), s.m3, ynumy, w);
-    s = centre_stencil(g, nx, ny, x, y);
+    s = centre_stencil(g, X, Y, x, y);
     w = wts(s);
     o.EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
     o.EXXE = bil(NOD(a, V2C_N_exxe), s.m3, ynumy, w);
@@ -523,11 +523,11 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
     o.SXXE = bil(NOD(a, V2C_N_sxxe), s.m3, ynumy, w);
     o.PR = bil(NOD(a, V2C_N_pr), s.m3, ynumy, w);
     o.SPPE = bil(NOD(a, V2C_N_sppe), s.m3, ynumy, w);
-    s = vx_stencil(g, nx, ny, x, y, false);
+    s = vx_stencil(g, X, Y, x, y, false);
     w = wts(s);
     o.VX = bil(NOD(a, V2C_N_vx), s.m3, ynumy, w);
     o.MVX = bil(NOD(a, V2C_N_mvx), s.m3, ynumy, w);
-    s = vy_stencil(g, nx, ny, x, y, false);
+    s = vy_stencil(g, X, Y, x, y, false);
     w = wts(s);
     o.VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, w);
     o.MVY = bil(NOD(a, V2C_N_mvy), s.m3, ynumy, w);
@@ -536,30 +536,30 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
 // Antidiffusion scatter-back contributions of one marker (interpolation.c:
 // 1159-1209, 1360-1374, 1405-1489): the interpolated PHIS, SXY, EXY, ES, PR,
 // SXX, EXX, VX, VY times the node weights, at the clamped position in cell
-// (nx, ny).  Only these nine interpolants are gathered.
-__device__ __forceinline__ void anti_contrib(const KArgs &a, double x, double y, int nx, int ny, double mro,
-                                             double *v /* N_NF, zeroed */)
+// (X.s, Y.s).  Only these nine interpolants are gathered.
+__device__ __forceinline__ void anti_contrib(const KArgs &a, double x, double y, const Axis4 &X, const Axis4 &Y,
+                                             double mro, double *v /* N_NF, zeroed */)
 {
     const DevGrid &g = a.g;
     const int ynumy = g.ynumy;
-    Stencil s = basic_stencil(g, nx, ny, x, y);
+    Stencil s = basic_stencil(g, X, Y, x, y);
     W4 w = wts(s);
     const double EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
     const double SXY = bil(NOD(a, V2C_N_sxy), s.m3, ynumy, w);
     const double ES = bil(NOD(a, V2C_N_pes), s.m3, ynumy, w);
     const double PHIS = bil(NOD(a, V2C_N_phis_it), s.m3, ynumy, w);
-    s = centre_stencil(g, nx, ny, x, y);
+    s = centre_stencil(g, X, Y, x, y);
     w = wts(s);
     const double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
     const double SXX = bil(NOD(a, V2C_N_sxx), s.m3, ynumy, w);
     const double PR = bil(NOD(a, V2C_N_pr), s.m3, ynumy, w);
-    s = vx_stencil(g, nx, ny, x, y, false);
+    s = vx_stencil(g, X, Y, x, y, false);
     const double VX = bil(NOD(a, V2C_N_vx), s.m3, ynumy, wts(s));
-    s = vy_stencil(g, nx, ny, x, y, false);
+    s = vy_stencil(g, X, Y, x, y, false);
     const double VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, wts(s));
-    const double swt1 = 1.0 / (g.gx[nx + 1] - g.gx[nx]) / (g.gy[ny + 1] - g.gy[ny]);
-    const double cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx]);
-    const double cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny]);
+    const double swt1 = 1.0 / (X.g[2] - X.g[1]) / (Y.g[2] - Y.g[1]);
+    const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
     double bw = 0.0;
     bool hasb = true;
     if (cx < 0.5 && cy < 0.5) bw = swt1 * (1.0 - 2 * cx) * (1.0 - 2 * cy);
@@ -620,9 +620,9 @@ __device__ __forceinline__ void n2m_update(const KArgs &a, int t, MState &s, con
 
 // moveandrotatemarkers body for one marker of type < 100 (moveandrotatemarkers.c:12-88).
 // Returns false (and changes nothing) for types >= 100.
-// (hx, hy): the cell of the marker's clamped position, or -1 to search.
+// (hx, hy): the cells of the marker's clamped position, or null to search.
 __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, bool &moved, bool &reset,
-                                           int hx = -1, int hy = -1)
+                                           const Axis4 *hx = nullptr, const Axis4 *hy = nullptr)
 {
     const DevGrid &g = a.g;
     const v2c_params &p = a.p;
@@ -683,7 +683,7 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
 // per-cell partials.
 __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
 {
-    __shared__ double con[N_NF][WAVE];
+    __shared__ double con[WAVE][N_NF];
     __shared__ double carry[2][N_NF];
     __shared__ uint32_t skey[WAVE];
     __shared__ int runs[WAVE + 1];
@@ -705,17 +705,17 @@ __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
         if (i < end) {
             const long m = a.order[i];
             key = a.skey[i];
-            int nx, ny;
-            key_cell(a, key, nx, ny);
+            Axis4 X, Y;
+            key_axes(a, key, X, Y);
             const double x = clampd(MK(a, MF_X, m), 0, g.xsize), y = clampd(MK(a, MF_Y, m), 0, g.ysize);
-            anti_contrib(a, x, y, nx, ny, a.mat->ro[a.t[m]], v);
+            anti_contrib(a, x, y, X, Y, a.mat->ro[a.t[m]], v);
         }
 #pragma unroll
-        for (int f = 0; f < N_NF; f++) con[f][lane] = v[f];
+        for (int f = 0; f < N_NF; f++) con[lane][f] = v[f];
         skey[lane] = key;
         const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
         __syncthreads();
-        scatter_batch<N_NF>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a.part, a.ncell);
+        scatter_batch<N_NF>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a.part);
         have_carry = cont;
         __syncthreads();
     }
@@ -734,8 +734,8 @@ __global__ __launch_bounds__(256) void k_n2m_update(KArgs a, int nwg)
         const DevGrid &g = a.g;
         const bool anti = a.p.antidiffinter == 1;
         const int t = a.t[m];
-        int nx, ny;
-        key_cell(a, a.key[m], nx, ny);
+        Axis4 X, Y;
+        key_axes(a, a.key[m], X, Y);
         MState s;
         s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
         s.dirty_state = false; s.dirty_k = false;
@@ -746,7 +746,7 @@ __global__ __launch_bounds__(256) void k_n2m_update(KArgs a, int nwg)
         }
         {
             Mech mv;
-            mech_gather(a, clampd(s.x, 0, g.xsize), clampd(s.y, 0, g.ysize), nx, ny, mv);
+            mech_gather(a, clampd(s.x, 0, g.xsize), clampd(s.y, 0, g.ysize), X, Y, mv);
             n2m_update(a, t, s, mv);
         }
         if (s.dirty_state) {
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(256) void k_n2m_update(KArgs a, int nwg)
         bool moved = false, reset = false;
         if (ADVECT) {
             if (!s.dirty_state) { s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); }
-            if (!advect_one(a, t, s, moved, reset, nx, ny)) unsupported = true;
+            if (!advect_one(a, t, s, moved, reset, &X, &Y)) unsupported = true;
         }
         if (s.dirty_state || moved) { MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy; }
         if (s.dirty_k) MK(a, MF_K, m) = s.k;
@@ -782,7 +782,7 @@ __global__ void k_n2m_phantom(KArgs a)
     if (threadIdx.x != 0 || blockIdx.x != 0 || a.p.antidiffinter != 1) return;
     double v[N_NF];
     for (int f = 0; f < N_NF; f++) v[f] = 0.0;
-    anti_contrib(a, 0.0, 0.0, find_nx(a.g, 0.0), find_ny(a.g, 0.0), a.mat->ro[0], v);
+    anti_contrib(a, 0.0, 0.0, find_x(a.g, 0.0), find_y(a.g, 0.0), a.mat->ro[0], v);
     const int ynumy = a.g.ynumy;
     // at (0,0): cx = cy = 0 -> basic node 0, vx node 0, vy node 0, centre (1,1)
     const long cn = (long)ynumy + 1;
@@ -807,7 +807,7 @@ __global__ void k_n2m_nodes(KArgs a)
     auto cell = [&](int cx, int cy) -> long {
         return (cx >= 0 && cy >= 0 && cx < xnumx - 1 && cy < ynumy - 1) ? (long)cx * ncy + cy : -1;
     };
-    auto S = [&](int f, int q, long c) -> double { return P[((long)f * 4 + q) * ncell + c]; };
+    auto S = [&](int f, int q, long c) -> double { return P[(c * 4 + q) * N_NF + f]; };
     double B[N_NB] = {0, 0, 0, 0, 0};
     {
         const long cs[4] = {cell(m11 - 1, m22 - 1), cell(m11 - 1, m22), cell(m11, m22 - 1), cell(m11, m22)};
@@ -876,10 +876,10 @@ __global__ __launch_bounds__(256) void k_advect(KArgs a)
         s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
         s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m);
         s.dirty_k = false;
-        int nx, ny;
-        key_cell(a, a.key[m], nx, ny);
+        Axis4 X, Y;
+        key_axes(a, a.key[m], X, Y);
         bool moved, reset;
-        if (!advect_one(a, t, s, moved, reset, nx, ny)) unsupported = 1;
+        if (!advect_one(a, t, s, moved, reset, &X, &Y)) unsupported = 1;
         if (moved) {
             MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
             MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy;

@@ -3,31 +3,59 @@
 // The reference bisects the gridline array.  Its result is the number of
 // interior gridlines g[1..n-2] not greater than x; NaN lands in the last cell
 // because every comparison with it fails.  The same count is found here from a
-// table of equal-width bins and a short walk: each bin stores the count at its
-// left edge and the two gridlines bounding that cell; the bin width is at most
-// the smallest spacing, so at most one gridline lies inside a bin and the walk
-// takes 0-2 steps (one dependent load each) instead of log2(n).
+// table of equal-width bins and a short walk.  Each bin stores the count at
+// its left edge together with the gridlines g[s-1..s+2] around that cell; the
+// bin width is at most the smallest spacing, so at most one gridline lies
+// inside a bin and the walk takes 0-2 steps.  In the common case one load of
+// the bin gives both the cell and every gridline the interpolation stencils
+// need, instead of log2(n) dependent loads and a second round of gridline
+// loads.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 namespace v2c {
 
-struct AxisBin { double glo, ghi; int s, pad; };
+// Cell s of an axis with its gridlines g[k] = grid[s-1+k], k = 0..3 (entries
+// past either end of the axis hold NaN; no stencil reads them).
+struct Axis4 { double g[4]; int s, pad; };
 
-__host__ __device__ inline int axis_find(const double *__restrict__ g, int n, const AxisBin *__restrict__ bins,
-                                         int nb, double b0, double binv, double x)
+__host__ __device__ inline double pick(const Axis4 &A, int m)
 {
-    if (x != x) return n - 2;
+    const int k = m - A.s + 1;
+    return k <= 0 ? A.g[0] : k == 1 ? A.g[1] : k == 2 ? A.g[2] : A.g[3];
+}
+
+__host__ __device__ inline Axis4 axis_cell(const double *__restrict__ grid, int n, int s)
+{
+    Axis4 a;
+    a.s = s;
+    a.pad = 0;
+    for (int k = 0; k < 4; k++) {
+        const int i = s - 1 + k;
+        a.g[k] = (i >= 0 && i < n) ? grid[i] : NAN;
+    }
+    return a;
+}
+
+// bins[j]: the cell at the bin's left edge.  cells[s]: cell s (0 <= s <= n-2).
+__host__ __device__ inline Axis4 axis_find(const double *__restrict__ grid, int n, const Axis4 *__restrict__ bins,
+                                           const Axis4 *__restrict__ cells, int nb, double b0, double binv,
+                                           double x)
+{
+    if (x != x) return cells[n - 2];
     const double t = (x - b0) * binv;
     const int j = t > 0 ? (t < (double)(nb - 1) ? (int)t : nb - 1) : 0;
-    const AxisBin e = bins[j];
+    const Axis4 e = bins[j];
     int s = e.s;
-    double lo = e.glo, hi = e.ghi;
-    while (s < n - 2 && !(hi > x)) { s++; lo = hi; hi = g[s + 1]; }
-    while (s > 0 && lo > x) { s--; hi = lo; lo = g[s]; }
-    return s;
+    const bool up = s < n - 2 && !(e.g[2] > x), down = s > 0 && e.g[1] > x;
+    if (!up && !down) return e;
+    double lo = e.g[1], hi = e.g[2];
+    while (s < n - 2 && !(hi > x)) { s++; lo = hi; hi = grid[s + 1]; }
+    while (s > 0 && lo > x) { s--; hi = lo; lo = grid[s]; }
+    return cells[s];
 }
 
 // The reference's bisection, used to fill the tables.
@@ -41,9 +69,10 @@ inline int host_bisect(const double *g, int n, double x)
     return lo > n - 2 ? n - 2 : lo;
 }
 
-// Table of one strictly increasing axis: bin width = the smallest spacing,
+// Tables of one strictly increasing axis: bin width = the smallest spacing,
 // at most 2^22 bins (wider bins on extreme grids only lengthen the walk).
-inline std::vector<AxisBin> make_bins(const double *g, int n, double &b0, double &binv)
+inline void make_tables(const double *g, int n, std::vector<Axis4> &bins, std::vector<Axis4> &cells,
+                        double &b0, double &binv)
 {
     double hmin = g[1] - g[0];
     for (int i = 1; i + 1 < n; i++) hmin = std::min(hmin, g[i + 1] - g[i]);
@@ -52,14 +81,12 @@ inline std::vector<AxisBin> make_bins(const double *g, int n, double &b0, double
     double h = hmin;
     if (range / h + 2 > (double)cap) h = range / (double)(cap - 2);
     const long nb = (long)(range / h) + 2;
-    std::vector<AxisBin> b((size_t)nb);
-    for (long j = 0; j < nb; j++) {
-        const int s = host_bisect(g, n, g[0] + (double)j * h);
-        b[(size_t)j] = AxisBin{g[s], g[s + 1], s, 0};
-    }
+    cells.resize((size_t)(n - 1));
+    for (int s = 0; s < n - 1; s++) cells[(size_t)s] = axis_cell(g, n, s);
+    bins.resize((size_t)nb);
+    for (long j = 0; j < nb; j++) bins[(size_t)j] = cells[(size_t)host_bisect(g, n, g[0] + (double)j * h)];
     b0 = g[0];
     binv = 1.0 / h;
-    return b;
 }
 
 }  // namespace v2c
