@@ -190,6 +190,13 @@ long v2c_marker_count(const v2c_ctx *ctx);
  * re-ordering, number of full radix sorts and of incremental merges so far. */
 int v2c_order_stats(v2c_ctx *ctx, long *last_movers, long *full_sorts, long *incremental);
 
+/* Brings the cell ordering up to date (as the next marker2nodes would) and
+ * copies it out: order[i] is the marker id at sorted position i and skey[i]
+ * its sub-cell key (cell*4 + quadrant), i < n <= marker count.  No reference
+ * counterpart: the reference keeps no ordering; this exposes the build's own
+ * state for inspection and tests. */
+int v2c_order_get(v2c_ctx *ctx, unsigned int *order, unsigned int *skey, long n);
+
 /* Nodal mirror transfer, one field at a time (length nodenum). */
 int v2c_nodal_set(v2c_ctx *ctx, int field, const double *host);
 int v2c_nodal_get(v2c_ctx *ctx, int field, double *host);

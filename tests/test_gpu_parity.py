@@ -246,3 +246,31 @@ def test_production_size_properties(gpu):
     assert cw[inside].sum() > 0
     ctx.close()
     ctx2.close()
+
+
+@pytest.mark.parametrize("dt", [1e9, 1e10])
+def test_incremental_order_equals_full_sort(gpu, dt):
+    """After advection, the incremental merge of movers into the kept order
+    gives exactly the (key, id) order a full radix sort of the new keys gives."""
+    from velo2cycles_amd.abi import Context
+    cfg = synth.config_small()
+    cfg.timesum = DT
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
+    ctx.set_time(cfg.timesum, dt)
+    ctx.set_all_nodal(cfg.nodal)
+    ctx.upload_markers(cfg.markers)
+    ctx.order_get()                       # full sort of the initial keys
+    ctx.nodes2markers_advect()
+    o_inc, k_inc = ctx.order_get()        # incremental merge
+    st = ctx.order_stats()
+    assert st["incremental"] == 1 and st["full_sorts"] == 1 and 0 < st["last_movers"] <= cfg.n // 4
+    m = cfg.markers.copy()
+    ctx.download_markers(m)
+    ctx.close()
+    ctx2 = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
+    ctx2.upload_markers(m)
+    o_full, k_full = ctx2.order_get()
+    ctx2.close()
+    assert np.array_equal(k_inc, k_full)
+    assert np.array_equal(o_inc, o_full)
+    assert np.all(np.diff(k_full.astype(np.int64)) >= 0)

@@ -213,6 +213,7 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                        ("v2c_set_friction_fields", [C.c_int]),
                        ("v2c_copy_calibration", [C.c_long, P(C.c_double)]),
                        ("v2c_order_stats", [P(C.c_long), P(C.c_long), P(C.c_long)]),
+                       ("v2c_order_get", [P(C.c_uint), P(C.c_uint), C.c_long]),
                        ("v2c_rsf_upload", [C.c_char_p] + [P(C.c_double)] * 7 + [C.c_long]),
                        ("v2c_local_update", [C.c_double, C.c_int, C.c_double, P(LUStats)]),
                        ("v2c_local_download", [P(C.c_double)] * 5 + [C.c_long]),
@@ -351,6 +352,15 @@ def _opt(a):
 
 
 def _ctx_extras():
+    def order_get(self):
+        """(order, skey) of the current cell ordering, brought up to date."""
+        n = self.lib.v2c_marker_count(self.h)
+        order = np.zeros(n, dtype=np.uint32)
+        skey = np.zeros(n, dtype=np.uint32)
+        self._check(self.lib.v2c_order_get(self.h, order.ctypes.data_as(C.POINTER(C.c_uint)),
+                                           skey.ctypes.data_as(C.POINTER(C.c_uint)), n), "order_get")
+        return order, skey
+
     def order_stats(self):
         v = [C.c_long(0) for _ in range(3)]
         self._check(self.lib.v2c_order_stats(self.h, *[C.byref(x) for x in v]), "order_stats")

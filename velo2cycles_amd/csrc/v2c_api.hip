@@ -44,8 +44,8 @@ struct v2c_ctx {
     double *d_nod = nullptr, *d_part = nullptr, *d_red = nullptr, *d_prop = nullptr;
     int *d_counters = nullptr;
     // incremental re-ordering buffers
-    uint32_t *d_order2 = nullptr, *d_skey2 = nullptr, *d_ids_mv = nullptr, *d_ids_st = nullptr;
-    uint8_t *d_flag_mv = nullptr, *d_flag_st = nullptr;
+    uint32_t *d_order2 = nullptr, *d_skey2 = nullptr, *d_tile_count = nullptr, *d_tile_off = nullptr;
+    unsigned long long *d_mask = nullptr;
     uint64_t *d_k64_in = nullptr, *d_k64_out = nullptr;
     long *d_nsel = nullptr;
     // local update
@@ -209,12 +209,13 @@ static int ensure_order(v2c_ctx *c)
         c->order_valid = true;
         return 0;
     }
-    const int n = (int)c->n;
+    const long n = c->n;
+    const long ntiles = (n + ORD_TILE - 1) / ORD_TILE;
     rc = launch(c, V2C_K_SORT, [&] {
-        hipLaunchKernelGGL(k_mover_flags, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->d_order,
-                           c->d_key, c->d_skey, (long)n, c->d_flag_mv, c->d_flag_st);
-        return hipcub::DeviceSelect::Flagged(c->d_sort_tmp, c->sort_tmp_bytes, c->d_order, c->d_flag_mv,
-                                             c->d_ids_mv, c->d_nsel, n, c->stream);
+        hipLaunchKernelGGL(k_order_flags, dim3((unsigned)ntiles), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
+                           c->d_key, c->d_skey, n, c->d_mask, c->d_tile_count);
+        hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, c->stream, c->d_tile_count, ntiles,
+                           c->d_tile_off, c->d_nsel);
     });
     if (rc) return rc;
     long nmov = 0;
@@ -231,21 +232,17 @@ static int ensure_order(v2c_ctx *c)
         c->order_valid = true;
         return 0;
     }
-    const long nstay = n - nmov;
     rc = launch(c, V2C_K_SORT, [&] {
-        hipError_t e = hipcub::DeviceSelect::Flagged(c->d_sort_tmp, c->sort_tmp_bytes, c->d_order, c->d_flag_st,
-                                                     c->d_ids_st, c->d_nsel + 1, n, c->stream);
+        hipLaunchKernelGGL(k_order_movers, dim3((unsigned)ntiles), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
+                           c->d_key, n, c->d_mask, c->d_tile_off, c->d_k64_in);
+        hipError_t e = hipcub::DeviceRadixSort::SortKeys(c->d_sort_tmp, c->sort_tmp_bytes, c->d_k64_in,
+                                                         c->d_k64_out, (int)nmov, 0, 32 + c->key_bits, c->stream);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_mover_keys, dim3(blocks_for(nmov, 256)), dim3(256), 0, c->stream, c->d_ids_mv,
-                           c->d_nsel, c->d_key, c->d_k64_in);
-        e = hipcub::DeviceRadixSort::SortKeys(c->d_sort_tmp, c->sort_tmp_bytes, c->d_k64_in, c->d_k64_out,
-                                              (int)nmov, 0, 32 + c->key_bits, c->stream);
-        if (e != hipSuccess) return e;
-        if (nstay > 0)
-            hipLaunchKernelGGL(k_merge_stayers, dim3(blocks_for(nstay, 256)), dim3(256), 0, c->stream,
-                               c->d_ids_st, nstay, c->d_k64_out, nmov, c->d_key, c->d_order2, c->d_skey2);
-        hipLaunchKernelGGL(k_merge_movers, dim3(blocks_for(nmov, 256)), dim3(256), 0, c->stream, c->d_k64_out,
-                           nmov, c->d_ids_st, nstay, c->d_key, c->d_order2, c->d_skey2);
+        hipLaunchKernelGGL(k_order_stayers, dim3((unsigned)ntiles), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
+                           c->d_skey, n, c->d_mask, c->d_tile_off, c->d_k64_out, nmov, c->d_order2, c->d_skey2);
+        hipLaunchKernelGGL(k_order_place_movers, dim3(blocks_for(nmov, 256)), dim3(256), 0, c->stream,
+                           c->d_k64_out, nmov, c->d_order, c->d_skey, n, c->d_mask, c->d_tile_off,
+                           c->d_order2, c->d_skey2);
         return hipSuccess;
     });
     if (rc) return rc;
@@ -303,7 +300,7 @@ static void free_all(v2c_ctx *c)
 {
     void *ptrs[] = {c->d_gx, c->d_gy, c->d_bx, c->d_by, c->d_cellx, c->d_celly, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
                     c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_prop, c->d_red, c->d_counters, c->d_sort_tmp,
-                    c->d_order2, c->d_skey2, c->d_ids_mv, c->d_ids_st, c->d_flag_mv, c->d_flag_st,
+                    c->d_order2, c->d_skey2, c->d_tile_count, c->d_tile_off, c->d_mask,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
                     c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm};
     for (void *p : ptrs)
@@ -362,16 +359,18 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_order, sizeof(uint32_t) * c->cap));
     CK(hipMalloc(&c->d_cs, sizeof(uint32_t) * (4 * c->ncell + 1)));
     CK(hipMalloc(&c->d_nod, sizeof(double) * V2C_NF * c->nodenum));
-    CK(hipMalloc(&c->d_part, sizeof(double) * 4 * M_NF * c->ncell));
+    CK(hipMalloc(&c->d_part, sizeof(double) * 4 * M_NF * c->nodenum));
     CK(hipMalloc(&c->d_prop, sizeof(double) * NPROP * c->cap));
     CK(hipMalloc(&c->d_red, sizeof(double) * (2 * blocks_for(c->nodenum, RED_BLOCK) + 8)));
     CK(hipMalloc(&c->d_counters, sizeof(int) * 8));
     CK(hipMalloc(&c->d_order2, sizeof(uint32_t) * c->cap));
     CK(hipMalloc(&c->d_skey2, sizeof(uint32_t) * c->cap));
-    CK(hipMalloc(&c->d_ids_mv, sizeof(uint32_t) * c->cap));
-    CK(hipMalloc(&c->d_ids_st, sizeof(uint32_t) * c->cap));
-    CK(hipMalloc(&c->d_flag_mv, c->cap));
-    CK(hipMalloc(&c->d_flag_st, c->cap));
+    {
+        const long ntiles = (c->cap + ORD_TILE - 1) / ORD_TILE;
+        CK(hipMalloc(&c->d_tile_count, sizeof(uint32_t) * (ntiles + 1)));
+        CK(hipMalloc(&c->d_tile_off, sizeof(uint32_t) * (ntiles + 1)));
+        CK(hipMalloc(&c->d_mask, sizeof(unsigned long long) * (ntiles * ORD_MASKS + 1)));
+    }
     CK(hipMalloc(&c->d_k64_in, sizeof(uint64_t) * c->cap));
     CK(hipMalloc(&c->d_k64_out, sizeof(uint64_t) * c->cap));
     CK(hipMalloc(&c->d_nsel, sizeof(long) * 2));
@@ -430,14 +429,12 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     hipLaunchKernelGGL(k_iota, dim3(blocks_for(c->cap, 256)), dim3(256), 0, c->stream, c->d_iota, c->cap);
     CK(hipGetLastError());
     {
-        size_t b1 = 0, b2 = 0, b3 = 0;
+        size_t b1 = 0, b3 = 0;
         CK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c->d_key, c->d_skey, c->d_iota, c->d_order,
                                               (int)c->cap, 0, c->key_bits, c->stream));
-        CK(hipcub::DeviceSelect::Flagged(nullptr, b2, c->d_order, c->d_flag_mv, c->d_ids_mv, c->d_nsel,
-                                         (int)c->cap, c->stream));
         CK(hipcub::DeviceRadixSort::SortKeys(nullptr, b3, c->d_k64_in, c->d_k64_out, (int)c->cap, 0,
                                              32 + c->key_bits, c->stream));
-        c->sort_tmp_bytes = std::max(b1, std::max(b2, b3));
+        c->sort_tmp_bytes = std::max(b1, b3);
     }
     CK(hipMalloc(&c->d_sort_tmp, c->sort_tmp_bytes > 0 ? c->sort_tmp_bytes : 16));
     CK(hipStreamSynchronize(c->stream));
@@ -522,6 +519,20 @@ int v2c_order_stats(v2c_ctx *c, long *last_movers, long *full_sorts, long *incre
     if (last_movers) *last_movers = c->last_movers;
     if (full_sorts) *full_sorts = c->n_full_sorts;
     if (incremental) *incremental = c->n_incremental;
+    return 0;
+}
+
+int v2c_order_get(v2c_ctx *c, unsigned int *order, unsigned int *skey, long n)
+{
+    if (!c || n < 0 || n > c->n) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_order(c);
+    if (rc) return rc;
+    if (n > 0 && order)
+        HIPCHK(c, hipMemcpyAsync(order, c->d_order, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    if (n > 0 && skey)
+        HIPCHK(c, hipMemcpyAsync(skey, c->d_skey, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -80,81 +80,279 @@ __global__ void k_sub_start(const uint32_t *__restrict__ skey, long n, long nsub
 // ---------------------------------------------------------------------------
 // Incremental re-ordering after advection.  The order is always the (key, id)
 // lexicographic order (a stable radix sort of keys over ids produces exactly
-// that).  Markers whose key did not change keep that order among themselves,
-// so the new order is the merge of the unchanged markers (already sorted) with
-// the few movers sorted by (key, id): every element's output position is its
-// own rank plus the number of elements of the other list that precede it.
+// that).  Markers whose key did not change ("stayers") keep that order among
+// themselves, so the new order is the merge of the stayers, already sorted,
+// with the few movers sorted by (key, id): a stayer lands at its rank among
+// stayers plus the number of movers before it, a mover at its rank among
+// movers plus the number of stayers before it.
+//
+// The old sorted positions are cut into tiles of ORD_TILE; a 64-bit ballot
+// mask per 64 positions marks the movers, and an exclusive scan of the tile
+// counts gives every position's mover prefix without a global compaction.
 // ---------------------------------------------------------------------------
-__global__ void k_mover_flags(const uint32_t *__restrict__ order, const uint32_t *__restrict__ key,
-                              const uint32_t *__restrict__ skey, long n, uint8_t *mv, uint8_t *stay)
-{
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const bool moved = key[order[i]] != skey[i];
-    mv[i] = moved ? 1 : 0;
-    stay[i] = moved ? 0 : 1;
-}
+constexpr int ORD_BLOCK = 256, ORD_ITEMS = 16, ORD_TILE = ORD_BLOCK * ORD_ITEMS;
+constexpr int ORD_MASKS = ORD_TILE / WAVE;        // masks per tile
 
 __device__ __forceinline__ uint64_t kid(uint32_t k, uint32_t id) { return ((uint64_t)k << 32) | id; }
 
-__global__ void k_mover_keys(const uint32_t *__restrict__ ids, const long *__restrict__ nsel,
-                             const uint32_t *__restrict__ key, uint64_t *out)
+// Pass 1: mover masks and mover count per tile.
+__global__ __launch_bounds__(ORD_BLOCK) void k_order_flags(const uint32_t *__restrict__ order,
+                                                           const uint32_t *__restrict__ key,
+                                                           const uint32_t *__restrict__ skey, long n,
+                                                           unsigned long long *__restrict__ mask,
+                                                           uint32_t *__restrict__ tile_count)
 {
-    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= *nsel) return;
-    out[r] = kid(key[ids[r]], ids[r]);
-}
-
-__global__ void k_merge_stayers(const uint32_t *__restrict__ stay, long nstay, const uint64_t *__restrict__ mov,
-                                long nmov, const uint32_t *__restrict__ key, uint32_t *order_out,
-                                uint32_t *skey_out)
-{
-    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nstay) return;
-    const uint32_t id = stay[j], k = key[id];
-    const uint64_t c = kid(k, id);
-    long lo = 0, hi = nmov;
-    while (lo < hi) {
-        long mid = (lo + hi) >> 1;
-        if (mov[mid] < c) lo = mid + 1; else hi = mid;
+    __shared__ uint32_t wsum[ORD_BLOCK / WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const long t0 = (long)blockIdx.x * ORD_TILE;
+    uint32_t cnt = 0;
+    for (int k = 0; k < ORD_ITEMS; k++) {
+        const long i = t0 + (long)k * ORD_BLOCK + threadIdx.x;
+        const bool mv = i < n && key[order[i]] != skey[i];
+        const unsigned long long b = __ballot(mv);
+        if (lane == 0 && t0 + (long)k * ORD_BLOCK + wv * WAVE < n) mask[(t0 + (long)k * ORD_BLOCK) / WAVE + wv] = b;
+        cnt += (uint32_t)__popcll(b);
     }
-    order_out[j + lo] = id;
-    skey_out[j + lo] = k;
+    if (lane == 0) wsum[wv] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < ORD_BLOCK / WAVE; w++) t += wsum[w];
+        tile_count[blockIdx.x] = t;
+    }
 }
 
-__global__ void k_merge_movers(const uint64_t *__restrict__ mov, long nmov, const uint32_t *__restrict__ stay,
-                               long nstay, const uint32_t *__restrict__ key, uint32_t *order_out,
-                               uint32_t *skey_out)
+// Pass 2: exclusive scan of the tile counts (one workgroup); off[ntiles] = total.
+__global__ __launch_bounds__(1024) void k_order_scan(const uint32_t *__restrict__ cnt, long ntiles,
+                                                     uint32_t *__restrict__ off, long *__restrict__ total)
 {
-    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t part[1024];
+    const long per = (ntiles + 1023) / 1024;
+    const long b = threadIdx.x * per, e = b + per < ntiles ? b + per : ntiles;
+    uint32_t s = 0;
+    for (long i = b; i < e; i++) s += cnt[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {          // inclusive Hillis-Steele scan
+        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0u;
+    for (long i = b; i < e; i++) { off[i] = run; run += cnt[i]; }
+    if (threadIdx.x == 1023) { off[ntiles] = part[1023]; *total = part[1023]; }
+}
+
+// Movers before old position i: tile offset + masks of the tile before i's
+// mask + the bits of i's mask below i.
+__device__ __forceinline__ uint32_t movers_before(const unsigned long long *__restrict__ mask,
+                                                  const uint32_t *__restrict__ off, long i)
+{
+    const long tile = i / ORD_TILE, w = i / WAVE;
+    uint32_t c = off[tile];
+    for (long q = tile * ORD_MASKS; q < w; q++) c += (uint32_t)__popcll(mask[q]);
+    const int bit = (int)(i & (WAVE - 1));
+    return c + (uint32_t)__popcll(mask[w] & ((1ull << bit) - 1ull));
+}
+
+// Pass 3: movers' (key, id) compacted in old-position order.
+__global__ __launch_bounds__(ORD_BLOCK) void k_order_movers(const uint32_t *__restrict__ order,
+                                                            const uint32_t *__restrict__ key, long n,
+                                                            const unsigned long long *__restrict__ mask,
+                                                            const uint32_t *__restrict__ off,
+                                                            uint64_t *__restrict__ mov)
+{
+    __shared__ uint32_t wpre[ORD_BLOCK / WAVE + 1];
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const long t0 = (long)blockIdx.x * ORD_TILE;
+    uint32_t base = off[blockIdx.x];
+    if (off[blockIdx.x + 1] == base) return;      // no movers in this tile
+    for (int k = 0; k < ORD_ITEMS; k++) {
+        const long c0 = t0 + (long)k * ORD_BLOCK;
+        if (c0 >= n) break;
+        const long i = c0 + threadIdx.x;
+        const unsigned long long b = c0 + wv * WAVE < n ? mask[c0 / WAVE + wv] : 0ull;
+        if (lane == 0) wpre[wv] = (uint32_t)__popcll(b);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t r = 0;
+            for (int w = 0; w < ORD_BLOCK / WAVE; w++) { const uint32_t t = wpre[w]; wpre[w] = r; r += t; }
+            wpre[ORD_BLOCK / WAVE] = r;
+        }
+        __syncthreads();
+        if ((b >> lane) & 1ull) {
+            const uint32_t id = order[i];
+            mov[base + wpre[wv] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = kid(key[id], id);
+        }
+        base += wpre[ORD_BLOCK / WAVE];
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ long lower_bound64(const uint64_t *__restrict__ a, long lo, long hi, uint64_t c)
+{
+    while (lo < hi) {
+        const long mid = (lo + hi) >> 1;
+        if (a[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Pass 5: stayers to their new positions.  The movers that can precede any
+// stayer of the tile after the first ones are confined to [lo, hi), found
+// once per tile; each stayer counts within that window.
+__global__ __launch_bounds__(ORD_BLOCK) void k_order_stayers(const uint32_t *__restrict__ order,
+                                                             const uint32_t *__restrict__ skey, long n,
+                                                             const unsigned long long *__restrict__ mask,
+                                                             const uint32_t *__restrict__ off,
+                                                             const uint64_t *__restrict__ mov, long nmov,
+                                                             uint32_t *__restrict__ order_out,
+                                                             uint32_t *__restrict__ skey_out)
+{
+    __shared__ long win[2];
+    __shared__ uint32_t wpre[ORD_BLOCK / WAVE + 1];
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const long t0 = (long)blockIdx.x * ORD_TILE;
+    const long t1 = t0 + ORD_TILE < n ? t0 + ORD_TILE : n;
+    if (threadIdx.x == 0) {
+        // stayer keys of this tile lie in [skey[t0], skey[t1-1]]
+        win[0] = lower_bound64(mov, 0, nmov, kid(skey[t0], 0u));
+        win[1] = lower_bound64(mov, win[0], nmov, kid(skey[t1 - 1], 0xffffffffu) + 1ull);
+        if (skey[t1 - 1] == 0xffffffffu) win[1] = nmov;
+    }
+    uint32_t moved = off[blockIdx.x];               // movers before the current chunk
+    __syncthreads();
+    const long lo = win[0], hi = win[1];
+    for (int k = 0; k < ORD_ITEMS; k++) {
+        const long c0 = t0 + (long)k * ORD_BLOCK;
+        if (c0 >= n) break;
+        const long i = c0 + threadIdx.x;
+        const unsigned long long b = c0 + wv * WAVE < n ? mask[c0 / WAVE + wv] : 0ull;
+        if (lane == 0) wpre[wv] = (uint32_t)__popcll(b);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t r = 0;
+            for (int w = 0; w < ORD_BLOCK / WAVE; w++) { const uint32_t t = wpre[w]; wpre[w] = r; r += t; }
+            wpre[ORD_BLOCK / WAVE] = r;
+        }
+        __syncthreads();
+        if (i < n && !((b >> lane) & 1ull)) {
+            const uint32_t id = order[i], k32 = skey[i];
+            const uint64_t c = kid(k32, id);
+            const uint32_t mb = moved + wpre[wv] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+            long before;
+            if (hi - lo <= 8) {
+                before = lo;
+                for (long j = lo; j < hi; j++) before += mov[j] < c ? 1 : 0;
+            } else {
+                before = lower_bound64(mov, lo, hi, c);
+            }
+            const long pos = (i - (long)mb) + before;
+            order_out[pos] = id;
+            skey_out[pos] = k32;
+        }
+        moved += wpre[ORD_BLOCK / WAVE];
+        __syncthreads();
+    }
+}
+
+// Pass 6: movers to their new positions.  Stayers before a mover = entries of
+// the old order below it (binary search; the old order is sorted by the keys
+// it was built with, and stayers kept theirs) minus the movers among them.
+__global__ void k_order_place_movers(const uint64_t *__restrict__ mov, long nmov,
+                                     const uint32_t *__restrict__ order, const uint32_t *__restrict__ skey, long n,
+                                     const unsigned long long *__restrict__ mask,
+                                     const uint32_t *__restrict__ off, uint32_t *__restrict__ order_out,
+                                     uint32_t *__restrict__ skey_out)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nmov) return;
     const uint64_t c = mov[r];
-    long lo = 0, hi = nstay;
+    long lo = 0, hi = n;
     while (lo < hi) {
-        long mid = (lo + hi) >> 1;
-        const uint32_t sid = stay[mid];
-        if (kid(key[sid], sid) < c) lo = mid + 1; else hi = mid;
+        const long mid = (lo + hi) >> 1;
+        if (kid(skey[mid], order[mid]) < c) lo = mid + 1; else hi = mid;
     }
-    order_out[r + lo] = (uint32_t)(c & 0xffffffffu);
-    skey_out[r + lo] = (uint32_t)(c >> 32);
+    const long stay_before = lo - (long)(lo < n ? movers_before(mask, off, lo) : off[(n + ORD_TILE - 1) / ORD_TILE]);
+    const long pos = r + stay_before;
+    order_out[pos] = (uint32_t)(c & 0xffffffffu);
+    skey_out[pos] = (uint32_t)(c >> 32);
 }
+
+// ---------------------------------------------------------------------------
+// Node records of the scatter passes.  Every sub-cell (cell (cx, cy),
+// quadrant q = 2*(cx > .5) + (cy > .5)) feeds exactly one basic, one vx, one
+// vy and one centre node, and every node is fed by four sub-cells, so each
+// sub-cell sum has a fixed slot in its node's record:
+//   basic  node (cx+q/2, cy+q%2), slot 3-q          (slot k: neighbour cells
+//          (m11-1,m22-1), (m11-1,m22), (m11,m22-1), (m11,m22) for k = 0..3)
+//   vx     q 2,3 -> node (cx+1, cy) slots 0,1;  q 0,1 -> node (cx, cy) slots 2,3
+//   vy     q 1,3 -> node (cx, cy+1) slots 0,1;  q 0,2 -> node (cx, cy) slots 2,3
+//   centre node (cx+1, cy+1), slot q
+// Records are node-major (four blocks: basic NB, vx NV, vy NV, centre NC
+// fields per slot), so a node thread reads one contiguous record per block.
+// ---------------------------------------------------------------------------
+template <int NB, int NV, int NC>
+struct NodeRec {
+    static constexpr int NF = NB + 2 * NV + NC;
+    __device__ static long index(uint32_t key, int f, int ynumy, long nodenum)
+    {
+        const uint32_t cell = key >> 2, ncy = (uint32_t)(ynumy - 1);
+        const int q = (int)(key & 3u);
+        const long cx = (long)(cell / ncy), cy = (long)cell - cx * (long)ncy;
+        if (f < NB) {
+            const long node = (cx + (q >> 1)) * ynumy + cy + (q & 1);
+            return node * 4 * NB + (3 - q) * NB + f;
+        }
+        if (f < NB + NV) {
+            const long node = (cx + (q >> 1)) * ynumy + cy;
+            const int slot = (q >> 1) ? (q & 1) : 2 + (q & 1);
+            return nodenum * 4 * NB + node * 4 * NV + slot * NV + (f - NB);
+        }
+        if (f < NB + 2 * NV) {
+            const long node = cx * ynumy + cy + (q & 1);
+            const int slot = (q & 1) ? (q >> 1) : 2 + (q >> 1);
+            return nodenum * 4 * (NB + NV) + node * 4 * NV + slot * NV + (f - NB - NV);
+        }
+        const long node = (cx + 1) * ynumy + cy + 1;
+        return nodenum * 4 * (NB + 2 * NV) + node * 4 * NC + q * NC + (f - NB - 2 * NV);
+    }
+    __device__ static const double *basic(const double *P, long m3) { return P + m3 * 4 * NB; }
+    __device__ static const double *vx(const double *P, long m3, long nodenum)
+    {
+        return P + nodenum * 4 * NB + m3 * 4 * NV;
+    }
+    __device__ static const double *vy(const double *P, long m3, long nodenum)
+    {
+        return P + nodenum * 4 * (NB + NV) + m3 * 4 * NV;
+    }
+    __device__ static const double *centre(const double *P, long m3, long nodenum)
+    {
+        return P + nodenum * 4 * (NB + 2 * NV) + m3 * 4 * NC;
+    }
+};
+
+using MRec = NodeRec<M_NB, M_NV, M_NC>;
+using NRec = NodeRec<N_NB, N_NV, N_NC>;
 
 // ---------------------------------------------------------------------------
 // Segmented sums of one batch of 64 sorted markers.  Lanes [0, nact) hold NF
 // contributions in con[lane][f] and their sub-cell key in skey[lane]; equal
 // keys are contiguous.  Work items (run, field), run-major, are spread over
 // the lanes: each sums its run serially in lane order (the result does not
-// depend on timing), consecutive lanes read consecutive LDS words and write
-// consecutive fields of one sub-cell record.  A sub-cell whose markers
+// depend on timing), and consecutive lanes read consecutive LDS words and
+// write consecutive fields of a node record.  A sub-cell whose markers
 // continue into the next batch (cont) is carried in LDS, double-buffered by
-// batch parity, and finished there; every other run is complete and its
-// record out[key*NF + f] is written once.
+// batch parity, and finished there; every other run is complete and written
+// once.
 // ---------------------------------------------------------------------------
-template <int NF>
-__device__ __forceinline__ void scatter_batch(const double (*con)[NF], const uint32_t *skey, int *runs,
-                                              double (*carry)[NF], int par, bool have_carry, bool cont,
-                                              int lane, int nact, double *__restrict__ out)
+template <class R>
+__device__ __forceinline__ void scatter_batch(const double (*con)[R::NF], const uint32_t *skey, int *runs,
+                                              double (*carry)[R::NF], int par, bool have_carry, bool cont,
+                                              int lane, int nact, const KArgs &a)
 {
+    constexpr int NF = R::NF;
     const bool head = lane < nact && (lane == 0 || skey[lane - 1] != skey[lane]);
     const unsigned long long hm = __ballot(head);
     const int nr = __popcll(hm);
@@ -167,18 +365,18 @@ __device__ __forceinline__ void scatter_batch(const double (*con)[NF], const uin
         double sum = (r == 0 && have_carry) ? carry[par][f] : 0.0;
         for (int j = b; j < e; j++) sum += con[j][f];
         if (r == nr - 1 && cont) carry[par ^ 1][f] = sum;
-        else out[(long)skey[b] * NF + f] = sum;
+        else a.part[R::index(skey[b], f, a.g.ynumy, a.nodenum)] = sum;
     }
 }
 
-// Zero the records of the group's empty sub-cells (the node passes read every
-// sub-cell of every cell).
-template <int NF>
-__device__ __forceinline__ void zero_empty(const KArgs &a, long c0, long c1, int lane, double *__restrict__ out)
+// Zero the slots fed by the group's empty sub-cells (node passes read every
+// slot whose source cell exists).
+template <class R>
+__device__ __forceinline__ void zero_empty(const KArgs &a, long c0, long c1, int lane)
 {
     const long k = c0 * 4 + lane;
     if (k < c1 * 4 && a.sub_start[k] == a.sub_start[k + 1])
-        for (int f = 0; f < NF; f++) out[k * NF + f] = 0.0;
+        for (int f = 0; f < R::NF; f++) a.part[R::index((uint32_t)k, f, a.g.ynumy, a.nodenum)] = 0.0;
 }
 
 // Cells of a marker from its sub-cell key (the key is computed by the same
@@ -315,11 +513,11 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
         skey[lane] = key;
         const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
         __syncthreads();
-        scatter_batch<M_NF>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a.part);
+        scatter_batch<MRec>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a);
         have_carry = cont;
         __syncthreads();
     }
-    zero_empty<M_NF>(a, c0, c1, lane, a.part);
+    zero_empty<MRec>(a, c0, c1, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -333,49 +531,41 @@ __global__ void k_m2n_nodes(KArgs a)
     if (m3 >= a.nodenum) return;
     const int ynumy = a.g.ynumy, xnumx = a.g.xnumx;
     const int m22 = (int)(m3 % ynumy), m11 = (int)(m3 / ynumy);
-    const long ncy = ynumy - 1, ncell = a.ncell;
     const double *P = a.part;
-    auto cell = [&](int cx, int cy) -> long {
-        return (cx >= 0 && cy >= 0 && cx < xnumx - 1 && cy < ynumy - 1) ? (long)cx * ncy + cy : -1;
-    };
-    // Sums of sub-cell (cell, quadrant q = 2*(cx > .5) + (cy > .5)): one
-    // record of M_NF doubles per sub-cell key cell*4 + q.
-    auto S = [&](int f, int q, long c) -> double { return P[(c * 4 + q) * M_NF + f]; };
-    // basic node: quadrant 3 of cell (m11-1,m22-1), 2 of (m11-1,m22), 1 of
-    // (m11,m22-1), 0 of (m11,m22)
+    // The node's record slots, summed in slot order; a slot counts only if
+    // its source cell exists.
+    const bool c00 = m11 >= 1 && m22 >= 1, c01 = m11 >= 1 && m22 < ynumy - 1;
+    const bool c10 = m11 < xnumx - 1 && m22 >= 1, c11 = m11 < xnumx - 1 && m22 < ynumy - 1;
     double B[M_NB];
 #pragma unroll
     for (int f = 0; f < M_NB; f++) B[f] = 0.0;
     {
-        const long cs[4] = {cell(m11 - 1, m22 - 1), cell(m11 - 1, m22), cell(m11, m22 - 1), cell(m11, m22)};
-        const int qs[4] = {3, 2, 1, 0};
+        const double *R = MRec::basic(P, m3);
+        const bool has[4] = {c00, c01, c10, c11};
         for (int k = 0; k < 4; k++) {
-            if (cs[k] < 0) continue;
+            if (!has[k]) continue;
 #pragma unroll
-            for (int f = 0; f < M_NB; f++) B[f] += S(f, qs[k], cs[k]);
+            for (int f = 0; f < M_NB; f++) B[f] += R[k * M_NB + f];
         }
     }
-    // vx node: right half (q 2, 3) of cell (m11-1,m22), left half (q 0, 1) of
-    // (m11,m22); vy node: upper half (q 1, 3) of (m11,m22-1), lower half
-    // (q 0, 2) of (m11,m22); centre node: all of (m11-1,m22-1)
     double VXs[M_NV] = {0, 0, 0}, VYs[M_NV] = {0, 0, 0}, Cs[M_NC];
 #pragma unroll
     for (int f = 0; f < M_NC; f++) Cs[f] = 0.0;
     {
-        const long cl = cell(m11 - 1, m22), cr = cell(m11, m22);
+        const double *R = MRec::vx(P, m3, a.nodenum);
         for (int f = 0; f < M_NV; f++) {
-            if (cl >= 0) VXs[f] += S(M_VX + f, 2, cl) + S(M_VX + f, 3, cl);
-            if (cr >= 0) VXs[f] += S(M_VX + f, 0, cr) + S(M_VX + f, 1, cr);
+            if (c01) VXs[f] += R[0 * M_NV + f] + R[1 * M_NV + f];
+            if (c11) VXs[f] += R[2 * M_NV + f] + R[3 * M_NV + f];
         }
-        const long ct = cell(m11, m22 - 1), cb = cell(m11, m22);
+        R = MRec::vy(P, m3, a.nodenum);
         for (int f = 0; f < M_NV; f++) {
-            if (ct >= 0) VYs[f] += S(M_VY + f, 1, ct) + S(M_VY + f, 3, ct);
-            if (cb >= 0) VYs[f] += S(M_VY + f, 0, cb) + S(M_VY + f, 2, cb);
+            if (c10) VYs[f] += R[0 * M_NV + f] + R[1 * M_NV + f];
+            if (c11) VYs[f] += R[2 * M_NV + f] + R[3 * M_NV + f];
         }
-        const long cc = cell(m11 - 1, m22 - 1);
-        if (cc >= 0)
+        R = MRec::centre(P, m3, a.nodenum);
+        if (c00)
             for (int f = 0; f < M_NC; f++)
-                Cs[f] = S(M_C + f, 0, cc) + S(M_C + f, 1, cc) + S(M_C + f, 2, cc) + S(M_C + f, 3, cc);
+                Cs[f] = R[0 * M_NC + f] + R[1 * M_NC + f] + R[2 * M_NC + f] + R[3 * M_NC + f];
     }
 
     // zeroed by the reference's initialisation loop (interpolation.c:58-153)
@@ -715,11 +905,11 @@ __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
         skey[lane] = key;
         const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
         __syncthreads();
-        scatter_batch<N_NF>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a.part);
+        scatter_batch<NRec>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a);
         have_carry = cont;
         __syncthreads();
     }
-    zero_empty<N_NF>(a, c0, c1, lane, a.part);
+    zero_empty<NRec>(a, c0, c1, lane);
 }
 
 // nodes2markers marker update (interpolation.c:943-1011), fused with
@@ -802,37 +992,34 @@ __global__ void k_n2m_nodes(KArgs a)
     if (m3 >= a.nodenum) return;
     const int ynumy = a.g.ynumy, xnumx = a.g.xnumx;
     const int m22 = (int)(m3 % ynumy), m11 = (int)(m3 / ynumy);
-    const long ncy = ynumy - 1, ncell = a.ncell;
     const double *P = a.part;
-    auto cell = [&](int cx, int cy) -> long {
-        return (cx >= 0 && cy >= 0 && cx < xnumx - 1 && cy < ynumy - 1) ? (long)cx * ncy + cy : -1;
-    };
-    auto S = [&](int f, int q, long c) -> double { return P[(c * 4 + q) * N_NF + f]; };
+    const bool c00 = m11 >= 1 && m22 >= 1, c01 = m11 >= 1 && m22 < ynumy - 1;
+    const bool c10 = m11 < xnumx - 1 && m22 >= 1, c11 = m11 < xnumx - 1 && m22 < ynumy - 1;
     double B[N_NB] = {0, 0, 0, 0, 0};
     {
-        const long cs[4] = {cell(m11 - 1, m22 - 1), cell(m11 - 1, m22), cell(m11, m22 - 1), cell(m11, m22)};
-        const int qs[4] = {3, 2, 1, 0};
+        const double *R = NRec::basic(P, m3);
+        const bool has[4] = {c00, c01, c10, c11};
         for (int k = 0; k < 4; k++) {
-            if (cs[k] < 0) continue;
-            for (int f = 0; f < N_NB; f++) B[f] += S(f, qs[k], cs[k]);
+            if (!has[k]) continue;
+            for (int f = 0; f < N_NB; f++) B[f] += R[k * N_NB + f];
         }
     }
     double VXs[N_NV] = {0, 0, 0}, VYs[N_NV] = {0, 0, 0}, Cs[N_NC] = {0, 0, 0, 0};
     {
-        const long cl = cell(m11 - 1, m22), cr = cell(m11, m22);
+        const double *R = NRec::vx(P, m3, a.nodenum);
         for (int f = 0; f < N_NV; f++) {
-            if (cl >= 0) VXs[f] += S(N_VX + f, 2, cl) + S(N_VX + f, 3, cl);
-            if (cr >= 0) VXs[f] += S(N_VX + f, 0, cr) + S(N_VX + f, 1, cr);
+            if (c01) VXs[f] += R[0 * N_NV + f] + R[1 * N_NV + f];
+            if (c11) VXs[f] += R[2 * N_NV + f] + R[3 * N_NV + f];
         }
-        const long ct = cell(m11, m22 - 1), cb = cell(m11, m22);
+        R = NRec::vy(P, m3, a.nodenum);
         for (int f = 0; f < N_NV; f++) {
-            if (ct >= 0) VYs[f] += S(N_VY + f, 1, ct) + S(N_VY + f, 3, ct);
-            if (cb >= 0) VYs[f] += S(N_VY + f, 0, cb) + S(N_VY + f, 2, cb);
+            if (c10) VYs[f] += R[0 * N_NV + f] + R[1 * N_NV + f];
+            if (c11) VYs[f] += R[2 * N_NV + f] + R[3 * N_NV + f];
         }
-        const long cc = cell(m11 - 1, m22 - 1);
-        if (cc >= 0)
+        R = NRec::centre(P, m3, a.nodenum);
+        if (c00)
             for (int f = 0; f < N_NC; f++)
-                Cs[f] = S(N_C + f, 0, cc) + S(N_C + f, 1, cc) + S(N_C + f, 2, cc) + S(N_C + f, 3, cc);
+                Cs[f] = R[0 * N_NC + f] + R[1 * N_NC + f] + R[2 * N_NC + f] + R[3 * N_NC + f];
     }
     double *phisum = NOD(a, V2C_N_phisum), *sxysum = NOD(a, V2C_N_sxysum), *exysum = NOD(a, V2C_N_exysum);
     double *esum = NOD(a, V2C_N_esum), *wtphisum = NOD(a, V2C_N_wtphisum);
