@@ -347,11 +347,6 @@ class Context:
         return ms.value, n.value
 
 
-def _opt(a):
-    return None if a is None else _dptr(a)
-
-
-def _ctx_extras():
     def order_get(self):
         """(order, skey) of the current cell ordering, brought up to date."""
         n = self.lib.v2c_marker_count(self.h)
@@ -394,8 +389,7 @@ def _ctx_extras():
     def set_friction_fields(self, on):
         self._check(self.lib.v2c_set_friction_fields(self.h, int(on)), "set_friction_fields")
 
-    for f in (order_stats, rsf_upload, local_update, local_download, local_set_guess, set_friction_fields):
-        setattr(Context, f.__name__, f)
+def _opt(a):
+    return None if a is None else _dptr(a)
 
 
-_ctx_extras()

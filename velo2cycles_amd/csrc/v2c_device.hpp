@@ -105,23 +105,39 @@ __device__ __forceinline__ Axis4 find_y(const DevGrid &g, double y)
 
 struct Stencil { long m3; double e, n; };
 
-// Stencils at (x, y) in cell (X.s, Y.s); every gridline they use lies in
-// X.g / Y.g (see Axis4).
+// Stencils at (x, y) in cell (X.s, Y.s).  Every gridline they use lies in
+// X.g / Y.g: after the reference's clamps the stencil's first gridline is
+// g[d] with d in {0,1} (three-point stencils) or {0,1,2} (two-point), so a
+// couple of selects replace gridline loads.
+__device__ __forceinline__ void sel2(const Axis4 &A, int d, double &a, double &b)
+{
+    a = d == 0 ? A.g[0] : (d == 1 ? A.g[1] : A.g[2]);
+    b = d == 0 ? A.g[1] : (d == 1 ? A.g[2] : A.g[3]);
+}
+__device__ __forceinline__ void sel3(const Axis4 &A, int d, double &a, double &b, double &c)
+{
+    a = d ? A.g[1] : A.g[0];
+    b = d ? A.g[2] : A.g[1];
+    c = d ? A.g[3] : A.g[2];
+}
 
 // Pressure-node stencil (interpolation.c:1942-1948, 1216-1224, 1723-1730).
 __device__ __forceinline__ Stencil centre_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
                                                   double x, double y)
 {
     int m10 = X.s, m20 = Y.s;
-    if (x > (pick(X, m10) + pick(X, m10 + 1)) / 2.0) m10++;
-    if (y > (pick(Y, m20) + pick(Y, m20 + 1)) / 2.0) m20++;
+    if (x > (X.g[1] + X.g[2]) / 2.0) m10++;
+    if (y > (Y.g[1] + Y.g[2]) / 2.0) m20++;
     if (m10 < 1) m10 = 1;
     if (m10 > g.xnumx - 2) m10 = g.xnumx - 2;
     if (m20 < 1) m20 = 1;
     if (m20 > g.ynumy - 2) m20 = g.ynumy - 2;
+    double a0, a1, a2, b0, b1, b2;            // grid[m-1], grid[m], grid[m+1]
+    sel3(X, m10 - X.s, a0, a1, a2);
+    sel3(Y, m20 - Y.s, b0, b1, b2);
     Stencil s;
-    s.e = (x - (pick(X, m10 - 1) + pick(X, m10)) / 2.0) / ((pick(X, m10 + 1) - pick(X, m10 - 1)) / 2.0);
-    s.n = (y - (pick(Y, m20 - 1) + pick(Y, m20)) / 2.0) / ((pick(Y, m20 + 1) - pick(Y, m20 - 1)) / 2.0);
+    s.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
+    s.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
     s.m3 = (long)m10 * g.ynumy + m20;
     return s;
 }
@@ -135,48 +151,49 @@ __device__ __forceinline__ Stencil basic_stencil(const DevGrid &g, const Axis4 &
     if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
     if (m20 < 1) m20 = 1;
     if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
+    double a0, a1, b0, b1;                    // grid[m], grid[m+1]
+    sel2(X, m10 - X.s + 1, a0, a1);
+    sel2(Y, m20 - Y.s + 1, b0, b1);
     Stencil s;
-    s.e = (x - pick(X, m10)) / (pick(X, m10 + 1) - pick(X, m10));
-    s.n = (y - pick(Y, m20)) / (pick(Y, m20 + 1) - pick(Y, m20));
+    s.e = (x - a0) / (a1 - a0);
+    s.n = (y - b0) / (b1 - b0);
     s.m3 = (long)m10 * g.ynumy + m20;
     return s;
 }
 
-// vx-node stencil (interpolation.c:1377-1388, 1738-1744).
+// vx-node stencil (interpolation.c:1377-1388, 1738-1744).  The x clamp of
+// allinteri is a no-op: the searched cell is already in [0, xnumx-2].
 __device__ __forceinline__ Stencil vx_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
-                                              double x, double y, bool clampx)
+                                              double x, double y)
 {
-    int m10 = X.s, m20 = Y.s;
-    if (y < (pick(Y, m20) + pick(Y, m20 + 1)) / 2.0) m20 -= 1;
-    if (clampx) {
-        if (m10 < 0) m10 = 0;
-        if (m10 > g.xnumx - 2) m10 = g.xnumx - 2;
-    }
+    int m20 = Y.s;
+    if (y < (Y.g[1] + Y.g[2]) / 2.0) m20 -= 1;
     if (m20 < 0) m20 = 0;
     if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
+    double b0, b1, b2;                        // grid[m], grid[m+1], grid[m+2]
+    sel3(Y, m20 - Y.s + 1, b0, b1, b2);
     Stencil s;
-    s.e = (x - pick(X, m10)) / (pick(X, m10 + 1) - pick(X, m10));
-    s.n = (y - (pick(Y, m20) + pick(Y, m20 + 1)) / 2.0) / ((pick(Y, m20 + 2) - pick(Y, m20)) / 2.0);
-    s.m3 = (long)m10 * g.ynumy + m20;
+    s.e = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    s.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
+    s.m3 = (long)X.s * g.ynumy + m20;
     return s;
 }
 
-// vy-node stencil (interpolation.c:1434-1445, 1749-1755).
+// vy-node stencil (interpolation.c:1434-1445, 1749-1755); the y clamp of
+// allinteri is likewise a no-op.
 __device__ __forceinline__ Stencil vy_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
-                                              double x, double y, bool clampy)
+                                              double x, double y)
 {
-    int m10 = X.s, m20 = Y.s;
-    if (x < (pick(X, m10) + pick(X, m10 + 1)) / 2.0) m10 -= 1;
+    int m10 = X.s;
+    if (x < (X.g[1] + X.g[2]) / 2.0) m10 -= 1;
     if (m10 < 0) m10 = 0;
     if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
-    if (clampy) {
-        if (m20 < 0) m20 = 0;
-        if (m20 > g.ynumy - 2) m20 = g.ynumy - 2;
-    }
+    double a0, a1, a2;
+    sel3(X, m10 - X.s + 1, a0, a1, a2);
     Stencil s;
-    s.e = (x - (pick(X, m10) + pick(X, m10 + 1)) / 2.0) / ((pick(X, m10 + 2) - pick(X, m10)) / 2.0);
-    s.n = (y - pick(Y, m20)) / (pick(Y, m20 + 1) - pick(Y, m20));
-    s.m3 = (long)m10 * g.ynumy + m20;
+    s.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
+    s.n = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    s.m3 = (long)m10 * g.ynumy + Y.s;
     return s;
 }
 
@@ -436,9 +453,9 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
                   + w.d2 * (vy[m3] + vy[m3 - 1]) + w.d3 * (vy[m3 + 1] + vy[m3])) * 0.5 * (1.0 - xrat);
     EE = sqrt(EXX * EXX + EXY * EXY);
 
-    s = vx_stencil(g, X, Y, x, y, true);
+    s = vx_stencil(g, X, Y, x, y);
     VX = vxa + bil(vx, s.m3, ynumy, wts(s)) * xrat;
-    s = vy_stencil(g, X, Y, x, y, true);
+    s = vy_stencil(g, X, Y, x, y);
     VY = vya + bil(vy, s.m3, ynumy, wts(s)) * xrat;
 }
 

@@ -713,11 +713,11 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
     o.SXXE = bil(NOD(a, V2C_N_sxxe), s.m3, ynumy, w);
     o.PR = bil(NOD(a, V2C_N_pr), s.m3, ynumy, w);
     o.SPPE = bil(NOD(a, V2C_N_sppe), s.m3, ynumy, w);
-    s = vx_stencil(g, X, Y, x, y, false);
+    s = vx_stencil(g, X, Y, x, y);
     w = wts(s);
     o.VX = bil(NOD(a, V2C_N_vx), s.m3, ynumy, w);
     o.MVX = bil(NOD(a, V2C_N_mvx), s.m3, ynumy, w);
-    s = vy_stencil(g, X, Y, x, y, false);
+    s = vy_stencil(g, X, Y, x, y);
     w = wts(s);
     o.VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, w);
     o.MVY = bil(NOD(a, V2C_N_mvy), s.m3, ynumy, w);
@@ -743,9 +743,9 @@ __device__ __forceinline__ void anti_contrib(const KArgs &a, double x, double y,
     const double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
     const double SXX = bil(NOD(a, V2C_N_sxx), s.m3, ynumy, w);
     const double PR = bil(NOD(a, V2C_N_pr), s.m3, ynumy, w);
-    s = vx_stencil(g, X, Y, x, y, false);
+    s = vx_stencil(g, X, Y, x, y);
     const double VX = bil(NOD(a, V2C_N_vx), s.m3, ynumy, wts(s));
-    s = vy_stencil(g, X, Y, x, y, false);
+    s = vy_stencil(g, X, Y, x, y);
     const double VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, wts(s));
     const double swt1 = 1.0 / (X.g[2] - X.g[1]) / (Y.g[2] - Y.g[1]);
     const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
