@@ -78,6 +78,41 @@ def cpu_baseline(cfg, sample: int, steps: int = 2):
                       f"{el:.1f} s"}
 
 
+def warm_start_eval(lib, dt: float):
+    """Newton iterations per marker and kernel time of the per-marker local
+    update, plain vs learned warm start, on the 401x101 / 16-per-cell cloud
+    (640,000 markers, 2 % fault markers) at residual 1e-10.  The MLP is trained
+    here, on the GPU, from 2 M samples of plain solves on four other seeds."""
+    from velo2cycles_amd.warmstart import WarmStart, evaluate, generate, init_params
+    cfg = synth.config_small()
+    cfg.timestep = dt
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=lib)
+    t0 = time.perf_counter()
+    X, M = generate(ctx, [11, 12, 13, 14], dt, max_rows=2_000_000)
+    ws = WarmStart(ctx)
+    ws.fit_norm(X, M)
+    ws.load(init_params(0))
+    loss = ws.train(X, M, epochs=8, batch=4096, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    res = evaluate(ctx, cfg, ws, dt, tol=1e-10, repeats=3)
+    ctx.close()
+    out = {"workload": "401x101 uniform, 16 markers/cell, 2 % fault markers", "dt_s": dt, "residual": 1e-10,
+           "training_rows": int(len(X)), "train_s": round(t_train, 2), "final_loss": float(loss[-1])}
+    n = cfg.n
+    for mode in ("plain", "warm"):
+        st = res[mode]["stats"]
+        out[mode] = {
+            "iters_per_marker": (st["ductile_iters"] + st["rsf_iters"]) / n,
+            "ductile_iters_per_marker": st["ductile_iters"] / max(st["ductile_markers"], 1),
+            "rsf_iters_per_fault_marker": st["rsf_iters"] / max(st["fault_markers"], 1),
+            "fallbacks": st["fallbacks"], "nonfinite": st["nonfinite"],
+            "ms_newton": res[mode]["ms_local"], "ms_mlp": res[mode]["ms_mlp"],
+            "ms_total": res[mode]["ms_total"]}
+    out["iters_ratio"] = out["plain"]["iters_per_marker"] / max(out["warm"]["iters_per_marker"], 1e-30)
+    out["wall_ratio"] = out["plain"]["ms_total"] / max(out["warm"]["ms_total"], 1e-30)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +121,7 @@ def main():
     ap.add_argument("--config", default="production", choices=["production", "small"])
     ap.add_argument("--cpu-sample", type=int, default=3_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -164,6 +200,8 @@ def main():
         "kernels_ms_per_step": {k: v["ms_per_step"] for k, v in kern.items()},
     }
     ctx.close()
+    if not args.no_warm:
+        out["warm_start"] = warm_start_eval(lib, synth.YEAR)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, min(args.cpu_sample, cfg.n))
     print(json.dumps(out), flush=True)

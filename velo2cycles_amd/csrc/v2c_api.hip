@@ -315,7 +315,9 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
 {
     if (!out || !g || !prm || !mat) return V2C_EINVAL;
     *out = nullptr;
-    if (g->xnumx < 4 || g->ynumy < 4 || !g->gx || !g->gy || capacity < 0 || capacity > 0x7fffffffL)
+    // per-field arrays are addressed with 32-bit byte offsets on the device
+    if (g->xnumx < 4 || g->ynumy < 4 || !g->gx || !g->gy || capacity < 0 || capacity >= (1L << 29) ||
+        g->xnumx * g->ynumy >= (1L << 29))
         return V2C_EINVAL;
     if (prm->interpolationmod != 2) return V2C_EUNSUPPORTED;
     if (mat->ntypes < 0 || mat->ntypes > V2C_MAX_TYPES) return V2C_EINVAL;

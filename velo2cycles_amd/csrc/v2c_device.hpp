@@ -14,6 +14,16 @@
 
 namespace v2c {
 
+// Element i of a wave-uniform array through a 32-bit byte offset: the load
+// takes the scalar base and one vector offset register instead of a 64-bit
+// vector address per field.  Arrays indexed this way hold < 2^29 elements
+// per field (checked when a context is created).
+template <class T>
+__device__ __forceinline__ T &at32(T *base, uint32_t i)
+{
+    return *(T *)((char *)base + (size_t)(i * (uint32_t)sizeof(T)));
+}
+
 // Marker SoA field slots (same order as the v2c_markers struct).
 enum MF {
     MF_X = 0, MF_Y, MF_K, MF_P, MF_XX, MF_XY, MF_EXX, MF_EXY, MF_E, MF_PHI, MF_VX, MF_VY,
@@ -81,8 +91,14 @@ struct KArgs {
 };
 
 __device__ __forceinline__ double *NOD(const KArgs &a, int f) { return a.nod + (long)f * a.nodenum; }
-__device__ __forceinline__ double &MK(const KArgs &a, int f, long i) { return a.mk[(long)f * a.cap + i]; }
-__device__ __forceinline__ double &MXF(const KArgs &a, int f, long i) { return a.mx[(long)f * a.cap + i]; }
+__device__ __forceinline__ double &MK(const KArgs &a, int f, long i)
+{
+    return at32(a.mk + (long)f * a.cap, (uint32_t)i);
+}
+__device__ __forceinline__ double &MXF(const KArgs &a, int f, long i)
+{
+    return at32(a.mx + (long)f * a.cap, (uint32_t)i);
+}
 
 // ---- small helpers ---------------------------------------------------------
 __device__ __forceinline__ double ref_abs(double a) { return a >= 0 ? a : -a; }
@@ -103,7 +119,10 @@ __device__ __forceinline__ Axis4 find_y(const DevGrid &g, double y)
     return axis_find(g.gy, g.ynumy, g.by, g.celly, g.nby, g.by0, g.byinv, y);
 }
 
-struct Stencil { long m3; double e, n; };
+// m3: nodal index; uint32 so that gathers address a wave-uniform field base
+// plus a 32-bit byte offset (nodenum < 2^29, checked at create).
+struct Stencil { uint32_t m3; double e, n; };
+
 
 // Stencils at (x, y) in cell (X.s, Y.s).  Every gridline they use lies in
 // X.g / Y.g: after the reference's clamps the stencil's first gridline is
@@ -138,63 +157,80 @@ __device__ __forceinline__ Stencil centre_stencil(const DevGrid &g, const Axis4 
     Stencil s;
     s.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
     s.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
-    s.m3 = (long)m10 * g.ynumy + m20;
+    s.m3 = (uint32_t)(m10 * g.ynumy + m20);
     return s;
 }
 
-// Basic-node stencil without ghost nodes (interpolation.c:1069-1083, 1711-1717).
-__device__ __forceinline__ Stencil basic_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
-                                                 double x, double y)
+// All four stencils of one point (basic interpolation.c:1069-1083, 1711-1717;
+// centre :1216-1224, 1723-1730; vx :1377-1388, 1738-1744; vy :1434-1445,
+// 1749-1755), each weight with the reference's formula and operands.  The x
+// clamp of the vx stencil and the y clamp of the vy stencil are no-ops: the
+// searched cell is already in range.
+struct Stencil4 { Stencil b, c, vx, vy; };
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ Stencil4 stencils4_full(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
+                                                   double x, double y)
 {
-    int m10 = X.s, m20 = Y.s;
-    if (m10 < 1) m10 = 1;
-    if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
-    if (m20 < 1) m20 = 1;
-    if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
-    double a0, a1, b0, b1;                    // grid[m], grid[m+1]
-    sel2(X, m10 - X.s + 1, a0, a1);
-    sel2(Y, m20 - Y.s + 1, b0, b1);
-    Stencil s;
-    s.e = (x - a0) / (a1 - a0);
-    s.n = (y - b0) / (b1 - b0);
-    s.m3 = (long)m10 * g.ynumy + m20;
-    return s;
+    const int nx = g.xnumx, ny = g.ynumy;
+    const double xmid = (X.g[1] + X.g[2]) / 2.0, ymid = (Y.g[1] + Y.g[2]) / 2.0;
+    const int bx = clampi(X.s, 1, nx - 3), by = clampi(Y.s, 1, ny - 3);
+    const int cx = clampi(X.s + (x > xmid ? 1 : 0), 1, nx - 2), cy = clampi(Y.s + (y > ymid ? 1 : 0), 1, ny - 2);
+    const int vyx = clampi(X.s - (x < xmid ? 1 : 0), 0, nx - 3), vxy = clampi(Y.s - (y < ymid ? 1 : 0), 0, ny - 3);
+    double a0, a1, a2, b0, b1, b2;
+    Stencil4 o;
+    sel2(X, bx - X.s + 1, a0, a1);
+    sel2(Y, by - Y.s + 1, b0, b1);
+    o.b.e = (x - a0) / (a1 - a0);
+    o.b.n = (y - b0) / (b1 - b0);
+    o.b.m3 = (uint32_t)(bx * ny + by);
+    sel3(X, cx - X.s, a0, a1, a2);
+    sel3(Y, cy - Y.s, b0, b1, b2);
+    o.c.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
+    o.c.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
+    o.c.m3 = (uint32_t)(cx * ny + cy);
+    sel3(Y, vxy - Y.s + 1, b0, b1, b2);
+    o.vx.e = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    o.vx.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
+    o.vx.m3 = (uint32_t)(X.s * ny + vxy);
+    sel3(X, vyx - X.s + 1, a0, a1, a2);
+    o.vy.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
+    o.vy.n = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    o.vy.m3 = (uint32_t)(vyx * ny + Y.s);
+    return o;
 }
 
-// vx-node stencil (interpolation.c:1377-1388, 1738-1744).  The x clamp of
-// allinteri is a no-op: the searched cell is already in [0, xnumx-2].
-__device__ __forceinline__ Stencil vx_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
-                                              double x, double y)
+// Interior points off the cell midlines (all but a thin set of markers)
+// share weights pairwise: the basic stencil's x weight is the vx one and its
+// y weight the vy one (clamps inactive), and the centre and vy x-weights
+// coincide (both stencils start at grid[s-1] below the midline and at
+// grid[s] above it), as do the centre and vx y-weights.  Such points need 4
+// divisions instead of 8 with the same operands, so the same results; the
+// other lanes take the full computation, in a branch the wavefront skips when
+// none needs it.
+__device__ __forceinline__ Stencil4 stencils4(const DevGrid &g, const Axis4 &X, const Axis4 &Y, double x, double y)
 {
-    int m20 = Y.s;
-    if (y < (Y.g[1] + Y.g[2]) / 2.0) m20 -= 1;
-    if (m20 < 0) m20 = 0;
-    if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
-    double b0, b1, b2;                        // grid[m], grid[m+1], grid[m+2]
-    sel3(Y, m20 - Y.s + 1, b0, b1, b2);
-    Stencil s;
-    s.e = (x - X.g[1]) / (X.g[2] - X.g[1]);
-    s.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
-    s.m3 = (long)X.s * g.ynumy + m20;
-    return s;
-}
-
-// vy-node stencil (interpolation.c:1434-1445, 1749-1755); the y clamp of
-// allinteri is likewise a no-op.
-__device__ __forceinline__ Stencil vy_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
-                                              double x, double y)
-{
-    int m10 = X.s;
-    if (x < (X.g[1] + X.g[2]) / 2.0) m10 -= 1;
-    if (m10 < 0) m10 = 0;
-    if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
-    double a0, a1, a2;
-    sel3(X, m10 - X.s + 1, a0, a1, a2);
-    Stencil s;
-    s.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
-    s.n = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
-    s.m3 = (long)m10 * g.ynumy + Y.s;
-    return s;
+    const int ny = g.ynumy;
+    const double xmid = (X.g[1] + X.g[2]) / 2.0, ymid = (Y.g[1] + Y.g[2]) / 2.0;
+    const bool xup = x > xmid, yup = y > ymid;
+    const bool rare = X.s == 0 || X.s == g.xnumx - 2 || Y.s == 0 || Y.s == ny - 2 || x == xmid || y == ymid ||
+                      x != x || y != y;
+    Stencil4 o;
+    const double ex = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    const double ny_ = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    double a0, a1, a2, b0, b1, b2;
+    sel3(X, xup ? 1 : 0, a0, a1, a2);
+    sel3(Y, yup ? 1 : 0, b0, b1, b2);
+    const double exc = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
+    const double nyc = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
+    const int cx = X.s + (xup ? 1 : 0), cy = Y.s + (yup ? 1 : 0);
+    o.b.e = ex; o.b.n = ny_; o.b.m3 = (uint32_t)(X.s * ny + Y.s);
+    o.c.e = exc; o.c.n = nyc; o.c.m3 = (uint32_t)(cx * ny + cy);
+    o.vx.e = ex; o.vx.n = nyc; o.vx.m3 = (uint32_t)(X.s * ny + (cy - 1));
+    o.vy.e = exc; o.vy.n = ny_; o.vy.m3 = (uint32_t)((cx - 1) * ny + Y.s);
+    if (__ballot(rare) && rare) o = stencils4_full(g, X, Y, x, y);
+    return o;
 }
 
 // Bilinear 4-point sum, reference term order.
@@ -208,9 +244,10 @@ __device__ __forceinline__ W4 wts(const Stencil &s)
     w.d3 = s.e * s.n;
     return w;
 }
-__device__ __forceinline__ double bil(const double *__restrict__ f, long m3, int ny, const W4 &w)
+__device__ __forceinline__ double bil(const double *__restrict__ f, uint32_t m3, int ny, const W4 &w)
 {
-    return w.d0 * f[m3] + w.d1 * f[m3 + 1] + w.d2 * f[m3 + ny] + w.d3 * f[m3 + ny + 1];
+    const uint32_t u = (uint32_t)ny;
+    return w.d0 * at32(f, m3) + w.d1 * at32(f, m3 + 1) + w.d2 * at32(f, m3 + u) + w.d3 * at32(f, m3 + u + 1);
 }
 
 // Sub-cell sort key of a position (after clamping to the domain): cell in
@@ -438,25 +475,26 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
     const Axis4 X = hx ? *hx : find_x(g, x), Y = hy ? *hy : find_y(g, y);
     const double *vx = NOD(a, V2C_N_vx), *vy = NOD(a, V2C_N_vy);
 
-    Stencil s = basic_stencil(g, X, Y, x, y);
-    W4 w = wts(s);
-    double EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
-    ESP = bil(NOD(a, V2C_N_esp), s.m3, ynumy, w);
+    const Stencil4 S = stencils4_full(g, X, Y, x, y);
+    W4 w = wts(S.b);
+    double EXY = bil(NOD(a, V2C_N_exy), S.b.m3, ynumy, w);
+    ESP = bil(NOD(a, V2C_N_esp), S.b.m3, ynumy, w);
 
-    s = centre_stencil(g, X, Y, x, y);
-    w = wts(s);
-    double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
-    long m3 = s.m3;
-    double vxa = (w.d0 * (vx[m3 - 1] + vx[m3 - ynumy - 1]) + w.d1 * (vx[m3] + vx[m3 - ynumy])
-                  + w.d2 * (vx[m3 + ynumy - 1] + vx[m3 - 1]) + w.d3 * (vx[m3 + ynumy] + vx[m3])) * 0.5 * (1.0 - xrat);
-    double vya = (w.d0 * (vy[m3 - ynumy] + vy[m3 - ynumy - 1]) + w.d1 * (vy[m3 - ynumy + 1] + vy[m3 - ynumy])
-                  + w.d2 * (vy[m3] + vy[m3 - 1]) + w.d3 * (vy[m3 + 1] + vy[m3])) * 0.5 * (1.0 - xrat);
+    w = wts(S.c);
+    double EXX = bil(NOD(a, V2C_N_exx), S.c.m3, ynumy, w);
+    // pressure-node averages of vx and vy (interpolation.c:1725-1736); the
+    // centre stencil's m3 >= ynumy + 1, so every index below is >= 0
+    const uint32_t m3 = S.c.m3, u = (uint32_t)ynumy;
+    double vxa = (w.d0 * (at32(vx, m3 - 1) + at32(vx, m3 - u - 1)) + w.d1 * (at32(vx, m3) + at32(vx, m3 - u))
+                  + w.d2 * (at32(vx, m3 + u - 1) + at32(vx, m3 - 1)) + w.d3 * (at32(vx, m3 + u) + at32(vx, m3)))
+                 * 0.5 * (1.0 - xrat);
+    double vya = (w.d0 * (at32(vy, m3 - u) + at32(vy, m3 - u - 1)) + w.d1 * (at32(vy, m3 - u + 1) + at32(vy, m3 - u))
+                  + w.d2 * (at32(vy, m3) + at32(vy, m3 - 1)) + w.d3 * (at32(vy, m3 + 1) + at32(vy, m3)))
+                 * 0.5 * (1.0 - xrat);
     EE = sqrt(EXX * EXX + EXY * EXY);
 
-    s = vx_stencil(g, X, Y, x, y);
-    VX = vxa + bil(vx, s.m3, ynumy, wts(s)) * xrat;
-    s = vy_stencil(g, X, Y, x, y);
-    VY = vya + bil(vy, s.m3, ynumy, wts(s)) * xrat;
+    VX = vxa + bil(vx, S.vx.m3, ynumy, wts(S.vx)) * xrat;
+    VY = vya + bil(vy, S.vy.m3, ynumy, wts(S.vy)) * xrat;
 }
 
 // ---- build-defined per-marker local update (not reference behaviour) -------

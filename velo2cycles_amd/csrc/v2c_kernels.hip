@@ -471,7 +471,7 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
                 const double sxy = MK(a, MF_XY, m), sxx = MK(a, MF_XX, m);
                 const double phi = MK(a, MF_PHI, m);
                 const double me = MK(a, MF_E, m);
-                const double swt1 = 1.0 / (X.g[2] - X.g[1]) / (Y.g[2] - Y.g[1]);
+                const double swt1 = X.inv / (Y.g[2] - Y.g[1]);
                 const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
                 const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
                 double bw = 0.0;
@@ -695,32 +695,31 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
 {
     const DevGrid &g = a.g;
     const int ynumy = g.ynumy;
-    Stencil s = basic_stencil(g, X, Y, x, y);
-    W4 w = wts(s);
-    o.EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
-    o.EXYE = bil(NOD(a, V2C_N_exye), s.m3, ynumy, w);
-    o.SXY = bil(NOD(a, V2C_N_sxy), s.m3, ynumy, w);
-    o.SXYE = bil(NOD(a, V2C_N_sxye), s.m3, ynumy, w);
-    o.ES = bil(NOD(a, V2C_N_pes), s.m3, ynumy, w);
-    o.ESE = bil(NOD(a, V2C_N_pe), s.m3, ynumy, w);
-    o.PHISE = bil(NOD(a, V2C_N_phis), s.m3, ynumy, w);
-    o.PHIS = bil(NOD(a, V2C_N_phis_it), s.m3, ynumy, w);
-    s = centre_stencil(g, X, Y, x, y);
-    w = wts(s);
-    o.EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
-    o.EXXE = bil(NOD(a, V2C_N_exxe), s.m3, ynumy, w);
-    o.SXX = bil(NOD(a, V2C_N_sxx), s.m3, ynumy, w);
-    o.SXXE = bil(NOD(a, V2C_N_sxxe), s.m3, ynumy, w);
-    o.PR = bil(NOD(a, V2C_N_pr), s.m3, ynumy, w);
-    o.SPPE = bil(NOD(a, V2C_N_sppe), s.m3, ynumy, w);
-    s = vx_stencil(g, X, Y, x, y);
-    w = wts(s);
-    o.VX = bil(NOD(a, V2C_N_vx), s.m3, ynumy, w);
-    o.MVX = bil(NOD(a, V2C_N_mvx), s.m3, ynumy, w);
-    s = vy_stencil(g, X, Y, x, y);
-    w = wts(s);
-    o.VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, w);
-    o.MVY = bil(NOD(a, V2C_N_mvy), s.m3, ynumy, w);
+    const Stencil4 S = stencils4_full(g, X, Y, x, y);
+    W4 w = wts(S.b);
+    uint32_t m3 = S.b.m3;
+    o.EXY = bil(NOD(a, V2C_N_exy), m3, ynumy, w);
+    o.EXYE = bil(NOD(a, V2C_N_exye), m3, ynumy, w);
+    o.SXY = bil(NOD(a, V2C_N_sxy), m3, ynumy, w);
+    o.SXYE = bil(NOD(a, V2C_N_sxye), m3, ynumy, w);
+    o.ES = bil(NOD(a, V2C_N_pes), m3, ynumy, w);
+    o.ESE = bil(NOD(a, V2C_N_pe), m3, ynumy, w);
+    o.PHISE = bil(NOD(a, V2C_N_phis), m3, ynumy, w);
+    o.PHIS = bil(NOD(a, V2C_N_phis_it), m3, ynumy, w);
+    w = wts(S.c);
+    m3 = S.c.m3;
+    o.EXX = bil(NOD(a, V2C_N_exx), m3, ynumy, w);
+    o.EXXE = bil(NOD(a, V2C_N_exxe), m3, ynumy, w);
+    o.SXX = bil(NOD(a, V2C_N_sxx), m3, ynumy, w);
+    o.SXXE = bil(NOD(a, V2C_N_sxxe), m3, ynumy, w);
+    o.PR = bil(NOD(a, V2C_N_pr), m3, ynumy, w);
+    o.SPPE = bil(NOD(a, V2C_N_sppe), m3, ynumy, w);
+    w = wts(S.vx);
+    o.VX = bil(NOD(a, V2C_N_vx), S.vx.m3, ynumy, w);
+    o.MVX = bil(NOD(a, V2C_N_mvx), S.vx.m3, ynumy, w);
+    w = wts(S.vy);
+    o.VY = bil(NOD(a, V2C_N_vy), S.vy.m3, ynumy, w);
+    o.MVY = bil(NOD(a, V2C_N_mvy), S.vy.m3, ynumy, w);
 }
 
 // Antidiffusion scatter-back contributions of one marker (interpolation.c:
@@ -732,24 +731,22 @@ __device__ __forceinline__ void anti_contrib(const KArgs &a, double x, double y,
 {
     const DevGrid &g = a.g;
     const int ynumy = g.ynumy;
-    Stencil s = basic_stencil(g, X, Y, x, y);
-    W4 w = wts(s);
-    const double EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
-    const double SXY = bil(NOD(a, V2C_N_sxy), s.m3, ynumy, w);
-    const double ES = bil(NOD(a, V2C_N_pes), s.m3, ynumy, w);
-    const double PHIS = bil(NOD(a, V2C_N_phis_it), s.m3, ynumy, w);
-    s = centre_stencil(g, X, Y, x, y);
-    w = wts(s);
-    const double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
-    const double SXX = bil(NOD(a, V2C_N_sxx), s.m3, ynumy, w);
-    const double PR = bil(NOD(a, V2C_N_pr), s.m3, ynumy, w);
-    s = vx_stencil(g, X, Y, x, y);
-    const double VX = bil(NOD(a, V2C_N_vx), s.m3, ynumy, wts(s));
-    s = vy_stencil(g, X, Y, x, y);
-    const double VY = bil(NOD(a, V2C_N_vy), s.m3, ynumy, wts(s));
-    const double swt1 = 1.0 / (X.g[2] - X.g[1]) / (Y.g[2] - Y.g[1]);
-    const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
-    const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    const Stencil4 S = stencils4(g, X, Y, x, y);
+    W4 w = wts(S.b);
+    const double EXY = bil(NOD(a, V2C_N_exy), S.b.m3, ynumy, w);
+    const double SXY = bil(NOD(a, V2C_N_sxy), S.b.m3, ynumy, w);
+    const double ES = bil(NOD(a, V2C_N_pes), S.b.m3, ynumy, w);
+    const double PHIS = bil(NOD(a, V2C_N_phis_it), S.b.m3, ynumy, w);
+    w = wts(S.c);
+    const double EXX = bil(NOD(a, V2C_N_exx), S.c.m3, ynumy, w);
+    const double SXX = bil(NOD(a, V2C_N_sxx), S.c.m3, ynumy, w);
+    const double PR = bil(NOD(a, V2C_N_pr), S.c.m3, ynumy, w);
+    const double VX = bil(NOD(a, V2C_N_vx), S.vx.m3, ynumy, wts(S.vx));
+    const double VY = bil(NOD(a, V2C_N_vy), S.vy.m3, ynumy, wts(S.vy));
+    // cell-local coordinates (interpolation.c:1140-1150) are the vx x- and
+    // vy y-weights: the same formula on the same cell
+    const double swt1 = X.inv / (Y.g[2] - Y.g[1]);
+    const double cx = S.vx.e, cy = S.vy.n;
     double bw = 0.0;
     bool hasb = true;
     if (cx < 0.5 && cy < 0.5) bw = swt1 * (1.0 - 2 * cx) * (1.0 - 2 * cy);
@@ -916,7 +913,7 @@ __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
 // moveandrotatemarkers (moveandrotatemarkers.c:1-138) when ADVECT, in physical
 // marker order: every marker reads only its own state and nodal fields.
 template <bool ADVECT>
-__global__ __launch_bounds__(256) void k_n2m_update(KArgs a, int nwg)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_n2m_update(KArgs a, int nwg)
 {
     const long m = (long)xcd_remap(blockIdx.x, nwg) * 256 + threadIdx.x;
     bool unsupported = false;

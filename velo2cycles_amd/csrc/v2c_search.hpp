@@ -19,8 +19,9 @@
 namespace v2c {
 
 // Cell s of an axis with its gridlines g[k] = grid[s-1+k], k = 0..3 (entries
-// past either end of the axis hold NaN; no stencil reads them).
-struct Axis4 { double g[4]; int s, pad; };
+// past either end of the axis hold NaN; no stencil reads them) and
+// inv = 1.0 / (grid[s+1] - grid[s]).
+struct Axis4 { double g[4]; double inv; int s, pad; };
 
 __host__ __device__ inline double pick(const Axis4 &A, int m)
 {
@@ -37,6 +38,7 @@ __host__ __device__ inline Axis4 axis_cell(const double *__restrict__ grid, int 
         const int i = s - 1 + k;
         a.g[k] = (i >= 0 && i < n) ? grid[i] : NAN;
     }
+    a.inv = 1.0 / (grid[s + 1] - grid[s]);
     return a;
 }
 
