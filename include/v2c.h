@@ -274,6 +274,9 @@ int v2c_local_download(v2c_ctx *ctx, double *eta, double *V, double *phi, double
                        double *syield, long n);
 /* Set the warm-start guesses directly (host arrays; NULL keeps a field). */
 int v2c_local_set_guess(v2c_ctx *ctx, const double *eta, const double *V, long n);
+/* Read the current warm-start guesses (set directly or proposed by
+ * v2c_mlp_predict; 0 where a solve does not apply). */
+int v2c_local_get_guess(v2c_ctx *ctx, double *eta, double *V, long n);
 
 /* ---- learned warm start (no reference counterpart) ------------------------
  * 6 -> 64 -> 64 -> 2 tanh MLP, BF16 MFMA with FP32 accumulation.  It predicts

@@ -1,6 +1,8 @@
 """Learned warm start: MFMA forward/backward against the FP64 reference,
 deterministic training, and the warm-started local update landing on the same
 roots as the plain one."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -98,4 +100,40 @@ def test_warm_start_same_roots_fewer_rsf_iterations(gpu):
         assert nm == 0 and mx <= 1e-9, (k, mx)
     assert wm["stats"]["rsf_iters"] < pl["stats"]["rsf_iters"]
     assert wm["stats"]["nonfinite"] == 0
+    ctx.close()
+
+
+def test_inference_kernel_matches_forward(gpu):
+    """The batched inference kernel proposes exactly default * 10^(forward(x))
+    with the forward pass of the training kernels on the same features."""
+    dt = synth.YEAR
+    cfg = synth.config_small()
+    ctx = _ctx(gpu, cfg)
+    upload_cloud(ctx, cfg)
+    ws = WarmStart(ctx)
+    X, M = ws.samples(dt)                  # features of markers with an active solve
+    ws.fit_norm(X, M)
+    ws.load(init_params(3) * 1.3)
+    # features of every marker, in marker order, from the extraction kernel
+    n = cfg.n
+    Xa = np.zeros((n, 8), dtype=np.float32)
+    Ma = np.zeros((n, 2), dtype=np.float32)
+    upload_cloud(ctx, cfg)
+    fp = C.POINTER(C.c_float)
+    ctx._check(ctx.lib.v2c_mlp_samples(ctx.h, dt, 1e-10, Xa.ctypes.data_as(fp), Ma.ctypes.data_as(fp)),
+               "mlp_samples")
+    upload_cloud(ctx, cfg)
+    ws.predict(dt)
+    geta, gV = ctx.local_get_guess(n)
+    Xs, _ = ws.standardise(Xa)
+    Y = ws.forward(Xs)
+    nrm = ws.norm
+    for o, g, col in ((0, geta, 6), (1, gV, 7)):
+        act = Ma[:, o] > 0
+        assert act.sum() > 1000
+        r = Y[act, o].astype(np.float64) * nrm[14 + o] + nrm[12 + o]
+        expect = 10.0 ** (Xa[act, col].astype(np.float64) + r)
+        rel = np.abs(g[act] - expect) / expect
+        # default starts pass through float32 in the extracted features
+        assert np.quantile(rel, 0.999) < 1e-5, (o, np.quantile(rel, 0.999))
     ctx.close()

@@ -218,6 +218,7 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                        ("v2c_local_update", [C.c_double, C.c_int, C.c_double, P(LUStats)]),
                        ("v2c_local_download", [P(C.c_double)] * 5 + [C.c_long]),
                        ("v2c_local_set_guess", [P(C.c_double), P(C.c_double), C.c_long]),
+                       ("v2c_local_get_guess", [P(C.c_double), P(C.c_double), C.c_long]),
                        ("v2c_kernel_time", [C.c_int, P(C.c_double), P(C.c_long)]),
                        ("v2c_reset_timing", [])]:
         fn = getattr(lib, name)
@@ -385,6 +386,12 @@ class Context:
         e = None if eta is None else np.ascontiguousarray(eta, dtype=np.float64)
         v = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
         self._check(self.lib.v2c_local_set_guess(self.h, _opt(e), _opt(v), n), "local_set_guess")
+
+    def local_get_guess(self, n):
+        eta = np.zeros(n)
+        V = np.zeros(n)
+        self._check(self.lib.v2c_local_get_guess(self.h, _dptr(eta), _dptr(V), n), "local_get_guess")
+        return eta, V
 
     def set_friction_fields(self, on):
         self._check(self.lib.v2c_set_friction_fields(self.h, int(on)), "set_friction_fields")

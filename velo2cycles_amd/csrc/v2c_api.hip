@@ -800,6 +800,18 @@ int v2c_local_set_guess(v2c_ctx *c, const double *eta, const double *V, long n)
     return 0;
 }
 
+int v2c_local_get_guess(v2c_ctx *c, double *eta, double *V, long n)
+{
+    if (!c || n < 0 || n > c->n) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (eta) HIPCHK(c, hipMemcpyAsync(eta, c->d_mx + (size_t)MX_GETA * c->cap, sizeof(double) * n,
+                                      hipMemcpyDeviceToHost, c->stream));
+    if (V) HIPCHK(c, hipMemcpyAsync(V, c->d_mx + (size_t)MX_GV * c->cap, sizeof(double) * n,
+                                    hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int v2c_local_update(v2c_ctx *c, double dt, int warm, double tol, v2c_lu_stats *st)
 {
     if (!c || !(tol > 0)) return V2C_EINVAL;
@@ -886,8 +898,8 @@ int v2c_mlp_predict(v2c_ctx *c, double dt)
     int rc = 0;
     if (c->n > 0) {
         rc = launch(c, V2C_K_MLP, [&] {
-            hipLaunchKernelGGL(k_mlp_guess, dim3(blocks_for(c->n, 64)), dim3(64), 0, c->stream, a, dt, c->d_mlp,
-                               c->d_mlp_norm);
+            hipLaunchKernelGGL(k_mlp_guess, dim3(blocks_for(c->n, 64 * MI_WAVES)), dim3(64 * MI_WAVES), 0, c->stream,
+                               a, dt, c->d_mlp, c->d_mlp_norm);
         });
         if (rc) return rc;
     }

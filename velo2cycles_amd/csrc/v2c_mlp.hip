@@ -124,8 +124,8 @@ __device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, 
     const double sxx = MK(a, MF_XX, m), sxy = MK(a, MF_XY, m);
     const double exx = MK(a, MF_EXX, m), exy = MK(a, MF_EXY, m);
     const double tk = MK(a, MF_K, m), p = MK(a, MF_P, m);
-    const double sig = pow(sxx * sxx + sxy * sxy, 0.5);
-    const double eii = pow(exx * exx + exy * exy, 0.5);
+    const double sig = sqrt(sxx * sxx + sxy * sxy);
+    const double eii = sqrt(exx * exx + exy * exy);
     o.duct = false;
     o.rsf = false;
     o.eta0 = 0;
@@ -166,26 +166,112 @@ __device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, 
 
 // Inference: standardised features -> guesses written to MX_GETA / MX_GV
 // (0 where a solve does not apply, which the local update treats as "no
-// guess").
-__global__ __launch_bounds__(64) void k_mlp_guess(KArgs a, double dt, const float *__restrict__ P,
-                                                  const float *__restrict__ norm)
+// guess").  Four wavefronts per block share the weights, converted to BF16
+// once per block into LDS in the layout the B operand reads (4 consecutive k
+// per lane); each wavefront keeps its 64 rows of activations as BF16 in LDS,
+// layer 2's 16 output tiles in registers until all of layer 1 has been read.
+// The arithmetic is the training forward's (same roundings, same k order).
+constexpr int MI_WAVES = 4;
+struct MlpInferLds {
+    short w1[MLP_H][16];          // [out][in], in padded to 16
+    short w2[MLP_H][MLP_H];       // [out][in]
+    short w3[16][MLP_H];          // [out][in], out padded to 16
+    float b1[MLP_H], b2[MLP_H], b3[MLP_OUT];
+    short x[MI_WAVES][64][16];
+    short h[MI_WAVES][64][MLP_H];
+};
+
+__device__ __forceinline__ bf16x4 lds_bf16x4(const short *p)
 {
-    __shared__ MlpLds s;
-    const int lane = threadIdx.x;
-    const long m = (long)blockIdx.x * 64 + lane;
+    bf16x4 v;
+    v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
+    return v;
+}
+
+__global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_guess(KArgs a, double dt, const float *__restrict__ P,
+                                                             const float *__restrict__ norm)
+{
+    __shared__ MlpInferLds s;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    for (int e = tid; e < MLP_H * 16; e += 64 * MI_WAVES) {
+        const int o = e >> 4, k = e & 15;
+        s.w1[o][k] = k < MLP_IN ? f2bf(P[P_W1 + o * MLP_IN + k]) : (short)0;
+    }
+    for (int e = tid; e < MLP_H * MLP_H; e += 64 * MI_WAVES) (&s.w2[0][0])[e] = f2bf(P[P_W2 + e]);
+    for (int e = tid; e < 16 * MLP_H; e += 64 * MI_WAVES) {
+        const int o = e / MLP_H;
+        (&s.w3[0][0])[e] = o < MLP_OUT ? f2bf(P[P_W3 + e]) : (short)0;
+    }
+    for (int e = tid; e < MLP_H; e += 64 * MI_WAVES) { s.b1[e] = P[P_B1 + e]; s.b2[e] = P[P_B2 + e]; }
+    if (tid < MLP_OUT) s.b3[tid] = P[P_B3 + tid];
+
+    const long m = ((long)blockIdx.x * MI_WAVES + w) * 64 + lane;
     MlpFeat ft;
     ft.duct = ft.rsf = false;
     if (m < a.n) mlp_features(a, m, dt, ft);
     for (int k = 0; k < 16; k++) {
         float v = 0.f;
         if (k < MLP_IN && m < a.n) v = (ft.f[k] - norm[k]) / norm[MLP_IN + k];
-        s.x[lane][k] = f2bf(v);
+        s.x[w][lane][k] = f2bf(v);
     }
     __syncthreads();
-    mlp_forward(P, s);
+    const int i = lane & 15, kb = 4 * (lane >> 4);
+    short(*h)[MLP_H] = s.h[w];
+    // layer 1: h = tanh(x W1^T + b1)
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&s.x[w][mt * 16 + i][kb]),
+                                                          lds_bf16x4(&s.w1[nt * 16 + i][kb]), d, 0, 0, 0);
+            for (int j = 0; j < 4; j++)
+                h[mt * 16 + kb + j][nt * 16 + i] = f2bf(fast_tanh(d[j] + s.b1[nt * 16 + i]));
+        }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 2: all 16 tiles read layer 1 before any is written back
+    f32x4 d2[4][4];
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < 4; ks++)
+                d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&h[mt * 16 + i][ks * 16 + kb]),
+                                                              lds_bf16x4(&s.w2[nt * 16 + i][ks * 16 + kb]), d, 0, 0, 0);
+            d2[mt][nt] = d;
+        }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++)
+            for (int j = 0; j < 4; j++)
+                h[mt * 16 + kb + j][nt * 16 + i] = f2bf(fast_tanh(d2[mt][nt][j] + s.b2[nt * 16 + i]));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 3: y = h W3^T + b3; row mt*16 + kb + j, output i (< 2 used)
+    float y0 = 0.f, y1 = 0.f;
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < 4; ks++)
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&h[mt * 16 + i][ks * 16 + kb]),
+                                                          lds_bf16x4(&s.w3[i][ks * 16 + kb]), d, 0, 0, 0);
+        // hand row (mt*16 + kb + j, output i) to the lane owning that marker
+        for (int j = 0; j < 4; j++) {
+            if (i < MLP_OUT) {
+                float *slot = (float *)&s.x[w][mt * 16 + kb + j][2 * i];   // x is no longer needed
+                *slot = d[j] + s.b3[i];
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    y0 = *(const float *)&s.x[w][lane][0];
+    y1 = *(const float *)&s.x[w][lane][2];
     if (m < a.n) {
-        const double r0 = s.y[lane][0] * norm[2 * MLP_IN + 2] + norm[2 * MLP_IN];
-        const double r1 = s.y[lane][1] * norm[2 * MLP_IN + 3] + norm[2 * MLP_IN + 1];
+        const double r0 = y0 * norm[2 * MLP_IN + 2] + norm[2 * MLP_IN];
+        const double r1 = y1 * norm[2 * MLP_IN + 3] + norm[2 * MLP_IN + 1];
         MXF(a, MX_GETA, m) = ft.duct ? ft.eta0 * pow(10.0, r0) : 0.0;
         MXF(a, MX_GV, m) = ft.rsf ? ft.vhi * pow(10.0, r1) : 0.0;
     }
