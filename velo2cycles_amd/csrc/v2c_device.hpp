@@ -161,6 +161,61 @@ __device__ __forceinline__ Stencil centre_stencil(const DevGrid &g, const Axis4 
     return s;
 }
 
+// Basic-node stencil without ghost nodes (interpolation.c:1069-1083, 1711-1717).
+__device__ __forceinline__ Stencil basic_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
+                                                 double x, double y)
+{
+    int m10 = X.s, m20 = Y.s;
+    if (m10 < 1) m10 = 1;
+    if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
+    if (m20 < 1) m20 = 1;
+    if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
+    double a0, a1, b0, b1;                    // grid[m], grid[m+1]
+    sel2(X, m10 - X.s + 1, a0, a1);
+    sel2(Y, m20 - Y.s + 1, b0, b1);
+    Stencil s;
+    s.e = (x - a0) / (a1 - a0);
+    s.n = (y - b0) / (b1 - b0);
+    s.m3 = (uint32_t)(m10 * g.ynumy + m20);
+    return s;
+}
+
+// vx-node stencil (interpolation.c:1377-1388, 1738-1744).  The x clamp of
+// allinteri is a no-op: the searched cell is already in [0, xnumx-2].
+__device__ __forceinline__ Stencil vx_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
+                                              double x, double y)
+{
+    int m20 = Y.s;
+    if (y < (Y.g[1] + Y.g[2]) / 2.0) m20 -= 1;
+    if (m20 < 0) m20 = 0;
+    if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
+    double b0, b1, b2;                        // grid[m], grid[m+1], grid[m+2]
+    sel3(Y, m20 - Y.s + 1, b0, b1, b2);
+    Stencil s;
+    s.e = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    s.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
+    s.m3 = (uint32_t)(X.s * g.ynumy + m20);
+    return s;
+}
+
+// vy-node stencil (interpolation.c:1434-1445, 1749-1755); the y clamp of
+// allinteri is likewise a no-op.
+__device__ __forceinline__ Stencil vy_stencil(const DevGrid &g, const Axis4 &X, const Axis4 &Y,
+                                              double x, double y)
+{
+    int m10 = X.s;
+    if (x < (X.g[1] + X.g[2]) / 2.0) m10 -= 1;
+    if (m10 < 0) m10 = 0;
+    if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
+    double a0, a1, a2;
+    sel3(X, m10 - X.s + 1, a0, a1, a2);
+    Stencil s;
+    s.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
+    s.n = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    s.m3 = (uint32_t)(m10 * g.ynumy + Y.s);
+    return s;
+}
+
 // All four stencils of one point (basic interpolation.c:1069-1083, 1711-1717;
 // centre :1216-1224, 1723-1730; vx :1377-1388, 1738-1744; vy :1434-1445,
 // 1749-1755), each weight with the reference's formula and operands.  The x
@@ -475,16 +530,17 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
     const Axis4 X = hx ? *hx : find_x(g, x), Y = hy ? *hy : find_y(g, y);
     const double *vx = NOD(a, V2C_N_vx), *vy = NOD(a, V2C_N_vy);
 
-    const Stencil4 S = stencils4_full(g, X, Y, x, y);
-    W4 w = wts(S.b);
-    double EXY = bil(NOD(a, V2C_N_exy), S.b.m3, ynumy, w);
-    ESP = bil(NOD(a, V2C_N_esp), S.b.m3, ynumy, w);
+    Stencil s = basic_stencil(g, X, Y, x, y);
+    W4 w = wts(s);
+    double EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
+    ESP = bil(NOD(a, V2C_N_esp), s.m3, ynumy, w);
 
-    w = wts(S.c);
-    double EXX = bil(NOD(a, V2C_N_exx), S.c.m3, ynumy, w);
+    s = centre_stencil(g, X, Y, x, y);
+    w = wts(s);
+    double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
     // pressure-node averages of vx and vy (interpolation.c:1725-1736); the
     // centre stencil's m3 >= ynumy + 1, so every index below is >= 0
-    const uint32_t m3 = S.c.m3, u = (uint32_t)ynumy;
+    const uint32_t m3 = s.m3, u = (uint32_t)ynumy;
     double vxa = (w.d0 * (at32(vx, m3 - 1) + at32(vx, m3 - u - 1)) + w.d1 * (at32(vx, m3) + at32(vx, m3 - u))
                   + w.d2 * (at32(vx, m3 + u - 1) + at32(vx, m3 - 1)) + w.d3 * (at32(vx, m3 + u) + at32(vx, m3)))
                  * 0.5 * (1.0 - xrat);
@@ -493,8 +549,10 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
                  * 0.5 * (1.0 - xrat);
     EE = sqrt(EXX * EXX + EXY * EXY);
 
-    VX = vxa + bil(vx, S.vx.m3, ynumy, wts(S.vx)) * xrat;
-    VY = vya + bil(vy, S.vy.m3, ynumy, wts(S.vy)) * xrat;
+    s = vx_stencil(g, X, Y, x, y);
+    VX = vxa + bil(vx, s.m3, ynumy, wts(s)) * xrat;
+    s = vy_stencil(g, X, Y, x, y);
+    VY = vya + bil(vy, s.m3, ynumy, wts(s)) * xrat;
 }
 
 // ---- build-defined per-marker local update (not reference behaviour) -------

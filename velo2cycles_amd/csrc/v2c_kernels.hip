@@ -374,9 +374,9 @@ __device__ __forceinline__ void scatter_batch(const double (*con)[R::NF], const 
 template <class R>
 __device__ __forceinline__ void zero_empty(const KArgs &a, long c0, long c1, int lane)
 {
-    const long k = c0 * 4 + lane;
-    if (k < c1 * 4 && a.sub_start[k] == a.sub_start[k + 1])
-        for (int f = 0; f < R::NF; f++) a.part[R::index((uint32_t)k, f, a.g.ynumy, a.nodenum)] = 0.0;
+    for (long k = c0 * 4 + lane; k < c1 * 4; k += WAVE)
+        if (a.sub_start[k] == a.sub_start[k + 1])
+            for (int f = 0; f < R::NF; f++) a.part[R::index((uint32_t)k, f, a.g.ynumy, a.nodenum)] = 0.0;
 }
 
 // Cells of a marker from its sub-cell key (the key is computed by the same
@@ -398,10 +398,8 @@ __device__ __forceinline__ void key_axes(const KArgs &a, uint32_t key, Axis4 &X,
 // ---------------------------------------------------------------------------
 enum MProp { P_NU = 0, P_RO, P_BB, P_KT, NPROP };
 
-__global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
+__device__ __forceinline__ void m2n_props_one(const KArgs &a, long m)
 {
-    const long m = (long)xcd_remap(blockIdx.x, nwg) * 256 + threadIdx.x;
-    if (m >= a.n) return;
     const DevGrid &g = a.g;
     const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
     if (!(x > 0 && y > 0 && x < g.xsize && y < g.ysize)) return;
@@ -426,21 +424,29 @@ __global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
     P[(long)P_KT * a.cap + m] = mkt;
 }
 
+__global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
+{
+    const long m = (long)xcd_remap(blockIdx.x, nwg) * 256 + threadIdx.x;
+    if (m < a.n) m2n_props_one(a, m);
+}
+
 // ---------------------------------------------------------------------------
 // marker2nodes, scatter pass (interpolation.c:215-629) over the sorted order:
 // weights and contributions of every marker inside the grid, reduced per
 // (cell, quadrant) and written as per-cell partials.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
+template <int NF>
+struct ScatterLds {
+    double con[WAVE][NF];
+    double carry[2][NF];
+    uint32_t skey[WAVE];
+    int runs[WAVE + 1];
+};
+
+// Cells [c0, c1) of one wavefront group.
+__device__ __forceinline__ void m2n_cells_group(const KArgs &a, long c0, long c1, int lane,
+                                                ScatterLds<M_NF> &L)
 {
-    __shared__ double con[WAVE][M_NF];
-    __shared__ double carry[2][M_NF];
-    __shared__ uint32_t skey[WAVE];
-    __shared__ int runs[WAVE + 1];
-    const int lane = threadIdx.x;
-    const int wg = xcd_remap(blockIdx.x, nwg);
-    const long c0 = (long)wg * CPW;
-    const long c1 = c0 + CPW < a.ncell ? c0 + CPW : a.ncell;
     const uint32_t beg = a.sub_start[c0 * 4], end = a.sub_start[c1 * 4];
     const DevMat *M = a.mat;
     const DevGrid &g = a.g;
@@ -509,15 +515,22 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
             }
         }
 #pragma unroll
-        for (int f = 0; f < M_NF; f++) con[lane][f] = v[f];
-        skey[lane] = key;
+        for (int f = 0; f < M_NF; f++) L.con[lane][f] = v[f];
+        L.skey[lane] = key;
         const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
         __syncthreads();
-        scatter_batch<MRec>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a);
+        scatter_batch<MRec>(L.con, L.skey, L.runs, L.carry, par, have_carry, cont, lane, nact, a);
         have_carry = cont;
         __syncthreads();
     }
     zero_empty<MRec>(a, c0, c1, lane);
+}
+
+__global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
+{
+    __shared__ ScatterLds<M_NF> L;
+    const long c0 = (long)xcd_remap(blockIdx.x, nwg) * CPW;
+    m2n_cells_group(a, c0, c0 + CPW < a.ncell ? c0 + CPW : a.ncell, threadIdx.x, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -525,10 +538,8 @@ __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
 // (interpolation.c:58-153 zeroing, :714-824 normalisation, :827-859 BC copy
 // runs as k_phis_bc)
 // ---------------------------------------------------------------------------
-__global__ void k_m2n_nodes(KArgs a)
+__device__ __forceinline__ void m2n_nodes_one(const KArgs &a, long m3)
 {
-    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m3 >= a.nodenum) return;
     const int ynumy = a.g.ynumy, xnumx = a.g.xnumx;
     const int m22 = (int)(m3 % ynumy), m11 = (int)(m3 / ynumy);
     const double *P = a.part;
@@ -662,12 +673,16 @@ __global__ void k_m2n_nodes(KArgs a)
     }
 }
 
-// State-variable boundary copy when its diffusion is on (interpolation.c:827-859).
-// Reads interior values only and writes boundary nodes only, so one pass is race free.
-__global__ void k_phis_bc(KArgs a)
+__global__ __launch_bounds__(256) void k_m2n_nodes(KArgs a)
 {
     const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m3 >= a.nodenum) return;
+    if (m3 < a.nodenum) m2n_nodes_one(a, m3);
+}
+
+// State-variable boundary copy when its diffusion is on (interpolation.c:827-859).
+// Reads interior values only and writes boundary nodes only, so one pass is race free.
+__device__ __forceinline__ void phis_bc_one(const KArgs &a, long m3)
+{
     const int ynumy = a.g.ynumy, xnumx = a.g.xnumx;
     const int m22 = (int)(m3 % ynumy), m11 = (int)(m3 / ynumy);
     double *phis = NOD(a, V2C_N_phis);
@@ -678,6 +693,12 @@ __global__ void k_phis_bc(KArgs a)
     else if ((m11 == 0 && m22 == 0) || (m11 == 0 && m22 == ynumy - 1) ||
              (m11 == xnumx - 1 && m22 == 0) || (m11 == xnumx - 1 && m22 == ynumy - 1))
         phis[m3] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_phis_bc(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m3 < a.nodenum) phis_bc_one(a, m3);
 }
 
 // ---------------------------------------------------------------------------
@@ -695,9 +716,9 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
 {
     const DevGrid &g = a.g;
     const int ynumy = g.ynumy;
-    const Stencil4 S = stencils4_full(g, X, Y, x, y);
-    W4 w = wts(S.b);
-    uint32_t m3 = S.b.m3;
+    Stencil S = basic_stencil(g, X, Y, x, y);
+    W4 w = wts(S);
+    uint32_t m3 = S.m3;
     o.EXY = bil(NOD(a, V2C_N_exy), m3, ynumy, w);
     o.EXYE = bil(NOD(a, V2C_N_exye), m3, ynumy, w);
     o.SXY = bil(NOD(a, V2C_N_sxy), m3, ynumy, w);
@@ -706,20 +727,23 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
     o.ESE = bil(NOD(a, V2C_N_pe), m3, ynumy, w);
     o.PHISE = bil(NOD(a, V2C_N_phis), m3, ynumy, w);
     o.PHIS = bil(NOD(a, V2C_N_phis_it), m3, ynumy, w);
-    w = wts(S.c);
-    m3 = S.c.m3;
+    S = centre_stencil(g, X, Y, x, y);
+    w = wts(S);
+    m3 = S.m3;
     o.EXX = bil(NOD(a, V2C_N_exx), m3, ynumy, w);
     o.EXXE = bil(NOD(a, V2C_N_exxe), m3, ynumy, w);
     o.SXX = bil(NOD(a, V2C_N_sxx), m3, ynumy, w);
     o.SXXE = bil(NOD(a, V2C_N_sxxe), m3, ynumy, w);
     o.PR = bil(NOD(a, V2C_N_pr), m3, ynumy, w);
     o.SPPE = bil(NOD(a, V2C_N_sppe), m3, ynumy, w);
-    w = wts(S.vx);
-    o.VX = bil(NOD(a, V2C_N_vx), S.vx.m3, ynumy, w);
-    o.MVX = bil(NOD(a, V2C_N_mvx), S.vx.m3, ynumy, w);
-    w = wts(S.vy);
-    o.VY = bil(NOD(a, V2C_N_vy), S.vy.m3, ynumy, w);
-    o.MVY = bil(NOD(a, V2C_N_mvy), S.vy.m3, ynumy, w);
+    S = vx_stencil(g, X, Y, x, y);
+    w = wts(S);
+    o.VX = bil(NOD(a, V2C_N_vx), S.m3, ynumy, w);
+    o.MVX = bil(NOD(a, V2C_N_mvx), S.m3, ynumy, w);
+    S = vy_stencil(g, X, Y, x, y);
+    w = wts(S);
+    o.VY = bil(NOD(a, V2C_N_vy), S.m3, ynumy, w);
+    o.MVY = bil(NOD(a, V2C_N_mvy), S.m3, ynumy, w);
 }
 
 // Antidiffusion scatter-back contributions of one marker (interpolation.c:
@@ -868,16 +892,10 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
 // 1405-1489) over the sorted order, before any marker moves: contributions of
 // every marker at its clamped position, reduced per (cell, quadrant) into
 // per-cell partials.
-__global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
+// Cells [c0, c1) of one wavefront group.
+__device__ __forceinline__ void n2m_scatter_group(const KArgs &a, long c0, long c1, int lane,
+                                                  ScatterLds<N_NF> &L)
 {
-    __shared__ double con[WAVE][N_NF];
-    __shared__ double carry[2][N_NF];
-    __shared__ uint32_t skey[WAVE];
-    __shared__ int runs[WAVE + 1];
-    const int lane = threadIdx.x;
-    const int wg = xcd_remap(blockIdx.x, nwg);
-    const long c0 = (long)wg * CPW;
-    const long c1 = c0 + CPW < a.ncell ? c0 + CPW : a.ncell;
     const DevGrid &g = a.g;
     const uint32_t beg = a.sub_start[c0 * 4], end = a.sub_start[c1 * 4];
     bool have_carry = false;
@@ -898,63 +916,76 @@ __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
             anti_contrib(a, x, y, X, Y, a.mat->ro[a.t[m]], v);
         }
 #pragma unroll
-        for (int f = 0; f < N_NF; f++) con[lane][f] = v[f];
-        skey[lane] = key;
+        for (int f = 0; f < N_NF; f++) L.con[lane][f] = v[f];
+        L.skey[lane] = key;
         const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
         __syncthreads();
-        scatter_batch<NRec>(con, skey, runs, carry, par, have_carry, cont, lane, nact, a);
+        scatter_batch<NRec>(L.con, L.skey, L.runs, L.carry, par, have_carry, cont, lane, nact, a);
         have_carry = cont;
         __syncthreads();
     }
     zero_empty<NRec>(a, c0, c1, lane);
 }
 
+__global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
+{
+    __shared__ ScatterLds<N_NF> L;
+    const long c0 = (long)xcd_remap(blockIdx.x, nwg) * CPW;
+    n2m_scatter_group(a, c0, c0 + CPW < a.ncell ? c0 + CPW : a.ncell, threadIdx.x, L);
+}
+
 // nodes2markers marker update (interpolation.c:943-1011), fused with
 // moveandrotatemarkers (moveandrotatemarkers.c:1-138) when ADVECT, in physical
 // marker order: every marker reads only its own state and nodal fields.
+// Returns true when the marker's type is not supported by the advection.
+template <bool ADVECT>
+__device__ __forceinline__ bool n2m_update_one(const KArgs &a, long m)
+{
+    bool unsupported = false;
+    const DevGrid &g = a.g;
+    const bool anti = a.p.antidiffinter == 1;
+    const int t = a.t[m];
+    Axis4 X, Y;
+    key_axes(a, a.key[m], X, Y);
+    MState s;
+    s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
+    s.dirty_state = false; s.dirty_k = false;
+    if (!anti) {
+        s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); s.p = MK(a, MF_P, m);
+        s.exx = MK(a, MF_EXX, m); s.exy = MK(a, MF_EXY, m); s.vx = MK(a, MF_VX, m);
+        s.vy = MK(a, MF_VY, m); s.phi = MK(a, MF_PHI, m); s.e = MK(a, MF_E, m);
+    }
+    {
+        Mech mv;
+        mech_gather(a, clampd(s.x, 0, g.xsize), clampd(s.y, 0, g.ysize), X, Y, mv);
+        n2m_update(a, t, s, mv);
+    }
+    if (s.dirty_state) {
+        MK(a, MF_PHI, m) = s.phi; MK(a, MF_EXX, m) = s.exx; MK(a, MF_EXY, m) = s.exy;
+        MK(a, MF_P, m) = s.p; MK(a, MF_VX, m) = s.vx; MK(a, MF_VY, m) = s.vy; MK(a, MF_E, m) = s.e;
+    }
+    bool moved = false, reset = false;
+    if (ADVECT) {
+        if (!s.dirty_state) { s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); }
+        if (!advect_one(a, t, s, moved, reset, &X, &Y)) unsupported = true;
+    }
+    if (s.dirty_state || moved) { MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy; }
+    if (s.dirty_k) MK(a, MF_K, m) = s.k;
+    if (ADVECT) {
+        if (moved) {
+            MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
+            a.key[m] = subcell_key(a.g, s.x, s.y);
+        }
+        if (reset) { MK(a, MF_D, m) = -1.0; MK(a, MF_W, m) = -1.0; MK(a, MF_E, m) = 0; }
+    }
+    return unsupported;
+}
+
 template <bool ADVECT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_n2m_update(KArgs a, int nwg)
 {
     const long m = (long)xcd_remap(blockIdx.x, nwg) * 256 + threadIdx.x;
-    bool unsupported = false;
-    if (m < a.n) {
-        const DevGrid &g = a.g;
-        const bool anti = a.p.antidiffinter == 1;
-        const int t = a.t[m];
-        Axis4 X, Y;
-        key_axes(a, a.key[m], X, Y);
-        MState s;
-        s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
-        s.dirty_state = false; s.dirty_k = false;
-        if (!anti) {
-            s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); s.p = MK(a, MF_P, m);
-            s.exx = MK(a, MF_EXX, m); s.exy = MK(a, MF_EXY, m); s.vx = MK(a, MF_VX, m);
-            s.vy = MK(a, MF_VY, m); s.phi = MK(a, MF_PHI, m); s.e = MK(a, MF_E, m);
-        }
-        {
-            Mech mv;
-            mech_gather(a, clampd(s.x, 0, g.xsize), clampd(s.y, 0, g.ysize), X, Y, mv);
-            n2m_update(a, t, s, mv);
-        }
-        if (s.dirty_state) {
-            MK(a, MF_PHI, m) = s.phi; MK(a, MF_EXX, m) = s.exx; MK(a, MF_EXY, m) = s.exy;
-            MK(a, MF_P, m) = s.p; MK(a, MF_VX, m) = s.vx; MK(a, MF_VY, m) = s.vy; MK(a, MF_E, m) = s.e;
-        }
-        bool moved = false, reset = false;
-        if (ADVECT) {
-            if (!s.dirty_state) { s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); }
-            if (!advect_one(a, t, s, moved, reset, &X, &Y)) unsupported = true;
-        }
-        if (s.dirty_state || moved) { MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy; }
-        if (s.dirty_k) MK(a, MF_K, m) = s.k;
-        if (ADVECT) {
-            if (moved) {
-                MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
-                a.key[m] = subcell_key(a.g, s.x, s.y);
-            }
-            if (reset) { MK(a, MF_D, m) = -1.0; MK(a, MF_W, m) = -1.0; MK(a, MF_E, m) = 0; }
-        }
-    }
+    const bool unsupported = m < a.n && n2m_update_one<ADVECT>(a, m);
     if (ADVECT) {
         const unsigned long long bad = __ballot(unsupported);
         if (bad && (threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&a.counters[1], (int)__popcll(bad));
@@ -964,9 +995,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // The extra iteration of the reference's `m3<=marknum` loop
 // (interpolation.c:937): a zero marker of type 0 at (0,0).  Only its
 // antidiffusion scatter has an effect; added into the persistent sums.
-__global__ void k_n2m_phantom(KArgs a)
+__device__ __forceinline__ void n2m_phantom_one(const KArgs &a)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || a.p.antidiffinter != 1) return;
     double v[N_NF];
     for (int f = 0; f < N_NF; f++) v[f] = 0.0;
     anti_contrib(a, 0.0, 0.0, find_x(a.g, 0.0), find_y(a.g, 0.0), a.mat->ro[0], v);
@@ -981,12 +1011,15 @@ __global__ void k_n2m_phantom(KArgs a)
     NOD(a, V2C_N_exxsum)[cn] += v[N_C + 2]; NOD(a, V2C_N_wtsxxsum)[cn] += v[N_C + 3];
 }
 
+__global__ void k_n2m_phantom(KArgs a)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && a.p.antidiffinter == 1) n2m_phantom_one(a);
+}
+
 // nodes2markers node pass: add the cell partials to the persistent sums and
 // normalise (interpolation.c:1014-1046).
-__global__ void k_n2m_nodes(KArgs a)
+__device__ __forceinline__ void n2m_nodes_one(const KArgs &a, long m3)
 {
-    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m3 >= a.nodenum) return;
     const int ynumy = a.g.ynumy, xnumx = a.g.xnumx;
     const int m22 = (int)(m3 % ynumy), m11 = (int)(m3 / ynumy);
     const double *P = a.part;
@@ -1049,6 +1082,12 @@ __global__ void k_n2m_nodes(KArgs a)
     if (s_wvy != 0) NOD(a, V2C_N_mvye)[m3] = s_vy / s_mry;
 }
 
+__global__ __launch_bounds__(256) void k_n2m_nodes(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m3 < a.nodenum) n2m_nodes_one(a, m3);
+}
+
 // Standalone moveandrotatemarkers over physical marker order.
 __global__ __launch_bounds__(256) void k_advect(KArgs a)
 {
@@ -1078,10 +1117,8 @@ __global__ __launch_bounds__(256) void k_advect(KArgs a)
 // ---------------------------------------------------------------------------
 // compute_before_solve / calc_vpvisc (plastic.c:2-56, calc_RSF.c:33-169)
 // ---------------------------------------------------------------------------
-__global__ void k_before(KArgs a, double dt, int rk)
+__device__ __forceinline__ void before_one(const KArgs &a, long m3, double dt, int rk)
 {
-    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m3 >= a.nodenum) return;
     const int ynumy = a.g.ynumy;
     const int m2 = (int)(m3 % ynumy), m1 = (int)(m3 / ynumy);
     if (!(m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ynumy - 1)) return;
@@ -1148,72 +1185,80 @@ __global__ void k_before(KArgs a, double dt, int rk)
     if (err) atomicAdd(&a.counters[0], 1);
 }
 
+__global__ __launch_bounds__(256) void k_before(KArgs a, double dt, int rk)
+{
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m3 < a.nodenum) before_one(a, m3, dt, rk);
+}
+
 // compute_after_solve / calc_sliprate_after_solve (plastic.c:61-129,
 // calc_RSF.c:172-221).  Per-block partial sums of the misfit (fixed tree
 // order) go to red[2*block], red[2*block+1].
 constexpr int RED_BLOCK = 256;
-__global__ __launch_bounds__(RED_BLOCK) void k_after(KArgs a, double dt, double *red)
+// One node of compute_after_solve; adds its misfit term to (dsq, cnt).
+__device__ __forceinline__ void after_one(const KArgs &a, long m3, double dt, double &dsq, double &cnt)
+{
+    const int ynumy = a.g.ynumy;
+    const int m2 = (int)(m3 % ynumy), m1 = (int)(m3 / ynumy);
+    if (m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ynumy - 1) {
+        const v2c_params &p = a.p;
+        const double W = p.faultwidth;
+        const double nP = (p.pmode == 0 || a.timesum == 0) ? p.nstress : NOD(a, V2C_N_sppes)[m3];
+        const double sxxs = NOD(a, V2C_N_sxxs)[m3], sxy = NOD(a, V2C_N_sxy)[m3];
+        const double sigin = pow(sxxs * sxxs + sxy * sxy, 0.5);
+        double *Vs = NOD(a, V2C_N_RK_sliprs_solved);
+        const double as = NOD(a, V2C_N_as)[m3], bs = NOD(a, V2C_N_bs)[m3], Ls = NOD(a, V2C_N_Ls)[m3];
+        const double V0s = NOD(a, V2C_N_V0s)[m3], mus = NOD(a, V2C_N_mus_s)[m3];
+        const double phi = NOD(a, V2C_N_phis_it)[m3];
+        double Vout;
+        if (p.RSFmode == 1) {
+            if (sigin > 0) {
+                double V;
+                if (p.power_law_RSF > 0) {
+                    const double sy0 = nP * mus;
+                    V = V0s * pow(sigin / sy0, (mus / as)) * exp((-phi * bs) / as);
+                } else {
+                    V = 2 * V0s * sinh(sigin / (as * nP)) * exp(-(mus + bs * phi) / as);
+                }
+                const double EPXX = NOD(a, V2C_N_exxpl)[m3], EXY = NOD(a, V2C_N_exy)[m3];
+                const double EII = pow(EPXX * EPXX + EXY * EXY, 0.5);
+                V = ref_minv(V, EII * W);
+                if (V < 1e-40) V = 1e-40;
+                Vout = V;
+            } else {
+                Vout = 0;
+            }
+            Vs[m3] = Vout;
+            NOD(a, V2C_N_RK_phidts_solved)[m3] = V0s / Ls * exp(-phi) - Vout / Ls;
+        } else {
+            const double G = NOD(a, V2C_N_gg)[m3], NUVP = NOD(a, V2C_N_nu)[m3];
+            const double NU = NOD(a, V2C_N_nu_m2c)[m3];
+            const double E2EL = sigin / (NUVP / (G * dt + NUVP));
+            const double A = NOD(a, V2C_N_syields)[m3] / E2EL;
+            if (A < 1) {
+                const double nup = G * dt * A / (1.0 - A);
+                Vout = W * (sigin / 2.0 * (1.0 / (nup) - 1.0 / NU));
+            } else {
+                Vout = 0;
+            }
+            Vs[m3] = Vout;
+        }
+        if (Vout > 0) {
+            const double d = NOD(a, V2C_N_syields)[m3] - sigin;
+            NOD(a, V2C_N_RK_deltas)[m3] = d;
+            dsq = pow(d, 2);
+            cnt = 1.0;
+        } else {
+            NOD(a, V2C_N_RK_deltas)[m3] = 0;
+        }
+    }
+}
+
+// Fixed-order tree sum of the block's (dsq, cnt) into red[2*slot], red[2*slot+1].
+__device__ __forceinline__ void after_block_sum(double dsq, double cnt, double *red, long slot)
 {
     __shared__ double ssum[RED_BLOCK];
     __shared__ double scnt[RED_BLOCK];
-    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double dsq = 0.0, cnt = 0.0;
-    if (m3 < a.nodenum) {
-        const int ynumy = a.g.ynumy;
-        const int m2 = (int)(m3 % ynumy), m1 = (int)(m3 / ynumy);
-        if (m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ynumy - 1) {
-            const v2c_params &p = a.p;
-            const double W = p.faultwidth;
-            const double nP = (p.pmode == 0 || a.timesum == 0) ? p.nstress : NOD(a, V2C_N_sppes)[m3];
-            const double sxxs = NOD(a, V2C_N_sxxs)[m3], sxy = NOD(a, V2C_N_sxy)[m3];
-            const double sigin = pow(sxxs * sxxs + sxy * sxy, 0.5);
-            double *Vs = NOD(a, V2C_N_RK_sliprs_solved);
-            const double as = NOD(a, V2C_N_as)[m3], bs = NOD(a, V2C_N_bs)[m3], Ls = NOD(a, V2C_N_Ls)[m3];
-            const double V0s = NOD(a, V2C_N_V0s)[m3], mus = NOD(a, V2C_N_mus_s)[m3];
-            const double phi = NOD(a, V2C_N_phis_it)[m3];
-            double Vout;
-            if (p.RSFmode == 1) {
-                if (sigin > 0) {
-                    double V;
-                    if (p.power_law_RSF > 0) {
-                        const double sy0 = nP * mus;
-                        V = V0s * pow(sigin / sy0, (mus / as)) * exp((-phi * bs) / as);
-                    } else {
-                        V = 2 * V0s * sinh(sigin / (as * nP)) * exp(-(mus + bs * phi) / as);
-                    }
-                    const double EPXX = NOD(a, V2C_N_exxpl)[m3], EXY = NOD(a, V2C_N_exy)[m3];
-                    const double EII = pow(EPXX * EPXX + EXY * EXY, 0.5);
-                    V = ref_minv(V, EII * W);
-                    if (V < 1e-40) V = 1e-40;
-                    Vout = V;
-                } else {
-                    Vout = 0;
-                }
-                Vs[m3] = Vout;
-                NOD(a, V2C_N_RK_phidts_solved)[m3] = V0s / Ls * exp(-phi) - Vout / Ls;
-            } else {
-                const double G = NOD(a, V2C_N_gg)[m3], NUVP = NOD(a, V2C_N_nu)[m3];
-                const double NU = NOD(a, V2C_N_nu_m2c)[m3];
-                const double E2EL = sigin / (NUVP / (G * dt + NUVP));
-                const double A = NOD(a, V2C_N_syields)[m3] / E2EL;
-                if (A < 1) {
-                    const double nup = G * dt * A / (1.0 - A);
-                    Vout = W * (sigin / 2.0 * (1.0 / (nup) - 1.0 / NU));
-                } else {
-                    Vout = 0;
-                }
-                Vs[m3] = Vout;
-            }
-            if (Vout > 0) {
-                const double d = NOD(a, V2C_N_syields)[m3] - sigin;
-                NOD(a, V2C_N_RK_deltas)[m3] = d;
-                dsq = pow(d, 2);
-                cnt = 1.0;
-            } else {
-                NOD(a, V2C_N_RK_deltas)[m3] = 0;
-            }
-        }
-    }
     ssum[threadIdx.x] = dsq;
     scnt[threadIdx.x] = cnt;
     __syncthreads();
@@ -1225,9 +1270,17 @@ __global__ __launch_bounds__(RED_BLOCK) void k_after(KArgs a, double dt, double 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        red[2 * blockIdx.x] = ssum[0];
-        red[2 * blockIdx.x + 1] = scnt[0];
+        red[2 * slot] = ssum[0];
+        red[2 * slot + 1] = scnt[0];
     }
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_after(KArgs a, double dt, double *red)
+{
+    const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dsq = 0.0, cnt = 0.0;
+    if (m3 < a.nodenum) after_one(a, m3, dt, dsq, cnt);
+    after_block_sum(dsq, cnt, red, blockIdx.x);
 }
 
 // Single-block fixed-order reduction of the per-block partials.

@@ -38,11 +38,13 @@ __device__ __forceinline__ short f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
 
+// tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2|x|), with the hardware
+// reciprocal (about 1 ulp) in place of an IEEE division.
 __device__ __forceinline__ float fast_tanh(float x)
 {
     const float ax = fabsf(x);
     const float e = __expf(-2.0f * ax);
-    const float t = (1.0f - e) / (1.0f + e);
+    const float t = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
     return copysignf(t, x);
 }
 
@@ -181,12 +183,8 @@ struct MlpInferLds {
     short h[MI_WAVES][64][MLP_H];
 };
 
-__device__ __forceinline__ bf16x4 lds_bf16x4(const short *p)
-{
-    bf16x4 v;
-    v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
-    return v;
-}
+// Four consecutive BF16 values (8-byte aligned) as one LDS read.
+__device__ __forceinline__ bf16x4 lds_bf16x4(const short *p) { return *(const bf16x4 *)p; }
 
 __global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_guess(KArgs a, double dt, const float *__restrict__ P,
                                                              const float *__restrict__ norm)
