@@ -396,6 +396,93 @@ class Context:
     def set_friction_fields(self, on):
         self._check(self.lib.v2c_set_friction_fields(self.h, int(on)), "set_friction_fields")
 
+class TileShape(C.Structure):
+    _fields_ = [("cells_per_group", C.c_int), ("markers_per_tile", C.c_int), ("nodes_per_tile", C.c_int)]
+
+
+ENS_CLASSES = 3
+
+
+def _bind_ens(lib):
+    if getattr(lib, "_ens_bound", False):
+        return
+    P = C.POINTER
+    lib.v2c_ens_create.argtypes = [P(C.c_void_p), P(C.c_void_p), C.c_int]
+    lib.v2c_ens_destroy.argtypes = [C.c_void_p]
+    lib.v2c_ens_destroy.restype = None
+    lib.v2c_ens_last_error.argtypes = [C.c_void_p]
+    lib.v2c_ens_last_error.restype = C.c_char_p
+    for name, args in [("v2c_ens_size_class", [C.c_int]), ("v2c_ens_set_shape", [C.c_int, P(TileShape)]),
+                       ("v2c_ens_get_shape", [C.c_int, P(TileShape)]), ("v2c_ens_set_blocks_per_cu", [C.c_int]),
+                       ("v2c_ens_step", [C.c_int]), ("v2c_ens_step_looped", []), ("v2c_ens_rms", [P(C.c_double)]),
+                       ("v2c_ens_autotune", [C.c_int, P(C.c_double)])]:
+        fn = getattr(lib, name)
+        fn.argtypes = [C.c_void_p] + args
+        fn.restype = C.c_int
+    lib._ens_bound = True
+
+
+class Ensemble:
+    """Batched ensemble of independent contexts (v2c_ens_* in the C-ABI)."""
+
+    def __init__(self, contexts):
+        self.ctxs = list(contexts)
+        self.lib = self.ctxs[0].lib
+        _bind_ens(self.lib)
+        arr = (C.c_void_p * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = C.c_void_p()
+        rc = self.lib.v2c_ens_create(C.byref(h), arr, len(self.ctxs))
+        if rc != 0:
+            raise V2CError(f"v2c_ens_create: {ERRORS.get(rc, rc)}")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise V2CError(f"{what}: {ERRORS.get(rc, rc)}: {self.lib.v2c_ens_last_error(self.h).decode()}")
+
+    def step(self, graph=True):
+        self._check(self.lib.v2c_ens_step(self.h, int(graph)), "ens_step")
+
+    def step_looped(self):
+        self._check(self.lib.v2c_ens_step_looped(self.h), "ens_step_looped")
+
+    def size_class(self, k):
+        return self.lib.v2c_ens_size_class(self.h, k)
+
+    def set_shape(self, cls, cells_per_group, markers_per_tile, nodes_per_tile):
+        s = TileShape(cells_per_group, markers_per_tile, nodes_per_tile)
+        self._check(self.lib.v2c_ens_set_shape(self.h, cls, C.byref(s)), "ens_set_shape")
+
+    def get_shape(self, cls):
+        s = TileShape()
+        self._check(self.lib.v2c_ens_get_shape(self.h, cls, C.byref(s)), "ens_get_shape")
+        return (s.cells_per_group, s.markers_per_tile, s.nodes_per_tile)
+
+    def set_blocks_per_cu(self, b):
+        self._check(self.lib.v2c_ens_set_blocks_per_cu(self.h, b), "ens_set_blocks_per_cu")
+
+    def rms(self):
+        out = np.zeros(2 * len(self.ctxs))
+        self._check(self.lib.v2c_ens_rms(self.h, _dptr(out)), "ens_rms")
+        return out.reshape(-1, 2)
+
+    def autotune(self, reps=3):
+        ms = C.c_double(0)
+        self._check(self.lib.v2c_ens_autotune(self.h, reps, C.byref(ms)), "ens_autotune")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            self.lib.v2c_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _opt(a):
     return None if a is None else _dptr(a)
 

@@ -136,29 +136,34 @@ static int collect_timing(v2c_ctx *c)
     return 0;
 }
 
+// Device pointers into the argument struct (its pointer members are typed as
+// global-memory pointers in the device pass).
+template <class D, class S>
+static inline void setp(D &dst, S *src) { dst = (D)src; }
+
 static KArgs make_args(v2c_ctx *c)
 {
     KArgs a;
-    a.g.gx = c->d_gx; a.g.gy = c->d_gy;
+    setp(a.g.gx, c->d_gx); setp(a.g.gy, c->d_gy);
     a.g.xnumx = (int)c->xnumx; a.g.ynumy = (int)c->ynumy;
     a.g.xsize = c->xsize; a.g.ysize = c->ysize;
-    a.g.bx = c->d_bx; a.g.by = c->d_by; a.g.cellx = c->d_cellx; a.g.celly = c->d_celly;
+    setp(a.g.bx, c->d_bx); setp(a.g.by, c->d_by); setp(a.g.cellx, c->d_cellx); setp(a.g.celly, c->d_celly);
     a.g.bx0 = c->bx0; a.g.by0 = c->by0; a.g.bxinv = c->bxinv; a.g.byinv = c->byinv;
     a.g.nbx = c->nbx; a.g.nby = c->nby;
     a.g.xr_lo = c->xr_lo; a.g.xr_hi = c->xr_hi; a.g.yr_lo = c->yr_lo; a.g.yr_hi = c->yr_hi;
     a.p = c->prm;
-    a.mat = c->d_mat;
-    a.mk = c->d_mk; a.cap = c->cap; a.t = c->d_t; a.n = c->n;
-    a.order = c->d_order; a.skey = c->d_skey; a.sub_start = c->d_cs; a.key = c->d_key;
-    a.nod = c->d_nod; a.nodenum = c->nodenum; a.ncell = c->ncell;
-    a.part = c->d_part;
-    a.prop = c->d_prop;
+    setp(a.mat, c->d_mat);
+    setp(a.mk, c->d_mk); a.cap = c->cap; setp(a.t, c->d_t); a.n = c->n;
+    setp(a.order, c->d_order); setp(a.skey, c->d_skey); setp(a.sub_start, c->d_cs); setp(a.key, c->d_key);
+    setp(a.nod, c->d_nod); a.nodenum = c->nodenum; a.ncell = c->ncell;
+    setp(a.part, c->d_part);
+    setp(a.prop, c->d_prop);
     a.timesum = c->timesum; a.timestep = c->timestep;
-    a.counters = c->d_counters;
+    setp(a.counters, c->d_counters);
     a.write_friction = c->write_friction;
-    a.mx = c->d_mx;
-    a.fault = c->d_fault;
-    a.lu_stats = c->d_lu;
+    setp(a.mx, c->d_mx);
+    setp(a.fault, c->d_fault);
+    setp(a.lu_stats, c->d_lu);
     return a;
 }
 
@@ -1127,3 +1132,5 @@ int v2c_kernel_time(v2c_ctx *c, int kernel, double *ms, long *launches)
 }
 
 }  // extern "C"
+
+#include "v2c_ensemble.hip"

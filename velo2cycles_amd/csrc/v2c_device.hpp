@@ -14,16 +14,6 @@
 
 namespace v2c {
 
-// Element i of a wave-uniform array through a 32-bit byte offset: the load
-// takes the scalar base and one vector offset register instead of a 64-bit
-// vector address per field.  Arrays indexed this way hold < 2^29 elements
-// per field (checked when a context is created).
-template <class T>
-__device__ __forceinline__ T &at32(T *base, uint32_t i)
-{
-    return *(T *)((char *)base + (size_t)(i * (uint32_t)sizeof(T)));
-}
-
 // Marker SoA field slots (same order as the v2c_markers struct).
 enum MF {
     MF_X = 0, MF_Y, MF_K, MF_P, MF_XX, MF_XY, MF_EXX, MF_EXY, MF_E, MF_PHI, MF_VX, MF_VY,
@@ -50,13 +40,24 @@ struct DevMat {
     int im[V2C_MAX_TYPES];
 };
 
+// Every pointer in the argument structs points to global memory.  Saying so
+// in the type (device pass only; the host sees plain pointers of the same
+// size) keeps global loads with a scalar base even when a struct is itself
+// read from memory, as the batched ensemble launches do; through a plain
+// pointer loaded from memory the compiler would have to emit flat loads.
+#ifdef __HIP_DEVICE_COMPILE__
+#define V2C_GLOBAL __attribute__((address_space(1)))
+#else
+#define V2C_GLOBAL
+#endif
+
 struct DevGrid {
-    const double *gx, *gy;
+    V2C_GLOBAL const double *gx, *gy;
     int xnumx, ynumy;
     double xsize, ysize;
     // search tables: bins of width h <= the smallest spacing from the first
     // gridline (bin j = (x - b0) * binv) and per-cell records
-    const Axis4 *bx, *by, *cellx, *celly;
+    V2C_GLOBAL const Axis4 *bx, *by, *cellx, *celly;
     double bx0, by0, bxinv, byinv;
     int nbx, nby;
     // allinteri's outer half-cell bounds (interpolation.c:1701-1705):
@@ -68,37 +69,31 @@ struct DevGrid {
 struct KArgs {
     DevGrid g;
     v2c_params p;
-    const DevMat *mat;
-    double *mk;            // NMF fields x cap
+    V2C_GLOBAL const DevMat *mat;
+    V2C_GLOBAL double *mk; // NMF fields x cap
     long cap;              // field stride
-    int8_t *t;             // marker types
+    V2C_GLOBAL int8_t *t;  // marker types
     long n;                // marker count
-    const uint32_t *order; // sorted position -> marker id
-    const uint32_t *skey;  // sorted position -> sub-cell key
-    const uint32_t *sub_start;  // sub-cell key -> first sorted position (4*ncell+1)
-    uint32_t *key;         // per marker (physical) sub-cell key
-    double *nod;           // V2C_NF fields x nodenum
+    V2C_GLOBAL const uint32_t *order;      // sorted position -> marker id
+    V2C_GLOBAL const uint32_t *skey;       // sorted position -> sub-cell key
+    V2C_GLOBAL const uint32_t *sub_start;  // sub-cell key -> first sorted position (4*ncell+1)
+    V2C_GLOBAL uint32_t *key;              // per marker (physical) sub-cell key
+    V2C_GLOBAL double *nod;                // V2C_NF fields x nodenum
     long nodenum;
     long ncell;
-    double *part;          // per-sub-cell sum records: [(cell*4 + quadrant)*nfields + field]
-    double *prop;          // per-marker properties of marker2nodes (NPROP x cap)
+    V2C_GLOBAL double *part;               // node records of the scatter passes
+    V2C_GLOBAL double *prop;               // per-marker properties of marker2nodes (NPROP x cap)
     double timesum, timestep;
-    int *counters;         // device counters (errors, unsupported types)
+    V2C_GLOBAL int *counters;              // device counters (errors, unsupported types)
     int write_friction;    // write the calc_frictionpara marker outputs
-    double *mx;            // NMX extension fields x cap
-    const uint8_t *fault;  // per-marker fault flag
-    unsigned long long *lu_stats;  // local-update statistics
+    V2C_GLOBAL double *mx;                 // NMX extension fields x cap
+    V2C_GLOBAL const uint8_t *fault;       // per-marker fault flag
+    V2C_GLOBAL unsigned long long *lu_stats;  // local-update statistics
 };
 
 __device__ __forceinline__ double *NOD(const KArgs &a, int f) { return a.nod + (long)f * a.nodenum; }
-__device__ __forceinline__ double &MK(const KArgs &a, int f, long i)
-{
-    return at32(a.mk + (long)f * a.cap, (uint32_t)i);
-}
-__device__ __forceinline__ double &MXF(const KArgs &a, int f, long i)
-{
-    return at32(a.mx + (long)f * a.cap, (uint32_t)i);
-}
+__device__ __forceinline__ double &MK(const KArgs &a, int f, long i) { return a.mk[(long)f * a.cap + i]; }
+__device__ __forceinline__ double &MXF(const KArgs &a, int f, long i) { return a.mx[(long)f * a.cap + i]; }
 
 // ---- small helpers ---------------------------------------------------------
 __device__ __forceinline__ double ref_abs(double a) { return a >= 0 ? a : -a; }
@@ -119,8 +114,7 @@ __device__ __forceinline__ Axis4 find_y(const DevGrid &g, double y)
     return axis_find(g.gy, g.ynumy, g.by, g.celly, g.nby, g.by0, g.byinv, y);
 }
 
-// m3: nodal index; uint32 so that gathers address a wave-uniform field base
-// plus a 32-bit byte offset (nodenum < 2^29, checked at create).
+// m3: nodal index (nodenum < 2^29, checked at create).
 struct Stencil { uint32_t m3; double e, n; };
 
 
@@ -299,10 +293,12 @@ __device__ __forceinline__ W4 wts(const Stencil &s)
     w.d3 = s.e * s.n;
     return w;
 }
+// 64-bit index arithmetic, so that the compiler merges each adjacent pair
+// (m3, m3+1) into one 16-byte load.
 __device__ __forceinline__ double bil(const double *__restrict__ f, uint32_t m3, int ny, const W4 &w)
 {
-    const uint32_t u = (uint32_t)ny;
-    return w.d0 * at32(f, m3) + w.d1 * at32(f, m3 + 1) + w.d2 * at32(f, m3 + u) + w.d3 * at32(f, m3 + u + 1);
+    const long i = m3;
+    return w.d0 * f[i] + w.d1 * f[i + 1] + w.d2 * f[i + ny] + w.d3 * f[i + ny + 1];
 }
 
 // Sub-cell sort key of a position (after clamping to the domain): cell in
@@ -538,15 +534,12 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
     s = centre_stencil(g, X, Y, x, y);
     w = wts(s);
     double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
-    // pressure-node averages of vx and vy (interpolation.c:1725-1736); the
-    // centre stencil's m3 >= ynumy + 1, so every index below is >= 0
-    const uint32_t m3 = s.m3, u = (uint32_t)ynumy;
-    double vxa = (w.d0 * (at32(vx, m3 - 1) + at32(vx, m3 - u - 1)) + w.d1 * (at32(vx, m3) + at32(vx, m3 - u))
-                  + w.d2 * (at32(vx, m3 + u - 1) + at32(vx, m3 - 1)) + w.d3 * (at32(vx, m3 + u) + at32(vx, m3)))
-                 * 0.5 * (1.0 - xrat);
-    double vya = (w.d0 * (at32(vy, m3 - u) + at32(vy, m3 - u - 1)) + w.d1 * (at32(vy, m3 - u + 1) + at32(vy, m3 - u))
-                  + w.d2 * (at32(vy, m3) + at32(vy, m3 - 1)) + w.d3 * (at32(vy, m3 + 1) + at32(vy, m3)))
-                 * 0.5 * (1.0 - xrat);
+    // pressure-node averages of vx and vy (interpolation.c:1725-1736)
+    const long m3 = s.m3;
+    double vxa = (w.d0 * (vx[m3 - 1] + vx[m3 - ynumy - 1]) + w.d1 * (vx[m3] + vx[m3 - ynumy])
+                  + w.d2 * (vx[m3 + ynumy - 1] + vx[m3 - 1]) + w.d3 * (vx[m3 + ynumy] + vx[m3])) * 0.5 * (1.0 - xrat);
+    double vya = (w.d0 * (vy[m3 - ynumy] + vy[m3 - ynumy - 1]) + w.d1 * (vy[m3 - ynumy + 1] + vy[m3 - ynumy])
+                  + w.d2 * (vy[m3] + vy[m3 - 1]) + w.d3 * (vy[m3 + 1] + vy[m3])) * 0.5 * (1.0 - xrat);
     EE = sqrt(EXX * EXX + EXY * EXY);
 
     s = vx_stencil(g, X, Y, x, y);

@@ -1,0 +1,571 @@
+// v2c_ensemble.hip -- batched ensembles of independent models.
+//
+// Every model is an ordinary context (its own grid, marker cloud and nodal
+// state).  A step of the whole ensemble runs each stage of the per-model step
+// (marker2nodes, compute_before_solve, compute_after_solve, nodes2markers +
+// advection) as ONE launch over a work list of (model, tile) entries: a
+// persistent grid strides over the list, and every tile calls the same device
+// function the single-model kernel calls, on that model's arguments.  The
+// stage sequence is captured once into a HIP graph and replayed each step;
+// only the cell ordering (which decides between an incremental merge and a
+// full sort on the host) and the argument upload run outside the graph.
+//
+// Tile shapes (cells per scatter group, markers and nodes per tile) are chosen
+// per model size class; v2c_ens_autotune times candidates on the ensemble
+// itself and keeps the fastest.  Results do not depend on the shapes: every
+// sub-cell sum is the same left-to-right sum over its sorted markers whatever
+// the batch and group boundaries, so a batched step equals the per-model
+// steps bit for bit.
+// Included at the end of v2c_api.hip: it uses the context internals.
+
+namespace v2c {
+
+// The model's arguments, read from global memory (their pointer members are
+// typed as global pointers, so loads through them stay global loads).
+__device__ __forceinline__ const KArgs &model_args(const KArgs *__restrict__ A, int mod) { return A[mod]; }
+
+struct EnsTile {
+    int model;
+    int pad;
+    long begin, end;            // item range: markers, cells or nodes
+};
+
+constexpr int ENS_BLOCK = 256;
+
+template <class F>
+__device__ __forceinline__ void ens_items(const KArgs *__restrict__ A, const EnsTile *__restrict__ tiles, int ntiles,
+                                          F &&f)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const EnsTile T = tiles[t];
+        const KArgs a = model_args(A, T.model);
+        for (long i = T.begin + threadIdx.x; i < T.end; i += blockDim.x) f(a, i);
+    }
+}
+
+__global__ __launch_bounds__(ENS_BLOCK) void k_ens_m2n_props(const KArgs *__restrict__ A,
+                                                             const EnsTile *__restrict__ tiles, int ntiles)
+{
+    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m) { m2n_props_one(a, m); });
+}
+
+__global__ __launch_bounds__(WAVE) void k_ens_m2n_cells(const KArgs *__restrict__ A,
+                                                        const EnsTile *__restrict__ tiles, int ntiles)
+{
+    __shared__ ScatterLds<M_NF> L;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const EnsTile T = tiles[t];
+        m2n_cells_group(model_args(A, T.model), T.begin, T.end, threadIdx.x, L);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(ENS_BLOCK) void k_ens_m2n_nodes(const KArgs *__restrict__ A,
+                                                             const EnsTile *__restrict__ tiles, int ntiles)
+{
+    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m3) { m2n_nodes_one(a, m3); });
+}
+
+__global__ __launch_bounds__(ENS_BLOCK) void k_ens_phis_bc(const KArgs *__restrict__ A,
+                                                           const EnsTile *__restrict__ tiles, int ntiles)
+{
+    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m3) {
+        if (a.p.Dphis > 0) phis_bc_one(a, m3);
+    });
+}
+
+__global__ __launch_bounds__(ENS_BLOCK) void k_ens_before(const KArgs *__restrict__ A,
+                                                          const EnsTile *__restrict__ tiles, int ntiles)
+{
+    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m3) { before_one(a, m3, a.timestep, 0); });
+}
+
+// Node tiles are whole RED_BLOCK chunks of one model, so each chunk's
+// fixed-order sum lands in the model's partial slot begin / RED_BLOCK, as in
+// the single-model launch.
+__global__ __launch_bounds__(RED_BLOCK) void k_ens_after(const KArgs *__restrict__ A, double *const *__restrict__ red,
+                                                         const EnsTile *__restrict__ tiles, int ntiles)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const EnsTile T = tiles[t];
+        const KArgs a = model_args(A, T.model);
+        for (long c = T.begin; c < T.end; c += RED_BLOCK) {
+            const long m3 = c + threadIdx.x;
+            double dsq = 0.0, cnt = 0.0;
+            if (m3 < T.end) after_one(a, m3, a.timestep, dsq, cnt);
+            after_block_sum(dsq, cnt, red[T.model], c / RED_BLOCK);
+            __syncthreads();
+        }
+    }
+}
+
+// One workgroup per model: fixed-order sum of the model's partials.
+__global__ __launch_bounds__(RED_BLOCK) void k_ens_after_final(double *const *__restrict__ red,
+                                                               const int *__restrict__ nblocks, double *out)
+{
+    __shared__ double ssum[RED_BLOCK];
+    __shared__ double scnt[RED_BLOCK];
+    const int mod = blockIdx.x;
+    const double *r = red[mod];
+    double s = 0, c = 0;
+    for (int b = threadIdx.x; b < nblocks[mod]; b += RED_BLOCK) { s += r[2 * b]; c += r[2 * b + 1]; }
+    ssum[threadIdx.x] = s;
+    scnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int k = RED_BLOCK / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) { ssum[threadIdx.x] += ssum[threadIdx.x + k]; scnt[threadIdx.x] += scnt[threadIdx.x + k]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * mod] = scnt[0] > 0 ? pow(ssum[0] / scnt[0], 0.5) : 0.0;
+        out[2 * mod + 1] = scnt[0];
+    }
+}
+
+__global__ void k_ens_phantom(const KArgs *__restrict__ A, int nmodels)
+{
+    const int mod = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mod < nmodels && A[mod].p.antidiffinter == 1 && A[mod].p.n2m_phantom) n2m_phantom_one(model_args(A, mod));
+}
+
+__global__ __launch_bounds__(WAVE) void k_ens_n2m_scatter(const KArgs *__restrict__ A,
+                                                          const EnsTile *__restrict__ tiles, int ntiles)
+{
+    __shared__ ScatterLds<N_NF> L;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const EnsTile T = tiles[t];
+        const KArgs a = model_args(A, T.model);
+        if (a.p.antidiffinter == 1) n2m_scatter_group(a, T.begin, T.end, threadIdx.x, L);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(ENS_BLOCK) void k_ens_n2m_nodes(const KArgs *__restrict__ A,
+                                                             const EnsTile *__restrict__ tiles, int ntiles)
+{
+    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m3) {
+        if (a.p.antidiffinter == 1) n2m_nodes_one(a, m3);
+    });
+}
+
+__global__ __launch_bounds__(ENS_BLOCK) void k_ens_n2m_update(
+    const KArgs *__restrict__ A, const EnsTile *__restrict__ tiles, int ntiles, int *unsupported)
+{
+    int bad = 0;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const EnsTile T = tiles[t];
+        const KArgs a = model_args(A, T.model);
+        for (long m = T.begin + threadIdx.x; m < T.end; m += blockDim.x) bad += n2m_update_one<true>(a, m) ? 1 : 0;
+    }
+    if (bad) atomicAdd(unsupported, bad);
+}
+
+__global__ void k_ens_clear(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace v2c
+
+using namespace v2c;
+
+// Work lists of one tile shape assignment.
+struct EnsPlan {
+    std::vector<EnsTile> markers, cells, nodes, after;
+};
+
+struct v2c_ens {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<v2c_ctx *> models;
+    std::vector<int> size_class;
+    v2c_tile_shape shape[V2C_ENS_CLASSES];
+    int blocks_per_cu = 8, ncu = 256;
+    // device buffers
+    KArgs *d_args = nullptr;
+    KArgs *h_args = nullptr;                 // pinned staging
+    EnsTile *d_tiles = nullptr;
+    long tile_cap = 0;
+    int n_mark = 0, n_cell = 0, n_node = 0, n_after = 0;
+    EnsTile *t_mark = nullptr, *t_cell = nullptr, *t_node = nullptr, *t_after = nullptr;
+    double **d_red = nullptr;
+    int *d_redn = nullptr;
+    double *d_rms = nullptr;
+    int *d_flags = nullptr;                  // [0] unsupported marker types
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_valid = false;
+    char err[512] = {0};
+};
+
+static int ens_err(v2c_ens *e, int code, const char *fmt, ...)
+{
+    if (e) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(e->err, sizeof e->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define ENSCHK(e, expr)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ens_err((e), V2C_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+static int ens_class_of(const v2c_ctx *c)
+{
+    if (c->n < 500000) return 0;
+    if (c->n < 2000000) return 1;
+    return 2;
+}
+
+static void ens_default_shape(v2c_tile_shape *s)
+{
+    s->cells_per_group = CPW;
+    s->markers_per_tile = 1024;
+    s->nodes_per_tile = 1024;
+}
+
+static EnsPlan ens_plan(const v2c_ens *e)
+{
+    EnsPlan p;
+    for (size_t k = 0; k < e->models.size(); k++) {
+        const v2c_ctx *c = e->models[k];
+        const v2c_tile_shape &s = e->shape[e->size_class[k]];
+        const int mod = (int)k;
+        for (long b = 0; b < c->n; b += s.markers_per_tile)
+            p.markers.push_back(EnsTile{mod, 0, b, std::min(c->n, b + (long)s.markers_per_tile)});
+        for (long b = 0; b < c->ncell; b += s.cells_per_group)
+            p.cells.push_back(EnsTile{mod, 0, b, std::min(c->ncell, b + (long)s.cells_per_group)});
+        for (long b = 0; b < c->nodenum; b += s.nodes_per_tile)
+            p.nodes.push_back(EnsTile{mod, 0, b, std::min(c->nodenum, b + (long)s.nodes_per_tile)});
+        const long rt = ((long)s.nodes_per_tile / RED_BLOCK) * RED_BLOCK;
+        for (long b = 0; b < c->nodenum; b += rt)
+            p.after.push_back(EnsTile{mod, 0, b, std::min(c->nodenum, b + rt)});
+    }
+    return p;
+}
+
+static int ens_upload_plan(v2c_ens *e)
+{
+    const EnsPlan p = ens_plan(e);
+    const long total = (long)(p.markers.size() + p.cells.size() + p.nodes.size() + p.after.size());
+    if (total > e->tile_cap) {
+        if (e->d_tiles) (void)hipFree(e->d_tiles);
+        e->d_tiles = nullptr;
+        ENSCHK(e, hipMalloc(&e->d_tiles, sizeof(EnsTile) * total));
+        e->tile_cap = total;
+    }
+    std::vector<EnsTile> all;
+    all.reserve(total);
+    all.insert(all.end(), p.markers.begin(), p.markers.end());
+    all.insert(all.end(), p.cells.begin(), p.cells.end());
+    all.insert(all.end(), p.nodes.begin(), p.nodes.end());
+    all.insert(all.end(), p.after.begin(), p.after.end());
+    ENSCHK(e, hipMemcpyAsync(e->d_tiles, all.data(), sizeof(EnsTile) * total, hipMemcpyHostToDevice, e->stream));
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    e->n_mark = (int)p.markers.size();
+    e->n_cell = (int)p.cells.size();
+    e->n_node = (int)p.nodes.size();
+    e->n_after = (int)p.after.size();
+    e->t_mark = e->d_tiles;
+    e->t_cell = e->t_mark + e->n_mark;
+    e->t_node = e->t_cell + e->n_cell;
+    e->t_after = e->t_node + e->n_node;
+    e->graph_valid = false;
+    return 0;
+}
+
+static unsigned ens_grid(const v2c_ens *e, int ntiles, int waves_per_block)
+{
+    const long g = (long)e->ncu * e->blocks_per_cu * (waves_per_block >= 4 ? 1 : 4);
+    return (unsigned)std::max(1L, std::min((long)ntiles, g));
+}
+
+// The graph-captured stage sequence of one step.
+static void ens_stages(v2c_ens *e)
+{
+    const int nm = (int)e->models.size();
+    hipLaunchKernelGGL(k_ens_clear, dim3(1), dim3(64), 0, e->stream, e->d_flags, 4);
+    hipLaunchKernelGGL(k_ens_m2n_props, dim3(ens_grid(e, e->n_mark, 4)), dim3(ENS_BLOCK), 0, e->stream, e->d_args,
+                       e->t_mark, e->n_mark);
+    hipLaunchKernelGGL(k_ens_m2n_cells, dim3(ens_grid(e, e->n_cell, 1)), dim3(WAVE), 0, e->stream, e->d_args,
+                       e->t_cell, e->n_cell);
+    hipLaunchKernelGGL(k_ens_m2n_nodes, dim3(ens_grid(e, e->n_node, 4)), dim3(ENS_BLOCK), 0, e->stream,
+                       e->d_args, e->t_node, e->n_node);
+    bool dphis = false;
+    for (auto *c : e->models) dphis = dphis || c->prm.Dphis > 0;
+    if (dphis)
+        hipLaunchKernelGGL(k_ens_phis_bc, dim3(ens_grid(e, e->n_node, 4)), dim3(ENS_BLOCK), 0, e->stream,
+                           e->d_args, e->t_node, e->n_node);
+    hipLaunchKernelGGL(k_ens_before, dim3(ens_grid(e, e->n_node, 4)), dim3(ENS_BLOCK), 0, e->stream, e->d_args,
+                       e->t_node, e->n_node);
+    hipLaunchKernelGGL(k_ens_after, dim3(ens_grid(e, e->n_after, 4)), dim3(RED_BLOCK), 0, e->stream, e->d_args,
+                       e->d_red, e->t_after, e->n_after);
+    hipLaunchKernelGGL(k_ens_after_final, dim3(nm), dim3(RED_BLOCK), 0, e->stream, e->d_red, e->d_redn, e->d_rms);
+    hipLaunchKernelGGL(k_ens_phantom, dim3((nm + 63) / 64), dim3(64), 0, e->stream, e->d_args, nm);
+    hipLaunchKernelGGL(k_ens_n2m_scatter, dim3(ens_grid(e, e->n_cell, 1)), dim3(WAVE), 0, e->stream, e->d_args,
+                       e->t_cell, e->n_cell);
+    hipLaunchKernelGGL(k_ens_n2m_nodes, dim3(ens_grid(e, e->n_node, 4)), dim3(ENS_BLOCK), 0, e->stream,
+                       e->d_args, e->t_node, e->n_node);
+    hipLaunchKernelGGL(k_ens_n2m_update, dim3(ens_grid(e, e->n_mark, 4)), dim3(ENS_BLOCK), 0, e->stream,
+                       e->d_args, e->t_mark, e->n_mark, e->d_flags);
+}
+
+// Per-step preparation outside the graph: cell ordering of every model (it
+// may choose a full sort on the host) and the argument array.
+static int ens_prepare(v2c_ens *e)
+{
+    for (size_t k = 0; k < e->models.size(); k++) {
+        v2c_ctx *c = e->models[k];
+        int rc = ensure_order(c);
+        if (rc) return ens_err(e, rc, "model %zu: ordering failed: %s", k, c->err);
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        e->h_args[k] = make_args(c);
+    }
+    ENSCHK(e, hipMemcpyAsync(e->d_args, e->h_args, sizeof(KArgs) * e->models.size(), hipMemcpyHostToDevice,
+                             e->stream));
+    return 0;
+}
+
+static int ens_finish_step(v2c_ens *e)
+{
+    for (auto *c : e->models) {
+        c->order_valid = false;            // markers moved
+        c->timesum += c->timestep;
+    }
+    return 0;
+}
+
+extern "C" {
+
+int v2c_ens_create(v2c_ens **out, v2c_ctx *const *models, int nmodels)
+{
+    if (!out || !models || nmodels <= 0) return V2C_EINVAL;
+    *out = nullptr;
+    const int dev = models[0]->device;
+    for (int k = 0; k < nmodels; k++)
+        if (!models[k] || models[k]->device != dev) return V2C_EINVAL;
+    v2c_ens *e = new v2c_ens();
+    e->device = dev;
+    for (int k = 0; k < nmodels; k++) {
+        e->models.push_back(models[k]);
+        e->size_class.push_back(ens_class_of(models[k]));
+    }
+    for (int s = 0; s < V2C_ENS_CLASSES; s++) ens_default_shape(&e->shape[s]);
+    auto fail = [&](int code) {
+        v2c_ens_destroy(e);
+        return code;
+    };
+    if (hipSetDevice(dev) != hipSuccess) return fail(V2C_EHIP);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) e->ncu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(V2C_EHIP);
+    if (hipMalloc(&e->d_args, sizeof(KArgs) * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
+    if (hipHostMalloc((void **)&e->h_args, sizeof(KArgs) * nmodels, hipHostMallocDefault) != hipSuccess)
+        return fail(V2C_ENOMEM);
+    std::vector<double *> reds(nmodels);
+    std::vector<int> nred(nmodels);
+    for (int k = 0; k < nmodels; k++) {
+        reds[k] = models[k]->d_red;
+        nred[k] = (int)blocks_for(models[k]->nodenum, RED_BLOCK);
+    }
+    if (hipMalloc(&e->d_red, sizeof(double *) * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
+    if (hipMalloc(&e->d_redn, sizeof(int) * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
+    if (hipMalloc(&e->d_rms, sizeof(double) * 2 * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
+    if (hipMalloc(&e->d_flags, sizeof(int) * 4) != hipSuccess) return fail(V2C_ENOMEM);
+    if (hipMemcpy(e->d_red, reds.data(), sizeof(double *) * nmodels, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->d_redn, nred.data(), sizeof(int) * nmodels, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(V2C_EHIP);
+    if (ens_upload_plan(e)) return fail(V2C_EHIP);
+    *out = e;
+    return 0;
+}
+
+void v2c_ens_destroy(v2c_ens *e)
+{
+    if (!e) return;
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    void *ptrs[] = {e->d_args, e->d_tiles, e->d_red, e->d_redn, e->d_rms, e->d_flags};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (e->h_args) (void)hipHostFree(e->h_args);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char *v2c_ens_last_error(const v2c_ens *e) { return e ? e->err : "null ensemble"; }
+
+int v2c_ens_size_class(const v2c_ens *e, int model)
+{
+    if (!e || model < 0 || model >= (int)e->models.size()) return V2C_EINVAL;
+    return e->size_class[model];
+}
+
+int v2c_ens_set_shape(v2c_ens *e, int size_class, const v2c_tile_shape *s)
+{
+    if (!e || !s || size_class < 0 || size_class >= V2C_ENS_CLASSES) return V2C_EINVAL;
+    if (s->cells_per_group < 1 || s->cells_per_group > 1024 || s->markers_per_tile < 64 ||
+        s->nodes_per_tile < RED_BLOCK || s->nodes_per_tile % RED_BLOCK)
+        return ens_err(e, V2C_EINVAL, "tile shape out of range");
+    e->shape[size_class] = *s;
+    ENSCHK(e, hipSetDevice(e->device));
+    return ens_upload_plan(e);
+}
+
+int v2c_ens_get_shape(const v2c_ens *e, int size_class, v2c_tile_shape *s)
+{
+    if (!e || !s || size_class < 0 || size_class >= V2C_ENS_CLASSES) return V2C_EINVAL;
+    *s = e->shape[size_class];
+    return 0;
+}
+
+int v2c_ens_set_blocks_per_cu(v2c_ens *e, int blocks)
+{
+    if (!e || blocks < 1 || blocks > 64) return V2C_EINVAL;
+    e->blocks_per_cu = blocks;
+    e->graph_valid = false;
+    return 0;
+}
+
+int v2c_ens_step(v2c_ens *e, int use_graph)
+{
+    if (!e) return V2C_EINVAL;
+    ENSCHK(e, hipSetDevice(e->device));
+    int rc = ens_prepare(e);
+    if (rc) return rc;
+    if (!use_graph) {
+        ens_stages(e);
+        ENSCHK(e, hipGetLastError());
+    } else {
+        if (!e->graph_valid) {
+            if (e->exec) (void)hipGraphExecDestroy(e->exec);
+            if (e->graph) (void)hipGraphDestroy(e->graph);
+            e->exec = nullptr;
+            e->graph = nullptr;
+            ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            ens_stages(e);
+            hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
+            if (ce != hipSuccess) return ens_err(e, V2C_EHIP, "graph capture failed: %s", hipGetErrorString(ce));
+            ENSCHK(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+            e->graph_valid = true;
+        }
+        ENSCHK(e, hipGraphLaunch(e->exec, e->stream));
+    }
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    int bad = 0;
+    ENSCHK(e, hipMemcpy(&bad, e->d_flags, sizeof(int), hipMemcpyDeviceToHost));
+    ens_finish_step(e);
+    if (bad) return ens_err(e, V2C_EUNSUPPORTED, "%d markers of type >= 100 (immobile spawning is out of scope)", bad);
+    return 0;
+}
+
+// The same step as per-model launches of the single-model entry points.
+int v2c_ens_step_looped(v2c_ens *e)
+{
+    if (!e) return V2C_EINVAL;
+    for (size_t k = 0; k < e->models.size(); k++) {
+        v2c_ctx *c = e->models[k];
+        const int was_async = c->async_mode;
+        c->async_mode = 1;
+        const double dt = c->timestep;
+        int rc = v2c_marker2nodes(c);
+        if (!rc) rc = v2c_compute_before_solve(c, dt, 0, nullptr);
+        if (!rc) rc = v2c_compute_after_solve(c, dt, 0, nullptr, nullptr);
+        if (!rc) rc = v2c_nodes2markers_advect(c);
+        c->async_mode = was_async;
+        if (rc) return ens_err(e, rc, "model %zu: %s", k, c->err);
+    }
+    for (auto *c : e->models) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->timesum += c->timestep;
+    }
+    return 0;
+}
+
+int v2c_ens_rms(v2c_ens *e, double *rms_count)
+{
+    if (!e || !rms_count) return V2C_EINVAL;
+    ENSCHK(e, hipMemcpy(rms_count, e->d_rms, sizeof(double) * 2 * e->models.size(), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Coordinate descent over the tile shapes of each size class present, then
+// the persistent grid size: each candidate is timed over `reps` graph-replayed
+// steps (after one untimed step) and the fastest kept.  The ensemble's state
+// advances by the steps run.
+int v2c_ens_autotune(v2c_ens *e, int reps, double *best_ms)
+{
+    if (!e || reps < 1) return V2C_EINVAL;
+    hipEvent_t t0, t1;
+    ENSCHK(e, hipEventCreate(&t0));
+    ENSCHK(e, hipEventCreate(&t1));
+    auto measure = [&](double &ms) -> int {
+        int rc = v2c_ens_step(e, 1);
+        if (rc) return rc;
+        double tot = 0;
+        for (int r = 0; r < reps; r++) {
+            rc = ens_prepare(e);
+            if (rc) return rc;
+            ENSCHK(e, hipEventRecord(t0, e->stream));
+            ENSCHK(e, hipGraphLaunch(e->exec, e->stream));
+            ENSCHK(e, hipEventRecord(t1, e->stream));
+            ENSCHK(e, hipEventSynchronize(t1));
+            float m = 0;
+            ENSCHK(e, hipEventElapsedTime(&m, t0, t1));
+            tot += m;
+            ens_finish_step(e);
+        }
+        ms = tot / reps;
+        return 0;
+    };
+    bool present[V2C_ENS_CLASSES] = {false};
+    for (int c : e->size_class) present[c] = true;
+    double best = 0;
+    int rc = measure(best);
+    const int cpg[] = {8, 16, 32, 64};
+    const int mpt[] = {256, 1024, 4096};
+    const int npt[] = {256, 1024, 4096};
+    for (int cls = 0; cls < V2C_ENS_CLASSES && !rc; cls++) {
+        if (!present[cls]) continue;
+        for (int pass = 0; pass < 3 && !rc; pass++) {
+            const int *cand = pass == 0 ? cpg : (pass == 1 ? mpt : npt);
+            const int ncand = pass == 0 ? 4 : 3;
+            v2c_tile_shape keep = e->shape[cls];
+            for (int q = 0; q < ncand && !rc; q++) {
+                v2c_tile_shape s = keep;
+                if (pass == 0) s.cells_per_group = cand[q];
+                else if (pass == 1) s.markers_per_tile = cand[q];
+                else s.nodes_per_tile = cand[q];
+                if (memcmp(&s, &keep, sizeof s) == 0) continue;
+                rc = v2c_ens_set_shape(e, cls, &s);
+                double ms = 0;
+                if (!rc) rc = measure(ms);
+                if (!rc && ms < best) { best = ms; keep = s; }
+            }
+            if (!rc) rc = v2c_ens_set_shape(e, cls, &keep);
+        }
+    }
+    const int bpc[] = {2, 4, 8, 16};
+    int keepb = e->blocks_per_cu;
+    for (int q = 0; q < 4 && !rc; q++) {
+        if (bpc[q] == keepb) continue;
+        v2c_ens_set_blocks_per_cu(e, bpc[q]);
+        double ms = 0;
+        rc = measure(ms);
+        if (!rc && ms < best) { best = ms; keepb = bpc[q]; }
+    }
+    if (!rc) v2c_ens_set_blocks_per_cu(e, keepb);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (best_ms) *best_ms = best;
+    return rc;
+}
+
+}  // extern "C"

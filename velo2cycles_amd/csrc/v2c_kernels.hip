@@ -982,7 +982,7 @@ __device__ __forceinline__ bool n2m_update_one(const KArgs &a, long m)
 }
 
 template <bool ADVECT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_n2m_update(KArgs a, int nwg)
+__global__ __launch_bounds__(256) void k_n2m_update(KArgs a, int nwg)
 {
     const long m = (long)xcd_remap(blockIdx.x, nwg) * 256 + threadIdx.x;
     const bool unsupported = m < a.n && n2m_update_one<ADVECT>(a, m);

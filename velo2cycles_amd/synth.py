@@ -399,3 +399,24 @@ def upload(ctx: Context, cfg: Config):
     ctx.set_time(cfg.timesum, cfg.timestep)
     ctx.set_all_nodal(cfg.nodal)
     ctx.upload_markers(cfg.markers)
+
+
+def config_ensemble(n_models=64, seed=DEFAULT_SEED):
+    """Batched-ensemble workload: independent models with grids drawn from
+    201x51 to 801x201 nodes (uniform 1 km spacing), 4 to 36 markers per cell,
+    a-b of the velocity-weakening zone ~ U(-0.006, 0.004) and convergence
+    2 to 8 cm/yr.  Yields (index, Config) one model at a time, so the host
+    holds one marker cloud at once."""
+    rng = np.random.default_rng(seed)
+    for k in range(n_models):
+        nx = int(rng.integers(201, 802))
+        ny = int(rng.integers(51, 202))
+        per = int(rng.integers(2, 7))
+        conv = rng.uniform(2.0, 8.0) * CM_PER_YR
+        amb = rng.uniform(-0.006, 0.004)
+        xs, ys = (nx - 1) * 1e3, (ny - 1) * 1e3
+        geom = Geometry(xtrench=0.3 * xs, ytop=min(8e3, 0.08 * ys), conv=conv)
+        cfg = build_config(f"ensemble-{k}", uniform_axis(nx, xs), uniform_axis(ny, ys), per, geom,
+                           seed + 1000 + k)
+        cfg.params.a_vw = cfg.params.b_vw + amb
+        yield k, cfg
