@@ -149,9 +149,12 @@ def ensemble_bench(args):
         ens.step(graph=True)
     t_default = timed(lambda: ens.step(graph=True), args.steps)
     t_looped = timed(ens.step_looped, args.steps)
-    ens.autotune(reps=2)
+    if args.no_tune:
+        t_graph_only, t_tuned = None, t_default
+    else:
+        t_graph_only = ens.autotune(reps=2)
+        t_tuned = timed(lambda: ens.step(graph=True), args.steps)
     shapes = {str(c): ens.get_shape(c) for c in range(3)}
-    t_tuned = timed(lambda: ens.step(graph=True), args.steps)
     ens.close()
     for c in ctxs:
         c.close()
@@ -164,7 +167,8 @@ def ensemble_bench(args):
                       "nodes": nodes_total, "markers_min": min(sizes), "markers_max": max(sizes),
                       "grids": "201x51 .. 801x201 nodes, 4-36 markers per cell", "dt_s": synth.YEAR},
            "batched_graph_default_ms": t_default * 1e3, "batched_graph_tuned_ms": t_tuned * 1e3,
-           "looped_single_model_ms": t_looped * 1e3, "speedup_vs_looped": t_looped / best,
+           "looped_single_model_ms": t_looped * 1e3,
+           "tuned_graph_only_ms": t_graph_only, "speedup_vs_looped": t_looped / best,
            "tuned_shapes": shapes, "setup_s": round(t_gen, 1)}
     print(json.dumps(out), flush=True)
 
@@ -180,6 +184,7 @@ def main():
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
     ap.add_argument("--ensemble", action="store_true", help="batched ensemble workload instead")
     ap.add_argument("--models", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true", help="ensemble: skip the autotuner")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()

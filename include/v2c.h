@@ -321,19 +321,22 @@ int v2c_reset_timing(v2c_ctx *ctx);
 /* ---- batched ensembles (no reference counterpart) -----------------------
  * Independent models, each an ordinary context (own grid, markers and nodal
  * state, same device), advanced together.  Each stage of the step runs as one
- * launch over a work list of (model, tile) entries processed by a persistent
- * grid; the stage sequence is captured once into a HIP graph and replayed.
+ * launch over a work list of (model, range) entries: per-marker and per-node
+ * stages as one 256-thread block per entry of 256 items, the scatter stages
+ * as a persistent grid of wavefronts striding over cell groups.  The stage
+ * sequence is captured once into a HIP graph and replayed.
  * A step is, per model: marker2nodes, compute_before_solve(timestep, 0),
  * compute_after_solve(timestep, 0), nodes2markers + advection, then
  * timesum += timestep.  Results equal the per-model entry points bit for bit
- * whatever the tile shapes.  Models are grouped in size classes by marker
- * count (< 0.5 M, < 2 M, larger); tile shapes apply per class. */
+ * whatever the shapes.  Models are grouped in size classes by marker count
+ * (< 0.5 M, < 2 M, larger); shapes apply per class. */
 typedef struct v2c_ens v2c_ens;
 #define V2C_ENS_CLASSES 3
 typedef struct {
     int cells_per_group;    /* cells per scatter wavefront group, 1..1024 */
-    int markers_per_tile;   /* markers per work-list entry, >= 64 */
-    int nodes_per_tile;     /* nodes per work-list entry, a multiple of 256 */
+    int markers_per_tile;   /* markers of one model kept on one XCD in a row
+                               (a run of entries), a multiple of 256 */
+    int nodes_per_tile;     /* the same for nodes, a multiple of 256 */
 } v2c_tile_shape;
 int v2c_ens_create(v2c_ens **ens, v2c_ctx *const *models, int nmodels);
 void v2c_ens_destroy(v2c_ens *ens);
@@ -341,7 +344,7 @@ const char *v2c_ens_last_error(const v2c_ens *ens);
 int v2c_ens_size_class(const v2c_ens *ens, int model);
 int v2c_ens_set_shape(v2c_ens *ens, int size_class, const v2c_tile_shape *shape);
 int v2c_ens_get_shape(const v2c_ens *ens, int size_class, v2c_tile_shape *shape);
-/* persistent grid = blocks_per_cu x compute units (x4 for 64-thread groups) */
+/* persistent scatter grid = 4 x blocks_per_cu x compute units wavefronts */
 int v2c_ens_set_blocks_per_cu(v2c_ens *ens, int blocks);
 /* one batched step; use_graph: replay the captured stage graph */
 int v2c_ens_step(v2c_ens *ens, int use_graph);

@@ -48,6 +48,7 @@ struct v2c_ctx {
     unsigned long long *d_mask = nullptr;
     uint64_t *d_k64_in = nullptr, *d_k64_out = nullptr;
     long *d_nsel = nullptr;
+    long *h_nsel = nullptr;   // pinned: mover count read back by the ordering
     // local update
     double *d_mx = nullptr;
     uint8_t *d_fault = nullptr;
@@ -201,14 +202,18 @@ static int full_sort(v2c_ctx *c)
 
 // Bring order[]/skey[]/sub_start[] up to date with the current keys.  After
 // an advection only markers whose key changed are re-sorted and merged in
-// (see k_mover_flags); the result is the same (key, id) order a full sort
-// gives.
-static int ensure_order(v2c_ctx *c)
+// (k_order_flags finds them); the result is the same (key, id) order a full
+// sort gives.  The work is split in two halves around the one read-back (the
+// mover count), so that several contexts can have their first halves in
+// flight together: order_begin launches the flagging and the count copy and
+// sets *pending; order_finish, once the stream has reached the copy, launches
+// the rest.
+static int order_begin(v2c_ctx *c, bool *pending)
 {
+    *pending = false;
     if (c->order_valid) return 0;
-    int rc;
     if (!c->ever_sorted || c->n == 0) {
-        rc = full_sort(c);
+        int rc = full_sort(c);
         if (rc) return rc;
         c->last_movers = c->n;
         c->order_valid = true;
@@ -216,16 +221,24 @@ static int ensure_order(v2c_ctx *c)
     }
     const long n = c->n;
     const long ntiles = (n + ORD_TILE - 1) / ORD_TILE;
-    rc = launch(c, V2C_K_SORT, [&] {
+    int rc = launch(c, V2C_K_SORT, [&] {
         hipLaunchKernelGGL(k_order_flags, dim3((unsigned)ntiles), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
                            c->d_key, c->d_skey, n, c->d_mask, c->d_tile_count);
         hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, c->stream, c->d_tile_count, ntiles,
                            c->d_tile_off, c->d_nsel);
     });
     if (rc) return rc;
-    long nmov = 0;
-    HIPCHK(c, hipMemcpyAsync(&nmov, c->d_nsel, sizeof(long), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_nsel, c->d_nsel, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    *pending = true;
+    return 0;
+}
+
+static int order_finish(v2c_ctx *c)
+{
+    const long n = c->n;
+    const long ntiles = (n + ORD_TILE - 1) / ORD_TILE;
+    const long nmov = *c->h_nsel;
+    int rc;
     c->last_movers = nmov;
     if (nmov == 0) {
         c->order_valid = true;
@@ -258,6 +271,15 @@ static int ensure_order(v2c_ctx *c)
     if (rc) return rc;
     c->order_valid = true;
     return 0;
+}
+
+static int ensure_order(v2c_ctx *c)
+{
+    bool pending = false;
+    int rc = order_begin(c, &pending);
+    if (rc || !pending) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return order_finish(c);
 }
 
 extern "C" {
@@ -312,6 +334,7 @@ static void free_all(v2c_ctx *c)
         if (p) (void)hipFree(p);
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->h_nsel) (void)hipHostFree(c->h_nsel);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -381,6 +404,7 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_k64_in, sizeof(uint64_t) * c->cap));
     CK(hipMalloc(&c->d_k64_out, sizeof(uint64_t) * c->cap));
     CK(hipMalloc(&c->d_nsel, sizeof(long) * 2));
+    CK(hipHostMalloc((void **)&c->h_nsel, sizeof(long), hipHostMallocDefault));
     CK(hipMalloc(&c->d_mx, sizeof(double) * NMX * c->cap));
     CK(hipMalloc(&c->d_fault, c->cap));
     CK(hipMalloc(&c->d_lu, sizeof(unsigned long long) * LS_COUNT));

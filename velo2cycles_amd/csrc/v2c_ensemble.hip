@@ -3,19 +3,30 @@
 // Every model is an ordinary context (its own grid, marker cloud and nodal
 // state).  A step of the whole ensemble runs each stage of the per-model step
 // (marker2nodes, compute_before_solve, compute_after_solve, nodes2markers +
-// advection) as ONE launch over a work list of (model, tile) entries: a
-// persistent grid strides over the list, and every tile calls the same device
-// function the single-model kernel calls, on that model's arguments.  The
-// stage sequence is captured once into a HIP graph and replayed each step;
-// only the cell ordering (which decides between an incremental merge and a
-// full sort on the host) and the argument upload run outside the graph.
+// advection) as ONE launch over a work list of (model, range) entries, and
+// every entry calls the same device function the single-model kernel calls,
+// on that model's arguments.  The stage sequence is captured once into a HIP
+// graph and replayed each step; only the cell ordering (which decides between
+// an incremental merge and a full sort on the host) and the argument upload
+// run outside the graph.
 //
-// Tile shapes (cells per scatter group, markers and nodes per tile) are chosen
+// Two launch forms:
+//  * item stages (per marker, per node): one 256-thread block per entry of at
+//    most 256 items of one model, one item per thread and no loop, so each
+//    model's arguments are loaded where they are used (a loop over entries
+//    would keep them live across the whole per-marker body).  The host orders
+//    the entry list so that runs of consecutive entries of one model land on
+//    one XCD (blocks are dealt to the 8 XCDs round-robin), which keeps a
+//    model's nodal rows in that XCD's L2.
+//  * scatter stages (per cell group, one wavefront each): a persistent grid
+//    strides over the list of cell groups.
+//
+// Shapes (cells per scatter group; markers and nodes per XCD run) are chosen
 // per model size class; v2c_ens_autotune times candidates on the ensemble
 // itself and keeps the fastest.  Results do not depend on the shapes: every
 // sub-cell sum is the same left-to-right sum over its sorted markers whatever
-// the batch and group boundaries, so a batched step equals the per-model
-// steps bit for bit.
+// the batch and group boundaries, and every item is computed alone, so a
+// batched step equals the per-model steps bit for bit.
 // Included at the end of v2c_api.hip: it uses the context internals.
 
 namespace v2c {
@@ -30,23 +41,22 @@ struct EnsTile {
     long begin, end;            // item range: markers, cells or nodes
 };
 
-constexpr int ENS_BLOCK = 256;
+constexpr int ENS_BLOCK = 256;  // items per item-stage entry (= RED_BLOCK)
+static_assert(ENS_BLOCK == RED_BLOCK, "node entries double as reduction blocks");
 
+// Item stages: block b takes entry b of the list; thread i its i-th item.
 template <class F>
-__device__ __forceinline__ void ens_items(const KArgs *__restrict__ A, const EnsTile *__restrict__ tiles, int ntiles,
-                                          F &&f)
+__device__ __forceinline__ void ens_item(const KArgs *__restrict__ A, const EnsTile *__restrict__ tiles, F &&f)
 {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const EnsTile T = tiles[t];
-        const KArgs a = model_args(A, T.model);
-        for (long i = T.begin + threadIdx.x; i < T.end; i += blockDim.x) f(a, i);
-    }
+    const EnsTile T = tiles[blockIdx.x];
+    const long i = T.begin + threadIdx.x;
+    if (i < T.end) f(model_args(A, T.model), i);
 }
 
 __global__ __launch_bounds__(ENS_BLOCK) void k_ens_m2n_props(const KArgs *__restrict__ A,
-                                                             const EnsTile *__restrict__ tiles, int ntiles)
+                                                             const EnsTile *__restrict__ tiles)
 {
-    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m) { m2n_props_one(a, m); });
+    ens_item(A, tiles, [&](const KArgs &a, long m) { m2n_props_one(a, m); });
 }
 
 __global__ __launch_bounds__(WAVE) void k_ens_m2n_cells(const KArgs *__restrict__ A,
@@ -61,42 +71,39 @@ __global__ __launch_bounds__(WAVE) void k_ens_m2n_cells(const KArgs *__restrict_
 }
 
 __global__ __launch_bounds__(ENS_BLOCK) void k_ens_m2n_nodes(const KArgs *__restrict__ A,
-                                                             const EnsTile *__restrict__ tiles, int ntiles)
+                                                             const EnsTile *__restrict__ tiles)
 {
-    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m3) { m2n_nodes_one(a, m3); });
+    ens_item(A, tiles, [&](const KArgs &a, long m3) { m2n_nodes_one(a, m3); });
 }
 
 __global__ __launch_bounds__(ENS_BLOCK) void k_ens_phis_bc(const KArgs *__restrict__ A,
-                                                           const EnsTile *__restrict__ tiles, int ntiles)
+                                                           const EnsTile *__restrict__ tiles)
 {
-    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m3) {
+    ens_item(A, tiles, [&](const KArgs &a, long m3) {
         if (a.p.Dphis > 0) phis_bc_one(a, m3);
     });
 }
 
 __global__ __launch_bounds__(ENS_BLOCK) void k_ens_before(const KArgs *__restrict__ A,
-                                                          const EnsTile *__restrict__ tiles, int ntiles)
+                                                          const EnsTile *__restrict__ tiles)
 {
-    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m3) { before_one(a, m3, a.timestep, 0); });
+    ens_item(A, tiles, [&](const KArgs &a, long m3) { before_one(a, m3, a.timestep, 0); });
 }
 
-// Node tiles are whole RED_BLOCK chunks of one model, so each chunk's
-// fixed-order sum lands in the model's partial slot begin / RED_BLOCK, as in
-// the single-model launch.
+// Node entries are whole RED_BLOCK chunks of one model (they start at
+// multiples of RED_BLOCK), so each chunk's fixed-order sum lands in the
+// model's partial slot begin / RED_BLOCK, as in the single-model launch.
 __global__ __launch_bounds__(RED_BLOCK) void k_ens_after(const KArgs *__restrict__ A, double *const *__restrict__ red,
-                                                         const EnsTile *__restrict__ tiles, int ntiles)
+                                                         const EnsTile *__restrict__ tiles)
 {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const EnsTile T = tiles[t];
-        const KArgs a = model_args(A, T.model);
-        for (long c = T.begin; c < T.end; c += RED_BLOCK) {
-            const long m3 = c + threadIdx.x;
-            double dsq = 0.0, cnt = 0.0;
-            if (m3 < T.end) after_one(a, m3, a.timestep, dsq, cnt);
-            after_block_sum(dsq, cnt, red[T.model], c / RED_BLOCK);
-            __syncthreads();
-        }
+    const EnsTile T = tiles[blockIdx.x];
+    const long m3 = T.begin + threadIdx.x;
+    double dsq = 0.0, cnt = 0.0;
+    if (m3 < T.end) {
+        const KArgs &a = model_args(A, T.model);
+        after_one(a, m3, a.timestep, dsq, cnt);
     }
+    after_block_sum(dsq, cnt, red[T.model], T.begin / RED_BLOCK);
 }
 
 // One workgroup per model: fixed-order sum of the model's partials.
@@ -134,30 +141,27 @@ __global__ __launch_bounds__(WAVE) void k_ens_n2m_scatter(const KArgs *__restric
     __shared__ ScatterLds<N_NF> L;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const EnsTile T = tiles[t];
-        const KArgs a = model_args(A, T.model);
+        const KArgs &a = model_args(A, T.model);
         if (a.p.antidiffinter == 1) n2m_scatter_group(a, T.begin, T.end, threadIdx.x, L);
         __syncthreads();
     }
 }
 
 __global__ __launch_bounds__(ENS_BLOCK) void k_ens_n2m_nodes(const KArgs *__restrict__ A,
-                                                             const EnsTile *__restrict__ tiles, int ntiles)
+                                                             const EnsTile *__restrict__ tiles)
 {
-    ens_items(A, tiles, ntiles, [&](const KArgs &a, long m3) {
+    ens_item(A, tiles, [&](const KArgs &a, long m3) {
         if (a.p.antidiffinter == 1) n2m_nodes_one(a, m3);
     });
 }
 
-__global__ __launch_bounds__(ENS_BLOCK) void k_ens_n2m_update(
-    const KArgs *__restrict__ A, const EnsTile *__restrict__ tiles, int ntiles, int *unsupported)
+__global__ __launch_bounds__(ENS_BLOCK) void k_ens_n2m_update(const KArgs *__restrict__ A,
+                                                              const EnsTile *__restrict__ tiles, int *unsupported)
 {
-    int bad = 0;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const EnsTile T = tiles[t];
-        const KArgs a = model_args(A, T.model);
-        for (long m = T.begin + threadIdx.x; m < T.end; m += blockDim.x) bad += n2m_update_one<true>(a, m) ? 1 : 0;
-    }
-    if (bad) atomicAdd(unsupported, bad);
+    bool bad = false;
+    ens_item(A, tiles, [&](const KArgs &a, long m) { bad = n2m_update_one<true>(a, m); });
+    const unsigned long long mask = __ballot(bad);
+    if (mask && (threadIdx.x & (WAVE - 1)) == 0) atomicAdd(unsupported, (int)__popcll(mask));
 }
 
 __global__ void k_ens_clear(int *p, int n)
@@ -170,9 +174,9 @@ __global__ void k_ens_clear(int *p, int n)
 
 using namespace v2c;
 
-// Work lists of one tile shape assignment.
+// Work lists of one shape assignment.
 struct EnsPlan {
-    std::vector<EnsTile> markers, cells, nodes, after;
+    std::vector<EnsTile> markers, cells, nodes;
 };
 
 struct v2c_ens {
@@ -187,12 +191,13 @@ struct v2c_ens {
     KArgs *h_args = nullptr;                 // pinned staging
     EnsTile *d_tiles = nullptr;
     long tile_cap = 0;
-    int n_mark = 0, n_cell = 0, n_node = 0, n_after = 0;
-    EnsTile *t_mark = nullptr, *t_cell = nullptr, *t_node = nullptr, *t_after = nullptr;
+    int n_mark = 0, n_cell = 0, n_node = 0;
+    EnsTile *t_mark = nullptr, *t_cell = nullptr, *t_node = nullptr;
     double **d_red = nullptr;
     int *d_redn = nullptr;
     double *d_rms = nullptr;
     int *d_flags = nullptr;                  // [0] unsupported marker types
+    std::vector<hipEvent_t> ordered;         // per model: its ordering is done
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_valid = false;
@@ -227,38 +232,74 @@ static int ens_class_of(const v2c_ctx *c)
 static void ens_default_shape(v2c_tile_shape *s)
 {
     s->cells_per_group = CPW;
-    s->markers_per_tile = 1024;
-    s->nodes_per_tile = 1024;
+    s->markers_per_tile = 16384;
+    s->nodes_per_tile = 16384;
+}
+
+// Item-stage list: every model's range cut into entries of ENS_BLOCK items,
+// grouped into runs of `run` consecutive entries of one model; each run goes
+// to the XCD queue with the fewest entries so far, and the queues are
+// interleaved so that list position b sits in queue b % 8, the XCD that the
+// round-robin block dispatch sends block b to.
+static void ens_item_list(std::vector<EnsTile> &out, const std::vector<std::vector<EnsTile>> &runs)
+{
+    constexpr int NXCD = 8;
+    std::vector<EnsTile> q[NXCD];
+    size_t qn[NXCD] = {0};
+    for (const auto &r : runs) {
+        int best = 0;
+        for (int x = 1; x < NXCD; x++)
+            if (qn[x] < qn[best]) best = x;
+        q[best].insert(q[best].end(), r.begin(), r.end());
+        qn[best] += r.size();
+    }
+    size_t pos[NXCD] = {0};
+    for (bool more = true; more;) {
+        more = false;
+        for (int x = 0; x < NXCD; x++)
+            if (pos[x] < q[x].size()) {
+                out.push_back(q[x][pos[x]++]);
+                more = true;
+            }
+    }
+}
+
+static void ens_cut_runs(std::vector<std::vector<EnsTile>> &runs, int mod, long n, int run)
+{
+    for (long b = 0; b < n; b += (long)ENS_BLOCK * run) {
+        std::vector<EnsTile> r;
+        const long e = std::min(n, b + (long)ENS_BLOCK * run);
+        for (long t = b; t < e; t += ENS_BLOCK) r.push_back(EnsTile{mod, 0, t, std::min(e, t + ENS_BLOCK)});
+        runs.push_back(std::move(r));
+    }
 }
 
 static EnsPlan ens_plan(const v2c_ens *e)
 {
     EnsPlan p;
+    std::vector<std::vector<EnsTile>> mruns, nruns;
     for (size_t k = 0; k < e->models.size(); k++) {
         const v2c_ctx *c = e->models[k];
         const v2c_tile_shape &s = e->shape[e->size_class[k]];
         const int mod = (int)k;
-        for (long b = 0; b < c->n; b += s.markers_per_tile)
-            p.markers.push_back(EnsTile{mod, 0, b, std::min(c->n, b + (long)s.markers_per_tile)});
+        ens_cut_runs(mruns, mod, c->n, s.markers_per_tile / ENS_BLOCK);
+        ens_cut_runs(nruns, mod, c->nodenum, s.nodes_per_tile / ENS_BLOCK);
         for (long b = 0; b < c->ncell; b += s.cells_per_group)
             p.cells.push_back(EnsTile{mod, 0, b, std::min(c->ncell, b + (long)s.cells_per_group)});
-        for (long b = 0; b < c->nodenum; b += s.nodes_per_tile)
-            p.nodes.push_back(EnsTile{mod, 0, b, std::min(c->nodenum, b + (long)s.nodes_per_tile)});
-        const long rt = ((long)s.nodes_per_tile / RED_BLOCK) * RED_BLOCK;
-        for (long b = 0; b < c->nodenum; b += rt)
-            p.after.push_back(EnsTile{mod, 0, b, std::min(c->nodenum, b + rt)});
     }
+    ens_item_list(p.markers, mruns);
+    ens_item_list(p.nodes, nruns);
     return p;
 }
 
 static int ens_upload_plan(v2c_ens *e)
 {
     const EnsPlan p = ens_plan(e);
-    const long total = (long)(p.markers.size() + p.cells.size() + p.nodes.size() + p.after.size());
+    const long total = (long)(p.markers.size() + p.cells.size() + p.nodes.size());
     if (total > e->tile_cap) {
         if (e->d_tiles) (void)hipFree(e->d_tiles);
         e->d_tiles = nullptr;
-        ENSCHK(e, hipMalloc(&e->d_tiles, sizeof(EnsTile) * total));
+        ENSCHK(e, hipMalloc(&e->d_tiles, sizeof(EnsTile) * std::max(total, 1L)));
         e->tile_cap = total;
     }
     std::vector<EnsTile> all;
@@ -266,24 +307,23 @@ static int ens_upload_plan(v2c_ens *e)
     all.insert(all.end(), p.markers.begin(), p.markers.end());
     all.insert(all.end(), p.cells.begin(), p.cells.end());
     all.insert(all.end(), p.nodes.begin(), p.nodes.end());
-    all.insert(all.end(), p.after.begin(), p.after.end());
-    ENSCHK(e, hipMemcpyAsync(e->d_tiles, all.data(), sizeof(EnsTile) * total, hipMemcpyHostToDevice, e->stream));
+    if (total)
+        ENSCHK(e, hipMemcpyAsync(e->d_tiles, all.data(), sizeof(EnsTile) * total, hipMemcpyHostToDevice, e->stream));
     ENSCHK(e, hipStreamSynchronize(e->stream));
     e->n_mark = (int)p.markers.size();
     e->n_cell = (int)p.cells.size();
     e->n_node = (int)p.nodes.size();
-    e->n_after = (int)p.after.size();
     e->t_mark = e->d_tiles;
     e->t_cell = e->t_mark + e->n_mark;
     e->t_node = e->t_cell + e->n_cell;
-    e->t_after = e->t_node + e->n_node;
     e->graph_valid = false;
     return 0;
 }
 
-static unsigned ens_grid(const v2c_ens *e, int ntiles, int waves_per_block)
+// Persistent grid of the scatter stages (one wavefront per block).
+static unsigned ens_grid(const v2c_ens *e, int ntiles)
 {
-    const long g = (long)e->ncu * e->blocks_per_cu * (waves_per_block >= 4 ? 1 : 4);
+    const long g = (long)e->ncu * e->blocks_per_cu * 4;
     return (unsigned)std::max(1L, std::min((long)ntiles, g));
 }
 
@@ -291,45 +331,59 @@ static unsigned ens_grid(const v2c_ens *e, int ntiles, int waves_per_block)
 static void ens_stages(v2c_ens *e)
 {
     const int nm = (int)e->models.size();
+    const dim3 mg((unsigned)e->n_mark), ng((unsigned)e->n_node);
     hipLaunchKernelGGL(k_ens_clear, dim3(1), dim3(64), 0, e->stream, e->d_flags, 4);
-    hipLaunchKernelGGL(k_ens_m2n_props, dim3(ens_grid(e, e->n_mark, 4)), dim3(ENS_BLOCK), 0, e->stream, e->d_args,
-                       e->t_mark, e->n_mark);
-    hipLaunchKernelGGL(k_ens_m2n_cells, dim3(ens_grid(e, e->n_cell, 1)), dim3(WAVE), 0, e->stream, e->d_args,
-                       e->t_cell, e->n_cell);
-    hipLaunchKernelGGL(k_ens_m2n_nodes, dim3(ens_grid(e, e->n_node, 4)), dim3(ENS_BLOCK), 0, e->stream,
-                       e->d_args, e->t_node, e->n_node);
-    bool dphis = false;
-    for (auto *c : e->models) dphis = dphis || c->prm.Dphis > 0;
-    if (dphis)
-        hipLaunchKernelGGL(k_ens_phis_bc, dim3(ens_grid(e, e->n_node, 4)), dim3(ENS_BLOCK), 0, e->stream,
-                           e->d_args, e->t_node, e->n_node);
-    hipLaunchKernelGGL(k_ens_before, dim3(ens_grid(e, e->n_node, 4)), dim3(ENS_BLOCK), 0, e->stream, e->d_args,
-                       e->t_node, e->n_node);
-    hipLaunchKernelGGL(k_ens_after, dim3(ens_grid(e, e->n_after, 4)), dim3(RED_BLOCK), 0, e->stream, e->d_args,
-                       e->d_red, e->t_after, e->n_after);
+    if (e->n_mark)
+        hipLaunchKernelGGL(k_ens_m2n_props, mg, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_mark);
+    if (e->n_cell)
+        hipLaunchKernelGGL(k_ens_m2n_cells, dim3(ens_grid(e, e->n_cell)), dim3(WAVE), 0, e->stream, e->d_args,
+                           e->t_cell, e->n_cell);
+    if (e->n_node) {
+        hipLaunchKernelGGL(k_ens_m2n_nodes, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
+        bool dphis = false;
+        for (auto *c : e->models) dphis = dphis || c->prm.Dphis > 0;
+        if (dphis) hipLaunchKernelGGL(k_ens_phis_bc, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
+        hipLaunchKernelGGL(k_ens_before, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
+        hipLaunchKernelGGL(k_ens_after, ng, dim3(RED_BLOCK), 0, e->stream, e->d_args, e->d_red, e->t_node);
+    }
     hipLaunchKernelGGL(k_ens_after_final, dim3(nm), dim3(RED_BLOCK), 0, e->stream, e->d_red, e->d_redn, e->d_rms);
     hipLaunchKernelGGL(k_ens_phantom, dim3((nm + 63) / 64), dim3(64), 0, e->stream, e->d_args, nm);
-    hipLaunchKernelGGL(k_ens_n2m_scatter, dim3(ens_grid(e, e->n_cell, 1)), dim3(WAVE), 0, e->stream, e->d_args,
-                       e->t_cell, e->n_cell);
-    hipLaunchKernelGGL(k_ens_n2m_nodes, dim3(ens_grid(e, e->n_node, 4)), dim3(ENS_BLOCK), 0, e->stream,
-                       e->d_args, e->t_node, e->n_node);
-    hipLaunchKernelGGL(k_ens_n2m_update, dim3(ens_grid(e, e->n_mark, 4)), dim3(ENS_BLOCK), 0, e->stream,
-                       e->d_args, e->t_mark, e->n_mark, e->d_flags);
+    if (e->n_cell)
+        hipLaunchKernelGGL(k_ens_n2m_scatter, dim3(ens_grid(e, e->n_cell)), dim3(WAVE), 0, e->stream, e->d_args,
+                           e->t_cell, e->n_cell);
+    if (e->n_node) hipLaunchKernelGGL(k_ens_n2m_nodes, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
+    if (e->n_mark)
+        hipLaunchKernelGGL(k_ens_n2m_update, mg, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_mark, e->d_flags);
 }
 
 // Per-step preparation outside the graph: cell ordering of every model (it
-// may choose a full sort on the host) and the argument array.
+// may choose a full sort on the host) and the argument array.  Every model
+// orders on its own stream; the first halves (flagging and the mover count)
+// of all models are launched before any count is read back, so the small
+// ordering kernels of different models overlap.  The ensemble stream then
+// waits for each model's ordering through an event.
 static int ens_prepare(v2c_ens *e)
 {
-    for (size_t k = 0; k < e->models.size(); k++) {
-        v2c_ctx *c = e->models[k];
-        int rc = ensure_order(c);
-        if (rc) return ens_err(e, rc, "model %zu: ordering failed: %s", k, c->err);
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        e->h_args[k] = make_args(c);
+    const size_t nm = e->models.size();
+    std::vector<char> pending(nm, 0);
+    for (size_t k = 0; k < nm; k++) {
+        bool p = false;
+        int rc = order_begin(e->models[k], &p);
+        if (rc) return ens_err(e, rc, "model %zu: ordering failed: %s", k, e->models[k]->err);
+        pending[k] = p;
     }
-    ENSCHK(e, hipMemcpyAsync(e->d_args, e->h_args, sizeof(KArgs) * e->models.size(), hipMemcpyHostToDevice,
-                             e->stream));
+    for (size_t k = 0; k < nm; k++) {
+        v2c_ctx *c = e->models[k];
+        if (pending[k]) {
+            ENSCHK(e, hipStreamSynchronize(c->stream));
+            int rc = order_finish(c);
+            if (rc) return ens_err(e, rc, "model %zu: ordering failed: %s", k, c->err);
+        }
+        e->h_args[k] = make_args(c);
+        ENSCHK(e, hipEventRecord(e->ordered[k], c->stream));
+        ENSCHK(e, hipStreamWaitEvent(e->stream, e->ordered[k], 0));
+    }
+    ENSCHK(e, hipMemcpyAsync(e->d_args, e->h_args, sizeof(KArgs) * nm, hipMemcpyHostToDevice, e->stream));
     return 0;
 }
 
@@ -366,6 +420,9 @@ int v2c_ens_create(v2c_ens **out, v2c_ctx *const *models, int nmodels)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) e->ncu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(V2C_EHIP);
+    e->ordered.assign(nmodels, nullptr);
+    for (int k = 0; k < nmodels; k++)
+        if (hipEventCreateWithFlags(&e->ordered[k], hipEventDisableTiming) != hipSuccess) return fail(V2C_EHIP);
     if (hipMalloc(&e->d_args, sizeof(KArgs) * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
     if (hipHostMalloc((void **)&e->h_args, sizeof(KArgs) * nmodels, hipHostMallocDefault) != hipSuccess)
         return fail(V2C_ENOMEM);
@@ -396,6 +453,8 @@ void v2c_ens_destroy(v2c_ens *e)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (e->h_args) (void)hipHostFree(e->h_args);
+    for (auto ev : e->ordered)
+        if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -411,8 +470,9 @@ int v2c_ens_size_class(const v2c_ens *e, int model)
 int v2c_ens_set_shape(v2c_ens *e, int size_class, const v2c_tile_shape *s)
 {
     if (!e || !s || size_class < 0 || size_class >= V2C_ENS_CLASSES) return V2C_EINVAL;
-    if (s->cells_per_group < 1 || s->cells_per_group > 1024 || s->markers_per_tile < 64 ||
-        s->nodes_per_tile < RED_BLOCK || s->nodes_per_tile % RED_BLOCK)
+    if (s->cells_per_group < 1 || s->cells_per_group > 1024 || s->markers_per_tile < ENS_BLOCK ||
+        s->markers_per_tile % ENS_BLOCK || s->markers_per_tile > (1 << 24) || s->nodes_per_tile < ENS_BLOCK ||
+        s->nodes_per_tile % ENS_BLOCK || s->nodes_per_tile > (1 << 24))
         return ens_err(e, V2C_EINVAL, "tile shape out of range");
     e->shape[size_class] = *s;
     ENSCHK(e, hipSetDevice(e->device));
@@ -530,8 +590,8 @@ int v2c_ens_autotune(v2c_ens *e, int reps, double *best_ms)
     double best = 0;
     int rc = measure(best);
     const int cpg[] = {8, 16, 32, 64};
-    const int mpt[] = {256, 1024, 4096};
-    const int npt[] = {256, 1024, 4096};
+    const int mpt[] = {256, 4096, 65536};
+    const int npt[] = {256, 4096, 65536};
     for (int cls = 0; cls < V2C_ENS_CLASSES && !rc; cls++) {
         if (!present[cls]) continue;
         for (int pass = 0; pass < 3 && !rc; pass++) {

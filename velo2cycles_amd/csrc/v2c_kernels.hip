@@ -453,17 +453,20 @@ __device__ __forceinline__ void m2n_cells_group(const KArgs &a, long c0, long c1
     const double *P = a.prop;
     bool have_carry = false;
     int par = 0;
+    // the next batch's marker ids and keys are loaded one batch ahead
+    uint32_t m_next = beg + lane < end ? a.order[beg + lane] : 0u;
+    uint32_t k_next = beg + lane < end ? a.skey[beg + lane] : 0xffffffffu;
     for (uint32_t base = beg; base < end; base += WAVE, par ^= 1) {
         const uint32_t i = base + lane;
         const bool active = i < end;
         const int nact = end - base < (uint32_t)WAVE ? (int)(end - base) : WAVE;
+        const long m = m_next;
+        uint32_t key = active ? k_next : 0xffffffffu;
+        if (i + WAVE < end) { m_next = a.order[i + WAVE]; k_next = a.skey[i + WAVE]; }
         double v[M_NF];
 #pragma unroll
         for (int f = 0; f < M_NF; f++) v[f] = 0.0;
-        uint32_t key = 0xffffffffu;
         if (active) {
-            const long m = a.order[i];
-            key = a.skey[i];
             const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
             if (x > 0 && y > 0 && x < g.xsize && y < g.ysize) {
                 Axis4 X, Y;
@@ -900,16 +903,18 @@ __device__ __forceinline__ void n2m_scatter_group(const KArgs &a, long c0, long 
     const uint32_t beg = a.sub_start[c0 * 4], end = a.sub_start[c1 * 4];
     bool have_carry = false;
     int par = 0;
+    uint32_t m_next = beg + lane < end ? a.order[beg + lane] : 0u;
+    uint32_t k_next = beg + lane < end ? a.skey[beg + lane] : 0xffffffffu;
     for (uint32_t base = beg; base < end; base += WAVE, par ^= 1) {
         const uint32_t i = base + lane;
         const int nact = end - base < (uint32_t)WAVE ? (int)(end - base) : WAVE;
+        const long m = m_next;
+        uint32_t key = i < end ? k_next : 0xffffffffu;
+        if (i + WAVE < end) { m_next = a.order[i + WAVE]; k_next = a.skey[i + WAVE]; }
         double v[N_NF];
 #pragma unroll
         for (int f = 0; f < N_NF; f++) v[f] = 0.0;
-        uint32_t key = 0xffffffffu;
         if (i < end) {
-            const long m = a.order[i];
-            key = a.skey[i];
             Axis4 X, Y;
             key_axes(a, key, X, Y);
             const double x = clampd(MK(a, MF_X, m), 0, g.xsize), y = clampd(MK(a, MF_Y, m), 0, g.ysize);
