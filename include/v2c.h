@@ -348,6 +348,9 @@ int v2c_ens_get_shape(const v2c_ens *ens, int size_class, v2c_tile_shape *shape)
 int v2c_ens_set_blocks_per_cu(v2c_ens *ens, int blocks);
 /* one batched step; use_graph: replay the captured stage graph */
 int v2c_ens_step(v2c_ens *ens, int use_graph);
+/* bring every model's cell ordering up to date in one batched pass (the
+ * movers of all models in one radix sort); a step does this itself */
+int v2c_ens_order(v2c_ens *ens);
 /* the same step through the per-model entry points (one stream per model) */
 int v2c_ens_step_looped(v2c_ens *ens);
 /* compute_after_solve misfit of every model's last batched step:

@@ -110,3 +110,32 @@ def test_ensemble_looped_baseline_and_autotune(gpu):
     eb.close()
     for c in a + b:
         c.close()
+
+
+def test_ensemble_batched_reordering_matches_single_models(gpu):
+    """Timesteps from 1 to 3000 years: some models re-order incrementally in
+    the batched pass (one mover sort for all), some fall back to a full sort;
+    orders, markers and nodes equal the per-model path after every step."""
+    cfgs = _models()
+    for c, years in zip(cfgs, (1.0, 30.0, 300.0, 3000.0)):
+        c.timestep = years * DT
+    singles = [_ctx(gpu, c) for c in cfgs]
+    batch = [_ctx(gpu, c) for c in cfgs]
+    ens = Ensemble(batch)
+    for step in range(3):
+        for ctx, cfg in zip(singles, cfgs):
+            _single_step(ctx, cfg, cfg.timesum + step * cfg.timestep)
+        ens.step(graph=True)
+        ens.order()                       # the batched pass the next step would run
+        for k, cfg in enumerate(cfgs):
+            oa, sa = batch[k].order_get()
+            ob, sb = singles[k].order_get()
+            assert np.array_equal(sa, sb) and np.array_equal(oa, ob), (step, k)
+            assert batch[k].order_stats()["last_movers"] == singles[k].order_stats()["last_movers"], (step, k)
+    for k, cfg in enumerate(cfgs):
+        _assert_same(_state(batch[k], cfg), _state(singles[k], cfg), f"model {k}")
+    stats = [c.order_stats() for c in batch]
+    assert any(s["incremental"] > 0 for s in stats), stats
+    ens.close()
+    for c in singles + batch:
+        c.close()

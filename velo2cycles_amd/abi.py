@@ -414,7 +414,8 @@ def _bind_ens(lib):
     lib.v2c_ens_last_error.restype = C.c_char_p
     for name, args in [("v2c_ens_size_class", [C.c_int]), ("v2c_ens_set_shape", [C.c_int, P(TileShape)]),
                        ("v2c_ens_get_shape", [C.c_int, P(TileShape)]), ("v2c_ens_set_blocks_per_cu", [C.c_int]),
-                       ("v2c_ens_step", [C.c_int]), ("v2c_ens_step_looped", []), ("v2c_ens_rms", [P(C.c_double)]),
+                       ("v2c_ens_step", [C.c_int]), ("v2c_ens_step_looped", []), ("v2c_ens_order", []),
+                       ("v2c_ens_rms", [P(C.c_double)]),
                        ("v2c_ens_autotune", [C.c_int, P(C.c_double)])]:
         fn = getattr(lib, name)
         fn.argtypes = [C.c_void_p] + args
@@ -442,6 +443,10 @@ class Ensemble:
 
     def step(self, graph=True):
         self._check(self.lib.v2c_ens_step(self.h, int(graph)), "ens_step")
+
+    def order(self):
+        """Bring every model's cell ordering up to date in one batched pass."""
+        self._check(self.lib.v2c_ens_order(self.h), "ens_order")
 
     def step_looped(self):
         self._check(self.lib.v2c_ens_step_looped(self.h), "ens_step_looped")

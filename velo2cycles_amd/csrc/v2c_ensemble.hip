@@ -170,13 +170,118 @@ __global__ void k_ens_clear(int *p, int n)
     if (i < n) p[i] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Batched incremental re-ordering: the passes of the single-model
+// re-ordering, each one launch over all models, with ONE radix sort of the
+// movers of every model (model number in the bits above the key, so each
+// model's movers come out contiguous and in their own (key, id) order) and
+// ONE read-back of the mover counts.  A model takes this path when it was
+// sorted before and 0 < movers <= n/4, the rule of the single-model path;
+// above n/4 it gets a full sort of its own.
+// ---------------------------------------------------------------------------
+struct EnsOrd {
+    V2C_GLOBAL const uint32_t *key;                 // physical key per marker
+    V2C_GLOBAL const uint32_t *order, *skey;        // current order
+    V2C_GLOBAL uint32_t *order2, *skey2;            // the new order
+    V2C_GLOBAL unsigned long long *mask;
+    V2C_GLOBAL uint32_t *tile_count, *tile_off, *sub_start;
+    long n, ntiles, nsub;
+    int active;                                     // re-ordered this step
+    int pad;
+};
+
+__host__ __device__ inline bool eo_incremental(const EnsOrd &o, long nmov)
+{
+    return o.active && nmov > 0 && nmov <= o.n / 4;
+}
+
+__global__ __launch_bounds__(ORD_BLOCK) void k_eo_flags(const EnsOrd *__restrict__ O, const EnsTile *__restrict__ tiles)
+{
+    const EnsTile T = tiles[blockIdx.x];
+    const EnsOrd &o = O[T.model];
+    if (o.active) order_flags_tile(o.order, o.key, o.skey, o.n, o.mask, o.tile_count, T.begin);
+}
+
+__global__ __launch_bounds__(1024) void k_eo_scan(const EnsOrd *__restrict__ O, long *__restrict__ nmov)
+{
+    const EnsOrd &o = O[blockIdx.x];
+    if (!o.active) {
+        if (threadIdx.x == 0) nmov[blockIdx.x] = 0;
+        return;
+    }
+    order_scan_block(o.tile_count, o.ntiles, o.tile_off, &nmov[blockIdx.x]);
+}
+
+// base[k] = first slot of model k's movers; base[nm] = all movers sorted.
+__global__ void k_eo_bases(const EnsOrd *__restrict__ O, const long *__restrict__ nmov, int nm, long *__restrict__ base)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    long run = 0;
+    for (int k = 0; k < nm; k++) {
+        base[k] = run;
+        if (eo_incremental(O[k], nmov[k])) run += nmov[k];
+    }
+    base[nm] = run;
+}
+
+__global__ __launch_bounds__(ORD_BLOCK) void k_eo_movers(const EnsOrd *__restrict__ O, const EnsTile *__restrict__ tiles,
+                                                         const long *__restrict__ nmov, const long *__restrict__ base,
+                                                         uint64_t *__restrict__ mov, int shift)
+{
+    const EnsTile T = tiles[blockIdx.x];
+    const EnsOrd &o = O[T.model];
+    if (eo_incremental(o, nmov[T.model]))
+        order_movers_tile(o.order, o.key, o.n, o.mask, o.tile_off, mov + base[T.model], (uint64_t)T.model << shift,
+                          T.begin);
+}
+
+__global__ __launch_bounds__(ORD_BLOCK) void k_eo_stayers(const EnsOrd *__restrict__ O, const EnsTile *__restrict__ tiles,
+                                                          const long *__restrict__ nmov, const long *__restrict__ base,
+                                                          const uint64_t *__restrict__ mov, int shift)
+{
+    const EnsTile T = tiles[blockIdx.x];
+    const EnsOrd &o = O[T.model];
+    const long nm_k = nmov[T.model];
+    if (eo_incremental(o, nm_k))
+        order_stayers_tile(o.order, o.skey, o.n, o.mask, o.tile_off, mov + base[T.model], nm_k,
+                           (uint64_t)T.model << shift, o.order2, o.skey2, T.begin);
+}
+
+__global__ void k_eo_place(const EnsOrd *__restrict__ O, const uint64_t *__restrict__ mov,
+                           const long *__restrict__ base, long total, int shift)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= total) return;
+    const uint64_t v = mov[r];
+    const int k = (int)(v >> shift);
+    const EnsOrd &o = O[k];
+    order_place_one(v & ((1ull << shift) - 1ull), r - base[k], o.order, o.skey, o.n, o.mask, o.tile_off, o.order2,
+                    o.skey2);
+}
+
+// Sub-cell starts of the new order over the marker entries (sorted
+// positions); the last position also writes the tail up to nsub.
+__global__ __launch_bounds__(ENS_BLOCK) void k_eo_sub_start(const EnsOrd *__restrict__ O,
+                                                            const EnsTile *__restrict__ tiles,
+                                                            const long *__restrict__ nmov)
+{
+    const EnsTile T = tiles[blockIdx.x];
+    const EnsOrd &o = O[T.model];
+    if (!eo_incremental(o, nmov[T.model])) return;
+    const long i = T.begin + threadIdx.x;
+    if (i < T.end) {
+        sub_start_at(o.skey2, o.n, o.nsub, o.sub_start, i);
+        if (i == o.n - 1) sub_start_at(o.skey2, o.n, o.nsub, o.sub_start, o.n);
+    }
+}
+
 }  // namespace v2c
 
 using namespace v2c;
 
 // Work lists of one shape assignment.
 struct EnsPlan {
-    std::vector<EnsTile> markers, cells, nodes;
+    std::vector<EnsTile> markers, cells, nodes, otiles;
 };
 
 struct v2c_ens {
@@ -198,6 +303,17 @@ struct v2c_ens {
     double *d_rms = nullptr;
     int *d_flags = nullptr;                  // [0] unsupported marker types
     std::vector<hipEvent_t> ordered;         // per model: its ordering is done
+    std::vector<long> plan_n;                // marker counts the work lists were cut for
+    // batched re-ordering
+    bool batch_order = false;                // packed mover values fit in 64 bits
+    int mov_shift = 0, mov_bits = 0;         // model number at bit mov_shift; sort bits
+    int n_otile = 0;
+    EnsTile *t_otile = nullptr;
+    EnsOrd *d_ord = nullptr, *h_ord = nullptr;
+    long *d_nmov = nullptr, *d_base = nullptr, *h_counts = nullptr;
+    uint64_t *d_mov_in = nullptr, *d_mov_out = nullptr;
+    void *d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_valid = false;
@@ -286,6 +402,7 @@ static EnsPlan ens_plan(const v2c_ens *e)
         ens_cut_runs(nruns, mod, c->nodenum, s.nodes_per_tile / ENS_BLOCK);
         for (long b = 0; b < c->ncell; b += s.cells_per_group)
             p.cells.push_back(EnsTile{mod, 0, b, std::min(c->ncell, b + (long)s.cells_per_group)});
+        for (long t = 0; t < (c->n + ORD_TILE - 1) / ORD_TILE; t++) p.otiles.push_back(EnsTile{mod, 0, t, t + 1});
     }
     ens_item_list(p.markers, mruns);
     ens_item_list(p.nodes, nruns);
@@ -295,7 +412,7 @@ static EnsPlan ens_plan(const v2c_ens *e)
 static int ens_upload_plan(v2c_ens *e)
 {
     const EnsPlan p = ens_plan(e);
-    const long total = (long)(p.markers.size() + p.cells.size() + p.nodes.size());
+    const long total = (long)(p.markers.size() + p.cells.size() + p.nodes.size() + p.otiles.size());
     if (total > e->tile_cap) {
         if (e->d_tiles) (void)hipFree(e->d_tiles);
         e->d_tiles = nullptr;
@@ -307,15 +424,19 @@ static int ens_upload_plan(v2c_ens *e)
     all.insert(all.end(), p.markers.begin(), p.markers.end());
     all.insert(all.end(), p.cells.begin(), p.cells.end());
     all.insert(all.end(), p.nodes.begin(), p.nodes.end());
+    all.insert(all.end(), p.otiles.begin(), p.otiles.end());
     if (total)
         ENSCHK(e, hipMemcpyAsync(e->d_tiles, all.data(), sizeof(EnsTile) * total, hipMemcpyHostToDevice, e->stream));
     ENSCHK(e, hipStreamSynchronize(e->stream));
     e->n_mark = (int)p.markers.size();
     e->n_cell = (int)p.cells.size();
     e->n_node = (int)p.nodes.size();
+    e->n_otile = (int)p.otiles.size();
     e->t_mark = e->d_tiles;
     e->t_cell = e->t_mark + e->n_mark;
     e->t_node = e->t_cell + e->n_cell;
+    e->t_otile = e->t_node + e->n_node;
+    for (size_t k = 0; k < e->models.size(); k++) e->plan_n[k] = e->models[k]->n;
     e->graph_valid = false;
     return 0;
 }
@@ -356,13 +477,13 @@ static void ens_stages(v2c_ens *e)
         hipLaunchKernelGGL(k_ens_n2m_update, mg, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_mark, e->d_flags);
 }
 
-// Per-step preparation outside the graph: cell ordering of every model (it
-// may choose a full sort on the host) and the argument array.  Every model
-// orders on its own stream; the first halves (flagging and the mover count)
-// of all models are launched before any count is read back, so the small
-// ordering kernels of different models overlap.  The ensemble stream then
-// waits for each model's ordering through an event.
-static int ens_prepare(v2c_ens *e)
+// Per-step preparation outside the graph: cell ordering of every model and
+// the argument array.  Without batched re-ordering every model orders on its
+// own stream; the first halves (flagging and the mover count) of all models
+// are launched before any count is read back, so the small ordering kernels
+// of different models overlap.  The ensemble stream then waits for each
+// model's ordering through an event.
+static int ens_order_streams(v2c_ens *e)
 {
     const size_t nm = e->models.size();
     std::vector<char> pending(nm, 0);
@@ -379,10 +500,107 @@ static int ens_prepare(v2c_ens *e)
             int rc = order_finish(c);
             if (rc) return ens_err(e, rc, "model %zu: ordering failed: %s", k, c->err);
         }
-        e->h_args[k] = make_args(c);
         ENSCHK(e, hipEventRecord(e->ordered[k], c->stream));
         ENSCHK(e, hipStreamWaitEvent(e->stream, e->ordered[k], 0));
     }
+    return 0;
+}
+
+// Batched re-ordering on the ensemble stream (see k_eo_flags and the
+// following kernels); models sorted for the first time, empty ones and those
+// with more than n/4 movers get a full sort on their own stream.
+static int ens_order_batched(v2c_ens *e)
+{
+    const int nm = (int)e->models.size();
+    auto full = [&](int k) -> int {
+        v2c_ctx *c = e->models[k];
+        int rc = full_sort(c);
+        if (rc) return ens_err(e, rc, "model %d: sort failed: %s", k, c->err);
+        c->order_valid = true;
+        ENSCHK(e, hipEventRecord(e->ordered[k], c->stream));
+        ENSCHK(e, hipStreamWaitEvent(e->stream, e->ordered[k], 0));
+        return 0;
+    };
+    bool any = false;
+    for (int k = 0; k < nm; k++) {
+        v2c_ctx *c = e->models[k];
+        EnsOrd &o = e->h_ord[k];
+        setp(o.key, c->d_key);
+        setp(o.order, c->d_order); setp(o.skey, c->d_skey);
+        setp(o.order2, c->d_order2); setp(o.skey2, c->d_skey2);
+        setp(o.mask, c->d_mask);
+        setp(o.tile_count, c->d_tile_count); setp(o.tile_off, c->d_tile_off); setp(o.sub_start, c->d_cs);
+        o.n = c->n;
+        o.ntiles = (c->n + ORD_TILE - 1) / ORD_TILE;
+        o.nsub = 4 * c->ncell;
+        o.active = 0;
+        if (c->order_valid) continue;
+        if (!c->ever_sorted || c->n == 0) {
+            c->last_movers = c->n;
+            int rc = full(k);
+            if (rc) return rc;
+            continue;
+        }
+        o.active = 1;
+        any = true;
+    }
+    if (!any) return 0;
+    ENSCHK(e, hipMemcpyAsync(e->d_ord, e->h_ord, sizeof(EnsOrd) * nm, hipMemcpyHostToDevice, e->stream));
+    if (e->n_otile) hipLaunchKernelGGL(k_eo_flags, dim3(e->n_otile), dim3(ORD_BLOCK), 0, e->stream, e->d_ord, e->t_otile);
+    hipLaunchKernelGGL(k_eo_scan, dim3(nm), dim3(1024), 0, e->stream, e->d_ord, e->d_nmov);
+    hipLaunchKernelGGL(k_eo_bases, dim3(1), dim3(64), 0, e->stream, e->d_ord, e->d_nmov, nm, e->d_base);
+    ENSCHK(e, hipGetLastError());
+    ENSCHK(e, hipMemcpyAsync(e->h_counts, e->d_nmov, sizeof(long) * nm, hipMemcpyDeviceToHost, e->stream));
+    ENSCHK(e, hipMemcpyAsync(e->h_counts + nm, e->d_base + nm, sizeof(long), hipMemcpyDeviceToHost, e->stream));
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    const long total = e->h_counts[nm];
+    if (total > 0) {
+        hipLaunchKernelGGL(k_eo_movers, dim3(e->n_otile), dim3(ORD_BLOCK), 0, e->stream, e->d_ord, e->t_otile,
+                           e->d_nmov, e->d_base, e->d_mov_in, e->mov_shift);
+        ENSCHK(e, hipcub::DeviceRadixSort::SortKeys(e->d_sort_tmp, e->sort_tmp_bytes, e->d_mov_in, e->d_mov_out,
+                                                    (int)total, 0, e->mov_bits, e->stream));
+        hipLaunchKernelGGL(k_eo_stayers, dim3(e->n_otile), dim3(ORD_BLOCK), 0, e->stream, e->d_ord, e->t_otile,
+                           e->d_nmov, e->d_base, e->d_mov_out, e->mov_shift);
+        hipLaunchKernelGGL(k_eo_place, dim3(blocks_for(total, 256)), dim3(256), 0, e->stream, e->d_ord, e->d_mov_out,
+                           e->d_base, total, e->mov_shift);
+        if (e->n_mark)
+            hipLaunchKernelGGL(k_eo_sub_start, dim3(e->n_mark), dim3(ENS_BLOCK), 0, e->stream, e->d_ord, e->t_mark,
+                               e->d_nmov);
+        ENSCHK(e, hipGetLastError());
+    }
+    for (int k = 0; k < nm; k++) {
+        v2c_ctx *c = e->models[k];
+        const EnsOrd &o = e->h_ord[k];
+        if (!o.active) continue;
+        const long nmov = e->h_counts[k];
+        c->last_movers = nmov;
+        if (eo_incremental(o, nmov)) {
+            std::swap(c->d_order, c->d_order2);
+            std::swap(c->d_skey, c->d_skey2);
+            c->n_incremental++;
+            c->order_valid = true;
+        } else if (nmov == 0) {
+            c->order_valid = true;
+        } else {
+            int rc = full(k);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+static int ens_prepare(v2c_ens *e)
+{
+    const size_t nm = e->models.size();
+    for (size_t k = 0; k < nm; k++)
+        if (e->models[k]->n != e->plan_n[k]) {
+            int rc = ens_upload_plan(e);     // marker counts changed: new work lists
+            if (rc) return rc;
+            break;
+        }
+    int rc = e->batch_order ? ens_order_batched(e) : ens_order_streams(e);
+    if (rc) return rc;
+    for (size_t k = 0; k < nm; k++) e->h_args[k] = make_args(e->models[k]);
     ENSCHK(e, hipMemcpyAsync(e->d_args, e->h_args, sizeof(KArgs) * nm, hipMemcpyHostToDevice, e->stream));
     return 0;
 }
@@ -403,8 +621,11 @@ int v2c_ens_create(v2c_ens **out, v2c_ctx *const *models, int nmodels)
     if (!out || !models || nmodels <= 0) return V2C_EINVAL;
     *out = nullptr;
     const int dev = models[0]->device;
-    for (int k = 0; k < nmodels; k++)
+    for (int k = 0; k < nmodels; k++) {
         if (!models[k] || models[k]->device != dev) return V2C_EINVAL;
+        for (int j = 0; j < k; j++)
+            if (models[j] == models[k]) return V2C_EINVAL;      // each context once
+    }
     v2c_ens *e = new v2c_ens();
     e->device = dev;
     for (int k = 0; k < nmodels; k++) {
@@ -439,6 +660,32 @@ int v2c_ens_create(v2c_ens **out, v2c_ctx *const *models, int nmodels)
     if (hipMemcpy(e->d_red, reds.data(), sizeof(double *) * nmodels, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(e->d_redn, nred.data(), sizeof(int) * nmodels, hipMemcpyHostToDevice) != hipSuccess)
         return fail(V2C_EHIP);
+    // batched re-ordering: model number above the widest sub-cell key
+    int kb = 1, mb = 0;
+    long capsum = 0;
+    for (int k = 0; k < nmodels; k++) {
+        kb = std::max(kb, models[k]->key_bits);
+        capsum += models[k]->cap;
+    }
+    while ((1 << mb) < nmodels) mb++;
+    e->mov_shift = 32 + kb;
+    e->mov_bits = e->mov_shift + mb;
+    e->batch_order = e->mov_bits <= 64 && capsum < (1L << 31);
+    if (e->batch_order) {
+        if (hipMalloc(&e->d_ord, sizeof(EnsOrd) * nmodels) != hipSuccess ||
+            hipHostMalloc((void **)&e->h_ord, sizeof(EnsOrd) * nmodels, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&e->d_nmov, sizeof(long) * nmodels) != hipSuccess ||
+            hipMalloc(&e->d_base, sizeof(long) * (nmodels + 1)) != hipSuccess ||
+            hipHostMalloc((void **)&e->h_counts, sizeof(long) * (nmodels + 1), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&e->d_mov_in, sizeof(uint64_t) * std::max(capsum, 1L)) != hipSuccess ||
+            hipMalloc(&e->d_mov_out, sizeof(uint64_t) * std::max(capsum, 1L)) != hipSuccess)
+            return fail(V2C_ENOMEM);
+        if (hipcub::DeviceRadixSort::SortKeys(nullptr, e->sort_tmp_bytes, e->d_mov_in, e->d_mov_out,
+                                              (int)std::max(capsum, 1L), 0, e->mov_bits, e->stream) != hipSuccess)
+            return fail(V2C_EHIP);
+        if (hipMalloc(&e->d_sort_tmp, std::max(e->sort_tmp_bytes, (size_t)16)) != hipSuccess) return fail(V2C_ENOMEM);
+    }
+    e->plan_n.assign(nmodels, -1);
     if (ens_upload_plan(e)) return fail(V2C_EHIP);
     *out = e;
     return 0;
@@ -449,10 +696,13 @@ void v2c_ens_destroy(v2c_ens *e)
     if (!e) return;
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
-    void *ptrs[] = {e->d_args, e->d_tiles, e->d_red, e->d_redn, e->d_rms, e->d_flags};
+    void *ptrs[] = {e->d_args, e->d_tiles, e->d_red, e->d_redn, e->d_rms, e->d_flags, e->d_ord,
+                    e->d_nmov, e->d_base, e->d_mov_in, e->d_mov_out, e->d_sort_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (e->h_args) (void)hipHostFree(e->h_args);
+    if (e->h_ord) (void)hipHostFree(e->h_ord);
+    if (e->h_counts) (void)hipHostFree(e->h_counts);
     for (auto ev : e->ordered)
         if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -523,6 +773,17 @@ int v2c_ens_step(v2c_ens *e, int use_graph)
     ENSCHK(e, hipMemcpy(&bad, e->d_flags, sizeof(int), hipMemcpyDeviceToHost));
     ens_finish_step(e);
     if (bad) return ens_err(e, V2C_EUNSUPPORTED, "%d markers of type >= 100 (immobile spawning is out of scope)", bad);
+    return 0;
+}
+
+int v2c_ens_order(v2c_ens *e)
+{
+    if (!e) return V2C_EINVAL;
+    ENSCHK(e, hipSetDevice(e->device));
+    int rc = e->batch_order ? ens_order_batched(e) : ens_order_streams(e);
+    if (rc) return rc;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    for (auto *c : e->models) ENSCHK(e, hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -68,13 +68,19 @@ __global__ void k_iota(uint32_t *v, long n)
 }
 
 // ss[k] = first sorted position whose sub-cell key is >= k; ss[nsub] = n.
-__global__ void k_sub_start(const uint32_t *__restrict__ skey, long n, long nsub, uint32_t *ss)
+// Position i (0 <= i <= n) writes the entries between its predecessor's key
+// and its own.
+__device__ __forceinline__ void sub_start_at(const uint32_t *__restrict__ skey, long n, long nsub, uint32_t *ss, long i)
 {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
     long cur = i < n ? (long)skey[i] : nsub;
     long prev = i > 0 ? (long)skey[i - 1] : -1;
     for (long k = prev + 1; k <= cur && k <= nsub; k++) ss[k] = (uint32_t)i;
+}
+
+__global__ void k_sub_start(const uint32_t *__restrict__ skey, long n, long nsub, uint32_t *ss)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) sub_start_at(skey, n, nsub, ss, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -89,22 +95,26 @@ __global__ void k_sub_start(const uint32_t *__restrict__ skey, long n, long nsub
 // The old sorted positions are cut into tiles of ORD_TILE; a 64-bit ballot
 // mask per 64 positions marks the movers, and an exclusive scan of the tile
 // counts gives every position's mover prefix without a global compaction.
+//
+// Movers are sorted as 64-bit values (key << 32 | id), optionally with
+// constant high bits `hi` above the key (a batched launch sorts the movers
+// of several models at once, with the model number there); comparisons
+// against stayers carry the same bits, and `low` masks them off again.
 // ---------------------------------------------------------------------------
 constexpr int ORD_BLOCK = 256, ORD_ITEMS = 16, ORD_TILE = ORD_BLOCK * ORD_ITEMS;
 constexpr int ORD_MASKS = ORD_TILE / WAVE;        // masks per tile
 
 __device__ __forceinline__ uint64_t kid(uint32_t k, uint32_t id) { return ((uint64_t)k << 32) | id; }
 
-// Pass 1: mover masks and mover count per tile.
-__global__ __launch_bounds__(ORD_BLOCK) void k_order_flags(const uint32_t *__restrict__ order,
-                                                           const uint32_t *__restrict__ key,
-                                                           const uint32_t *__restrict__ skey, long n,
-                                                           unsigned long long *__restrict__ mask,
-                                                           uint32_t *__restrict__ tile_count)
+// Pass 1: mover masks and mover count of one tile.
+__device__ __forceinline__ void order_flags_tile(const uint32_t *__restrict__ order, const uint32_t *__restrict__ key,
+                                                 const uint32_t *__restrict__ skey, long n,
+                                                 unsigned long long *__restrict__ mask,
+                                                 uint32_t *__restrict__ tile_count, long tile)
 {
     __shared__ uint32_t wsum[ORD_BLOCK / WAVE];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const long t0 = (long)blockIdx.x * ORD_TILE;
+    const long t0 = tile * ORD_TILE;
     uint32_t cnt = 0;
     for (int k = 0; k < ORD_ITEMS; k++) {
         const long i = t0 + (long)k * ORD_BLOCK + threadIdx.x;
@@ -118,13 +128,23 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_flags(const uint32_t *__res
     if (threadIdx.x == 0) {
         uint32_t t = 0;
         for (int w = 0; w < ORD_BLOCK / WAVE; w++) t += wsum[w];
-        tile_count[blockIdx.x] = t;
+        tile_count[tile] = t;
     }
 }
 
-// Pass 2: exclusive scan of the tile counts (one workgroup); off[ntiles] = total.
-__global__ __launch_bounds__(1024) void k_order_scan(const uint32_t *__restrict__ cnt, long ntiles,
-                                                     uint32_t *__restrict__ off, long *__restrict__ total)
+__global__ __launch_bounds__(ORD_BLOCK) void k_order_flags(const uint32_t *__restrict__ order,
+                                                           const uint32_t *__restrict__ key,
+                                                           const uint32_t *__restrict__ skey, long n,
+                                                           unsigned long long *__restrict__ mask,
+                                                           uint32_t *__restrict__ tile_count)
+{
+    order_flags_tile(order, key, skey, n, mask, tile_count, blockIdx.x);
+}
+
+// Pass 2: exclusive scan of the tile counts (one 1024-thread workgroup);
+// off[ntiles] = total.
+__device__ __forceinline__ void order_scan_block(const uint32_t *__restrict__ cnt, long ntiles,
+                                                 uint32_t *__restrict__ off, long *__restrict__ total)
 {
     __shared__ uint32_t part[1024];
     const long per = (ntiles + 1023) / 1024;
@@ -144,6 +164,12 @@ __global__ __launch_bounds__(1024) void k_order_scan(const uint32_t *__restrict_
     if (threadIdx.x == 1023) { off[ntiles] = part[1023]; *total = part[1023]; }
 }
 
+__global__ __launch_bounds__(1024) void k_order_scan(const uint32_t *__restrict__ cnt, long ntiles,
+                                                     uint32_t *__restrict__ off, long *__restrict__ total)
+{
+    order_scan_block(cnt, ntiles, off, total);
+}
+
 // Movers before old position i: tile offset + masks of the tile before i's
 // mask + the bits of i's mask below i.
 __device__ __forceinline__ uint32_t movers_before(const unsigned long long *__restrict__ mask,
@@ -156,18 +182,18 @@ __device__ __forceinline__ uint32_t movers_before(const unsigned long long *__re
     return c + (uint32_t)__popcll(mask[w] & ((1ull << bit) - 1ull));
 }
 
-// Pass 3: movers' (key, id) compacted in old-position order.
-__global__ __launch_bounds__(ORD_BLOCK) void k_order_movers(const uint32_t *__restrict__ order,
-                                                            const uint32_t *__restrict__ key, long n,
-                                                            const unsigned long long *__restrict__ mask,
-                                                            const uint32_t *__restrict__ off,
-                                                            uint64_t *__restrict__ mov)
+// Pass 3: the tile's movers' (key, id) | hi, compacted in old-position order.
+__device__ __forceinline__ void order_movers_tile(const uint32_t *__restrict__ order,
+                                                  const uint32_t *__restrict__ key, long n,
+                                                  const unsigned long long *__restrict__ mask,
+                                                  const uint32_t *__restrict__ off, uint64_t *__restrict__ mov,
+                                                  uint64_t hi, long tile)
 {
     __shared__ uint32_t wpre[ORD_BLOCK / WAVE + 1];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const long t0 = (long)blockIdx.x * ORD_TILE;
-    uint32_t base = off[blockIdx.x];
-    if (off[blockIdx.x + 1] == base) return;      // no movers in this tile
+    const long t0 = tile * ORD_TILE;
+    uint32_t base = off[tile];
+    if (off[tile + 1] == base) return;            // no movers in this tile
     for (int k = 0; k < ORD_ITEMS; k++) {
         const long c0 = t0 + (long)k * ORD_BLOCK;
         if (c0 >= n) break;
@@ -183,11 +209,20 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_movers(const uint32_t *__re
         __syncthreads();
         if ((b >> lane) & 1ull) {
             const uint32_t id = order[i];
-            mov[base + wpre[wv] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = kid(key[id], id);
+            mov[base + wpre[wv] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = kid(key[id], id) | hi;
         }
         base += wpre[ORD_BLOCK / WAVE];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(ORD_BLOCK) void k_order_movers(const uint32_t *__restrict__ order,
+                                                            const uint32_t *__restrict__ key, long n,
+                                                            const unsigned long long *__restrict__ mask,
+                                                            const uint32_t *__restrict__ off,
+                                                            uint64_t *__restrict__ mov)
+{
+    order_movers_tile(order, key, n, mask, off, mov, 0ull, blockIdx.x);
 }
 
 __device__ __forceinline__ long lower_bound64(const uint64_t *__restrict__ a, long lo, long hi, uint64_t c)
@@ -199,29 +234,29 @@ __device__ __forceinline__ long lower_bound64(const uint64_t *__restrict__ a, lo
     return lo;
 }
 
-// Pass 5: stayers to their new positions.  The movers that can precede any
-// stayer of the tile after the first ones are confined to [lo, hi), found
-// once per tile; each stayer counts within that window.
-__global__ __launch_bounds__(ORD_BLOCK) void k_order_stayers(const uint32_t *__restrict__ order,
-                                                             const uint32_t *__restrict__ skey, long n,
-                                                             const unsigned long long *__restrict__ mask,
-                                                             const uint32_t *__restrict__ off,
-                                                             const uint64_t *__restrict__ mov, long nmov,
-                                                             uint32_t *__restrict__ order_out,
-                                                             uint32_t *__restrict__ skey_out)
+// Pass 5: the tile's stayers to their new positions.  The movers that can
+// precede any stayer of the tile after the first ones are confined to
+// [lo, hi), found once per tile; each stayer counts within that window.
+__device__ __forceinline__ void order_stayers_tile(const uint32_t *__restrict__ order,
+                                                   const uint32_t *__restrict__ skey, long n,
+                                                   const unsigned long long *__restrict__ mask,
+                                                   const uint32_t *__restrict__ off,
+                                                   const uint64_t *__restrict__ mov, long nmov, uint64_t hib,
+                                                   uint32_t *__restrict__ order_out,
+                                                   uint32_t *__restrict__ skey_out, long tile)
 {
     __shared__ long win[2];
     __shared__ uint32_t wpre[ORD_BLOCK / WAVE + 1];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    const long t0 = (long)blockIdx.x * ORD_TILE;
+    const long t0 = tile * ORD_TILE;
     const long t1 = t0 + ORD_TILE < n ? t0 + ORD_TILE : n;
     if (threadIdx.x == 0) {
         // stayer keys of this tile lie in [skey[t0], skey[t1-1]]
-        win[0] = lower_bound64(mov, 0, nmov, kid(skey[t0], 0u));
-        win[1] = lower_bound64(mov, win[0], nmov, kid(skey[t1 - 1], 0xffffffffu) + 1ull);
+        win[0] = lower_bound64(mov, 0, nmov, kid(skey[t0], 0u) | hib);
+        win[1] = lower_bound64(mov, win[0], nmov, (kid(skey[t1 - 1], 0xffffffffu) | hib) + 1ull);
         if (skey[t1 - 1] == 0xffffffffu) win[1] = nmov;
     }
-    uint32_t moved = off[blockIdx.x];               // movers before the current chunk
+    uint32_t moved = off[tile];                     // movers before the current chunk
     __syncthreads();
     const long lo = win[0], hi = win[1];
     for (int k = 0; k < ORD_ITEMS; k++) {
@@ -239,7 +274,7 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_stayers(const uint32_t *__r
         __syncthreads();
         if (i < n && !((b >> lane) & 1ull)) {
             const uint32_t id = order[i], k32 = skey[i];
-            const uint64_t c = kid(k32, id);
+            const uint64_t c = kid(k32, id) | hib;
             const uint32_t mb = moved + wpre[wv] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
             long before;
             if (hi - lo <= 8) {
@@ -257,18 +292,27 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_stayers(const uint32_t *__r
     }
 }
 
-// Pass 6: movers to their new positions.  Stayers before a mover = entries of
-// the old order below it (binary search; the old order is sorted by the keys
-// it was built with, and stayers kept theirs) minus the movers among them.
-__global__ void k_order_place_movers(const uint64_t *__restrict__ mov, long nmov,
-                                     const uint32_t *__restrict__ order, const uint32_t *__restrict__ skey, long n,
-                                     const unsigned long long *__restrict__ mask,
-                                     const uint32_t *__restrict__ off, uint32_t *__restrict__ order_out,
-                                     uint32_t *__restrict__ skey_out)
+__global__ __launch_bounds__(ORD_BLOCK) void k_order_stayers(const uint32_t *__restrict__ order,
+                                                             const uint32_t *__restrict__ skey, long n,
+                                                             const unsigned long long *__restrict__ mask,
+                                                             const uint32_t *__restrict__ off,
+                                                             const uint64_t *__restrict__ mov, long nmov,
+                                                             uint32_t *__restrict__ order_out,
+                                                             uint32_t *__restrict__ skey_out)
 {
-    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nmov) return;
-    const uint64_t c = mov[r];
+    order_stayers_tile(order, skey, n, mask, off, mov, nmov, 0ull, order_out, skey_out, blockIdx.x);
+}
+
+// Pass 6: mover r (value c, high bits removed by `low`) to its new position.
+// Stayers before a mover = entries of the old order below it (binary search;
+// the old order is sorted by the keys it was built with, and stayers kept
+// theirs) minus the movers among them.
+__device__ __forceinline__ void order_place_one(uint64_t c, long r, const uint32_t *__restrict__ order,
+                                                const uint32_t *__restrict__ skey, long n,
+                                                const unsigned long long *__restrict__ mask,
+                                                const uint32_t *__restrict__ off, uint32_t *__restrict__ order_out,
+                                                uint32_t *__restrict__ skey_out)
+{
     long lo = 0, hi = n;
     while (lo < hi) {
         const long mid = (lo + hi) >> 1;
@@ -278,6 +322,16 @@ __global__ void k_order_place_movers(const uint64_t *__restrict__ mov, long nmov
     const long pos = r + stay_before;
     order_out[pos] = (uint32_t)(c & 0xffffffffu);
     skey_out[pos] = (uint32_t)(c >> 32);
+}
+
+__global__ void k_order_place_movers(const uint64_t *__restrict__ mov, long nmov,
+                                     const uint32_t *__restrict__ order, const uint32_t *__restrict__ skey, long n,
+                                     const unsigned long long *__restrict__ mask,
+                                     const uint32_t *__restrict__ off, uint32_t *__restrict__ order_out,
+                                     uint32_t *__restrict__ skey_out)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nmov) order_place_one(mov[r], r, order, skey, n, mask, off, order_out, skey_out);
 }
 
 // ---------------------------------------------------------------------------
