@@ -51,6 +51,17 @@ ALG = {
 B_STEP_MARKER = 161 + 113
 B_STEP_NODE = 360
 
+# Nodal fields the reference-side binding moves per step when the host keeps
+# the nodal arrays (marker2nodes 24 in / 43 out, nodes2markers 28 / 9,
+# moveandrotatemarkers 5 / 0, before solve 22 / 6, after solve 19 / 3).
+PCIE_H2D_FIELDS = 24 + 28 + 5 + 22 + 19
+PCIE_D2H_FIELDS = 43 + 9 + 6 + 3
+
+# FP64 vector issue: one 64-lane wavefront instruction per SIMD every 4
+# cycles (16 FP64 lanes per SIMD); 256 CUs x 4 SIMDs at 2.4 GHz.  Quarter-rate
+# transcendentals (rcp, rsq, sqrt of f64) take 16 cycles.
+SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
+
 
 def cpu_baseline(cfg, sample: int, steps: int = 2):
     """FP64 oracle, one thread, full step on the first `sample` markers of the
@@ -76,6 +87,48 @@ def cpu_baseline(cfg, sample: int, steps: int = 2):
             "sample": f"oracle (FP64 C restatement, 1 thread) on the first {sample:,} markers of the "
                       f"same cloud with the full {cfg.grid.nodenum:,}-node grid, {steps} full steps, "
                       f"{el:.1f} s"}
+
+
+def pcie_inclusive(ctx, cfg, ms_step):
+    """Step time if the host keeps the nodal arrays (reference-side binding):
+    the device step plus the per-step nodal transfers, each field timed as a
+    synchronous copy of the reference's pageable host array."""
+    names = list(cfg.nodal)
+    host = {k: np.ascontiguousarray(cfg.nodal[k]) for k in names}
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in names:
+        ctx.set_nodal(k, host[k])
+    t_h2d = (time.perf_counter() - t0) / len(names)
+    t0 = time.perf_counter()
+    for k in names:
+        ctx.get_nodal(k)
+    t_d2h = (time.perf_counter() - t0) / len(names)
+    ms = ms_step + (PCIE_H2D_FIELDS * t_h2d + PCIE_D2H_FIELDS * t_d2h) * 1e3
+    return {"ms_per_step": ms, "value": cfg.n / (ms * 1e-3), "unit": "marker-steps/s",
+            "h2d_fields": PCIE_H2D_FIELDS, "d2h_fields": PCIE_D2H_FIELDS,
+            "ms_per_field_h2d": t_h2d * 1e3, "ms_per_field_d2h": t_d2h * 1e3,
+            "field_mb": cfg.grid.nodenum * 8 / 1e6}
+
+
+def valu_issue(workload, kern):
+    """Per-kernel FP64-VALU issue bound from the committed counter profile
+    (profiles/valu.json: wavefront VALU and f64-transcendental instructions
+    per launch, from rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_VALU_TRANS_F64 on
+    this workload) against the kernel times measured here."""
+    path = os.path.join(REPO, "profiles", "valu.json")
+    if not os.path.exists(path):
+        return None
+    prof = json.load(open(path)).get(workload, {})
+    out = {}
+    for k, ent in prof.items():
+        if k not in kern:
+            continue
+        cycles = 4.0 * ent["valu_per_launch"] + 12.0 * ent.get("trans_f64_per_launch", 0.0)
+        bound_ms = cycles / SIMDS / CLOCK_HZ * 1e3
+        ms = kern[k]["ms_total"] / max(kern[k]["launches"], 1)
+        out[k] = {"issue_bound_ms": bound_ms, "avg_launch_ms": ms, "frac": bound_ms / ms}
+    return out or None
 
 
 def warm_start_eval(lib, dt: float):
@@ -122,7 +175,7 @@ def ensemble_bench(args):
     lib = load_lib()
     ctxs, n_total, nodes_total, sizes = [], 0, 0, []
     t0 = time.perf_counter()
-    for k, cfg in synth.config_ensemble(args.models):
+    for k, cfg in synth.config_ensemble(args.models, small=args.small_models):
         ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=lib)
         ctx.set_time(cfg.timestep, cfg.timestep)
         ctx.set_all_nodal(cfg.nodal)
@@ -163,9 +216,11 @@ def ensemble_bench(args):
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": best * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded uneven models, analytic corner-flow nodal fields)",
-           "config": {"workload": f"ensemble-{args.models}", "models": args.models, "markers": n_total,
+           "config": {"workload": f"ensemble-{args.models}" + ("-small" if args.small_models else ""),
+                      "models": args.models, "markers": n_total,
                       "nodes": nodes_total, "markers_min": min(sizes), "markers_max": max(sizes),
-                      "grids": "201x51 .. 801x201 nodes, 4-36 markers per cell", "dt_s": synth.YEAR},
+                      "grids": ("41x21 .. 161x61" if args.small_models else "201x51 .. 801x201")
+                      + " nodes, 4-36 markers per cell", "dt_s": synth.YEAR},
            "batched_graph_default_ms": t_default * 1e3, "batched_graph_tuned_ms": t_tuned * 1e3,
            "looped_single_model_ms": t_looped * 1e3,
            "tuned_graph_only_ms": t_graph_only, "speedup_vs_looped": t_looped / best,
@@ -185,6 +240,7 @@ def main():
     ap.add_argument("--ensemble", action="store_true", help="batched ensemble workload instead")
     ap.add_argument("--models", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="ensemble: skip the autotuner")
+    ap.add_argument("--small-models", action="store_true", help="ensemble: 41x21 .. 161x61-node models")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -244,6 +300,8 @@ def main():
     per_launch_bytes = n * ALG[dom][0] + nn * ALG[dom][1]
     avg_ms = kern[dom]["ms_total"] / max(kern[dom]["launches"], 1)
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    pcie = pcie_inclusive(ctx, cfg, ms_step)
+    valu = valu_issue(cfg.name, kern)
     traffic = None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -264,7 +322,10 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms},
         "kernels_ms_per_step": {k: v["ms_per_step"] for k, v in kern.items()},
+        "pcie_inclusive": pcie,
     }
+    if valu:
+        out["valu_issue"] = valu
     ctx.close()
     if not args.no_warm:
         out["warm_start"] = warm_start_eval(lib, synth.YEAR)

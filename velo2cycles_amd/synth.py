@@ -401,16 +401,17 @@ def upload(ctx: Context, cfg: Config):
     ctx.upload_markers(cfg.markers)
 
 
-def config_ensemble(n_models=64, seed=DEFAULT_SEED):
+def config_ensemble(n_models=64, seed=DEFAULT_SEED, small=False):
     """Batched-ensemble workload: independent models with grids drawn from
     201x51 to 801x201 nodes (uniform 1 km spacing), 4 to 36 markers per cell,
     a-b of the velocity-weakening zone ~ U(-0.006, 0.004) and convergence
-    2 to 8 cm/yr.  Yields (index, Config) one model at a time, so the host
-    holds one marker cloud at once."""
+    2 to 8 cm/yr.  small=True draws 41x21 to 161x61 nodes instead (many small
+    models, where per-model launches dominate).  Yields (index, Config) one
+    model at a time, so the host holds one marker cloud at once."""
     rng = np.random.default_rng(seed)
     for k in range(n_models):
-        nx = int(rng.integers(201, 802))
-        ny = int(rng.integers(51, 202))
+        nx = int(rng.integers(41, 162)) if small else int(rng.integers(201, 802))
+        ny = int(rng.integers(21, 62)) if small else int(rng.integers(51, 202))
         per = int(rng.integers(2, 7))
         conv = rng.uniform(2.0, 8.0) * CM_PER_YR
         amb = rng.uniform(-0.006, 0.004)
