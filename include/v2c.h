@@ -150,6 +150,13 @@ enum v2c_nodal_field {
     V2C_N_RK_sliprs_final, V2C_N_RK_phidts_final, V2C_N_RK_sliprs_solved,
     V2C_N_RK_phidts_solved, V2C_N_RK_deltas,
     V2C_N_RK_sliprs0, V2C_N_RK_sliprs1, V2C_N_RK_phidts0, V2C_N_RK_phidts1,
+    /* post-solve nodal fields (momentum_continuity_iterate.c:214-283,
+     * interpolation.c:1575-1637) */
+    V2C_N_deltasxx, V2C_N_deltasxy, V2C_N_deltaVs, V2C_N_exxs, V2C_N_exxes,
+    V2C_N_sxxes, V2C_N_prs,
+    /* heat solve: temperature before it (temperature_iterate.c:58-62) and
+     * the marker-temperature averages (temperature_iterate.c:183-184,353-364) */
+    V2C_N_tk2, V2C_N_tol0, V2C_N_tol1,
     V2C_NF
 };
 
@@ -241,6 +248,82 @@ int v2c_ductilevisc_batch(v2c_ctx *ctx, long n, const char *type,
                           const double *tk, const double *pb,
                           double *nu_out, int *iters);
 
+/* ---- post-solve nodal stencils (device-resident nodal fields) ------------
+ * The reference's per-node routines between the Stokes solve and the next
+ * marker2nodes, on the context's nodal fields.  Each writes exactly the
+ * fields the reference routine writes. */
+
+/* recompute_stress_strainrate(dt, rk) (momentum_continuity_iterate.c:214-263,
+ * sxxcalc / sxycalc / nueffcalc momentum_continuity_addmatrix_checkerror.c:
+ * 380-393, 446-473, 521-569): sxx, exx, deltasxx on every node with m1, m2
+ * >= 1; sxy, exy, esp, deltasxy on the interior ones.  deltas divide by the
+ * context's timestep.  stats (may be NULL) receives the routine's extrema as
+ * the sequential MINV/MAXV folds of its node loop (the reference reduces them
+ * with OpenMP, which leaves NaN handling to the compiler; without NaN the
+ * two agree); minnu / maxnu start from the reference's -1e50 / 1e50 and so
+ * stay there, as in the reference. */
+typedef struct {
+    double maxds;      /* max |sxx - sxxe|, |sxy - sxye|                      */
+    double minnu;      /* min(-1e50, effective viscosities)                  */
+    double maxnu;      /* max(1e50, effective viscosities)                   */
+    double maxdphi;    /* max |phis - phis_it| over interior nodes (global)  */
+    double maxnvp;     /* min(1e30, nu / gg) over interior nodes (global)    */
+} v2c_stress_stats;
+int v2c_recompute_stress_strainrate(v2c_ctx *ctx, double dt, int rk, v2c_stress_stats *stats);
+
+/* interpolate_center2basic(dt) (interpolation.c:1600-1637): exxs, exxpl,
+ * sxxs, prs, exxel on interior basic nodes from the four surrounding
+ * pressure nodes. */
+int v2c_interpolate_center2basic(v2c_ctx *ctx, double dt);
+/* interpolate_center2basic_old(dt) (interpolation.c:1575-1598): exxes,
+ * sxxes, sppes. */
+int v2c_interpolate_center2basic_old(v2c_ctx *ctx);
+/* interpolate_basic2center() (interpolation.c:1640-1685): nd on every
+ * pressure node with m1, m2 >= 1, harmonic average of the basic-node nu. */
+int v2c_interpolate_basic2center(v2c_ctx *ctx);
+/* copyarrays(rk) (momentum_continuity_iterate.c:265-283): deltaVs, sliprs,
+ * RK_sliprs_final, RK_phidts_final on interior nodes (rk is unused, as in
+ * the reference). */
+int v2c_copyarrays(v2c_ctx *ctx, int rk);
+
+/* calc_timestep(oldornew) (calc_RSF.c:352-572) without its sticky-air
+ * diagnostics: the adaptive time step from the visco-plastic, displacement
+ * and rate-and-state limits over interior nodes.  Every running minimum and
+ * maximum follows the reference's sequential MINV/MAXV semantics exactly
+ * (including NaN and the reset of dtphi/dtL to maxtmstep at nodes with
+ * bs <= 0), whatever the reduction order on the device.  Where the reference
+ * leaves its state-step factor lambda unset at a node (chi NaN) it keeps the
+ * previous node's value; before any node sets it, lambda0 is used (the
+ * reference's value there is uninitialised).  chi == 0 at a node, where the
+ * reference exits, returns V2C_EUNSUPPORTED; pmode other than 0 or 1 with
+ * RSFmode == 1 returns V2C_EUNSUPPORTED. */
+typedef struct {
+    double maxtmstep;  /* upper limit, the mode file's time step (i2evps.c:78) */
+    double tcnvp, tcd, tcphih, dtfactor;      /* input_RSF.c:17-20           */
+    double lambda0;    /* lambda before the first node that sets it        */
+} v2c_timestep_params;
+typedef struct {
+    double dt;         /* dtfactor * min(maxtmstep, dtphi, dtL, dtnvp, dtd)   */
+    double dtnvp, dtd, dtphi, dtL;
+    double maxvel, maxVx;
+} v2c_timestep_result;
+/* reference defaults (input_RSF.c:17-20), maxtmstep = 1 yr, lambda0 = NaN */
+void v2c_timestep_params_default(v2c_timestep_params *p);
+int v2c_calc_timestep(v2c_ctx *ctx, int oldornew, const v2c_timestep_params *p, v2c_timestep_result *out);
+
+/* ---- marker temperatures after the heat solve ------------------------------
+ * The marker part of solve_temperature (temperature_iterate.c:181-419), on
+ * the device-resident markers: fluid markers (types 50..99) take the new
+ * nodal temperature; rock markers (types < 50) get the solved increment
+ * tk - tk2, with (heatdif != 0) the numerical subgrid diffusion toward the
+ * old nodal temperature and its antidiffusion correction through the
+ * per-node averages tol1 = sum(dT w) / sum(w) (tol0 = sum(w)), each sum over
+ * the markers of the cell whose upper-left node it is.  Inputs: nodal tk
+ * (after the solve), tk2 (before it), kt, cp, ro; the context's timestep
+ * and tdeep.  Outputs: marker temperatures, nodal tol0 and tol1.  Markers
+ * outside the open domain are untouched, as in the reference. */
+int v2c_temperature_markers(v2c_ctx *ctx, double heatdif);
+
 /* ---- build-defined per-marker local update (no reference counterpart) ----
  * The reference evaluates rate-and-state friction per node (plastic.c,
  * calc_RSF.c:33-221) and the ductile viscosity with a quasi-Newton that stops
@@ -308,7 +391,7 @@ int v2c_mlp_forward(v2c_ctx *ctx, const float *Xs, long ns, float *Ys);
 enum v2c_kernel_id {
     V2C_K_M2N_CELLS = 0, V2C_K_M2N_NODES, V2C_K_N2M_CELLS, V2C_K_N2M_NODES,
     V2C_K_ADVECT, V2C_K_SORT, V2C_K_BEFORE, V2C_K_AFTER, V2C_K_LOCAL, V2C_K_MLP,
-    V2C_K_M2N_PROPS, V2C_K_N2M_UPDATE,
+    V2C_K_M2N_PROPS, V2C_K_N2M_UPDATE, V2C_K_POST,
     V2C_K_COUNT
 };
 int v2c_enable_timing(v2c_ctx *ctx, int on);

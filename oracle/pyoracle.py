@@ -11,7 +11,8 @@ import subprocess
 
 import numpy as np
 
-from velo2cycles_amd.abi import (NF, NODAL_FIELDS, Grid, Markers, Materials, Params, _dptr)
+from velo2cycles_amd.abi import (NF, NODAL_FIELDS, Grid, Markers, Materials, Params, TimestepParams,
+                                 TimestepResult, _dptr)
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(ORACLE_DIR, "liboracle.so")
@@ -63,6 +64,19 @@ def lib():
         L.or_local_update.argtypes = [P(OrState), C.c_double, C.c_char_p, P(C.c_double), C.c_int,
                                       C.c_double, P(C.c_long), P(C.c_int), P(C.c_int)]
         L.or_local_update.restype = None
+        L.or_recompute_stress_strainrate.argtypes = [P(OrState), C.c_double, C.c_int, P(C.c_double)]
+        L.or_recompute_stress_strainrate.restype = None
+        L.or_interpolate_center2basic.argtypes = [P(OrState), C.c_double]
+        L.or_interpolate_center2basic.restype = None
+        for name in ("or_interpolate_center2basic_old", "or_interpolate_basic2center"):
+            getattr(L, name).argtypes = [P(OrState)]
+            getattr(L, name).restype = None
+        L.or_copyarrays.argtypes = [P(OrState), C.c_int]
+        L.or_copyarrays.restype = None
+        L.or_calc_timestep.argtypes = [P(OrState), C.c_int, P(TimestepParams), P(TimestepResult)]
+        L.or_calc_timestep.restype = C.c_int
+        L.or_temperature_markers.argtypes = [P(OrState), C.c_double]
+        L.or_temperature_markers.restype = None
         _lib = L
     return _lib
 
@@ -110,6 +124,33 @@ class Oracle:
         nerr = C.c_long(0)
         d = lib().or_compute_after_solve(C.byref(self.st), dt, rk, C.byref(nerr))
         return d, nerr.value
+
+    def recompute_stress_strainrate(self, dt, rk):
+        st = (C.c_double * 5)()
+        lib().or_recompute_stress_strainrate(C.byref(self.st), dt, rk, st)
+        return dict(zip(("maxds", "minnu", "maxnu", "maxdphi", "maxnvp"), list(st)))
+
+    def interpolate_center2basic(self, dt):
+        lib().or_interpolate_center2basic(C.byref(self.st), dt)
+
+    def interpolate_center2basic_old(self):
+        lib().or_interpolate_center2basic_old(C.byref(self.st))
+
+    def interpolate_basic2center(self):
+        lib().or_interpolate_basic2center(C.byref(self.st))
+
+    def copyarrays(self, rk=0):
+        lib().or_copyarrays(C.byref(self.st), rk)
+
+    def calc_timestep(self, oldornew, params=None):
+        """(return code, result dict); code -1 where chi == 0, -2 for an unknown pmode."""
+        p = params or TimestepParams.default()
+        r = TimestepResult()
+        rc = lib().or_calc_timestep(C.byref(self.st), oldornew, C.byref(p), C.byref(r))
+        return rc, r.as_dict()
+
+    def temperature_markers(self, heatdif):
+        lib().or_temperature_markers(C.byref(self.st), heatdif)
 
     def ductilevisc(self, t, sxy, sxx, tk, pb):
         it = C.c_int(0)

@@ -12,6 +12,8 @@
  * IN sets reference globals by name; the ops run in order; OUT receives every
  * global of the table after the last op.  Ops:
  *   m2n | n2m | move | before:<dt>:<rk> | after:<dt>:<rk> | ductile | allinteri
+ *   stress:<dt>:<rk> | c2b:<dt> | c2bold:<dt> | b2c | copy:<rk> | tstep:<oldornew>
+ * (tstep leaves the returned time step in drv_ret[2]).
  */
 #ifndef _GNU_SOURCE
 #define _GNU_SOURCE 1
@@ -33,6 +35,7 @@
 #include "moveandrotatemarkers.c"
 #include "temperature_iterate.c"
 #include "momentum_continuity_addmatrix_checkerror.c"
+#include "momentum_continuity_iterate.c"
 #include "plastic.c"
 #include "recalc_marker_properties.c"
 
@@ -80,6 +83,10 @@ static entry table[] = {
     D(pes), D(phis_it), D(syields), D(nuvp_it), D(sliprs), D(RK_sliprs_final), D(RK_phidts_final),
     D(RK_sliprs_solved), D(RK_phidts_solved), D(RK_deltas),
     D(RK_sliprs0), D(RK_sliprs1), D(RK_phidts0), D(RK_phidts1),
+    D(deltasxx), D(deltasxy), D(deltaVs), D(exxs), D(exxes), D(sxxes), D(prs), D(tk2), D(tol0), D(tol1),
+    /* post-solve scalars (headevps.c:120-122, 181, 339, 345-346) */
+    S(maxvel), S(maxdphi), S(maxnvp), S(dtL), S(dtphi), S(dtd), S(dtnvp),
+    S(maxtmstep), S(tcnvp), S(tcd), S(tcphih), S(dtfactor),
     /* pure-function op arrays */
     D(dv_sxy), D(dv_sxx), D(dv_tk), D(dv_pb), D(dv_out), {"dv_type", 'c', (void *)dv_type, DRV_MAX},
     D(ai_x), D(ai_y), D(ai_vx), D(ai_vy), D(ai_esp), D(ai_ee), L(drv_n), D(drv_ret),
@@ -167,7 +174,13 @@ int main(int argc, char **argv)
         } else if (!strcmp(op, "allinteri")) {
             for (long i = 0; i < drv_n; i++)
                 allinteri(ai_x[i], ai_y[i], &ai_vx[i], &ai_vy[i], &ai_esp[i], &ai_ee[i]);
-        } else { fprintf(stderr, "ref_driver: unknown op %s\n", op); return 4; }
+        } else if (sscanf(op, "stress:%lf:%d", &dt, &rk) == 2) recompute_stress_strainrate(dt, rk);
+        else if (sscanf(op, "c2bold:%lf", &dt) == 1) interpolate_center2basic_old(dt);
+        else if (sscanf(op, "c2b:%lf", &dt) == 1) interpolate_center2basic(dt);
+        else if (!strcmp(op, "b2c")) interpolate_basic2center();
+        else if (sscanf(op, "copy:%d", &rk) == 1) copyarrays(rk);
+        else if (sscanf(op, "tstep:%d", &rk) == 1) drv_ret[2] = calc_timestep(rk);
+        else { fprintf(stderr, "ref_driver: unknown op %s\n", op); return 4; }
     }
     for (long i = 0; i < nodenum; i++) {
         RK_sliprs0[i] = RK_sliprs[i][0]; RK_sliprs1[i] = RK_sliprs[i][1];

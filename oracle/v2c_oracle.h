@@ -61,6 +61,17 @@ double or_compute_after_solve(or_state *s, double dt, int rk, long *nerr);
 void or_allinteri(const or_state *s, double x, double y, double *vx,
                   double *vy, double *esp, double *ee);
 
+/* post-solve nodal routines (v2c_oracle_post.c) */
+void or_recompute_stress_strainrate(or_state *s, double dt, int rk, double *stats /* 5, may be NULL */);
+void or_interpolate_center2basic(or_state *s, double dt);
+void or_interpolate_center2basic_old(or_state *s);
+void or_interpolate_basic2center(or_state *s);
+void or_copyarrays(or_state *s, int rk);
+/* 0, or -1 where chi == 0 (the reference exits), -2 for an unknown pmode */
+int or_calc_timestep(or_state *s, int oldornew, const v2c_timestep_params *p, v2c_timestep_result *r);
+/* marker temperatures after the heat solve */
+void or_temperature_markers(or_state *s, double heatdif);
+
 #ifdef __cplusplus
 }
 #endif

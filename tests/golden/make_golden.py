@@ -59,8 +59,48 @@ def case(name, cfg, ops):
     print("wrote", name, cfg.n, "markers", cfg.grid.nodenum, "nodes", ops)
 
 
+def post_cases():
+    """Post-solve nodal routines on a synthetic post-solve state (with the
+    degenerate nodes of synth.post_solve_nodal).  Markers are not used; the
+    fixture keeps the nonzero input fields, the fields the ops changed and
+    the scalars the reference leaves behind."""
+    post = [f"stress:{DT}:0", f"c2b:{DT}", f"c2bold:{DT}", "b2c", "copy:0"]
+    for name, pmode, rsf, refine, seed in (("post_rsf_p0", 0, 1, True, 3), ("post_rsf_p1", 1, 1, False, 4),
+                                           ("post_dp", 0, 0, True, 5)):
+        cfg = synth.config_grid(25, 15, 24e3, 14e3, 2, refine=refine, seed=seed)
+        cfg.params.pmode = pmode
+        cfg.params.RSFmode = rsf
+        cfg.nodal = synth.post_solve_nodal(cfg, seed=seed + 7)
+        runs = {"ops": post}
+        for oo in (0, 1):
+            runs[f"tstep{oo}"] = [f"tstep:{oo}"]
+        d = {k: v for k, v in pack(cfg, [], {f: cfg.markers[f] for f in MARKER_FIELDS} | {"t": cfg.markers.t},
+                                   cfg.nodal, refrun.run(cfg, [])[2]).items()
+             if not k.startswith(("in_m_", "out_m_", "in_n_", "out_n_", "g_", "ret", "errcheck", "ops"))}
+        for k in NODAL_FIELDS:
+            if np.any(cfg.nodal[k] != 0):
+                d["in_n_" + k] = cfg.nodal[k]
+        extra = {"maxtmstep": ("d", np.array([DT]))}
+        for tag, ops in runs.items():
+            _, nd, raw = refrun.run(cfg, ops, extra=extra)
+            pre = "" if tag == "ops" else tag + "_"
+            d[pre + "opslist"] = np.array(json.dumps(ops))
+            for k in NODAL_FIELDS:
+                if not np.array_equal(nd[k], cfg.nodal[k], equal_nan=True):
+                    d[pre + "out_n_" + k] = nd[k]
+            d[pre + "ret"] = raw["drv_ret"][:3]
+            for k in ("maxvel", "maxdphi", "maxnvp", "dtL", "dtphi", "dtd", "dtnvp"):
+                d[pre + "g_" + k] = raw[k]
+        d["maxtmstep"] = np.array(DT)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+        print("wrote", name, cfg.grid.nodenum, "nodes")
+
+
 def main():
     assert refrun.available(), "build oracle/_ref first (make -C oracle)"
+    if sys.argv[1:] == ["post"]:
+        post_cases()
+        return
     step = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m", "move"]
     cfg = synth.config_edge()
     case("edge_step_t0", cfg, step)

@@ -1,0 +1,147 @@
+"""Post-solve nodal routines on the GPU (recompute_stress_strainrate,
+interpolate_center2basic(_old), interpolate_basic2center, copyarrays,
+calc_timestep) against the reference's golden outputs and the oracle.
+
+Bar: every nodal field within 1e-12 of the scaled oracle / reference value
+(tests/util.py); calc_timestep's results within rel 1e-12 (its running
+minima select among per-node candidates computed with device libm)."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from velo2cycles_amd import synth
+from velo2cycles_amd.abi import NODAL_FIELDS, Context, TimestepParams, V2CError
+from oracle.pyoracle import Oracle
+from tests.util import compare, load_post_fixture, run_gpu, run_oracle
+
+pytestmark = pytest.mark.gpu
+DT = synth.YEAR
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+POST_OPS = [f"stress:{DT}:0", f"c2b:{DT}", f"c2bold:{DT}", "b2c", "copy:0"]
+TS_KEYS = ("dt", "dtnvp", "dtd", "dtphi", "dtL", "maxvel")
+
+
+def _close(a, b, rel=1e-12):
+    if math.isnan(a) or math.isnan(b):
+        return math.isnan(a) and math.isnan(b)
+    return a == b or abs(a - b) <= rel * max(abs(a), abs(b))
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "post_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[:-4])
+def test_post_routines_match_reference_golden(gpu, path):
+    cfg, runs, maxtmstep = load_post_fixture(path)
+    ops, out, scal = runs["ops"]
+    print("case", os.path.basename(path), flush=True)
+    _, gn, ret, ctx = run_gpu(gpu, cfg, ops)
+    exp = {k: out.get(k, cfg.nodal[k]) for k in NODAL_FIELDS}
+    bad = compare(None, None, gn, exp)
+    assert not bad, bad
+    assert _close(ret["stress"]["maxdphi"], scal["maxdphi"]) and _close(ret["stress"]["maxnvp"], scal["maxnvp"])
+    for tag, oo in (("tstep0", 0), ("tstep1", 1)):
+        ctx.set_all_nodal(cfg.nodal)
+        r = ctx.calc_timestep(oo, TimestepParams.default(maxtmstep=maxtmstep))
+        _, _, s = runs[tag]
+        for k in TS_KEYS:
+            assert _close(r[k], s[k]), (tag, k, r[k], s[k])
+    ctx.close()
+
+
+def _large(pmode=0, rsf=1, seed=21, long_nan=False):
+    cfg = synth.config_grid(401, 101, 400e3, 100e3, 1, refine=True, seed=seed)
+    cfg.params.pmode = pmode
+    cfg.params.RSFmode = rsf
+    cfg.nodal = synth.post_solve_nodal(cfg, seed=seed + 1)
+    if long_nan:
+        # a = 0 (chi NaN: lambda carried over) across several 256-node blocks,
+        # b <= 0 resets and NaN velocities near block edges
+        cfg.nodal["as"][5000:6100] = 0.0
+        cfg.nodal["bs"][[255, 256, 511, 20000]] = 0.0
+        cfg.nodal["mvx"][[767, 768, 30001]] = np.nan
+        cfg.nodal["vx"][[1023, 1024]] = np.nan
+    return cfg
+
+
+@pytest.mark.parametrize("pmode,rsf,long_nan", [(0, 1, False), (1, 1, True), (0, 0, False)])
+def test_post_routines_match_oracle_large(gpu, pmode, rsf, long_nan):
+    cfg = _large(pmode, rsf, long_nan=long_nan)
+    print("case large", pmode, rsf, long_nan, cfg.grid.nodenum, flush=True)
+    _, gn, gret, ctx = run_gpu(gpu, cfg, POST_OPS)
+    _, on, oret = run_oracle(cfg, POST_OPS)
+    bad = compare(None, None, gn, on)
+    assert not bad, bad
+    for k, v in oret["stress"].items():
+        assert _close(gret["stress"][k], v), k
+    for oo in (0, 1):
+        for lam0 in (float("nan"), 0.05):
+            p = TimestepParams.default(maxtmstep=DT)
+            p.lambda0 = lam0
+            ctx.set_all_nodal(cfg.nodal)
+            r = ctx.calc_timestep(oo, p)
+            rc, o = Oracle(cfg).calc_timestep(oo, p)
+            assert rc == 0
+            for k in TS_KEYS + ("maxVx",):
+                assert _close(r[k], o[k]), (oo, lam0, k, r[k], o[k])
+    ctx.close()
+
+
+def test_timestep_all_chi_nan_uses_lambda0(gpu):
+    """a = 0 everywhere: chi is NaN at every node, so the reference never sets
+    lambda; the value before the first node is what counts (capped at 1/5)."""
+    cfg = _large(seed=33)
+    cfg.nodal["as"][:] = 0.0
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=1, lib=gpu)
+    ctx.set_time(cfg.timesum, cfg.timestep)
+    ctx.set_all_nodal(cfg.nodal)
+    for lam0 in (0.01, 0.5, float("nan")):
+        p = TimestepParams.default(maxtmstep=DT)
+        p.lambda0 = lam0
+        r = ctx.calc_timestep(0, p)
+        rc, o = Oracle(cfg).calc_timestep(0, p)
+        assert rc == 0
+        assert _close(r["dtL"], o["dtL"]), (lam0, r["dtL"], o["dtL"])
+    ctx.close()
+
+
+def test_timestep_unknown_pmode_is_an_error(gpu):
+    cfg = _large(seed=44)
+    cfg.params.pmode = 2
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=1, lib=gpu)
+    ctx.set_all_nodal(cfg.nodal)
+    with pytest.raises(V2CError, match="pmode"):
+        ctx.calc_timestep(0)
+    assert Oracle(cfg).calc_timestep(0)[0] == -2
+    ctx.close()
+
+
+def _heat_case(seed=51, per=4):
+    cfg = synth.config_grid(401, 101, 400e3, 100e3, per, refine=True, seed=seed)
+    rng = np.random.default_rng(seed)
+    nn, m = cfg.grid.nodenum, cfg.markers
+    cfg.nodal["tk"] = rng.uniform(300, 1600, nn)
+    cfg.nodal["tk2"] = cfg.nodal["tk"] + rng.normal(0, 5, nn)
+    cfg.nodal["kt"] = rng.uniform(1.5, 3.5, nn)
+    cfg.nodal["cp"] = rng.uniform(800, 1300, nn)
+    cfg.nodal["ro"] = rng.uniform(2600, 3400, nn)
+    m["k"][:] = rng.uniform(300, 1600, m.n)
+    k = rng.permutation(m.n)
+    m["k"][k[:500]] = -1.0                 # newly incoming
+    m.t[k[500:3000]] = rng.integers(50, 100, 2500).astype(np.int8)   # fluid
+    m.t[k[3000:3500]] = 110                # immobile kinds: untouched
+    m["x"][k[3500:3600]] = -1.0            # outside
+    m["y"][k[3600:3700]] = cfg.grid.ysize + 1.0
+    return cfg
+
+
+@pytest.mark.parametrize("heatdif", [0.0, 1e-3, 1.0, 1e6])
+def test_marker_temperatures_match_oracle(gpu, heatdif):
+    cfg = _heat_case()
+    print("case heat", heatdif, cfg.n, flush=True)
+    gm, gn, _, ctx = run_gpu(gpu, cfg, [f"heat:{heatdif}"])
+    om, on, _ = run_oracle(cfg, [f"heat:{heatdif}"])
+    bad = compare(gm, om, gn, on)
+    assert not bad, bad
+    ctx.close()

@@ -42,3 +42,26 @@ def test_allinteri_vectors():
     got = np.array([o.allinteri(x, y) for x, y in zip(z["ai_x"], z["ai_y"])])
     for j, k in enumerate(("ai_vx", "ai_vy", "ai_esp", "ai_ee")):
         assert np.array_equal(got[:, j], z[k]), k
+
+
+POST_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "post_*.npz")))
+
+
+@pytest.mark.parametrize("path", POST_FIXTURES, ids=lambda p: os.path.basename(p)[:-4])
+def test_oracle_post_routines_match_reference_bitwise(path):
+    from velo2cycles_amd.abi import TimestepParams
+    from tests.util import load_post_fixture
+    cfg, runs, maxtmstep = load_post_fixture(path)
+    ops, out, scal = runs["ops"]
+    _, on, ret = run_oracle(cfg, ops)
+    for k in NODAL_FIELDS:
+        exp = out.get(k, cfg.nodal[k])
+        assert np.array_equal(on[k], exp, equal_nan=True), f"nodal {k}"
+    assert ret["stress"]["maxdphi"] == scal["maxdphi"] and ret["stress"]["maxnvp"] == scal["maxnvp"]
+    for tag, oo in (("tstep0", 0), ("tstep1", 1)):
+        _, _, scal = runs[tag]
+        rc, r = Oracle(cfg).calc_timestep(oo, TimestepParams.default(maxtmstep=maxtmstep))
+        assert rc == 0
+        assert r["dt"] == scal["dt"], tag
+        for k in ("dtnvp", "dtd", "dtphi", "dtL", "maxvel"):
+            assert r[k] == scal[k], (tag, k)

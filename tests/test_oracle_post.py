@@ -1,0 +1,218 @@
+"""Post-solve nodal routines of the oracle (oracle/v2c_oracle_post.c):
+
+* against the reference's own routines, compiled in place and run here
+  (skipped where the reference is absent), bit for bit on every nodal field
+  and on the scalars they leave behind;
+* against what the mathematics fixes: a linear velocity field gives its exact
+  strain and rotation rates, four-point averages of a constant field return
+  it, the harmonic average of a constant viscosity returns it, and a uniform
+  velocity gives the displacement time step in closed form."""
+import math
+
+import numpy as np
+import pytest
+
+from velo2cycles_amd import synth
+from velo2cycles_amd.abi import NODAL_FIELDS, TimestepParams
+from oracle import refrun
+from oracle.pyoracle import Oracle
+
+DT = synth.YEAR
+POST_OPS = [f"stress:{DT}:0", f"c2b:{DT}", f"c2bold:{DT}", "b2c", "copy:0"]
+
+
+def _cfg(pmode=0, rsf=1, refine=True, seed=3):
+    cfg = synth.config_grid(41, 23, 40e3, 22e3, 2, refine=refine, seed=seed)
+    cfg.params.pmode = pmode
+    cfg.params.RSFmode = rsf
+    cfg.nodal = synth.post_solve_nodal(cfg, seed=seed + 7)
+    return cfg
+
+
+def _oracle_run(cfg, ops):
+    o = Oracle(cfg)
+    for op in ops:
+        name, *a = op.split(":")
+        if name == "stress":
+            o.recompute_stress_strainrate(float(a[0]), int(a[1]))
+        elif name == "c2b":
+            o.interpolate_center2basic(float(a[0]))
+        elif name == "c2bold":
+            o.interpolate_center2basic_old()
+        elif name == "b2c":
+            o.interpolate_basic2center()
+        elif name == "copy":
+            o.copyarrays(int(a[0]))
+    return o
+
+
+needs_ref = pytest.mark.skipif(not refrun.available(), reason="reference sources not present")
+
+
+@needs_ref
+@pytest.mark.parametrize("pmode,rsf,refine", [(0, 1, True), (1, 1, False), (0, 0, True)])
+def test_post_routines_match_reference(pmode, rsf, refine):
+    cfg = _cfg(pmode, rsf, refine)
+    _, rn, raw = refrun.run(cfg, POST_OPS)
+    o = _oracle_run(cfg, POST_OPS)
+    bad = [k for k in NODAL_FIELDS if not np.array_equal(o.nodal[k], rn[k], equal_nan=True)]
+    assert not bad, bad
+    st = Oracle(cfg).recompute_stress_strainrate(DT, 0)
+    assert st["maxdphi"] == raw["maxdphi"][0] and st["maxnvp"] == raw["maxnvp"][0]
+
+
+@needs_ref
+@pytest.mark.parametrize("oldornew", [0, 1])
+@pytest.mark.parametrize("pmode,rsf", [(0, 1), (1, 1), (0, 0)])
+def test_calc_timestep_matches_reference(oldornew, pmode, rsf):
+    cfg = _cfg(pmode, rsf)
+    p = TimestepParams.default(maxtmstep=DT)
+    extra = {"maxtmstep": ("d", np.array([p.maxtmstep]))}
+    _, _, raw = refrun.run(cfg, [f"tstep:{oldornew}"], extra=extra)
+    rc, r = Oracle(cfg).calc_timestep(oldornew, p)
+    assert rc == 0
+    assert r["dt"] == raw["drv_ret"][2]
+    for k in ("dtnvp", "dtd", "dtphi", "dtL", "maxvel"):
+        assert r[k] == raw[k][0] or (math.isnan(r[k]) and math.isnan(raw[k][0])), k
+
+
+def test_linear_velocity_gives_exact_rates():
+    # power-of-two spacing and integer coefficients: every difference exact
+    gx = np.arange(17) * 1024.0
+    gy = np.arange(9) * 512.0
+    cfg = synth.config_grid(17, 9, gx[-1], gy[-1], 2, refine=False)
+    cfg.grid = synth.GridArrays(gx, gy)
+    nn = cfg.grid.nodenum
+    m1, m2 = np.divmod(np.arange(nn), 9)
+    nod = {k: np.zeros(nn) for k in NODAL_FIELDS}
+    # vx lives at (gx[m1], (gy[m2]+gy[m2+1])/2), vy at ((gx[m1]+gx[m1+1])/2, gy[m2])
+    a, b, c, d = 3.0, 5.0, -2.0, 7.0          # vx = a x + b y, vy = c x + d y (per 2^20 m)
+    s = 2.0 ** -20
+    nod["vx"] = (a * m1 * 1024.0 + b * (m2 + 0.5) * 512.0) * s
+    nod["vy"] = (c * (m1 + 0.5) * 1024.0 + d * m2 * 512.0) * s
+    for k in ("nd", "nu"):
+        nod[k][:] = 1e21
+    for k in ("gd", "gg"):
+        nod[k][:] = 3e10
+    cfg.nodal = nod
+    o = Oracle(cfg)
+    o.recompute_stress_strainrate(DT, 0)
+    inner = (m1 >= 1) & (m2 >= 1)
+    interior = inner & (m1 < 16) & (m2 < 8)
+    assert np.all(o.nodal["exx"][inner] == 0.5 * (a - d) * s)
+    # the reference's exy and esp are half the velocity derivatives' sum and difference
+    assert np.all(o.nodal["exy"][interior] == 0.5 * (b + c) * s)
+    assert np.all(o.nodal["esp"][interior] == 0.5 * (b - c) * s)
+    Z = 3e10 * DT / (3e10 * DT + 1e21)
+    assert np.allclose(o.nodal["sxx"][inner], 2.0 * (Z * 1e21) * 0.5 * (a - d) * s, rtol=1e-15)
+
+
+def test_averages_of_constant_fields():
+    cfg = synth.config_grid(31, 17, 30e3, 16e3, 2, refine=True)
+    nn = cfg.grid.nodenum
+    nod = {k: np.zeros(nn) for k in NODAL_FIELDS}
+    for k, v in (("exx", 2e-15), ("sxx", 3e6), ("sxxe", 1e6), ("pr", 5e7), ("gd", 3e10), ("nd_m2c", 1e21),
+                 ("exxe", -1e-15), ("sppe", 4e6), ("nu", 7.5e20)):
+        nod[k][:] = v
+    cfg.nodal = nod
+    o = Oracle(cfg)
+    o.interpolate_center2basic(DT)
+    o.interpolate_center2basic_old()
+    o.interpolate_basic2center()
+    m1, m2 = np.divmod(np.arange(nn), cfg.grid.ynumy)
+    inner = (m1 >= 1) & (m2 >= 1) & (m1 < cfg.grid.xnumx - 1) & (m2 < cfg.grid.ynumy - 1)
+    for k, v in (("exxs", 2e-15), ("sxxs", 3e6), ("prs", 5e7), ("exxes", -1e-15), ("sxxes", 1e6), ("sppes", 4e6)):
+        assert np.allclose(o.nodal[k][inner], v, rtol=1e-15, atol=0), k
+        assert np.all(o.nodal[k][~inner] == 0), k
+    epl = 2e-15 - (3e6 - 1e6) / (2 * 3e10 * DT) - 3e6 / (2 * 1e21)
+    assert np.allclose(o.nodal["exxpl"][inner], epl, rtol=1e-15)
+    assert np.allclose(o.nodal["exxel"][inner], 2 * 3e10 * DT * 2e-15 + 1e6, rtol=1e-15)
+    centre = (m1 >= 1) & (m2 >= 1)
+    assert np.allclose(o.nodal["nd"][centre], 7.5e20, rtol=4e-16, atol=0)
+
+
+def test_uniform_velocity_displacement_step():
+    cfg = synth.config_grid(21, 11, 20e3, 10e3, 2, refine=True)
+    cfg.params.RSFmode = 0
+    nn = cfg.grid.nodenum
+    nod = {k: np.zeros(nn) for k in NODAL_FIELDS}
+    nod["vx"][:] = 2e-9
+    nod["vy"][:] = -1e-9
+    nod["nu_m2c"][:] = 1e20
+    nod["gg"][:] = 3e10
+    cfg.nodal = nod
+    p = TimestepParams.default(maxtmstep=1e12)
+    rc, r = Oracle(cfg).calc_timestep(1, p)
+    assert rc == 0
+    gx, gy = cfg.grid.gx, cfg.grid.gy
+    dx, dy = np.diff(gx)[1:].min(), np.diff(gy)[1:].min()
+    expect = 1e-3 / max(2e-9 / dx, 1e-9 / dy)
+    assert r["dtd"] == pytest.approx(expect, rel=1e-15)
+    assert r["maxVx"] == 2e-9
+    assert r["dtnvp"] == 0.2 * 5e26 / 3e10
+    assert r["dt"] == min(1e12, r["dtd"], r["dtnvp"])
+
+
+# ---- marker temperatures after the heat solve ------------------------------------
+
+def _heat_cfg(per=1, seed=8):
+    cfg = synth.config_grid(21, 11, 20e3, 10e3, per, refine=True, seed=seed)
+    nn = cfg.grid.nodenum
+    for k in NODAL_FIELDS:
+        cfg.nodal[k] = np.zeros(nn)
+    cfg.nodal["kt"][:] = 2.5
+    cfg.nodal["cp"][:] = 1000.0
+    cfg.nodal["ro"][:] = 3000.0
+    m = cfg.markers
+    m.t[:] = 3
+    return cfg
+
+
+def test_heat_without_diffusion_adds_the_solved_increment():
+    cfg = _heat_cfg(per=2)
+    cfg.nodal["tk"][:] = 1100.0
+    cfg.nodal["tk2"][:] = 1000.0
+    m = cfg.markers
+    n = m.n
+    m["k"][:] = 900.0
+    m["k"][:5] = -1.0                # newly incoming: reset to the old nodal T (>= tdeep)
+    m.t[10:20] = 60                  # fluid: takes the new nodal T
+    m.t[20:25] = 110                 # types >= 100: untouched
+    m["x"][25:28] = -5.0             # outside: untouched
+    o = Oracle(cfg)
+    o.temperature_markers(0.0)
+    k = o.markers["k"]
+    tdeep = cfg.params.tdeep
+    assert np.allclose(k[:5], max(1000.0, tdeep) + 100.0, rtol=1e-15)
+    assert np.allclose(k[10:20], 1100.0, rtol=1e-15)
+    assert np.all(k[20:28] == 900.0)
+    assert np.allclose(k[28:], 1000.0, rtol=1e-15)
+    assert np.all(o.nodal["tol0"] == 0) and np.all(o.nodal["tol1"] == 0)
+    assert n > 28
+
+
+def test_heat_diffusion_single_marker_closed_form():
+    """One marker per cell, uniform properties and old/new nodal T: the marker
+    relaxes toward T0 by dT (1 - exp(-c)), c = heatdif k/(rho Cp) 2 (1/dx^2 +
+    1/dy^2) dt; its node's average is that change, and the antidiffusion
+    takes back its (1-cx)(1-cy) share."""
+    cfg = _heat_cfg(per=1, seed=12)
+    cfg.nodal["tk"][:] = 1000.0
+    cfg.nodal["tk2"][:] = 1000.0
+    m = cfg.markers
+    rng = np.random.default_rng(3)
+    m["k"][:] = 1000.0 + rng.uniform(-50, 50, m.n)
+    heatdif = 1e-3
+    o = Oracle(cfg)
+    o.temperature_markers(heatdif)
+    gx, gy = cfg.grid.gx, cfg.grid.gy
+    i = np.searchsorted(gx, m["x"], side="right") - 1
+    j = np.searchsorted(gy, m["y"], side="right") - 1
+    dx, dy = gx[i + 1] - gx[i], gy[j + 1] - gy[j]
+    cx, cy = (m["x"] - gx[i]) / dx, (m["y"] - gy[j]) / dy
+    c = heatdif * 2.5 / (1000.0 * 3000.0) * 2.0 * (1 / dx ** 2 + 1 / dy ** 2) * cfg.timestep
+    d1 = (1000.0 - m["k"]) * (1.0 - np.exp(-np.minimum(c, 150.0)))
+    expect = m["k"] + d1 - d1 * (1 - cx) * (1 - cy)
+    assert np.allclose(o.markers["k"], expect, rtol=1e-13, atol=0)
+    node = i * cfg.grid.ynumy + j
+    assert np.allclose(o.nodal["tol1"][node], d1, rtol=1e-13)

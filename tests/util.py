@@ -37,12 +37,18 @@ def rel_errors(a: np.ndarray, b: np.ndarray) -> tuple[float, float, int]:
     return float(e.max()), float(np.sqrt(np.mean(e * e))), nan_mis
 
 
+def _field(z, name, n):
+    """A nodal array of a golden file; fields newer than the file (which its
+    operations never touch) are zero, as the reference's globals start."""
+    return z[name] if name in z.files else np.zeros(n)
+
+
 def compare(gpu_m, ora_m, gpu_n, ora_n, rel=REL, nodal_fields=NODAL_FIELDS, skip=()):
-    """Return a list of failure strings (empty == parity)."""
+    """Return a list of failure strings (empty == parity); markers None: nodal only."""
     bad = []
-    if not np.array_equal(gpu_m.t, ora_m.t):
+    if gpu_m is not None and not np.array_equal(gpu_m.t, ora_m.t):
         bad.append("marker types differ")
-    for f in MARKER_FIELDS:
+    for f in (MARKER_FIELDS if gpu_m is not None else ()):
         if f in skip:
             continue
         a, b = gpu_m[f], ora_m[f]
@@ -83,6 +89,22 @@ def run_oracle(cfg, ops, markers=None, nodal=None):
             ret["after"] = o.compute_after_solve(float(args[0]), int(args[1]))
         elif name == "time":
             o.set_time(float(args[0]), float(args[1]))
+        elif name == "stress":
+            ret["stress"] = o.recompute_stress_strainrate(float(args[0]), int(args[1]))
+        elif name == "c2b":
+            o.interpolate_center2basic(float(args[0]))
+        elif name == "c2bold":
+            o.interpolate_center2basic_old()
+        elif name == "b2c":
+            o.interpolate_basic2center()
+        elif name == "copy":
+            o.copyarrays(int(args[0]))
+        elif name == "tstep":
+            rc, r = o.calc_timestep(int(args[0]))
+            assert rc == 0, rc
+            ret["tstep"] = r
+        elif name == "heat":
+            o.temperature_markers(float(args[0]))
         else:
             raise ValueError(op)
     return o.markers, o.nodal, ret
@@ -111,6 +133,20 @@ def run_gpu(lib, cfg, ops, markers=None, nodal=None, ctx=None):
             ret["after"] = ctx.compute_after_solve(float(args[0]), int(args[1]))
         elif name == "time":
             ctx.set_time(float(args[0]), float(args[1]))
+        elif name == "stress":
+            ret["stress"] = ctx.recompute_stress_strainrate(float(args[0]), int(args[1]))
+        elif name == "c2b":
+            ctx.interpolate_center2basic(float(args[0]))
+        elif name == "c2bold":
+            ctx.interpolate_center2basic_old()
+        elif name == "b2c":
+            ctx.interpolate_basic2center()
+        elif name == "copy":
+            ctx.copyarrays(int(args[0]))
+        elif name == "tstep":
+            ret["tstep"] = ctx.calc_timestep(int(args[0]))
+        elif name == "heat":
+            ctx.temperature_markers(float(args[0]))
         else:
             raise ValueError(op)
     ctx.download_markers(m)
@@ -141,11 +177,42 @@ def load_fixture(path):
         ef = {f: z["out_m_" + f] for f in MARKER_FIELDS}
         ef["t"] = z["out_m_t"]
         out["markers"] = MarkerArrays(len(z["in_m_x"]), ef)
-        out["nodal"] = {k: z["out_n_" + k] for k in NODAL_FIELDS}
+        out["nodal"] = {k: _field(z, "out_n_" + k, grid.nodenum) for k in NODAL_FIELDS}
     else:
         markers = MarkerArrays(0)
-    nodal = {k: np.array(z["in_n_" + k]) for k in NODAL_FIELDS}
+    nodal = {k: np.array(_field(z, "in_n_" + k, grid.nodenum)) for k in NODAL_FIELDS}
     cfg = synth.Config(os.path.basename(path), grid, p, mats, markers, nodal,
                        float(z["timestep"]), float(z["timesum"]))
     ops = json.loads(str(z["ops"]))
     return cfg, ops, out, z
+
+
+def load_post_fixture(path):
+    """(cfg, runs) from a post-solve golden .npz: cfg holds the grid, flags
+    and nodal input (fields absent from the file are zero); runs maps a tag
+    ('ops', 'tstep0', 'tstep1') to (ops, {field: expected output} for the
+    fields the ops changed, reference scalars)."""
+    import json
+    from velo2cycles_amd import synth
+    from velo2cycles_amd.abi import GridArrays, Materials, Params, MarkerArrays
+    z = np.load(path)
+    grid = GridArrays(z["gx"], z["gy"], float(z["xsize"]), float(z["ysize"]))
+    p = Params()
+    for k, v in json.loads(str(z["params"])).items():
+        setattr(p, k, v)
+    mats = Materials()
+    mats.ntypes = int(z["mat_ntypes"])
+    for k in "nu dh dv ss mm gg n0 n1 ro bb aa cp kt kf kp ht".split():
+        getattr(mats, k)[:] = list(z["mat_" + k])
+    mats.im[:] = [int(v) for v in z["mat_im"]]
+    nodal = {k: np.array(_field(z, "in_n_" + k, grid.nodenum)) for k in NODAL_FIELDS}
+    cfg = synth.Config(os.path.basename(path), grid, p, mats, MarkerArrays(0), nodal,
+                       float(z["timestep"]), float(z["timesum"]))
+    runs = {}
+    for tag, pre in (("ops", ""), ("tstep0", "tstep0_"), ("tstep1", "tstep1_")):
+        ops = json.loads(str(z[pre + "opslist"]))
+        out = {k[len(pre) + 6:]: z[k] for k in z.files if k.startswith(pre + "out_n_")}
+        scal = {k[len(pre) + 2:]: float(z[k][0]) for k in z.files if k.startswith(pre + "g_")}
+        scal["dt"] = float(z[pre + "ret"][2])
+        runs[tag] = (ops, out, scal)
+    return cfg, runs, float(z["maxtmstep"])

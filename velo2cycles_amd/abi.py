@@ -30,7 +30,9 @@ NODAL_FIELDS = (
     "wtphisum phisum sxysum exysum esum wtsxxsum psum sxxsum exxsum wtvxsum vxsum wtvysum vysum "
     "mrx0 mry0 "
     "pes phis_it syields nuvp_it sliprs RK_sliprs_final RK_phidts_final RK_sliprs_solved "
-    "RK_phidts_solved RK_deltas RK_sliprs0 RK_sliprs1 RK_phidts0 RK_phidts1"
+    "RK_phidts_solved RK_deltas RK_sliprs0 RK_sliprs1 RK_phidts0 RK_phidts1 "
+    "deltasxx deltasxy deltaVs exxs exxes sxxes prs "
+    "tk2 tol0 tol1"
 ).split()
 NF = len(NODAL_FIELDS)
 FIELD_ID = {n: i for i, n in enumerate(NODAL_FIELDS)}
@@ -38,7 +40,7 @@ FIELD_ID = {n: i for i, n in enumerate(NODAL_FIELDS)}
 MARKER_FIELDS = ("x y k p xx xy exx exy e phi vx vy C d w a b L L0 V0 fric_static C0").split()
 
 KERNELS = ("m2n_cells", "m2n_nodes", "n2m_cells", "n2m_nodes", "advect", "sort",
-           "before", "after", "local", "mlp", "m2n_props", "n2m_update")
+           "before", "after", "local", "mlp", "m2n_props", "n2m_update", "post")
 
 ERRORS = {0: "ok", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECAPACITY",
           -5: "EUNSUPPORTED", -6: "ENOGPU"}
@@ -182,6 +184,29 @@ class MarkerArrays:
         return MarkerArrays(self.n, d)
 
 
+class StressStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("maxds", "minnu", "maxnu", "maxdphi", "maxnvp")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TimestepParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("maxtmstep", "tcnvp", "tcd", "tcphih", "dtfactor", "lambda0")]
+
+    @classmethod
+    def default(cls, maxtmstep: float = 3.15576e7):
+        """input_RSF.c:17-20 values; lambda0 NaN (uninitialised in the reference)."""
+        return cls(maxtmstep, 0.2, 1e-3, 0.2, 1.0, float("nan"))
+
+
+class TimestepResult(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dt", "dtnvp", "dtd", "dtphi", "dtL", "maxvel", "maxVx")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 def load_lib(path: str = LIB_PATH) -> C.CDLL:
     if not os.path.exists(path):
         raise RuntimeError(f"libv2c.so not built at {path}: run __graft_entry__.build()")
@@ -220,6 +245,11 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                        ("v2c_local_set_guess", [P(C.c_double), P(C.c_double), C.c_long]),
                        ("v2c_local_get_guess", [P(C.c_double), P(C.c_double), C.c_long]),
                        ("v2c_kernel_time", [C.c_int, P(C.c_double), P(C.c_long)]),
+                       ("v2c_recompute_stress_strainrate", [C.c_double, C.c_int, P(StressStats)]),
+                       ("v2c_interpolate_center2basic", [C.c_double]),
+                       ("v2c_interpolate_center2basic_old", []), ("v2c_interpolate_basic2center", []),
+                       ("v2c_copyarrays", [C.c_int]), ("v2c_temperature_markers", [C.c_double]),
+                       ("v2c_calc_timestep", [C.c_int, P(TimestepParams), P(TimestepResult)]),
                        ("v2c_reset_timing", [])]:
         fn = getattr(lib, name)
         fn.argtypes = [C.c_void_p] + args
@@ -300,6 +330,33 @@ class Context:
 
     def marker2nodes(self):
         self._check(self.lib.v2c_marker2nodes(self.h), "marker2nodes")
+
+    def recompute_stress_strainrate(self, dt, rk=0) -> dict:
+        st = StressStats()
+        self._check(self.lib.v2c_recompute_stress_strainrate(self.h, dt, rk, C.byref(st)),
+                    "recompute_stress_strainrate")
+        return st.as_dict()
+
+    def interpolate_center2basic(self, dt):
+        self._check(self.lib.v2c_interpolate_center2basic(self.h, dt), "interpolate_center2basic")
+
+    def interpolate_center2basic_old(self):
+        self._check(self.lib.v2c_interpolate_center2basic_old(self.h), "interpolate_center2basic_old")
+
+    def interpolate_basic2center(self):
+        self._check(self.lib.v2c_interpolate_basic2center(self.h), "interpolate_basic2center")
+
+    def copyarrays(self, rk=0):
+        self._check(self.lib.v2c_copyarrays(self.h, rk), "copyarrays")
+
+    def temperature_markers(self, heatdif):
+        self._check(self.lib.v2c_temperature_markers(self.h, heatdif), "temperature_markers")
+
+    def calc_timestep(self, oldornew, params=None) -> dict:
+        p = params or TimestepParams.default()
+        r = TimestepResult()
+        self._check(self.lib.v2c_calc_timestep(self.h, oldornew, C.byref(p), C.byref(r)), "calc_timestep")
+        return r.as_dict()
 
     def nodes2markers(self):
         self._check(self.lib.v2c_nodes2markers(self.h), "nodes2markers")

@@ -49,6 +49,8 @@ struct v2c_ctx {
     uint64_t *d_k64_in = nullptr, *d_k64_out = nullptr;
     long *d_nsel = nullptr;
     long *h_nsel = nullptr;   // pinned: mover count read back by the ordering
+    void *d_post = nullptr;   // per-block partials of the post-solve routines
+    size_t post_bytes = 0;
     // local update
     double *d_mx = nullptr;
     uint8_t *d_fault = nullptr;
@@ -314,7 +316,9 @@ static const char *const kNodalNames[V2C_NF] = {
     "wtvxsum", "vxsum", "wtvysum", "vysum", "mrx0", "mry0",
     "pes", "phis_it", "syields", "nuvp_it", "sliprs", "RK_sliprs_final", "RK_phidts_final",
     "RK_sliprs_solved", "RK_phidts_solved", "RK_deltas", "RK_sliprs0", "RK_sliprs1",
-    "RK_phidts0", "RK_phidts1"};
+    "RK_phidts0", "RK_phidts1",
+    "deltasxx", "deltasxy", "deltaVs", "exxs", "exxes", "sxxes", "prs",
+    "tk2", "tol0", "tol1"};
 
 const char *v2c_nodal_name(int field)
 {
@@ -329,7 +333,7 @@ static void free_all(v2c_ctx *c)
                     c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_prop, c->d_red, c->d_counters, c->d_sort_tmp,
                     c->d_order2, c->d_skey2, c->d_tile_count, c->d_tile_off, c->d_mask,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
-                    c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm};
+                    c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm, c->d_post};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
@@ -1158,3 +1162,5 @@ int v2c_kernel_time(v2c_ctx *c, int kernel, double *ms, long *launches)
 }  // extern "C"
 
 #include "v2c_ensemble.hip"
+#include "v2c_post.hip"
+#include "v2c_heat.hip"

@@ -1,0 +1,146 @@
+// v2c_heat.hip -- marker temperatures after the heat solve: the marker part
+// of solve_temperature (temperature_iterate.c:181-419) on the device.
+//
+//   pass 1, per marker (physical order): fluid markers take the new nodal
+//     temperature; rock markers get the reset of newly incoming markers, the
+//     subgrid diffusion toward the old nodal temperature and the solved
+//     increment, and leave their node weight w and w * dT in a scratch pair;
+//   pass 2, per cell (sorted order): the weights of the cell's markers summed
+//     left to right into its upper-left node (tol0, tol1), then tol1 / tol0;
+//   pass 3, per marker: the antidiffusion correction from that node.
+// No atomics: each node's sums are one thread's fold over its cell's markers
+// in (sub-cell, marker id) order, so runs are bitwise reproducible.
+// Included at the end of v2c_api.hip: it uses the context internals.
+
+namespace v2c {
+
+constexpr int HEAT_BLOCK = 256;
+
+__device__ __forceinline__ bool heat_inside(const DevGrid &g, double x, double y)
+{
+    return x > 0 && y > 0 && x < g.xsize && y < g.ysize;
+}
+
+// interpolate_temperature (interpolation.c:1492-1533) at an inside point of
+// cell (X.s, Y.s): the new (tk) and old (tk2) nodal temperatures.
+__device__ __forceinline__ void heat_tk(const KArgs &a, const Axis4 &X, const Axis4 &Y, double x, double y,
+                                        double &TK, double &TK2)
+{
+    Stencil s;
+    s.e = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    s.n = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    s.m3 = (uint32_t)(X.s * a.g.ynumy + Y.s);
+    const W4 w = wts(s);
+    TK = bil(NOD(a, V2C_N_tk), s.m3, a.g.ynumy, w);
+    TK2 = bil(NOD(a, V2C_N_tk2), s.m3, a.g.ynumy, w);
+}
+
+// Pass 1 (temperature_iterate.c:196-208, 213-331).
+__global__ __launch_bounds__(HEAT_BLOCK) void k_heat_markers(KArgs a, double heatdif)
+{
+    const long m = (long)blockIdx.x * HEAT_BLOCK + threadIdx.x;
+    if (m >= a.n) return;
+    const DevGrid &g = a.g;
+    const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
+    double w = 0.0, wd = 0.0;
+    const int t = a.t[m];
+    if (heat_inside(g, x, y) && t < 100) {
+        Axis4 X, Y;
+        key_axes(a, a.key[m], X, Y);
+        double TK, TK2;
+        heat_tk(a, X, Y, x, y, TK, TK2);
+        if (t >= 50) {
+            MK(a, MF_K, m) = TK;
+        } else {
+            double mk = MK(a, MF_K, m);
+            if (mk <= 0) {
+                mk = TK2;
+                if (mk < a.p.tdeep) mk = a.p.tdeep;
+            }
+            if (heatdif != 0) {
+                const double dtk = TK2 - mk;
+                const double dx = X.g[2] - X.g[1], dy = Y.g[2] - Y.g[1];
+                const double swt1 = 1.0 / dx / dy;
+                const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
+                const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+                // k and rho*Cp at the marker, nodes in the reference's loop order
+                const double *kt = NOD(a, V2C_N_kt), *cp = NOD(a, V2C_N_cp), *ro = NOD(a, V2C_N_ro);
+                const long ny = g.ynumy, m4 = (long)X.s * ny + Y.s;
+                const long n4[4] = {m4, m4 + 1, m4 + ny, m4 + ny + 1};
+                const double iv[4] = {(1.0 - cx) * (1.0 - cy), (1.0 - cx) * cy, cx * (1.0 - cy), cx * cy};
+                double mkt = 0, mrocp = 0;
+                for (int k = 0; k < 4; k++) {
+                    mkt += kt[n4[k]] * iv[k];
+                    mrocp += cp[n4[k]] * ro[n4[k]] * iv[k];
+                }
+                double dtk1 = -heatdif * mkt / mrocp * 2.0 * (1.0 / dx / dx + 1.0 / dy / dy) * a.timestep;
+                if (dtk1 < -150.0) dtk1 = -150.0;
+                dtk1 = dtk * (1.0 - exp(dtk1));
+                mk += dtk1;
+                w = (1.0 - cx) * (1.0 - cy) * swt1;
+                wd = dtk1 * w;
+            }
+            mk += (TK - TK2);
+            MK(a, MF_K, m) = mk;
+        }
+    }
+    a.prop[m] = w;
+    a.prop[a.cap + m] = wd;
+}
+
+// Pass 2 (temperature_iterate.c:183-184, 325-364): node (m1, m2) collects the
+// markers of cell (m1, m2) in sorted order.
+__global__ __launch_bounds__(HEAT_BLOCK) void k_heat_nodes(KArgs a)
+{
+    const long m4 = (long)blockIdx.x * HEAT_BLOCK + threadIdx.x;
+    if (m4 >= a.nodenum) return;
+    const long ny = a.g.ynumy, m2 = m4 % ny, m1 = m4 / ny;
+    double t0 = 0.0, t1 = 0.0;
+    if (m1 < a.g.xnumx - 1 && m2 < ny - 1) {
+        const long cell = m1 * (ny - 1) + m2;
+        const uint32_t b = a.sub_start[4 * cell], e = a.sub_start[4 * cell + 4];
+        for (uint32_t i = b; i < e; i++) {
+            const uint32_t m = a.order[i];
+            t0 += a.prop[m];
+            t1 += a.prop[a.cap + m];
+        }
+    }
+    if (t0) t1 /= t0;
+    NOD(a, V2C_N_tol0)[m4] = t0;
+    NOD(a, V2C_N_tol1)[m4] = t1;
+}
+
+// Pass 3 (temperature_iterate.c:367-416).
+__global__ __launch_bounds__(HEAT_BLOCK) void k_heat_antidiff(KArgs a)
+{
+    const long m = (long)blockIdx.x * HEAT_BLOCK + threadIdx.x;
+    if (m >= a.n) return;
+    const double x = MK(a, MF_X, m), y = MK(a, MF_Y, m);
+    if (!(heat_inside(a.g, x, y) && a.t[m] < 50)) return;
+    Axis4 X, Y;
+    key_axes(a, a.key[m], X, Y);
+    const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    const double tol1 = NOD(a, V2C_N_tol1)[(long)X.s * a.g.ynumy + Y.s];
+    if (tol1) MK(a, MF_K, m) -= tol1 * ((1.0 - cx) * (1.0 - cy));
+}
+
+}  // namespace v2c
+
+extern "C" int v2c_temperature_markers(v2c_ctx *c, double heatdif)
+{
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_order(c);
+    if (rc) return rc;
+    const KArgs a = make_args(c);
+    rc = launch(c, V2C_K_POST, [&] {
+        if (c->n > 0) hipLaunchKernelGGL(k_heat_markers, dim3(blocks_for(c->n, HEAT_BLOCK)), dim3(HEAT_BLOCK), 0,
+                                         c->stream, a, heatdif);
+        hipLaunchKernelGGL(k_heat_nodes, dim3(blocks_for(c->nodenum, HEAT_BLOCK)), dim3(HEAT_BLOCK), 0, c->stream, a);
+        if (heatdif != 0 && c->n > 0)
+            hipLaunchKernelGGL(k_heat_antidiff, dim3(blocks_for(c->n, HEAT_BLOCK)), dim3(HEAT_BLOCK), 0, c->stream, a);
+    });
+    if (rc) return rc;
+    return finish(c);
+}

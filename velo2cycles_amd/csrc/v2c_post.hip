@@ -1,0 +1,568 @@
+// v2c_post.hip -- the reference's per-node routines between the Stokes solve
+// and the next marker2nodes, on the device-resident nodal fields:
+// recompute_stress_strainrate, interpolate_center2basic(_old),
+// interpolate_basic2center, copyarrays and calc_timestep.
+//
+// Every node is one thread in node order (m3 = m1*ynumy + m2).  No kernel
+// writes a field it reads at another node, so each routine is one launch.
+// The running minima and maxima of recompute_stress_strainrate and
+// calc_timestep are reduced as compositions of per-node update functions,
+// which reproduces a sequential MINV/MAXV fold over the nodes exactly (NaN
+// and resets included) in any grouping.
+// Included at the end of v2c_api.hip: it uses the context internals.
+
+namespace v2c {
+
+constexpr int POST_BLOCK = 256;
+
+__device__ __forceinline__ double ref_maxv(double a, double b) { return a >= b ? a : b; }
+__device__ __forceinline__ double ref_absv(double a) { return a >= 0 ? a : -a; }
+
+#define NF_(f) NOD(a, V2C_N_##f)
+
+// nueffcalc (momentum_continuity_addmatrix_checkerror.c:380-393)
+__device__ __forceinline__ double nueff_calc(double nvp, double ggeff, double dt, double nubeg, double &Z)
+{
+    Z = ggeff * dt / (ggeff * dt + nvp);
+    double nueff = Z * nvp;
+    if (nueff < nubeg) nueff = nubeg;
+    return nueff;
+}
+
+// ---- running minima and maxima -----------------------------------------------------
+// A running minimum m <- MINV(m, x) is the function f_x(m) = m <= x ? m : x
+// (f_x(NaN) = x; f_NaN = the constant NaN); a reset m <- r is the constant
+// c_r.  Compositions stay in {identity, f_x with x not NaN, c_r}:
+//   f_w . f_v = f_{f_w(v)},  f_w . c_r = c_{f_w(r)},  c_r . anything = c_r,
+// so an ordered tree of compositions gives the sequential fold bit for bit.
+// MAXV likewise with >=.
+struct Fold {
+    int kind;      // 0 identity, 1 running min/max with v, 2 constant v
+    double v;
+};
+
+template <bool MAX>
+__host__ __device__ inline double fold_step(double m, double x)
+{
+    return MAX ? (m >= x ? m : x) : (m <= x ? m : x);
+}
+
+template <bool MAX>
+__host__ __device__ inline Fold fold_of(double x)
+{
+    return x != x ? Fold{2, x} : Fold{1, x};
+}
+
+// b after a
+template <bool MAX>
+__host__ __device__ inline Fold fold_then(Fold a, Fold b)
+{
+    if (b.kind == 0) return a;
+    if (a.kind == 0 || b.kind == 2) return b;
+    return Fold{a.kind, fold_step<MAX>(a.v, b.v)};
+}
+
+template <bool MAX>
+__host__ __device__ inline double fold_apply(Fold f, double m)
+{
+    return f.kind == 0 ? m : (f.kind == 2 ? f.v : fold_step<MAX>(m, f.v));
+}
+
+// ---- recompute_stress_strainrate (momentum_continuity_iterate.c:214-263) ----
+constexpr int NSTAT = 5;   // maxds, minnu, maxnu, maxdphi, maxnvp
+__device__ constexpr bool STAT_MAX[NSTAT] = {true, false, true, true, false};
+
+template <int K>
+__device__ __forceinline__ Fold stat_then(Fold a, Fold b)
+{
+    return STAT_MAX[K] ? fold_then<true>(a, b) : fold_then<false>(a, b);
+}
+
+__device__ __forceinline__ void stat_compose(Fold *dst, const Fold *src)
+{
+    dst[0] = stat_then<0>(dst[0], src[0]);
+    dst[1] = stat_then<1>(dst[1], src[1]);
+    dst[2] = stat_then<2>(dst[2], src[2]);
+    dst[3] = stat_then<3>(dst[3], src[3]);
+    dst[4] = stat_then<4>(dst[4], src[4]);
+}
+
+// The routine's extrema as the sequential MINV/MAXV folds of its node loop
+// (each node's updates in the routine's order), composed over nodes in order.
+__global__ __launch_bounds__(POST_BLOCK) void k_stress(KArgs a, double dt, Fold *__restrict__ part)
+{
+    __shared__ Fold F[NSTAT][POST_BLOCK];
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    Fold st[NSTAT];
+    for (int k = 0; k < NSTAT; k++) st[k] = Fold{0, 0.0};
+    if (m3 < a.nodenum) {
+        const DevGrid &g = a.g;
+        const long ny = g.ynumy, m2 = m3 % ny, m1 = m3 / ny;
+        if (m1 >= 1 && m2 >= 1) {
+            const double *vx = NF_(vx), *vy = NF_(vy);
+            const long v0 = (m1 - 1) * ny + (m2 - 1), v1 = v0 + 1, v2 = v0 + ny, v3 = v2 + 1;
+            // sxxcalc (:446-473)
+            double Z;
+            const double nux = nueff_calc(NF_(nd)[v3], NF_(gd)[v3], dt, a.p.nubeg, Z);
+            const double exx = 0.5 * ((vx[v2] - vx[v0]) / (g.gx[m1] - g.gx[m1 - 1]) -
+                                      (vy[v1] - vy[v0]) / (g.gy[m2] - g.gy[m2 - 1]));
+            const double sxxe = NF_(sxxe)[v3];
+            const double sxx = 2.0 * nux * exx + sxxe * (1.0 - Z);
+            NF_(sxx)[m3] = sxx;
+            NF_(exx)[m3] = exx;
+            st[0] = fold_of<true>(ref_absv(sxx - sxxe));
+            NF_(deltasxx)[m3] = (sxx - sxxe) / a.timestep;
+            st[1] = fold_of<false>(nux);
+            st[2] = fold_of<true>(nux);
+            if (m1 < g.xnumx - 1 && m2 < ny - 1) {
+                // sxycalc (:521-569)
+                const double nuy = nueff_calc(NF_(nu)[v3], NF_(gg)[v3], dt, a.p.nubeg, Z);
+                const double exy1 = (vx[v3] - vx[v2]) / (g.gy[m2 + 1] - g.gy[m2 - 1]);
+                const double exy2 = (vy[v3] - vy[v1]) / (g.gx[m1 + 1] - g.gx[m1 - 1]);
+                const double exy = exy1 + exy2;
+                const double sxye = NF_(sxye)[v3];
+                const double sxy = 2.0 * nuy * exy + sxye * (1.0 - Z);
+                NF_(sxy)[m3] = sxy;
+                NF_(exy)[m3] = exy;
+                NF_(esp)[m3] = exy1 - exy2;
+                NF_(deltasxy)[m3] = (sxy - sxye) / a.timestep;
+                st[4] = fold_of<false>(NF_(nu)[m3] / NF_(gg)[m3]);
+                st[0] = fold_then<true>(st[0], fold_of<true>(ref_absv(sxy - sxye)));
+                st[3] = fold_of<true>(ref_absv(NF_(phis)[m3] - NF_(phis_it)[m3]));
+                st[1] = fold_then<false>(st[1], fold_of<false>(nuy));
+                st[2] = fold_then<true>(st[2], fold_of<true>(nuy));
+            }
+        }
+    }
+    for (int k = 0; k < NSTAT; k++) F[k][threadIdx.x] = st[k];
+    __syncthreads();
+    for (int s = 1; s < POST_BLOCK; s <<= 1) {           // ordered pairwise composition
+        if ((threadIdx.x & (2 * s - 1)) == 0) {
+            Fold l[NSTAT], r[NSTAT];
+            for (int k = 0; k < NSTAT; k++) { l[k] = F[k][threadIdx.x]; r[k] = F[k][threadIdx.x + s]; }
+            stat_compose(l, r);
+            for (int k = 0; k < NSTAT; k++) F[k][threadIdx.x] = l[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < NSTAT) part[(long)blockIdx.x * NSTAT + threadIdx.x] = F[threadIdx.x][0];
+}
+
+// One thread: block folds composed in order, applied to the routine's initial
+// values (0, -1e50, 1e50, 0, 1e30).
+__global__ void k_stress_final(const Fold *__restrict__ part, int nblocks, double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Fold acc[NSTAT];
+    for (int k = 0; k < NSTAT; k++) acc[k] = Fold{0, 0.0};
+    for (int b = 0; b < nblocks; b++) stat_compose(acc, part + (long)b * NSTAT);
+    out[0] = fold_apply<true>(acc[0], 0.0);
+    out[1] = fold_apply<false>(acc[1], -1e50);
+    out[2] = fold_apply<true>(acc[2], 1e50);
+    out[3] = fold_apply<true>(acc[3], 0.0);
+    out[4] = fold_apply<false>(acc[4], 1e30);
+}
+
+// ---- interpolate_center2basic (interpolation.c:1600-1637) ----------------------
+__global__ __launch_bounds__(POST_BLOCK) void k_center2basic(KArgs a, double dt)
+{
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const long ny = a.g.ynumy, m2 = m3 % ny, m1 = m3 / ny;
+    if (!(m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1)) return;
+    const double *exx = NF_(exx), *sxx = NF_(sxx), *sxxe = NF_(sxxe), *gd = NF_(gd), *ndm = NF_(nd_m2c);
+    const double *pr = NF_(pr);
+    const long j[4] = {m3, m3 + 1, m3 + ny, m3 + ny + 1};
+    double epl[4], el[4];
+    for (int k = 0; k < 4; k++) {
+        const long i = j[k];
+        epl[k] = exx[i] - (sxx[i] - sxxe[i]) / (2 * gd[i] * dt) - sxx[i] / (2 * ndm[i]);
+        el[k] = 2 * gd[i] * dt * exx[i] + sxxe[i];
+    }
+    NF_(exxs)[m3] = 0.25 * (exx[j[0]] + exx[j[1]] + exx[j[2]] + exx[j[3]]);
+    NF_(exxpl)[m3] = 0.25 * (epl[0] + epl[1] + epl[2] + epl[3]);
+    NF_(sxxs)[m3] = 0.25 * (sxx[j[0]] + sxx[j[1]] + sxx[j[2]] + sxx[j[3]]);
+    NF_(prs)[m3] = 0.25 * (pr[j[0]] + pr[j[1]] + pr[j[2]] + pr[j[3]]);
+    NF_(exxel)[m3] = 0.25 * (el[0] + el[1] + el[2] + el[3]);
+}
+
+// ---- interpolate_center2basic_old (interpolation.c:1575-1598) -----------------
+__global__ __launch_bounds__(POST_BLOCK) void k_center2basic_old(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const long ny = a.g.ynumy, m2 = m3 % ny, m1 = m3 / ny;
+    if (!(m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1)) return;
+    const double *exxe = NF_(exxe), *sxxe = NF_(sxxe), *sppe = NF_(sppe);
+    NF_(exxes)[m3] = 0.25 * (exxe[m3] + exxe[m3 + 1] + exxe[m3 + ny] + exxe[m3 + ny + 1]);
+    NF_(sxxes)[m3] = 0.25 * (sxxe[m3] + sxxe[m3 + 1] + sxxe[m3 + ny] + sxxe[m3 + ny + 1]);
+    NF_(sppes)[m3] = 0.25 * (sppe[m3] + sppe[m3 + 1] + sppe[m3 + ny] + sppe[m3 + ny + 1]);
+}
+
+// ---- interpolate_basic2center (interpolation.c:1640-1685) ---------------------
+// Pressure-node viscosity: harmonic bilinear average of the four basic nodes
+// around the pressure point, found by the cell search of that point.
+__global__ __launch_bounds__(POST_BLOCK) void k_basic2center(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const DevGrid &g = a.g;
+    const long ny = g.ynumy, m2 = m3 % ny, m1 = m3 / ny;
+    if (!(m1 >= 1 && m2 >= 1)) return;
+    const double x = (g.gx[m1] + g.gx[m1 - 1]) / 2, y = (g.gy[m2] + g.gy[m2 - 1]) / 2;
+    long m10 = find_x(g, x).s, m20 = find_y(g, y).s;
+    if (m10 < 1) m10 = 1;
+    if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
+    if (m20 < 1) m20 = 1;
+    if (m20 > ny - 3) m20 = ny - 3;
+    double e = (x - g.gx[m10]) / (g.gx[m10 + 1] - g.gx[m10]);
+    double n = (y - g.gy[m20]) / (g.gy[m20 + 1] - g.gy[m20]);
+    if (e < 0) e = 0.0;
+    if (n < 0) n = 0.0;
+    if (e > 1) e = 1.0;
+    if (n > 1) n = 1.0;
+    const double dm0 = (1.0 - e) * (1.0 - n), dm1 = (1.0 - e) * n, dm2 = e * (1.0 - n), dm3 = e * n;
+    const double *nu = NF_(nu);
+    const long m4 = m10 * ny + m20;
+    NF_(nd)[m3] = 1.0 / (dm0 * 1.0 / nu[m4] + dm1 * 1.0 / nu[m4 + 1] + dm2 * 1.0 / nu[m4 + ny] +
+                         dm3 * 1.0 / nu[m4 + ny + 1]);
+}
+
+// ---- copyarrays (momentum_continuity_iterate.c:265-283) ------------------------
+__global__ __launch_bounds__(POST_BLOCK) void k_copyarrays(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const long ny = a.g.ynumy, m2 = m3 % ny, m1 = m3 / ny;
+    if (!(m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1)) return;
+    const double solved = NF_(RK_sliprs_solved)[m3];
+    NF_(deltaVs)[m3] = (log(solved) - log(NF_(RK_sliprs_final)[m3])) / a.timestep;
+    NF_(sliprs)[m3] = solved;
+    NF_(RK_sliprs_final)[m3] = solved;
+    NF_(RK_phidts_final)[m3] = NF_(RK_phidts_solved)[m3];
+}
+
+// ---- calc_timestep (calc_RSF.c:352-572) ----------------------------------------
+constexpr int NTS = 6;   // dtnvp, dtd, dtphi, dtL (min); maxvel, maxVx (max)
+
+struct TsArgs {
+    double maxtmstep, tcnvp, tcd, tcphih, dtfactor, lambda0;
+    int oldornew;
+};
+
+// chi and the lambda it sets at an interior node (calc_RSF.c:464-481);
+// returns 1 when chi sets lambda, 0 when it leaves it (chi NaN), -1 for
+// chi == 0 (the reference exits there) and -2 for an unknown pmode.
+__device__ __forceinline__ int ts_lambda(const KArgs &a, const TsArgs &T, long m3, double &lam)
+{
+    double nps;
+    if (a.p.pmode == 0) nps = a.p.nstress;
+    else if (a.p.pmode == 1) nps = T.oldornew == 0 ? NF_(sppes)[m3] : NF_(prs)[m3];
+    else return -2;
+    const double bm = NF_(bm)[m3], gg = NF_(gg)[m3], Ls = NF_(Ls)[m3], as = NF_(as)[m3], bs = NF_(bs)[m3];
+    const double v = (3 * bm - 2 * gg) / (2 * (3 * bm + gg));
+    const double k = 2.0 / 3.14 * (gg / (1 - v)) / a.p.xstpx;
+    const double chi = 1.0 / 4.0 * pow(k * Ls / (as * nps) - (bs - as) / as, 2) - k * Ls / (as * nps);
+    if (chi > 0) {
+        lam = as * nps / (k * Ls - (bs - as) * nps);
+        return 1;
+    }
+    if (chi < 0) {
+        lam = 1 - (bs - as) * nps / (k * Ls);
+        return 1;
+    }
+    if (chi == 0) return -1;
+    return 0;
+}
+
+__device__ __forceinline__ bool ts_interior(const KArgs &a, long m3)
+{
+    const long ny = a.g.ynumy, m2 = m3 % ny, m1 = m3 / ny;
+    return m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1;
+}
+
+// Pass 1: the last lambda set in each block (node order); error flags.
+__global__ __launch_bounds__(POST_BLOCK) void k_ts_lambda(KArgs a, TsArgs T, int *__restrict__ bset,
+                                                          double *__restrict__ bval, int *__restrict__ err)
+{
+    __shared__ int last[POST_BLOCK];
+    __shared__ double val[POST_BLOCK];
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    int set = 0;
+    double lam = 0.0;
+    if (m3 < a.nodenum && a.p.RSFmode == 1 && ts_interior(a, m3)) {
+        const int r = ts_lambda(a, T, m3, lam);
+        if (r < 0) atomicOr(err, r == -1 ? 1 : 2);
+        set = r == 1;
+    }
+    last[threadIdx.x] = set ? (int)threadIdx.x : -1;
+    val[threadIdx.x] = lam;
+    __syncthreads();
+    for (int s = POST_BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) last[threadIdx.x] = max(last[threadIdx.x], last[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bset[blockIdx.x] = last[0] >= 0;
+        bval[blockIdx.x] = last[0] >= 0 ? val[last[0]] : 0.0;
+    }
+}
+
+// Pass 2 (one thread): lambda carried into every block.
+__global__ void k_ts_carry(const int *__restrict__ bset, const double *__restrict__ bval, int nblocks,
+                           double lambda0, double *__restrict__ carry)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double c = lambda0;
+    for (int b = 0; b < nblocks; b++) {
+        carry[b] = c;
+        if (bset[b]) c = bval[b];
+    }
+}
+
+// Pass 3: every interior node's update functions, composed in node order
+// within the block.
+__global__ __launch_bounds__(POST_BLOCK) void k_ts_node(KArgs a, TsArgs T, const double *__restrict__ carry,
+                                                        Fold *__restrict__ part)
+{
+    __shared__ int last[POST_BLOCK];
+    __shared__ double val[POST_BLOCK];
+    __shared__ Fold F[NTS][POST_BLOCK];
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    const bool in = m3 < a.nodenum && ts_interior(a, m3);
+    const bool rsf = in && a.p.RSFmode == 1;
+    int set = 0;
+    double lam_set = 0.0;
+    if (rsf) set = ts_lambda(a, T, m3, lam_set) == 1;
+    // lambda in force at this node: the last one set at or before it
+    last[threadIdx.x] = set ? (int)threadIdx.x : -1;
+    val[threadIdx.x] = lam_set;
+    __syncthreads();
+    for (int s = 1; s < POST_BLOCK; s <<= 1) {          // inclusive max-scan
+        const int v = threadIdx.x >= (unsigned)s ? last[threadIdx.x - s] : -1;
+        __syncthreads();
+        last[threadIdx.x] = max(last[threadIdx.x], v);
+        __syncthreads();
+    }
+    Fold f[NTS];
+    for (int k = 0; k < NTS; k++) f[k] = Fold{0, 0.0};
+    if (in) {
+        const DevGrid &g = a.g;
+        const long ny = g.ynumy, m2 = m3 % ny, m1 = m3 / ny;
+        const double dx = g.gx[m1 + 1] - g.gx[m1], dy = g.gy[m2 + 1] - g.gy[m2];
+        const double fw = 2 * 0.5 * (dx + dy);
+        const double Vf = NF_(RK_sliprs_final)[m3], Vs = NF_(RK_sliprs_solved)[m3];
+        const double num2c = NF_(nu_m2c)[m3];
+        double nuvp;
+        if (T.oldornew == 0) {
+            if (num2c > 1.00e+26) {
+                const double sxxes = NF_(sxxes)[m3], sxye = NF_(sxye)[m3];
+                const double sigin = sqrt(sxxes * sxxes + sxye * sxye);
+                nuvp = (sigin > 0 && Vf > 0) ? num2c * sigin / (2 * num2c * Vf / fw + sigin) : num2c;
+            } else {
+                nuvp = 5.00e+26;
+            }
+        } else {
+            nuvp = num2c > 1.00e+26 ? NF_(nu)[m3] : 5.00e+26;
+        }
+        f[0] = fold_of<false>(T.tcnvp * nuvp / NF_(gg)[m3]);
+        const double ux = T.oldornew == 1 ? NF_(vx)[m3] : NF_(mvx)[m3];
+        const double uy = T.oldornew == 1 ? NF_(vy)[m3] : NF_(mvy)[m3];
+        if (ux != 0.0 && uy != 0.0) {
+            f[5] = fold_of<true>(ref_maxv(ref_absv(ux), ref_absv(uy)));
+            const double maxdr = ref_maxv(ref_absv(ux / dx), ref_absv(uy / dy));
+            f[1] = fold_of<false>(T.tcd * 1.0 / maxdr);
+        }
+        if (rsf) {
+            const int li = last[threadIdx.x];
+            double lam = li >= 0 ? val[li] : carry[blockIdx.x];
+            lam = ref_minv(lam, 1.0 / 5.0);
+            const double bs = NF_(bs)[m3], Ls = NF_(Ls)[m3], V0s = NF_(V0s)[m3];
+            if (bs > 0) {
+                if (T.oldornew == 0 && Vf > 0) {
+                    const double phisc = V0s / Ls * exp(-NF_(phis)[m3]);
+                    f[2] = fold_of<false>(T.tcphih * 1.0 / phisc);
+                    f[3] = fold_of<false>(lam * Ls / Vf);
+                } else if (T.oldornew == 1 && Vs > 0) {
+                    const double phisc = V0s / Ls * exp(-NF_(phis_it)[m3]);
+                    f[2] = fold_of<false>(T.tcphih * 1.0 / phisc);
+                    f[3] = fold_of<false>(lam * Ls / Vs);
+                }
+            } else {
+                f[2] = Fold{2, T.maxtmstep};
+                f[3] = Fold{2, T.maxtmstep};
+            }
+        }
+        if (T.oldornew == 0 && Vf > 0) f[4] = fold_of<true>(Vf);
+        if (T.oldornew == 1 && Vs > 0) f[4] = fold_of<true>(Vs);
+    }
+    for (int k = 0; k < NTS; k++) F[k][threadIdx.x] = f[k];
+    __syncthreads();
+    for (int s = 1; s < POST_BLOCK; s <<= 1) {           // ordered pairwise composition
+        if ((threadIdx.x & (2 * s - 1)) == 0) {
+            const int j = threadIdx.x + s;
+            for (int k = 0; k < 4; k++) F[k][threadIdx.x] = fold_then<false>(F[k][threadIdx.x], F[k][j]);
+            for (int k = 4; k < NTS; k++) F[k][threadIdx.x] = fold_then<true>(F[k][threadIdx.x], F[k][j]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < NTS) part[(long)blockIdx.x * NTS + threadIdx.x] = F[threadIdx.x][0];
+}
+
+// Pass 4 (one thread): blocks composed in order, applied to the reference's
+// initial values; out = dt, dtnvp, dtd, dtphi, dtL, maxvel, maxVx.
+__global__ void k_ts_final(const Fold *__restrict__ part, int nblocks, TsArgs T, double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Fold acc[NTS];
+    for (int k = 0; k < NTS; k++) acc[k] = Fold{0, 0.0};
+    for (int b = 0; b < nblocks; b++) {
+        for (int k = 0; k < 4; k++) acc[k] = fold_then<false>(acc[k], part[(long)b * NTS + k]);
+        for (int k = 4; k < NTS; k++) acc[k] = fold_then<true>(acc[k], part[(long)b * NTS + k]);
+    }
+    const double dtnvp = fold_apply<false>(acc[0], 1e+30), dtd = fold_apply<false>(acc[1], 1e+30);
+    const double dtphi = fold_apply<false>(acc[2], 1e+30), dtL = fold_apply<false>(acc[3], 1e+30);
+    const double maxvel = fold_apply<true>(acc[4], 0.0), maxVx = fold_apply<true>(acc[5], 0.0);
+    out[0] = T.dtfactor * ref_minv(T.maxtmstep, ref_minv(dtphi, ref_minv(ref_minv(dtL, dtnvp), dtd)));
+    out[1] = dtnvp;
+    out[2] = dtd;
+    out[3] = dtphi;
+    out[4] = dtL;
+    out[5] = maxvel;
+    out[6] = maxVx;
+}
+
+__global__ void k_post_clear(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+#undef NF_
+
+}  // namespace v2c
+
+using namespace v2c;
+
+// ---- host side -------------------------------------------------------------------
+
+// Device scratch of the post-solve routines: per-block partials, sized for the
+// context's node count on first use.
+static int post_scratch(v2c_ctx *c, size_t bytes)
+{
+    if (bytes <= c->post_bytes) return 0;
+    if (c->d_post) (void)hipFree(c->d_post);
+    c->d_post = nullptr;
+    c->post_bytes = 0;
+    HIPCHK(c, hipMalloc(&c->d_post, bytes));
+    c->post_bytes = bytes;
+    return 0;
+}
+
+extern "C" {
+
+int v2c_recompute_stress_strainrate(v2c_ctx *c, double dt, int rk, v2c_stress_stats *stats)
+{
+    (void)rk;
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const unsigned nb = blocks_for(c->nodenum, POST_BLOCK);
+    int rc = post_scratch(c, sizeof(Fold) * NSTAT * (size_t)nb + sizeof(double) * NSTAT);
+    if (rc) return rc;
+    Fold *part = (Fold *)c->d_post;
+    double *out = (double *)(part + (size_t)NSTAT * nb);
+    const KArgs a = make_args(c);
+    rc = launch(c, V2C_K_POST, [&] {
+        hipLaunchKernelGGL(k_stress, dim3(nb), dim3(POST_BLOCK), 0, c->stream, a, dt, part);
+        hipLaunchKernelGGL(k_stress_final, dim3(1), dim3(64), 0, c->stream, part, (int)nb, out);
+    });
+    if (rc) return rc;
+    double h[NSTAT];
+    HIPCHK(c, hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (stats) {
+        stats->maxds = h[0];
+        stats->minnu = h[1];
+        stats->maxnu = h[2];
+        stats->maxdphi = h[3];
+        stats->maxnvp = h[4];
+    }
+    return 0;
+}
+
+#define POST_NODE_LAUNCH(kernel, ...)                                                               \
+    do {                                                                                            \
+        if (!c) return V2C_EINVAL;                                                                  \
+        HIPCHK(c, hipSetDevice(c->device));                                                         \
+        const KArgs a = make_args(c);                                                               \
+        int rc_ = launch(c, V2C_K_POST, [&] {                                                       \
+            hipLaunchKernelGGL(kernel, dim3(blocks_for(c->nodenum, POST_BLOCK)), dim3(POST_BLOCK), 0, \
+                               c->stream, a, ##__VA_ARGS__);                                        \
+        });                                                                                         \
+        if (rc_) return rc_;                                                                        \
+        return finish(c);                                                                           \
+    } while (0)
+
+int v2c_interpolate_center2basic(v2c_ctx *c, double dt) { POST_NODE_LAUNCH(k_center2basic, dt); }
+int v2c_interpolate_center2basic_old(v2c_ctx *c) { POST_NODE_LAUNCH(k_center2basic_old); }
+int v2c_interpolate_basic2center(v2c_ctx *c) { POST_NODE_LAUNCH(k_basic2center); }
+int v2c_copyarrays(v2c_ctx *c, int rk)
+{
+    (void)rk;
+    POST_NODE_LAUNCH(k_copyarrays);
+}
+
+#undef POST_NODE_LAUNCH
+
+void v2c_timestep_params_default(v2c_timestep_params *p)
+{
+    if (!p) return;
+    p->maxtmstep = 3.15576e7;
+    p->tcnvp = 0.2;
+    p->tcd = 1e-3;
+    p->tcphih = 0.2;
+    p->dtfactor = 1.0;
+    p->lambda0 = NAN;
+}
+
+int v2c_calc_timestep(v2c_ctx *c, int oldornew, const v2c_timestep_params *p, v2c_timestep_result *r)
+{
+    if (!c || !p || !r || (oldornew != 0 && oldornew != 1)) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const unsigned nb = blocks_for(c->nodenum, POST_BLOCK);
+    // layout: Fold part[NTS*nb] | double bval[nb], carry[nb], out[8] | int bset[nb], err
+    const size_t bytes = sizeof(Fold) * NTS * nb + sizeof(double) * (2 * (size_t)nb + 8) + sizeof(int) * (nb + 1);
+    int rc = post_scratch(c, bytes);
+    if (rc) return rc;
+    Fold *part = (Fold *)c->d_post;
+    double *bval = (double *)(part + (size_t)NTS * nb), *carry = bval + nb, *out = carry + nb;
+    int *bset = (int *)(out + 8), *err = bset + nb;
+    TsArgs T{p->maxtmstep, p->tcnvp, p->tcd, p->tcphih, p->dtfactor, p->lambda0, oldornew};
+    const KArgs a = make_args(c);
+    rc = launch(c, V2C_K_POST, [&] {
+        hipLaunchKernelGGL(k_post_clear, dim3(1), dim3(64), 0, c->stream, err, 1);
+        hipLaunchKernelGGL(k_ts_lambda, dim3(nb), dim3(POST_BLOCK), 0, c->stream, a, T, bset, bval, err);
+        hipLaunchKernelGGL(k_ts_carry, dim3(1), dim3(64), 0, c->stream, bset, bval, (int)nb, p->lambda0, carry);
+        hipLaunchKernelGGL(k_ts_node, dim3(nb), dim3(POST_BLOCK), 0, c->stream, a, T, carry, part);
+        hipLaunchKernelGGL(k_ts_final, dim3(1), dim3(64), 0, c->stream, part, (int)nb, T, out);
+    });
+    if (rc) return rc;
+    double h[7];
+    int e = 0;
+    HIPCHK(c, hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (e & 2) return set_err(c, V2C_EUNSUPPORTED, "calc_timestep: pmode %d (only 0 and 1 define the normal stress)",
+                              c->prm.pmode);
+    if (e & 1) return set_err(c, V2C_EUNSUPPORTED, "calc_timestep: chi is zero at a node (the reference exits)");
+    r->dt = h[0];
+    r->dtnvp = h[1];
+    r->dtd = h[2];
+    r->dtphi = h[3];
+    r->dtL = h[4];
+    r->maxvel = h[5];
+    r->maxVx = h[6];
+    return 0;
+}
+
+}  // extern "C"

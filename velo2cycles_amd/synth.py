@@ -421,3 +421,67 @@ def config_ensemble(n_models=64, seed=DEFAULT_SEED, small=False):
                            seed + 1000 + k)
         cfg.params.a_vw = cfg.params.b_vw + amb
         yield k, cfg
+
+
+def post_solve_nodal(cfg, seed=DEFAULT_SEED, degenerate=True):
+    """Nodal state as the post-solve routines (recompute_stress_strainrate,
+    interpolate_center2basic(_old), interpolate_basic2center, copyarrays,
+    calc_timestep) find it after a Stokes solve: velocities with random
+    gradients around the corner flow, positive viscosities / moduli of
+    mixed magnitudes (some above the 1e26 'locked' threshold), RSF parameters
+    of the velocity-weakening and -strengthening kinds and slip rates from
+    1e-12 to 1 m/s.  degenerate=True also plants the branches the reference
+    handles specially: zero and NaN velocities, b <= 0 nodes, a = 0 nodes
+    (chi NaN, so lambda carries over from the previous node), non-positive
+    slip rates.  Returns a new nodal dict; cfg.nodal is left as it is."""
+    rng = np.random.default_rng(seed)
+    nn = cfg.grid.nodenum
+    nod = {k: np.array(v, dtype=np.float64, copy=True) for k, v in cfg.nodal.items()}
+    for k in NODAL_FIELDS:
+        nod.setdefault(k, np.zeros(nn))
+
+    def logu(lo, hi):
+        return 10.0 ** rng.uniform(lo, hi, nn)
+
+    nod["vx"] = rng.normal(0, 1e-9, nn)
+    nod["vy"] = rng.normal(0, 1e-9, nn)
+    nod["mvx"] = rng.normal(0, 1e-9, nn)
+    nod["mvy"] = rng.normal(0, 1e-9, nn)
+    for k in ("nd", "nu", "nd_m2c"):
+        nod[k] = logu(18, 24)
+    nod["nu_m2c"] = logu(18, 27.5)
+    for k in ("gd", "gg"):
+        nod[k] = rng.uniform(1e10, 7e10, nn)
+    nod["bm"] = rng.uniform(3e10, 9e10, nn)
+    for k in ("sxxe", "sxye", "sppe", "exxe"):
+        nod[k] = rng.normal(0, 1e7 if k != "exxe" else 1e-14, nn)
+    nod["pr"] = rng.uniform(1e6, 1e9, nn)
+    nod["sppes"] = rng.uniform(1e6, 3e8, nn)
+    nod["prs"] = rng.uniform(1e6, 3e8, nn)
+    nod["sxxes"] = rng.normal(0, 1e7, nn)
+    nod["phis"] = rng.uniform(-5, 25, nn)
+    nod["phis_it"] = nod["phis"] + rng.normal(0, 0.1, nn)
+    nod["as"] = rng.uniform(0.008, 0.02, nn)
+    nod["bs"] = rng.uniform(0.001, 0.017, nn)
+    nod["Ls"] = rng.uniform(0.005, 0.05, nn)
+    nod["V0s"] = np.full(nn, 4e-9)
+    nod["RK_sliprs_final"] = logu(-12, 0)
+    nod["RK_sliprs_solved"] = logu(-12, 0)
+    nod["RK_phidts_solved"] = rng.normal(0, 1e-3, nn)
+    nod["RK_phidts_final"] = rng.normal(0, 1e-3, nn)
+    if degenerate:
+        idx = rng.permutation(nn)
+        ny = cfg.grid.ynumy
+        first = ny + 1                                 # first interior node keeps a != 0
+        pick = [i for i in idx if i > first]
+        z, zn, bneg, a0, vneg = pick[:8], pick[8:12], pick[12:20], pick[20:26], pick[26:34]
+        nod["vx"][z[:4]] = 0.0
+        nod["mvy"][z[4:]] = 0.0
+        nod["vx"][zn[:2]] = np.nan
+        nod["mvx"][zn[2:]] = np.nan
+        nod["bs"][bneg[:4]] = 0.0
+        nod["bs"][bneg[4:]] = -0.002
+        nod["as"][a0] = 0.0
+        nod["RK_sliprs_final"][vneg[:4]] = 0.0
+        nod["RK_sliprs_solved"][vneg[4:]] = -1e-9
+    return nod
