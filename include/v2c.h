@@ -311,6 +311,32 @@ typedef struct {
 void v2c_timestep_params_default(v2c_timestep_params *p);
 int v2c_calc_timestep(v2c_ctx *ctx, int oldornew, const v2c_timestep_params *p, v2c_timestep_result *out);
 
+/* ---- boundary-condition tail of marker2nodes and the heat solve -----------
+ * Boundary equations in the reference's layout (headevps.c:197-202, read by
+ * load_save.c:279-304): parameter p of node m3 is 3*m3 (P), 3*m3+1 (vx),
+ * 3*m3+2 (vy) or 3*nodenum+m3 (T); bondm[p] = 0 for none, else the row k
+ * (1 <= k < nbond) of bondn / bondv, which reads
+ *     PAR = bondv[k][0] + sum_i bondv[k][i+1] * PAR(bondn[k][i] - 1)
+ * over the i with bondn[k][i] != 0.  The arrays are copied to the device. */
+int v2c_set_boundary(v2c_ctx *ctx, const long *bondm /* 4*nodenum */, const long *bondn /* nbond x 3 */,
+                     const double *bondv /* nbond x 4 */, long nbond);
+/* vxvy_boundarycorrection (momentum_continuity_addmatrix_checkerror.c:
+ * 1000-1108), P_boundarycorrection (:1110-1184) and tkrecalc
+ * (temperature_iterate.c:828-891): mvx / mvy, sppe and tk replaced by the
+ * solution of the identity-plus-boundary system (free rows keep their value,
+ * bounded rows take their equation; the coupled parameter of a vx, vy or P
+ * row is located with the reference's integer formulas).  The system is
+ * solved on the device by substitution to a fixed point, which chains of
+ * equations referring to bounded parameters reach after one sweep per link;
+ * equations that refer to each other in a cycle are solved if the
+ * substitution settles within 64 sweeps and return V2C_EUNSUPPORTED
+ * otherwise.  A row referring to an absent earlier row or to its own
+ * parameter returns V2C_EUNSUPPORTED (the reference exits on the first and
+ * on a zero diagonal). */
+int v2c_vxvy_boundarycorrection(v2c_ctx *ctx);
+int v2c_P_boundarycorrection(v2c_ctx *ctx);
+int v2c_tkrecalc(v2c_ctx *ctx);
+
 /* ---- marker temperatures after the heat solve ------------------------------
  * The marker part of solve_temperature (temperature_iterate.c:181-419), on
  * the device-resident markers: fluid markers (types 50..99) take the new

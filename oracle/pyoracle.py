@@ -77,6 +77,9 @@ def lib():
         L.or_calc_timestep.restype = C.c_int
         L.or_temperature_markers.argtypes = [P(OrState), C.c_double]
         L.or_temperature_markers.restype = None
+        for name in ("or_vxvy_boundarycorrection", "or_P_boundarycorrection", "or_tkrecalc"):
+            getattr(L, name).argtypes = [P(OrState), P(C.c_long), P(C.c_long), P(C.c_double)]
+            getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
 
@@ -148,6 +151,16 @@ class Oracle:
         r = TimestepResult()
         rc = lib().or_calc_timestep(C.byref(self.st), oldornew, C.byref(p), C.byref(r))
         return rc, r.as_dict()
+
+    def boundary(self, which, bondm, bondn, bondv):
+        """which: 'vxvy', 'P' or 'tk'; returns the oracle's code (0, or -1 / -2
+        where the reference exits)."""
+        fn = {"vxvy": "or_vxvy_boundarycorrection", "P": "or_P_boundarycorrection", "tk": "or_tkrecalc"}[which]
+        bm = np.ascontiguousarray(bondm, dtype=np.int64)
+        bn = np.ascontiguousarray(bondn, dtype=np.int64)
+        bv = np.ascontiguousarray(bondv, dtype=np.float64)
+        return getattr(lib(), fn)(C.byref(self.st), bm.ctypes.data_as(C.POINTER(C.c_long)),
+                                  bn.ctypes.data_as(C.POINTER(C.c_long)), _dptr(bv))
 
     def temperature_markers(self, heatdif):
         lib().or_temperature_markers(C.byref(self.st), heatdif)

@@ -13,6 +13,7 @@
  * global of the table after the last op.  Ops:
  *   m2n | n2m | move | before:<dt>:<rk> | after:<dt>:<rk> | ductile | allinteri
  *   stress:<dt>:<rk> | c2b:<dt> | c2bold:<dt> | b2c | copy:<rk> | tstep:<oldornew>
+ *   bcvxvy | bcp | bctk
  * (tstep leaves the returned time step in drv_ret[2]).
  */
 #ifndef _GNU_SOURCE
@@ -52,6 +53,7 @@ typedef struct { const char *name; char kind; void *ptr; long cap; } entry;
 #define D(a) {#a, 'd', (void *)(a), (long)(sizeof(a) / sizeof(double))}
 #define S(a) {#a, 'd', (void *)&(a), 1}
 #define L(a) {#a, 'l', (void *)&(a), 1}
+#define LA(a) {#a, 'l', (void *)(a), (long)(sizeof(a) / sizeof(long))}
 #define I(a) {#a, 'i', (void *)&(a), 1}
 static entry table[] = {
     /* grid and scalars */
@@ -87,6 +89,8 @@ static entry table[] = {
     /* post-solve scalars (headevps.c:120-122, 181, 339, 345-346) */
     S(maxvel), S(maxdphi), S(maxnvp), S(dtL), S(dtphi), S(dtd), S(dtnvp),
     S(maxtmstep), S(tcnvp), S(tcd), S(tcphih), S(dtfactor),
+    /* boundary equations (headevps.c:197-202), input only */
+    LA(bondm), LA(bondn), D(bondv),
     /* pure-function op arrays */
     D(dv_sxy), D(dv_sxx), D(dv_tk), D(dv_pb), D(dv_out), {"dv_type", 'c', (void *)dv_type, DRV_MAX},
     D(ai_x), D(ai_y), D(ai_vx), D(ai_vy), D(ai_esp), D(ai_ee), L(drv_n), D(drv_ret),
@@ -120,6 +124,7 @@ static long used_count(const entry *e)
 {
     long nodenum_ = xnumx * ynumy;
     if (e->cap == 1) return 1;
+    if (!strncmp(e->name, "bond", 4)) return 0;
     if (!strcmp(e->name, "gx")) return xnumx;
     if (!strcmp(e->name, "gy")) return ynumy;
     if (!strncmp(e->name, "mark", 4) && e->cap == MAXMRK) return marknum + 1;
@@ -180,6 +185,9 @@ int main(int argc, char **argv)
         else if (!strcmp(op, "b2c")) interpolate_basic2center();
         else if (sscanf(op, "copy:%d", &rk) == 1) copyarrays(rk);
         else if (sscanf(op, "tstep:%d", &rk) == 1) drv_ret[2] = calc_timestep(rk);
+        else if (!strcmp(op, "bcvxvy")) vxvy_boundarycorrection();
+        else if (!strcmp(op, "bcp")) P_boundarycorrection();
+        else if (!strcmp(op, "bctk")) tkrecalc();
         else { fprintf(stderr, "ref_driver: unknown op %s\n", op); return 4; }
     }
     for (long i = 0; i < nodenum; i++) {

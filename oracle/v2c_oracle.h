@@ -71,6 +71,10 @@ void or_copyarrays(or_state *s, int rk);
 int or_calc_timestep(or_state *s, int oldornew, const v2c_timestep_params *p, v2c_timestep_result *r);
 /* marker temperatures after the heat solve */
 void or_temperature_markers(or_state *s, double heatdif);
+/* boundary-condition tail: 0, or -1 / -2 where the reference exits */
+int or_vxvy_boundarycorrection(or_state *s, const long *bondm, const long *bondn, const double *bondv);
+int or_P_boundarycorrection(or_state *s, const long *bondm, const long *bondn, const double *bondv);
+int or_tkrecalc(or_state *s, const long *bondm, const long *bondn, const double *bondv);
 
 #ifdef __cplusplus
 }

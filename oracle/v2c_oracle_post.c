@@ -18,6 +18,7 @@
  *                                solve_temperature after the solve)
  */
 #include <math.h>
+#include <stdlib.h>
 #include "v2c_oracle.h"
 
 #define N(f) (s->nod[V2C_N_##f])
@@ -333,4 +334,207 @@ void or_temperature_markers(or_state *s, double heatdif)
             if (tol1[m4]) mk[m3] -= (tol1[m4] * ((1.0 - celdx) * (1.0 - celdy)));
         }
     }
+}
+
+/* ---- boundary-condition tail ------------------------------------------------
+ * Restatement of the reference's row-by-row band elimination
+ * (gausmat3, gausevps.c:216-385): each added row is reduced by the earlier
+ * rows it refers to (fill-in included) and stored normalised; the solve is a
+ * back substitution from the last row.  Rows are added in the routines' node
+ * order; an unknown without a row solves to 0.  Returns 0, -1 where a row
+ * refers to an absent earlier row, -2 for a zero diagonal (where the
+ * reference exits). */
+typedef struct {
+    long n;
+    double *bufv, *fre, *sol, *val;
+    long *lin, *pos, *num;
+    long cur, cap;
+} band_t;
+
+static int band_init(band_t *B, long n)
+{
+    B->n = n;
+    B->bufv = calloc(n, sizeof(double));
+    B->fre = calloc(n, sizeof(double));
+    B->sol = calloc(n, sizeof(double));
+    B->pos = calloc(n, sizeof(long));
+    B->num = calloc(n, sizeof(long));
+    B->cap = 8 * n + 16;
+    B->val = malloc(B->cap * sizeof(double));
+    B->lin = malloc(B->cap * sizeof(long));
+    B->cur = 0;
+    return B->bufv && B->fre && B->sol && B->pos && B->num && B->val && B->lin ? 0 : -3;
+}
+
+static void band_free(band_t *B)
+{
+    free(B->bufv); free(B->fre); free(B->sol); free(B->pos); free(B->num); free(B->val); free(B->lin);
+}
+
+/* row r: r's own coefficient 1 first, then nw couplings (w[i], -k[i]) */
+static int band_add(band_t *B, long r, double rhs, int nw, const long *w, const double *k)
+{
+    long lo = r, hi = r;
+    if (nw > 0) hi = w[nw - 1];        /* the reference starts from the last listed parameter */
+    if (hi < r) hi = r;                /* ... and widens the range by every nonzero entry, r included */
+    B->fre[r] = rhs;
+    B->bufv[r] += 1.0;
+    for (int i = 0; i < nw; i++) {
+        B->bufv[w[i]] += -k[i];
+        if (-k[i]) {
+            if (lo > w[i]) lo = w[i];
+            if (hi < w[i]) hi = w[i];
+        }
+    }
+    for (long m2 = lo; m2 < r; m2++) {
+        if (!B->bufv[m2]) continue;
+        const double iv = B->bufv[m2];
+        B->bufv[m2] = 0;
+        if (!B->num[m2]) return -1;
+        for (long m3 = 1; m3 < B->num[m2]; m3++) B->bufv[B->lin[B->pos[m2] + m3]] -= iv * B->val[B->pos[m2] + m3];
+        B->fre[r] -= B->fre[m2] * iv;
+        const long last = B->lin[B->pos[m2] + B->num[m2] - 1];
+        if (hi < last) hi = last;
+    }
+    if (!B->bufv[r]) return -2;
+    B->pos[r] = B->cur;
+    B->num[r] = 0;
+    const double iv = B->bufv[r];
+    for (long m2 = r; m2 <= hi; m2++) {
+        if (B->bufv[m2]) {
+            if (B->cur >= B->cap) {
+                B->cap *= 2;
+                B->val = realloc(B->val, B->cap * sizeof(double));
+                B->lin = realloc(B->lin, B->cap * sizeof(long));
+            }
+            B->lin[B->cur] = m2;
+            B->val[B->cur] = B->bufv[m2] / iv;
+            B->cur++;
+            B->num[r]++;
+        }
+        B->bufv[m2] = 0;
+    }
+    B->fre[r] /= iv;
+    return 0;
+}
+
+static void band_solve(band_t *B)
+{
+    for (long m1 = B->n - 1; m1 >= 0; m1--) {
+        if (B->num[m1]) {
+            double iv = B->fre[m1];
+            for (long m2 = 0; m2 < B->num[m1]; m2++) iv -= B->val[B->pos[m1] + m2] * B->sol[B->lin[B->pos[m1] + m2]];
+            B->sol[m1] = iv;
+        } else {
+            B->sol[m1] = 0;
+        }
+    }
+}
+
+/* A bounded row: its constant and the listed couplings, each parameter
+ * located by map(q) from bondn = q. */
+static int bc_listed(const long *bondn, const double *bondv, long k, long (*map)(long, long, long), long nn,
+                     long u, long *w, double *kv)
+{
+    int nw = 0;
+    for (int i = 0; i < 3; i++) {
+        const long q = bondn[3 * k + i];
+        if (!q) continue;
+        w[nw] = map(q, nn, u);
+        kv[nw] = bondv[4 * k + i + 1];
+        nw++;
+    }
+    return nw;
+}
+
+static long map_vx(long q, long nn, long u) { (void)nn; (void)u; return ((q - 1) - 1) / 3; }
+static long map_vy(long q, long nn, long u) { (void)u; return ((q - 1) - 2) / 3 + nn; }
+static long map_p(long q, long nn, long u) { (void)nn; (void)u; return (q - 1) / 3; }
+static long map_t(long q, long nn, long u) { (void)u; return q - 1 - 3 * nn; }
+
+int or_vxvy_boundarycorrection(or_state *s, const long *bondm, const long *bondn, const double *bondv)
+{
+    const long nx = s->grid.xnumx, ny = s->grid.ynumy, nn = nx * ny;
+    band_t B;
+    if (band_init(&B, 2 * nn)) return -3;
+    double *mvx = N(mvx), *mvy = N(mvy);
+    int rc = 0;
+    long w[3];
+    double kv[3];
+    for (long m1 = 0; m1 < nx && !rc; m1++)
+        for (long m2 = 0; m2 < ny && !rc; m2++) {
+            const long m3 = m1 * ny + m2;
+            if (m2 < ny - 1) {
+                const long k = bondm[3 * m3 + 1];
+                if (!k) rc = band_add(&B, m3, mvx[m3], 0, w, kv);
+                else rc = band_add(&B, m3, bondv[4 * k], bc_listed(bondn, bondv, k, map_vx, nn, m3, w, kv), w, kv);
+            }
+            if (!rc && m1 < nx - 1) {
+                const long k = bondm[3 * m3 + 2];
+                if (!k) rc = band_add(&B, m3 + nn, mvy[m3], 0, w, kv);
+                else rc = band_add(&B, m3 + nn, bondv[4 * k], bc_listed(bondn, bondv, k, map_vy, nn, m3, w, kv), w, kv);
+            }
+        }
+    if (!rc) {
+        band_solve(&B);
+        for (long m3 = 0; m3 < nn; m3++) {
+            const long m2 = m3 % ny, m1 = m3 / ny;
+            mvx[m3] = m2 < ny - 1 ? B.sol[m3] : 0.0;
+            mvy[m3] = m1 < nx - 1 ? B.sol[m3 + nn] : 0.0;
+        }
+    }
+    band_free(&B);
+    return rc;
+}
+
+int or_P_boundarycorrection(or_state *s, const long *bondm, const long *bondn, const double *bondv)
+{
+    const long nx = s->grid.xnumx, ny = s->grid.ynumy, nn = nx * ny;
+    band_t B;
+    if (band_init(&B, nn)) return -3;
+    double *sppe = N(sppe);
+    int rc = 0;
+    long w[3];
+    double kv[3];
+    for (long m1 = 0; m1 < nx && !rc; m1++)
+        for (long m2 = 0; m2 < ny && !rc; m2++) {
+            const long m3 = m1 * ny + m2;
+            if (!(m1 && m2)) continue;
+            const long k = bondm[3 * m3];
+            if (!k || s->prm.compressiblemode == 1) rc = band_add(&B, m3, sppe[m3], 0, w, kv);
+            else rc = band_add(&B, m3, bondv[4 * k], bc_listed(bondn, bondv, k, map_p, nn, m3, w, kv), w, kv);
+        }
+    if (!rc) {
+        band_solve(&B);
+        for (long m3 = 0; m3 < nn; m3++) {
+            const long m2 = m3 % ny, m1 = m3 / ny;
+            sppe[m3] = (m1 && m2) ? B.sol[m3] : 0.0;
+        }
+    }
+    band_free(&B);
+    return rc;
+}
+
+int or_tkrecalc(or_state *s, const long *bondm, const long *bondn, const double *bondv)
+{
+    const long nx = s->grid.xnumx, ny = s->grid.ynumy, nn = nx * ny;
+    band_t B;
+    if (band_init(&B, nn)) return -3;
+    double *tk = N(tk);
+    int rc = 0;
+    long w[3];
+    double kv[3];
+    for (long m1 = 0; m1 < nx && !rc; m1++)
+        for (long m2 = 0; m2 < ny && !rc; m2++) {
+            const long m3 = m1 * ny + m2;
+            const long k = bondm[3 * nn + m3];
+            if (!k) rc = band_add(&B, m3, tk[m3], 0, w, kv);
+            else rc = band_add(&B, m3, bondv[4 * k], bc_listed(bondn, bondv, k, map_t, nn, m3, w, kv), w, kv);
+        }
+    if (!rc) {
+        band_solve(&B);
+        for (long m3 = 0; m3 < nn; m3++) tk[m3] = B.sol[m3];
+    }
+    band_free(&B);
+    return rc;
 }

@@ -2,7 +2,9 @@
 hot-path routines, compiled in place from /root/reference/src
 (oracle/_ref/ref_driver, built by oracle/Makefile), single-threaded.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # step fixtures and pure functions
+    python tests/golden/make_golden.py post       # post-solve nodal routines
+    python tests/golden/make_golden.py bc         # boundary-condition tail
 
 Each fixture holds its complete input (grid, flags, rock table, markers, nodal
 arrays) and the reference's output after the listed ops, so the tests never
@@ -96,10 +98,39 @@ def post_cases():
         print("wrote", name, cfg.grid.nodenum, "nodes")
 
 
+def bc_cases():
+    """Boundary-condition tail (vxvy_boundarycorrection, P_boundarycorrection,
+    tkrecalc) with the equations of synth.boundary_equations."""
+    for name, comp, seed in (("bc_walls_chains", 0, 4), ("bc_compressible", 1, 5)):
+        cfg = synth.config_grid(25, 15, 24e3, 14e3, 2, refine=True, seed=seed)
+        cfg.params.compressiblemode = comp
+        rng = np.random.default_rng(seed)
+        nn = cfg.grid.nodenum
+        inp = {"mvx": rng.normal(0, 1e-9, nn), "mvy": rng.normal(0, 1e-9, nn),
+               "sppe": rng.normal(0, 1e6, nn), "tk": rng.uniform(273, 1600, nn)}
+        cfg.nodal.update(inp)
+        bm, bn, bv = synth.boundary_equations(cfg, seed)
+        extra = {"bondm": ("l", bm), "bondn": ("l", bn.ravel()), "bondv": ("d", bv.ravel())}
+        d = {"gx": cfg.grid.gx, "gy": cfg.grid.gy, "xsize": np.array(cfg.grid.xsize),
+             "ysize": np.array(cfg.grid.ysize), "params": np.array(json.dumps(cfg.params.as_dict())),
+             "bondm": bm, "bondn": bn, "bondv": bv}
+        for k, v in inp.items():
+            d["in_" + k] = v
+        for op, fields in (("bcvxvy", ("mvx", "mvy")), ("bcp", ("sppe",)), ("bctk", ("tk",))):
+            nd = refrun.run(cfg, [op], extra=extra)[1]
+            for k in fields:
+                d[f"out_{op}_{k}"] = nd[k]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+        print("wrote", name, nn, "nodes", len(bn), "equations")
+
+
 def main():
     assert refrun.available(), "build oracle/_ref first (make -C oracle)"
     if sys.argv[1:] == ["post"]:
         post_cases()
+        return
+    if sys.argv[1:] == ["bc"]:
+        bc_cases()
         return
     step = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m", "move"]
     cfg = synth.config_edge()

@@ -145,3 +145,74 @@ def test_marker_temperatures_match_oracle(gpu, heatdif):
     bad = compare(gm, om, gn, on)
     assert not bad, bad
     ctx.close()
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "bc_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[:-4])
+def test_boundary_tail_matches_reference_golden(gpu, path):
+    from tests.test_oracle_golden import load_bc_fixture
+    cfg, bc, z = load_bc_fixture(path)
+    print("case", os.path.basename(path), flush=True)
+    for which, op, fields in (("vxvy", "bcvxvy", ("mvx", "mvy")), ("P", "bcp", ("sppe",)), ("tk", "bctk", ("tk",))):
+        ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=1, lib=gpu)
+        ctx.set_all_nodal(cfg.nodal)
+        ctx.set_boundary(*bc)
+        {"vxvy": ctx.vxvy_boundarycorrection, "P": ctx.P_boundarycorrection, "tk": ctx.tkrecalc}[which]()
+        for k in fields:
+            got, exp = ctx.get_nodal(k), z[f"out_{op}_{k}"]
+            scale = max(np.max(np.abs(exp)), 1e-300)
+            assert np.max(np.abs(got - exp)) <= 1e-12 * scale, (which, k)
+        ctx.close()
+
+
+def test_boundary_tail_matches_oracle_large(gpu):
+    cfg = synth.config_grid(401, 101, 400e3, 100e3, 1, refine=True, seed=61)
+    rng = np.random.default_rng(61)
+    nn = cfg.grid.nodenum
+    for k, s in (("mvx", 1e-9), ("mvy", 1e-9), ("sppe", 1e6)):
+        cfg.nodal[k] = rng.normal(0, s, nn)
+    cfg.nodal["tk"] = rng.uniform(273, 1600, nn)
+    bc = synth.boundary_equations(cfg, 61)
+    print("case bc large", nn, len(bc[1]), flush=True)
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=1, lib=gpu)
+    ctx.set_all_nodal(cfg.nodal)
+    ctx.set_boundary(*bc)
+    ctx.vxvy_boundarycorrection()
+    ctx.P_boundarycorrection()
+    ctx.tkrecalc()
+    o = Oracle(cfg)
+    for which in ("vxvy", "P", "tk"):
+        assert o.boundary(which, *bc) == 0
+    for k in ("mvx", "mvy", "sppe", "tk"):
+        got, exp = ctx.get_nodal(k), o.nodal[k]
+        assert np.max(np.abs(got - exp)) <= 1e-12 * np.max(np.abs(exp)), k
+    ctx.close()
+
+
+def test_boundary_tail_errors(gpu):
+    cfg = synth.config_grid(25, 15, 24e3, 14e3, 1, refine=True, seed=62)
+    ny = cfg.grid.ynumy
+    bm, bn, bv = synth.boundary_equations(cfg, 62, extra=False)
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=1, lib=gpu)
+    ctx.set_all_nodal(cfg.nodal)
+    with pytest.raises(V2CError, match="no boundary"):
+        ctx.tkrecalc()
+    # a cycle that never settles: vx(3, 4) = 1 + vx(3, 5) and vx(3, 5) = 1 + vx(3, 4)
+    a, b = 3 * ny + 4, 3 * ny + 5
+    bn2 = np.vstack([bn, [[3 * b + 1 + 1, 0, 0], [3 * a + 1 + 1, 0, 0]]])
+    bv2 = np.vstack([bv, [[1.0, 1.0, 0.0, 0.0], [1.0, 1.0, 0.0, 0.0]]])
+    bm2 = bm.copy()
+    bm2[3 * a + 1], bm2[3 * b + 1] = len(bn2) - 2, len(bn2) - 1
+    ctx.set_boundary(bm2, bn2, bv2)
+    with pytest.raises(V2CError, match="cycle"):
+        ctx.vxvy_boundarycorrection()
+    # a row tied to the vx of (2, ny-1), which has no row (the reference exits)
+    bn3 = np.vstack([bn, [[3 * (2 * ny + ny - 1) + 1 + 1, 0, 0]]])
+    bv3 = np.vstack([bv, [[0.0, 1.0, 0.0, 0.0]]])
+    bm3 = bm.copy()
+    bm3[3 * (3 * ny + 4) + 1] = len(bn3) - 1
+    ctx.set_boundary(bm3, bn3, bv3)
+    with pytest.raises(V2CError, match="absent"):
+        ctx.vxvy_boundarycorrection()
+    assert Oracle(cfg).boundary("vxvy", bm3, bn3, bv3) == -1
+    ctx.close()

@@ -65,3 +65,33 @@ def test_oracle_post_routines_match_reference_bitwise(path):
         assert r["dt"] == scal["dt"], tag
         for k in ("dtnvp", "dtd", "dtphi", "dtL", "maxvel"):
             assert r[k] == scal[k], (tag, k)
+
+
+BC_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "bc_*.npz")))
+
+
+def load_bc_fixture(path):
+    import json
+    from velo2cycles_amd import synth
+    from velo2cycles_amd.abi import GridArrays, Params
+    z = np.load(path)
+    cfg = synth.config_grid(len(z["gx"]), len(z["gy"]), float(z["xsize"]), float(z["ysize"]), 1)
+    cfg.grid = GridArrays(z["gx"], z["gy"], float(z["xsize"]), float(z["ysize"]))
+    p = Params()
+    for k, v in json.loads(str(z["params"])).items():
+        setattr(p, k, v)
+    cfg.params = p
+    cfg.nodal = {k: np.zeros(cfg.grid.nodenum) for k in NODAL_FIELDS}
+    for k in ("mvx", "mvy", "sppe", "tk"):
+        cfg.nodal[k] = np.array(z["in_" + k])
+    return cfg, (z["bondm"], z["bondn"], z["bondv"]), z
+
+
+@pytest.mark.parametrize("path", BC_FIXTURES, ids=lambda p: os.path.basename(p)[:-4])
+def test_oracle_boundary_tail_matches_reference_bitwise(path):
+    cfg, bc, z = load_bc_fixture(path)
+    for which, op, fields in (("vxvy", "bcvxvy", ("mvx", "mvy")), ("P", "bcp", ("sppe",)), ("tk", "bctk", ("tk",))):
+        o = Oracle(cfg)
+        assert o.boundary(which, *bc) == 0
+        for k in fields:
+            assert np.array_equal(o.nodal[k], z[f"out_{op}_{k}"]), (which, k)

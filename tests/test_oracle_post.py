@@ -216,3 +216,71 @@ def test_heat_diffusion_single_marker_closed_form():
     assert np.allclose(o.markers["k"], expect, rtol=1e-13, atol=0)
     node = i * cfg.grid.ynumy + j
     assert np.allclose(o.nodal["tol1"][node], d1, rtol=1e-13)
+
+
+# ---- boundary-condition tail ---------------------------------------------------
+
+def _bc_case(seed=4, compressible=0, extra=True):
+    cfg = synth.config_grid(25, 15, 24e3, 14e3, 2, refine=True, seed=seed)
+    cfg.params.compressiblemode = compressible
+    rng = np.random.default_rng(seed)
+    nn = cfg.grid.nodenum
+    cfg.nodal["mvx"] = rng.normal(0, 1e-9, nn)
+    cfg.nodal["mvy"] = rng.normal(0, 1e-9, nn)
+    cfg.nodal["sppe"] = rng.normal(0, 1e6, nn)
+    cfg.nodal["tk"] = rng.uniform(273, 1600, nn)
+    return cfg, synth.boundary_equations(cfg, seed, extra=extra)
+
+
+def _bc_ref(cfg, bc, op):
+    bm, bn, bv = bc
+    extra = {"bondm": ("l", bm), "bondn": ("l", bn.ravel()), "bondv": ("d", bv.ravel())}
+    return refrun.run(cfg, [op], extra=extra)[1]
+
+
+@needs_ref
+@pytest.mark.parametrize("which,op,field", [("vxvy", "bcvxvy", ("mvx", "mvy")), ("P", "bcp", ("sppe",)),
+                                             ("tk", "bctk", ("tk",))])
+@pytest.mark.parametrize("compressible", [0, 1])
+def test_boundary_tail_matches_reference(which, op, field, compressible):
+    cfg, bc = _bc_case(compressible=compressible)
+    rn = _bc_ref(cfg, bc, op)
+    o = Oracle(cfg)
+    assert o.boundary(which, *bc) == 0
+    for k in NODAL_FIELDS:
+        assert np.array_equal(o.nodal[k], rn[k], equal_nan=True), k
+
+
+def test_boundary_chains_closed_form():
+    """Free rows keep their value, walls take their constant, free-slip rows
+    copy their neighbour, and a chain row combines rows that are themselves
+    bounded: x = c + k1 x1 + k2 x2 with x1, x2 already solved."""
+    cfg, bc = _bc_case(seed=9)
+    bm, bn, bv = bc
+    nx, ny = cfg.grid.xnumx, cfg.grid.ynumy
+    mvx0 = cfg.nodal["mvx"].copy()
+    tk0 = cfg.nodal["tk"].copy()
+    o = Oracle(cfg)
+    assert o.boundary("vxvy", *bc) == 0 and o.boundary("tk", *bc) == 0
+    vx, tk = o.nodal["mvx"], o.nodal["tk"]
+    m = lambda i, j: i * ny + j  # noqa: E731
+    assert vx[m(5, 5)] == mvx0[m(5, 5)]                     # free
+    assert vx[m(0, 5)] == 0.0 and vx[m(nx - 1, 5)] == 0.0   # walls
+    assert vx[m(5, 0)] == vx[m(5, 1)]                      # free slip
+    assert vx[m(2, 0)] == mvx0[m(2, 1)]
+    assert vx[m(1, 0)] == 1e-10 + 0.25 * mvx0[m(1, 1)] + 0.5 * mvx0[m(2, 1)]
+    assert np.all(vx[[m(i, ny - 1) for i in range(nx)]] == 0)  # no vx row on the last line
+    assert tk[m(5, 0)] == 273.0 and tk[m(5, ny - 1)] == 1600.0
+    assert tk[m(0, 4)] == tk0[m(1, 4)]
+    assert tk[m(0, 1)] == 10.0 + 0.9 * tk0[m(1, 2)]
+
+
+def test_boundary_absent_earlier_row_is_reported():
+    cfg, (bm, bn, bv) = _bc_case(seed=10, extra=False)
+    ny = cfg.grid.ynumy
+    # vx of node (3, 4) tied to the vx of (2, ny-1), which has no row
+    bn = np.vstack([bn, [[3 * (2 * ny + ny - 1) + 1 + 1, 0, 0]]])
+    bv = np.vstack([bv, [[0.0, 1.0, 0.0, 0.0]]])
+    bm = bm.copy()
+    bm[3 * (3 * ny + 4) + 1] = len(bn) - 1
+    assert Oracle(cfg).boundary("vxvy", bm, bn, bv) == -1

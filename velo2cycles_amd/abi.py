@@ -249,6 +249,8 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                        ("v2c_interpolate_center2basic", [C.c_double]),
                        ("v2c_interpolate_center2basic_old", []), ("v2c_interpolate_basic2center", []),
                        ("v2c_copyarrays", [C.c_int]), ("v2c_temperature_markers", [C.c_double]),
+                       ("v2c_set_boundary", [P(C.c_long), P(C.c_long), P(C.c_double), C.c_long]),
+                       ("v2c_vxvy_boundarycorrection", []), ("v2c_P_boundarycorrection", []), ("v2c_tkrecalc", []),
                        ("v2c_calc_timestep", [C.c_int, P(TimestepParams), P(TimestepResult)]),
                        ("v2c_reset_timing", [])]:
         fn = getattr(lib, name)
@@ -348,6 +350,27 @@ class Context:
 
     def copyarrays(self, rk=0):
         self._check(self.lib.v2c_copyarrays(self.h, rk), "copyarrays")
+
+    def set_boundary(self, bondm, bondn, bondv):
+        """Boundary equations: bondm over 4*nodenum parameters, bondn (nbond x 3)
+        and bondv (nbond x 4) rows, row 0 unused (include/v2c.h)."""
+        bm = np.ascontiguousarray(bondm, dtype=np.int64)
+        bn = np.ascontiguousarray(bondn, dtype=np.int64)
+        bv = np.ascontiguousarray(bondv, dtype=np.float64)
+        assert bm.size == 4 * self.nodenum and bn.shape[1:] == (3,) and bv.shape == (bn.shape[0], 4)
+        self._bond = (bm, bn, bv)
+        self._check(self.lib.v2c_set_boundary(self.h, bm.ctypes.data_as(C.POINTER(C.c_long)),
+                                              bn.ctypes.data_as(C.POINTER(C.c_long)), _dptr(bv), bn.shape[0]),
+                    "set_boundary")
+
+    def vxvy_boundarycorrection(self):
+        self._check(self.lib.v2c_vxvy_boundarycorrection(self.h), "vxvy_boundarycorrection")
+
+    def P_boundarycorrection(self):
+        self._check(self.lib.v2c_P_boundarycorrection(self.h), "P_boundarycorrection")
+
+    def tkrecalc(self):
+        self._check(self.lib.v2c_tkrecalc(self.h), "tkrecalc")
 
     def temperature_markers(self, heatdif):
         self._check(self.lib.v2c_temperature_markers(self.h, heatdif), "temperature_markers")

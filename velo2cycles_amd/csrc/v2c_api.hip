@@ -51,6 +51,11 @@ struct v2c_ctx {
     long *h_nsel = nullptr;   // pinned: mover count read back by the ordering
     void *d_post = nullptr;   // per-block partials of the post-solve routines
     size_t post_bytes = 0;
+    // boundary equations (v2c_set_boundary) and the substitution buffers
+    long *d_bondm = nullptr, *d_bondn = nullptr;
+    double *d_bondv = nullptr, *d_bcx = nullptr;
+    int *d_bcflag = nullptr;
+    long nbond = 0, bond_cap = 0;
     // local update
     double *d_mx = nullptr;
     uint8_t *d_fault = nullptr;
@@ -333,7 +338,8 @@ static void free_all(v2c_ctx *c)
                     c->d_order, c->d_cs, c->d_nod, c->d_part, c->d_prop, c->d_red, c->d_counters, c->d_sort_tmp,
                     c->d_order2, c->d_skey2, c->d_tile_count, c->d_tile_off, c->d_mask,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
-                    c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm, c->d_post};
+                    c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm, c->d_post, c->d_bondm, c->d_bondn,
+                    c->d_bondv, c->d_bcx, c->d_bcflag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
@@ -1164,3 +1170,4 @@ int v2c_kernel_time(v2c_ctx *c, int kernel, double *ms, long *launches)
 #include "v2c_ensemble.hip"
 #include "v2c_post.hip"
 #include "v2c_heat.hip"
+#include "v2c_bc.hip"

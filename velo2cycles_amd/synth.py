@@ -485,3 +485,78 @@ def post_solve_nodal(cfg, seed=DEFAULT_SEED, degenerate=True):
         nod["RK_sliprs_final"][vneg[:4]] = 0.0
         nod["RK_sliprs_solved"][vneg[4:]] = -1e-9
     return nod
+
+
+def boundary_equations(cfg, seed=DEFAULT_SEED, extra=True):
+    """Boundary equations in the reference's layout (bondm over 4*nodenum
+    parameters, bondn / bondv rows from 1; in2evps.c:395-720 builds them the
+    same way, a coupled parameter being the current one shifted by whole
+    nodes): no-penetration and free-slip walls for vx / vy, one fixed
+    pressure node, fixed top / bottom temperatures with insulating sides.
+    extra=True adds rows with two and three couplings, listed out of order,
+    and chains (a bounded parameter referring to another bounded one).
+    Returns (bondm, bondn, bondv)."""
+    rng = np.random.default_rng(seed)
+    nx, ny = cfg.grid.xnumx, cfg.grid.ynumy
+    nn = nx * ny
+    bondm = np.zeros(4 * nn, dtype=np.int64)
+    rows_n, rows_v = [[0, 0, 0]], [[0.0, 0.0, 0.0, 0.0]]
+
+    def add(p, const, couplings=()):
+        """couplings: (parameter number, koef) pairs; stored as number + 1."""
+        bondm[p] = len(rows_n)
+        n3, v4 = [0, 0, 0], [const, 0.0, 0.0, 0.0]
+        for i, (q, kf) in enumerate(couplings):
+            n3[i] = q + 1
+            v4[i + 1] = kf
+        rows_n.append(n3)
+        rows_v.append(v4)
+
+    def m(i, j):
+        return i * ny + j
+
+    for i in range(nx):
+        for j in range(ny):
+            m3 = m(i, j)
+            if j < ny - 1:                                   # vx
+                p = 3 * m3 + 1
+                if i in (0, nx - 1):
+                    add(p, 0.0)
+                elif j == 0:
+                    add(p, 0.0, [(3 * m(i, 1) + 1, 1.0)])
+                elif j == ny - 2:
+                    add(p, 0.0, [(3 * m(i, ny - 3) + 1, 1.0)])
+            if i < nx - 1:                                   # vy
+                p = 3 * m3 + 2
+                if j in (0, ny - 1):
+                    add(p, 0.0)
+                elif i == 0:
+                    add(p, 0.0, [(3 * m(1, j) + 2, 1.0)])
+                elif i == nx - 2:
+                    add(p, 0.0, [(3 * m(nx - 3, j) + 2, 1.0)])
+            pt = 3 * nn + m3                                 # T
+            if j == 0:
+                add(pt, 273.0)
+            elif j == ny - 1:
+                add(pt, 1600.0)
+            elif i == 0:
+                add(pt, 0.0, [(3 * nn + m(1, j), 1.0)])
+            elif i == nx - 1:
+                add(pt, 0.0, [(3 * nn + m(nx - 2, j), 1.0)])
+    add(3 * m(1, 1), 1e5)                                    # P
+    if extra:
+        # vx at the top-left interior corner: two couplings, the first to a
+        # later bounded row (a chain), listed in decreasing order
+        add(3 * m(1, 0) + 1, 1e-10, [(3 * m(2, 0) + 1, 0.5), (3 * m(1, 1) + 1, 0.25)])
+        # vy next to the right wall: three couplings, earlier and later rows
+        jj = ny // 2
+        add(3 * m(nx - 2, jj) + 2, 0.0, [(3 * m(nx - 3, jj) + 2, 0.5), (3 * m(nx - 2, jj - 1) + 2, 0.3),
+                                          (3 * m(nx - 2, jj + 1) + 2, 0.2)])
+        # pressure nodes tied to neighbours (one of them to the fixed node)
+        for _ in range(6):
+            i, j = int(rng.integers(2, nx - 2)), int(rng.integers(2, ny - 2))
+            add(3 * m(i, j), float(rng.normal(0, 1e5)), [(3 * m(i + 1, j), 0.5), (3 * m(i, j - 1), 0.5)])
+        add(3 * m(2, 1), 0.0, [(3 * m(1, 1), 1.0)])
+        # a temperature chain: corner -> side -> interior
+        add(3 * nn + m(0, 1), 10.0, [(3 * nn + m(0, 2), 0.9)])
+    return bondm, np.array(rows_n, dtype=np.int64), np.array(rows_v, dtype=np.float64)
