@@ -350,6 +350,62 @@ int v2c_tkrecalc(v2c_ctx *ctx);
  * outside the open domain are untouched, as in the reference. */
 int v2c_temperature_markers(v2c_ctx *ctx, double heatdif);
 
+/* ---- .prn checkpoint files (host only) -----------------------------------
+ * The reference's binary model file: written by save_model
+ * (load_save.c:353-527 with extra_save_grid / extra_save_marker,
+ * input_RSF.c:106-132), read by load_model (load_save.c:164-347).  Layout:
+ * four size bytes (int, long, float, double), the header, the rock table
+ * (markim as int, then 27 FP64 columns), one record per node (19 FP64, the
+ * four bondm entries as long, 2 FP64), the gridlines, boundary rows
+ * 1..bondnum-1 (4 FP64 + 3 long), timesave[5], and one record per marker
+ * (14 FP64, the type byte, 5 FP64).  Arrays are column-wise in file order;
+ * v2c_prn_read allocates them (v2c_prn_free releases them), v2c_prn_write
+ * takes them from the caller.  Files whose size bytes differ from this
+ * host's int/long/float/double are rejected, as the reference rejects
+ * them. */
+enum v2c_prn_rock {    /* rock[] columns after markim */
+    V2C_PRN_R_n0 = 0, V2C_PRN_R_n1, V2C_PRN_R_s0, V2C_PRN_R_s1, V2C_PRN_R_nu, V2C_PRN_R_dh,
+    V2C_PRN_R_dv, V2C_PRN_R_ss, V2C_PRN_R_mm, V2C_PRN_R_gg, V2C_PRN_R_ll, V2C_PRN_R_a0,
+    V2C_PRN_R_a1, V2C_PRN_R_b0, V2C_PRN_R_b1, V2C_PRN_R_e0, V2C_PRN_R_e1, V2C_PRN_R_f0,
+    V2C_PRN_R_f1, V2C_PRN_R_ro, V2C_PRN_R_bb, V2C_PRN_R_aa, V2C_PRN_R_cp, V2C_PRN_R_kt,
+    V2C_PRN_R_kf, V2C_PRN_R_kp, V2C_PRN_R_ht, V2C_PRN_NROCK
+};
+enum v2c_prn_node {    /* node[] columns; the file stores RK_sliprs_final twice */
+    V2C_PRN_N_sxxee = 0, V2C_PRN_N_sxx, V2C_PRN_N_exxee, V2C_PRN_N_exx, V2C_PRN_N_sxyee,
+    V2C_PRN_N_sxy, V2C_PRN_N_exyee, V2C_PRN_N_exy, V2C_PRN_N_sppee, V2C_PRN_N_pr,
+    V2C_PRN_N_phie, V2C_PRN_N_phis_it, V2C_PRN_N_mvxe, V2C_PRN_N_vx, V2C_PRN_N_mvye,
+    V2C_PRN_N_vy, V2C_PRN_N_pes, V2C_PRN_N_pee, V2C_PRN_N_RK_sliprs_final_a,
+    V2C_PRN_N_RK_sliprs_final, V2C_PRN_N_RK_phidts_final, V2C_PRN_NNODE
+};
+enum v2c_prn_mark {    /* mark[] columns */
+    V2C_PRN_M_x = 0, V2C_PRN_M_y, V2C_PRN_M_k, V2C_PRN_M_e, V2C_PRN_M_xx, V2C_PRN_M_v,
+    V2C_PRN_M_xy, V2C_PRN_M_p, V2C_PRN_M_exx, V2C_PRN_M_exy, V2C_PRN_M_d, V2C_PRN_M_w,
+    V2C_PRN_M_vx, V2C_PRN_M_vy, V2C_PRN_M_phi, V2C_PRN_M_L, V2C_PRN_M_L0, V2C_PRN_M_C,
+    V2C_PRN_M_C0, V2C_PRN_NMARK
+};
+typedef struct v2c_prn {
+    long xnumx, ynumy, mnumx, mnumy, marknum;
+    double xsize, ysize, pinit, pkf[4], timestep, timesum, GXKOEF, GYKOEF;
+    int rocknum;
+    long bondnum;            /* boundary rows incl. the unused row 0          */
+    int cycle;               /* save_model's circle number (n0)               */
+    int *markim;             /* rocknum                                       */
+    double *rock[V2C_PRN_NROCK];     /* rocknum each                          */
+    double *node[V2C_PRN_NNODE];     /* xnumx*ynumy each                      */
+    long *bondm;             /* 4*nodenum: P, vx, vy of node m3 at 3*m3+0..2, T at 3*nodenum+m3 */
+    double *gx, *gy;
+    double *bondv;           /* 4*bondnum, row 0 unused                       */
+    long *bondn;             /* 3*bondnum, row 0 unused                       */
+    double timesave[5];
+    double *mark[V2C_PRN_NMARK];     /* marknum each                          */
+    char *t;                 /* marknum                                       */
+} v2c_prn;
+int v2c_prn_read(const char *path, v2c_prn *prn);
+int v2c_prn_write(const char *path, const v2c_prn *prn);
+void v2c_prn_free(v2c_prn *prn);
+/* message of the last failed v2c_prn_read / v2c_prn_write on this thread */
+const char *v2c_prn_last_error(void);
+
 /* ---- build-defined per-marker local update (no reference counterpart) ----
  * The reference evaluates rate-and-state friction per node (plastic.c,
  * calc_RSF.c:33-221) and the ductile viscosity with a quasi-Newton that stops

@@ -13,8 +13,10 @@
  * global of the table after the last op.  Ops:
  *   m2n | n2m | move | before:<dt>:<rk> | after:<dt>:<rk> | ductile | allinteri
  *   stress:<dt>:<rk> | c2b:<dt> | c2bold:<dt> | b2c | copy:<rk> | tstep:<oldornew>
- *   bcvxvy | bcp | bctk
- * (tstep leaves the returned time step in drv_ret[2]).
+ *   bcvxvy | bcp | bctk | save:<cycle>:<path> | load:<path>
+ * (tstep leaves the returned time step in drv_ret[2]; save and load are the
+ * reference's .prn writer and reader, save also writes file.t3c and
+ * file_hdf5.t3c into the working directory).
  */
 #ifndef _GNU_SOURCE
 #define _GNU_SOURCE 1
@@ -30,6 +32,7 @@
 #include "headevps.c"
 #include "input_RSF.c"
 #include "calc_RSF.c"
+#include "load_save.c"
 #include "gausevps.c"
 #include "interpolation.c"
 #include "density_ductilevisc_calc.c"
@@ -89,8 +92,13 @@ static entry table[] = {
     /* post-solve scalars (headevps.c:120-122, 181, 339, 345-346) */
     S(maxvel), S(maxdphi), S(maxnvp), S(dtL), S(dtphi), S(dtd), S(dtnvp),
     S(maxtmstep), S(tcnvp), S(tcd), S(tcphih), S(dtfactor),
-    /* boundary equations (headevps.c:197-202), input only */
-    LA(bondm), LA(bondn), D(bondv),
+    /* boundary equations (headevps.c:197-202) */
+    LA(bondm), LA(bondn), D(bondv), L(bondnum),
+    /* the rest of the .prn header and rock table (headevps.c:37-58, 180, 205,
+     * 230-241) */
+    L(mnumx), L(mnumy), S(pinit), D(pkf), S(GXKOEF), S(GYKOEF), I(rocknum), D(timesave),
+    D(marks0), D(marks1), D(markll), D(marka0), D(marka1), D(markb0), D(markb1),
+    D(marke0), D(marke1), D(markf0), D(markf1), D(markv),
     /* pure-function op arrays */
     D(dv_sxy), D(dv_sxx), D(dv_tk), D(dv_pb), D(dv_out), {"dv_type", 'c', (void *)dv_type, DRV_MAX},
     D(ai_x), D(ai_y), D(ai_vx), D(ai_vy), D(ai_esp), D(ai_ee), L(drv_n), D(drv_ret),
@@ -124,14 +132,18 @@ static long used_count(const entry *e)
 {
     long nodenum_ = xnumx * ynumy;
     if (e->cap == 1) return 1;
-    if (!strncmp(e->name, "bond", 4)) return 0;
+    if (!strcmp(e->name, "bondm")) return 4 * nodenum_;
+    if (!strcmp(e->name, "bondn")) return 3 * bondnum;
+    if (!strcmp(e->name, "bondv")) return 4 * bondnum;
     if (!strcmp(e->name, "gx")) return xnumx;
     if (!strcmp(e->name, "gy")) return ynumy;
     if (!strncmp(e->name, "mark", 4) && e->cap == MAXMRK) return marknum + 1;
     if (e->cap == MAXMRK) return marknum + 1;     /* fric_static */
     if (!strncmp(e->name, "mark", 4) || !strcmp(e->name, "markim")) return MAXTMR;
     if (!strncmp(e->name, "dv_", 3) || !strncmp(e->name, "ai_", 3)) return drv_n;
-    if (!strcmp(e->name, "eps") || !strcmp(e->name, "drv_ret")) return e->cap;
+    if (!strcmp(e->name, "eps") || !strcmp(e->name, "drv_ret") || !strcmp(e->name, "pkf") ||
+        !strcmp(e->name, "timesave"))
+        return e->cap;
     return nodenum_ + 1;
 }
 
@@ -188,6 +200,14 @@ int main(int argc, char **argv)
         else if (!strcmp(op, "bcvxvy")) vxvy_boundarycorrection();
         else if (!strcmp(op, "bcp")) P_boundarycorrection();
         else if (!strcmp(op, "bctk")) tkrecalc();
+        else if (sscanf(op, "save:%d:", &rk) == 1 && strlen(strchr(op + 5, ':') + 1) < sizeof fl1out) {
+            strcpy(fl1out, strchr(op + 5, ':') + 1);
+            save_model(0, rk, 0);
+        } else if (!strncmp(op, "load:", 5) && strlen(op + 5) < sizeof fl1in) {
+            strcpy(fl1in, op + 5);
+            load_model();
+            nodenum = xnumx * ynumy; nodenum3 = nodenum * 3;
+        }
         else { fprintf(stderr, "ref_driver: unknown op %s\n", op); return 4; }
     }
     for (long i = 0; i < nodenum; i++) {

@@ -112,10 +112,39 @@ def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=N
         # its racy antidiffusion scatter (:942) make multithreaded runs vary
         env = dict(os.environ, OMP_NUM_THREADS="1")
         with open(os.devnull, "w") as dn:
-            subprocess.run([DRIVER, fin, fout] + list(ops), check=True, stdout=dn, env=env)
+            subprocess.run([DRIVER, fin, fout] + list(ops), check=True, stdout=dn, env=env, cwd=td)
         raw = _read(fout)
     nn = g.nodenum
     mk = {fld: raw[_MARK_NAME[fld]][:m.n].copy() for fld in MARKER_FIELDS}
     mk["t"] = raw["markt"][:m.n].copy()
     nd = {k: raw[k][:nn].copy() for k in NODAL_FIELDS}
     return mk, nd, raw
+
+
+# .prn marker columns -> reference globals (load_save.c:494-509, input_RSF.c:97-101)
+PRN_MARK_GLOBAL = {k: ("mark" + k) for k in "x y k e xx v xy p exx exy d w vx vy phi C C0".split()}
+PRN_MARK_GLOBAL.update(L="mark_L", L0="mark_L0")
+
+
+def prn_extra(d):
+    """Driver input records that set the reference's globals to the .prn model
+    `d` (velo2cycles_amd.prn.read's dict); pass as run(..., extra=...)."""
+    ex = {"mnumx": ("l", np.array([d["mnumx"]], np.int64)), "mnumy": ("l", np.array([d["mnumy"]], np.int64)),
+          "pinit": ("d", np.array([d["pinit"]])), "pkf": ("d", np.asarray(d["pkf"], np.float64)),
+          "GXKOEF": ("d", np.array([d["GXKOEF"]])), "GYKOEF": ("d", np.array([d["GYKOEF"]])),
+          "rocknum": ("i", np.array([d["rocknum"]], np.int32)),
+          "bondnum": ("l", np.array([d["bondnum"]], np.int64)),
+          "timesave": ("d", np.asarray(d["timesave"], np.float64)),
+          "markim": ("i", np.asarray(d["markim"], np.int32)),
+          "bondm": ("l", np.asarray(d["bondm"], np.int64)),
+          "bondn": ("l", np.asarray(d["bondn"], np.int64).ravel()),
+          "bondv": ("d", np.asarray(d["bondv"], np.float64).ravel()),
+          "markt": ("c", np.asarray(d["t"], np.int8))}
+    for k, v in d["rock"].items():
+        ex["mark" + k] = ("d", np.asarray(v, np.float64))
+    for k, v in d["node"].items():
+        if k != "RK_sliprs_final_a":
+            ex[k] = ("d", np.asarray(v, np.float64))
+    for k, v in d["mark"].items():
+        ex[PRN_MARK_GLOBAL[k]] = ("d", np.asarray(v, np.float64))
+    return ex

@@ -5,6 +5,7 @@ hot-path routines, compiled in place from /root/reference/src
     python tests/golden/make_golden.py            # step fixtures and pure functions
     python tests/golden/make_golden.py post       # post-solve nodal routines
     python tests/golden/make_golden.py bc         # boundary-condition tail
+    python tests/golden/make_golden.py prn        # a .prn written by save_model
 
 Each fixture holds its complete input (grid, flags, rock table, markers, nodal
 arrays) and the reference's output after the listed ops, so the tests never
@@ -124,8 +125,29 @@ def bc_cases():
         print("wrote", name, nn, "nodes", len(bn), "equations")
 
 
+def prn_cases():
+    """prn_small.prn: the reference's save_model output for a seeded model
+    (synth.prn_model of a 9 x 7 grid, 2 x 2 markers per cell, boundary
+    equations), plus prn_small.npz listing some of its values."""
+    import shutil
+    import tempfile
+    cfg = synth.config_grid(9, 7, 8e3, 6e3, 2, refine=True, seed=91)
+    d = synth.prn_model(cfg, 91)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "r.prn")
+        refrun.run(cfg, [f"save:{d['cycle']}:{path}"], extra=refrun.prn_extra(d))
+        shutil.copyfile(path, os.path.join(HERE, "prn_small.prn"))
+    np.savez_compressed(os.path.join(HERE, "prn_small.npz"), gx=d["gx"], gy=d["gy"], markx=d["mark"]["x"],
+                        markC0=d["mark"]["C0"], RK_phidts_final=d["node"]["RK_phidts_final"], markt=d["t"],
+                        cycle=np.array(d["cycle"]))
+    print("wrote prn_small", cfg.grid.nodenum, "nodes", d["marknum"], "markers")
+
+
 def main():
     assert refrun.available(), "build oracle/_ref first (make -C oracle)"
+    if sys.argv[1:] == ["prn"]:
+        prn_cases()
+        return
     if sys.argv[1:] == ["post"]:
         post_cases()
         return

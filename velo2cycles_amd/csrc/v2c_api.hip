@@ -1171,3 +1171,4 @@ int v2c_kernel_time(v2c_ctx *c, int kernel, double *ms, long *launches)
 #include "v2c_post.hip"
 #include "v2c_heat.hip"
 #include "v2c_bc.hip"
+#include "v2c_prn.cpp"

@@ -560,3 +560,36 @@ def boundary_equations(cfg, seed=DEFAULT_SEED, extra=True):
         # a temperature chain: corner -> side -> interior
         add(3 * nn + m(0, 1), 10.0, [(3 * nn + m(0, 2), 0.9)])
     return bondm, np.array(rows_n, dtype=np.int64), np.array(rows_v, dtype=np.float64)
+
+
+def prn_model(cfg, seed=DEFAULT_SEED, bonds=True) -> dict:
+    """A .prn model (velo2cycles_amd.prn.read's dict) holding cfg's grid,
+    markers, node columns and rock table, with seeded values for what cfg
+    does not carry (marker viscosity, rock columns outside the build's
+    table, the header scalars, timesave) and cfg's boundary equations."""
+    from .prn import PRN_MARK, PRN_NODE, PRN_ROCK
+    rng = np.random.default_rng(seed)
+    g, m, mats = cfg.grid, cfg.markers, cfg.mats
+    nn, nr = g.nodenum, int(mats.ntypes)
+    table = {f for f, _ in type(mats)._fields_} - {"ntypes", "im"}
+    rock = {k: (np.array(getattr(mats, k)[:nr], dtype=np.float64) if k in table
+                else rng.uniform(0.0, 1.0, nr)) for k in PRN_ROCK}
+    node = {}
+    for k in PRN_NODE:
+        src = cfg.nodal.get("RK_sliprs_final" if k == "RK_sliprs_final_a" else k)
+        node[k] = (np.array(src, dtype=np.float64) if src is not None else rng.normal(0.0, 1.0, nn))
+    mark = {}
+    for k in PRN_MARK:
+        mark[k] = (np.array(m[k], dtype=np.float64) if k in m.arrays
+                   else rng.uniform(1e18, 1e24, m.n))
+    if bonds:
+        bondm, bondn, bondv = boundary_equations(cfg, seed)
+    else:
+        bondm, bondn, bondv = (np.zeros(4 * nn, np.int64), np.zeros((1, 3), np.int64), np.zeros((1, 4)))
+    return {"xnumx": g.xnumx, "ynumy": g.ynumy, "mnumx": 4, "mnumy": 4, "marknum": m.n,
+            "xsize": g.xsize, "ysize": g.ysize, "pinit": 1e5, "pkf": rng.normal(0.0, 1.0, 4),
+            "timestep": cfg.timestep, "timesum": cfg.timesum, "GXKOEF": 0.0, "GYKOEF": 9.81,
+            "rocknum": nr, "bondnum": len(bondn), "cycle": int(rng.integers(0, 1000)),
+            "markim": np.array(mats.im[:nr], dtype=np.int32), "rock": rock, "node": node, "mark": mark,
+            "bondm": bondm, "gx": g.gx.copy(), "gy": g.gy.copy(), "bondv": bondv, "bondn": bondn,
+            "timesave": rng.uniform(0.0, 1e12, 5), "t": m.t.copy()}
