@@ -17,6 +17,14 @@
  * (tstep leaves the returned time step in drv_ret[2]; save and load are the
  * reference's .prn writer and reader, save also writes file.t3c and
  * file_hdf5.t3c into the working directory).
+ *
+ * Built with -DV2C_DROPIN (oracle/_ref/dropin_driver) the reference's
+ * hot-path routines are compiled under ref_* names and the reference-side
+ * binding velo2cycles_amd/csrc/dropin/v2c_dropin.c supplies the live ones,
+ * so every op goes through libv2c on the GPU exactly as the reference's own
+ * call sites would once the binding is linked in.  The device context is
+ * attached after IN is read and the markers come back before OUT is written.
+ * Extra op there: heat:<heatdif> (v2c_dropin_temperature_markers).
  */
 #ifndef _GNU_SOURCE
 #define _GNU_SOURCE 1
@@ -27,6 +35,23 @@
 #include <string.h>
 #include <omp.h>
 #include <stdbool.h>
+
+#ifdef V2C_DROPIN
+#define marker2nodes ref_marker2nodes
+#define nodes2markers ref_nodes2markers
+#define moveandrotatemarkers ref_moveandrotatemarkers
+#define compute_before_solve ref_compute_before_solve
+#define compute_after_solve ref_compute_after_solve
+#define recompute_stress_strainrate ref_recompute_stress_strainrate
+#define interpolate_center2basic ref_interpolate_center2basic
+#define interpolate_center2basic_old ref_interpolate_center2basic_old
+#define interpolate_basic2center ref_interpolate_basic2center
+#define copyarrays ref_copyarrays
+#define calc_timestep ref_calc_timestep
+#define vxvy_boundarycorrection ref_vxvy_boundarycorrection
+#define P_boundarycorrection ref_P_boundarycorrection
+#define tkrecalc ref_tkrecalc
+#endif
 
 /* The reference translation unit (i2evps.c:12-30 order, hot-path files only). */
 #include "headevps.c"
@@ -42,6 +67,24 @@
 #include "momentum_continuity_iterate.c"
 #include "plastic.c"
 #include "recalc_marker_properties.c"
+
+#ifdef V2C_DROPIN
+#undef marker2nodes
+#undef nodes2markers
+#undef moveandrotatemarkers
+#undef compute_before_solve
+#undef compute_after_solve
+#undef recompute_stress_strainrate
+#undef interpolate_center2basic
+#undef interpolate_center2basic_old
+#undef interpolate_basic2center
+#undef copyarrays
+#undef calc_timestep
+#undef vxvy_boundarycorrection
+#undef P_boundarycorrection
+#undef tkrecalc
+#include "../velo2cycles_amd/csrc/dropin/v2c_dropin.c"
+#endif
 
 /* driver-local arrays for the pure-function ops */
 #define DRV_MAX 2000000
@@ -177,6 +220,12 @@ int main(int argc, char **argv)
         RK_sliprs[i][0] = RK_sliprs0[i]; RK_sliprs[i][1] = RK_sliprs1[i];
         RK_phidts[i][0] = RK_phidts0[i]; RK_phidts[i][1] = RK_phidts1[i];
     }
+#ifdef V2C_DROPIN
+    {
+        const int rc = v2c_dropin_attach(marknum > 0 ? marknum : 1, 0);
+        if (rc) { fprintf(stderr, "ref_driver: v2c_dropin_attach failed (%d)\n", rc); return 6; }
+    }
+#endif
     for (int a = 3; a < argc; a++) {
         const char *op = argv[a];
         double dt; int rk;
@@ -208,8 +257,16 @@ int main(int argc, char **argv)
             load_model();
             nodenum = xnumx * ynumy; nodenum3 = nodenum * 3;
         }
+#ifdef V2C_DROPIN
+        else if (sscanf(op, "heat:%lf", &dt) == 1) v2c_dropin_temperature_markers(dt);
+#endif
         else { fprintf(stderr, "ref_driver: unknown op %s\n", op); return 4; }
     }
+#ifdef V2C_DROPIN
+    v2c_dropin_markers_to_host();
+    v2c_dropin_detach();
+    if (errcheck) fprintf(stderr, "ref_driver: errcheck set by the binding\n");
+#endif
     for (long i = 0; i < nodenum; i++) {
         RK_sliprs0[i] = RK_sliprs[i][0]; RK_sliprs1[i] = RK_sliprs[i][1];
         RK_phidts0[i] = RK_phidts[i][0]; RK_phidts1[i] = RK_phidts[i][1];

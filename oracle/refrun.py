@@ -1,9 +1,12 @@
 """Run the reference's own hot-path routines (oracle/_ref/ref_driver, compiled
 in place from /root/reference/src by oracle/Makefile) on a Config.
 
-TEST INFRASTRUCTURE ONLY: used in this container to write golden vectors and
-to pin the oracle.  The reference is absent on the GPU box; callers check
-`available()` first.
+TEST INFRASTRUCTURE ONLY: writes golden vectors and pins the oracle where the
+reference sources are mounted (callers check `available()`).  Both drivers
+are built binaries under oracle/_ref and need no sources at run time:
+`built(DROPIN)` gates the drop-in check, which runs the same ops through the
+reference-side binding and libv2c on a GPU (ref_driver.c built with
+-DV2C_DROPIN) beside the plain reference (`built(DRIVER)`).
 """
 from __future__ import annotations
 
@@ -18,6 +21,7 @@ from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(ORACLE_DIR, "_ref", "ref_driver")
+DROPIN = os.path.join(ORACLE_DIR, "_ref", "dropin_driver")
 REF_SRC = "/root/reference/src"
 
 # reference capacity (headevps.c:10-20)
@@ -38,6 +42,10 @@ def available() -> bool:
     if os.path.exists(REF_SRC) and not os.path.exists(DRIVER):
         subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=False)
     return os.path.exists(DRIVER) and os.path.exists(REF_SRC)
+
+
+def built(driver=DRIVER) -> bool:
+    return os.path.exists(driver)
 
 
 def legal(cfg) -> bool:
@@ -69,11 +77,11 @@ def _read(path):
     return out
 
 
-def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=None):
+def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=None, driver=DRIVER):
     """Write cfg (+ overrides) into the reference globals, run `ops`, return
     (markers dict, nodal dict, raw dict).  Markers are returned by the build's
     field names, nodal arrays by reference names (length nodenum)."""
-    assert available(), "reference driver unavailable"
+    assert built(driver) or (driver == DRIVER and available()), "reference driver unavailable"
     assert legal(cfg), "size exceeds the reference capacity macros"
     m = markers or cfg.markers
     nod = nodal or cfg.nodal
@@ -112,7 +120,7 @@ def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=N
         # its racy antidiffusion scatter (:942) make multithreaded runs vary
         env = dict(os.environ, OMP_NUM_THREADS="1")
         with open(os.devnull, "w") as dn:
-            subprocess.run([DRIVER, fin, fout] + list(ops), check=True, stdout=dn, env=env, cwd=td)
+            subprocess.run([driver, fin, fout] + list(ops), check=True, stdout=dn, env=env, cwd=td)
         raw = _read(fout)
     nn = g.nodenum
     mk = {fld: raw[_MARK_NAME[fld]][:m.n].copy() for fld in MARKER_FIELDS}

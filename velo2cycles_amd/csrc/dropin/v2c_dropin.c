@@ -3,11 +3,19 @@
  * libv2c.  A maintainer compiles this file into the reference translation unit
  * in place of the hot-path routines, so every call site stays as it is:
  *
- *   marker2nodes()            i2evps.c:89, in2evps.c:1345
- *   nodes2markers()           i2evps.c:108
- *   moveandrotatemarkers()    i2evps.c:113
- *   compute_before_solve()    momentum_continuity_iterate.c:69
- *   compute_after_solve()     momentum_continuity_iterate.c:89
+ *   marker2nodes()                  i2evps.c:89, in2evps.c:1345
+ *   nodes2markers()                 i2evps.c:108
+ *   moveandrotatemarkers()          i2evps.c:113
+ *   compute_before_solve()          momentum_continuity_iterate.c:69
+ *   compute_after_solve()           momentum_continuity_iterate.c:89
+ *   recompute_stress_strainrate()   momentum_continuity_iterate.c:85
+ *   interpolate_center2basic()      momentum_continuity_iterate.c:87
+ *   interpolate_center2basic_old()  i2evps.c:90
+ *   interpolate_basic2center()      momentum_continuity_iterate.c:71
+ *   copyarrays()                    momentum_continuity_iterate.c:128
+ *   calc_timestep()                 momentum_continuity_iterate.c:44, 92
+ *   vxvy_boundarycorrection(), P_boundarycorrection(), tkrecalc()
+ *                                   interpolation.c:919-924, temperature_iterate.c
  *
  * The reference globals (headevps.c:33-250, input_RSF.c:3-11) are declared
  * extern here with the reference's own types; the file owns no state beyond
@@ -17,7 +25,12 @@
  * which read markers on the host).  Nodal arrays cross PCIe on every call:
  * inputs are sent, outputs fetched, exactly the arrays each routine reads and
  * writes.  The boundary-condition tail of marker2nodes
- * (interpolation.c:918-925) keeps running on the host.
+ * (interpolation.c:918-925) runs on the device inside marker2nodes().
+ *
+ * solve_temperature (temperature_iterate.c:2-419) keeps its heat solve on
+ * the host; its marker part (:181-419) is replaced by a call to
+ * v2c_dropin_temperature_markers(heatdif), which updates the device-resident
+ * markers, so the markers need not come to the host for the heat step.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -51,9 +64,10 @@ extern double wtphisum[], phisum[], sxysum[], exysum[], esum[], wtsxxsum[], psum
 extern double exxsum[], wtvxsum[], vxsum[], wtvysum[], vysum[], mrx0[], mry0[];
 extern double pes[], phis_it[], syields[], nuvp_it[], sliprs[], RK_sliprs_final[], RK_phidts_final[];
 extern double RK_sliprs_solved[], RK_phidts_solved[], RK_deltas[];
-void vxvy_boundarycorrection(void);
-void P_boundarycorrection(void);
-void tkrecalc(void);
+extern double deltasxx[], deltasxy[], deltaVs[], exxs[], exxes[], sxxes[], prs[], tk2[], tol0[], tol1[];
+extern long int bondm[], bondn[][3], bondnum;
+extern double bondv[][4];
+extern double maxvel, maxdphi, maxnvp, dtL, dtphi, dtd, dtnvp, maxtmstep, tcnvp, tcd, tcphih, dtfactor;
 
 static v2c_ctx *g_ctx;
 
@@ -68,7 +82,10 @@ static double *nodal_ptr(int f)
         wtphisum, phisum, sxysum, exysum, esum, wtsxxsum, psum, sxxsum, exxsum, wtvxsum, vxsum,
         wtvysum, vysum, mrx0, mry0,
         pes, phis_it, syields, nuvp_it, sliprs, RK_sliprs_final, RK_phidts_final, RK_sliprs_solved,
-        RK_phidts_solved, RK_deltas, NULL, NULL, NULL, NULL};
+        RK_phidts_solved, RK_deltas, NULL, NULL, NULL, NULL,
+        deltasxx, deltasxy, deltaVs, exxs, exxes, sxxes, prs, tk2, tol0, tol1};
+    typedef char one_pointer_per_field[sizeof tab / sizeof tab[0] == V2C_NF ? 1 : -1];
+    (void)sizeof(one_pointer_per_field);
     return f >= 0 && f < V2C_NF ? tab[f] : NULL;
 }
 
@@ -110,6 +127,8 @@ static void fetch(const int *f, int n)
     for (int i = 0; i < n; i++) die_on(v2c_nodal_get(g_ctx, f[i], nodal_ptr(f[i])), "nodal_get");
 }
 
+#define NOF(a) ((int)(sizeof(a) / sizeof((a)[0])))
+
 static v2c_markers marker_view(void)
 {
     v2c_markers m = {markx, marky, markk, markp, markxx, markxy, markexx, markexy, marke, markphi,
@@ -148,6 +167,11 @@ int v2c_dropin_attach(long capacity, int device)
     }
     int rc = v2c_create(&g_ctx, &g, &p, &mat, capacity, device);
     if (rc) return rc;
+    /* boundary equations as load_model left them (load_save.c:279-304) */
+    if (bondnum > 1) {
+        rc = v2c_set_boundary(g_ctx, bondm, &bondn[0][0], &bondv[0][0], bondnum);
+        if (rc) return rc;
+    }
     v2c_markers m = marker_view();
     return v2c_markers_upload(g_ctx, &m, marknum);
 }
@@ -190,11 +214,13 @@ void marker2nodes(void)
     v2c_set_time(g_ctx, timesum, timestep);
     send(in, (int)(sizeof in / sizeof in[0]));
     die_on(v2c_marker2nodes(g_ctx), "marker2nodes");
+    /* the routine's boundary-condition tail (interpolation.c:918-925) */
+    if (bondnum > 1) {
+        die_on(v2c_vxvy_boundarycorrection(g_ctx), "vxvy_boundarycorrection");
+        die_on(v2c_P_boundarycorrection(g_ctx), "P_boundarycorrection");
+        if (tempmod) die_on(v2c_tkrecalc(g_ctx), "tkrecalc");
+    }
     fetch(out, (int)(sizeof out / sizeof out[0]));
-    /* host tail of the reference routine (interpolation.c:918-925) */
-    vxvy_boundarycorrection();
-    P_boundarycorrection();
-    if (tempmod) tkrecalc();
 }
 
 double nodes2markers(void)
@@ -256,4 +282,139 @@ double compute_after_solve(double dt, int rk)
     fetch(out, (int)(sizeof out / sizeof out[0]));
     if (nerr) errcheck = 1;
     return deltas;
+}
+
+/* ---- post-solve nodal routines ------------------------------------------ */
+
+int recompute_stress_strainrate(double dt, int rk)
+{
+    static const int in[] = {V2C_N_vx, V2C_N_vy, V2C_N_nd, V2C_N_gd, V2C_N_sxxe, V2C_N_nu, V2C_N_gg,
+                             V2C_N_sxye, V2C_N_phis, V2C_N_phis_it,
+                             /* written on interior nodes only */
+                             V2C_N_sxx, V2C_N_exx, V2C_N_deltasxx, V2C_N_sxy, V2C_N_exy, V2C_N_esp,
+                             V2C_N_deltasxy};
+    static const int out[] = {V2C_N_sxx, V2C_N_exx, V2C_N_deltasxx, V2C_N_sxy, V2C_N_exy, V2C_N_esp,
+                              V2C_N_deltasxy};
+    v2c_stress_stats st;
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, NOF(in));
+    die_on(v2c_recompute_stress_strainrate(g_ctx, dt, rk, &st), "recompute_stress_strainrate");
+    fetch(out, NOF(out));
+    maxdphi = st.maxdphi;
+    maxnvp = st.maxnvp;
+    printf("VISKOS: min = %e max = %e \n", st.minnu, st.maxnu);
+    printf("STRESSS CHANGE:    max = %e \n", st.maxds);
+    printf("MAX DPHI:    max = %e \n", st.maxdphi);
+    return 0;
+}
+
+double interpolate_center2basic(double dt)
+{
+    static const int in[] = {V2C_N_exx, V2C_N_sxx, V2C_N_sxxe, V2C_N_gd, V2C_N_nd_m2c, V2C_N_pr,
+                             V2C_N_exxs, V2C_N_exxpl, V2C_N_sxxs, V2C_N_prs, V2C_N_exxel};
+    static const int out[] = {V2C_N_exxs, V2C_N_exxpl, V2C_N_sxxs, V2C_N_prs, V2C_N_exxel};
+    send(in, NOF(in));
+    die_on(v2c_interpolate_center2basic(g_ctx, dt), "interpolate_center2basic");
+    fetch(out, NOF(out));
+    return 0;
+}
+
+double interpolate_center2basic_old(double dt)
+{
+    static const int in[] = {V2C_N_exxe, V2C_N_sxxe, V2C_N_sppe, V2C_N_exxes, V2C_N_sxxes, V2C_N_sppes};
+    static const int out[] = {V2C_N_exxes, V2C_N_sxxes, V2C_N_sppes};
+    (void)dt;   /* unused by the reference routine too */
+    send(in, NOF(in));
+    die_on(v2c_interpolate_center2basic_old(g_ctx), "interpolate_center2basic_old");
+    fetch(out, NOF(out));
+    return 0;
+}
+
+double interpolate_basic2center(void)
+{
+    static const int in[] = {V2C_N_nu, V2C_N_nd};
+    static const int out[] = {V2C_N_nd};
+    send(in, NOF(in));
+    die_on(v2c_interpolate_basic2center(g_ctx), "interpolate_basic2center");
+    fetch(out, NOF(out));
+    return 0;
+}
+
+void copyarrays(int rk)
+{
+    static const int in[] = {V2C_N_RK_sliprs_solved, V2C_N_RK_phidts_solved, V2C_N_RK_sliprs_final,
+                             V2C_N_RK_phidts_final, V2C_N_deltaVs, V2C_N_sliprs};
+    static const int out[] = {V2C_N_deltaVs, V2C_N_sliprs, V2C_N_RK_sliprs_final, V2C_N_RK_phidts_final};
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, NOF(in));
+    die_on(v2c_copyarrays(g_ctx, rk), "copyarrays");
+    fetch(out, NOF(out));
+}
+
+/* The sticky-air diagnostics of the reference routine (calc_RSF.c:506-551)
+ * only print; they are not computed here. */
+double calc_timestep(int oldornew)
+{
+    static const int in[] = {V2C_N_sppes, V2C_N_prs, V2C_N_bm, V2C_N_gg, V2C_N_Ls, V2C_N_as, V2C_N_bs,
+                             V2C_N_V0s, V2C_N_RK_sliprs_final, V2C_N_RK_sliprs_solved, V2C_N_nu_m2c,
+                             V2C_N_sxxes, V2C_N_sxye, V2C_N_nu, V2C_N_vx, V2C_N_mvx, V2C_N_vy, V2C_N_mvy,
+                             V2C_N_phis, V2C_N_phis_it};
+    v2c_timestep_params p;
+    v2c_timestep_result r;
+    v2c_timestep_params_default(&p);
+    p.maxtmstep = maxtmstep; p.tcnvp = tcnvp; p.tcd = tcd; p.tcphih = tcphih; p.dtfactor = dtfactor;
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, NOF(in));
+    const int rc = v2c_calc_timestep(g_ctx, oldornew, &p, &r);
+    if (rc) {
+        /* the reference exits on chi == 0 (calc_RSF.c:478) */
+        die_on(rc, "calc_timestep");
+        exit(0);
+    }
+    maxvel = r.maxvel; dtnvp = r.dtnvp; dtd = r.dtd; dtphi = r.dtphi; dtL = r.dtL;
+    printf("===================================\n");
+    printf(">>> dt_f =%e s, Max Slip-Vel=%e m/s, Max Vx=%e m/s", r.dt, r.maxvel, r.maxVx);
+    printf("\n>>> dtnvp=%e s, dtd =%e s, dtphi =%e s, dtL=%e s \n", r.dtnvp, r.dtd, r.dtphi, r.dtL);
+    printf("===================================\n");
+    return r.dt;
+}
+
+/* ---- boundary-condition tail --------------------------------------------- */
+
+void vxvy_boundarycorrection(void)
+{
+    static const int io[] = {V2C_N_mvx, V2C_N_mvy};
+    send(io, NOF(io));
+    die_on(v2c_vxvy_boundarycorrection(g_ctx), "vxvy_boundarycorrection");
+    fetch(io, NOF(io));
+}
+
+void P_boundarycorrection(void)
+{
+    static const int io[] = {V2C_N_sppe};
+    send(io, NOF(io));
+    die_on(v2c_P_boundarycorrection(g_ctx), "P_boundarycorrection");
+    fetch(io, NOF(io));
+}
+
+void tkrecalc(void)
+{
+    static const int io[] = {V2C_N_tk};
+    send(io, NOF(io));
+    die_on(v2c_tkrecalc(g_ctx), "tkrecalc");
+    fetch(io, NOF(io));
+}
+
+/* ---- the marker part of solve_temperature -------------------------------- */
+
+/* temperature_iterate.c:181-419 on the device-resident markers; call it where
+ * that block stood, after the thermal cycle restored timestep (:178). */
+void v2c_dropin_temperature_markers(double heatdif)
+{
+    static const int in[] = {V2C_N_tk, V2C_N_tk2, V2C_N_kt, V2C_N_cp, V2C_N_ro};
+    static const int out[] = {V2C_N_tol0, V2C_N_tol1};
+    v2c_set_time(g_ctx, timesum, timestep);
+    send(in, NOF(in));
+    die_on(v2c_temperature_markers(g_ctx, heatdif), "temperature_markers");
+    fetch(out, NOF(out));
 }
