@@ -111,6 +111,104 @@ def pcie_inclusive(ctx, cfg, ms_step):
             "field_mb": cfg.grid.nodenum * 8 / 1e6}
 
 
+# Algorithmic bytes of the routines after the solves, per node (and per
+# marker, per boundary row): the unique arrays each routine must read and write.
+#   recompute_stress_strainrate  reads vx vy nd gd sxxe nu gg sxye phis phis_it,
+#                                writes sxx exx deltasxx sxy exy esp deltasxy  17 x 8
+#   interpolate_center2basic     reads exx sxx sxxe gd nd_m2c pr, writes exxs
+#                                exxpl sxxs prs exxel                          11 x 8
+#   interpolate_center2basic_old reads exxe sxxe sppe, writes exxes sxxes sppes 6 x 8
+#   interpolate_basic2center     reads nu, writes nd                            2 x 8
+#   copyarrays                   reads RK_sliprs_solved RK_sliprs_final
+#                                RK_phidts_solved, writes deltaVs sliprs
+#                                RK_sliprs_final RK_phidts_final                7 x 8
+#   calc_timestep(0)             reads bm gg Ls as bs V0s RK_sliprs_final
+#                                RK_sliprs_solved nu_m2c sxxes sxye nu mvx mvy
+#                                phis                                          15 x 8
+#   temperature markers          per marker x y k type + k written (33 B); per
+#                                node tk tk2 kt cp ro read, tol0 tol1 written
+#   boundary tail                per node the field(s) read and written and the
+#                                bondm entries; per row bondn + bondv (56 B)
+NEXT_ALG = {
+    "recompute_stress_strainrate": (0, 17 * 8),
+    "interpolate_center2basic": (0, 11 * 8),
+    "interpolate_center2basic_old": (0, 6 * 8),
+    "interpolate_basic2center": (0, 2 * 8),
+    "copyarrays": (0, 7 * 8),
+    "calc_timestep": (0, 15 * 8),
+    "temperature_markers": (33, 7 * 8),
+    "vxvy_boundarycorrection": (0, 2 * 8 * 3),
+    "P_boundarycorrection": (0, 8 * 3),
+    "tkrecalc": (0, 8 * 3),
+}
+
+
+def next_rows(ctx, cfg, reps: int = 20):
+    """The routines after the solves on the same workload: per call, the
+    HIP-event kernel time (all kernels of the call), the wall time including
+    its host synchronisations, and the algorithmic bytes over the kernel time.
+    Also the .prn writer and reader on the grid plus the first 1 M markers."""
+    import tempfile
+    from velo2cycles_amd import prn
+    from velo2cycles_amd.abi import TimestepParams
+    n, nn = cfg.n, cfg.grid.nodenum
+    rng = np.random.default_rng(5)
+    nod = synth.post_solve_nodal(cfg, seed=5, degenerate=False)
+    nod["tk2"] = nod["tk"] + rng.normal(0.0, 5.0, nn)
+    bc = synth.boundary_equations(cfg, 5, extra=False)
+    rows = len(bc[1])
+    ctx.set_all_nodal(nod)
+    ctx.set_boundary(*bc)
+    dt = cfg.timestep
+    calls = {
+        "recompute_stress_strainrate": lambda: ctx.recompute_stress_strainrate(dt, 0),
+        "interpolate_center2basic": lambda: ctx.interpolate_center2basic(dt),
+        "interpolate_center2basic_old": ctx.interpolate_center2basic_old,
+        "interpolate_basic2center": ctx.interpolate_basic2center,
+        "copyarrays": ctx.copyarrays,
+        "calc_timestep": lambda: ctx.calc_timestep(0, TimestepParams.default(maxtmstep=dt)),
+        "temperature_markers": lambda: ctx.temperature_markers(1.0),
+        "vxvy_boundarycorrection": ctx.vxvy_boundarycorrection,
+        "P_boundarycorrection": ctx.P_boundarycorrection,
+        "tkrecalc": ctx.tkrecalc,
+    }
+    out = {}
+    for name, fn in calls.items():
+        fn()                                   # warm
+        ctx.synchronize()
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / reps * 1e3
+        ctx.enable_timing(False)
+        ms, _ = ctx.kernel_time("post")
+        ms /= reps
+        per_m, per_n = NEXT_ALG[name]
+        alg = n * per_m + nn * per_n + (rows * 56 if name.endswith(("correction", "recalc")) else 0)
+        out[name] = {"kernel_ms": ms, "wall_ms": wall, "alg_bytes": alg,
+                     "achieved_gbs": alg / (ms * 1e-3) / 1e9 if ms > 0 else None}
+    d = synth.prn_model(cfg, 5, bonds=False)
+    k = min(n, 1_000_000)
+    d["marknum"] = k
+    d["mark"] = {c: v[:k] for c, v in d["mark"].items()}
+    d["t"] = d["t"][:k]
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "m.prn")
+        t0 = time.perf_counter()
+        prn.write(path, d)
+        t_w = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        prn.read(path)
+        t_r = time.perf_counter() - t0
+    out["prn"] = {"markers": k, "file_mb": size / 1e6, "write_mb_s": size / t_w / 1e6,
+                  "read_mb_s": size / t_r / 1e6}
+    return out
+
+
 def valu_issue(workload, kern):
     """Per-kernel FP64-VALU issue bound from the committed counter profile
     (profiles/valu.json: wavefront VALU and f64-transcendental instructions
@@ -237,6 +335,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the post-solve / boundary / heat / .prn measurements")
     ap.add_argument("--ensemble", action="store_true", help="batched ensemble workload instead")
     ap.add_argument("--models", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="ensemble: skip the autotuner")
@@ -326,6 +426,9 @@ def main():
     }
     if valu:
         out["valu_issue"] = valu
+    if not args.no_next:
+        ctx.set_async(False)
+        out["next_rows"] = next_rows(ctx, cfg)
     ctx.close()
     if not args.no_warm:
         out["warm_start"] = warm_start_eval(lib, synth.YEAR)

@@ -50,8 +50,8 @@ def test_post_routines_match_reference_golden(gpu, path):
     ctx.close()
 
 
-def _large(pmode=0, rsf=1, seed=21, long_nan=False):
-    cfg = synth.config_grid(401, 101, 400e3, 100e3, 1, refine=True, seed=seed)
+def _large(pmode=0, rsf=1, seed=21, long_nan=False, nx=401, ny=101):
+    cfg = synth.config_grid(nx, ny, (nx - 1) * 1e3, (ny - 1) * 1e3, 1, refine=True, seed=seed)
     cfg.params.pmode = pmode
     cfg.params.RSFmode = rsf
     cfg.nodal = synth.post_solve_nodal(cfg, seed=seed + 1)
@@ -62,12 +62,21 @@ def _large(pmode=0, rsf=1, seed=21, long_nan=False):
         cfg.nodal["bs"][[255, 256, 511, 20000]] = 0.0
         cfg.nodal["mvx"][[767, 768, 30001]] = np.nan
         cfg.nodal["vx"][[1023, 1024]] = np.nan
+        if cfg.grid.nodenum > 300_000:
+            # more than 256 blocks of 256 nodes: the final compositions and
+            # the lambda carry run over several blocks per thread; a = 0 over
+            # 16 blocks so lambda is carried across those runs
+            cfg.nodal["as"][100_000:104_100] = 0.0
+            cfg.nodal["bs"][[1791, 1792, 262_143]] = 0.0
+            cfg.nodal["mvx"][[262_144, 300_001]] = np.nan
     return cfg
 
 
-@pytest.mark.parametrize("pmode,rsf,long_nan", [(0, 1, False), (1, 1, True), (0, 0, False)])
-def test_post_routines_match_oracle_large(gpu, pmode, rsf, long_nan):
-    cfg = _large(pmode, rsf, long_nan=long_nan)
+@pytest.mark.parametrize("pmode,rsf,long_nan,nx,ny", [(0, 1, False, 401, 101), (1, 1, True, 401, 101),
+                                                      (0, 0, False, 401, 101), (1, 1, True, 1501, 301),
+                                                      (0, 1, False, 1501, 301)])
+def test_post_routines_match_oracle_large(gpu, pmode, rsf, long_nan, nx, ny):
+    cfg = _large(pmode, rsf, long_nan=long_nan, nx=nx, ny=ny)
     print("case large", pmode, rsf, long_nan, cfg.grid.nodenum, flush=True)
     _, gn, gret, ctx = run_gpu(gpu, cfg, POST_OPS)
     _, on, oret = run_oracle(cfg, POST_OPS)
@@ -133,6 +142,10 @@ def _heat_case(seed=51, per=4):
     m.t[k[3000:3500]] = 110                # immobile kinds: untouched
     m["x"][k[3500:3600]] = -1.0            # outside
     m["y"][k[3600:3700]] = cfg.grid.ysize + 1.0
+    # 300 markers in one cell: its segment spans several 64-marker batches
+    gx, gy = cfg.grid.gx, cfg.grid.gy
+    m["x"][k[3700:4000]] = rng.uniform(gx[200], gx[201], 300)
+    m["y"][k[3700:4000]] = rng.uniform(gy[50], gy[51], 300)
     return cfg
 
 

@@ -5,11 +5,13 @@
 //     temperature; rock markers get the reset of newly incoming markers, the
 //     subgrid diffusion toward the old nodal temperature and the solved
 //     increment, and leave their node weight w and w * dT in a scratch pair;
-//   pass 2, per cell (sorted order): the weights of the cell's markers summed
-//     left to right into its upper-left node (tol0, tol1), then tol1 / tol0;
+//   pass 2, per group of 64 cells (one wavefront, sorted order): the weights
+//     of each cell's markers summed into its upper-left node (tol0, tol1),
+//     then tol1 / tol0; the last column and row of nodes get zeros;
 //   pass 3, per marker: the antidiffusion correction from that node.
-// No atomics: each node's sums are one thread's fold over its cell's markers
-// in (sub-cell, marker id) order, so runs are bitwise reproducible.
+// No atomics: each node's sums are a fixed sequence of lane scans over its
+// cell's markers in (sub-cell, marker id) order, so runs are bitwise
+// reproducible.
 // Included at the end of v2c_api.hip: it uses the context internals.
 
 namespace v2c {
@@ -89,25 +91,68 @@ __global__ __launch_bounds__(HEAT_BLOCK) void k_heat_markers(KArgs a, double hea
 }
 
 // Pass 2 (temperature_iterate.c:183-184, 325-364): node (m1, m2) collects the
-// markers of cell (m1, m2) in sorted order.
-__global__ __launch_bounds__(HEAT_BLOCK) void k_heat_nodes(KArgs a)
+// markers of cell (m1, m2).  One wavefront takes HEAT_CELLS consecutive cells
+// and walks their sorted markers 64 at a time with coalesced loads: a
+// segmented lane scan by cell sums each batch, the last lane of a cell's
+// segment adds the batch sum to the cell's accumulator in LDS.
+constexpr int HEAT_CELLS = 64;
+static_assert(HEAT_CELLS == 64, "one accumulator per lane of a wavefront");
+
+__global__ __launch_bounds__(64) void k_heat_cells(KArgs a)
 {
-    const long m4 = (long)blockIdx.x * HEAT_BLOCK + threadIdx.x;
-    if (m4 >= a.nodenum) return;
-    const long ny = a.g.ynumy, m2 = m4 % ny, m1 = m4 / ny;
-    double t0 = 0.0, t1 = 0.0;
-    if (m1 < a.g.xnumx - 1 && m2 < ny - 1) {
-        const long cell = m1 * (ny - 1) + m2;
-        const uint32_t b = a.sub_start[4 * cell], e = a.sub_start[4 * cell + 4];
-        for (uint32_t i = b; i < e; i++) {
-            const uint32_t m = a.order[i];
-            t0 += a.prop[m];
-            t1 += a.prop[a.cap + m];
+    __shared__ double acc0[HEAT_CELLS], acc1[HEAT_CELLS];
+    const int lane = threadIdx.x;
+    const long c0 = (long)blockIdx.x * HEAT_CELLS;
+    const long c1 = c0 + HEAT_CELLS < a.ncell ? c0 + HEAT_CELLS : a.ncell;
+    acc0[lane] = 0.0;
+    acc1[lane] = 0.0;
+    __syncthreads();
+    const uint32_t p0 = a.sub_start[4 * c0], p1 = a.sub_start[4 * c1];
+    for (uint32_t base = p0; base < p1; base += 64) {
+        const uint32_t p = base + lane;
+        const bool ok = p < p1;
+        int cell = -1;
+        double w = 0.0, wd = 0.0;
+        if (ok) {
+            const uint32_t m = a.order[p];
+            cell = (int)(a.skey[p] >> 2);
+            w = a.prop[m];
+            wd = a.prop[a.cap + m];
         }
+        for (int off = 1; off < 64; off <<= 1) {
+            const double w2 = __shfl_up(w, off), wd2 = __shfl_up(wd, off);
+            const int c2 = __shfl_up(cell, off);
+            if (lane >= off && c2 == cell) {
+                w += w2;
+                wd += wd2;
+            }
+        }
+        const int next = __shfl_down(cell, 1);
+        if (ok && (lane == 63 || next != cell)) {
+            acc0[cell - c0] += w;
+            acc1[cell - c0] += wd;
+        }
+        __syncthreads();
     }
+    const long cell = c0 + lane;
+    if (cell >= c1) return;
+    const long ny = a.g.ynumy, m1 = cell / (ny - 1), m2 = cell % (ny - 1), m4 = m1 * ny + m2;
+    const double t0 = acc0[lane];
+    double t1 = acc1[lane];
     if (t0) t1 /= t0;
     NOD(a, V2C_N_tol0)[m4] = t0;
     NOD(a, V2C_N_tol1)[m4] = t1;
+}
+
+// Pass 2, nodes that are no cell's upper-left node (last column, last row).
+__global__ __launch_bounds__(HEAT_BLOCK) void k_heat_border(KArgs a)
+{
+    const long i = (long)blockIdx.x * HEAT_BLOCK + threadIdx.x;
+    const long nx = a.g.xnumx, ny = a.g.ynumy;
+    if (i >= nx + ny - 1) return;
+    const long m4 = i < ny ? (nx - 1) * ny + i : (i - ny) * ny + (ny - 1);
+    NOD(a, V2C_N_tol0)[m4] = 0.0;
+    NOD(a, V2C_N_tol1)[m4] = 0.0;
 }
 
 // Pass 3 (temperature_iterate.c:367-416).
@@ -137,7 +182,9 @@ extern "C" int v2c_temperature_markers(v2c_ctx *c, double heatdif)
     rc = launch(c, V2C_K_POST, [&] {
         if (c->n > 0) hipLaunchKernelGGL(k_heat_markers, dim3(blocks_for(c->n, HEAT_BLOCK)), dim3(HEAT_BLOCK), 0,
                                          c->stream, a, heatdif);
-        hipLaunchKernelGGL(k_heat_nodes, dim3(blocks_for(c->nodenum, HEAT_BLOCK)), dim3(HEAT_BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL(k_heat_cells, dim3(blocks_for(c->ncell, HEAT_CELLS)), dim3(64), 0, c->stream, a);
+        hipLaunchKernelGGL(k_heat_border, dim3(blocks_for(c->xnumx + c->ynumy - 1, HEAT_BLOCK)), dim3(HEAT_BLOCK), 0,
+                           c->stream, a);
         if (heatdif != 0 && c->n > 0)
             hipLaunchKernelGGL(k_heat_antidiff, dim3(blocks_for(c->n, HEAT_BLOCK)), dim3(HEAT_BLOCK), 0, c->stream, a);
     });

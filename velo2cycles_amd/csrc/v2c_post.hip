@@ -148,14 +148,50 @@ __global__ __launch_bounds__(POST_BLOCK) void k_stress(KArgs a, double dt, Fold 
     if (threadIdx.x < NSTAT) part[(long)blockIdx.x * NSTAT + threadIdx.x] = F[threadIdx.x][0];
 }
 
-// One thread: block folds composed in order, applied to the routine's initial
-// values (0, -1e50, 1e50, 0, 1e30).
-__global__ void k_stress_final(const Fold *__restrict__ part, int nblocks, double *__restrict__ out)
+// ---- ordered composition of per-block folds --------------------------------------
+// One workgroup of FIN_BLOCK threads: thread t composes, in order, the folds of
+// a contiguous run of blocks, then the threads' results are composed pairwise
+// in order in LDS.  Composition is associative, so the result is the
+// sequential fold over every block whatever the grouping.
+constexpr int FIN_BLOCK = 256;
+
+// Bit k of MAXMASK: component k is a running maximum (else a minimum).
+template <int NS, unsigned MAXMASK>
+__device__ __forceinline__ void fin_compose(Fold (*F)[FIN_BLOCK], const Fold *__restrict__ part, int nblocks,
+                                            Fold *acc)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    constexpr auto is_max = [](int k) { return ((MAXMASK >> k) & 1u) != 0; };
+    const int per = (nblocks + FIN_BLOCK - 1) / FIN_BLOCK;
+    const int b0 = threadIdx.x * per, b1 = min(b0 + per, nblocks);
+    Fold f[NS];
+    for (int k = 0; k < NS; k++) f[k] = Fold{0, 0.0};
+    for (int b = b0; b < b1; b++)
+        for (int k = 0; k < NS; k++) {
+            const Fold x = part[(long)b * NS + k];
+            f[k] = is_max(k) ? fold_then<true>(f[k], x) : fold_then<false>(f[k], x);
+        }
+    for (int k = 0; k < NS; k++) F[k][threadIdx.x] = f[k];
+    __syncthreads();
+    for (int s = 1; s < FIN_BLOCK; s <<= 1) {
+        if ((threadIdx.x & (2 * s - 1)) == 0)
+            for (int k = 0; k < NS; k++) {
+                const Fold l = F[k][threadIdx.x], r = F[k][threadIdx.x + s];
+                F[k][threadIdx.x] = is_max(k) ? fold_then<true>(l, r) : fold_then<false>(l, r);
+            }
+        __syncthreads();
+    }
+    for (int k = 0; k < NS; k++) acc[k] = F[k][0];
+}
+
+// Block folds composed in order, applied to the routine's initial values
+// (0, -1e50, 1e50, 0, 1e30).
+__global__ __launch_bounds__(FIN_BLOCK) void k_stress_final(const Fold *__restrict__ part, int nblocks,
+                                                            double *__restrict__ out)
+{
+    __shared__ Fold F[NSTAT][FIN_BLOCK];
     Fold acc[NSTAT];
-    for (int k = 0; k < NSTAT; k++) acc[k] = Fold{0, 0.0};
-    for (int b = 0; b < nblocks; b++) stat_compose(acc, part + (long)b * NSTAT);
+    fin_compose<NSTAT, 0b01101u>(F, part, nblocks, acc);     // maxds, maxnu, maxdphi
+    if (threadIdx.x != 0) return;
     out[0] = fold_apply<true>(acc[0], 0.0);
     out[1] = fold_apply<false>(acc[1], -1e50);
     out[2] = fold_apply<true>(acc[2], 1e50);
@@ -308,13 +344,45 @@ __global__ __launch_bounds__(POST_BLOCK) void k_ts_lambda(KArgs a, TsArgs T, int
     }
 }
 
-// Pass 2 (one thread): lambda carried into every block.
-__global__ void k_ts_carry(const int *__restrict__ bset, const double *__restrict__ bval, int nblocks,
-                           double lambda0, double *__restrict__ carry)
+// Pass 2: lambda carried into every block, the last one set before it (or
+// lambda0).  One workgroup: each thread takes a contiguous run of blocks; an
+// inclusive scan of (set, value) with "the later set one wins" gives each
+// run's carry-in.
+__global__ __launch_bounds__(FIN_BLOCK) void k_ts_carry(const int *__restrict__ bset,
+                                                        const double *__restrict__ bval, int nblocks,
+                                                        double lambda0, double *__restrict__ carry)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ int sset[FIN_BLOCK];
+    __shared__ double sval[FIN_BLOCK];
+    const int per = (nblocks + FIN_BLOCK - 1) / FIN_BLOCK;
+    const int b0 = threadIdx.x * per, b1 = min(b0 + per, nblocks);
+    int set = 0;
+    double v = 0.0;
+    for (int b = b0; b < b1; b++)
+        if (bset[b]) {
+            set = 1;
+            v = bval[b];
+        }
+    sset[threadIdx.x] = set;
+    sval[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < FIN_BLOCK; s <<= 1) {
+        int ps = 0;
+        double pv = 0.0;
+        if ((int)threadIdx.x >= s) {
+            ps = sset[threadIdx.x - s];
+            pv = sval[threadIdx.x - s];
+        }
+        __syncthreads();
+        if (ps && !sset[threadIdx.x]) {
+            sset[threadIdx.x] = 1;
+            sval[threadIdx.x] = pv;
+        }
+        __syncthreads();
+    }
     double c = lambda0;
-    for (int b = 0; b < nblocks; b++) {
+    if (threadIdx.x > 0 && sset[threadIdx.x - 1]) c = sval[threadIdx.x - 1];
+    for (int b = b0; b < b1; b++) {
         carry[b] = c;
         if (bset[b]) c = bval[b];
     }
@@ -409,17 +477,15 @@ __global__ __launch_bounds__(POST_BLOCK) void k_ts_node(KArgs a, TsArgs T, const
     if (threadIdx.x < NTS) part[(long)blockIdx.x * NTS + threadIdx.x] = F[threadIdx.x][0];
 }
 
-// Pass 4 (one thread): blocks composed in order, applied to the reference's
-// initial values; out = dt, dtnvp, dtd, dtphi, dtL, maxvel, maxVx.
-__global__ void k_ts_final(const Fold *__restrict__ part, int nblocks, TsArgs T, double *__restrict__ out)
+// Pass 4: blocks composed in order, applied to the reference's initial
+// values; out = dt, dtnvp, dtd, dtphi, dtL, maxvel, maxVx.
+__global__ __launch_bounds__(FIN_BLOCK) void k_ts_final(const Fold *__restrict__ part, int nblocks, TsArgs T,
+                                                        double *__restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ Fold F[NTS][FIN_BLOCK];
     Fold acc[NTS];
-    for (int k = 0; k < NTS; k++) acc[k] = Fold{0, 0.0};
-    for (int b = 0; b < nblocks; b++) {
-        for (int k = 0; k < 4; k++) acc[k] = fold_then<false>(acc[k], part[(long)b * NTS + k]);
-        for (int k = 4; k < NTS; k++) acc[k] = fold_then<true>(acc[k], part[(long)b * NTS + k]);
-    }
+    fin_compose<NTS, 0b110000u>(F, part, nblocks, acc);      // maxvel, maxVx
+    if (threadIdx.x != 0) return;
     const double dtnvp = fold_apply<false>(acc[0], 1e+30), dtd = fold_apply<false>(acc[1], 1e+30);
     const double dtphi = fold_apply<false>(acc[2], 1e+30), dtL = fold_apply<false>(acc[3], 1e+30);
     const double maxvel = fold_apply<true>(acc[4], 0.0), maxVx = fold_apply<true>(acc[5], 0.0);
@@ -474,7 +540,7 @@ int v2c_recompute_stress_strainrate(v2c_ctx *c, double dt, int rk, v2c_stress_st
     const KArgs a = make_args(c);
     rc = launch(c, V2C_K_POST, [&] {
         hipLaunchKernelGGL(k_stress, dim3(nb), dim3(POST_BLOCK), 0, c->stream, a, dt, part);
-        hipLaunchKernelGGL(k_stress_final, dim3(1), dim3(64), 0, c->stream, part, (int)nb, out);
+        hipLaunchKernelGGL(k_stress_final, dim3(1), dim3(FIN_BLOCK), 0, c->stream, part, (int)nb, out);
     });
     if (rc) return rc;
     double h[NSTAT];
@@ -542,9 +608,10 @@ int v2c_calc_timestep(v2c_ctx *c, int oldornew, const v2c_timestep_params *p, v2
     rc = launch(c, V2C_K_POST, [&] {
         hipLaunchKernelGGL(k_post_clear, dim3(1), dim3(64), 0, c->stream, err, 1);
         hipLaunchKernelGGL(k_ts_lambda, dim3(nb), dim3(POST_BLOCK), 0, c->stream, a, T, bset, bval, err);
-        hipLaunchKernelGGL(k_ts_carry, dim3(1), dim3(64), 0, c->stream, bset, bval, (int)nb, p->lambda0, carry);
+        hipLaunchKernelGGL(k_ts_carry, dim3(1), dim3(FIN_BLOCK), 0, c->stream, bset, bval, (int)nb, p->lambda0,
+                           carry);
         hipLaunchKernelGGL(k_ts_node, dim3(nb), dim3(POST_BLOCK), 0, c->stream, a, T, carry, part);
-        hipLaunchKernelGGL(k_ts_final, dim3(1), dim3(64), 0, c->stream, part, (int)nb, T, out);
+        hipLaunchKernelGGL(k_ts_final, dim3(1), dim3(FIN_BLOCK), 0, c->stream, part, (int)nb, T, out);
     });
     if (rc) return rc;
     double h[7];
