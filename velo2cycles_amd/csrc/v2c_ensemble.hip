@@ -62,7 +62,7 @@ __global__ __launch_bounds__(ENS_BLOCK) void k_ens_m2n_props(const KArgs *__rest
 __global__ __launch_bounds__(WAVE) void k_ens_m2n_cells(const KArgs *__restrict__ A,
                                                         const EnsTile *__restrict__ tiles, int ntiles)
 {
-    __shared__ ScatterLds<M_NF> L;
+    __shared__ M2nLds L;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const EnsTile T = tiles[t];
         m2n_cells_group(model_args(A, T.model), T.begin, T.end, threadIdx.x, L);
@@ -138,7 +138,7 @@ __global__ void k_ens_phantom(const KArgs *__restrict__ A, int nmodels)
 __global__ __launch_bounds__(WAVE) void k_ens_n2m_scatter(const KArgs *__restrict__ A,
                                                           const EnsTile *__restrict__ tiles, int ntiles)
 {
-    __shared__ ScatterLds<N_NF> L;
+    __shared__ N2mLds L;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const EnsTile T = tiles[t];
         const KArgs &a = model_args(A, T.model);

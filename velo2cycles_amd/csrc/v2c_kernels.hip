@@ -392,35 +392,77 @@ using NRec = NodeRec<N_NB, N_NV, N_NC>;
 
 // ---------------------------------------------------------------------------
 // Segmented sums of one batch of 64 sorted markers.  Lanes [0, nact) hold NF
-// contributions in con[lane][f] and their sub-cell key in skey[lane]; equal
-// keys are contiguous.  Work items (run, field), run-major, are spread over
-// the lanes: each sums its run serially in lane order (the result does not
-// depend on timing), and consecutive lanes read consecutive LDS words and
-// write consecutive fields of a node record.  A sub-cell whose markers
-// continue into the next batch (cont) is carried in LDS, double-buffered by
-// batch parity, and finished there; every other run is complete and written
-// once.
+// contributions in registers and their sub-cell key in skey[lane]; equal
+// keys are contiguous.  The contributions pass through LDS CH fields at a
+// time (con[lane][f]), so a wavefront holds 64 x CH doubles of LDS rather
+// than 64 x NF and more wavefronts fit on a CU.  Per chunk, work items
+// (run, field), run-major, are spread over the lanes: each sums its run
+// serially in lane order (the result depends neither on timing nor on the
+// chunking), and consecutive lanes read consecutive LDS words and write
+// consecutive fields of a node record.  A sub-cell whose markers continue
+// into the next batch (cont) is carried in LDS, double-buffered by batch
+// parity, and finished there; every other run is complete and written once.
 // ---------------------------------------------------------------------------
-template <class R>
-__device__ __forceinline__ void scatter_batch(const double (*con)[R::NF], const uint32_t *skey, int *runs,
-                                              double (*carry)[R::NF], int par, bool have_carry, bool cont,
-                                              int lane, int nact, const KArgs &a)
+template <int NF, int CH = NF>
+struct ScatterLds {
+    static constexpr int CHUNK = CH;
+    double con[WAVE][CH];
+    double carry[2][NF];
+    uint32_t skey[WAVE];
+    int runs[WAVE + 1];
+};
+
+// Run heads of the batch into runs[0..nr], runs[nr] = nact; returns nr.  The
+// caller synchronises before the runs are read.
+__device__ __forceinline__ int scatter_runs(const uint32_t *skey, int *runs, int lane, int nact)
 {
-    constexpr int NF = R::NF;
     const bool head = lane < nact && (lane == 0 || skey[lane - 1] != skey[lane]);
     const unsigned long long hm = __ballot(head);
     const int nr = __popcll(hm);
     if (head) runs[__popcll(hm & ((1ull << lane) - 1ull))] = lane;
     if (lane == 0) runs[nr] = nact;
-    __syncthreads();
-    for (int w = lane; w < nr * NF; w += WAVE) {
-        const int r = w / NF, f = w - r * NF;
+    return nr;
+}
+
+// Fields [F0, F0 + W) of the batch, staged in con (row stride CH).
+template <class R, int F0, int W, int CH>
+__device__ __forceinline__ void scatter_fields(const double (*con)[CH], const uint32_t *skey, const int *runs,
+                                               int nr, double (*carry)[R::NF], int par, bool have_carry,
+                                               bool cont, int lane, const KArgs &a)
+{
+    for (int w = lane; w < nr * W; w += WAVE) {
+        const int r = w / W, f = w - r * W;
         const int b = runs[r], e = runs[r + 1];
-        double sum = (r == 0 && have_carry) ? carry[par][f] : 0.0;
+        double sum = (r == 0 && have_carry) ? carry[par][F0 + f] : 0.0;
         for (int j = b; j < e; j++) sum += con[j][f];
-        if (r == nr - 1 && cont) carry[par ^ 1][f] = sum;
-        else a.part[R::index(skey[b], f, a.g.ynumy, a.nodenum)] = sum;
+        if (r == nr - 1 && cont) carry[par ^ 1][F0 + f] = sum;
+        else a.part[R::index(skey[b], F0 + f, a.g.ynumy, a.nodenum)] = sum;
     }
+}
+
+// One batch: the lane's key and NF contributions v in, sums out.  Starts
+// and ends with the wavefront synchronised.
+template <class R, int CH, int F0 = 0>
+__device__ __forceinline__ void scatter_chunks(ScatterLds<R::NF, CH> &L, const double *v, int nr, int par,
+                                               bool have_carry, bool cont, int lane, const KArgs &a)
+{
+    constexpr int W = R::NF - F0 < CH ? R::NF - F0 : CH;
+#pragma unroll
+    for (int f = 0; f < W; f++) L.con[lane][f] = v[F0 + f];
+    __syncthreads();
+    scatter_fields<R, F0, W, CH>(L.con, L.skey, L.runs, nr, L.carry, par, have_carry, cont, lane, a);
+    __syncthreads();
+    if constexpr (F0 + CH < R::NF) scatter_chunks<R, CH, F0 + CH>(L, v, nr, par, have_carry, cont, lane, a);
+}
+
+template <class R, int CH>
+__device__ __forceinline__ void scatter_batch(ScatterLds<R::NF, CH> &L, const double *v, uint32_t key, int par,
+                                              bool have_carry, bool cont, int lane, int nact, const KArgs &a)
+{
+    L.skey[lane] = key;
+    __syncthreads();
+    const int nr = scatter_runs(L.skey, L.runs, lane, nact);
+    scatter_chunks<R, CH>(L, v, nr, par, have_carry, cont, lane, a);
 }
 
 // Zero the slots fed by the group's empty sub-cells (node passes read every
@@ -489,17 +531,13 @@ __global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
 // weights and contributions of every marker inside the grid, reduced per
 // (cell, quadrant) and written as per-cell partials.
 // ---------------------------------------------------------------------------
-template <int NF>
-struct ScatterLds {
-    double con[WAVE][NF];
-    double carry[2][NF];
-    uint32_t skey[WAVE];
-    int runs[WAVE + 1];
-};
-
 // Cells [c0, c1) of one wavefront group.
-__device__ __forceinline__ void m2n_cells_group(const KArgs &a, long c0, long c1, int lane,
-                                                ScatterLds<M_NF> &L)
+// Contributions staged through LDS 9 fields at a time (3 chunks of the 27):
+// 5.6 KB of LDS per wavefront instead of 14.8 KB.
+constexpr int M_CH = 9;
+using M2nLds = ScatterLds<M_NF, M_CH>;
+
+__device__ __forceinline__ void m2n_cells_group(const KArgs &a, long c0, long c1, int lane, M2nLds &L)
 {
     const uint32_t beg = a.sub_start[c0 * 4], end = a.sub_start[c1 * 4];
     const DevMat *M = a.mat;
@@ -571,21 +609,16 @@ __device__ __forceinline__ void m2n_cells_group(const KArgs &a, long c0, long c1
                 v[M_C + 6] = cw;
             }
         }
-#pragma unroll
-        for (int f = 0; f < M_NF; f++) L.con[lane][f] = v[f];
-        L.skey[lane] = key;
         const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
-        __syncthreads();
-        scatter_batch<MRec>(L.con, L.skey, L.runs, L.carry, par, have_carry, cont, lane, nact, a);
+        scatter_batch<MRec, M_CH>(L, v, key, par, have_carry, cont, lane, nact, a);
         have_carry = cont;
-        __syncthreads();
     }
     zero_empty<MRec>(a, c0, c1, lane);
 }
 
 __global__ __launch_bounds__(WAVE) void k_m2n_cells(KArgs a, int nwg)
 {
-    __shared__ ScatterLds<M_NF> L;
+    __shared__ M2nLds L;
     const long c0 = (long)xcd_remap(blockIdx.x, nwg) * CPW;
     m2n_cells_group(a, c0, c0 + CPW < a.ncell ? c0 + CPW : a.ncell, threadIdx.x, L);
 }
@@ -950,8 +983,9 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
 // every marker at its clamped position, reduced per (cell, quadrant) into
 // per-cell partials.
 // Cells [c0, c1) of one wavefront group.
-__device__ __forceinline__ void n2m_scatter_group(const KArgs &a, long c0, long c1, int lane,
-                                                  ScatterLds<N_NF> &L)
+using N2mLds = ScatterLds<N_NF>;
+
+__device__ __forceinline__ void n2m_scatter_group(const KArgs &a, long c0, long c1, int lane, N2mLds &L)
 {
     const DevGrid &g = a.g;
     const uint32_t beg = a.sub_start[c0 * 4], end = a.sub_start[c1 * 4];
@@ -974,21 +1008,16 @@ __device__ __forceinline__ void n2m_scatter_group(const KArgs &a, long c0, long 
             const double x = clampd(MK(a, MF_X, m), 0, g.xsize), y = clampd(MK(a, MF_Y, m), 0, g.ysize);
             anti_contrib(a, x, y, X, Y, a.mat->ro[a.t[m]], v);
         }
-#pragma unroll
-        for (int f = 0; f < N_NF; f++) L.con[lane][f] = v[f];
-        L.skey[lane] = key;
         const bool cont = base + WAVE < end && a.skey[base + WAVE] == a.skey[base + WAVE - 1];
-        __syncthreads();
-        scatter_batch<NRec>(L.con, L.skey, L.runs, L.carry, par, have_carry, cont, lane, nact, a);
+        scatter_batch<NRec, N_NF>(L, v, key, par, have_carry, cont, lane, nact, a);
         have_carry = cont;
-        __syncthreads();
     }
     zero_empty<NRec>(a, c0, c1, lane);
 }
 
 __global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
 {
-    __shared__ ScatterLds<N_NF> L;
+    __shared__ N2mLds L;
     const long c0 = (long)xcd_remap(blockIdx.x, nwg) * CPW;
     n2m_scatter_group(a, c0, c0 + CPW < a.ncell ? c0 + CPW : a.ncell, threadIdx.x, L);
 }
