@@ -64,6 +64,34 @@ SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
 
 
 def cpu_baseline(cfg, sample: int, steps: int = 2):
+    """The reference's own OpenMP routines (oracle/_ref/ref_driver, built from
+    its sources) timed on this host's cores for `steps` full steps on the
+    largest window of the production grid its capacity macros allow
+    (synth.config_production_window: 250 x 120 nodes of the 500 m refined
+    part, 36 markers per cell, 1.07 M markers), with OMP_NUM_THREADS threads
+    and with one; plus the FP64 oracle on a sample of the production cloud
+    itself.  Without the built driver, the oracle figure alone ("port")."""
+    port = cpu_port(cfg, sample, steps)
+    from oracle import refrun
+    if not refrun.built(refrun.DRIVER):
+        return port
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    win = synth.config_production_window()
+    dt = cfg.timestep
+    ops = ["m2n", f"before:{dt}:0", f"after:{dt}:0", "n2m", "move"] * steps
+    rates = {}
+    for th in sorted({threads, 1}):
+        _, _, raw = refrun.run(win, ops, timesum=dt, threads=th)
+        rates[th] = win.n * steps / float(raw["drv_time"][0])
+    return {"value": rates[threads], "unit": "marker-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"reference routines (marker2nodes, compute_before_solve, compute_after_solve, "
+                      f"nodes2markers, moveandrotatemarkers; OpenMP, {threads} threads) on "
+                      f"{win.grid.xnumx}x{win.grid.ynumy} nodes of the production grid's 500 m part with "
+                      f"{win.n:,} markers (the reference's capacity), {steps} steps, op time only",
+            "value_1thread": rates[1], "port_1thread": port}
+
+
+def cpu_port(cfg, sample: int, steps: int = 2):
     """FP64 oracle, one thread, full step on the first `sample` markers of the
     same cloud (x-major cell order) with the full nodal grid."""
     from oracle.pyoracle import Oracle

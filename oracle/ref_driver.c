@@ -35,6 +35,7 @@
 #include <string.h>
 #include <omp.h>
 #include <stdbool.h>
+#include <time.h>
 
 #ifdef V2C_DROPIN
 #define marker2nodes ref_marker2nodes
@@ -93,6 +94,7 @@ static char dv_type[DRV_MAX];
 static double ai_x[DRV_MAX], ai_y[DRV_MAX], ai_vx[DRV_MAX], ai_vy[DRV_MAX], ai_esp[DRV_MAX], ai_ee[DRV_MAX];
 static long drv_n;
 static double drv_ret[8];
+static double drv_time;   /* wall seconds of the op sequence */
 static double RK_sliprs0[MAXNOD], RK_sliprs1[MAXNOD], RK_phidts0[MAXNOD], RK_phidts1[MAXNOD];
 
 typedef struct { const char *name; char kind; void *ptr; long cap; } entry;
@@ -144,7 +146,7 @@ static entry table[] = {
     D(marke0), D(marke1), D(markf0), D(markf1), D(markv),
     /* pure-function op arrays */
     D(dv_sxy), D(dv_sxx), D(dv_tk), D(dv_pb), D(dv_out), {"dv_type", 'c', (void *)dv_type, DRV_MAX},
-    D(ai_x), D(ai_y), D(ai_vx), D(ai_vy), D(ai_esp), D(ai_ee), L(drv_n), D(drv_ret),
+    D(ai_x), D(ai_y), D(ai_vx), D(ai_vy), D(ai_esp), D(ai_ee), L(drv_n), D(drv_ret), S(drv_time),
 };
 #define NTAB ((long)(sizeof table / sizeof table[0]))
 
@@ -226,6 +228,8 @@ int main(int argc, char **argv)
         if (rc) { fprintf(stderr, "ref_driver: v2c_dropin_attach failed (%d)\n", rc); return 6; }
     }
 #endif
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
     for (int a = 3; a < argc; a++) {
         const char *op = argv[a];
         double dt; int rk;
@@ -262,6 +266,8 @@ int main(int argc, char **argv)
 #endif
         else { fprintf(stderr, "ref_driver: unknown op %s\n", op); return 4; }
     }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    drv_time = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 #ifdef V2C_DROPIN
     v2c_dropin_markers_to_host();
     v2c_dropin_detach();

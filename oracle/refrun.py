@@ -77,7 +77,8 @@ def _read(path):
     return out
 
 
-def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=None, driver=DRIVER):
+def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=None, driver=DRIVER,
+        threads=1):
     """Write cfg (+ overrides) into the reference globals, run `ops`, return
     (markers dict, nodal dict, raw dict).  Markers are returned by the build's
     field names, nodal arrays by reference names (length nodenum)."""
@@ -116,9 +117,10 @@ def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=N
                 _rec(f, k, "d", nod[k])
             for k, (kind, arr) in (extra or {}).items():
                 _rec(f, k, kind, arr)
-        # one thread: the reference's merge order (interpolation.c:631-674) and
-        # its racy antidiffusion scatter (:942) make multithreaded runs vary
-        env = dict(os.environ, OMP_NUM_THREADS="1")
+        # one thread by default: the reference's merge order
+        # (interpolation.c:631-674) and its racy antidiffusion scatter (:942)
+        # make multithreaded runs vary (threads > 1 is for timing only)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         with open(os.devnull, "w") as dn:
             subprocess.run([driver, fin, fout] + list(ops), check=True, stdout=dn, env=env, cwd=td)
         raw = _read(fout)

@@ -346,6 +346,22 @@ def config_production(seed=DEFAULT_SEED, **kw):
     return build_config("plain-production-1501x301-36pc", gx, gy, 6, geom, seed, **kw)
 
 
+def config_production_window(seed=DEFAULT_SEED, nx=250, ny=120, **kw):
+    """A window of the production grid that fits the reference's capacity
+    (headevps.c:10-20: 300 x 120 gridlines, 36,000 nodes, 1.08 M markers):
+    nx x ny nodes of its 500 m refined part from x = 450 km and y = 0 (shifted
+    to start at 0), the trench 50 km into the window, 36 markers per cell
+    (~1.07 M at 250 x 120), ~2 % fault markers."""
+    gx = refined_axis(1501, 1500e3, 450e3, 800e3, 500.0)
+    gy = refined_axis(301, 300e3, 0.0, 60e3, 500.0)
+    i0 = int(np.searchsorted(gx, 450e3))
+    gx = gx[i0:i0 + nx] - gx[i0]
+    gy = gy[:ny].copy()
+    geom = Geometry(xtrench=50e3, ytop=10e3)
+    kw.setdefault("fault_frac", 0.02)
+    return build_config(f"production-window-{nx}x{ny}-36pc", gx, gy, 6, geom, seed, **kw)
+
+
 def config_grid(nx, ny, xsize, ysize, per, seed=DEFAULT_SEED, refine=False, **kw):
     """Generic case (used for reference-legal parity sizes)."""
     if refine:
