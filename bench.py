@@ -257,28 +257,26 @@ def valu_issue(workload, kern):
     return out or None
 
 
-def warm_start_eval(lib, dt: float):
-    """Newton iterations per marker and kernel time of the per-marker local
-    update, plain vs learned warm start, on the 401x101 / 16-per-cell cloud
-    (640,000 markers, 2 % fault markers) at residual 1e-10.  The MLP is trained
-    here, on the GPU, from 2 M samples of plain solves on four other seeds."""
+def _warm_regime(lib, dt: float, maker, label: str, seeds=(11, 12, 13, 14), keep_other=1.0, epochs=8):
     from velo2cycles_amd.warmstart import WarmStart, evaluate, generate, init_params
-    cfg = synth.config_small()
+    cfg = maker(seed=synth.DEFAULT_SEED)
     cfg.timestep = dt
     ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=lib)
     t0 = time.perf_counter()
-    X, M = generate(ctx, [11, 12, 13, 14], dt, max_rows=2_000_000)
+    X, M = generate(ctx, list(seeds), dt, maker=maker, max_rows=2_000_000, keep_other=keep_other)
     ws = WarmStart(ctx)
     ws.fit_norm(X, M)
     ws.load(init_params(0))
-    loss = ws.train(X, M, epochs=8, batch=4096, lr=2e-3)
+    loss = ws.train(X, M, epochs=epochs, batch=4096, lr=2e-3)
     t_train = time.perf_counter() - t0
-    res = evaluate(ctx, cfg, ws, dt, tol=1e-10, repeats=3)
+    modes = ("plain", "warm", "warm_rsf")
+    res = evaluate(ctx, cfg, ws, dt, tol=1e-10, repeats=3, modes=modes)
     ctx.close()
-    out = {"workload": "401x101 uniform, 16 markers/cell, 2 % fault markers", "dt_s": dt, "residual": 1e-10,
-           "training_rows": int(len(X)), "train_s": round(t_train, 2), "final_loss": float(loss[-1])}
+    out = {"workload": label, "dt_s": dt, "residual": 1e-10, "training_rows": int(len(X)),
+           "slip_rate_rows": int((M[:, 1] > 0).sum()), "epochs": epochs,
+           "train_s": round(t_train, 2), "final_loss": float(loss[-1])}
     n = cfg.n
-    for mode in ("plain", "warm"):
+    for mode in modes:
         st = res[mode]["stats"]
         out[mode] = {
             "iters_per_marker": (st["ductile_iters"] + st["rsf_iters"]) / n,
@@ -287,8 +285,35 @@ def warm_start_eval(lib, dt: float):
             "fallbacks": st["fallbacks"], "nonfinite": st["nonfinite"],
             "ms_newton": res[mode]["ms_local"], "ms_mlp": res[mode]["ms_mlp"],
             "ms_total": res[mode]["ms_total"]}
-    out["iters_ratio"] = out["plain"]["iters_per_marker"] / max(out["warm"]["iters_per_marker"], 1e-30)
-    out["wall_ratio"] = out["plain"]["ms_total"] / max(out["warm"]["ms_total"], 1e-30)
+    p = out["plain"]
+    for mode in modes[1:]:
+        w = out[mode]
+        out[mode]["iters_ratio"] = p["iters_per_marker"] / max(w["iters_per_marker"], 1e-30)
+        out[mode]["rsf_iters_ratio"] = p["rsf_iters_per_fault_marker"] / max(w["rsf_iters_per_fault_marker"], 1e-30)
+        out[mode]["wall_ratio"] = p["ms_total"] / max(w["ms_total"], 1e-30)
+    return out
+
+
+def warm_start_eval(lib, dt: float):
+    """Newton iterations per marker and kernel time of the per-marker local
+    update, plain vs learned warm start (both guesses, or the slip-rate guess
+    only), at residual 1e-10 on the 401x101 / 16-per-cell cloud (640,000
+    markers, 2 % fault markers), in two regimes: interseismic (dt = 1 yr, the
+    synthetic RSF state) and coseismic (dt = 0.01 s, fault markers slipping at
+    1e-2..10 m/s, synth.coseismic).  Per regime the MLP is trained here, on
+    the GPU, from plain solves on other seeds: 2 M rows of four clouds
+    (interseismic); every fault-marker row of 32 clouds and 5 % of the other
+    rows, 16 epochs (coseismic, where only the slip-rate guess matters)."""
+    inter = _warm_regime(lib, dt, synth.config_small,
+                         "401x101 uniform, 16 markers/cell, 2 % fault markers, interseismic")
+    dtc = 1e-2
+    cos = _warm_regime(lib, dtc, lambda seed: synth.coseismic(synth.config_small(seed=seed), seed=seed, dt=dtc),
+                       "the same cloud, fault markers coseismic (V 1e-3..10 m/s, roots 1e-2..10 m/s)",
+                       seeds=range(11, 43), keep_other=0.05, epochs=16)
+    out = dict(inter)
+    out["iters_ratio"] = inter["warm"]["iters_ratio"]
+    out["wall_ratio"] = inter["warm"]["wall_ratio"]
+    out["coseismic"] = cos
     return out
 
 

@@ -454,6 +454,11 @@ int v2c_mlp_set(v2c_ctx *ctx, const float *params, const float *norm);
 int v2c_mlp_get(v2c_ctx *ctx, float *params, float *norm);
 /* Write warm-start guesses for every resident marker (inputs of dt). */
 int v2c_mlp_predict(v2c_ctx *ctx, double dt);
+/* The same for a subset of the outputs: bit 0 the ductile guess, bit 1 the
+ * slip-rate guess (the other is set to 0, the default start).  Workgroups of
+ * 256 markers none of which needs a selected guess skip the network, so with
+ * outputs = 2 the cost follows the fault markers. */
+int v2c_mlp_predict_outputs(v2c_ctx *ctx, double dt, int outputs);
 /* Training samples from one plain local update of the resident markers:
  * X[n][8] = 6 raw features + 2 labels, mask[n][2] (1 where a label exists).
  * Runs the plain local update (state advances as in v2c_local_update). */

@@ -928,9 +928,11 @@ int v2c_mlp_get(v2c_ctx *c, float *params, float *norm)
     return 0;
 }
 
-int v2c_mlp_predict(v2c_ctx *c, double dt)
+int v2c_mlp_predict(v2c_ctx *c, double dt) { return v2c_mlp_predict_outputs(c, dt, 3); }
+
+int v2c_mlp_predict_outputs(v2c_ctx *c, double dt, int outputs)
 {
-    if (!c) return V2C_EINVAL;
+    if (!c || outputs < 0 || outputs > 3) return V2C_EINVAL;
     if (!c->mlp_ready) return set_err(c, V2C_EINVAL, "no MLP parameters loaded (v2c_mlp_set)");
     HIPCHK(c, hipSetDevice(c->device));
     KArgs a = make_args(c);
@@ -938,7 +940,7 @@ int v2c_mlp_predict(v2c_ctx *c, double dt)
     if (c->n > 0) {
         rc = launch(c, V2C_K_MLP, [&] {
             hipLaunchKernelGGL(k_mlp_guess, dim3(blocks_for(c->n, 64 * MI_WAVES)), dim3(64 * MI_WAVES), 0, c->stream,
-                               a, dt, c->d_mlp, c->d_mlp_norm);
+                               a, dt, c->d_mlp, c->d_mlp_norm, outputs);
         });
         if (rc) return rc;
     }

@@ -186,11 +186,35 @@ struct MlpInferLds {
 // Four consecutive BF16 values (8-byte aligned) as one LDS read.
 __device__ __forceinline__ bf16x4 lds_bf16x4(const short *p) { return *(const bf16x4 *)p; }
 
+// outputs: bit 0 the ductile guess, bit 1 the slip-rate guess; the other is
+// written as 0 (default start).  Markers that can need a selected guess are
+// found first from the type and fault flag alone: a workgroup with none
+// writes zeros and stops before the features and the weights, and a
+// wavefront with none stops after the weights are staged.
 __global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_guess(KArgs a, double dt, const float *__restrict__ P,
-                                                             const float *__restrict__ norm)
+                                                             const float *__restrict__ norm, int outputs)
 {
     __shared__ MlpInferLds s;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const long m = ((long)blockIdx.x * MI_WAVES + w) * 64 + lane;
+    bool need = false;
+    if (m < a.n) {
+        const int t = a.t[m];
+        need = ((outputs & 2) && a.fault[m]) || ((outputs & 1) && power_law_type(a.mat, t) && t != 1);
+    }
+    if (!__syncthreads_or(need)) {
+        if (m < a.n) {
+            MXF(a, MX_GETA, m) = 0.0;
+            MXF(a, MX_GV, m) = 0.0;
+        }
+        return;
+    }
+    const bool wave_need = __ballot(need) != 0;
+    MlpFeat ft;
+    ft.duct = ft.rsf = false;
+    if (m < a.n && wave_need) mlp_features(a, m, dt, ft);
+    ft.duct = ft.duct && (outputs & 1);
+    ft.rsf = ft.rsf && (outputs & 2);
     for (int e = tid; e < MLP_H * 16; e += 64 * MI_WAVES) {
         const int o = e >> 4, k = e & 15;
         s.w1[o][k] = k < MLP_IN ? f2bf(P[P_W1 + o * MLP_IN + k]) : (short)0;
@@ -203,16 +227,19 @@ __global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_guess(KArgs a, double dt,
     for (int e = tid; e < MLP_H; e += 64 * MI_WAVES) { s.b1[e] = P[P_B1 + e]; s.b2[e] = P[P_B2 + e]; }
     if (tid < MLP_OUT) s.b3[tid] = P[P_B3 + tid];
 
-    const long m = ((long)blockIdx.x * MI_WAVES + w) * 64 + lane;
-    MlpFeat ft;
-    ft.duct = ft.rsf = false;
-    if (m < a.n) mlp_features(a, m, dt, ft);
     for (int k = 0; k < 16; k++) {
         float v = 0.f;
         if (k < MLP_IN && m < a.n) v = (ft.f[k] - norm[k]) / norm[MLP_IN + k];
         s.x[w][lane][k] = f2bf(v);
     }
     __syncthreads();
+    if (!wave_need) {
+        if (m < a.n) {
+            MXF(a, MX_GETA, m) = 0.0;
+            MXF(a, MX_GV, m) = 0.0;
+        }
+        return;
+    }
     const int i = lane & 15, kb = 4 * (lane >> 4);
     short(*h)[MLP_H] = s.h[w];
     // layer 1: h = tanh(x W1^T + b1)

@@ -609,3 +609,38 @@ def prn_model(cfg, seed=DEFAULT_SEED, bonds=True) -> dict:
             "markim": np.array(mats.im[:nr], dtype=np.int32), "rock": rock, "node": node, "mark": mark,
             "bondm": bondm, "gx": g.gx.copy(), "gy": g.gy.copy(), "bondv": bondv, "bondn": bondn,
             "timesave": rng.uniform(0.0, 1e12, 5), "t": m.t.copy()}
+
+
+def coseismic(cfg, seed=DEFAULT_SEED, dt=1e-2):
+    """Put cfg's fault markers in the coseismic regime for a local update of
+    time step dt (returns cfg, changed in place, timestep = dt).  Slip rate V
+    log-uniform on [1e-3, 10] m/s with the state near steady state
+    (theta = L/V * U(0.5, 2)); the marker stress is the rate-and-state
+    strength at a target slip rate Vt, log-uniform on [1e-2, 10] m/s, after
+    the aging-law update over dt, plus radiation damping eta_rad * Vt, so the
+    slip-rate root lies near Vt; the strain-rate invariant is 10 Vt / W, so
+    the cap V <= EII * W stays inactive."""
+    rng = np.random.default_rng(seed)
+    p = cfg.params
+    r = cfg.rsf
+    f = r["fault"].astype(bool)
+    k = int(f.sum())
+    a, b, L, V0, mu0 = r["a"][f], r["b"][f], r["L"][f], r["V0"][f], r["mu0"][f]
+    V = 10.0 ** rng.uniform(-3.0, 1.0, k)
+    theta = L / V * rng.uniform(0.5, 2.0, k)
+    phi = np.log(V0 * theta / L)
+    phin = np.log(V0 / V + (np.exp(phi) - V0 / V) * np.exp(-V * dt / L))
+    Vt = 10.0 ** rng.uniform(-2.0, 1.0, k)
+    m = cfg.markers
+    nP = np.full(k, p.nstress) if p.pmode == 0 else m["p"][f]
+    eta_rad = p.rad_damping if p.inertyn == 0 else 0.0
+    sig = nP * a * np.arcsinh(Vt / (2.0 * V0) * np.exp((mu0 + b * phin) / a)) + eta_rad * Vt
+    m["xx"][f] = sig / math.sqrt(2.0)
+    m["xy"][f] = sig / math.sqrt(2.0)
+    eii = 10.0 * Vt / p.faultwidth
+    m["exx"][f] = eii / math.sqrt(2.0)
+    m["exy"][f] = eii / math.sqrt(2.0)
+    r["V"][f] = V
+    r["phi"][f] = phi
+    cfg.timestep = dt
+    return cfg

@@ -27,6 +27,7 @@ def _bind(lib):
     fp = P(C.c_float)
     for name, args in [("v2c_mlp_set", [fp, fp]), ("v2c_mlp_get", [fp, fp]),
                        ("v2c_mlp_predict", [C.c_double]),
+                       ("v2c_mlp_predict_outputs", [C.c_double, C.c_int]),
                        ("v2c_mlp_samples", [C.c_double, C.c_double, fp, fp]),
                        ("v2c_mlp_train", [fp, fp, fp, C.c_long, C.c_int, C.c_int, C.c_float, C.c_uint, fp]),
                        ("v2c_mlp_grad", [fp, fp, fp, C.c_long, fp, fp]),
@@ -125,8 +126,9 @@ class WarmStart:
         self.ctx._check(self.ctx.lib.v2c_mlp_forward(self.ctx.h, _fptr(Xs), len(Xs), _fptr(Y)), "mlp_forward")
         return Y
 
-    def predict(self, dt):
-        self.ctx._check(self.ctx.lib.v2c_mlp_predict(self.ctx.h, dt), "mlp_predict")
+    def predict(self, dt, outputs=3):
+        """Guesses for the resident markers: outputs bit 0 ductile, bit 1 slip rate."""
+        self.ctx._check(self.ctx.lib.v2c_mlp_predict_outputs(self.ctx.h, dt, outputs), "mlp_predict")
 
 
 # ----------------------------------------------------------------------------- FP64 reference
@@ -177,7 +179,10 @@ def upload_cloud(ctx, cfg):
     ctx.rsf_upload(r["fault"], r["V"], r["phi"], r["a"], r["b"], r["L"], r["V0"], r["mu0"])
 
 
-def generate(ctx, seeds, dt, maker=synth.config_small, max_rows=2_000_000):
+def generate(ctx, seeds, dt, maker=synth.config_small, max_rows=2_000_000, keep_other=1.0):
+    """Training rows from plain solves on the clouds of `seeds`.  keep_other < 1
+    keeps every row with a slip-rate label but only that fraction (seeded) of
+    the others, so more clouds fit in max_rows when fault markers are rare."""
     ws = WarmStart(ctx)
     Xs, Ms = [], []
     for s in seeds:
@@ -185,6 +190,10 @@ def generate(ctx, seeds, dt, maker=synth.config_small, max_rows=2_000_000):
         cfg.timestep = dt
         upload_cloud(ctx, cfg)
         X, M = ws.samples(dt)
+        if keep_other < 1.0:
+            rng = np.random.default_rng(s)
+            keep = (M[:, 1] > 0) | (rng.uniform(size=len(M)) < keep_other)
+            X, M = X[keep], M[keep]
         Xs.append(X)
         Ms.append(M)
         if sum(len(x) for x in Xs) >= max_rows:
@@ -194,19 +203,26 @@ def generate(ctx, seeds, dt, maker=synth.config_small, max_rows=2_000_000):
     return X, M
 
 
-def evaluate(ctx, cfg, ws: WarmStart, dt, tol=1e-10, repeats=3):
+MODES = {"plain": 0, "warm": 3, "warm_rsf": 2}   # guesses predicted (bit 0 ductile, bit 1 slip rate)
+
+
+def evaluate(ctx, cfg, ws: WarmStart, dt, tol=1e-10, repeats=3, modes=("plain", "warm")):
     """Plain vs warm local update on the same cloud: iterations, residual
-    evaluations, fallbacks and kernel times (best of `repeats`)."""
+    evaluations, fallbacks and kernel times (best of `repeats`).  Modes:
+    "plain" (default starts), "warm" (network guesses for both solves),
+    "warm_rsf" (network guess for the slip rate only; the network runs only
+    for workgroups holding fault markers)."""
     res = {}
-    for mode in ("plain", "warm"):
+    for mode in modes:
+        outputs = MODES[mode]
         best = None
         for _ in range(repeats):
             upload_cloud(ctx, cfg)
             ctx.enable_timing(True)
             ctx.reset_timing()
-            if mode == "warm":
-                ws.predict(dt)
-            st = ctx.local_update(dt, warm=(mode == "warm"), tol=tol)
+            if outputs:
+                ws.predict(dt, outputs)
+            st = ctx.local_update(dt, warm=bool(outputs), tol=tol)
             ms_lu, _ = ctx.kernel_time("local")
             ms_mlp, _ = ctx.kernel_time("mlp")
             ctx.enable_timing(False)
