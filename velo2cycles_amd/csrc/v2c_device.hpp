@@ -526,12 +526,14 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
     const Axis4 X = hx ? *hx : find_x(g, x), Y = hy ? *hy : find_y(g, y);
     const double *vx = NOD(a, V2C_N_vx), *vy = NOD(a, V2C_N_vy);
 
-    Stencil s = basic_stencil(g, X, Y, x, y);
+    // the four stencils with their shared divisions (identical weights)
+    const Stencil4 S4 = stencils4(g, X, Y, x, y);
+    Stencil s = S4.b;
     W4 w = wts(s);
     double EXY = bil(NOD(a, V2C_N_exy), s.m3, ynumy, w);
     ESP = bil(NOD(a, V2C_N_esp), s.m3, ynumy, w);
 
-    s = centre_stencil(g, X, Y, x, y);
+    s = S4.c;
     w = wts(s);
     double EXX = bil(NOD(a, V2C_N_exx), s.m3, ynumy, w);
     // pressure-node averages of vx and vy (interpolation.c:1725-1736)
@@ -542,9 +544,9 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
                   + w.d2 * (vy[m3] + vy[m3 - 1]) + w.d3 * (vy[m3 + 1] + vy[m3])) * 0.5 * (1.0 - xrat);
     EE = sqrt(EXX * EXX + EXY * EXY);
 
-    s = vx_stencil(g, X, Y, x, y);
+    s = S4.vx;
     VX = vxa + bil(vx, s.m3, ynumy, wts(s)) * xrat;
-    s = vy_stencil(g, X, Y, x, y);
+    s = S4.vy;
     VY = vya + bil(vy, s.m3, ynumy, wts(s)) * xrat;
 }
 
