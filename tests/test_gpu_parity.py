@@ -274,3 +274,34 @@ def test_incremental_order_equals_full_sort(gpu, dt):
     assert np.array_equal(k_inc, k_full)
     assert np.array_equal(o_inc, o_full)
     assert np.all(np.diff(k_full.astype(np.int64)) >= 0)
+
+
+def _coseismic_flow(seed=83):
+    """201x101 refined grid, 16 markers per cell, the corner flow scaled from
+    5 cm/yr to ~5 m/s (velocities and strain rates x 3e9), slip rates up to
+    10 m/s on the nodes."""
+    cfg = synth.config_grid(201, 101, 200e3, 100e3, 4, refine=True, seed=seed)
+    s = 3e9
+    for k in ("vx", "vy", "mvx", "mvy", "exx", "exy", "esp", "exxpl"):
+        cfg.nodal[k] = cfg.nodal[k] * s
+    rng = np.random.default_rng(seed)
+    nn = cfg.grid.nodenum
+    cfg.nodal["RK_sliprs_final"] = 10.0 ** rng.uniform(-3.0, 1.0, nn)
+    cfg.nodal["RK_sliprs_solved"] = cfg.nodal["RK_sliprs_final"] * np.exp(rng.uniform(-0.5, 0.5, nn))
+    return cfg
+
+
+@pytest.mark.parametrize("markmod,stoksmod", [(2, 1), (0, 1), (1, 2)])
+def test_coseismic_rates_adaptive_steps(gpu, markmod, stoksmod):
+    """Four steps at coseismic rates with dt growing from 1 ms to 1000 s:
+    displacements up to ~10 cells per step, rotation angles up to ~0.5 rad,
+    slip rates 1e-3..10 m/s; positions and types bit-exact after the last."""
+    cfg = _coseismic_flow()
+    cfg.params.markmod = markmod
+    cfg.params.stoksmod = stoksmod
+    ops, ts = [], 0.0
+    for dt in (1e-3, 1e-1, 10.0, 1e3):
+        ops += [f"time:{ts}:{dt}", "m2n", f"before:{dt}:0", f"after:{dt}:0", "n2m_move"]
+        ts += dt
+    print("case coseismic steps", markmod, stoksmod, cfg.n, flush=True)
+    _check(gpu, cfg, ops)
