@@ -30,7 +30,7 @@ from velo2cycles_amd.abi import KERNELS, Context, load_lib  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes (FP64 = 8 B, type = 1 B): what each kernel must read and
-# write for its function, excluding the build's own intermediates (sort
+# write for its function, excluding the library's own intermediates (sort
 # order, keys, per-cell partials, per-marker property hand-over).
 #   m2n_props   reads x y T sxx sxy phi + type; writes the calc_frictionpara
 #               outputs (6 fields for the markers outside the fault band)
@@ -398,7 +398,7 @@ def main():
                     help="also run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of it "
                                    "(ensemble sharding across GPUs is a design note only)",
                           "n_gpus": args.gpus}))
         sys.exit(2)

@@ -41,7 +41,7 @@ extern "C" {
 #define V2C_ENOMEM (-2)      /* device or host allocation failed            */
 #define V2C_EHIP (-3)        /* HIP runtime error (message in last_error)   */
 #define V2C_ECAPACITY (-4)   /* marker count above the context capacity     */
-#define V2C_EUNSUPPORTED (-5)/* mode the build does not implement           */
+#define V2C_EUNSUPPORTED (-5)/* mode the library does not implement         */
 #define V2C_ENOGPU (-6)      /* no usable gfx950 device                      */
 
 /* ---- grid (headevps.c:36-40,91) ---------------------------------------- */
@@ -200,7 +200,7 @@ int v2c_order_stats(v2c_ctx *ctx, long *last_movers, long *full_sorts, long *inc
 /* Brings the cell ordering up to date (as the next marker2nodes would) and
  * copies it out: order[i] is the marker id at sorted position i and skey[i]
  * its sub-cell key (cell*4 + quadrant), i < n <= marker count.  No reference
- * counterpart: the reference keeps no ordering; this exposes the build's own
+ * counterpart: the reference keeps no ordering; this exposes the library's own
  * state for inspection and tests. */
 int v2c_order_get(v2c_ctx *ctx, unsigned int *order, unsigned int *skey, long n);
 

@@ -80,7 +80,7 @@ def _read(path):
 def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=None, driver=DRIVER,
         threads=1):
     """Write cfg (+ overrides) into the reference globals, run `ops`, return
-    (markers dict, nodal dict, raw dict).  Markers are returned by the build's
+    (markers dict, nodal dict, raw dict).  Markers are returned by the library's
     field names, nodal arrays by reference names (length nodenum)."""
     assert built(driver) or (driver == DRIVER and available()), "reference driver unavailable"
     assert legal(cfg), "size exceeds the reference capacity macros"

@@ -1,6 +1,6 @@
 """The oracle against the live in-place reference build (oracle/_ref), at
 reference-legal sizes beyond the committed fixtures.  Skipped where the
-reference sources are absent (the GPU box)."""
+reference sources are absent."""
 import numpy as np
 import pytest
 

@@ -120,18 +120,18 @@ def write(path: str, d: dict, lib=None):
         raise V2CError(f"prn_write: {ERRORS.get(rc, rc)}: {lib.v2c_prn_last_error().decode()}")
 
 
-# .prn marker columns that are marker fields of this build (markv, the
+# .prn marker columns that are marker fields of the library (markv, the
 # marker's viscosity, only travels through the file).
 _MARK_TO_FIELD = {k: k for k in PRN_MARK if k in MARKER_FIELDS}
 
 
 def materials(d: dict) -> Materials:
     """The rock table of a read() result as the Materials a Context takes
-    (columns outside the build's table, such as the brittle koefficients
+    (columns outside the library's material table, such as the brittle koefficients
     ll, a0 ... f1, stay in the file only)."""
     nr = int(d["rocknum"])
     if nr > V2C_MAX_TYPES:
-        raise V2CError(f"prn: {nr} rock types, the build holds {V2C_MAX_TYPES}")
+        raise V2CError(f"prn: {nr} rock types, the library holds {V2C_MAX_TYPES}")
     mats = Materials()
     mats.ntypes = nr
     for f, _ in Materials._fields_:
@@ -146,7 +146,7 @@ def to_state(d: dict):
     """(GridArrays, Materials, MarkerArrays, nodal dict) from a read() result,
     ready for Context(grid, params, mats, ...), upload_markers and
     set_all_nodal.  The nodal dict holds the file's node columns that are
-    nodal fields of the build (of the two RK_sliprs_final columns, the second,
+    nodal fields of the library (of the two RK_sliprs_final columns, the second,
     the one load_model keeps: input_RSF.c:126)."""
     grid = GridArrays(d["gx"], d["gy"], d["xsize"], d["ysize"])
     fields = {f: d["mark"][k] for k, f in _MARK_TO_FIELD.items()}

@@ -581,7 +581,7 @@ def boundary_equations(cfg, seed=DEFAULT_SEED, extra=True):
 def prn_model(cfg, seed=DEFAULT_SEED, bonds=True) -> dict:
     """A .prn model (velo2cycles_amd.prn.read's dict) holding cfg's grid,
     markers, node columns and rock table, with seeded values for what cfg
-    does not carry (marker viscosity, rock columns outside the build's
+    does not carry (marker viscosity, rock columns outside the library's
     table, the header scalars, timesave) and cfg's boundary equations."""
     from .prn import PRN_MARK, PRN_NODE, PRN_ROCK
     rng = np.random.default_rng(seed)

@@ -4,7 +4,7 @@ evaluation of iterations and wall time, plain against warm.
 
 The network (6 -> 64 -> 64 -> 2, tanh) predicts log10(root / default start)
 for the ductile Newton unknown and for the fault slip rate; Newton to the
-same residual remains the judge, so a poor guess costs iterations, never
+same residual remains the arbiter, so a poor guess costs iterations, never
 accuracy.
 """
 from __future__ import annotations
