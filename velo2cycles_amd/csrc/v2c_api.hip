@@ -19,6 +19,10 @@ struct TimedLaunch { int kernel; hipEvent_t start, stop; };
 struct v2c_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // marker2nodes runs its per-marker property pass here, beside the
+    // re-ordering on `stream` (default priority, like `stream`)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int async_mode = 0;
     int write_friction = 1;
     v2c_params prm;
@@ -108,13 +112,14 @@ static hipEvent_t take_event(v2c_ctx *c)
 
 // Launch helper: records events around the launch when timing is on.
 template <typename F>
-static int launch(v2c_ctx *c, int kid, F &&fn)
+static int launch(v2c_ctx *c, int kid, F &&fn, hipStream_t st = nullptr)
 {
+    if (!st) st = c->stream;
     TimedLaunch tl{kid, nullptr, nullptr};
     if (c->timing) {
         tl.start = take_event(c);
         tl.stop = take_event(c);
-        if (tl.start) (void)hipEventRecord(tl.start, c->stream);
+        if (tl.start) (void)hipEventRecord(tl.start, st);
     }
     hipError_t e = hipSuccess;
     if constexpr (std::is_same_v<decltype(fn()), hipError_t>) e = fn();
@@ -122,7 +127,7 @@ static int launch(v2c_ctx *c, int kid, F &&fn)
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return set_err(c, V2C_EHIP, "kernel %d launch failed: %s", kid, hipGetErrorString(e));
     if (c->timing && tl.start && tl.stop) {
-        (void)hipEventRecord(tl.stop, c->stream);
+        (void)hipEventRecord(tl.stop, st);
         c->pending.push_back(tl);
     }
     return 0;
@@ -345,6 +350,9 @@ static void free_all(v2c_ctx *c)
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     if (c->h_nsel) (void)hipHostFree(c->h_nsel);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -388,6 +396,9 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     } while (0)
     CK(hipSetDevice(device));
     CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     CK(hipMalloc(&c->d_gx, sizeof(double) * g->xnumx));
     CK(hipMalloc(&c->d_gy, sizeof(double) * g->ynumy));
     CK(hipMalloc(&c->d_mat, sizeof(DevMat)));
@@ -615,16 +626,26 @@ int v2c_marker2nodes(v2c_ctx *c)
 {
     if (!c) return V2C_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_order(c);
+    // The property pass reads markers in physical order and writes only the
+    // property hand-over and marker phi, which the re-ordering never touches:
+    // it runs on the side stream while `stream` brings the order up to date,
+    // and the scatter waits for both.
+    int rc = 0;
+    if (c->n > 0) {
+        const KArgs a = make_args(c);
+        const int nb = (int)blocks_for(c->n, 256);
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        rc = launch(c, V2C_K_M2N_PROPS, [&] {
+            hipLaunchKernelGGL(k_m2n_props, dim3(nb), dim3(256), 0, c->side, a, nb);
+        }, c->side);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+    }
+    rc = ensure_order(c);
+    if (c->n > 0) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (rc) return rc;
     KArgs a = make_args(c);
-    if (c->n > 0) {
-        const int nb = (int)blocks_for(c->n, 256);
-        rc = launch(c, V2C_K_M2N_PROPS, [&] {
-            hipLaunchKernelGGL(k_m2n_props, dim3(nb), dim3(256), 0, c->stream, a, nb);
-        });
-        if (rc) return rc;
-    }
     const int nwg = (int)((c->ncell + CPW - 1) / CPW);
     rc = launch(c, V2C_K_M2N_CELLS, [&] {
         hipLaunchKernelGGL(k_m2n_cells, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
