@@ -302,14 +302,14 @@ def warm_start_eval(lib, dt: float):
     synthetic RSF state) and coseismic (dt = 0.01 s, fault markers slipping at
     1e-2..10 m/s, synth.coseismic).  Per regime the MLP is trained here, on
     the GPU, from plain solves on other seeds: 2 M rows of four clouds
-    (interseismic); every fault-marker row of 32 clouds and 5 % of the other
-    rows, 16 epochs (coseismic, where only the slip-rate guess matters)."""
+    (interseismic); every fault-marker row of 64 clouds and 2 % of the other
+    rows, 32 epochs (coseismic, where only the slip-rate guess matters)."""
     inter = _warm_regime(lib, dt, synth.config_small,
                          "401x101 uniform, 16 markers/cell, 2 % fault markers, interseismic")
     dtc = 1e-2
     cos = _warm_regime(lib, dtc, lambda seed: synth.coseismic(synth.config_small(seed=seed), seed=seed, dt=dtc),
                        "the same cloud, fault markers coseismic (V 1e-3..10 m/s, roots 1e-2..10 m/s)",
-                       seeds=range(11, 43), keep_other=0.05, epochs=16)
+                       seeds=range(11, 75), keep_other=0.02, epochs=32)
     out = dict(inter)
     out["iters_ratio"] = inter["warm"]["iters_ratio"]
     out["wall_ratio"] = inter["warm"]["wall_ratio"]
