@@ -56,6 +56,29 @@ def test_struct_layout_matches_c():
                      ctypes.sizeof(abi.Materials), ctypes.sizeof(abi.Markers)]
 
 
+def test_prn_struct_matches_c():
+    """The ctypes mirror of v2c_prn has the C struct's size and field offsets."""
+    from velo2cycles_amd.prn import Prn
+    fields = ["marknum", "pkf", "rocknum", "bondnum", "cycle", "markim", "rock", "node", "bondm", "gx",
+              "bondn", "timesave", "mark", "t"]
+    body = ",".join(f"(size_t)offsetof(v2c_prn, {f})" for f in fields)
+    fmt = " ".join(["%zu"] * (len(fields) + 1))
+    src = ('#include "v2c.h"\n#include <stdio.h>\n#include <stddef.h>\nint main(void){printf("' + fmt +
+           '\\n", sizeof(v2c_prn),' + body + ');return 0;}')
+    exe = "/tmp/v2c_prn_layout"
+    subprocess.run(["gcc", "-x", "c", "-I", os.path.dirname(HEADER), "-o", exe, "-"], input=src,
+                   text=True, check=True)
+    got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
+    assert got == [ctypes.sizeof(Prn)] + [getattr(Prn, f).offset for f in fields]
+
+
+def test_dropin_binding_compiles_cleanly():
+    """The reference-side binding is plain C99 against the header: no warnings."""
+    src = os.path.join(os.path.dirname(abi.LIB_PATH), "csrc", "dropin", "v2c_dropin.c")
+    subprocess.run(["gcc", "-std=gnu99", "-Wall", "-Wextra", "-Werror", "-Wno-unused-parameter", "-fsyntax-only",
+                    src], check=True)
+
+
 def test_create_without_gpu_fails_cleanly():
     """With no visible device v2c_create returns ENOGPU, never crashes."""
     from tests.conftest import gpu_available
