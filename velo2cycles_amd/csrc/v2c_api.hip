@@ -683,24 +683,35 @@ static int n2m_impl(v2c_ctx *c, bool advect)
     }
     KArgs a = make_args(c);
     if (advect) HIPCHK(c, hipMemsetAsync(c->d_counters + 1, 0, sizeof(int), c->stream));
+    // The antidiffusion scatter reads marker positions and types and the
+    // solve's nodal fields; its node pass writes the sums and the *ee fields.
+    // The marker update reads neither and, without advection, writes no
+    // position, so then the two run side by side; with advection the scatter
+    // goes first (it reads positions before any marker moves).
+    const bool side = anti && !advect && c->n > 0;
+    hipStream_t st = side ? c->side : c->stream;
+    if (side) {
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    }
     if (c->prm.n2m_phantom && anti) {
         rc = launch(c, V2C_K_N2M_NODES, [&] {
-            hipLaunchKernelGGL(k_n2m_phantom, dim3(1), dim3(1), 0, c->stream, a);
-        });
+            hipLaunchKernelGGL(k_n2m_phantom, dim3(1), dim3(1), 0, st, a);
+        }, st);
         if (rc) return rc;
     }
     if (anti) {
-        // the scatter reads positions before the update moves any marker
         const int nwg = (int)((c->ncell + CPW - 1) / CPW);
         rc = launch(c, V2C_K_N2M_CELLS, [&] {
-            hipLaunchKernelGGL(k_n2m_scatter, dim3(nwg), dim3(WAVE), 0, c->stream, a, nwg);
-        });
+            hipLaunchKernelGGL(k_n2m_scatter, dim3(nwg), dim3(WAVE), 0, st, a, nwg);
+        }, st);
         if (rc) return rc;
         rc = launch(c, V2C_K_N2M_NODES, [&] {
-            hipLaunchKernelGGL(k_n2m_nodes, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, c->stream, a);
-        });
+            hipLaunchKernelGGL(k_n2m_nodes, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, st, a);
+        }, st);
         if (rc) return rc;
     }
+    if (side) HIPCHK(c, hipEventRecord(c->ev_join, c->side));
     if (c->n > 0) {
         const int nb = (int)blocks_for(c->n, 256);
         rc = launch(c, V2C_K_N2M_UPDATE, [&] {
@@ -709,8 +720,9 @@ static int n2m_impl(v2c_ctx *c, bool advect)
             else
                 hipLaunchKernelGGL(k_n2m_update<false>, dim3(nb), dim3(256), 0, c->stream, a, nb);
         });
-        if (rc) return rc;
     }
+    if (side) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (rc) return rc;
     if (advect) c->order_valid = false;
     rc = finish(c);
     if (rc) return rc;
