@@ -439,6 +439,28 @@ def config_ensemble(n_models=64, seed=DEFAULT_SEED, small=False):
         yield k, cfg
 
 
+def config_ensemble_production(n_models=24, seed=DEFAULT_SEED):
+    """n_models production-size models (1501x301 nodes refined to 500 m, 36
+    markers per cell, 16.2 M markers each) sharing one seeded marker cloud;
+    each model has its own convergence rate, 2 to 8 cm/yr (nodal velocities
+    and strain rates of the corner flow scaled), and a-b of the
+    velocity-weakening zone ~ U(-0.006, 0.004).  Yields (index, Config); the
+    host holds the one cloud."""
+    base = config_production(seed)
+    rng = np.random.default_rng(seed + 7)
+    for k in range(n_models):
+        conv = rng.uniform(2.0, 8.0) * CM_PER_YR
+        amb = rng.uniform(-0.006, 0.004)
+        sc = conv / base.geom.conv
+        nod = dict(base.nodal)
+        for f in ("vx", "vy", "mvx", "mvy", "exx", "exy", "esp", "exxpl"):
+            nod[f] = base.nodal[f] * sc
+        prm = type(base.params).from_buffer_copy(base.params)
+        prm.a_vw = prm.b_vw + amb
+        yield k, Config(f"production-ensemble-{k}", base.grid, prm, base.mats, base.markers, nod,
+                        base.timestep, base.timesum, base.geom)
+
+
 def post_solve_nodal(cfg, seed=DEFAULT_SEED, degenerate=True):
     """Nodal state as the post-solve routines (recompute_stress_strainrate,
     interpolate_center2basic(_old), interpolate_basic2center, copyarrays,
