@@ -516,7 +516,8 @@ int v2c_ens_set_shape(v2c_ens *ens, int size_class, const v2c_tile_shape *shape)
 int v2c_ens_get_shape(const v2c_ens *ens, int size_class, v2c_tile_shape *shape);
 /* persistent scatter grid = 4 x blocks_per_cu x compute units wavefronts */
 int v2c_ens_set_blocks_per_cu(v2c_ens *ens, int blocks);
-/* one batched step; use_graph: replay the captured stage graph */
+/* one batched step; use_graph: replay the captured stage graph (every model's
+ * property pass runs beside the batched re-ordering, before the graph) */
 int v2c_ens_step(v2c_ens *ens, int use_graph);
 /* bring every model's cell ordering up to date in one batched pass (the
  * movers of all models in one radix sort); a step does this itself */
@@ -527,8 +528,8 @@ int v2c_ens_step_looped(v2c_ens *ens);
  * rms_count[2*k] = RMS, rms_count[2*k+1] = node count */
 int v2c_ens_rms(v2c_ens *ens, double *rms_count);
 /* time tile shapes per size class and the grid size on the ensemble itself
- * (coordinate descent, reps timed steps each); keeps the fastest.  Advances
- * the models by the steps it runs. */
+ * (coordinate descent, reps timed steps each, the graph's time); keeps the
+ * fastest.  Advances the models by the steps it runs. */
 int v2c_ens_autotune(v2c_ens *ens, int reps, double *best_ms);
 
 #ifdef __cplusplus

@@ -287,6 +287,11 @@ struct EnsPlan {
 struct v2c_ens {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the property pass of every model runs here, beside the re-ordering
+    // (default priority, like `stream`), with its own copy of the arguments
+    hipStream_t side = nullptr;
+    hipEvent_t ev_props = nullptr;
+    KArgs *d_args_p = nullptr, *h_args_p = nullptr;
     std::vector<v2c_ctx *> models;
     std::vector<int> size_class;
     v2c_tile_shape shape[V2C_ENS_CLASSES];
@@ -454,8 +459,7 @@ static void ens_stages(v2c_ens *e)
     const int nm = (int)e->models.size();
     const dim3 mg((unsigned)e->n_mark), ng((unsigned)e->n_node);
     hipLaunchKernelGGL(k_ens_clear, dim3(1), dim3(64), 0, e->stream, e->d_flags, 4);
-    if (e->n_mark)
-        hipLaunchKernelGGL(k_ens_m2n_props, mg, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_mark);
+    // (the property pass ran on the side stream: ens_props)
     if (e->n_cell)
         hipLaunchKernelGGL(k_ens_m2n_cells, dim3(ens_grid(e, e->n_cell)), dim3(WAVE), 0, e->stream, e->d_args,
                            e->t_cell, e->n_cell);
@@ -589,15 +593,33 @@ static int ens_order_batched(v2c_ens *e)
     return 0;
 }
 
-static int ens_prepare(v2c_ens *e)
+static int ens_plan_check(v2c_ens *e)
 {
     const size_t nm = e->models.size();
     for (size_t k = 0; k < nm; k++)
-        if (e->models[k]->n != e->plan_n[k]) {
-            int rc = ens_upload_plan(e);     // marker counts changed: new work lists
-            if (rc) return rc;
-            break;
-        }
+        if (e->models[k]->n != e->plan_n[k]) return ens_upload_plan(e);   // new marker counts: new work lists
+    return 0;
+}
+
+// The property pass of every model, on the side stream: it reads markers in
+// physical order and writes only the property hand-over and marker phi,
+// which the re-ordering never touches, so it runs while the ordering does.
+static int ens_props(v2c_ens *e)
+{
+    if (!e->n_mark) return 0;
+    const size_t nm = e->models.size();
+    for (size_t k = 0; k < nm; k++) e->h_args_p[k] = make_args(e->models[k]);
+    ENSCHK(e, hipMemcpyAsync(e->d_args_p, e->h_args_p, sizeof(KArgs) * nm, hipMemcpyHostToDevice, e->side));
+    hipLaunchKernelGGL(k_ens_m2n_props, dim3((unsigned)e->n_mark), dim3(ENS_BLOCK), 0, e->side, e->d_args_p,
+                       e->t_mark);
+    ENSCHK(e, hipGetLastError());
+    ENSCHK(e, hipEventRecord(e->ev_props, e->side));
+    return 0;
+}
+
+static int ens_prepare(v2c_ens *e)
+{
+    const size_t nm = e->models.size();
     int rc = e->batch_order ? ens_order_batched(e) : ens_order_streams(e);
     if (rc) return rc;
     for (size_t k = 0; k < nm; k++) e->h_args[k] = make_args(e->models[k]);
@@ -641,11 +663,16 @@ int v2c_ens_create(v2c_ens **out, v2c_ctx *const *models, int nmodels)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) e->ncu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(V2C_EHIP);
+    if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess) return fail(V2C_EHIP);
+    if (hipEventCreateWithFlags(&e->ev_props, hipEventDisableTiming) != hipSuccess) return fail(V2C_EHIP);
     e->ordered.assign(nmodels, nullptr);
     for (int k = 0; k < nmodels; k++)
         if (hipEventCreateWithFlags(&e->ordered[k], hipEventDisableTiming) != hipSuccess) return fail(V2C_EHIP);
     if (hipMalloc(&e->d_args, sizeof(KArgs) * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
     if (hipHostMalloc((void **)&e->h_args, sizeof(KArgs) * nmodels, hipHostMallocDefault) != hipSuccess)
+        return fail(V2C_ENOMEM);
+    if (hipMalloc(&e->d_args_p, sizeof(KArgs) * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
+    if (hipHostMalloc((void **)&e->h_args_p, sizeof(KArgs) * nmodels, hipHostMallocDefault) != hipSuccess)
         return fail(V2C_ENOMEM);
     std::vector<double *> reds(nmodels);
     std::vector<int> nred(nmodels);
@@ -696,11 +723,15 @@ void v2c_ens_destroy(v2c_ens *e)
     if (!e) return;
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
-    void *ptrs[] = {e->d_args, e->d_tiles, e->d_red, e->d_redn, e->d_rms, e->d_flags, e->d_ord,
+    if (e->side) (void)hipStreamSynchronize(e->side);
+    void *ptrs[] = {e->d_args, e->d_args_p, e->d_tiles, e->d_red, e->d_redn, e->d_rms, e->d_flags, e->d_ord,
                     e->d_nmov, e->d_base, e->d_mov_in, e->d_mov_out, e->d_sort_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (e->h_args) (void)hipHostFree(e->h_args);
+    if (e->h_args_p) (void)hipHostFree(e->h_args_p);
+    if (e->ev_props) (void)hipEventDestroy(e->ev_props);
+    if (e->side) (void)hipStreamDestroy(e->side);
     if (e->h_ord) (void)hipHostFree(e->h_ord);
     if (e->h_counts) (void)hipHostFree(e->h_counts);
     for (auto ev : e->ordered)
@@ -748,7 +779,11 @@ int v2c_ens_step(v2c_ens *e, int use_graph)
 {
     if (!e) return V2C_EINVAL;
     ENSCHK(e, hipSetDevice(e->device));
-    int rc = ens_prepare(e);
+    int rc = ens_plan_check(e);
+    if (!rc) rc = ens_props(e);
+    if (rc) return rc;
+    rc = ens_prepare(e);
+    if (e->n_mark) ENSCHK(e, hipStreamWaitEvent(e->stream, e->ev_props, 0));
     if (rc) return rc;
     if (!use_graph) {
         ens_stages(e);
@@ -819,7 +854,8 @@ int v2c_ens_rms(v2c_ens *e, double *rms_count)
 
 // Coordinate descent over the tile shapes of each size class present, then
 // the persistent grid size: each candidate is timed over `reps` graph-replayed
-// steps (after one untimed step) and the fastest kept.  The ensemble's state
+// steps (after one untimed step) and the fastest kept.  The time is the
+// graph's: the re-ordering and the property pass beside it run before.  The ensemble's state
 // advances by the steps run.
 int v2c_ens_autotune(v2c_ens *e, int reps, double *best_ms)
 {
@@ -832,8 +868,12 @@ int v2c_ens_autotune(v2c_ens *e, int reps, double *best_ms)
         if (rc) return rc;
         double tot = 0;
         for (int r = 0; r < reps; r++) {
-            rc = ens_prepare(e);
+            rc = ens_plan_check(e);
+            if (!rc) rc = ens_props(e);
+            if (!rc) rc = ens_prepare(e);
+            if (e->n_mark) ENSCHK(e, hipStreamWaitEvent(e->stream, e->ev_props, 0));
             if (rc) return rc;
+            ENSCHK(e, hipEventSynchronize(e->ev_props));    // time the graph alone
             ENSCHK(e, hipEventRecord(t0, e->stream));
             ENSCHK(e, hipGraphLaunch(e->exec, e->stream));
             ENSCHK(e, hipEventRecord(t1, e->stream));
