@@ -23,7 +23,10 @@
  * calls exit().  A context is not thread-safe: one host thread per context.
  * Entry points are synchronous on return unless the context was switched to
  * asynchronous mode with v2c_set_async (then only the calls that return a
- * value to the host synchronise).
+ * value to the host synchronise).  A context owns two HIP streams of the
+ * default priority: work that shares no data runs side by side on them
+ * within one call, and every call joins them before it returns or hands the
+ * next call's work to the first.
  */
 #ifndef V2C_H
 #define V2C_H
