@@ -305,3 +305,32 @@ def test_coseismic_rates_adaptive_steps(gpu, markmod, stoksmod):
         ts += dt
     print("case coseismic steps", markmod, stoksmod, cfg.n, flush=True)
     _check(gpu, cfg, ops)
+
+
+def test_async_and_timed_runs_match_sync(gpu):
+    """Two full steps queued without host waits (asynchronous mode, with the
+    per-kernel timing events on both of the context's streams) give the same
+    bits as the synchronous calls: the side-stream work of marker2nodes and
+    nodes2markers is joined before anything that reads it."""
+    cfg = synth.config_grid(201, 101, 200e3, 100e3, 4, refine=True, seed=29)
+    cfg.timesum = DT
+    ops = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m", "move", f"time:{2 * DT}:{DT}",
+           "m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m_move"]
+    ms, ns, _, cs = run_gpu(gpu, cfg, ops)
+    cs.close()
+    from velo2cycles_amd.abi import Context
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
+    ctx.set_async(True)
+    ctx.enable_timing(True)
+    ma, na, _, ca = run_gpu(gpu, cfg, ops, ctx=ctx)
+    ca.synchronize()
+    ma2 = ma.copy()
+    ca.download_markers(ma2)
+    for f in ms.arrays:
+        assert np.array_equal(ms[f], ma2[f], equal_nan=True), f
+    assert np.array_equal(ms.t, ma2.t)
+    for k in NODAL_FIELDS:
+        assert np.array_equal(ns[k], ca.get_nodal(k), equal_nan=True), k
+    ms_props, n_props = ca.kernel_time("m2n_props")
+    assert n_props == 2 and ms_props > 0
+    ca.close()
