@@ -937,19 +937,23 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
     if (p.markmod == 1) {
         allinteri(a, s.x, s.y, vx0, vy0, sp0, ee0, hx, hy);
     } else {
-        double vx1, vy1, sp1, ee1, vx2, vy2, sp2, ee2, vx3, vy3, sp3, ee3, vx4, vy4, sp4, ee4;
-        allinteri(a, s.x, s.y, vx1, vy1, sp1, ee1, hx, hy);
-        allinteri(a, s.x + vx1 * dt / 2.0, s.y + vy1 * dt / 2.0, vx2, vy2, sp2, ee2);
-        allinteri(a, s.x + vx2 * dt / 2.0, s.y + vy2 * dt / 2.0, vx3, vy3, sp3, ee3);
-        allinteri(a, s.x + vx3 * dt, s.y + vy3 * dt, vx4, vy4, sp4, ee4);
-        vx0 = (vx1 + 2.0 * vx2 + 2.0 * vx3 + vx4) / 6.0;
-        vy0 = (vy1 + 2.0 * vy2 + 2.0 * vy3 + vy4) / 6.0;
-        if (p.markmod == 2) {
-            sp0 = (sp1 + 2.0 * sp2 + 2.0 * sp3 + sp4) / 6.0;
-            ee0 = (ee1 + 2.0 * ee2 + 2.0 * ee3 + ee4) / 6.0;
-        } else {
-            sp0 = sp1; ee0 = ee1;
-        }
+        // (k1 + 2 k2 + 2 k3 + k4) / 6 summed stage by stage: the same
+        // left-to-right additions, with one accumulator live instead of
+        // every stage's rates.  The strain-rate invariant is not used.
+        double vx, vy, sp, ee;
+        allinteri(a, s.x, s.y, vx, vy, sp, ee, hx, hy);
+        const double sp1 = sp;
+        double ax = vx, ay = vy, as = sp;
+        allinteri(a, s.x + vx * dt / 2.0, s.y + vy * dt / 2.0, vx, vy, sp, ee);
+        ax = ax + 2.0 * vx; ay = ay + 2.0 * vy; as = as + 2.0 * sp;
+        allinteri(a, s.x + vx * dt / 2.0, s.y + vy * dt / 2.0, vx, vy, sp, ee);
+        ax = ax + 2.0 * vx; ay = ay + 2.0 * vy; as = as + 2.0 * sp;
+        allinteri(a, s.x + vx * dt, s.y + vy * dt, vx, vy, sp, ee);
+        ax = ax + vx; ay = ay + vy; as = as + sp;
+        vx0 = ax / 6.0;
+        vy0 = ay / 6.0;
+        sp0 = p.markmod == 2 ? as / 6.0 : sp1;
+        ee0 = 0.0;
     }
     (void)ee0;
     if (p.outgrid == 2) {
