@@ -801,6 +801,9 @@ struct Mech {
 
 // interpolate_mechanical gathers (interpolation.c:1060-1092, 1211-1237,
 // 1376-1397, 1433-1454) at a clamped position in cell (X.s, Y.s).
+// WITH_E = false gathers only the nine fields the antidiffusion branch of the
+// update reads (the *E fields are left unset).
+template <bool WITH_E>
 __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, const Axis4 &X, const Axis4 &Y,
                                             Mech &o)
 {
@@ -810,30 +813,34 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
     W4 w = wts(S);
     uint32_t m3 = S.m3;
     o.EXY = bil(NOD(a, V2C_N_exy), m3, ynumy, w);
-    o.EXYE = bil(NOD(a, V2C_N_exye), m3, ynumy, w);
     o.SXY = bil(NOD(a, V2C_N_sxy), m3, ynumy, w);
-    o.SXYE = bil(NOD(a, V2C_N_sxye), m3, ynumy, w);
     o.ES = bil(NOD(a, V2C_N_pes), m3, ynumy, w);
-    o.ESE = bil(NOD(a, V2C_N_pe), m3, ynumy, w);
-    o.PHISE = bil(NOD(a, V2C_N_phis), m3, ynumy, w);
     o.PHIS = bil(NOD(a, V2C_N_phis_it), m3, ynumy, w);
+    if (WITH_E) {
+        o.EXYE = bil(NOD(a, V2C_N_exye), m3, ynumy, w);
+        o.SXYE = bil(NOD(a, V2C_N_sxye), m3, ynumy, w);
+        o.ESE = bil(NOD(a, V2C_N_pe), m3, ynumy, w);
+        o.PHISE = bil(NOD(a, V2C_N_phis), m3, ynumy, w);
+    }
     S = centre_stencil(g, X, Y, x, y);
     w = wts(S);
     m3 = S.m3;
     o.EXX = bil(NOD(a, V2C_N_exx), m3, ynumy, w);
-    o.EXXE = bil(NOD(a, V2C_N_exxe), m3, ynumy, w);
     o.SXX = bil(NOD(a, V2C_N_sxx), m3, ynumy, w);
-    o.SXXE = bil(NOD(a, V2C_N_sxxe), m3, ynumy, w);
     o.PR = bil(NOD(a, V2C_N_pr), m3, ynumy, w);
-    o.SPPE = bil(NOD(a, V2C_N_sppe), m3, ynumy, w);
+    if (WITH_E) {
+        o.EXXE = bil(NOD(a, V2C_N_exxe), m3, ynumy, w);
+        o.SXXE = bil(NOD(a, V2C_N_sxxe), m3, ynumy, w);
+        o.SPPE = bil(NOD(a, V2C_N_sppe), m3, ynumy, w);
+    }
     S = vx_stencil(g, X, Y, x, y);
     w = wts(S);
     o.VX = bil(NOD(a, V2C_N_vx), S.m3, ynumy, w);
-    o.MVX = bil(NOD(a, V2C_N_mvx), S.m3, ynumy, w);
+    if (WITH_E) o.MVX = bil(NOD(a, V2C_N_mvx), S.m3, ynumy, w);
     S = vy_stencil(g, X, Y, x, y);
     w = wts(S);
     o.VY = bil(NOD(a, V2C_N_vy), S.m3, ynumy, w);
-    o.MVY = bil(NOD(a, V2C_N_mvy), S.m3, ynumy, w);
+    if (WITH_E) o.MVY = bil(NOD(a, V2C_N_mvy), S.m3, ynumy, w);
 }
 
 // Antidiffusion scatter-back contributions of one marker (interpolation.c:
@@ -1049,7 +1056,9 @@ __device__ __forceinline__ bool n2m_update_one(const KArgs &a, long m)
     }
     {
         Mech mv;
-        mech_gather(a, clampd(s.x, 0, g.xsize), clampd(s.y, 0, g.ysize), X, Y, mv);
+        const double cxp = clampd(s.x, 0, g.xsize), cyp = clampd(s.y, 0, g.ysize);
+        if (anti) mech_gather<false>(a, cxp, cyp, X, Y, mv);
+        else mech_gather<true>(a, cxp, cyp, X, Y, mv);
         n2m_update(a, t, s, mv);
     }
     if (s.dirty_state) {
