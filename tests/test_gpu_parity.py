@@ -47,9 +47,13 @@ def test_edge_full_step(gpu, timesum):
     _check(gpu, cfg, STEP)
 
 
-def test_edge_fused_n2m_advect(gpu):
+@pytest.mark.parametrize("antidiffinter", [1, 0])
+def test_edge_fused_n2m_advect(gpu, antidiffinter):
+    """The fused update gathers 9 nodal interpolants with antidiffusion and
+    18 without; both against the oracle."""
     cfg = synth.config_edge()
     cfg.timesum = DT
+    cfg.params.antidiffinter = antidiffinter
     ops = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m_move"]
     _check(gpu, cfg, ops)
 
