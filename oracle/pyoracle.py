@@ -42,6 +42,8 @@ def lib():
         L.or_nxsearch.restype = C.c_long
         L.or_nysearch.argtypes = [P(Grid), C.c_double]
         L.or_nysearch.restype = C.c_long
+        L.or_cells.argtypes = [P(Grid), C.c_long, P(C.c_double), P(C.c_double), P(C.c_long), P(C.c_long)]
+        L.or_cells.restype = None
         L.or_allinterpomp.argtypes = [P(OrState), C.c_double, C.c_double]
         L.or_allinterpomp.restype = C.c_double
         L.or_ductilevisc.argtypes = [P(OrState), C.c_int] + [C.c_double] * 6 + [P(C.c_int)]
@@ -190,6 +192,16 @@ class Oracle:
 
     def nysearch(self, y):
         return lib().or_nysearch(C.byref(self.grid.struct), y)
+
+    def cells(self, x, y):
+        """(nxsearch(x_i), nysearch(y_i)) of every point (interpolation.c:1536-1573)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        nx = np.zeros(len(x), np.int64)
+        ny = np.zeros(len(x), np.int64)
+        lib().or_cells(C.byref(self.grid.struct), len(x), _dptr(x), _dptr(y),
+                       nx.ctypes.data_as(C.POINTER(C.c_long)), ny.ctypes.data_as(C.POINTER(C.c_long)))
+        return nx, ny
 
 
 MX_FIELDS = ("V", "phi", "a", "b", "L", "V0", "mu0", "eta", "eta_vp", "sy", "geta", "gV",

@@ -1,12 +1,12 @@
 """Run the reference's own hot-path routines (oracle/_ref/ref_driver, compiled
 in place from /root/reference/src by oracle/Makefile) on a Config.
 
-TEST INFRASTRUCTURE ONLY: writes golden vectors and pins the oracle where the
-reference sources are mounted (callers check `available()`).  Both drivers
-are built binaries under oracle/_ref and need no sources at run time:
-`built(DROPIN)` gates the drop-in check, which runs the same ops through the
-reference-side binding and libv2c on a GPU (ref_driver.c built with
--DV2C_DROPIN) beside the plain reference (`built(DRIVER)`).
+TEST INFRASTRUCTURE ONLY, and only in the container that holds the reference
+sources: it writes golden vectors (tests/golden/) and pins the oracle.  The
+binaries under oracle/_ref never leave that container.  The same input/output
+record format drives tests/native/dropin_harness (the builder's stand-in for
+the reference program around the reference-side binding), which is what runs
+on a GPU: `write_input`, `run_file` and `split_output` serve both.
 """
 from __future__ import annotations
 
@@ -21,6 +21,8 @@ from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(ORACLE_DIR, "_ref", "ref_driver")
+# the binding compiled into the reference translation unit (a link check of
+# the binding's declarations against the reference globals; not run)
 DROPIN = os.path.join(ORACLE_DIR, "_ref", "dropin_driver")
 REF_SRC = "/root/reference/src"
 
@@ -77,6 +79,64 @@ def _read(path):
     return out
 
 
+def write_input(path, cfg, ops=None, markers=None, nodal=None, timesum=None, timestep=None, extra=None):
+    """Write cfg (+ overrides and `extra` records) as a driver input file."""
+    m = markers or cfg.markers
+    nod = nodal or cfg.nodal
+    p = cfg.params
+    g = cfg.grid
+    with open(path, "wb") as f:
+        _rec(f, "gx", "d", g.gx)
+        _rec(f, "gy", "d", g.gy)
+        _rec(f, "xnumx", "l", np.array([g.xnumx], np.int64))
+        _rec(f, "ynumy", "l", np.array([g.ynumy], np.int64))
+        _rec(f, "marknum", "l", np.array([m.n], np.int64))
+        _rec(f, "xsize", "d", np.array([g.xsize]))
+        _rec(f, "ysize", "d", np.array([g.ysize]))
+        _rec(f, "timesum", "d", np.array([cfg.timesum if timesum is None else timesum]))
+        _rec(f, "timestep", "d", np.array([cfg.timestep if timestep is None else timestep]))
+        for k in _PARAM_INT:
+            _rec(f, k, "i", np.array([getattr(p, k)], np.int32))
+        for k in _PARAM_DBL:
+            _rec(f, k, "d", np.array([getattr(p, k)]))
+        eps = np.zeros(100)
+        eps[2] = p.incoming_T
+        _rec(f, "eps", "d", eps)
+        for k in _MAT:
+            _rec(f, "mark" + k, "d", np.array(getattr(cfg.mats, k)[:200]))
+        _rec(f, "markim", "i", np.array(cfg.mats.im[:200], np.int32))
+        for fld in MARKER_FIELDS:
+            _rec(f, _MARK_NAME[fld], "d", m[fld])
+        _rec(f, "markt", "c", m.t)
+        for k in NODAL_FIELDS:
+            _rec(f, k, "d", nod[k])
+        for k, (kind, arr) in (extra or {}).items():
+            _rec(f, k, kind, arr)
+
+
+def run_file(driver, fin, ops, threads=1):
+    """Run `driver IN OUT ops...` on an input file; returns the raw output
+    records (name -> array)."""
+    with tempfile.TemporaryDirectory() as td:
+        fout = os.path.join(td, "out.bin")
+        # one thread by default: the reference's merge order
+        # (interpolation.c:631-674) and its racy antidiffusion scatter (:942)
+        # make multithreaded runs vary (threads > 1 is for timing only)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        with open(os.devnull, "w") as dn:
+            subprocess.run([driver, os.path.abspath(fin), fout] + list(ops), check=True, stdout=dn, env=env,
+                           cwd=td)
+        return _read(fout)
+
+
+def split_output(raw, n, nodenum):
+    """(markers dict by library field names, nodal dict by reference names)."""
+    mk = {fld: raw[_MARK_NAME[fld]][:n].copy() for fld in MARKER_FIELDS}
+    mk["t"] = raw["markt"][:n].copy()
+    nd = {k: raw[k][:nodenum].copy() for k in NODAL_FIELDS}
+    return mk, nd
+
+
 def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=None, driver=DRIVER,
         threads=1):
     """Write cfg (+ overrides) into the reference globals, run `ops`, return
@@ -85,49 +145,11 @@ def run(cfg, ops, markers=None, nodal=None, timesum=None, timestep=None, extra=N
     assert built(driver) or (driver == DRIVER and available()), "reference driver unavailable"
     assert legal(cfg), "size exceeds the reference capacity macros"
     m = markers or cfg.markers
-    nod = nodal or cfg.nodal
-    p = cfg.params
-    g = cfg.grid
     with tempfile.TemporaryDirectory() as td:
-        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        with open(fin, "wb") as f:
-            _rec(f, "gx", "d", g.gx)
-            _rec(f, "gy", "d", g.gy)
-            _rec(f, "xnumx", "l", np.array([g.xnumx], np.int64))
-            _rec(f, "ynumy", "l", np.array([g.ynumy], np.int64))
-            _rec(f, "marknum", "l", np.array([m.n], np.int64))
-            _rec(f, "xsize", "d", np.array([g.xsize]))
-            _rec(f, "ysize", "d", np.array([g.ysize]))
-            _rec(f, "timesum", "d", np.array([cfg.timesum if timesum is None else timesum]))
-            _rec(f, "timestep", "d", np.array([cfg.timestep if timestep is None else timestep]))
-            for k in _PARAM_INT:
-                _rec(f, k, "i", np.array([getattr(p, k)], np.int32))
-            for k in _PARAM_DBL:
-                _rec(f, k, "d", np.array([getattr(p, k)]))
-            eps = np.zeros(100)
-            eps[2] = p.incoming_T
-            _rec(f, "eps", "d", eps)
-            for k in _MAT:
-                _rec(f, "mark" + k, "d", np.array(getattr(cfg.mats, k)[:200]))
-            _rec(f, "markim", "i", np.array(cfg.mats.im[:200], np.int32))
-            for fld in MARKER_FIELDS:
-                _rec(f, _MARK_NAME[fld], "d", m[fld])
-            _rec(f, "markt", "c", m.t)
-            for k in NODAL_FIELDS:
-                _rec(f, k, "d", nod[k])
-            for k, (kind, arr) in (extra or {}).items():
-                _rec(f, k, kind, arr)
-        # one thread by default: the reference's merge order
-        # (interpolation.c:631-674) and its racy antidiffusion scatter (:942)
-        # make multithreaded runs vary (threads > 1 is for timing only)
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        with open(os.devnull, "w") as dn:
-            subprocess.run([driver, fin, fout] + list(ops), check=True, stdout=dn, env=env, cwd=td)
-        raw = _read(fout)
-    nn = g.nodenum
-    mk = {fld: raw[_MARK_NAME[fld]][:m.n].copy() for fld in MARKER_FIELDS}
-    mk["t"] = raw["markt"][:m.n].copy()
-    nd = {k: raw[k][:nn].copy() for k in NODAL_FIELDS}
+        fin = os.path.join(td, "in.bin")
+        write_input(fin, cfg, markers=markers, nodal=nodal, timesum=timesum, timestep=timestep, extra=extra)
+        raw = run_file(driver, fin, ops, threads=threads)
+    mk, nd = split_output(raw, m.n, cfg.grid.nodenum)
     return mk, nd, raw
 
 

@@ -37,6 +37,14 @@ static long bisect_left(const double *g, long n, double x)
 }
 long or_nxsearch(const v2c_grid *g, double x) { return bisect_left(g->gx, g->xnumx, x); }
 long or_nysearch(const v2c_grid *g, double y) { return bisect_left(g->gy, g->ynumy, y); }
+/* nxsearch / nysearch of n points (for array checks of cell indices). */
+void or_cells(const v2c_grid *g, long n, const double *x, const double *y, long *nx, long *ny)
+{
+    for (long i = 0; i < n; i++) {
+        nx[i] = or_nxsearch(g, x[i]);
+        ny[i] = or_nysearch(g, y[i]);
+    }
+}
 
 static inline double clampd(double v, double lo, double hi)
 {

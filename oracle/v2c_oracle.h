@@ -30,6 +30,7 @@ typedef struct or_state {
 
 long or_nxsearch(const v2c_grid *g, double x);
 long or_nysearch(const v2c_grid *g, double y);
+void or_cells(const v2c_grid *g, long n, const double *x, const double *y, long *nx, long *ny);
 double or_allinterpomp(const or_state *s, double x, double y);
 double or_ductilevisc(const or_state *s, int type, double sxy, double sxx,
                       double exy, double exx, double tk, double pb, int *iters);

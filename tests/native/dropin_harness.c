@@ -1,0 +1,260 @@
+/*
+ * dropin_harness.c -- a C host for the reference-side binding
+ * (velo2cycles_amd/csrc/dropin/v2c_dropin.c) that stands in for the
+ * reference program: it defines, with the reference's types, every global the
+ * binding declares extern (headevps.c:33-250, input_RSF.c:3-11), fills them
+ * from an input file, calls the binding's routines by the reference names in
+ * the order given on the command line, and writes the globals back.  No
+ * reference source is compiled into it; the binding and libv2c do the work.
+ *
+ * TEST INFRASTRUCTURE: the drop-in tests run it on the GPU and compare its
+ * output with fixtures the reference wrote (tests/golden/dropin_*.npz).
+ *
+ * Protocol (the one oracle/refrun.py writes and reads):
+ *   dropin_harness IN OUT op...
+ * IN and OUT are sequences of records
+ *   char name[32]; char kind ('d' double, 'l' long, 'i' int, 'c' char);
+ *   int64 count; payload
+ * Ops: m2n | n2m | move | before:<dt>:<rk> | after:<dt>:<rk> |
+ *      stress:<dt>:<rk> | c2b:<dt> | c2bold:<dt> | b2c | copy:<rk> |
+ *      tstep:<oldornew> | bcvxvy | bcp | bctk | heat:<heatdif>
+ * before / after / tstep leave their return value in drv_ret[0] / [1] / [2].
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/v2c.h"
+
+/* capacities of this host (the binding takes sizes from xnumx, ynumy, marknum) */
+#define H_XLN 4096
+#define H_NOD (1L << 18)
+#define H_MRK (1L << 21)
+#define H_TMR 256
+#define H_BON (1L << 16)
+
+/* ---- the globals the binding declares ------------------------------------- */
+long int xnumx, ynumy, marknum, nodenum;
+double xsize, ysize, xstpx, timesum, timestep, errcheck, nstress, Dphis;
+double gx[H_XLN], gy[H_XLN];
+char markt[H_MRK];
+double markx[H_MRK], marky[H_MRK], markk[H_MRK], markp[H_MRK], markxx[H_MRK], markxy[H_MRK];
+double markexx[H_MRK], markexy[H_MRK], marke[H_MRK], markphi[H_MRK], markvx[H_MRK], markvy[H_MRK];
+double markC[H_MRK], markd[H_MRK], markw[H_MRK], mark_a[H_MRK], mark_b[H_MRK], mark_L[H_MRK];
+double mark_L0[H_MRK], mark_V0[H_MRK], fric_static[H_MRK], markC0[H_MRK];
+double marknu[H_TMR], markdh[H_TMR], markdv[H_TMR], markss[H_TMR], markmm[H_TMR], markgg[H_TMR];
+double markn0[H_TMR], markn1[H_TMR], markro[H_TMR], markbb[H_TMR], markaa[H_TMR], markcp[H_TMR];
+double markkt[H_TMR], markkf[H_TMR], markkp[H_TMR], markht[H_TMR];
+int markim[H_TMR];
+int markmod, stoksmod, outgrid, densimod, interpolationmod, antidiffinter, rkmod, RSFmode;
+int power_law_RSF, reloadfromDP, compressiblemode, stickymod, inertyn, pmode, tempmod;
+double nubeg, nuend, tdeep, zdeep, eps[100];
+double half_range, start_updip, end_updip, start_downdip, end_downdip;
+double a_vw, b_vw, L_vw, V0_vw, a_vs, b_vs, L_vs, V0_vs, a_hr, b_hr, L_hr, V0_hr;
+double RK_sliprs[H_NOD][2], RK_phidts[H_NOD][2];
+double pr[H_NOD], vx[H_NOD], vy[H_NOD], exx[H_NOD], exy[H_NOD], esp[H_NOD], sxx[H_NOD], sxy[H_NOD];
+double sxxs[H_NOD], sppes[H_NOD], exxpl[H_NOD], exxel[H_NOD];
+double bm[H_NOD], ro[H_NOD], et[H_NOD], nu[H_NOD], nd[H_NOD], gg[H_NOD], gd[H_NOD], sppe[H_NOD];
+double phis[H_NOD], phin[H_NOD], sxyen[H_NOD], sxxen[H_NOD], exyen[H_NOD], exxen[H_NOD];
+double sppen[H_NOD], mus_s[H_NOD], as[H_NOD], bs[H_NOD], Ls[H_NOD], V0s[H_NOD], sxxe[H_NOD];
+double sxye[H_NOD], exxe[H_NOD], exye[H_NOD], cohesion[H_NOD], dro[H_NOD], drp[H_NOD], cp[H_NOD];
+double kt[H_NOD], ht[H_NOD], tk[H_NOD], mrx[H_NOD], mry[H_NOD], mvx[H_NOD], mvy[H_NOD];
+double pe[H_NOD], pen[H_NOD], nu_m2c[H_NOD], nd_m2c[H_NOD], mvxn[H_NOD], mvyn[H_NOD];
+double phie[H_NOD], sxyee[H_NOD], exyee[H_NOD], pee[H_NOD], sppee[H_NOD], sxxee[H_NOD];
+double exxee[H_NOD], mvxe[H_NOD], mvye[H_NOD];
+double wtphisum[H_NOD], phisum[H_NOD], sxysum[H_NOD], exysum[H_NOD], esum[H_NOD], wtsxxsum[H_NOD];
+double psum[H_NOD], sxxsum[H_NOD], exxsum[H_NOD], wtvxsum[H_NOD], vxsum[H_NOD], wtvysum[H_NOD];
+double vysum[H_NOD], mrx0[H_NOD], mry0[H_NOD];
+double pes[H_NOD], phis_it[H_NOD], syields[H_NOD], nuvp_it[H_NOD], sliprs[H_NOD];
+double RK_sliprs_final[H_NOD], RK_phidts_final[H_NOD], RK_sliprs_solved[H_NOD], RK_phidts_solved[H_NOD];
+double RK_deltas[H_NOD];
+double deltasxx[H_NOD], deltasxy[H_NOD], deltaVs[H_NOD], exxs[H_NOD], exxes[H_NOD], sxxes[H_NOD];
+double prs[H_NOD], tk2[H_NOD], tol0[H_NOD], tol1[H_NOD];
+long int bondm[4 * H_NOD], bondn[H_BON][3], bondnum;
+double bondv[H_BON][4];
+double maxvel, maxdphi, maxnvp, dtL, dtphi, dtd, dtnvp, maxtmstep, tcnvp, tcd, tcphih, dtfactor;
+
+/* the binding's routines (v2c_dropin.c) */
+int v2c_dropin_attach(long capacity, int device);
+void v2c_dropin_markers_to_host(void);
+void v2c_dropin_detach(void);
+void v2c_dropin_temperature_markers(double heatdif);
+void marker2nodes(void);
+double nodes2markers(void);
+void moveandrotatemarkers(void);
+double compute_before_solve(double dt, int rk);
+double compute_after_solve(double dt, int rk);
+int recompute_stress_strainrate(double dt, int rk);
+double interpolate_center2basic(double dt);
+double interpolate_center2basic_old(double dt);
+double interpolate_basic2center(void);
+void copyarrays(int rk);
+double calc_timestep(int oldornew);
+void vxvy_boundarycorrection(void);
+void P_boundarycorrection(void);
+void tkrecalc(void);
+
+/* host-side columns of RK_sliprs / RK_phidts, as the input file names them */
+static double RK_sliprs0[H_NOD], RK_sliprs1[H_NOD], RK_phidts0[H_NOD], RK_phidts1[H_NOD];
+static double drv_ret[8];
+
+typedef struct { const char *name; char kind; void *ptr; long cap; } entry;
+#define D(a) {#a, 'd', (void *)(a), (long)(sizeof(a) / sizeof(double))}
+#define S(a) {#a, 'd', (void *)&(a), 1}
+#define L(a) {#a, 'l', (void *)&(a), 1}
+#define I(a) {#a, 'i', (void *)&(a), 1}
+static const entry table[] = {
+    D(gx), D(gy), L(xnumx), L(ynumy), L(marknum), S(xsize), S(ysize), S(timesum), S(timestep),
+    I(markmod), I(stoksmod), I(outgrid), I(densimod), I(interpolationmod), I(antidiffinter),
+    I(rkmod), I(RSFmode), I(power_law_RSF), I(reloadfromDP), I(compressiblemode), I(stickymod),
+    I(inertyn), I(pmode), I(tempmod), S(nubeg), S(nuend), S(tdeep), S(zdeep), S(nstress), S(Dphis),
+    S(half_range), S(start_updip), S(end_updip), S(start_downdip), S(end_downdip),
+    S(a_vw), S(b_vw), S(L_vw), S(V0_vw), S(a_vs), S(b_vs), S(L_vs), S(V0_vs),
+    S(a_hr), S(b_hr), S(L_hr), S(V0_hr), S(errcheck), D(eps),
+    D(marknu), D(markdh), D(markdv), D(markss), D(markmm), D(markgg), D(markn0), D(markn1),
+    D(markro), D(markbb), D(markaa), D(markcp), D(markkt), D(markkf), D(markkp), D(markht),
+    {"markim", 'i', (void *)markim, H_TMR},
+    D(markx), D(marky), D(markk), D(markp), D(markxx), D(markxy), D(markexx), D(markexy),
+    D(marke), D(markphi), D(markvx), D(markvy), D(markC), D(markd), D(markw), D(mark_a),
+    D(mark_b), D(mark_L), D(mark_L0), D(mark_V0), D(fric_static), D(markC0),
+    {"markt", 'c', (void *)markt, H_MRK},
+    D(pr), D(vx), D(vy), D(exx), D(exy), D(esp), D(sxx), D(sxy), D(sxxs), D(sppes), D(exxpl), D(exxel),
+    D(bm), D(ro), D(et), D(nu), D(nd), D(gg), D(gd), D(sppe), D(phis), D(phin), D(sxyen), D(sxxen),
+    D(exyen), D(exxen), D(sppen), D(mus_s), D(as), D(bs), D(Ls), D(V0s), D(sxxe), D(sxye), D(exxe),
+    D(exye), D(cohesion), D(dro), D(drp), D(cp), D(kt), D(ht), D(tk), D(mrx), D(mry), D(mvx), D(mvy),
+    D(pe), D(pen), D(nu_m2c), D(nd_m2c), D(mvxn), D(mvyn),
+    D(phie), D(sxyee), D(exyee), D(pee), D(sppee), D(sxxee), D(exxee), D(mvxe), D(mvye),
+    D(wtphisum), D(phisum), D(sxysum), D(exysum), D(esum), D(wtsxxsum), D(psum), D(sxxsum),
+    D(exxsum), D(wtvxsum), D(vxsum), D(wtvysum), D(vysum), D(mrx0), D(mry0),
+    D(pes), D(phis_it), D(syields), D(nuvp_it), D(sliprs), D(RK_sliprs_final), D(RK_phidts_final),
+    D(RK_sliprs_solved), D(RK_phidts_solved), D(RK_deltas),
+    D(RK_sliprs0), D(RK_sliprs1), D(RK_phidts0), D(RK_phidts1),
+    D(deltasxx), D(deltasxy), D(deltaVs), D(exxs), D(exxes), D(sxxes), D(prs), D(tk2), D(tol0), D(tol1),
+    S(maxvel), S(maxdphi), S(maxnvp), S(dtL), S(dtphi), S(dtd), S(dtnvp),
+    S(maxtmstep), S(tcnvp), S(tcd), S(tcphih), S(dtfactor),
+    {"bondm", 'l', (void *)bondm, 4 * H_NOD}, {"bondn", 'l', (void *)bondn, 3 * H_BON},
+    {"bondv", 'd', (void *)bondv, 4 * H_BON}, L(bondnum),
+    D(drv_ret),
+};
+#define NTAB ((long)(sizeof table / sizeof table[0]))
+
+static size_t ksize(char k) { return k == 'd' ? 8 : k == 'l' ? 8 : k == 'i' ? 4 : 1; }
+
+/* Records naming a global this host does not have (the reference's own .prn
+ * header fields, say) are skipped. */
+static int read_input(const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) { perror(path); return -1; }
+    for (;;) {
+        char name[32]; char kind; long long cnt;
+        if (fread(name, 1, 32, f) != 32) break;
+        if (fread(&kind, 1, 1, f) != 1 || fread(&cnt, 8, 1, f) != 1 || cnt < 0) { fclose(f); return -1; }
+        name[31] = 0;
+        long i;
+        for (i = 0; i < NTAB; i++) if (!strcmp(table[i].name, name)) break;
+        if (i == NTAB) {
+            if (fseek(f, (long)(cnt * (long long)ksize(kind)), SEEK_CUR)) { fclose(f); return -1; }
+            continue;
+        }
+        if (table[i].kind != kind || cnt > table[i].cap) {
+            fprintf(stderr, "dropin_harness: bad record %s kind %c count %lld\n", name, kind, cnt);
+            fclose(f); return -1;
+        }
+        if (fread(table[i].ptr, ksize(kind), (size_t)cnt, f) != (size_t)cnt) { fclose(f); return -1; }
+    }
+    fclose(f);
+    return 0;
+}
+
+static long used_count(const entry *e)
+{
+    if (e->cap == 1) return 1;
+    if (!strcmp(e->name, "bondm")) return 4 * nodenum;
+    if (!strcmp(e->name, "bondn")) return 3 * bondnum;
+    if (!strcmp(e->name, "bondv")) return 4 * bondnum;
+    if (!strcmp(e->name, "gx")) return xnumx;
+    if (!strcmp(e->name, "gy")) return ynumy;
+    if (e->cap == H_MRK) return marknum;
+    if (e->cap == H_TMR) return H_TMR;
+    if (e->cap == H_NOD) return nodenum;
+    return e->cap;
+}
+
+static int write_output(const char *path)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) { perror(path); return -1; }
+    for (long i = 0; i < NTAB; i++) {
+        char name[32] = {0};
+        strncpy(name, table[i].name, 31);
+        long long cnt = used_count(&table[i]);
+        fwrite(name, 1, 32, f);
+        fwrite(&table[i].kind, 1, 1, f);
+        fwrite(&cnt, 8, 1, f);
+        fwrite(table[i].ptr, ksize(table[i].kind), (size_t)cnt, f);
+    }
+    return fclose(f) ? -1 : 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) { fprintf(stderr, "usage: dropin_harness IN OUT op...\n"); return 2; }
+    /* defaults the reference reads from its set-up (input_RSF.c:13-63) */
+    {
+        v2c_params p;
+        v2c_params_default(&p);
+        nstress = p.nstress; half_range = p.half_range;
+        start_updip = p.start_updip; end_updip = p.end_updip;
+        start_downdip = p.start_downdip; end_downdip = p.end_downdip;
+        a_vw = p.a_vw; b_vw = p.b_vw; L_vw = p.L_vw; V0_vw = p.V0_vw;
+        a_vs = p.a_vs; b_vs = p.b_vs; L_vs = p.L_vs; V0_vs = p.V0_vs;
+        a_hr = p.a_hr; b_hr = p.b_hr; L_hr = p.L_hr; V0_hr = p.V0_hr;
+        v2c_timestep_params tp;
+        v2c_timestep_params_default(&tp);
+        maxtmstep = tp.maxtmstep; tcnvp = tp.tcnvp; tcd = tp.tcd; tcphih = tp.tcphih;
+        dtfactor = tp.dtfactor;
+    }
+    if (read_input(argv[1])) return 3;
+    if (xnumx < 4 || ynumy < 4 || xnumx > H_XLN || ynumy > H_XLN || xnumx * ynumy > H_NOD ||
+        marknum < 0 || marknum > H_MRK || bondnum < 0 || bondnum > H_BON) {
+        fprintf(stderr, "dropin_harness: sizes above this host's capacity\n");
+        return 3;
+    }
+    nodenum = xnumx * ynumy;
+    xstpx = xsize / (double)(xnumx - 1);
+    for (long i = 0; i < nodenum; i++) {
+        RK_sliprs[i][0] = RK_sliprs0[i]; RK_sliprs[i][1] = RK_sliprs1[i];
+        RK_phidts[i][0] = RK_phidts0[i]; RK_phidts[i][1] = RK_phidts1[i];
+    }
+    const int rc = v2c_dropin_attach(marknum > 0 ? marknum : 1, 0);
+    if (rc) { fprintf(stderr, "dropin_harness: v2c_dropin_attach failed (%d)\n", rc); return 6; }
+    for (int a = 3; a < argc; a++) {
+        const char *op = argv[a];
+        double dt; int rk;
+        if (!strcmp(op, "m2n")) marker2nodes();
+        else if (!strcmp(op, "n2m")) nodes2markers();
+        else if (!strcmp(op, "move")) moveandrotatemarkers();
+        else if (sscanf(op, "before:%lf:%d", &dt, &rk) == 2) drv_ret[0] = compute_before_solve(dt, rk);
+        else if (sscanf(op, "after:%lf:%d", &dt, &rk) == 2) drv_ret[1] = compute_after_solve(dt, rk);
+        else if (sscanf(op, "stress:%lf:%d", &dt, &rk) == 2) recompute_stress_strainrate(dt, rk);
+        else if (sscanf(op, "c2bold:%lf", &dt) == 1) interpolate_center2basic_old(dt);
+        else if (sscanf(op, "c2b:%lf", &dt) == 1) interpolate_center2basic(dt);
+        else if (!strcmp(op, "b2c")) interpolate_basic2center();
+        else if (sscanf(op, "copy:%d", &rk) == 1) copyarrays(rk);
+        else if (sscanf(op, "tstep:%d", &rk) == 1) drv_ret[2] = calc_timestep(rk);
+        else if (!strcmp(op, "bcvxvy")) vxvy_boundarycorrection();
+        else if (!strcmp(op, "bcp")) P_boundarycorrection();
+        else if (!strcmp(op, "bctk")) tkrecalc();
+        else if (sscanf(op, "heat:%lf", &dt) == 1) v2c_dropin_temperature_markers(dt);
+        else { fprintf(stderr, "dropin_harness: unknown op %s\n", op); v2c_dropin_detach(); return 4; }
+    }
+    v2c_dropin_markers_to_host();
+    v2c_dropin_detach();
+    if (errcheck) fprintf(stderr, "dropin_harness: errcheck set by the binding\n");
+    for (long i = 0; i < nodenum; i++) {
+        RK_sliprs0[i] = RK_sliprs[i][0]; RK_sliprs1[i] = RK_sliprs[i][1];
+        RK_phidts0[i] = RK_phidts[i][0]; RK_phidts1[i] = RK_phidts[i][1];
+    }
+    return write_output(argv[2]) ? 5 : 0;
+}

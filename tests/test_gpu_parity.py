@@ -183,73 +183,89 @@ def test_capacity_error(gpu):
     ctx.close()
 
 
-def test_production_size_properties(gpu):
-    """1501x301 refined grid, 36 markers per cell (~16.2 M markers).
-
-    Size-independent checks at full size: (1) marker-local results of the fused
-    nodes2markers + advection equal the oracle's on a random subset of markers
-    (each marker's update depends only on its own state and the nodal fields);
-    (2) two runs are bitwise identical; (3) the m2n weight sums are consistent:
-    every marker strictly inside contributes one centre weight, so the sum over
-    centre nodes of wtsxx-type weights equals the oracle's per-marker sum."""
-    from velo2cycles_amd.abi import Context, MarkerArrays
+def test_production_full_step_parity(gpu):
+    """The production workload itself: 1501x301 nodes refined to 500 m, 36
+    markers per cell (16.2 M markers), one full step (marker2nodes,
+    compute_before_solve, compute_after_solve, nodes2markers + advection)
+    against the oracle on every marker: positions and types bit-exact, every
+    FP64 marker and nodal field at the 1e-12 bar; then a second device run of
+    marker2nodes gives the same bits."""
+    from velo2cycles_amd.abi import Context
     cfg = synth.config_production()
     cfg.timesum = DT
-    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
-    ctx.set_time(cfg.timesum, cfg.timestep)
-    ctx.set_all_nodal(cfg.nodal)
-    ctx.upload_markers(cfg.markers)
-    ctx.marker2nodes()
-    ctx.compute_before_solve(DT, 0)
-    ctx.compute_after_solve(DT, 0)
-    nod = ctx.get_all_nodal()
-    ctx.nodes2markers_advect()
-    m = cfg.markers.copy()
-    ctx.download_markers(m)
-    # (1) marker-local parity on a subset, using the GPU's own nodal state
-    rng = np.random.default_rng(11)
-    sub = np.sort(rng.choice(cfg.n, 200_000, replace=False))
-    fields = {f: cfg.markers[f][sub] for f in cfg.markers.arrays}
-    fields["t"] = cfg.markers.t[sub]
-    sm = MarkerArrays(len(sub), fields)
-    # phi after m2n is marker2nodes' output (friction reset); take it from a
-    # rerun of m2n through the oracle's marker-local rule: the band reset only
-    # depends on the marker itself
-    nod_in = dict(nod)
-    cfg_sub = synth.Config(cfg.name, cfg.grid, cfg.params, cfg.mats, sm, nod_in, cfg.timestep,
-                           cfg.timesum)
-    cfg_sub.params.n2m_phantom = 0
-    om, _, _ = run_oracle(cfg_sub, ["n2m", "move"])
-    for f in ("x", "y"):
-        assert np.array_equal(m[f][sub], om[f]), f
-    for f in ("xx", "xy", "exx", "exy", "p", "vx", "vy", "e", "k"):
-        a, b = m[f][sub], om[f]
-        scale = np.maximum(np.abs(b), np.sqrt(np.mean(b * b)))
-        assert np.max(np.abs(a - b) / scale) <= 1e-12, f
-    # (2) determinism at full size
+    print("case production full step", cfg.n, flush=True)
+    ops = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m", "move"]
+    om, on, oret = run_oracle(cfg, ops)
+    gm, gn, gret, ctx = run_gpu(gpu, cfg, ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m_move"])
+    ctx.close()
+    bad = compare(gm, om, gn, on)
+    assert not bad, "\n".join(bad)
+    assert gret["before"] == oret["before"]
+    (d_g, e_g), (d_o, e_o) = gret["after"], oret["after"]
+    assert e_g == e_o and abs(d_g - d_o) <= 1e-12 * abs(d_o)
+    del om, on, gm
+    # determinism at full size
     ctx2 = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
     ctx2.set_time(cfg.timesum, cfg.timestep)
     ctx2.set_all_nodal(cfg.nodal)
     ctx2.upload_markers(cfg.markers)
     ctx2.marker2nodes()
-    for k in ("nu_m2c", "sxye", "sxxe", "mvx", "mvy", "phis", "pe", "tk"):
-        assert np.array_equal(ctx2.get_nodal(k), nod[k], equal_nan=True), k
-    # (3) centre weights: sum over nodes equals the sum over markers inside
-    g = cfg.grid
-    x, y = cfg.markers["x"], cfg.markers["y"]
-    inside = (x > 0) & (y > 0) & (x < g.xsize) & (y < g.ysize)
-    nx = np.clip(np.searchsorted(g.gx, x, side="right") - 1, 0, g.xnumx - 2)
-    ny = np.clip(np.searchsorted(g.gy, y, side="right") - 1, 0, g.ynumy - 2)
-    cx = (x - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx])
-    cy = (y - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny])
-    swt1 = 1.0 / (g.gx[nx + 1] - g.gx[nx]) / (g.gy[ny + 1] - g.gy[ny])
-    cw = swt1 * (1 - 2 * np.abs(cx - 0.5)) * (1 - 2 * np.abs(cy - 0.5))
-    # bm = sum(50e9 w)/sum(w) = 50e9 wherever a centre node received weight
-    bm = nod["bm"].reshape(g.xnumx, g.ynumy)
-    assert np.allclose(bm[bm != 0], cfg.params.bulk_modulus, rtol=1e-14, atol=0)
-    assert cw[inside].sum() > 0
-    ctx.close()
+    ctx3 = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
+    ctx3.set_time(cfg.timesum, cfg.timestep)
+    ctx3.set_all_nodal(cfg.nodal)
+    ctx3.upload_markers(cfg.markers)
+    ctx3.marker2nodes()
+    for k in NODAL_FIELDS:
+        assert np.array_equal(ctx2.get_nodal(k), ctx3.get_nodal(k), equal_nan=True), k
     ctx2.close()
+    ctx3.close()
+
+
+def _oracle_keys(cfg, x, y):
+    """Sub-cell key of every marker from the oracle's nxsearch / nysearch
+    (the reference bisection, interpolation.c:1536-1573) at the position
+    clamped to the domain: cell nx*(ynumy-1)+ny, times 4, plus the quadrant
+    (2 if past the cell's x midline, 1 if past its y midline)."""
+    from oracle.pyoracle import Oracle
+    g = cfg.grid
+    xc = np.clip(x, 0.0, g.xsize)
+    yc = np.clip(y, 0.0, g.ysize)
+    nx, ny = Oracle(cfg).cells(xc, yc)
+    cx = (xc - g.gx[nx]) / (g.gx[nx + 1] - g.gx[nx])
+    cy = (yc - g.gy[ny]) / (g.gy[ny + 1] - g.gy[ny])
+    q = np.where(cx > 0.5, 2, 0) + np.where(cy > 0.5, 1, 0)
+    return ((nx * (g.ynumy - 1) + ny) * 4 + q).astype(np.int64)
+
+
+@pytest.mark.parametrize("case", ["edge", "clustered", "refined", "small"])
+def test_subcell_keys_match_oracle_cells(gpu, case):
+    """The device's cell of every marker (its table-driven search, kept as the
+    sort key) equals the oracle's bisection cell, before and after an
+    advection step; the edge cloud puts markers on gridlines, midlines, the
+    domain boundary and outside it."""
+    from velo2cycles_amd.abi import Context
+    cfg = {"edge": synth.config_edge, "clustered": _clustered,
+           "refined": lambda: synth.config_grid(201, 101, 200e3, 100e3, 4, refine=True, seed=41),
+           "small": synth.config_small}[case]()
+    cfg.timesum = DT
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
+    ctx.set_time(cfg.timesum, cfg.timestep)
+    ctx.set_all_nodal(cfg.nodal)
+    m = cfg.markers.copy()
+    ctx.upload_markers(m)
+    for stage in ("uploaded", "advected"):
+        order, skey = ctx.order_get()
+        key = np.empty(cfg.n, np.int64)
+        key[order.astype(np.int64)] = skey
+        ctx.download_markers(m)
+        ref = _oracle_keys(cfg, m["x"], m["y"])
+        bad = np.nonzero(key != ref)[0]
+        assert bad.size == 0, f"{stage}: {bad.size} keys differ, first marker {bad[:5]}"
+        assert np.all(np.diff(skey.astype(np.int64)) >= 0)
+        if stage == "uploaded":
+            ctx.marker2nodes()
+            ctx.nodes2markers_advect()
+    ctx.close()
 
 
 @pytest.mark.parametrize("dt", [1e9, 1e10])

@@ -218,6 +218,53 @@ def test_heat_diffusion_single_marker_closed_form():
     assert np.allclose(o.nodal["tol1"][node], d1, rtol=1e-13)
 
 
+@pytest.mark.parametrize("per", [2, 3])
+def test_heat_diffusion_several_markers_closed_form(per):
+    """per x per markers per cell with different temperatures, uniform nodal
+    properties, old = new nodal T: every marker relaxes by
+    d_i = (T0 - k_i)(1 - exp(-c)); the cell's upper-left node collects
+    tol0 = sum w_i and tol1 = sum d_i w_i / sum w_i with
+    w_i = (1-cx_i)(1-cy_i)/(dx dy) (temperature_iterate.c:301-327), and each
+    marker gives back tol1 (1-cx_i)(1-cy_i).  With several markers per node
+    the weights no longer cancel: a wrong weight (another corner's, or a
+    missing area factor in tol0) changes the result."""
+    cfg = _heat_cfg(per=per, seed=31)
+    cfg.nodal["tk"][:] = 1000.0
+    cfg.nodal["tk2"][:] = 1000.0
+    m = cfg.markers
+    rng = np.random.default_rng(17)
+    m["k"][:] = 1000.0 + rng.uniform(-80, 80, m.n)
+    heatdif = 3e-3
+    o = Oracle(cfg)
+    o.temperature_markers(heatdif)
+    g = cfg.grid
+    gx, gy = g.gx, g.gy
+    x, y, k0 = m["x"], m["y"], m["k"]
+    i = np.clip(np.searchsorted(gx, x, side="right") - 1, 0, g.xnumx - 2)
+    j = np.clip(np.searchsorted(gy, y, side="right") - 1, 0, g.ynumy - 2)
+    dx, dy = gx[i + 1] - gx[i], gy[j + 1] - gy[j]
+    cx, cy = (x - gx[i]) / dx, (y - gy[j]) / dy
+    c = heatdif * 2.5 / (1000.0 * 3000.0) * 2.0 * (1 / dx ** 2 + 1 / dy ** 2) * cfg.timestep
+    d = (1000.0 - k0) * (1.0 - np.exp(-np.minimum(c, 150.0)))
+    w = (1 - cx) * (1 - cy) / (dx * dy)
+    node = i * g.ynumy + j
+    tol0 = np.bincount(node, weights=w, minlength=g.nodenum)
+    tol1 = np.bincount(node, weights=d * w, minlength=g.nodenum)
+    has = tol0 != 0
+    tol1[has] /= tol0[has]
+    expect = k0 + d - tol1[node] * (1 - cx) * (1 - cy)
+    assert np.allclose(o.nodal["tol0"], tol0, rtol=1e-12, atol=0)
+    assert np.allclose(o.nodal["tol1"], tol1, rtol=1e-11, atol=1e-12)
+    assert np.allclose(o.markers["k"], expect, rtol=1e-13, atol=0)
+    # the averaging weight matters: the upper-right corner's weight or
+    # uniform weights give a different answer on this cloud
+    for alt in (cx * (1 - cy), np.ones_like(cx)):
+        t1 = np.bincount(node, weights=d * alt, minlength=g.nodenum)
+        t0 = np.bincount(node, weights=alt, minlength=g.nodenum)
+        t1[t0 != 0] /= t0[t0 != 0]
+        assert not np.allclose(k0 + d - t1[node] * (1 - cx) * (1 - cy), expect, rtol=1e-9, atol=0)
+
+
 # ---- boundary-condition tail ---------------------------------------------------
 
 def _bc_case(seed=4, compressible=0, extra=True):
