@@ -14,16 +14,25 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-@pytest.fixture(scope="module")
-def checker(tmp_path_factory):
+def _build(tmp_path_factory, name):
     if not os.path.exists(HIPCC):
         pytest.skip("hipcc not available")
-    out = tmp_path_factory.mktemp("search") / "search_check"
-    src = os.path.join(HERE, "native", "search_check.hip")
+    out = tmp_path_factory.mktemp(name) / name
+    src = os.path.join(HERE, "native", name + ".hip")
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-ffp-contract=off",
                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-o", str(out), src],
                    check=True, capture_output=True)
     return str(out)
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    return _build(tmp_path_factory, "search_check")
+
+
+@pytest.fixture(scope="module")
+def divrcp(tmp_path_factory):
+    return _build(tmp_path_factory, "divrcp_check")
 
 
 def _axes():
@@ -51,3 +60,18 @@ def test_table_search_matches_bisection(checker, tmp_path):
     for line in r.stdout.strip().splitlines():
         probes, bad = map(int, line.split())
         assert probes > 1000 and bad == 0
+
+
+def test_division_by_reciprocal_is_exact(divrcp, tmp_path):
+    """div_rcp(a, b, RN(1/b)) == a / b bit for bit: on every stencil
+    numerator/denominator pair of the test axes and on 20 M random pairs.
+    The kernels use it for every weight that decides a marker position."""
+    files = []
+    for name, g in _axes().items():
+        p = tmp_path / f"{name}.bin"
+        np.ascontiguousarray(g, dtype=np.float64).tofile(p)
+        files.append(str(p))
+    r = subprocess.run([divrcp] + files, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    checks, bad = map(int, r.stdout.strip().splitlines()[-1].split())
+    assert checks > 20_000_000 and bad == 0

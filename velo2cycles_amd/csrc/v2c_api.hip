@@ -63,6 +63,7 @@ struct v2c_ctx {
     // local update
     double *d_mx = nullptr;
     uint8_t *d_fault = nullptr;
+    uint32_t *d_defer = nullptr;  // markers the fast fused update hands to the general one
     unsigned long long *d_lu = nullptr;
     // warm-start MLP
     float *d_mlp = nullptr, *d_mlp_m = nullptr, *d_mlp_v = nullptr, *d_mlp_norm = nullptr;
@@ -344,7 +345,7 @@ static void free_all(v2c_ctx *c)
                     c->d_order2, c->d_skey2, c->d_tile_count, c->d_tile_off, c->d_mask,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
                     c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm, c->d_post, c->d_bondm, c->d_bondn,
-                    c->d_bondv, c->d_bcx, c->d_bcflag};
+                    c->d_bondv, c->d_bcx, c->d_bcflag, c->d_defer};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
@@ -428,6 +429,7 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipHostMalloc((void **)&c->h_nsel, sizeof(long), hipHostMallocDefault));
     CK(hipMalloc(&c->d_mx, sizeof(double) * NMX * c->cap));
     CK(hipMalloc(&c->d_fault, c->cap));
+    CK(hipMalloc(&c->d_defer, sizeof(uint32_t) * c->cap));
     CK(hipMalloc(&c->d_lu, sizeof(unsigned long long) * LS_COUNT));
     CK(hipMemsetAsync(c->d_mx, 0, sizeof(double) * NMX * c->cap, c->stream));
     CK(hipMemsetAsync(c->d_fault, 0, c->cap, c->stream));
@@ -714,12 +716,27 @@ static int n2m_impl(v2c_ctx *c, bool advect)
     if (side) HIPCHK(c, hipEventRecord(c->ev_join, c->side));
     if (c->n > 0) {
         const int nb = (int)blocks_for(c->n, 256);
-        rc = launch(c, V2C_K_N2M_UPDATE, [&] {
-            if (advect)
-                hipLaunchKernelGGL(k_n2m_update<true>, dim3(nb), dim3(256), 0, c->stream, a, nb);
-            else
-                hipLaunchKernelGGL(k_n2m_update<false>, dim3(nb), dim3(256), 0, c->stream, a, nb);
-        });
+        // With antidiffusion and advection the fast form takes every marker
+        // whose RK stages stay in one stencil configuration and lists the
+        // others for the general form.
+        const bool fast = advect && anti;
+        if (fast) {
+            const hipError_t e = hipMemsetAsync(c->d_counters + 2, 0, sizeof(int), c->stream);
+            if (e != hipSuccess) rc = set_err(c, V2C_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+        }
+        if (!rc)
+            rc = launch(c, V2C_K_N2M_UPDATE, [&] {
+                if (fast) {
+                    hipLaunchKernelGGL(k_n2m_update_fast, dim3(nb), dim3(256), 0, c->stream, a, nb, c->d_defer,
+                                       c->d_counters + 2);
+                    hipLaunchKernelGGL(k_n2m_update_list, dim3(N2M_LIST_BLOCKS), dim3(256), 0, c->stream, a,
+                                       c->d_defer, c->d_counters + 2);
+                } else if (advect) {
+                    hipLaunchKernelGGL(k_n2m_update<true>, dim3(nb), dim3(256), 0, c->stream, a, nb);
+                } else {
+                    hipLaunchKernelGGL(k_n2m_update<false>, dim3(nb), dim3(256), 0, c->stream, a, nb);
+                }
+            });
     }
     if (side) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (rc) return rc;

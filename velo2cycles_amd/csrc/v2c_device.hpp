@@ -4,8 +4,10 @@
 // reference's operation order.  The library is compiled with
 // -ffp-contract=off, so the arithmetic that decides cell indices and marker
 // positions (searches, weights, bilinear sums, position updates) rounds exactly
-// like the reference's gcc/x86-64 build; only libm calls (exp, log, pow, sin,
-// ...) may differ, by a few ulp.
+// like the reference's gcc/x86-64 build; divisions by gridline spacings use the
+// cells' precomputed reciprocals through div_rcp, which returns the IEEE
+// quotient bit for bit.  Only libm calls (exp, log, pow, sin, ...) may differ,
+// by a few ulp.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -146,11 +148,14 @@ __device__ __forceinline__ Stencil centre_stencil(const DevGrid &g, const Axis4 
     if (m20 < 1) m20 = 1;
     if (m20 > g.ynumy - 2) m20 = g.ynumy - 2;
     double a0, a1, a2, b0, b1, b2;            // grid[m-1], grid[m], grid[m+1]
-    sel3(X, m10 - X.s, a0, a1, a2);
-    sel3(Y, m20 - Y.s, b0, b1, b2);
+    // after the clamps m - s is 0 or 1 on both axes (a cell's own midline
+    // decides), so the half-spacing's reciprocal is in the cell record
+    const int dx = m10 - X.s, dy = m20 - Y.s;
+    sel3(X, dx, a0, a1, a2);
+    sel3(Y, dy, b0, b1, b2);
     Stencil s;
-    s.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
-    s.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
+    s.e = div_rcp(x - (a0 + a1) / 2.0, (a2 - a0) / 2.0, dx ? X.rc[1] : X.rc[0]);
+    s.n = div_rcp(y - (b0 + b1) / 2.0, (b2 - b0) / 2.0, dy ? Y.rc[1] : Y.rc[0]);
     s.m3 = (uint32_t)(m10 * g.ynumy + m20);
     return s;
 }
@@ -168,8 +173,9 @@ __device__ __forceinline__ Stencil basic_stencil(const DevGrid &g, const Axis4 &
     sel2(X, m10 - X.s + 1, a0, a1);
     sel2(Y, m20 - Y.s + 1, b0, b1);
     Stencil s;
-    s.e = (x - a0) / (a1 - a0);
-    s.n = (y - b0) / (b1 - b0);
+    // the cell's own spacing unless a clamp moved the stencil (edge cells)
+    s.e = m10 == X.s ? div_rcp(x - a0, a1 - a0, X.inv) : (x - a0) / (a1 - a0);
+    s.n = m20 == Y.s ? div_rcp(y - b0, b1 - b0, Y.inv) : (y - b0) / (b1 - b0);
     s.m3 = (uint32_t)(m10 * g.ynumy + m20);
     return s;
 }
@@ -184,10 +190,11 @@ __device__ __forceinline__ Stencil vx_stencil(const DevGrid &g, const Axis4 &X, 
     if (m20 < 0) m20 = 0;
     if (m20 > g.ynumy - 3) m20 = g.ynumy - 3;
     double b0, b1, b2;                        // grid[m], grid[m+1], grid[m+2]
-    sel3(Y, m20 - Y.s + 1, b0, b1, b2);
+    const int dy = m20 - Y.s + 1;             // 0 or 1 after the clamps
+    sel3(Y, dy, b0, b1, b2);
     Stencil s;
-    s.e = (x - X.g[1]) / (X.g[2] - X.g[1]);
-    s.n = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
+    s.e = div_rcp(x - X.g[1], X.g[2] - X.g[1], X.inv);
+    s.n = div_rcp(y - (b0 + b1) / 2.0, (b2 - b0) / 2.0, dy ? Y.rc[1] : Y.rc[0]);
     s.m3 = (uint32_t)(X.s * g.ynumy + m20);
     return s;
 }
@@ -202,10 +209,11 @@ __device__ __forceinline__ Stencil vy_stencil(const DevGrid &g, const Axis4 &X, 
     if (m10 < 0) m10 = 0;
     if (m10 > g.xnumx - 3) m10 = g.xnumx - 3;
     double a0, a1, a2;
-    sel3(X, m10 - X.s + 1, a0, a1, a2);
+    const int dx = m10 - X.s + 1;             // 0 or 1 after the clamps
+    sel3(X, dx, a0, a1, a2);
     Stencil s;
-    s.e = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
-    s.n = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    s.e = div_rcp(x - (a0 + a1) / 2.0, (a2 - a0) / 2.0, dx ? X.rc[1] : X.rc[0]);
+    s.n = div_rcp(y - Y.g[1], Y.g[2] - Y.g[1], Y.inv);
     s.m3 = (uint32_t)(m10 * g.ynumy + Y.s);
     return s;
 }
@@ -266,13 +274,13 @@ __device__ __forceinline__ Stencil4 stencils4(const DevGrid &g, const Axis4 &X, 
     const bool rare = X.s == 0 || X.s == g.xnumx - 2 || Y.s == 0 || Y.s == ny - 2 || x == xmid || y == ymid ||
                       x != x || y != y;
     Stencil4 o;
-    const double ex = (x - X.g[1]) / (X.g[2] - X.g[1]);
-    const double ny_ = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    const double ex = div_rcp(x - X.g[1], X.g[2] - X.g[1], X.inv);
+    const double ny_ = div_rcp(y - Y.g[1], Y.g[2] - Y.g[1], Y.inv);
     double a0, a1, a2, b0, b1, b2;
     sel3(X, xup ? 1 : 0, a0, a1, a2);
     sel3(Y, yup ? 1 : 0, b0, b1, b2);
-    const double exc = (x - (a0 + a1) / 2.0) / ((a2 - a0) / 2.0);
-    const double nyc = (y - (b0 + b1) / 2.0) / ((b2 - b0) / 2.0);
+    const double exc = div_rcp(x - (a0 + a1) / 2.0, (a2 - a0) / 2.0, xup ? X.rc[1] : X.rc[0]);
+    const double nyc = div_rcp(y - (b0 + b1) / 2.0, (b2 - b0) / 2.0, yup ? Y.rc[1] : Y.rc[0]);
     const int cx = X.s + (xup ? 1 : 0), cy = Y.s + (yup ? 1 : 0);
     o.b.e = ex; o.b.n = ny_; o.b.m3 = (uint32_t)(X.s * ny + Y.s);
     o.c.e = exc; o.c.n = nyc; o.c.m3 = (uint32_t)(cx * ny + cy);
@@ -310,8 +318,8 @@ __device__ __forceinline__ uint32_t subcell_key(const DevGrid &g, double x, doub
     x = clampd(x, 0, g.xsize);
     y = clampd(y, 0, g.ysize);
     const Axis4 X = find_x(g, x), Y = find_y(g, y);
-    double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
-    double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    double cx = div_rcp(x - X.g[1], X.g[2] - X.g[1], X.inv);
+    double cy = div_rcp(y - Y.g[1], Y.g[2] - Y.g[1], Y.inv);
     uint32_t q = (cx > 0.5 ? 2u : 0u) | (cy > 0.5 ? 1u : 0u);
     return (((uint32_t)X.s * (uint32_t)(g.ynumy - 1) + (uint32_t)Y.s) << 2) | q;
 }
@@ -548,6 +556,141 @@ __device__ __forceinline__ void allinteri(const KArgs &a, double x, double y, do
     VX = vxa + bil(vx, s.m3, ynumy, wts(s)) * xrat;
     s = S4.vy;
     VY = vya + bil(vy, s.m3, ynumy, wts(s)) * xrat;
+}
+
+// ---- allinteri over the RK4 stages, reusing one stencil configuration -------
+// Inside an interior cell and off both cell midlines (the common case) the
+// four stencils of allinteri are fixed by the cell and the quadrant: the basic
+// stencil at the cell's corner node, the pressure node (cx, cy) of the
+// quadrant, and vx- and vy-node stencils whose nodes are among the six vx and
+// six vy nodes around that pressure node.  RkCache holds what allinteri reads
+// there, in the form its sums use: the esp values, the pairwise sums of vx
+// (vy) that the pressure-node average adds, and the four vx (vy) values of
+// the vx- (vy-) node stencil; per axis the cell's gridlines, its width and
+// reciprocal, and the pressure-node stencil's centre and half-spacing with
+// their reciprocal.  Every later RK stage whose point lies in the same
+// configuration evaluates allinteri from it with the same operands and the
+// same operations, so the result is allinteri's bit for bit, without new
+// gathers or cell searches.
+struct RkAxis {
+    double lo, hi, w, inv;              // cell [lo, hi), hi - lo, 1/(hi - lo)
+    double mid;                         // (lo + hi) / 2: the cell midline
+    double cm, hd, rc;                  // pressure-node stencil: (g[m-1] + g[m]) / 2, (g[m+1] - g[m-1]) / 2, 1/hd
+};
+struct RkCache {
+    RkAxis X, Y;
+    double e[4];                        // esp at the basic stencil, bil order
+    double P[4], F[4];                  // vx: the average's pair sums, the vx-node stencil (bil order)
+    double Q[4], H[4];                  // vy: the same
+    int sx, sy;                         // the cell
+    bool xup, yup, ok;
+};
+
+// Is the clamped point (x, y) in the cached cell (any quadrant)?
+__device__ __forceinline__ bool rk_in_cell(const RkCache &c, double x, double y)
+{
+    return c.ok && x >= c.X.lo && x < c.X.hi && y >= c.Y.lo && y < c.Y.hi;
+}
+
+// Does the clamped point (x, y) lie in the cached configuration?
+__device__ __forceinline__ bool rk_hit(const RkCache &c, double x, double y)
+{
+    return rk_in_cell(c, x, y) && (x > c.X.mid) == c.xup && (y > c.Y.mid) == c.yup && x != c.X.mid &&
+           y != c.Y.mid;
+}
+
+__device__ __forceinline__ void rk_axis(const Axis4 &A, bool up, RkAxis &r)
+{
+    r.lo = A.g[1];
+    r.hi = A.g[2];
+    r.w = A.g[2] - A.g[1];
+    r.inv = A.inv;
+    r.mid = (A.g[1] + A.g[2]) / 2.0;
+    // sel3 of the pressure-node stencil: g[m-1], g[m], g[m+1] with m = s + up
+    const double a0 = up ? A.g[1] : A.g[0], a1 = up ? A.g[2] : A.g[1], a2 = up ? A.g[3] : A.g[2];
+    r.cm = (a0 + a1) / 2.0;
+    r.hd = (a2 - a0) / 2.0;
+    r.rc = up ? A.rc[1] : A.rc[0];
+}
+
+// Fill the cache from the cells (X, Y) of the clamped point (x, y); returns
+// false (cache left empty) where the general path applies.
+__device__ __forceinline__ bool rk_fill(const KArgs &a, const Axis4 &X, const Axis4 &Y, double x, double y,
+                                        RkCache &c)
+{
+    const DevGrid &g = a.g;
+    const int ny = g.ynumy;
+    const double xmid = (X.g[1] + X.g[2]) / 2.0, ymid = (Y.g[1] + Y.g[2]) / 2.0;
+    const bool rare = X.s == 0 || X.s == g.xnumx - 2 || Y.s == 0 || Y.s == ny - 2 || x == xmid || y == ymid ||
+                      x != x || y != y;
+    c.ok = !rare;
+    if (rare) return false;
+    c.xup = x > xmid;
+    c.yup = y > ymid;
+    c.sx = X.s;
+    c.sy = Y.s;
+    rk_axis(X, c.xup, c.X);
+    rk_axis(Y, c.yup, c.Y);
+    const long b = (long)X.s * ny + Y.s;
+    const long m3 = (long)(X.s + (c.xup ? 1 : 0)) * ny + Y.s + (c.yup ? 1 : 0);
+    const double *esp = NOD(a, V2C_N_esp), *vx = NOD(a, V2C_N_vx), *vy = NOD(a, V2C_N_vy);
+    c.e[0] = esp[b]; c.e[1] = esp[b + 1]; c.e[2] = esp[b + ny]; c.e[3] = esp[b + ny + 1];
+    // vx at columns cx-1..cx+1 (u_c), rows cy-1, cy; vy at columns cx-1, cx,
+    // rows cy-1..cy+1 (interpolation.c:1725-1736 reads these)
+    const double u00 = vx[m3 - ny - 1], u01 = vx[m3 - ny], u10 = vx[m3 - 1], u11 = vx[m3];
+    const double u20 = vx[m3 + ny - 1], u21 = vx[m3 + ny];
+    const double v00 = vy[m3 - ny - 1], v01 = vy[m3 - ny], v02 = vy[m3 - ny + 1];
+    const double v10 = vy[m3 - 1], v11 = vy[m3], v12 = vy[m3 + 1];
+    c.P[0] = u10 + u00; c.P[1] = u11 + u01; c.P[2] = u20 + u10; c.P[3] = u21 + u11;
+    c.Q[0] = v01 + v00; c.Q[1] = v02 + v01; c.Q[2] = v11 + v10; c.Q[3] = v12 + v11;
+    // vx-node stencil: column cx - xup, rows cy-1, cy; vy-node stencil:
+    // columns cx-1, cx, rows cy - yup, cy - yup + 1 (bil order)
+    c.F[0] = c.xup ? u00 : u10; c.F[1] = c.xup ? u01 : u11;
+    c.F[2] = c.xup ? u10 : u20; c.F[3] = c.xup ? u11 : u21;
+    c.H[0] = c.yup ? v00 : v01; c.H[1] = c.yup ? v01 : v02;
+    c.H[2] = c.yup ? v10 : v11; c.H[3] = c.yup ? v11 : v12;
+    return true;
+}
+
+// The common-path weights of stencils4 at a point of the configuration:
+// ex, ny the cell-local coordinates (basic-node stencil), exc, nyc the
+// pressure-node stencil's.
+__device__ __forceinline__ void rk_weights(const RkCache &c, double x, double y, double &ex, double &ny_,
+                                           double &exc, double &nyc)
+{
+    ex = div_rcp(x - c.X.lo, c.X.w, c.X.inv);
+    ny_ = div_rcp(y - c.Y.lo, c.Y.w, c.Y.inv);
+    exc = div_rcp(x - c.X.cm, c.X.hd, c.X.rc);
+    nyc = div_rcp(y - c.Y.cm, c.Y.hd, c.Y.rc);
+}
+
+// allinteri's velocity and spin (interpolation.c:1688-1758) at a clamped
+// point of the cached configuration: allinteri's sums, term for term.
+__device__ __forceinline__ void rk_eval(const KArgs &a, const RkCache &c, double x, double y, double &VX,
+                                        double &VY, double &ESP)
+{
+    const DevGrid &g = a.g;
+    double xrat = 2.0 / 3.0;
+    if (x < g.xr_lo) xrat = 1.0;
+    if (x > g.xr_hi) xrat = 1.0;
+    if (y < g.yr_lo) xrat = 1.0;
+    if (y > g.yr_hi) xrat = 1.0;
+    double ex, ny_, exc, nyc;
+    rk_weights(c, x, y, ex, ny_, exc, nyc);
+    Stencil s;
+    s.e = ex; s.n = ny_;
+    W4 w = wts(s);
+    ESP = w.d0 * c.e[0] + w.d1 * c.e[1] + w.d2 * c.e[2] + w.d3 * c.e[3];
+    s.e = exc; s.n = nyc;
+    w = wts(s);
+    const double vxa = (w.d0 * c.P[0] + w.d1 * c.P[1] + w.d2 * c.P[2] + w.d3 * c.P[3]) * 0.5 * (1.0 - xrat);
+    const double vya = (w.d0 * c.Q[0] + w.d1 * c.Q[1] + w.d2 * c.Q[2] + w.d3 * c.Q[3]) * 0.5 * (1.0 - xrat);
+    s.e = ex; s.n = nyc;
+    w = wts(s);
+    VX = vxa + (w.d0 * c.F[0] + w.d1 * c.F[1] + w.d2 * c.F[2] + w.d3 * c.F[3]) * xrat;
+    s.e = exc; s.n = ny_;
+    w = wts(s);
+    VY = vya + (w.d0 * c.H[0] + w.d1 * c.H[1] + w.d2 * c.H[2] + w.d3 * c.H[3]) * xrat;
 }
 
 // ---- build-defined per-marker local update (not reference behaviour) -------

@@ -29,8 +29,8 @@ __device__ __forceinline__ void heat_tk(const KArgs &a, const Axis4 &X, const Ax
                                         double &TK, double &TK2)
 {
     Stencil s;
-    s.e = (x - X.g[1]) / (X.g[2] - X.g[1]);
-    s.n = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    s.e = div_rcp(x - X.g[1], X.g[2] - X.g[1], X.inv);
+    s.n = div_rcp(y - Y.g[1], Y.g[2] - Y.g[1], Y.inv);
     s.m3 = (uint32_t)(X.s * a.g.ynumy + Y.s);
     const W4 w = wts(s);
     TK = bil(NOD(a, V2C_N_tk), s.m3, a.g.ynumy, w);
@@ -62,9 +62,9 @@ __global__ __launch_bounds__(HEAT_BLOCK) void k_heat_markers(KArgs a, double hea
             if (heatdif != 0) {
                 const double dtk = TK2 - mk;
                 const double dx = X.g[2] - X.g[1], dy = Y.g[2] - Y.g[1];
-                const double swt1 = 1.0 / dx / dy;
-                const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
-                const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+                const double swt1 = div_rcp(X.inv, dy, Y.inv);
+                const double cx = div_rcp(x - X.g[1], dx, X.inv);
+                const double cy = div_rcp(y - Y.g[1], dy, Y.inv);
                 // k and rho*Cp at the marker, nodes in the reference's loop order
                 const double *kt = NOD(a, V2C_N_kt), *cp = NOD(a, V2C_N_cp), *ro = NOD(a, V2C_N_ro);
                 const long ny = g.ynumy, m4 = (long)X.s * ny + Y.s;
@@ -164,8 +164,8 @@ __global__ __launch_bounds__(HEAT_BLOCK) void k_heat_antidiff(KArgs a)
     if (!(heat_inside(a.g, x, y) && a.t[m] < 50)) return;
     Axis4 X, Y;
     key_axes(a, a.key[m], X, Y);
-    const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
-    const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    const double cx = div_rcp(x - X.g[1], X.g[2] - X.g[1], X.inv);
+    const double cy = div_rcp(y - Y.g[1], Y.g[2] - Y.g[1], Y.inv);
     const double tol1 = NOD(a, V2C_N_tol1)[(long)X.s * a.g.ynumy + Y.s];
     if (tol1) MK(a, MF_K, m) -= tol1 * ((1.0 - cx) * (1.0 - cy));
 }

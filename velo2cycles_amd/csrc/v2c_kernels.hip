@@ -572,9 +572,10 @@ __device__ __forceinline__ void m2n_cells_group(const KArgs &a, long c0, long c1
                 const double sxy = MK(a, MF_XY, m), sxx = MK(a, MF_XX, m);
                 const double phi = MK(a, MF_PHI, m);
                 const double me = MK(a, MF_E, m);
-                const double swt1 = X.inv / (Y.g[2] - Y.g[1]);
-                const double cx = (x - X.g[1]) / (X.g[2] - X.g[1]);
-                const double cy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+                // 1/dx/dy and the cell-local coordinates, exact quotients
+                const double swt1 = div_rcp(X.inv, Y.g[2] - Y.g[1], Y.inv);
+                const double cx = div_rcp(x - X.g[1], X.g[2] - X.g[1], X.inv);
+                const double cy = div_rcp(y - Y.g[1], Y.g[2] - Y.g[1], Y.inv);
                 double bw = 0.0;
                 bool hasb = true;
                 if (cx < 0.5 && cy < 0.5) bw = swt1 * (1.0 - 2.0 * cx) * (1.0 - 2.0 * cy);
@@ -843,6 +844,47 @@ __device__ __forceinline__ void mech_gather(const KArgs &a, double x, double y, 
     if (WITH_E) o.MVY = bil(NOD(a, V2C_N_mvy), S.m3, ynumy, w);
 }
 
+// The nine interpolants of mech_gather<false> at the clamped point (x, y) of
+// a filled RK cache (the same point, so the same stencils): the basic- and
+// pressure-node fields are gathered, vx and vy come from the cached nodes the
+// vx- and vy-node stencils read; weights and sums as mech_gather's.
+__device__ __forceinline__ void mech_from_cache(const KArgs &a, const RkCache &c, double x, double y, Mech &o)
+{
+    const int ynumy = a.g.ynumy;
+    double ex, ny_, exc, nyc;
+    rk_weights(c, x, y, ex, ny_, exc, nyc);
+    Stencil S;
+    S.e = ex; S.n = ny_;
+    W4 w = wts(S);
+    uint32_t m3 = (uint32_t)(c.sx * ynumy + c.sy);
+    o.EXY = bil(NOD(a, V2C_N_exy), m3, ynumy, w);
+    o.SXY = bil(NOD(a, V2C_N_sxy), m3, ynumy, w);
+    o.ES = bil(NOD(a, V2C_N_pes), m3, ynumy, w);
+    o.PHIS = bil(NOD(a, V2C_N_phis_it), m3, ynumy, w);
+    S.e = exc; S.n = nyc;
+    w = wts(S);
+    m3 = (uint32_t)((c.sx + (c.xup ? 1 : 0)) * ynumy + c.sy + (c.yup ? 1 : 0));
+    o.EXX = bil(NOD(a, V2C_N_exx), m3, ynumy, w);
+    o.SXX = bil(NOD(a, V2C_N_sxx), m3, ynumy, w);
+    o.PR = bil(NOD(a, V2C_N_pr), m3, ynumy, w);
+    S.e = ex; S.n = nyc;
+    w = wts(S);
+    o.VX = w.d0 * c.F[0] + w.d1 * c.F[1] + w.d2 * c.F[2] + w.d3 * c.F[3];
+    S.e = exc; S.n = ny_;
+    w = wts(S);
+    o.VY = w.d0 * c.H[0] + w.d1 * c.H[1] + w.d2 * c.H[2] + w.d3 * c.H[3];
+}
+
+// Sub-cell key of a clamped point in the cached cell: subcell_key's cell and
+// quadrant without the search.
+__device__ __forceinline__ uint32_t key_from_cache(const KArgs &a, const RkCache &c, double x, double y)
+{
+    const double cx = div_rcp(x - c.X.lo, c.X.w, c.X.inv);
+    const double cy = div_rcp(y - c.Y.lo, c.Y.w, c.Y.inv);
+    const uint32_t q = (cx > 0.5 ? 2u : 0u) | (cy > 0.5 ? 1u : 0u);
+    return (((uint32_t)c.sx * (uint32_t)(a.g.ynumy - 1) + (uint32_t)c.sy) << 2) | q;
+}
+
 // Antidiffusion scatter-back contributions of one marker (interpolation.c:
 // 1159-1209, 1360-1374, 1405-1489): the interpolated PHIS, SXY, EXY, ES, PR,
 // SXX, EXX, VX, VY times the node weights, at the clamped position in cell
@@ -866,7 +908,7 @@ __device__ __forceinline__ void anti_contrib(const KArgs &a, double x, double y,
     const double VY = bil(NOD(a, V2C_N_vy), S.vy.m3, ynumy, wts(S.vy));
     // cell-local coordinates (interpolation.c:1140-1150) are the vx x- and
     // vy y-weights: the same formula on the same cell
-    const double swt1 = X.inv / (Y.g[2] - Y.g[1]);
+    const double swt1 = div_rcp(X.inv, Y.g[2] - Y.g[1], Y.inv);
     const double cx = S.vx.e, cy = S.vy.n;
     double bw = 0.0;
     bool hasb = true;
@@ -926,56 +968,31 @@ __device__ __forceinline__ void n2m_update(const KArgs &a, int t, MState &s, con
     }
 }
 
-// moveandrotatemarkers body for one marker of type < 100 (moveandrotatemarkers.c:12-88).
-// Returns false (and changes nothing) for types >= 100.
-// (hx, hy): the cells of the marker's clamped position, or null to search.
-__device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, bool &moved, bool &reset,
-                                           const Axis4 *hx = nullptr, const Axis4 *hy = nullptr)
+// The end of moveandrotatemarkers for one marker (moveandrotatemarkers.c:
+// 45-87) given its velocity and spin (vx0, vy0, sp0): orthogonal-only motion
+// outside (outgrid == 2), the move, the stress rotation, the deep-temperature
+// and out-of-grid resets.
+__device__ __forceinline__ void advect_apply(const KArgs &a, MState &s, double vx0, double vy0, double sp0,
+                                             bool &reset)
 {
     const DevGrid &g = a.g;
     const v2c_params &p = a.p;
     const double dt = a.timestep;
-    moved = false;
-    reset = false;
-    const bool inside = s.x >= 0 && s.y >= 0 && s.x <= g.xsize && s.y <= g.ysize;
-    if (!((inside || p.outgrid != 1) && !a.mat->im[t])) return true;
-    if (t >= 100) return false;
-    double vx0, vy0, sp0, ee0;
-    if (p.markmod == 1) {
-        allinteri(a, s.x, s.y, vx0, vy0, sp0, ee0, hx, hy);
-    } else {
-        // (k1 + 2 k2 + 2 k3 + k4) / 6 summed stage by stage: the same
-        // left-to-right additions, with one accumulator live instead of
-        // every stage's rates.  The strain-rate invariant is not used.
-        double vx, vy, sp, ee;
-        allinteri(a, s.x, s.y, vx, vy, sp, ee, hx, hy);
-        const double sp1 = sp;
-        double ax = vx, ay = vy, as = sp;
-        allinteri(a, s.x + vx * dt / 2.0, s.y + vy * dt / 2.0, vx, vy, sp, ee);
-        ax = ax + 2.0 * vx; ay = ay + 2.0 * vy; as = as + 2.0 * sp;
-        allinteri(a, s.x + vx * dt / 2.0, s.y + vy * dt / 2.0, vx, vy, sp, ee);
-        ax = ax + 2.0 * vx; ay = ay + 2.0 * vy; as = as + 2.0 * sp;
-        allinteri(a, s.x + vx * dt, s.y + vy * dt, vx, vy, sp, ee);
-        ax = ax + vx; ay = ay + vy; as = as + sp;
-        vx0 = ax / 6.0;
-        vy0 = ay / 6.0;
-        sp0 = p.markmod == 2 ? as / 6.0 : sp1;
-        ee0 = 0.0;
-    }
-    (void)ee0;
     if (p.outgrid == 2) {
         if (s.x < 0 || s.x > g.xsize) vy0 = 0;
         if (s.y < 0 || s.y > g.ysize) vx0 = 0;
     }
     s.x += dt * vx0;
     s.y += dt * vy0;
-    moved = true;
     const double th = sp0 * dt;
     const double sxx = s.xx, sxy = s.xy;
     if (p.stoksmod == 1) {
-        double c, sn, s2, c2;
+        // sin 2th and cos 2th from the one sincos (double-angle identities,
+        // a few ulp from sin(2 th), cos(2 th)); stresses carry the 1e-12 bar,
+        // positions do not depend on them
+        double c, sn;
         sincos(th, &sn, &c);
-        sincos(2.0 * th, &s2, &c2);
+        const double s2 = 2.0 * sn * c, c2 = fma(-2.0 * sn, sn, 1.0);
         s.xx = sxx * c * c - sxx * sn * sn + sxy * s2;
         s.xy = 0.5 * (-sxx - sxx) * s2 + sxy * c2;
     } else if (p.stoksmod == 2) {
@@ -986,6 +1003,68 @@ __device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, boo
     if (s.x < 0 || s.y < 0 || s.x > g.xsize || s.y > g.ysize) {
         s.k = 0; s.dirty_k = true; s.e = 0; reset = true;
     }
+}
+
+// Does the marker move at all (moveandrotatemarkers.c:12-16)?
+__device__ __forceinline__ bool advect_moves(const KArgs &a, int t, const MState &s)
+{
+    const DevGrid &g = a.g;
+    const bool inside = s.x >= 0 && s.y >= 0 && s.x <= g.xsize && s.y <= g.ysize;
+    return (inside || a.p.outgrid != 1) && !a.mat->im[t];
+}
+
+// (k1 + 2 k2 + 2 k3 + k4) / 6, summed stage by stage: the same left-to-right
+// additions as the reference, one accumulator live instead of every stage's
+// rates; the strain-rate invariant is not used.  The spin is the RK average
+// (markmod == 2) or the first stage's.  interp(x, y, vx, vy, esp) evaluates
+// allinteri; it returns false to abandon the marker.
+template <class F>
+__device__ __forceinline__ bool rk4_velocity(const KArgs &a, const MState &s, double &vx0, double &vy0,
+                                             double &sp0, F &&interp)
+{
+    const double dt = a.timestep;
+    double vx, vy, sp;
+    if (!interp(s.x, s.y, vx, vy, sp)) return false;
+    const double sp1 = sp;
+    double ax = vx, ay = vy, as = sp;
+    if (!interp(s.x + vx * dt / 2.0, s.y + vy * dt / 2.0, vx, vy, sp)) return false;
+    ax = ax + 2.0 * vx; ay = ay + 2.0 * vy; as = as + 2.0 * sp;
+    if (!interp(s.x + vx * dt / 2.0, s.y + vy * dt / 2.0, vx, vy, sp)) return false;
+    ax = ax + 2.0 * vx; ay = ay + 2.0 * vy; as = as + 2.0 * sp;
+    if (!interp(s.x + vx * dt, s.y + vy * dt, vx, vy, sp)) return false;
+    ax = ax + vx; ay = ay + vy; as = as + sp;
+    vx0 = div_rcp(ax, 6.0, 1.0 / 6.0);
+    vy0 = div_rcp(ay, 6.0, 1.0 / 6.0);
+    sp0 = a.p.markmod == 2 ? div_rcp(as, 6.0, 1.0 / 6.0) : sp1;
+    return true;
+}
+
+// moveandrotatemarkers body for one marker of type < 100 (moveandrotatemarkers.c:12-88),
+// general form (any cell, any stage point).  Returns false (and changes
+// nothing) for types >= 100.  (hx, hy): the cells of the marker's clamped
+// position, or null to search.
+__device__ __forceinline__ bool advect_one(const KArgs &a, int t, MState &s, bool &moved, bool &reset,
+                                           const Axis4 *hx = nullptr, const Axis4 *hy = nullptr)
+{
+    moved = false;
+    reset = false;
+    if (!advect_moves(a, t, s)) return true;
+    if (t >= 100) return false;
+    double vx0, vy0, sp0;
+    if (a.p.markmod == 1) {
+        double ee;
+        allinteri(a, s.x, s.y, vx0, vy0, sp0, ee, hx, hy);
+    } else {
+        int st = 0;
+        rk4_velocity(a, s, vx0, vy0, sp0, [&](double x, double y, double &vx, double &vy, double &sp) {
+            double ee;
+            if (st++ == 0) allinteri(a, x, y, vx, vy, sp, ee, hx, hy);
+            else allinteri(a, x, y, vx, vy, sp, ee);
+            return true;
+        });
+    }
+    advect_apply(a, s, vx0, vy0, sp0, reset);
+    moved = true;
     return true;
 }
 
@@ -1090,6 +1169,108 @@ __global__ __launch_bounds__(256) void k_n2m_update(KArgs a, int nwg)
     if (ADVECT) {
         const unsigned long long bad = __ballot(unsupported);
         if (bad && (threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&a.counters[1], (int)__popcll(bad));
+    }
+}
+
+// nodes2markers + moveandrotatemarkers of one marker through one stencil
+// configuration (antidiffinter == 1): the marker's clamped position lies in
+// an interior cell off both midlines, every RK stage point in the same
+// configuration and the new position in the same cell.  Then the update's
+// gather, all RK stages and the new key come from the one RkCache fill, with
+// the same operands and operations as the general path.  Returns false,
+// having written nothing, for any other marker (the general path takes it).
+__device__ __forceinline__ bool n2m_update_fast(const KArgs &a, long m)
+{
+    const DevGrid &g = a.g;
+    const int t = a.t[m];
+    if (t >= 100) return false;
+    Axis4 X, Y;
+    key_axes(a, a.key[m], X, Y);
+    MState s;
+    s.x = MK(a, MF_X, m); s.y = MK(a, MF_Y, m); s.k = MK(a, MF_K, m);
+    s.dirty_state = false; s.dirty_k = false;
+    const double cxp = clampd(s.x, 0, g.xsize), cyp = clampd(s.y, 0, g.ysize);
+    RkCache c;
+    if (!rk_fill(a, X, Y, cxp, cyp, c)) return false;
+    {
+        Mech mv;
+        mech_from_cache(a, c, cxp, cyp, mv);
+        n2m_update(a, t, s, mv);
+    }
+    // The update's values are written now (a marker handed on to the general
+    // path gets the same values written again: with antidiffinter == 1 they
+    // are the interpolants at the unchanged position, whatever the marker's
+    // previous state; an incoming marker's new temperature makes the general
+    // path take the other branch of the update, which writes the same
+    // interpolants).  Only the stresses, which the rotation still changes,
+    // stay in registers.
+    if (s.dirty_state) {
+        MK(a, MF_PHI, m) = s.phi; MK(a, MF_EXX, m) = s.exx; MK(a, MF_EXY, m) = s.exy;
+        MK(a, MF_P, m) = s.p; MK(a, MF_VX, m) = s.vx; MK(a, MF_VY, m) = s.vy; MK(a, MF_E, m) = s.e;
+    }
+    if (s.dirty_k) MK(a, MF_K, m) = s.k;
+    s.dirty_k = false;
+    if (!s.dirty_state) { s.xx = MK(a, MF_XX, m); s.xy = MK(a, MF_XY, m); }
+    bool moved = false, reset = false;
+    uint32_t key = 0;
+    if (advect_moves(a, t, s)) {
+        double vx0, vy0, sp0;
+        auto interp = [&](double x, double y, double &vx, double &vy, double &sp) {
+            const double xc = clampd(x, 0, g.xsize), yc = clampd(y, 0, g.ysize);
+            if (!rk_hit(c, xc, yc)) return false;
+            rk_eval(a, c, xc, yc, vx, vy, sp);
+            return true;
+        };
+        if (a.p.markmod == 1) {
+            if (!interp(s.x, s.y, vx0, vy0, sp0)) return false;
+        } else if (!rk4_velocity(a, s, vx0, vy0, sp0, interp)) {
+            return false;
+        }
+        advect_apply(a, s, vx0, vy0, sp0, reset);
+        const double xc = clampd(s.x, 0, g.xsize), yc = clampd(s.y, 0, g.ysize);
+        if (!rk_in_cell(c, xc, yc)) return false;
+        key = key_from_cache(a, c, xc, yc);
+        moved = true;
+    }
+    if (s.dirty_state || moved) { MK(a, MF_XX, m) = s.xx; MK(a, MF_XY, m) = s.xy; }
+    if (s.dirty_k) MK(a, MF_K, m) = s.k;
+    if (moved) {
+        MK(a, MF_X, m) = s.x; MK(a, MF_Y, m) = s.y;
+        a.key[m] = key;
+    }
+    if (reset) { MK(a, MF_D, m) = -1.0; MK(a, MF_W, m) = -1.0; MK(a, MF_E, m) = 0; }
+    return true;
+}
+
+// Append the lanes' flagged items to a list, one atomic per wavefront (the
+// list order varies from run to run; every item is processed on its own).
+__device__ __forceinline__ void list_append(bool flag, uint32_t item, uint32_t *list, int *count)
+{
+    const unsigned long long mask = __ballot(flag);
+    if (!mask) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    int base = 0;
+    if (lane == 0) base = atomicAdd(count, (int)__popcll(mask));
+    base = __shfl(base, 0);
+    if (flag) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = item;
+}
+
+// Fused update, fast form: markers the fast path cannot take go to `defer`.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_n2m_update_fast(KArgs a, int nwg, uint32_t *defer, int *ndefer)
+{
+    const long m = (long)xcd_remap(blockIdx.x, nwg) * 256 + threadIdx.x;
+    const bool def = m < a.n && !n2m_update_fast(a, m);
+    list_append(def, (uint32_t)m, defer, ndefer);
+}
+
+// The deferred markers through the general fused update (grid-stride over
+// the device-side count, a fixed grid of N2M_LIST_BLOCKS blocks).
+constexpr int N2M_LIST_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void k_n2m_update_list(KArgs a, const uint32_t *list, const int *count)
+{
+    const int n = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (n2m_update_one<true>(a, list[i])) atomicAdd(&a.counters[1], 1);
     }
 }
 

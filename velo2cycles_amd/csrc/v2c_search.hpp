@@ -19,9 +19,28 @@
 namespace v2c {
 
 // Cell s of an axis with its gridlines g[k] = grid[s-1+k], k = 0..3 (entries
-// past either end of the axis hold NaN; no stencil reads them) and
-// inv = 1.0 / (grid[s+1] - grid[s]).
-struct Axis4 { double g[4]; double inv; int s, pad; };
+// past either end of the axis hold NaN; no stencil reads them),
+// inv = 1.0 / (grid[s+1] - grid[s]) and the reciprocals of the two
+// pressure-node half-spacings a stencil in this cell divides by:
+// rc[d] = 1.0 / ((g[d+2] - g[d]) / 2.0), d = 0 below the cell midline, 1 above
+// (each rounded once, from the value the reference's expression rounds to).
+// 64 bytes: one cache line.
+struct Axis4 { double g[4]; double inv; double rc[2]; int s, pad; };
+
+// a / b rounded to nearest, computed from rb = RN(1/b): y = RN(a rb) is within
+// one ulp of a/b, the residual a - b y is exact in one fused multiply-add, and
+// RN(y + rb (a - b y)) is the correctly rounded quotient (Markstein's
+// correction theorem; it holds for normal a, b with a/b away from overflow and
+// underflow, which covers every use here: cell-local coordinates and stencil
+// weights, |a/b| <= 2 or a == 0).  The result equals the IEEE division a / b
+// bit for bit, at the cost of one multiply and two FMAs instead of a
+// division sequence.
+__host__ __device__ inline double div_rcp(double a, double b, double rb)
+{
+    const double y = a * rb;
+    const double e = fma(-b, y, a);
+    return fma(e, rb, y);
+}
 
 __host__ __device__ inline double pick(const Axis4 &A, int m)
 {
@@ -39,6 +58,7 @@ __host__ __device__ inline Axis4 axis_cell(const double *__restrict__ grid, int 
         a.g[k] = (i >= 0 && i < n) ? grid[i] : NAN;
     }
     a.inv = 1.0 / (grid[s + 1] - grid[s]);
+    for (int d = 0; d < 2; d++) a.rc[d] = 1.0 / ((a.g[d + 2] - a.g[d]) / 2.0);
     return a;
 }
 
