@@ -64,6 +64,7 @@ struct v2c_ctx {
     double *d_mx = nullptr;
     uint8_t *d_fault = nullptr;
     uint32_t *d_defer = nullptr;  // markers the fast fused update hands to the general one
+    uint8_t *d_fclass = nullptr;  // constant friction set each marker's fields hold (friction_marker)
     unsigned long long *d_lu = nullptr;
     // warm-start MLP
     float *d_mlp = nullptr, *d_mlp_m = nullptr, *d_mlp_v = nullptr, *d_mlp_norm = nullptr;
@@ -175,6 +176,7 @@ static KArgs make_args(v2c_ctx *c)
     a.timesum = c->timesum; a.timestep = c->timestep;
     setp(a.counters, c->d_counters);
     a.write_friction = c->write_friction;
+    setp(a.fclass, c->d_fclass);
     setp(a.mx, c->d_mx);
     setp(a.fault, c->d_fault);
     setp(a.lu_stats, c->d_lu);
@@ -345,7 +347,7 @@ static void free_all(v2c_ctx *c)
                     c->d_order2, c->d_skey2, c->d_tile_count, c->d_tile_off, c->d_mask,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
                     c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm, c->d_post, c->d_bondm, c->d_bondn,
-                    c->d_bondv, c->d_bcx, c->d_bcflag, c->d_defer};
+                    c->d_bondv, c->d_bcx, c->d_bcflag, c->d_defer, c->d_fclass};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
@@ -430,6 +432,8 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_mx, sizeof(double) * NMX * c->cap));
     CK(hipMalloc(&c->d_fault, c->cap));
     CK(hipMalloc(&c->d_defer, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_fclass, c->cap));
+    CK(hipMemsetAsync(c->d_fclass, 0, c->cap, c->stream));
     CK(hipMalloc(&c->d_lu, sizeof(unsigned long long) * LS_COUNT));
     CK(hipMemsetAsync(c->d_mx, 0, sizeof(double) * NMX * c->cap, c->stream));
     CK(hipMemsetAsync(c->d_fault, 0, c->cap, c->stream));
@@ -555,6 +559,8 @@ int v2c_markers_upload(v2c_ctx *c, const v2c_markers *m, long n)
         HIPCHK(c, hipMemcpyAsync(c->d_mk + (size_t)f * c->cap, host_field(m, f), sizeof(double) * n,
                                  hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_t, m->t, n, hipMemcpyHostToDevice, c->stream));
+    // the friction fields came from the host: none is known to hold a constant set
+    HIPCHK(c, hipMemsetAsync(c->d_fclass, 0, n, c->stream));
     c->n = n;
     KArgs a = make_args(c);
     if (n > 0) {

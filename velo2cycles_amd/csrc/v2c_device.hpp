@@ -88,6 +88,7 @@ struct KArgs {
     double timesum, timestep;
     V2C_GLOBAL int *counters;              // device counters (errors, unsupported types)
     int write_friction;    // write the calc_frictionpara marker outputs
+    V2C_GLOBAL uint8_t *fclass;            // per marker: which constant friction set it holds (0: none)
     V2C_GLOBAL double *mx;                 // NMX extension fields x cap
     V2C_GLOBAL const uint8_t *fault;       // per-marker fault flag
     V2C_GLOBAL unsigned long long *lu_stats;  // local-update statistics
@@ -432,12 +433,19 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
 
 // calc_frictionpara (calc_RSF.c:224-295): marker RSF parameters from the
 // fault-band geometry.  Returns the (possibly reset) state phi; writes the
-// parameter fields when requested.
+// parameter fields when requested.  Outside the band the fields are one of
+// two constant sets (rock types > 0, type 0); fclass records which set a
+// marker's fields already hold, so an unchanged set is not written again (the
+// values in memory are the ones the reference writes every call).
+enum { FC_NONE = 0, FC_OUT = 1, FC_TYPE0 = 2 };
 __device__ __forceinline__ double friction_marker(const KArgs &a, long m, int t, double x, double y, double phi)
 {
     const v2c_params &p = a.p;
     double radius = ref_abs(y - p.band_y);
     double hr2 = p.half_range;
+    const int fc = t > 0 ? (radius <= p.band_halfwidth ? FC_NONE : FC_OUT) : FC_TYPE0;
+    const bool fresh = fc != FC_NONE && a.write_friction && a.fclass[m] == fc;
+    if (a.write_friction && !fresh) a.fclass[m] = (uint8_t)fc;
     if (t > 0) {
         if (radius <= p.band_halfwidth) {
             double A, B, L, V0;
@@ -466,52 +474,49 @@ __device__ __forceinline__ double friction_marker(const KArgs &a, long m, int t,
             if (a.timesum == a.timestep) phi = 0.0;
         } else {
             phi = 50.0;
-            if (a.write_friction) {
+            if (a.write_friction && !fresh) {
                 MK(a, MF_FS, m) = 0.6;
                 MK(a, MF_A, m) = p.a_hr; MK(a, MF_B, m) = p.b_hr; MK(a, MF_L0, m) = p.L_hr;
                 MK(a, MF_L, m) = p.L_hr; MK(a, MF_V0, m) = p.V0_hr;
             }
         }
-    } else if (a.write_friction) {
+    } else if (a.write_friction && !fresh) {
         MK(a, MF_FS, m) = 0.6;
         MK(a, MF_A, m) = 1.0; MK(a, MF_B, m) = 0.01; MK(a, MF_L0, m) = 0.2; MK(a, MF_V0, m) = p.V0_hr;
     }
     return phi;
 }
 
-// calc_frictionpara_nodes (calc_RSF.c:298-350)
-__device__ __forceinline__ void friction_node(const KArgs &a, double x, double y, long m3)
+// calc_frictionpara_nodes (calc_RSF.c:298-350) as values: mus_s, as, bs,
+// Ls, V0s of the node at (x, y).
+__device__ __forceinline__ void friction_node_values(const KArgs &a, double x, double y, double &mus, double &A_,
+                                                     double &B_, double &L_, double &V0_)
 {
     const v2c_params &p = a.p;
     double radius = ref_abs(y - p.band_y);
     double hr2 = p.half_range;
     if (radius <= p.band_halfwidth) {
-        double A, B, L, V0;
         if (x > p.end_updip && x < p.start_downdip) {
-            A = p.a_vw; B = p.b_vw; L = p.L_vw; V0 = p.V0_vw;
+            A_ = p.a_vw; B_ = p.b_vw; L_ = p.L_vw; V0_ = p.V0_vw;
         } else if (x >= p.start_updip && x <= p.end_updip && p.half_range > 0) {
             double f = (x - (p.start_updip)) / (2 * hr2);
-            A = p.a_vs + f * (p.a_vw - p.a_vs);
-            B = p.b_vs + f * (p.b_vw - p.b_vs);
-            L = p.L_vs + f * (p.L_vw - p.L_vs);
-            V0 = p.V0_vs + f * (p.V0_vw - p.V0_vs);
+            A_ = p.a_vs + f * (p.a_vw - p.a_vs);
+            B_ = p.b_vs + f * (p.b_vw - p.b_vs);
+            L_ = p.L_vs + f * (p.L_vw - p.L_vs);
+            V0_ = p.V0_vs + f * (p.V0_vw - p.V0_vs);
         } else if (x >= p.start_downdip && x < p.end_downdip && p.half_range > 0) {
             double f = (x - (p.start_downdip)) / (2 * hr2);
-            A = p.a_vw - f * (p.a_vw - p.a_vs);
-            B = p.b_vw - f * (p.b_vw - p.b_vs);
-            L = p.L_vw - f * (p.L_vw - p.L_vs);
-            V0 = p.V0_vw - f * (p.V0_vw - p.V0_vs);
+            A_ = p.a_vw - f * (p.a_vw - p.a_vs);
+            B_ = p.b_vw - f * (p.b_vw - p.b_vs);
+            L_ = p.L_vw - f * (p.L_vw - p.L_vs);
+            V0_ = p.V0_vw - f * (p.V0_vw - p.V0_vs);
         } else {
-            A = p.a_vs; B = p.b_vs; L = p.L_vs; V0 = p.V0_vs;
+            A_ = p.a_vs; B_ = p.b_vs; L_ = p.L_vs; V0_ = p.V0_vs;
         }
-        NOD(a, V2C_N_mus_s)[m3] = 0.2;
-        NOD(a, V2C_N_as)[m3] = A; NOD(a, V2C_N_bs)[m3] = B; NOD(a, V2C_N_Ls)[m3] = L;
-        NOD(a, V2C_N_V0s)[m3] = V0;
-        NOD(a, V2C_N_cohesion)[m3] = 0.0;
+        mus = 0.2;
     } else {
-        NOD(a, V2C_N_mus_s)[m3] = 0.6;
-        NOD(a, V2C_N_as)[m3] = p.a_hr; NOD(a, V2C_N_bs)[m3] = p.b_hr; NOD(a, V2C_N_Ls)[m3] = p.L_hr;
-        NOD(a, V2C_N_V0s)[m3] = p.V0_hr;
+        mus = 0.6;
+        A_ = p.a_hr; B_ = p.b_hr; L_ = p.L_hr; V0_ = p.V0_hr;
     }
 }
 

@@ -492,7 +492,11 @@ __device__ __forceinline__ void key_axes(const KArgs &a, uint32_t key, Axis4 &X,
 // calc_frictionpara.  Separate from the scatter so the transcendental-heavy
 // part runs at full occupancy with coalesced access.
 // ---------------------------------------------------------------------------
-enum MProp { P_NU = 0, P_RO, P_BB, P_KT, NPROP };
+// Hand-over of the property pass to the scatter: the viscosity (ductilevisc)
+// and the conductivity (it needs the pressure at the marker); the density
+// only where it depends on that pressure (densimod != 0).  The expansivity
+// depends on T and type alone and is recomputed by the scatter.
+enum MProp { P_NU = 0, P_KT, P_RO, NPROP };
 
 __device__ __forceinline__ void m2n_props_one(const KArgs &a, long m)
 {
@@ -514,10 +518,10 @@ __device__ __forceinline__ void m2n_props_one(const KArgs &a, long m)
     const double phi = friction_marker(a, m, t, x, y, phi0);
     if (__double_as_longlong(phi) != __double_as_longlong(phi0)) MK(a, MF_PHI, m) = phi;
     double *P = a.prop;
+    (void)mbb;
     P[(long)P_NU * a.cap + m] = mnu;
-    P[(long)P_RO * a.cap + m] = mro;
-    P[(long)P_BB * a.cap + m] = mbb;
     P[(long)P_KT * a.cap + m] = mkt;
+    if (a.p.densimod != 0) P[(long)P_RO * a.cap + m] = mro;
 }
 
 __global__ __launch_bounds__(256) void k_m2n_props(KArgs a, int nwg)
@@ -567,8 +571,10 @@ __device__ __forceinline__ void m2n_cells_group(const KArgs &a, long c0, long c1
                 const double mbm = a.p.bulk_modulus;
                 const double mgg = M->gg[t];
                 const double mtk = MK(a, MF_K, m);
-                const double mnu = P[(long)P_NU * a.cap + m], mro = P[(long)P_RO * a.cap + m];
-                const double mbb = P[(long)P_BB * a.cap + m], mkt = P[(long)P_KT * a.cap + m];
+                const double mnu = P[(long)P_NU * a.cap + m], mkt = P[(long)P_KT * a.cap + m];
+                const double mro = a.p.densimod == 0 ? M->ro[t] : P[(long)P_RO * a.cap + m];
+                // dencalcomp's expansivity (density_ductilevisc_calc.c:2-27)
+                const double mbb = M->bb[t] / (1.0 - M->bb[t] * (mtk - 298.15));
                 const double sxy = MK(a, MF_XY, m), sxx = MK(a, MF_XX, m);
                 const double phi = MK(a, MF_PHI, m);
                 const double me = MK(a, MF_E, m);
@@ -670,98 +676,112 @@ __device__ __forceinline__ void m2n_nodes_one(const KArgs &a, long m3)
                 Cs[f] = R[0 * M_NC + f] + R[1 * M_NC + f] + R[2 * M_NC + f] + R[3 * M_NC + f];
     }
 
-    // zeroed by the reference's initialisation loop (interpolation.c:58-153)
-    static constexpr int zeroed[] = {
-        V2C_N_bm, V2C_N_ro, V2C_N_et, V2C_N_nu, V2C_N_nd, V2C_N_gg, V2C_N_gd, V2C_N_sppe,
-        V2C_N_phis, V2C_N_phin, V2C_N_sxyen, V2C_N_sxxen, V2C_N_exyen, V2C_N_exxen,
-        V2C_N_sppen, V2C_N_mus_s, V2C_N_as, V2C_N_bs, V2C_N_Ls, V2C_N_V0s, V2C_N_sxxe,
-        V2C_N_sxye, V2C_N_exxe, V2C_N_exye, V2C_N_cohesion, V2C_N_dro, V2C_N_drp,
-        V2C_N_cp, V2C_N_kt, V2C_N_ht, V2C_N_tk, V2C_N_mrx, V2C_N_mry, V2C_N_mvx, V2C_N_mvy,
-        V2C_N_wtphisum, V2C_N_phisum, V2C_N_sxysum, V2C_N_exysum, V2C_N_esum,
-        V2C_N_wtsxxsum, V2C_N_psum, V2C_N_sxxsum, V2C_N_exxsum, V2C_N_wtvxsum,
-        V2C_N_vxsum, V2C_N_wtvysum, V2C_N_vysum, V2C_N_mrx0, V2C_N_mry0};
-    // read the antidiffusion inputs before any output of this node is written
+    // Every field the reference's initialisation loop zeroes
+    // (interpolation.c:58-153) is written exactly once below, with its value
+    // or with the zero that loop left; the others only where the reference
+    // writes them.  The antidiffusion inputs are read before any output.
     const bool anti = a.timesum > 0 && a.p.antidiffinter == 1;
+    const bool inA = B[13] != 0 && m11 >= 1 && m22 >= 1 && m11 < xnumx - 1 && m22 < ynumy - 1;
+    const bool inC = Cs[6] != 0 && m11 >= 1 && m22 >= 1;
     double in_phis_it = 0, in_phie = 0, in_sxy = 0, in_sxyee = 0, in_exy = 0, in_exyee = 0,
            in_pes = 0, in_pee = 0, in_sxx = 0, in_sxxee = 0, in_pr = 0, in_sppee = 0,
            in_exx = 0, in_exxee = 0, in_vx = 0, in_mvxe = 0, in_vy = 0, in_mvye = 0;
     if (anti) {
-        in_phis_it = NOD(a, V2C_N_phis_it)[m3]; in_phie = NOD(a, V2C_N_phie)[m3];
-        in_sxy = NOD(a, V2C_N_sxy)[m3]; in_sxyee = NOD(a, V2C_N_sxyee)[m3];
-        in_exy = NOD(a, V2C_N_exy)[m3]; in_exyee = NOD(a, V2C_N_exyee)[m3];
-        in_pes = NOD(a, V2C_N_pes)[m3]; in_pee = NOD(a, V2C_N_pee)[m3];
-        in_sxx = NOD(a, V2C_N_sxx)[m3]; in_sxxee = NOD(a, V2C_N_sxxee)[m3];
-        in_pr = NOD(a, V2C_N_pr)[m3]; in_sppee = NOD(a, V2C_N_sppee)[m3];
-        in_exx = NOD(a, V2C_N_exx)[m3]; in_exxee = NOD(a, V2C_N_exxee)[m3];
-        in_vx = NOD(a, V2C_N_vx)[m3]; in_mvxe = NOD(a, V2C_N_mvxe)[m3];
-        in_vy = NOD(a, V2C_N_vy)[m3]; in_mvye = NOD(a, V2C_N_mvye)[m3];
+        if (inA) {
+            in_phis_it = NOD(a, V2C_N_phis_it)[m3]; in_phie = NOD(a, V2C_N_phie)[m3];
+            in_sxy = NOD(a, V2C_N_sxy)[m3]; in_sxyee = NOD(a, V2C_N_sxyee)[m3];
+            in_exy = NOD(a, V2C_N_exy)[m3]; in_exyee = NOD(a, V2C_N_exyee)[m3];
+            in_pes = NOD(a, V2C_N_pes)[m3]; in_pee = NOD(a, V2C_N_pee)[m3];
+        }
+        if (inC) {
+            in_sxx = NOD(a, V2C_N_sxx)[m3]; in_sxxee = NOD(a, V2C_N_sxxee)[m3];
+            in_pr = NOD(a, V2C_N_pr)[m3]; in_sppee = NOD(a, V2C_N_sppee)[m3];
+            in_exx = NOD(a, V2C_N_exx)[m3]; in_exxee = NOD(a, V2C_N_exxee)[m3];
+        }
+        if (VXs[2] != 0) { in_vx = NOD(a, V2C_N_vx)[m3]; in_mvxe = NOD(a, V2C_N_mvxe)[m3]; }
+        if (VYs[2] != 0) { in_vy = NOD(a, V2C_N_vy)[m3]; in_mvye = NOD(a, V2C_N_mvye)[m3]; }
     }
-    for (int z = 0; z < (int)(sizeof(zeroed) / sizeof(zeroed[0])); z++) NOD(a, zeroed[z])[m3] = 0.0;
-
     const double w = B[13];
-    if (w != 0 && m11 >= 1 && m22 >= 1 && m11 < xnumx - 1 && m22 < ynumy - 1) {
-        const double phin = B[6] / w, sxyen = B[3] / w, exyen = B[4] / w, pen = B[2] / w;
-        NOD(a, V2C_N_phin)[m3] = phin; NOD(a, V2C_N_sxyen)[m3] = sxyen;
-        NOD(a, V2C_N_exyen)[m3] = exyen; NOD(a, V2C_N_pen)[m3] = pen;
-        double pe;
+    // basic nodes (interpolation.c:714-767)
+    double phin = 0, sxyen = 0, exyen = 0, phis = 0, sxye = 0, exye = 0, pe = 0, coh = 0, gg = 0;
+    double mus = 0, as_ = 0, bs = 0, Ls = 0, V0s = 0;
+    if (inA) {
+        phin = B[6] / w; sxyen = B[3] / w; exyen = B[4] / w;
+        const double pen = B[2] / w;
+        NOD(a, V2C_N_pen)[m3] = pen;
         if (anti) {
-            NOD(a, V2C_N_phis)[m3] = in_phis_it + (phin - in_phie);
-            NOD(a, V2C_N_sxye)[m3] = in_sxy + (sxyen - in_sxyee);
-            NOD(a, V2C_N_exye)[m3] = in_exy + (exyen - in_exyee);
+            phis = in_phis_it + (phin - in_phie);
+            sxye = in_sxy + (sxyen - in_sxyee);
+            exye = in_exy + (exyen - in_exyee);
             pe = in_pes + (pen - in_pee);
         } else {
-            NOD(a, V2C_N_phis)[m3] = phin; NOD(a, V2C_N_sxye)[m3] = sxyen;
-            NOD(a, V2C_N_exye)[m3] = exyen; pe = pen;
+            phis = phin; sxye = sxyen; exye = exyen; pe = pen;
         }
         if (pe < 0) pe = 0;
-        NOD(a, V2C_N_pe)[m3] = pe;
-        friction_node(a, a.g.gx[m11], a.g.gy[m22], m3);
-        NOD(a, V2C_N_cohesion)[m3] = B[5] / w;
+        friction_node_values(a, a.g.gx[m11], a.g.gy[m22], mus, as_, bs, Ls, V0s);
+        coh = B[5] / w;
         const double nu_m2c = B[0] / w;
         NOD(a, V2C_N_nu_m2c)[m3] = nu_m2c;
-        NOD(a, V2C_N_gg)[m3] = nu_m2c / (B[1] / w);
+        gg = nu_m2c / (B[1] / w);
     }
-    const double wc = Cs[6];
-    if (wc != 0 && m11 >= 1 && m22 >= 1) {
-        const double sxxen = Cs[3] / wc, sppen = Cs[5] / wc, exxen = Cs[4] / wc;
-        NOD(a, V2C_N_sxxen)[m3] = sxxen; NOD(a, V2C_N_sppen)[m3] = sppen; NOD(a, V2C_N_exxen)[m3] = exxen;
-        double sppe;
+    NOD(a, V2C_N_phin)[m3] = phin; NOD(a, V2C_N_sxyen)[m3] = sxyen; NOD(a, V2C_N_exyen)[m3] = exyen;
+    NOD(a, V2C_N_phis)[m3] = phis; NOD(a, V2C_N_sxye)[m3] = sxye; NOD(a, V2C_N_exye)[m3] = exye;
+    NOD(a, V2C_N_pe)[m3] = pe; NOD(a, V2C_N_cohesion)[m3] = coh; NOD(a, V2C_N_gg)[m3] = gg;
+    NOD(a, V2C_N_mus_s)[m3] = mus; NOD(a, V2C_N_as)[m3] = as_; NOD(a, V2C_N_bs)[m3] = bs;
+    NOD(a, V2C_N_Ls)[m3] = Ls; NOD(a, V2C_N_V0s)[m3] = V0s;
+    // pressure nodes (interpolation.c:769-801)
+    double sxxen = 0, sppen = 0, exxen = 0, sxxe = 0, sppe = 0, exxe = 0, gd = 0, bm = 0, dro = 0, drp = 0;
+    if (inC) {
+        const double wc = Cs[6];
+        sxxen = Cs[3] / wc; sppen = Cs[5] / wc; exxen = Cs[4] / wc;
         if (anti) {
-            NOD(a, V2C_N_sxxe)[m3] = in_sxx + (sxxen - in_sxxee);
+            sxxe = in_sxx + (sxxen - in_sxxee);
             sppe = in_pr + (sppen - in_sppee);
-            NOD(a, V2C_N_exxe)[m3] = in_exx + (exxen - in_exxee);
+            exxe = in_exx + (exxen - in_exxee);
         } else {
-            NOD(a, V2C_N_sxxe)[m3] = sxxen; sppe = sppen; NOD(a, V2C_N_exxe)[m3] = exxen;
+            sxxe = sxxen; sppe = sppen; exxe = exxen;
         }
         if (a.timesum == 0 && a.p.reloadfromDP == 0 && a.p.compressiblemode == 1) sppe = a.p.nstress;
-        NOD(a, V2C_N_sppe)[m3] = sppe;
         const double nd = Cs[1] / wc;
         NOD(a, V2C_N_nd_m2c)[m3] = nd;
-        NOD(a, V2C_N_gd)[m3] = nd / (Cs[2] / wc);
-        NOD(a, V2C_N_bm)[m3] = Cs[0] / wc;
-        NOD(a, V2C_N_dro)[m3] = 0 / wc;
-        NOD(a, V2C_N_drp)[m3] = 0 / wc;
+        gd = nd / (Cs[2] / wc);
+        bm = Cs[0] / wc;
+        dro = 0 / wc;
+        drp = 0 / wc;
     }
+    NOD(a, V2C_N_sxxen)[m3] = sxxen; NOD(a, V2C_N_sppen)[m3] = sppen; NOD(a, V2C_N_exxen)[m3] = exxen;
+    NOD(a, V2C_N_sxxe)[m3] = sxxe; NOD(a, V2C_N_sppe)[m3] = sppe; NOD(a, V2C_N_exxe)[m3] = exxe;
+    NOD(a, V2C_N_gd)[m3] = gd; NOD(a, V2C_N_bm)[m3] = bm; NOD(a, V2C_N_dro)[m3] = dro;
+    NOD(a, V2C_N_drp)[m3] = drp;
+    // velocity nodes (interpolation.c:803-824)
+    double mvx = 0, mrx = 0, mvy = 0, mry = 0;
     if (VXs[2] != 0) {
         const double mvxn = VXs[0] / VXs[1];
         NOD(a, V2C_N_mvxn)[m3] = mvxn;
-        NOD(a, V2C_N_mvx)[m3] = anti ? in_vx + (mvxn - in_mvxe) : mvxn;
-        NOD(a, V2C_N_mrx)[m3] = VXs[1] / VXs[2];
+        mvx = anti ? in_vx + (mvxn - in_mvxe) : mvxn;
+        mrx = VXs[1] / VXs[2];
     }
     if (VYs[2] != 0) {
         const double mvyn = VYs[0] / VYs[1];
         NOD(a, V2C_N_mvyn)[m3] = mvyn;
-        NOD(a, V2C_N_mvy)[m3] = anti ? in_vy + (mvyn - in_mvye) : mvyn;
-        NOD(a, V2C_N_mry)[m3] = VYs[1] / VYs[2];
+        mvy = anti ? in_vy + (mvyn - in_mvye) : mvyn;
+        mry = VYs[1] / VYs[2];
     }
+    NOD(a, V2C_N_mvx)[m3] = mvx; NOD(a, V2C_N_mrx)[m3] = mrx;
+    NOD(a, V2C_N_mvy)[m3] = mvy; NOD(a, V2C_N_mry)[m3] = mry;
+    // thermal and density fields of the basic node
+    double ro = 0, et = 0, ht = 0, kt = 0, cp = 0, tk = 0;
     if (w != 0) {
-        NOD(a, V2C_N_ro)[m3] = B[7] / w;
-        NOD(a, V2C_N_et)[m3] = B[8] / w;
-        NOD(a, V2C_N_ht)[m3] = B[9] / w;
-        NOD(a, V2C_N_kt)[m3] = B[11] / w;
-        NOD(a, V2C_N_cp)[m3] = B[10] / w;
-        NOD(a, V2C_N_tk)[m3] = B[12] / w;
+        ro = B[7] / w; et = B[8] / w; ht = B[9] / w; kt = B[11] / w; cp = B[10] / w; tk = B[12] / w;
     }
+    NOD(a, V2C_N_ro)[m3] = ro; NOD(a, V2C_N_et)[m3] = et; NOD(a, V2C_N_ht)[m3] = ht;
+    NOD(a, V2C_N_kt)[m3] = kt; NOD(a, V2C_N_cp)[m3] = cp; NOD(a, V2C_N_tk)[m3] = tk;
+    // fields zeroed and not set again here: nu, nd, and nodes2markers' sums
+    static constexpr int zero_only[] = {
+        V2C_N_nu, V2C_N_nd, V2C_N_wtphisum, V2C_N_phisum, V2C_N_sxysum, V2C_N_exysum, V2C_N_esum,
+        V2C_N_wtsxxsum, V2C_N_psum, V2C_N_sxxsum, V2C_N_exxsum, V2C_N_wtvxsum, V2C_N_vxsum,
+        V2C_N_wtvysum, V2C_N_vysum, V2C_N_mrx0, V2C_N_mry0};
+    for (int z = 0; z < (int)(sizeof(zero_only) / sizeof(zero_only[0])); z++) NOD(a, zero_only[z])[m3] = 0.0;
 }
 
 __global__ __launch_bounds__(256) void k_m2n_nodes(KArgs a)
