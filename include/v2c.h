@@ -431,11 +431,19 @@ typedef struct v2c_lu_stats {
 } v2c_lu_stats;
 
 /* Per-marker RSF inputs: fault flag, slip rate V, state phi and the marker's
- * own a, b, L, V0, mu0 (all length n, host arrays). */
+ * own a, b, L, V0, mu0 (all length n, host arrays).  The fault markers' ids
+ * are kept as a list: the slip-rate solve runs over them alone. */
 int v2c_rsf_upload(v2c_ctx *ctx, const unsigned char *fault, const double *V, const double *phi,
                    const double *a, const double *b, const double *L, const double *V0,
                    const double *mu0, long n);
-/* warm: 0 plain start, 1 MLP start (v2c_mlp_predict must have run). */
+/* warm: 0 default starts; 1 the stored guesses (v2c_mlp_predict or
+ * v2c_local_set_guess) for both solves; 2 the slip-rate guess evaluated by
+ * the network inside the slip-rate solve (v2c_mlp_set must have run; the
+ * ductile solve starts from its default, where it needs about one Newton
+ * step).  The ductile solve runs over all markers, the slip-rate solve over
+ * the fault markers on the context's second stream beside it; their kernel
+ * times are V2C_K_LOCAL and V2C_K_LOCAL_RSF, the whole call's device time
+ * (fork to join) V2C_K_LOCAL_WALL. */
 int v2c_local_update(v2c_ctx *ctx, double dt, int warm, double tol, v2c_lu_stats *stats);
 /* Outputs of the last local update (any pointer may be NULL). */
 int v2c_local_download(v2c_ctx *ctx, double *eta, double *V, double *phi, double *eta_vp,
@@ -447,12 +455,13 @@ int v2c_local_set_guess(v2c_ctx *ctx, const double *eta, const double *V, long n
 int v2c_local_get_guess(v2c_ctx *ctx, double *eta, double *V, long n);
 
 /* ---- learned warm start (no reference counterpart) ------------------------
- * 6 -> 64 -> 64 -> 2 tanh MLP, BF16 MFMA with FP32 accumulation.  It predicts
+ * 9 -> 64 -> 64 -> 2 tanh MLP, BF16 MFMA with FP32 accumulation.  It predicts
  * log10(root / default start) of the ductile unknown and of the fault slip
  * rate from standardised (log10 strain-rate invariant, T, P, log10 stress
- * invariant, log10 theta, a-b).  params: 4738 FP32 values, weights [out][in]
- * in the order W1 b1 W2 b2 W3 b3; norm: in_mean[6] in_std[6] out_mean[2]
- * out_std[2]. */
+ * invariant, log10 theta, a-b, and for fault markers log10 of the slip-rate
+ * default start, log10 L, log10 of the previous slip rate).  params: 4930
+ * FP32 values, weights [out][in] in the order W1 b1 W2 b2 W3 b3; norm:
+ * in_mean[9] in_std[9] out_mean[2] out_std[2]. */
 int v2c_mlp_set(v2c_ctx *ctx, const float *params, const float *norm);
 int v2c_mlp_get(v2c_ctx *ctx, float *params, float *norm);
 /* Write warm-start guesses for every resident marker (inputs of dt). */
@@ -463,10 +472,10 @@ int v2c_mlp_predict(v2c_ctx *ctx, double dt);
  * outputs = 2 the cost follows the fault markers. */
 int v2c_mlp_predict_outputs(v2c_ctx *ctx, double dt, int outputs);
 /* Training samples from one plain local update of the resident markers:
- * X[n][8] = 6 raw features + 2 labels, mask[n][2] (1 where a label exists).
+ * X[n][11] = 9 raw features + 2 labels, mask[n][2] (1 where a label exists).
  * Runs the plain local update (state advances as in v2c_local_update). */
 int v2c_mlp_samples(v2c_ctx *ctx, double dt, double tol, float *X, float *mask);
-/* Train on standardised Xs[ns][6], Ts[ns][2] with masks Ms[ns][2]; batch is a
+/* Train on standardised Xs[ns][9], Ts[ns][2] with masks Ms[ns][2]; batch is a
  * multiple of 64; loss_out[epochs] receives each epoch's mean masked MSE. */
 int v2c_mlp_train(v2c_ctx *ctx, const float *Xs, const float *Ts, const float *Ms, long ns, int epochs,
                   int batch, float lr, unsigned seed, float *loss_out);
@@ -481,7 +490,7 @@ int v2c_mlp_forward(v2c_ctx *ctx, const float *Xs, long ns, float *Ys);
 enum v2c_kernel_id {
     V2C_K_M2N_CELLS = 0, V2C_K_M2N_NODES, V2C_K_N2M_CELLS, V2C_K_N2M_NODES,
     V2C_K_ADVECT, V2C_K_SORT, V2C_K_BEFORE, V2C_K_AFTER, V2C_K_LOCAL, V2C_K_MLP,
-    V2C_K_M2N_PROPS, V2C_K_N2M_UPDATE, V2C_K_POST,
+    V2C_K_M2N_PROPS, V2C_K_N2M_UPDATE, V2C_K_POST, V2C_K_LOCAL_RSF, V2C_K_LOCAL_WALL,
     V2C_K_COUNT
 };
 int v2c_enable_timing(v2c_ctx *ctx, int on);

@@ -8,11 +8,16 @@ import pytest
 
 from velo2cycles_amd import synth
 from velo2cycles_amd.abi import Context
-from velo2cycles_amd.warmstart import (WarmStart, evaluate, generate, init_params, reference_grad,
-                                       upload_cloud)
+from velo2cycles_amd.warmstart import (IN, NPARAM, OFF, ROW, WarmStart, evaluate, generate,
+                                       init_params, reference_grad, upload_cloud)
 from tests.util import rel_errors
 
 pytestmark = pytest.mark.gpu
+
+
+def _unit_norm():
+    """Normalisation block that leaves inputs and outputs unchanged."""
+    return np.concatenate([np.zeros(IN), np.ones(IN), np.zeros(2), np.ones(2)]).astype(np.float32)
 
 
 def _ctx(lib, cfg, cap=None):
@@ -21,7 +26,7 @@ def _ctx(lib, cfg, cap=None):
 
 def _batch(n, seed):
     rng = np.random.default_rng(seed)
-    Xs = rng.normal(size=(n, 6)).astype(np.float32)
+    Xs = rng.normal(size=(n, IN)).astype(np.float32)
     Ts = rng.normal(size=(n, 2)).astype(np.float32)
     Ms = (rng.uniform(size=(n, 2)) > 0.25).astype(np.float32)
     return Xs, Ts, Ms
@@ -32,7 +37,7 @@ def test_mfma_forward_matches_reference(gpu):
     ctx = _ctx(gpu, cfg, 1)
     ws = WarmStart(ctx)
     p = init_params(5) * 1.5
-    ws.load(p, np.concatenate([np.zeros(6), np.ones(6), np.zeros(2), np.ones(2)]).astype(np.float32))
+    ws.load(p, _unit_norm())
     Xs, _, _ = _batch(1000, 1)
     Y = ws.forward(Xs)
     _, _, Yr = reference_grad(p, Xs, np.zeros((1000, 2)), np.ones((1000, 2)))
@@ -46,7 +51,7 @@ def test_mfma_gradient_matches_fp64_reference(gpu):
     ctx = _ctx(gpu, cfg, 1)
     ws = WarmStart(ctx)
     p = init_params(9) * 1.2
-    ws.load(p, np.zeros(16, dtype=np.float32) + np.repeat([0, 1, 0, 1], [6, 6, 2, 2]).astype(np.float32))
+    ws.load(p, _unit_norm())
     Xs, Ts, Ms = _batch(4096, 2)
     g, loss = ws.grad(Xs, Ts, Ms)
     gr, lr_, _ = reference_grad(p, Xs, Ts, Ms, emulate_bf16=True)
@@ -57,7 +62,9 @@ def test_mfma_gradient_matches_fp64_reference(gpu):
     assert rel_exact < 5e-2, rel_exact
     assert abs(loss - lr_) <= 1e-2 * lr_
     # every parameter block carries gradient
-    for a, b in ((0, 384), (384, 448), (448, 4544), (4544, 4608), (4608, 4736), (4736, 4738)):
+    keys = ("W1", "b1", "W2", "b2", "W3", "b3")
+    for k, key in enumerate(keys):
+        a, b = OFF[key], (OFF[keys[k + 1]] if k + 1 < len(keys) else NPARAM)
         assert np.abs(g[a:b]).max() > 0
     ctx.close()
 
@@ -66,10 +73,10 @@ def test_training_is_deterministic_and_learns(gpu):
     cfg = synth.config_edge()
     rng = np.random.default_rng(4)
     n = 20000
-    Xs = rng.normal(size=(n, 6)).astype(np.float32)
+    Xs = rng.normal(size=(n, IN)).astype(np.float32)
     Ts = np.stack([np.sin(Xs[:, 0]) + 0.5 * Xs[:, 1], Xs[:, 2] * Xs[:, 3] * 0.3], 1).astype(np.float32)
     Ms = np.ones((n, 2), dtype=np.float32)
-    norm = np.concatenate([np.zeros(6), np.ones(6), np.zeros(2), np.ones(2)]).astype(np.float32)
+    norm = _unit_norm()
     runs = []
     for _ in range(2):
         ctx = _ctx(gpu, cfg, 1)
@@ -114,7 +121,7 @@ def test_inference_kernel_matches_forward(gpu):
     upload_cloud(ctx, cfg)
     ws = WarmStart(ctx)
     n = cfg.n
-    Xa = np.zeros((n, 8), dtype=np.float32)
+    Xa = np.zeros((n, ROW), dtype=np.float32)
     Ma = np.zeros((n, 2), dtype=np.float32)
     fp = C.POINTER(C.c_float)
     ctx._check(ctx.lib.v2c_mlp_samples(ctx.h, dt, 1e-10, Xa.ctypes.data_as(fp), Ma.ctypes.data_as(fp)),
@@ -129,7 +136,7 @@ def test_inference_kernel_matches_forward(gpu):
         ws.predict(dt)
         g = ctx.local_get_guess(n)
         Y = ws.forward(Xs).astype(np.float64)
-        r = Y * ws.norm[14:16] + ws.norm[12:14]
+        r = Y * ws.norm[2 * IN + 2:2 * IN + 4] + ws.norm[2 * IN:2 * IN + 2]
         out.append((g, r))
     for o in range(2):
         g1, g2 = out[0][0][o], out[1][0][o]
@@ -144,10 +151,15 @@ def test_inference_kernel_matches_forward(gpu):
 
 def test_coseismic_warm_start_cuts_rsf_iterations(gpu):
     """Fault markers slipping at 1e-2..10 m/s (dt = 0.01 s): the network
-    trained on the method's own coseismic solves cuts the slip-rate Newton
-    iterations by at least a quarter (about a third measured), roots unchanged
-    within tolerance; with only the slip-rate guess selected, ductile guesses
-    are zero and the ductile solves take exactly the plain iterations."""
+    trained on the method's own coseismic solves (16 clouds here) cuts the
+    slip-rate Newton iterations per fault marker to at most 0.52 of plain
+    (2.0x measured; 2.2x with the 64-cloud training of the benchmark), roots
+    unchanged within tolerance, whether the guesses are predicted beforehand
+    ("warm") or by the network inside the slip-rate solve ("warm_rsf").
+    With the guess inside the solve, the ductile solves take exactly the plain
+    iterations, and the local update's device time (the slip-rate solve runs
+    beside the ductile solve of every marker) stays within a quarter of
+    plain's (about equal measured; best of three runs each)."""
     dt = 1e-2
     maker = lambda seed: synth.coseismic(synth.config_small(seed=seed), seed=seed, dt=dt)  # noqa: E731
     cfg = maker(synth.DEFAULT_SEED)
@@ -158,7 +170,7 @@ def test_coseismic_warm_start_cuts_rsf_iterations(gpu):
     ws.fit_norm(X, M)
     ws.load(init_params(0))
     ws.train(X, M, epochs=16, batch=4096, lr=2e-3)
-    res = evaluate(ctx, cfg, ws, dt, repeats=1, modes=("plain", "warm", "warm_rsf"))
+    res = evaluate(ctx, cfg, ws, dt, repeats=3, modes=("plain", "warm", "warm_rsf"))
     pl, wm, wr = res["plain"], res["warm"], res["warm_rsf"]
     fault = cfg.rsf["fault"].astype(bool)
     for mode in (wm, wr):
@@ -167,9 +179,11 @@ def test_coseismic_warm_start_cuts_rsf_iterations(gpu):
             assert nm == 0 and mx <= 1e-9, (k, mx)
         assert mode["stats"]["nonfinite"] == 0
         print("rsf iterations", pl["stats"]["rsf_iters"], mode["stats"]["rsf_iters"], flush=True)
-        assert 4 * mode["stats"]["rsf_iters"] <= 3 * pl["stats"]["rsf_iters"], (mode["stats"]["rsf_iters"],
-                                                                                pl["stats"]["rsf_iters"])
+        assert mode["stats"]["rsf_iters"] <= 0.52 * pl["stats"]["rsf_iters"], (mode["stats"]["rsf_iters"],
+                                                                               pl["stats"]["rsf_iters"])
     assert wr["stats"]["ductile_iters"] == pl["stats"]["ductile_iters"]
+    print("wall ms", pl["ms_wall"], wr["ms_wall"], flush=True)
+    assert wr["ms_wall"] <= 1.25 * pl["ms_wall"], (wr["ms_wall"], pl["ms_wall"])
     upload_cloud(ctx, cfg)
     ws.predict(dt, 2)
     geta, gv = ctx.local_get_guess(cfg.n)

@@ -2,13 +2,13 @@
 central finite differences (CPU)."""
 import numpy as np
 
-from velo2cycles_amd.warmstart import NPARAM, init_params, reference_grad
+from velo2cycles_amd.warmstart import IN, NPARAM, init_params, reference_grad
 
 
 def test_reference_gradient_matches_finite_differences():
     rng = np.random.default_rng(0)
     p = init_params(3).astype(np.float64) * 1.3
-    Xs = rng.normal(size=(50, 6))
+    Xs = rng.normal(size=(50, IN))
     Ts = rng.normal(size=(50, 2))
     Ms = (rng.uniform(size=(50, 2)) > 0.3).astype(np.float64)
     g, loss, _ = reference_grad(p, Xs, Ts, Ms, emulate_bf16=False)

@@ -40,7 +40,8 @@ FIELD_ID = {n: i for i, n in enumerate(NODAL_FIELDS)}
 MARKER_FIELDS = ("x y k p xx xy exx exy e phi vx vy C d w a b L L0 V0 fric_static C0").split()
 
 KERNELS = ("m2n_cells", "m2n_nodes", "n2m_cells", "n2m_nodes", "advect", "sort",
-           "before", "after", "local", "mlp", "m2n_props", "n2m_update", "post")
+           "before", "after", "local", "mlp", "m2n_props", "n2m_update", "post", "local_rsf",
+           "local_wall")
 
 ERRORS = {0: "ok", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECAPACITY",
           -5: "EUNSUPPORTED", -6: "ENOGPU"}
@@ -449,7 +450,9 @@ class Context:
         self._check(self.lib.v2c_rsf_upload(self.h, C.cast(f8.ctypes.data, C.c_char_p),
                                             *[_dptr(x) for x in arrs], n), "rsf_upload")
 
-    def local_update(self, dt, warm=False, tol=1e-10):
+    def local_update(self, dt, warm=0, tol=1e-10):
+        """warm: 0 default starts, 1 stored guesses, 2 network guess for the
+        slip rate inside the solve."""
         st = LUStats()
         self._check(self.lib.v2c_local_update(self.h, dt, int(warm), tol, C.byref(st)), "local_update")
         return st.as_dict()

@@ -10,7 +10,7 @@
 #include <vector>
 
 #include "v2c_kernels.hip"
-#include "v2c_mlp.hip"
+#include "v2c_local.hip"
 
 using namespace v2c;
 
@@ -63,6 +63,8 @@ struct v2c_ctx {
     // local update
     double *d_mx = nullptr;
     uint8_t *d_fault = nullptr;
+    uint32_t *d_flist = nullptr;  // fault markers' ids, in increasing order
+    long nfault = 0;
     uint32_t *d_defer = nullptr;  // markers the fast fused update hands to the general one
     uint8_t *d_fclass = nullptr;  // constant friction set each marker's fields hold (friction_marker)
     unsigned long long *d_lu = nullptr;
@@ -180,6 +182,10 @@ static KArgs make_args(v2c_ctx *c)
     setp(a.mx, c->d_mx);
     setp(a.fault, c->d_fault);
     setp(a.lu_stats, c->d_lu);
+    setp(a.fault_list, c->d_flist);
+    a.nfault = c->nfault;
+    setp(a.mlp, c->d_mlp);
+    setp(a.mlp_norm, c->d_mlp_norm);
     return a;
 }
 
@@ -347,7 +353,7 @@ static void free_all(v2c_ctx *c)
                     c->d_order2, c->d_skey2, c->d_tile_count, c->d_tile_off, c->d_mask,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
                     c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm, c->d_post, c->d_bondm, c->d_bondn,
-                    c->d_bondv, c->d_bcx, c->d_bcflag, c->d_defer, c->d_fclass};
+                    c->d_bondv, c->d_bcx, c->d_bcflag, c->d_defer, c->d_fclass, c->d_flist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
@@ -432,6 +438,7 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_mx, sizeof(double) * NMX * c->cap));
     CK(hipMalloc(&c->d_fault, c->cap));
     CK(hipMalloc(&c->d_defer, sizeof(uint32_t) * c->cap));
+    CK(hipMalloc(&c->d_flist, sizeof(uint32_t) * c->cap));
     CK(hipMalloc(&c->d_fclass, c->cap));
     CK(hipMemsetAsync(c->d_fclass, 0, c->cap, c->stream));
     CK(hipMalloc(&c->d_lu, sizeof(unsigned long long) * LS_COUNT));
@@ -879,7 +886,14 @@ int v2c_rsf_upload(v2c_ctx *c, const unsigned char *fault, const double *V, cons
         HIPCHK(c, hipMemcpyAsync(c->d_mx + (size_t)dst[k] * c->cap, src[k], sizeof(double) * n,
                                  hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_fault, fault, n, hipMemcpyHostToDevice, c->stream));
+    std::vector<uint32_t> list;
+    for (long i = 0; i < n; i++)
+        if (fault[i]) list.push_back((uint32_t)i);
+    if (!list.empty())
+        HIPCHK(c, hipMemcpyAsync(c->d_flist, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice,
+                                 c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nfault = (long)list.size();
     return 0;
 }
 
@@ -909,17 +923,52 @@ int v2c_local_get_guess(v2c_ctx *c, double *eta, double *V, long n)
 
 int v2c_local_update(v2c_ctx *c, double dt, int warm, double tol, v2c_lu_stats *st)
 {
-    if (!c || !(tol > 0)) return V2C_EINVAL;
+    if (!c || !(tol > 0) || warm < 0 || warm > 2) return V2C_EINVAL;
+    if (warm == 2 && !c->mlp_ready) return set_err(c, V2C_EINVAL, "no MLP parameters loaded (v2c_mlp_set)");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemsetAsync(c->d_lu, 0, sizeof(unsigned long long) * LS_COUNT, c->stream));
     KArgs a = make_args(c);
     int rc = 0;
-    if (c->n > 0) {
+    // The slip-rate solve (fault markers, with the network for warm == 2)
+    // runs on the side stream beside the ductile solve of every marker; the
+    // visco-plastic viscosity that needs both follows the join.  The region
+    // from the fork to the last kernel is timed as V2C_K_LOCAL_WALL.
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (c->timing) {
+        t0 = take_event(c);
+        t1 = take_event(c);
+        if (t0) (void)hipEventRecord(t0, c->stream);
+    }
+    const bool side = c->nfault > 0;
+    if (side) {
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        const unsigned nb = blocks_for(c->nfault, LU_BLOCK);
+        rc = launch(c, V2C_K_LOCAL_RSF, [&] {
+            if (warm == 2) hipLaunchKernelGGL(k_local_rsf<2>, dim3(nb), dim3(LU_BLOCK), 0, c->side, a, dt, tol);
+            else if (warm == 1) hipLaunchKernelGGL(k_local_rsf<1>, dim3(nb), dim3(LU_BLOCK), 0, c->side, a, dt, tol);
+            else hipLaunchKernelGGL(k_local_rsf<0>, dim3(nb), dim3(LU_BLOCK), 0, c->side, a, dt, tol);
+        }, c->side);
+        const hipError_t ej = hipEventRecord(c->ev_join, c->side);
+        if (!rc && ej != hipSuccess) rc = set_err(c, V2C_EHIP, "hipEventRecord: %s", hipGetErrorString(ej));
+    }
+    if (!rc && c->n > 0)
         rc = launch(c, V2C_K_LOCAL, [&] {
-            hipLaunchKernelGGL(k_local_update, dim3(blocks_for(c->n, LU_BLOCK)), dim3(LU_BLOCK), 0, c->stream, a,
-                               dt, warm, tol);
+            hipLaunchKernelGGL(k_local_ductile, dim3(blocks_for(c->n, LU_BLOCK)), dim3(LU_BLOCK), 0, c->stream, a,
+                               warm, tol);
+        });
+    // join before any return, so no side-stream work is left unordered
+    if (side) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (rc) return rc;
+    if (side) {
+        rc = launch(c, V2C_K_LOCAL, [&] {
+            hipLaunchKernelGGL(k_local_etavp, dim3(blocks_for(c->nfault, LU_BLOCK)), dim3(LU_BLOCK), 0, c->stream, a);
         });
         if (rc) return rc;
+    }
+    if (t0 && t1) {
+        (void)hipEventRecord(t1, c->stream);
+        c->pending.push_back(TimedLaunch{V2C_K_LOCAL_WALL, t0, t1});
     }
     rc = finish(c);
     if (rc) return rc;
@@ -1010,7 +1059,7 @@ int v2c_mlp_samples(v2c_ctx *c, double dt, double tol, float *X, float *mask)
     const long n = c->n;
     if (n == 0) return 0;
     float *dX = nullptr, *dM = nullptr;
-    HIPCHK(c, hipMalloc(&dX, sizeof(float) * 8 * n));
+    HIPCHK(c, hipMalloc(&dX, sizeof(float) * MLP_ROW * n));
     hipError_t e = hipMalloc(&dM, sizeof(float) * 2 * n);
     if (e != hipSuccess) { (void)hipFree(dX); return set_err(c, V2C_ENOMEM, "mlp_samples"); }
     KArgs a = make_args(c);
@@ -1022,7 +1071,7 @@ int v2c_mlp_samples(v2c_ctx *c, double dt, double tol, float *X, float *mask)
         hipLaunchKernelGGL(k_mlp_labels, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a, dX, dM);
         e = hipGetLastError();
     }
-    if (e == hipSuccess && rc == 0) e = hipMemcpyAsync(X, dX, sizeof(float) * 8 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && rc == 0) e = hipMemcpyAsync(X, dX, sizeof(float) * MLP_ROW * n, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && rc == 0) e = hipMemcpyAsync(mask, dM, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, c->stream);
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(dX);

@@ -92,6 +92,9 @@ struct KArgs {
     V2C_GLOBAL double *mx;                 // NMX extension fields x cap
     V2C_GLOBAL const uint8_t *fault;       // per-marker fault flag
     V2C_GLOBAL unsigned long long *lu_stats;  // local-update statistics
+    V2C_GLOBAL const uint32_t *fault_list;  // the fault markers' ids (nfault)
+    long nfault;
+    V2C_GLOBAL const float *mlp, *mlp_norm; // warm-start network: FP32 parameters, normalisation
 };
 
 __device__ __forceinline__ double *NOD(const KArgs &a, int f) { return a.nod + (long)f * a.nodenum; }

@@ -1617,78 +1617,6 @@ __global__ void k_ductile_batch(KArgs a, long n, const int8_t *t, const double *
     iters[i] = it;
 }
 
-// ---------------------------------------------------------------------------
-// Build-defined per-marker local update: ductile Newton to |f|/eta <= tol for
-// every marker, RSF slip-rate Newton for fault markers.  Statistics go to
-// lu_stats with integer atomics (order independent).
-// ---------------------------------------------------------------------------
-constexpr int LU_BLOCK = 256;
-constexpr int LU_HIST = 32;
-enum { LS_N = 0, LS_DUCT, LS_DUCT_IT, LS_FAULT, LS_RSF_IT, LS_DUCT_MAX, LS_RSF_MAX, LS_FALLBACK,
-       LS_NONFINITE, LS_DUCT_EV, LS_RSF_EV, LS_HIST_D, LS_HIST_R = LS_HIST_D + LU_HIST,
-       LS_COUNT = LS_HIST_R + LU_HIST };
-
-__global__ __launch_bounds__(LU_BLOCK) void k_local_update(KArgs a, double dt, int warm, double tol)
-{
-    __shared__ unsigned long long sh[LS_COUNT];
-    for (int i = threadIdx.x; i < LS_COUNT; i += LU_BLOCK) sh[i] = 0;
-    __syncthreads();
-    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < a.n) {
-        const int t = a.t[m];
-        const double sxx = MK(a, MF_XX, m), sxy = MK(a, MF_XY, m);
-        const double tk = MK(a, MF_K, m), p = MK(a, MF_P, m);
-        int it = 0, fb = 0, ev = 0;
-        double root_eta;
-        const double eta = ductile_newton(a, t, sxy, sxx, tk, 1e-5 * p, warm ? MXF(a, MX_GETA, m) : 0.0, tol,
-                                          &it, &ev, &fb, &root_eta);
-        MXF(a, MX_ETA, m) = eta;
-        MXF(a, MX_ROOT_ETA, m) = root_eta;
-        atomicAdd(&sh[LS_N], 1ull);
-        unsigned long long bad = isfinite(eta) ? 0 : 1;
-        if (power_law_type(a.mat, t) && t != 1) {
-            atomicAdd(&sh[LS_DUCT], 1ull);
-            atomicAdd(&sh[LS_DUCT_IT], (unsigned long long)it);
-            atomicAdd(&sh[LS_DUCT_EV], (unsigned long long)ev);
-            atomicMax(&sh[LS_DUCT_MAX], (unsigned long long)it);
-            atomicAdd(&sh[LS_HIST_D + (it < LU_HIST ? it : LU_HIST - 1)], 1ull);
-        }
-        if (a.fault[m]) {
-            const double exx = MK(a, MF_EXX, m), exy = MK(a, MF_EXY, m);
-            const double sig = pow(sxx * sxx + sxy * sxy, 0.5);
-            const double nP = a.p.pmode == 0 ? a.p.nstress : p;
-            const double EII = pow(exx * exx + exy * exy, 0.5);
-            const double eta_rad = a.p.inertyn == 0 ? a.p.rad_damping : 0.0;
-            const double W = a.p.faultwidth;
-            double V, phin, root_v;
-            int rfb = 0, rev = 0;
-            const int rit = rsf_newton(sig, nP, MXF(a, MX_A, m), MXF(a, MX_B, m), MXF(a, MX_L, m), MXF(a, MX_V0, m),
-                                       MXF(a, MX_MU0, m), MXF(a, MX_PHI, m), MXF(a, MX_V, m), dt, eta_rad, EII,
-                                       W, warm ? MXF(a, MX_GV, m) : 0.0, tol, V, phin, rfb, root_v, rev);
-            MXF(a, MX_ROOT_V, m) = root_v;
-            MXF(a, MX_V, m) = V;
-            MXF(a, MX_PHI, m) = phin;
-            MXF(a, MX_SY, m) = sig;
-            MXF(a, MX_ETAVP, m) = sig > 0 ? eta * sig / (2 * eta * V / W + sig) : eta;
-            fb |= rfb;
-            if (!isfinite(V) || !isfinite(phin)) bad = 1;
-            atomicAdd(&sh[LS_FAULT], 1ull);
-            atomicAdd(&sh[LS_RSF_IT], (unsigned long long)rit);
-            atomicAdd(&sh[LS_RSF_EV], (unsigned long long)rev);
-            atomicMax(&sh[LS_RSF_MAX], (unsigned long long)rit);
-            atomicAdd(&sh[LS_HIST_R + (rit < LU_HIST ? rit : LU_HIST - 1)], 1ull);
-        }
-        if (fb) atomicAdd(&sh[LS_FALLBACK], 1ull);
-        if (bad) atomicAdd(&sh[LS_NONFINITE], 1ull);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < LS_COUNT; i += LU_BLOCK) {
-        if (!sh[i]) continue;
-        if (i == LS_DUCT_MAX || i == LS_RSF_MAX) atomicMax(&a.lu_stats[i], sh[i]);
-        else atomicAdd(&a.lu_stats[i], sh[i]);
-    }
-}
-
 // Plain streaming copy of known size (calibrates the HBM byte counters).
 __global__ void k_copy_calib(const double *__restrict__ src, double *__restrict__ dst, long n)
 {

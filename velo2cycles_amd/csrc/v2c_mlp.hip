@@ -1,11 +1,15 @@
 // v2c_mlp.hip -- learned warm start of the per-marker local update.
 //
-// A 6 -> 64 -> 64 -> 2 tanh MLP proposes, per marker, the log10 ratio of the
+// A 9 -> 64 -> 64 -> 2 tanh MLP proposes, per marker, the log10 ratio of the
 // Newton root to the analytic default start for (i) the ductile unknown
 // (dislocation viscosity; default eta_disl0 of density_ductilevisc_calc.c:159)
 // and (ii) the fault slip rate (default min(undamped closed form
 // calc_RSF.c:192, sigII/eta_rad)).  Inputs: log10 strain-rate invariant, T, P,
-// log10 stress invariant, log10 theta, a-b, standardised.
+// log10 stress invariant, log10 theta, a-b, and for fault markers the log10 of
+// the slip-rate default start, log10 L and log10 of the previous slip rate,
+// standardised.  (The last three fix where the default start sits relative to
+// the root: it follows the aged state, which depends on L and the previous
+// rate.)
 //
 // Forward and backward run on the matrix cores: v_mfma_f32_16x16x16_bf16 with
 // BF16 operands and FP32 accumulation.  One wavefront owns 64 rows (markers or
@@ -21,31 +25,26 @@ namespace v2c {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int MLP_IN = 6, MLP_H = 64, MLP_OUT = 2;
+constexpr int MLP_IN = 9, MLP_H = 64, MLP_OUT = 2;
 // flat FP32 parameter layout, weights stored [out][in]
 constexpr int P_W1 = 0, P_B1 = P_W1 + MLP_H * MLP_IN, P_W2 = P_B1 + MLP_H, P_B2 = P_W2 + MLP_H * MLP_H,
               P_W3 = P_B2 + MLP_H, P_B3 = P_W3 + MLP_OUT * MLP_H, MLP_NPARAM = P_B3 + MLP_OUT;
-static_assert(MLP_NPARAM == 4738, "parameter count");
-// normalisation block: in_mean[6] in_std[6] out_mean[2] out_std[2]
-constexpr int MLP_NNORM = 16;
+static_assert(MLP_NPARAM == 4930, "parameter count");
+static_assert(MLP_IN <= 16, "layer 1 pads the inputs to one 16-wide MFMA step");
+// normalisation block: in_mean[MLP_IN] in_std[MLP_IN] out_mean[2] out_std[2]
+constexpr int MLP_NNORM = 2 * MLP_IN + 4;
 
-__device__ __forceinline__ short f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (short)(u >> 16);   // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);                                     // round to nearest even
-    return (short)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ short f2bf(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
 
-// tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2|x|), with the hardware
-// reciprocal (about 1 ulp) in place of an IEEE division.
+// tanh(x) = 1 - 2 / (exp(2x) + 1): one exponential and one hardware
+// reciprocal (about 1 ulp); saturates to +-1 through exp's overflow and
+// underflow.  Absolute error near 1e-7, ample for a guess network (its
+// training uses the same function).
 __device__ __forceinline__ float fast_tanh(float x)
 {
-    const float ax = fabsf(x);
-    const float e = __expf(-2.0f * ax);
-    const float t = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
-    return copysignf(t, x);
+    return fmaf(-2.0f, __builtin_amdgcn_rcpf(__expf(2.0f * x) + 1.0f), 1.0f);
 }
 
 // One 16x16 output tile, K = 16*ksteps.  A(i,k) and B(k,n) are fetched by
@@ -140,7 +139,7 @@ __device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, 
         o.eta0 = 0.5 * sig * M->nu[t] * e1 / pow(sig, n);
         o.duct = o.eta0 > 0 && isfinite(o.eta0);
     }
-    double theta_l10 = 0.0, amb = 0.0;
+    double theta_l10 = 0.0, amb = 0.0, vhi_l10 = 0.0, L_l10 = 0.0, vp_l10 = 0.0;
     if (a.fault[m]) {
         const double ra = MXF(a, MX_A, m), rb = MXF(a, MX_B, m), rL = MXF(a, MX_L, m), rV0 = MXF(a, MX_V0, m);
         const double mu0 = MXF(a, MX_MU0, m), phi = MXF(a, MX_PHI, m), Vp = MXF(a, MX_V, m);
@@ -154,9 +153,12 @@ __device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, 
             if (sig / eta_rad < vhi) vhi = sig / eta_rad;
             o.vhi = vhi;
             o.rsf = vhi > 0 && isfinite(vhi);
+            if (o.rsf) vhi_l10 = log10(vhi);
         }
         theta_l10 = (phi + log(rL / rV0)) / 2.302585092994046;   // theta = L e^phi / V0
         amb = ra - rb;
+        L_l10 = log10(rL);
+        vp_l10 = log10(Vp > 1e-40 ? Vp : 1e-40);
     }
     o.f[0] = (float)log10(eii > 1e-30 ? eii : 1e-30);
     o.f[1] = (float)tk;
@@ -164,27 +166,119 @@ __device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, 
     o.f[3] = (float)log10(sig > 1e-30 ? sig : 1e-30);
     o.f[4] = (float)theta_l10;
     o.f[5] = (float)amb;
+    o.f[6] = (float)vhi_l10;
+    o.f[7] = (float)L_l10;
+    o.f[8] = (float)vp_l10;
 }
 
-// Inference: standardised features -> guesses written to MX_GETA / MX_GV
-// (0 where a solve does not apply, which the local update treats as "no
-// guess").  Four wavefronts per block share the weights, converted to BF16
-// once per block into LDS in the layout the B operand reads (4 consecutive k
-// per lane); each wavefront keeps its 64 rows of activations as BF16 in LDS,
-// layer 2's 16 output tiles in registers until all of layer 1 has been read.
-// The arithmetic is the training forward's (same roundings, same k order).
-constexpr int MI_WAVES = 4;
-struct MlpInferLds {
+// BF16 weights of one workgroup, staged in LDS in the layout the MFMA B
+// operand reads (4 consecutive k per lane), and the FP32 biases.
+struct MlpW {
     short w1[MLP_H][16];          // [out][in], in padded to 16
     short w2[MLP_H][MLP_H];       // [out][in]
     short w3[16][MLP_H];          // [out][in], out padded to 16
     float b1[MLP_H], b2[MLP_H], b3[MLP_OUT];
-    short x[MI_WAVES][64][16];
-    short h[MI_WAVES][64][MLP_H];
 };
+
+// All threads of the workgroup (nthreads) convert the FP32 master weights;
+// the caller synchronises before use.
+__device__ __forceinline__ void mlp_stage_weights(const float *__restrict__ P, MlpW &s, int tid, int nthreads)
+{
+    for (int e = tid; e < MLP_H * 16; e += nthreads) {
+        const int o = e >> 4, k = e & 15;
+        s.w1[o][k] = k < MLP_IN ? f2bf(P[P_W1 + o * MLP_IN + k]) : (short)0;
+    }
+    for (int e = tid; e < MLP_H * MLP_H; e += nthreads) (&s.w2[0][0])[e] = f2bf(P[P_W2 + e]);
+    for (int e = tid; e < 16 * MLP_H; e += nthreads) {
+        const int o = e / MLP_H;
+        (&s.w3[0][0])[e] = o < MLP_OUT ? f2bf(P[P_W3 + e]) : (short)0;
+    }
+    for (int e = tid; e < MLP_H; e += nthreads) { s.b1[e] = P[P_B1 + e]; s.b2[e] = P[P_B2 + e]; }
+    if (tid < MLP_OUT) s.b3[tid] = P[P_B3 + tid];
+}
 
 // Four consecutive BF16 values (8-byte aligned) as one LDS read.
 __device__ __forceinline__ bf16x4 lds_bf16x4(const short *p) { return *(const bf16x4 *)p; }
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward pass of one wavefront's 64 rows: x[lane][0..15] holds the lane's
+// standardised inputs as BF16 (zero padded); on return (y0, y1) are the lane's
+// two standardised outputs.  x and h are the wavefront's own LDS rows; the
+// weights must be staged.  The arithmetic is the training forward's (same
+// roundings, same k order).
+__device__ __forceinline__ void mlp_infer_wave(const MlpW &s, short (*x)[16], short (*h)[MLP_H], float &y0,
+                                               float &y1)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kb = 4 * (lane >> 4);
+    // layer 1: h = tanh(x W1^T + b1)
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&x[mt * 16 + i][kb]),
+                                                          lds_bf16x4(&s.w1[nt * 16 + i][kb]), d, 0, 0, 0);
+            for (int j = 0; j < 4; j++)
+                h[mt * 16 + kb + j][nt * 16 + i] = f2bf(fast_tanh(d[j] + s.b1[nt * 16 + i]));
+        }
+    wave_lds_sync();
+    // layer 2: all 16 tiles read layer 1 before any is written back
+    f32x4 d2[4][4];
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < 4; ks++)
+                d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&h[mt * 16 + i][ks * 16 + kb]),
+                                                              lds_bf16x4(&s.w2[nt * 16 + i][ks * 16 + kb]), d, 0, 0, 0);
+            d2[mt][nt] = d;
+        }
+    wave_lds_sync();
+    for (int mt = 0; mt < 4; mt++)
+        for (int nt = 0; nt < 4; nt++)
+            for (int j = 0; j < 4; j++)
+                h[mt * 16 + kb + j][nt * 16 + i] = f2bf(fast_tanh(d2[mt][nt][j] + s.b2[nt * 16 + i]));
+    wave_lds_sync();
+    // layer 3: y = h W3^T + b3; row mt*16 + kb + j, output i (< 2 used), handed
+    // to the lane owning that row through x (no longer needed)
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < 4; ks++)
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&h[mt * 16 + i][ks * 16 + kb]),
+                                                          lds_bf16x4(&s.w3[i][ks * 16 + kb]), d, 0, 0, 0);
+        for (int j = 0; j < 4; j++)
+            if (i < MLP_OUT) *(float *)&x[mt * 16 + kb + j][2 * i] = d[j] + s.b3[i];
+    }
+    wave_lds_sync();
+    y0 = *(const float *)&x[lane][0];
+    y1 = *(const float *)&x[lane][2];
+}
+
+// Standardised BF16 inputs of one lane into its LDS row.
+__device__ __forceinline__ void mlp_load_row(const MlpFeat &ft, const float *__restrict__ norm, bool live,
+                                             short *row)
+{
+    for (int k = 0; k < 16; k++) {
+        float v = 0.f;
+        if (k < MLP_IN && live) v = (ft.f[k] - norm[k]) / norm[MLP_IN + k];
+        row[k] = f2bf(v);
+    }
+}
+
+// Inference: standardised features -> guesses written to MX_GETA / MX_GV
+// (0 where a solve does not apply, which the local update treats as "no
+// guess").  Four wavefronts per block share the weights staged in LDS; each
+// wavefront keeps its 64 rows of activations as BF16 in LDS.
+constexpr int MI_WAVES = 4;
+struct MlpInferLds {
+    MlpW w;
+    short x[MI_WAVES][64][16];
+    short h[MI_WAVES][64][MLP_H];
+};
 
 // outputs: bit 0 the ductile guess, bit 1 the slip-rate guess; the other is
 // written as 0 (default start).  Markers that can need a selected guess are
@@ -215,23 +309,8 @@ __global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_guess(KArgs a, double dt,
     if (m < a.n && wave_need) mlp_features(a, m, dt, ft);
     ft.duct = ft.duct && (outputs & 1);
     ft.rsf = ft.rsf && (outputs & 2);
-    for (int e = tid; e < MLP_H * 16; e += 64 * MI_WAVES) {
-        const int o = e >> 4, k = e & 15;
-        s.w1[o][k] = k < MLP_IN ? f2bf(P[P_W1 + o * MLP_IN + k]) : (short)0;
-    }
-    for (int e = tid; e < MLP_H * MLP_H; e += 64 * MI_WAVES) (&s.w2[0][0])[e] = f2bf(P[P_W2 + e]);
-    for (int e = tid; e < 16 * MLP_H; e += 64 * MI_WAVES) {
-        const int o = e / MLP_H;
-        (&s.w3[0][0])[e] = o < MLP_OUT ? f2bf(P[P_W3 + e]) : (short)0;
-    }
-    for (int e = tid; e < MLP_H; e += 64 * MI_WAVES) { s.b1[e] = P[P_B1 + e]; s.b2[e] = P[P_B2 + e]; }
-    if (tid < MLP_OUT) s.b3[tid] = P[P_B3 + tid];
-
-    for (int k = 0; k < 16; k++) {
-        float v = 0.f;
-        if (k < MLP_IN && m < a.n) v = (ft.f[k] - norm[k]) / norm[MLP_IN + k];
-        s.x[w][lane][k] = f2bf(v);
-    }
+    mlp_stage_weights(P, s.w, tid, 64 * MI_WAVES);
+    mlp_load_row(ft, norm, m < a.n, s.x[w][lane]);
     __syncthreads();
     if (!wave_need) {
         if (m < a.n) {
@@ -240,60 +319,8 @@ __global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_guess(KArgs a, double dt,
         }
         return;
     }
-    const int i = lane & 15, kb = 4 * (lane >> 4);
-    short(*h)[MLP_H] = s.h[w];
-    // layer 1: h = tanh(x W1^T + b1)
-    for (int mt = 0; mt < 4; mt++)
-        for (int nt = 0; nt < 4; nt++) {
-            f32x4 d = {0.f, 0.f, 0.f, 0.f};
-            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&s.x[w][mt * 16 + i][kb]),
-                                                          lds_bf16x4(&s.w1[nt * 16 + i][kb]), d, 0, 0, 0);
-            for (int j = 0; j < 4; j++)
-                h[mt * 16 + kb + j][nt * 16 + i] = f2bf(fast_tanh(d[j] + s.b1[nt * 16 + i]));
-        }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // layer 2: all 16 tiles read layer 1 before any is written back
-    f32x4 d2[4][4];
-    for (int mt = 0; mt < 4; mt++)
-        for (int nt = 0; nt < 4; nt++) {
-            f32x4 d = {0.f, 0.f, 0.f, 0.f};
-            for (int ks = 0; ks < 4; ks++)
-                d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&h[mt * 16 + i][ks * 16 + kb]),
-                                                              lds_bf16x4(&s.w2[nt * 16 + i][ks * 16 + kb]), d, 0, 0, 0);
-            d2[mt][nt] = d;
-        }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int mt = 0; mt < 4; mt++)
-        for (int nt = 0; nt < 4; nt++)
-            for (int j = 0; j < 4; j++)
-                h[mt * 16 + kb + j][nt * 16 + i] = f2bf(fast_tanh(d2[mt][nt][j] + s.b2[nt * 16 + i]));
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // layer 3: y = h W3^T + b3; row mt*16 + kb + j, output i (< 2 used)
-    float y0 = 0.f, y1 = 0.f;
-    for (int mt = 0; mt < 4; mt++) {
-        f32x4 d = {0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < 4; ks++)
-            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_bf16x4(&h[mt * 16 + i][ks * 16 + kb]),
-                                                          lds_bf16x4(&s.w3[i][ks * 16 + kb]), d, 0, 0, 0);
-        // hand row (mt*16 + kb + j, output i) to the lane owning that marker
-        for (int j = 0; j < 4; j++) {
-            if (i < MLP_OUT) {
-                float *slot = (float *)&s.x[w][mt * 16 + kb + j][2 * i];   // x is no longer needed
-                *slot = d[j] + s.b3[i];
-            }
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    y0 = *(const float *)&s.x[w][lane][0];
-    y1 = *(const float *)&s.x[w][lane][2];
+    float y0, y1;
+    mlp_infer_wave(s.w, s.x[w], s.h[w], y0, y1);
     if (m < a.n) {
         const double r0 = y0 * norm[2 * MLP_IN + 2] + norm[2 * MLP_IN];
         const double r1 = y1 * norm[2 * MLP_IN + 3] + norm[2 * MLP_IN + 1];
@@ -303,18 +330,19 @@ __global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_guess(KArgs a, double dt,
 }
 
 // Training-set extraction.  Before a plain local update: features and the
-// log10 default starts into X[n][8] = f0..f5, log10 eta0, log10 vhi, with the
-// masks of the two outputs in Mask[n][2].  After it: k_mlp_labels turns the
-// last two columns into log10(root / default).
+// log10 default starts into X[n][MLP_IN + 2] = f0..f8, log10 eta0, log10 vhi,
+// with the masks of the two outputs in Mask[n][2].  After it: k_mlp_labels
+// turns the last two columns into log10(root / default).
+constexpr int MLP_ROW = MLP_IN + 2;
 __global__ void k_mlp_features(KArgs a, double dt, float *X, float *Mask)
 {
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= a.n) return;
     MlpFeat ft;
     mlp_features(a, m, dt, ft);
-    for (int k = 0; k < MLP_IN; k++) X[m * 8 + k] = ft.f[k];
-    X[m * 8 + 6] = ft.duct ? (float)log10(ft.eta0) : 0.f;
-    X[m * 8 + 7] = ft.rsf ? (float)log10(ft.vhi) : 0.f;
+    for (int k = 0; k < MLP_IN; k++) X[m * MLP_ROW + k] = ft.f[k];
+    X[m * MLP_ROW + MLP_IN] = ft.duct ? (float)log10(ft.eta0) : 0.f;
+    X[m * MLP_ROW + MLP_IN + 1] = ft.rsf ? (float)log10(ft.vhi) : 0.f;
     Mask[m * 2 + 0] = ft.duct ? 1.f : 0.f;
     Mask[m * 2 + 1] = ft.rsf ? 1.f : 0.f;
 }
@@ -324,10 +352,11 @@ __global__ void k_mlp_labels(KArgs a, float *X, float *Mask)
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= a.n) return;
     const double re = MXF(a, MX_ROOT_ETA, m), rv = MXF(a, MX_ROOT_V, m);
-    if (Mask[m * 2 + 0] != 0.f && re > 0 && isfinite(re)) X[m * 8 + 6] = (float)(log10(re) - (double)X[m * 8 + 6]);
-    else { Mask[m * 2 + 0] = 0.f; X[m * 8 + 6] = 0.f; }
-    if (Mask[m * 2 + 1] != 0.f && rv > 0 && isfinite(rv)) X[m * 8 + 7] = (float)(log10(rv) - (double)X[m * 8 + 7]);
-    else { Mask[m * 2 + 1] = 0.f; X[m * 8 + 7] = 0.f; }
+    float *r = X + m * MLP_ROW + MLP_IN;
+    if (Mask[m * 2 + 0] != 0.f && re > 0 && isfinite(re)) r[0] = (float)(log10(re) - (double)r[0]);
+    else { Mask[m * 2 + 0] = 0.f; r[0] = 0.f; }
+    if (Mask[m * 2 + 1] != 0.f && rv > 0 && isfinite(rv)) r[1] = (float)(log10(rv) - (double)r[1]);
+    else { Mask[m * 2 + 1] = 0.f; r[1] = 0.f; }
 }
 
 // One training step, one wavefront per 64 samples: forward, masked MSE,

@@ -2,10 +2,12 @@
 plain FP64 Newton solves, training on the GPU (libv2c's MFMA kernels),
 evaluation of iterations and wall time, plain against warm.
 
-The network (6 -> 64 -> 64 -> 2, tanh) predicts log10(root / default start)
+The network (9 -> 64 -> 64 -> 2, tanh) predicts log10(root / default start)
 for the ductile Newton unknown and for the fault slip rate; Newton to the
 same residual remains the arbiter, so a poor guess costs iterations, never
-accuracy.
+accuracy.  Inputs: log10 strain-rate invariant, T, P, log10 stress invariant,
+log10 theta, a-b, and for fault markers log10 of the slip-rate default start,
+log10 L and log10 of the previous slip rate.
 """
 from __future__ import annotations
 
@@ -17,9 +19,15 @@ import numpy as np
 from . import synth
 from .abi import Context, _dptr
 
-NPARAM = 4738
-IN, H, OUT = 6, 64, 2
-OFF = {"W1": 0, "b1": 384, "W2": 448, "b2": 4544, "W3": 4608, "b3": 4736}
+IN, H, OUT = 9, 64, 2
+OFF = {"W1": 0, "b1": H * IN}
+OFF["W2"] = OFF["b1"] + H
+OFF["b2"] = OFF["W2"] + H * H
+OFF["W3"] = OFF["b2"] + H
+OFF["b3"] = OFF["W3"] + OUT * H
+NPARAM = OFF["b3"] + OUT
+NNORM = 2 * IN + 4
+ROW = IN + 2                     # sample row: IN features, 2 labels
 
 
 def _bind(lib):
@@ -64,7 +72,7 @@ class WarmStart:
     def samples(self, dt: float, tol: float = 1e-10):
         """Features/labels from one plain local update of the resident markers."""
         n = self.ctx.lib.v2c_marker_count(self.ctx.h)
-        X = np.zeros((n, 8), dtype=np.float32)
+        X = np.zeros((n, ROW), dtype=np.float32)
         M = np.zeros((n, 2), dtype=np.float32)
         self.ctx._check(self.ctx.lib.v2c_mlp_samples(self.ctx.h, dt, tol, _fptr(X), _fptr(M)), "mlp_samples")
         keep = M.sum(1) > 0
@@ -97,7 +105,7 @@ class WarmStart:
 
     def fetch(self):
         p = np.zeros(NPARAM, dtype=np.float32)
-        nrm = np.zeros(16, dtype=np.float32)
+        nrm = np.zeros(NNORM, dtype=np.float32)
         self.ctx._check(self.ctx.lib.v2c_mlp_get(self.ctx.h, _fptr(p), _fptr(nrm)), "mlp_get")
         self.params = p
         return p
@@ -146,9 +154,10 @@ def reference_grad(params, Xs, Ts, Ms, emulate_bf16=True):
     the GPU rounds them."""
     q = bf16 if emulate_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     p = params.astype(np.float64)
-    W1 = p[0:384].reshape(H, IN); b1 = p[384:448]
-    W2 = p[448:4544].reshape(H, H); b2 = p[4544:4608]
-    W3 = p[4608:4736].reshape(OUT, H); b3 = p[4736:4738]
+    o = OFF
+    W1 = p[o["W1"]:o["b1"]].reshape(H, IN); b1 = p[o["b1"]:o["W2"]]
+    W2 = p[o["W2"]:o["b2"]].reshape(H, H); b2 = p[o["b2"]:o["W3"]]
+    W3 = p[o["W3"]:o["b3"]].reshape(OUT, H); b3 = p[o["b3"]:NPARAM]
     X = q(Xs)
     Z1 = X @ q(W1).T + b1
     H1 = np.tanh(Z1)
@@ -160,14 +169,14 @@ def reference_grad(params, Xs, Ts, Ms, emulate_bf16=True):
     loss = float((Ms * E * E).sum() / cnt)
     dY = 2 * Ms * E / cnt
     g = np.zeros(NPARAM)
-    g[4608:4736] = (q(dY).T @ q(H2)).ravel()
-    g[4736:4738] = dY.sum(0)
+    g[o["W3"]:o["b3"]] = (q(dY).T @ q(H2)).ravel()
+    g[o["b3"]:NPARAM] = dY.sum(0)
     dZ2 = (q(dY) @ q(W3)) * (1 - H2 * H2)
-    g[448:4544] = (q(dZ2).T @ q(H1)).ravel()
-    g[4544:4608] = dZ2.sum(0)
+    g[o["W2"]:o["b2"]] = (q(dZ2).T @ q(H1)).ravel()
+    g[o["b2"]:o["W3"]] = dZ2.sum(0)
     dZ1 = (q(dZ2) @ q(W2)) * (1 - H1 * H1)
-    g[0:384] = (q(dZ1).T @ X).ravel()
-    g[384:448] = dZ1.sum(0)
+    g[o["W1"]:o["b1"]] = (q(dZ1).T @ X).ravel()
+    g[o["b1"]:o["W2"]] = dZ1.sum(0)
     return g, loss, Y
 
 
@@ -203,18 +212,25 @@ def generate(ctx, seeds, dt, maker=synth.config_small, max_rows=2_000_000, keep_
     return X, M
 
 
-MODES = {"plain": 0, "warm": 3, "warm_rsf": 2}   # guesses predicted (bit 0 ductile, bit 1 slip rate)
+# mode -> (guesses predicted beforehand: bit 0 ductile, bit 1 slip rate; warm
+# argument of the local update)
+MODES = {"plain": (0, 0), "warm": (3, 1), "warm_rsf": (0, 2)}
 
 
 def evaluate(ctx, cfg, ws: WarmStart, dt, tol=1e-10, repeats=3, modes=("plain", "warm")):
     """Plain vs warm local update on the same cloud: iterations, residual
     evaluations, fallbacks and kernel times (best of `repeats`).  Modes:
-    "plain" (default starts), "warm" (network guesses for both solves),
-    "warm_rsf" (network guess for the slip rate only; the network runs only
-    for workgroups holding fault markers)."""
+    "plain" (default starts), "warm" (network guesses for both solves,
+    predicted for every marker by v2c_mlp_predict before the solves),
+    "warm_rsf" (the slip-rate guess evaluated by the network inside the
+    slip-rate solve, which runs over the fault markers only).  Times: the
+    ductile solve (ms_ductile), the slip-rate solve (ms_rsf, network included
+    for warm_rsf; it runs beside the ductile solve), the local update from
+    fork to join (ms_wall), the separate inference (ms_mlp), and
+    ms_total = ms_wall + ms_mlp."""
     res = {}
     for mode in modes:
-        outputs = MODES[mode]
+        outputs, warm = MODES[mode]
         best = None
         for _ in range(repeats):
             upload_cloud(ctx, cfg)
@@ -222,13 +238,15 @@ def evaluate(ctx, cfg, ws: WarmStart, dt, tol=1e-10, repeats=3, modes=("plain", 
             ctx.reset_timing()
             if outputs:
                 ws.predict(dt, outputs)
-            st = ctx.local_update(dt, warm=bool(outputs), tol=tol)
-            ms_lu, _ = ctx.kernel_time("local")
+            st = ctx.local_update(dt, warm=warm, tol=tol)
+            ms_d, _ = ctx.kernel_time("local")
+            ms_r, _ = ctx.kernel_time("local_rsf")
             ms_mlp, _ = ctx.kernel_time("mlp")
+            ms_wall, _ = ctx.kernel_time("local_wall")
             ctx.enable_timing(False)
-            tot = ms_lu + ms_mlp
+            tot = ms_wall + ms_mlp
             if best is None or tot < best["ms_total"]:
-                best = dict(stats=st, ms_local=ms_lu, ms_mlp=ms_mlp, ms_total=tot)
+                best = dict(stats=st, ms_ductile=ms_d, ms_rsf=ms_r, ms_mlp=ms_mlp, ms_wall=ms_wall, ms_total=tot)
         out = ctx.local_download(cfg.n)
         best["out"] = out
         res[mode] = best
