@@ -508,8 +508,9 @@ int v2c_reset_timing(v2c_ctx *ctx);
  * as a persistent grid of wavefronts striding over cell groups.  The stage
  * sequence is captured once into a HIP graph and replayed.
  * A step is, per model: marker2nodes, compute_before_solve(timestep, 0),
- * compute_after_solve(timestep, 0), nodes2markers + advection, then
- * timesum += timestep.  Results equal the per-model entry points bit for bit
+ * compute_after_solve(timestep, 0), [the local update], nodes2markers +
+ * advection, [copyarrays and calc_timestep], then timesum += timestep (and,
+ * with adaptive steps, the next timestep).  Results equal the per-model entry points bit for bit
  * whatever the shapes.  Models are grouped in size classes by marker count
  * (< 0.5 M, < 2 M, larger); shapes apply per class. */
 typedef struct v2c_ens v2c_ens;
@@ -536,6 +537,20 @@ int v2c_ens_step(v2c_ens *ens, int use_graph);
 int v2c_ens_order(v2c_ens *ens);
 /* the same step through the per-model entry points (one stream per model) */
 int v2c_ens_step_looped(v2c_ens *ens);
+/* Adds the per-marker local update (v2c_local_update with this warm mode and
+ * tolerance, on each model's RSF inputs and network) to every batched step,
+ * after compute_after_solve; warm < 0 removes it.  v2c_ens_step_looped runs
+ * the same sequence through the per-model entry points. */
+int v2c_ens_set_local(v2c_ens *ens, int warm, double tol);
+/* local-update statistics of one model's last step */
+int v2c_ens_local_stats(v2c_ens *ens, int model, v2c_lu_stats *stats);
+/* Adaptive time steps: every batched step ends with copyarrays(0)
+ * (momentum_continuity_iterate.c:265-283) and calc_timestep(oldornew)
+ * (calc_RSF.c:352-572) of every model on the device; the model's next step is
+ * that result clamped to [dt_min, dt_max].  p == NULL turns it off. */
+int v2c_ens_set_adaptive(v2c_ens *ens, const v2c_timestep_params *p, int oldornew, double dt_min, double dt_max);
+/* every model's (timesum, timestep) after the last step: 2 * nmodels values */
+int v2c_ens_times(v2c_ens *ens, double *timesum_timestep);
 /* compute_after_solve misfit of every model's last batched step:
  * rms_count[2*k] = RMS, rms_count[2*k+1] = node count */
 int v2c_ens_rms(v2c_ens *ens, double *rms_count);

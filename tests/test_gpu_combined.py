@@ -1,0 +1,286 @@
+"""The combined workload: a batched ensemble in the coseismic regime with the
+per-marker local update warm-started by the network evaluated inside the
+slip-rate solve, and adaptive time steps from each model's own
+calc_timestep (calc_RSF.c:352-572) between 1e-3 s and 1 yr.
+
+A combined step is, per model: marker2nodes, compute_before_solve(dt, 0),
+compute_after_solve(dt, 0), the local update (Newton to residual 1e-12),
+nodes2markers + advection, copyarrays(0), calc_timestep(0) -> the next dt
+(i2evps.c:82-132 without the Stokes and heat solves, whose nodal fields are
+synthetic).  Checks, per model: the batched step equals the per-model entry
+points bit for bit; every marker and nodal field within 1e-12 of the oracle
+running the same sequence, positions and types bit-exact; the local update's
+outputs (viscosity, slip rate, state, visco-plastic viscosity) within 1e-10
+of the oracle's plain Newton; the time steps within 1e-12."""
+import numpy as np
+import pytest
+
+from velo2cycles_amd import synth
+from velo2cycles_amd.abi import NODAL_FIELDS, Context, Ensemble, MarkerArrays, TimestepParams
+from velo2cycles_amd.warmstart import WarmStart, generate, init_params, upload_cloud
+from oracle.pyoracle import MX_FIELDS, Oracle, local_update
+from tests.util import compare, rel_errors
+
+pytestmark = pytest.mark.gpu
+
+DT_MIN, DT_MAX = 1e-3, synth.YEAR
+TOL = 1e-12
+LOCAL_OUT = ("eta", "V", "phi", "eta_vp", "sy")
+
+
+def coseismic_model(cfg, seed, scale=1e6):
+    return synth.coseismic_run(cfg, seed=seed, dt=DT_MIN, scale=scale)
+
+
+def small_models():
+    """Four reduced-size models; velocity scales 1e5 .. 3e9 give next steps
+    from about 3 s down to about 2 ms."""
+    specs = [(61, 31, 4, False, 1, 1e5), (81, 41, 3, True, 2, 1e6), (101, 51, 3, True, 3, 1e7),
+             (71, 61, 2, False, 4, 3e9)]
+    cfgs = []
+    for nx, ny, per, ref, seed, scale in specs:
+        cfg = synth.config_grid(nx, ny, (nx - 1) * 1e3, (ny - 1) * 1e3, per, refine=ref, seed=seed,
+                                fault_frac=0.03)
+        cfgs.append(coseismic_model(cfg, seed, scale))
+    cfgs[1].params.stoksmod = 2
+    cfgs[2].params.markmod = 1
+    return cfgs
+
+
+@pytest.fixture(scope="module")
+def net(gpu):
+    """A network trained on the method's own coseismic solves (small clouds)."""
+    maker = lambda seed: coseismic_model(  # noqa: E731
+        synth.config_grid(101, 51, 100e3, 50e3, 4, seed=seed, fault_frac=0.03), seed)
+    cfg = maker(101)
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
+    X, M = generate(ctx, range(101, 109), DT_MIN, maker=maker, keep_other=0.05)
+    ws = WarmStart(ctx)
+    ws.fit_norm(X, M)
+    ws.load(init_params(0))
+    ws.train(X, M, epochs=8, batch=4096, lr=2e-3)
+    params, norm = ws.fetch().copy(), ws.norm.copy()
+    ctx.close()
+    return params, norm
+
+
+def make_ctx(lib, cfg, net):
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=lib)
+    ctx.set_all_nodal(cfg.nodal)
+    upload_cloud(ctx, cfg)
+    WarmStart(ctx).load(*net)
+    return ctx
+
+
+def ts_params():
+    return TimestepParams.default(maxtmstep=DT_MAX)
+
+
+def initial_mx(cfg, idx=slice(None)):
+    r = cfg.rsf
+    n = len(r["fault"][idx])
+    mx = {k: np.zeros(n) for k in MX_FIELDS}
+    for k in ("V", "phi", "a", "b", "L", "V0", "mu0"):
+        mx[k] = np.array(r[k][idx], dtype=np.float64)
+    return mx
+
+
+def next_dt(o):
+    rc, res = o.calc_timestep(0, ts_params())
+    assert rc == 0, rc
+    return min(DT_MAX, max(DT_MIN, res["dt"]))
+
+
+def oracle_run(cfg, steps):
+    """The combined steps through the oracle; returns (oracle, mx, dts, the
+    nodal sliprs after each step)."""
+    o = Oracle(cfg)
+    fault = cfg.rsf["fault"]
+    mx = initial_mx(cfg)
+    ts, dt, dts, slip = cfg.timesum, cfg.timestep, [], []
+    for _ in range(steps):
+        dts.append(dt)
+        o.set_time(ts, dt)
+        o.marker2nodes()
+        o.compute_before_solve(dt, 0)
+        o.compute_after_solve(dt, 0)
+        mx, _, _, _ = local_update(o, dt, fault, mx, warm=False, tol=TOL)
+        o.nodes2markers()
+        o.moveandrotatemarkers()
+        o.copyarrays(0)
+        slip.append(o.nodal["sliprs"].copy())
+        ts += dt
+        dt = next_dt(o)
+    dts.append(dt)
+    return o, mx, dts, slip
+
+
+def check_local(got, mx, fault, what):
+    """Local-update outputs against the oracle's plain Newton: viscosity of
+    every marker, slip rate, state, visco-plastic viscosity and stress
+    invariant of the fault markers, within 1e-10."""
+    for f in LOCAL_OUT:
+        sel = slice(None) if f == "eta" else fault
+        err, _, nm = rel_errors(got[f][sel], mx[f][sel])
+        assert nm == 0 and err <= 1e-10, (what, f, err)
+
+
+def check_deltavs(got, ora, slip_dev, slip_ora, dt, what):
+    """copyarrays' deltaVs = (log(solved) - log(final)) / dt subtracts two
+    nearly equal logarithms, so last-bit differences of its inputs show
+    undamped (one bit of a logarithm is |log V| eps / dt, 1.3e-12 of the
+    field's rms here).  Allowed per node: the first-order effect of the
+    inputs' relative differences, (r_solved + r_final) / dt, the two slip
+    rates being the device's and the oracle's sliprs after this step and the
+    one before, plus two last bits of each logarithm, 8 (|log V| + 1) eps / dt,
+    plus 1e-15 relative for the step."""
+    g, o = got["deltaVs"], ora["deltaVs"]
+
+    def rel(a, b):
+        return np.abs(a - b) / np.where(np.abs(b) > 0, np.abs(b), 1.0)
+
+    r = rel(slip_dev[-1], slip_ora[-1]) + rel(slip_dev[-2], slip_ora[-2])
+    v = slip_ora[-1]
+    lv = np.abs(np.log(np.where(v > 0, v, 1.0)))
+    allow = (r + 8.0 * np.finfo(np.float64).eps * (lv + 1.0)) / dt + 1e-15 * np.abs(o)
+    both_nan = np.isnan(g) & np.isnan(o)
+    ok = both_nan | (np.abs(g - o) <= allow)
+    worst = np.max(np.where(both_nan, 0.0, np.abs(g - o) / allow))
+    assert ok.all(), (what, int((~ok).sum()), float(worst))
+    assert np.max(r) <= 1e-12, (what, float(np.max(r)))
+
+
+def same_model(ca, cb, n, what):
+    """Two contexts hold bit-identical markers, nodal fields and local outputs."""
+    ma, mb = MarkerArrays(n), MarkerArrays(n)
+    ca.download_markers(ma)
+    cb.download_markers(mb)
+    for f in ma.arrays:
+        assert np.array_equal(ma[f], mb[f], equal_nan=True), (what, f)
+    assert np.array_equal(ma.t, mb.t), what
+    na, nb = ca.get_all_nodal(), cb.get_all_nodal()
+    for f in NODAL_FIELDS:
+        assert np.array_equal(na[f], nb[f], equal_nan=True), (what, f)
+    la, lb = ca.local_download(n), cb.local_download(n)
+    for f in LOCAL_OUT:
+        assert np.array_equal(la[f], lb[f], equal_nan=True), (what, f)
+    return ma, na, la
+
+
+@pytest.mark.timeout(600)
+def test_combined_small_models_against_oracle(gpu, net):
+    """Four reduced-size models (61x31 .. 101x51 nodes, uniform and refined,
+    3 % fault markers, one with stoksmod 2 and one with markmod 1), three
+    combined steps: batched == per-model bit for bit; against the oracle,
+    every marker and nodal field, the local update's outputs and the time
+    step sequence."""
+    steps = 3
+    cfgs = small_models()
+    batch = [make_ctx(gpu, c, net) for c in cfgs]
+    looped = [make_ctx(gpu, c, net) for c in cfgs]
+    eb, el = Ensemble(batch), Ensemble(looped)
+    for e in (eb, el):
+        e.set_local(warm=2, tol=TOL)
+        e.set_adaptive(ts_params(), 0, DT_MIN, DT_MAX)
+    times = [eb.times().copy()]
+    slip = [[] for _ in cfgs]
+    for s in range(steps):
+        print("combined small: step", s, flush=True)
+        eb.step(graph=True)
+        el.step_looped()
+        times.append(eb.times().copy())
+        assert np.array_equal(times[-1], el.times()), s
+        for k in range(len(cfgs)):
+            slip[k].append(batch[k].get_all_nodal()["sliprs"])
+    for k, cfg in enumerate(cfgs):
+        print("case combined small", k, cfg.name, cfg.n, flush=True)
+        mb, nb, lb = same_model(batch[k], looped[k], cfg.n, f"model {k}")
+        o, mx, dts, slip_ora = oracle_run(cfg, steps)
+        got = [t[k, 1] for t in times]
+        for s, (a, b) in enumerate(zip(got, dts)):
+            assert abs(a - b) <= 1e-12 * abs(b), (k, s, a, b)
+        assert abs(times[-1][k, 0] - (cfg.timesum + sum(dts[:steps]))) <= 1e-12 * times[-1][k, 0]
+        assert DT_MIN <= min(dts) and max(dts) < DT_MAX, dts
+        bad = compare(mb, o.markers, nb, o.nodal, skip=("deltaVs",))
+        assert not bad, f"model {k}:\n" + "\n".join(bad)
+        check_deltavs(nb, o.nodal, slip[k], slip_ora, dts[steps - 1], f"model {k}")
+        fault = cfg.rsf["fault"].astype(bool)
+        check_local(lb, mx, fault, f"model {k}")
+        st = eb.local_stats(k)
+        assert st["fault_markers"] == int(fault.sum()) and st["nonfinite"] == 0, st
+        assert st["markers"] == cfg.n, st
+    eb.close()
+    el.close()
+    for c in batch + looped:
+        c.close()
+
+
+SUB = 20_000
+
+
+@pytest.mark.timeout(900)
+def test_combined_production_24_models(gpu, net):
+    """24 production-size models (1501x301 nodes, 16.2 M markers each, 389 M
+    in all) in the coseismic regime, two combined steps with the network warm
+    start and adaptive time steps.  Checks: models 0 and 23 equal, bit for
+    bit, the same steps through the per-model entry points; after the first
+    step, the local update of a seeded subset (20,000 fault and 20,000 other
+    markers) of models 0, 11 and 23 against the oracle's plain Newton, and
+    each model's next time step against the oracle's calc_timestep on the
+    model's nodal fields; after both, every model's markers finite and moved
+    and its local update free of non-finite results."""
+    models = list(synth.config_ensemble_production(24, coseismic_dt=DT_MIN))
+    assert all(c.rsf is not None for _, c in models)
+    cfg0 = models[0][1]
+    n = cfg0.n
+    print(f"24 production models, {24 * n} markers", flush=True)
+    batch = []
+    for k, cfg in models:
+        batch.append(make_ctx(gpu, cfg, net))
+    pick = (0, 23)
+    single = [make_ctx(gpu, models[k][1], net) for k in pick]
+    eb, el = Ensemble(batch), Ensemble(single)
+    for e in (eb, el):
+        e.set_local(warm=2, tol=TOL)
+        e.set_adaptive(ts_params(), 0, DT_MIN, DT_MAX)
+    eb.step(graph=True)
+    el.step_looped()
+    t1 = eb.times().copy()
+    print("step 1 done", flush=True)
+    fault = cfg0.rsf["fault"].astype(bool)
+    rng = np.random.default_rng(5)
+    idx = np.sort(np.concatenate([rng.choice(np.where(fault)[0], SUB, replace=False),
+                                  rng.choice(np.where(~fault)[0], SUB, replace=False)]))
+    for k in (0, 11, 23):
+        cfg = models[k][1]
+        got = batch[k].local_download(n)
+        sub = MarkerArrays(len(idx), {**{f: cfg.markers[f][idx] for f in cfg.markers.arrays},
+                                      "t": cfg.markers.t[idx]})
+        o = Oracle(cfg, markers=sub)
+        mx, _, _, _ = local_update(o, cfg.timestep, cfg.rsf["fault"][idx], initial_mx(cfg, idx), warm=False, tol=TOL)
+        check_local({f: got[f][idx] for f in LOCAL_OUT}, mx, fault[idx], f"model {k} step 1")
+        o = Oracle(cfg, markers=sub, nodal=batch[k].get_all_nodal())
+        dt = next_dt(o)
+        assert abs(t1[k, 1] - dt) <= 1e-12 * dt, (k, t1[k, 1], dt)
+        print(f"model {k}: subset local update and next step {dt:.3e} s checked", flush=True)
+    assert (t1[:, 1] >= DT_MIN).all() and (t1[:, 1] <= DT_MAX).all()
+    eb.step(graph=True)
+    el.step_looped()
+    print("step 2 done", flush=True)
+    t2 = eb.times()
+    for j, k in enumerate(pick):
+        same_model(batch[k], single[j], n, f"model {k} batched vs per-model")
+        assert np.array_equal(t2[k], el.times()[j]), k
+    for k in range(24):
+        st = eb.local_stats(k)
+        assert st["markers"] == n and st["fault_markers"] == int(fault.sum()) and st["nonfinite"] == 0, (k, st)
+        if k % 6 == 0:
+            m = MarkerArrays(n)
+            batch[k].download_markers(m)
+            for f in m.arrays:
+                assert np.isfinite(m[f]).all(), (k, f)
+            assert np.sum(m["x"] != cfg0.markers["x"]) > 0.9 * n, k
+    eb.close()
+    el.close()
+    for c in batch + single:
+        c.close()

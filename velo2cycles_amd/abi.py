@@ -499,6 +499,10 @@ def _bind_ens(lib):
                        ("v2c_ens_get_shape", [C.c_int, P(TileShape)]), ("v2c_ens_set_blocks_per_cu", [C.c_int]),
                        ("v2c_ens_step", [C.c_int]), ("v2c_ens_step_looped", []), ("v2c_ens_order", []),
                        ("v2c_ens_rms", [P(C.c_double)]),
+                       ("v2c_ens_set_local", [C.c_int, C.c_double]),
+                       ("v2c_ens_local_stats", [C.c_int, P(LUStats)]),
+                       ("v2c_ens_set_adaptive", [P(TimestepParams), C.c_int, C.c_double, C.c_double]),
+                       ("v2c_ens_times", [P(C.c_double)]),
                        ("v2c_ens_autotune", [C.c_int, P(C.c_double)])]:
         fn = getattr(lib, name)
         fn.argtypes = [C.c_void_p] + args
@@ -552,6 +556,26 @@ class Ensemble:
     def rms(self):
         out = np.zeros(2 * len(self.ctxs))
         self._check(self.lib.v2c_ens_rms(self.h, _dptr(out)), "ens_rms")
+        return out.reshape(-1, 2)
+
+    def set_local(self, warm=0, tol=1e-10):
+        """Add the per-marker local update to every step (warm < 0 removes it)."""
+        self._check(self.lib.v2c_ens_set_local(self.h, int(warm), tol), "ens_set_local")
+
+    def local_stats(self, k):
+        st = LUStats()
+        self._check(self.lib.v2c_ens_local_stats(self.h, k, C.byref(st)), "ens_local_stats")
+        return st.as_dict()
+
+    def set_adaptive(self, params=None, oldornew=0, dt_min=1e-3, dt_max=3.15576e7):
+        """Adaptive steps from every model's calc_timestep (params None: off)."""
+        p = C.byref(params) if params is not None else None
+        self._check(self.lib.v2c_ens_set_adaptive(self.h, p, oldornew, dt_min, dt_max), "ens_set_adaptive")
+
+    def times(self):
+        """(timesum, timestep) of every model."""
+        out = np.zeros(2 * len(self.ctxs))
+        self._check(self.lib.v2c_ens_times(self.h, _dptr(out)), "ens_times")
         return out.reshape(-1, 2)
 
     def autotune(self, reps=3):

@@ -1274,8 +1274,8 @@ int v2c_kernel_time(v2c_ctx *c, int kernel, double *ms, long *launches)
 
 }  // extern "C"
 
-#include "v2c_ensemble.hip"
 #include "v2c_post.hip"
+#include "v2c_ensemble.hip"
 #include "v2c_heat.hip"
 #include "v2c_bc.hip"
 #include "v2c_prn.cpp"

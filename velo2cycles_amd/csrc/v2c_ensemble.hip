@@ -155,13 +155,145 @@ __global__ __launch_bounds__(ENS_BLOCK) void k_ens_n2m_nodes(const KArgs *__rest
     });
 }
 
-__global__ __launch_bounds__(ENS_BLOCK) void k_ens_n2m_update(const KArgs *__restrict__ A,
-                                                              const EnsTile *__restrict__ tiles, int *unsupported)
+// Append the lanes' flagged (model, marker) items to the ensemble's list.
+__device__ __forceinline__ void list_append64(bool flag, uint64_t item, uint64_t *list, int *count)
 {
-    bool bad = false;
-    ens_item(A, tiles, [&](const KArgs &a, long m) { bad = n2m_update_one<true>(a, m); });
-    const unsigned long long mask = __ballot(bad);
-    if (mask && (threadIdx.x & (WAVE - 1)) == 0) atomicAdd(unsupported, (int)__popcll(mask));
+    const unsigned long long mask = __ballot(flag);
+    if (!mask) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    int base = 0;
+    if (lane == 0) base = atomicAdd(count, (int)__popcll(mask));
+    base = __shfl(base, 0);
+    if (flag) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = item;
+}
+
+// Fused node-to-marker update and advection, fast form (n2m_update_fast);
+// the markers it cannot take, and every marker of a model without
+// antidiffusion, go to the deferred list for the general form.
+__global__ __launch_bounds__(ENS_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_ens_n2m_update_fast(const KArgs *__restrict__ A,
+                                                                   const EnsTile *__restrict__ tiles,
+                                                                   uint64_t *defer, int *ndefer)
+{
+    const EnsTile T = tiles[blockIdx.x];
+    const long m = T.begin + threadIdx.x;
+    bool def = false;
+    if (m < T.end) {
+        const KArgs &a = model_args(A, T.model);
+        def = a.p.antidiffinter != 1 || !n2m_update_fast(a, m);
+    }
+    list_append64(def, ((uint64_t)T.model << 32) | (uint64_t)(uint32_t)m, defer, ndefer);
+}
+
+__global__ __launch_bounds__(256) void k_ens_n2m_update_list(const KArgs *__restrict__ A,
+                                                             const uint64_t *__restrict__ list,
+                                                             const int *__restrict__ count, int *unsupported)
+{
+    const int n = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t v = list[i];
+        if (n2m_update_one<true>(model_args(A, (int)(v >> 32)), (long)(uint32_t)v)) atomicAdd(unsupported, 1);
+    }
+}
+
+// ---- per-marker local update of every model (v2c_local.hip) ----------------
+__global__ void k_ens_lu_clear(const KArgs *__restrict__ A)
+{
+    unsigned long long *st = model_args(A, blockIdx.x).lu_stats;
+    for (int i = threadIdx.x; i < LS_COUNT; i += blockDim.x) st[i] = 0;
+}
+
+__global__ __launch_bounds__(LU_BLOCK) void k_ens_local_ductile(const KArgs *__restrict__ A,
+                                                                const EnsTile *__restrict__ tiles, int warm,
+                                                                double tol)
+{
+    __shared__ unsigned long long sh[LS_COUNT];
+    lu_clear(sh);
+    const EnsTile T = tiles[blockIdx.x];
+    const KArgs &a = model_args(A, T.model);
+    const long m = T.begin + threadIdx.x;
+    if (m < T.end) lu_ductile_one(a, m, warm == 1, tol, sh);
+    lu_flush(sh, a.lu_stats);
+}
+
+// Fault-list entries of the models (index ranges of each model's list).
+template <int WARM>
+__global__ __launch_bounds__(LU_BLOCK) void k_ens_local_rsf(const KArgs *__restrict__ A,
+                                                            const EnsTile *__restrict__ tiles, double tol)
+{
+    __shared__ unsigned long long sh[LS_COUNT];
+    lu_clear(sh);
+    const EnsTile T = tiles[blockIdx.x];
+    const KArgs &a = model_args(A, T.model);
+    const long i = T.begin + threadIdx.x;
+    if constexpr (WARM == 2) {
+        __shared__ LuRsfLds L;
+        lu_rsf_block<2>(a, i, T.end, a.timestep, tol, sh, &L);
+    } else {
+        lu_rsf_block<WARM>(a, i, T.end, a.timestep, tol, sh, nullptr);
+    }
+    lu_flush(sh, a.lu_stats);
+}
+
+__global__ __launch_bounds__(LU_BLOCK) void k_ens_local_etavp(const KArgs *__restrict__ A,
+                                                              const EnsTile *__restrict__ tiles)
+{
+    const EnsTile T = tiles[blockIdx.x];
+    const long i = T.begin + threadIdx.x;
+    if (i < T.end) lu_etavp_one(model_args(A, T.model), i);
+}
+
+// ---- copyarrays and calc_timestep of every model (v2c_post.hip) ------------
+// Per-model scratch of the batched calc_timestep; out (8 doubles) and err
+// point into ensemble-wide arrays read back after the step.
+struct EnsTs {
+    V2C_GLOBAL Fold *part;
+    V2C_GLOBAL double *bval, *carry, *out;
+    V2C_GLOBAL int *bset, *err;
+    int nblocks, pad;
+};
+
+__global__ __launch_bounds__(ENS_BLOCK) void k_ens_copyarrays(const KArgs *__restrict__ A,
+                                                              const EnsTile *__restrict__ tiles)
+{
+    ens_item(A, tiles, [&](const KArgs &a, long m3) { copyarrays_one(a, m3); });
+}
+
+__global__ void k_ens_ts_clear(const EnsTs *__restrict__ S, int nm)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nm) S[k].err[0] = 0;
+}
+
+// Node entries start at multiples of POST_BLOCK: entry = block begin / POST_BLOCK.
+static_assert(ENS_BLOCK == POST_BLOCK, "node entries double as calc_timestep blocks");
+__global__ __launch_bounds__(POST_BLOCK) void k_ens_ts_lambda(const KArgs *__restrict__ A,
+                                                              const EnsTile *__restrict__ tiles,
+                                                              const EnsTs *__restrict__ S, TsArgs T)
+{
+    const EnsTile E = tiles[blockIdx.x];
+    const EnsTs &s = S[E.model];
+    ts_lambda_block(model_args(A, E.model), T, E.begin / POST_BLOCK, s.bset, s.bval, s.err);
+}
+
+__global__ __launch_bounds__(FIN_BLOCK) void k_ens_ts_carry(const EnsTs *__restrict__ S, TsArgs T)
+{
+    const EnsTs &s = S[blockIdx.x];
+    ts_carry_block(s.bset, s.bval, s.nblocks, T.lambda0, s.carry);
+}
+
+__global__ __launch_bounds__(POST_BLOCK) void k_ens_ts_node(const KArgs *__restrict__ A,
+                                                            const EnsTile *__restrict__ tiles,
+                                                            const EnsTs *__restrict__ S, TsArgs T)
+{
+    const EnsTile E = tiles[blockIdx.x];
+    const EnsTs &s = S[E.model];
+    ts_node_block(model_args(A, E.model), T, E.begin / POST_BLOCK, s.carry, s.part);
+}
+
+__global__ __launch_bounds__(FIN_BLOCK) void k_ens_ts_final(const EnsTs *__restrict__ S, TsArgs T)
+{
+    const EnsTs &s = S[blockIdx.x];
+    ts_final_block(s.part, s.nblocks, T, s.out);
 }
 
 __global__ void k_ens_clear(int *p, int n)
@@ -281,7 +413,7 @@ using namespace v2c;
 
 // Work lists of one shape assignment.
 struct EnsPlan {
-    std::vector<EnsTile> markers, cells, nodes, otiles;
+    std::vector<EnsTile> markers, cells, nodes, otiles, faults;
 };
 
 struct v2c_ens {
@@ -308,7 +440,22 @@ struct v2c_ens {
     double *d_rms = nullptr;
     int *d_flags = nullptr;                  // [0] unsupported marker types
     std::vector<hipEvent_t> ordered;         // per model: its ordering is done
-    std::vector<long> plan_n;                // marker counts the work lists were cut for
+    std::vector<long> plan_n, plan_nf;       // marker and fault counts the work lists were cut for
+    int n_fault = 0;
+    EnsTile *t_fault = nullptr;              // fault-list entries (local update)
+    uint64_t *d_defer = nullptr;             // (model, marker) the fast fused update hands on
+    long defer_cap = 0;
+    // optional stages: the local update (warm mode, tolerance) and adaptive steps
+    int local_warm = -1;
+    double local_tol = 1e-10;
+    hipEvent_t ev_lfork = nullptr, ev_ljoin = nullptr;
+    int adaptive = 0;
+    TsArgs ts{};
+    double dt_min = 0, dt_max = 0;
+    EnsTs *d_ts = nullptr;
+    std::vector<void *> ts_mem;              // per-model calc_timestep scratch
+    double *d_tsout = nullptr, *h_tsout = nullptr;   // 8 per model
+    int *d_tserr = nullptr;
     // batched re-ordering
     bool batch_order = false;                // packed mover values fit in 64 bits
     int mov_shift = 0, mov_bits = 0;         // model number at bit mov_shift; sort bits
@@ -408,6 +555,8 @@ static EnsPlan ens_plan(const v2c_ens *e)
         for (long b = 0; b < c->ncell; b += s.cells_per_group)
             p.cells.push_back(EnsTile{mod, 0, b, std::min(c->ncell, b + (long)s.cells_per_group)});
         for (long t = 0; t < (c->n + ORD_TILE - 1) / ORD_TILE; t++) p.otiles.push_back(EnsTile{mod, 0, t, t + 1});
+        for (long b = 0; b < c->nfault; b += LU_BLOCK)
+            p.faults.push_back(EnsTile{mod, 0, b, std::min(c->nfault, b + (long)LU_BLOCK)});
     }
     ens_item_list(p.markers, mruns);
     ens_item_list(p.nodes, nruns);
@@ -417,7 +566,8 @@ static EnsPlan ens_plan(const v2c_ens *e)
 static int ens_upload_plan(v2c_ens *e)
 {
     const EnsPlan p = ens_plan(e);
-    const long total = (long)(p.markers.size() + p.cells.size() + p.nodes.size() + p.otiles.size());
+    const long total = (long)(p.markers.size() + p.cells.size() + p.nodes.size() + p.otiles.size() +
+                              p.faults.size());
     if (total > e->tile_cap) {
         if (e->d_tiles) (void)hipFree(e->d_tiles);
         e->d_tiles = nullptr;
@@ -430,6 +580,7 @@ static int ens_upload_plan(v2c_ens *e)
     all.insert(all.end(), p.cells.begin(), p.cells.end());
     all.insert(all.end(), p.nodes.begin(), p.nodes.end());
     all.insert(all.end(), p.otiles.begin(), p.otiles.end());
+    all.insert(all.end(), p.faults.begin(), p.faults.end());
     if (total)
         ENSCHK(e, hipMemcpyAsync(e->d_tiles, all.data(), sizeof(EnsTile) * total, hipMemcpyHostToDevice, e->stream));
     ENSCHK(e, hipStreamSynchronize(e->stream));
@@ -437,11 +588,16 @@ static int ens_upload_plan(v2c_ens *e)
     e->n_cell = (int)p.cells.size();
     e->n_node = (int)p.nodes.size();
     e->n_otile = (int)p.otiles.size();
+    e->n_fault = (int)p.faults.size();
     e->t_mark = e->d_tiles;
     e->t_cell = e->t_mark + e->n_mark;
     e->t_node = e->t_cell + e->n_cell;
     e->t_otile = e->t_node + e->n_node;
-    for (size_t k = 0; k < e->models.size(); k++) e->plan_n[k] = e->models[k]->n;
+    e->t_fault = e->t_otile + e->n_otile;
+    for (size_t k = 0; k < e->models.size(); k++) {
+        e->plan_n[k] = e->models[k]->n;
+        e->plan_nf[k] = e->models[k]->nfault;
+    }
     e->graph_valid = false;
     return 0;
 }
@@ -472,13 +628,55 @@ static void ens_stages(v2c_ens *e)
         hipLaunchKernelGGL(k_ens_after, ng, dim3(RED_BLOCK), 0, e->stream, e->d_args, e->d_red, e->t_node);
     }
     hipLaunchKernelGGL(k_ens_after_final, dim3(nm), dim3(RED_BLOCK), 0, e->stream, e->d_red, e->d_redn, e->d_rms);
+    if (e->local_warm >= 0) {
+        // local update of every model: the slip-rate solves (fault-list
+        // entries) on the side stream beside the ductile solves (marker
+        // entries), then the visco-plastic viscosity that needs both
+        hipLaunchKernelGGL(k_ens_lu_clear, dim3(nm), dim3(64), 0, e->stream, e->d_args);
+        if (e->n_fault) {
+            (void)hipEventRecord(e->ev_lfork, e->stream);
+            (void)hipStreamWaitEvent(e->side, e->ev_lfork, 0);
+            const dim3 fg((unsigned)e->n_fault);
+            if (e->local_warm == 2)
+                hipLaunchKernelGGL(k_ens_local_rsf<2>, fg, dim3(LU_BLOCK), 0, e->side, e->d_args, e->t_fault,
+                                   e->local_tol);
+            else if (e->local_warm == 1)
+                hipLaunchKernelGGL(k_ens_local_rsf<1>, fg, dim3(LU_BLOCK), 0, e->side, e->d_args, e->t_fault,
+                                   e->local_tol);
+            else
+                hipLaunchKernelGGL(k_ens_local_rsf<0>, fg, dim3(LU_BLOCK), 0, e->side, e->d_args, e->t_fault,
+                                   e->local_tol);
+            (void)hipEventRecord(e->ev_ljoin, e->side);
+        }
+        if (e->n_mark)
+            hipLaunchKernelGGL(k_ens_local_ductile, mg, dim3(LU_BLOCK), 0, e->stream, e->d_args, e->t_mark,
+                               e->local_warm, e->local_tol);
+        if (e->n_fault) {
+            (void)hipStreamWaitEvent(e->stream, e->ev_ljoin, 0);
+            hipLaunchKernelGGL(k_ens_local_etavp, dim3((unsigned)e->n_fault), dim3(LU_BLOCK), 0, e->stream,
+                               e->d_args, e->t_fault);
+        }
+    }
     hipLaunchKernelGGL(k_ens_phantom, dim3((nm + 63) / 64), dim3(64), 0, e->stream, e->d_args, nm);
     if (e->n_cell)
         hipLaunchKernelGGL(k_ens_n2m_scatter, dim3(ens_grid(e, e->n_cell)), dim3(WAVE), 0, e->stream, e->d_args,
                            e->t_cell, e->n_cell);
     if (e->n_node) hipLaunchKernelGGL(k_ens_n2m_nodes, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
-    if (e->n_mark)
-        hipLaunchKernelGGL(k_ens_n2m_update, mg, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_mark, e->d_flags);
+    if (e->n_mark) {
+        hipLaunchKernelGGL(k_ens_n2m_update_fast, mg, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_mark,
+                           e->d_defer, e->d_flags + 1);
+        hipLaunchKernelGGL(k_ens_n2m_update_list, dim3(N2M_LIST_BLOCKS), dim3(256), 0, e->stream, e->d_args,
+                           e->d_defer, e->d_flags + 1, e->d_flags);
+    }
+    if (e->adaptive && e->n_node) {
+        // the next time step of every model: copyarrays, then calc_timestep
+        hipLaunchKernelGGL(k_ens_copyarrays, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
+        hipLaunchKernelGGL(k_ens_ts_clear, dim3((nm + 63) / 64), dim3(64), 0, e->stream, e->d_ts, nm);
+        hipLaunchKernelGGL(k_ens_ts_lambda, ng, dim3(POST_BLOCK), 0, e->stream, e->d_args, e->t_node, e->d_ts, e->ts);
+        hipLaunchKernelGGL(k_ens_ts_carry, dim3(nm), dim3(FIN_BLOCK), 0, e->stream, e->d_ts, e->ts);
+        hipLaunchKernelGGL(k_ens_ts_node, ng, dim3(POST_BLOCK), 0, e->stream, e->d_args, e->t_node, e->d_ts, e->ts);
+        hipLaunchKernelGGL(k_ens_ts_final, dim3(nm), dim3(FIN_BLOCK), 0, e->stream, e->d_ts, e->ts);
+    }
 }
 
 // Per-step preparation outside the graph: cell ordering of every model and
@@ -597,7 +795,8 @@ static int ens_plan_check(v2c_ens *e)
 {
     const size_t nm = e->models.size();
     for (size_t k = 0; k < nm; k++)
-        if (e->models[k]->n != e->plan_n[k]) return ens_upload_plan(e);   // new marker counts: new work lists
+        if (e->models[k]->n != e->plan_n[k] || e->models[k]->nfault != e->plan_nf[k])
+            return ens_upload_plan(e);   // new marker or fault counts: new work lists
     return 0;
 }
 
@@ -627,12 +826,36 @@ static int ens_prepare(v2c_ens *e)
     return 0;
 }
 
+// After a step: markers moved; every model's time advances by its step and,
+// with adaptive steps, takes the next step from its calc_timestep (read back
+// with the step's flags), clamped to [dt_min, dt_max].
 static int ens_finish_step(v2c_ens *e)
 {
-    for (auto *c : e->models) {
-        c->order_valid = false;            // markers moved
+    for (size_t k = 0; k < e->models.size(); k++) {
+        v2c_ctx *c = e->models[k];
+        c->order_valid = false;
         c->timesum += c->timestep;
+        if (e->adaptive) c->timestep = std::min(e->dt_max, std::max(e->dt_min, e->h_tsout[8 * k]));
     }
+    return 0;
+}
+
+// The step's read-back: unsupported-type count, and with adaptive steps the
+// calc_timestep results and error flags of every model.
+static int ens_readback(v2c_ens *e, int *bad)
+{
+    ENSCHK(e, hipMemcpyAsync(bad, e->d_flags, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    const size_t nm = e->models.size();
+    std::vector<int> err(nm, 0);
+    if (e->adaptive) {
+        ENSCHK(e, hipMemcpyAsync(e->h_tsout, e->d_tsout, sizeof(double) * 8 * nm, hipMemcpyDeviceToHost, e->stream));
+        ENSCHK(e, hipMemcpyAsync(err.data(), e->d_tserr, sizeof(int) * nm, hipMemcpyDeviceToHost, e->stream));
+    }
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    for (size_t k = 0; k < nm; k++)
+        if (err[k])
+            return ens_err(e, V2C_EUNSUPPORTED, "model %zu: calc_timestep: %s", k,
+                           (err[k] & 2) ? "pmode other than 0 and 1" : "chi is zero at a node (the reference exits)");
     return 0;
 }
 
@@ -712,7 +935,13 @@ int v2c_ens_create(v2c_ens **out, v2c_ctx *const *models, int nmodels)
             return fail(V2C_EHIP);
         if (hipMalloc(&e->d_sort_tmp, std::max(e->sort_tmp_bytes, (size_t)16)) != hipSuccess) return fail(V2C_ENOMEM);
     }
+    if (hipMalloc(&e->d_defer, sizeof(uint64_t) * std::max(capsum, 1L)) != hipSuccess) return fail(V2C_ENOMEM);
+    e->defer_cap = capsum;
+    if (hipEventCreateWithFlags(&e->ev_lfork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_ljoin, hipEventDisableTiming) != hipSuccess)
+        return fail(V2C_EHIP);
     e->plan_n.assign(nmodels, -1);
+    e->plan_nf.assign(nmodels, -1);
     if (ens_upload_plan(e)) return fail(V2C_EHIP);
     *out = e;
     return 0;
@@ -724,10 +953,16 @@ void v2c_ens_destroy(v2c_ens *e)
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     if (e->side) (void)hipStreamSynchronize(e->side);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
     void *ptrs[] = {e->d_args, e->d_args_p, e->d_tiles, e->d_red, e->d_redn, e->d_rms, e->d_flags, e->d_ord,
-                    e->d_nmov, e->d_base, e->d_mov_in, e->d_mov_out, e->d_sort_tmp};
+                    e->d_nmov, e->d_base, e->d_mov_in, e->d_mov_out, e->d_sort_tmp, e->d_defer, e->d_ts,
+                    e->d_tsout, e->d_tserr};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (void *p : e->ts_mem) (void)hipFree(p);
+    if (e->h_tsout) (void)hipHostFree(e->h_tsout);
+    if (e->ev_lfork) (void)hipEventDestroy(e->ev_lfork);
+    if (e->ev_ljoin) (void)hipEventDestroy(e->ev_ljoin);
     if (e->h_args) (void)hipHostFree(e->h_args);
     if (e->h_args_p) (void)hipHostFree(e->h_args_p);
     if (e->ev_props) (void)hipEventDestroy(e->ev_props);
@@ -803,9 +1038,9 @@ int v2c_ens_step(v2c_ens *e, int use_graph)
         }
         ENSCHK(e, hipGraphLaunch(e->exec, e->stream));
     }
-    ENSCHK(e, hipStreamSynchronize(e->stream));
     int bad = 0;
-    ENSCHK(e, hipMemcpy(&bad, e->d_flags, sizeof(int), hipMemcpyDeviceToHost));
+    rc = ens_readback(e, &bad);
+    if (rc) return rc;
     ens_finish_step(e);
     if (bad) return ens_err(e, V2C_EUNSUPPORTED, "%d markers of type >= 100 (immobile spawning is out of scope)", bad);
     return 0;
@@ -834,13 +1069,112 @@ int v2c_ens_step_looped(v2c_ens *e)
         int rc = v2c_marker2nodes(c);
         if (!rc) rc = v2c_compute_before_solve(c, dt, 0, nullptr);
         if (!rc) rc = v2c_compute_after_solve(c, dt, 0, nullptr, nullptr);
+        if (!rc && e->local_warm >= 0) rc = v2c_local_update(c, dt, e->local_warm, e->local_tol, nullptr);
         if (!rc) rc = v2c_nodes2markers_advect(c);
         c->async_mode = was_async;
         if (rc) return ens_err(e, rc, "model %zu: %s", k, c->err);
     }
-    for (auto *c : e->models) {
+    for (size_t k = 0; k < e->models.size(); k++) {
+        v2c_ctx *c = e->models[k];
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        double next = 0;
+        if (e->adaptive) {
+            int rc = v2c_copyarrays(c, 0);
+            v2c_timestep_params tp{e->ts.maxtmstep, e->ts.tcnvp, e->ts.tcd, e->ts.tcphih, e->ts.dtfactor,
+                                   e->ts.lambda0};
+            v2c_timestep_result r;
+            if (!rc) rc = v2c_calc_timestep(c, e->ts.oldornew, &tp, &r);
+            if (rc) return ens_err(e, rc, "model %zu: %s", k, c->err);
+            next = std::min(e->dt_max, std::max(e->dt_min, r.dt));
+        }
         c->timesum += c->timestep;
+        if (e->adaptive) c->timestep = next;
+    }
+    return 0;
+}
+
+// Adds the per-marker local update to every batched step (after
+// compute_after_solve); warm < 0 removes it.
+int v2c_ens_set_local(v2c_ens *e, int warm, double tol)
+{
+    if (!e || warm > 2 || (warm >= 0 && !(tol > 0))) return V2C_EINVAL;
+    if (warm == 2)
+        for (size_t k = 0; k < e->models.size(); k++)
+            if (!e->models[k]->mlp_ready)
+                return ens_err(e, V2C_EINVAL, "model %zu: no MLP parameters loaded (v2c_mlp_set)", k);
+    e->local_warm = warm < 0 ? -1 : warm;
+    e->local_tol = tol;
+    e->graph_valid = false;
+    return 0;
+}
+
+int v2c_ens_local_stats(v2c_ens *e, int model, v2c_lu_stats *st)
+{
+    if (!e || !st || model < 0 || model >= (int)e->models.size()) return V2C_EINVAL;
+    v2c_ctx *c = e->models[model];
+    unsigned long long h[LS_COUNT];
+    ENSCHK(e, hipMemcpy(h, c->d_lu, sizeof h, hipMemcpyDeviceToHost));
+    st->markers = (long)h[LS_N]; st->ductile_markers = (long)h[LS_DUCT];
+    st->ductile_iters = (long)h[LS_DUCT_IT]; st->fault_markers = (long)h[LS_FAULT];
+    st->rsf_iters = (long)h[LS_RSF_IT]; st->max_ductile_iters = (long)h[LS_DUCT_MAX];
+    st->max_rsf_iters = (long)h[LS_RSF_MAX]; st->fallbacks = (long)h[LS_FALLBACK];
+    st->nonfinite = (long)h[LS_NONFINITE];
+    st->ductile_evals = (long)h[LS_DUCT_EV];
+    st->rsf_evals = (long)h[LS_RSF_EV];
+    for (int k = 0; k < LU_HIST; k++) {
+        st->hist_ductile[k] = (long)h[LS_HIST_D + k];
+        st->hist_rsf[k] = (long)h[LS_HIST_R + k];
+    }
+    return 0;
+}
+
+int v2c_ens_set_adaptive(v2c_ens *e, const v2c_timestep_params *p, int oldornew, double dt_min, double dt_max)
+{
+    if (!e || (p && (oldornew < 0 || oldornew > 1 || !(dt_min > 0) || !(dt_max >= dt_min)))) return V2C_EINVAL;
+    ENSCHK(e, hipSetDevice(e->device));
+    e->graph_valid = false;
+    if (!p) {
+        e->adaptive = 0;
+        return 0;
+    }
+    const size_t nm = e->models.size();
+    if (!e->d_ts) {
+        std::vector<EnsTs> h(nm);
+        ENSCHK(e, hipMalloc(&e->d_ts, sizeof(EnsTs) * nm));
+        ENSCHK(e, hipMalloc(&e->d_tsout, sizeof(double) * 8 * nm));
+        ENSCHK(e, hipMalloc(&e->d_tserr, sizeof(int) * nm));
+        ENSCHK(e, hipHostMalloc((void **)&e->h_tsout, sizeof(double) * 8 * nm, hipHostMallocDefault));
+        for (size_t k = 0; k < nm; k++) {
+            const int nb = (int)blocks_for(e->models[k]->nodenum, POST_BLOCK);
+            void *mem = nullptr;
+            ENSCHK(e, hipMalloc(&mem, sizeof(Fold) * NTS * nb + sizeof(double) * 2 * nb + sizeof(int) * nb));
+            e->ts_mem.push_back(mem);
+            Fold *part = (Fold *)mem;
+            double *bval = (double *)(part + (size_t)NTS * nb);
+            setp(h[k].part, part);
+            setp(h[k].bval, bval);
+            setp(h[k].carry, bval + nb);
+            setp(h[k].bset, (int *)(bval + 2 * (size_t)nb));
+            setp(h[k].out, e->d_tsout + 8 * k);
+            setp(h[k].err, e->d_tserr + k);
+            h[k].nblocks = nb;
+            h[k].pad = 0;
+        }
+        ENSCHK(e, hipMemcpy(e->d_ts, h.data(), sizeof(EnsTs) * nm, hipMemcpyHostToDevice));
+    }
+    e->ts = TsArgs{p->maxtmstep, p->tcnvp, p->tcd, p->tcphih, p->dtfactor, p->lambda0, oldornew};
+    e->dt_min = dt_min;
+    e->dt_max = dt_max;
+    e->adaptive = 1;
+    return 0;
+}
+
+int v2c_ens_times(v2c_ens *e, double *timesum_timestep)
+{
+    if (!e || !timesum_timestep) return V2C_EINVAL;
+    for (size_t k = 0; k < e->models.size(); k++) {
+        timesum_timestep[2 * k] = e->models[k]->timesum;
+        timesum_timestep[2 * k + 1] = e->models[k]->timestep;
     }
     return 0;
 }
@@ -881,6 +1215,9 @@ int v2c_ens_autotune(v2c_ens *e, int reps, double *best_ms)
             float m = 0;
             ENSCHK(e, hipEventElapsedTime(&m, t0, t1));
             tot += m;
+            int bad = 0;
+            rc = ens_readback(e, &bad);
+            if (rc) return rc;
             ens_finish_step(e);
         }
         ms = tot / reps;

@@ -116,11 +116,12 @@ struct LuRsfLds {
     short h[LU_BLOCK / 64][64][MLP_H];
 };
 
+// Entry i (< iend) of the fault list.
 template <int WARM>
-__device__ __forceinline__ void lu_rsf_block(const KArgs &a, long i, double dt, double tol, unsigned long long *sh,
-                                             LuRsfLds *L)
+__device__ __forceinline__ void lu_rsf_block(const KArgs &a, long i, long iend, double dt, double tol,
+                                             unsigned long long *sh, LuRsfLds *L)
 {
-    const bool live = i < a.nfault;
+    const bool live = i < iend;
     const long m = live ? (long)a.fault_list[i] : 0;
     double guess = 0.0;
     if (WARM == 1) {
@@ -153,23 +154,27 @@ __global__ __launch_bounds__(LU_BLOCK) void k_local_rsf(KArgs a, double dt, doub
     const long i = (long)blockIdx.x * LU_BLOCK + threadIdx.x;
     if constexpr (WARM == 2) {
         __shared__ LuRsfLds L;     // only the network form holds the staging
-        lu_rsf_block<2>(a, i, dt, tol, sh, &L);
+        lu_rsf_block<2>(a, i, a.nfault, dt, tol, sh, &L);
     } else {
-        lu_rsf_block<WARM>(a, i, dt, tol, sh, nullptr);
+        lu_rsf_block<WARM>(a, i, a.nfault, dt, tol, sh, nullptr);
     }
     lu_flush(sh, a.lu_stats);
 }
 
 // Visco-plastic viscosity of the fault markers from both solves
 // (calc_RSF.c:138-152 with the marker's eta, V and sigII).
-__global__ __launch_bounds__(LU_BLOCK) void k_local_etavp(KArgs a)
+__device__ __forceinline__ void lu_etavp_one(const KArgs &a, long i)
 {
-    const long i = (long)blockIdx.x * LU_BLOCK + threadIdx.x;
-    if (i >= a.nfault) return;
     const long m = a.fault_list[i];
     const double eta = MXF(a, MX_ETA, m), V = MXF(a, MX_V, m), sig = MXF(a, MX_SY, m);
     const double W = a.p.faultwidth;
     MXF(a, MX_ETAVP, m) = sig > 0 ? eta * sig / (2 * eta * V / W + sig) : eta;
+}
+
+__global__ __launch_bounds__(LU_BLOCK) void k_local_etavp(KArgs a)
+{
+    const long i = (long)blockIdx.x * LU_BLOCK + threadIdx.x;
+    if (i < a.nfault) lu_etavp_one(a, i);
 }
 
 }  // namespace v2c

@@ -265,10 +265,8 @@ __global__ __launch_bounds__(POST_BLOCK) void k_basic2center(KArgs a)
 }
 
 // ---- copyarrays (momentum_continuity_iterate.c:265-283) ------------------------
-__global__ __launch_bounds__(POST_BLOCK) void k_copyarrays(KArgs a)
+__device__ __forceinline__ void copyarrays_one(const KArgs &a, long m3)
 {
-    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
-    if (m3 >= a.nodenum) return;
     const long ny = a.g.ynumy, m2 = m3 % ny, m1 = m3 / ny;
     if (!(m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1)) return;
     const double solved = NF_(RK_sliprs_solved)[m3];
@@ -276,6 +274,12 @@ __global__ __launch_bounds__(POST_BLOCK) void k_copyarrays(KArgs a)
     NF_(sliprs)[m3] = solved;
     NF_(RK_sliprs_final)[m3] = solved;
     NF_(RK_phidts_final)[m3] = NF_(RK_phidts_solved)[m3];
+}
+
+__global__ __launch_bounds__(POST_BLOCK) void k_copyarrays(KArgs a)
+{
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (m3 < a.nodenum) copyarrays_one(a, m3);
 }
 
 // ---- calc_timestep (calc_RSF.c:352-572) ----------------------------------------
@@ -317,13 +321,13 @@ __device__ __forceinline__ bool ts_interior(const KArgs &a, long m3)
     return m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1;
 }
 
-// Pass 1: the last lambda set in each block (node order); error flags.
-__global__ __launch_bounds__(POST_BLOCK) void k_ts_lambda(KArgs a, TsArgs T, int *__restrict__ bset,
-                                                          double *__restrict__ bval, int *__restrict__ err)
+// Pass 1: the last lambda set in block b of the nodes (node order); error flags.
+__device__ __forceinline__ void ts_lambda_block(const KArgs &a, const TsArgs &T, long b, int *__restrict__ bset,
+                                                double *__restrict__ bval, int *__restrict__ err)
 {
     __shared__ int last[POST_BLOCK];
     __shared__ double val[POST_BLOCK];
-    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    const long m3 = b * POST_BLOCK + threadIdx.x;
     int set = 0;
     double lam = 0.0;
     if (m3 < a.nodenum && a.p.RSFmode == 1 && ts_interior(a, m3)) {
@@ -339,18 +343,23 @@ __global__ __launch_bounds__(POST_BLOCK) void k_ts_lambda(KArgs a, TsArgs T, int
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        bset[blockIdx.x] = last[0] >= 0;
-        bval[blockIdx.x] = last[0] >= 0 ? val[last[0]] : 0.0;
+        bset[b] = last[0] >= 0;
+        bval[b] = last[0] >= 0 ? val[last[0]] : 0.0;
     }
+}
+
+__global__ __launch_bounds__(POST_BLOCK) void k_ts_lambda(KArgs a, TsArgs T, int *__restrict__ bset,
+                                                          double *__restrict__ bval, int *__restrict__ err)
+{
+    ts_lambda_block(a, T, blockIdx.x, bset, bval, err);
 }
 
 // Pass 2: lambda carried into every block, the last one set before it (or
 // lambda0).  One workgroup: each thread takes a contiguous run of blocks; an
 // inclusive scan of (set, value) with "the later set one wins" gives each
 // run's carry-in.
-__global__ __launch_bounds__(FIN_BLOCK) void k_ts_carry(const int *__restrict__ bset,
-                                                        const double *__restrict__ bval, int nblocks,
-                                                        double lambda0, double *__restrict__ carry)
+__device__ __forceinline__ void ts_carry_block(const int *__restrict__ bset, const double *__restrict__ bval,
+                                               int nblocks, double lambda0, double *__restrict__ carry)
 {
     __shared__ int sset[FIN_BLOCK];
     __shared__ double sval[FIN_BLOCK];
@@ -388,15 +397,22 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_ts_carry(const int *__restrict__ 
     }
 }
 
+__global__ __launch_bounds__(FIN_BLOCK) void k_ts_carry(const int *__restrict__ bset,
+                                                        const double *__restrict__ bval, int nblocks,
+                                                        double lambda0, double *__restrict__ carry)
+{
+    ts_carry_block(bset, bval, nblocks, lambda0, carry);
+}
+
 // Pass 3: every interior node's update functions, composed in node order
-// within the block.
-__global__ __launch_bounds__(POST_BLOCK) void k_ts_node(KArgs a, TsArgs T, const double *__restrict__ carry,
-                                                        Fold *__restrict__ part)
+// within block b.
+__device__ __forceinline__ void ts_node_block(const KArgs &a, const TsArgs &T, long b,
+                                              const double *__restrict__ carry, Fold *__restrict__ part)
 {
     __shared__ int last[POST_BLOCK];
     __shared__ double val[POST_BLOCK];
     __shared__ Fold F[NTS][POST_BLOCK];
-    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    const long m3 = b * POST_BLOCK + threadIdx.x;
     const bool in = m3 < a.nodenum && ts_interior(a, m3);
     const bool rsf = in && a.p.RSFmode == 1;
     int set = 0;
@@ -443,7 +459,7 @@ __global__ __launch_bounds__(POST_BLOCK) void k_ts_node(KArgs a, TsArgs T, const
         }
         if (rsf) {
             const int li = last[threadIdx.x];
-            double lam = li >= 0 ? val[li] : carry[blockIdx.x];
+            double lam = li >= 0 ? val[li] : carry[b];
             lam = ref_minv(lam, 1.0 / 5.0);
             const double bs = NF_(bs)[m3], Ls = NF_(Ls)[m3], V0s = NF_(V0s)[m3];
             if (bs > 0) {
@@ -474,13 +490,19 @@ __global__ __launch_bounds__(POST_BLOCK) void k_ts_node(KArgs a, TsArgs T, const
         }
         __syncthreads();
     }
-    if (threadIdx.x < NTS) part[(long)blockIdx.x * NTS + threadIdx.x] = F[threadIdx.x][0];
+    if (threadIdx.x < NTS) part[b * NTS + threadIdx.x] = F[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(POST_BLOCK) void k_ts_node(KArgs a, TsArgs T, const double *__restrict__ carry,
+                                                        Fold *__restrict__ part)
+{
+    ts_node_block(a, T, blockIdx.x, carry, part);
 }
 
 // Pass 4: blocks composed in order, applied to the reference's initial
 // values; out = dt, dtnvp, dtd, dtphi, dtL, maxvel, maxVx.
-__global__ __launch_bounds__(FIN_BLOCK) void k_ts_final(const Fold *__restrict__ part, int nblocks, TsArgs T,
-                                                        double *__restrict__ out)
+__device__ __forceinline__ void ts_final_block(const Fold *__restrict__ part, int nblocks, const TsArgs &T,
+                                               double *__restrict__ out)
 {
     __shared__ Fold F[NTS][FIN_BLOCK];
     Fold acc[NTS];
@@ -496,6 +518,12 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_ts_final(const Fold *__restrict__
     out[4] = dtL;
     out[5] = maxvel;
     out[6] = maxVx;
+}
+
+__global__ __launch_bounds__(FIN_BLOCK) void k_ts_final(const Fold *__restrict__ part, int nblocks, TsArgs T,
+                                                        double *__restrict__ out)
+{
+    ts_final_block(part, nblocks, T, out);
 }
 
 __global__ void k_post_clear(int *p, int n)

@@ -439,14 +439,18 @@ def config_ensemble(n_models=64, seed=DEFAULT_SEED, small=False):
         yield k, cfg
 
 
-def config_ensemble_production(n_models=24, seed=DEFAULT_SEED):
+def config_ensemble_production(n_models=24, seed=DEFAULT_SEED, coseismic_dt=None):
     """n_models production-size models (1501x301 nodes refined to 500 m, 36
     markers per cell, 16.2 M markers each) sharing one seeded marker cloud;
     each model has its own convergence rate, 2 to 8 cm/yr (nodal velocities
     and strain rates of the corner flow scaled), and a-b of the
-    velocity-weakening zone ~ U(-0.006, 0.004).  Yields (index, Config); the
-    host holds the one cloud."""
+    velocity-weakening zone ~ U(-0.006, 0.004).  coseismic_dt: the shared
+    cloud in the coseismic regime of coseismic_run(dt = coseismic_dt), its
+    fault markers' RSF state shared too.  Yields (index, Config); the host
+    holds the one cloud."""
     base = config_production(seed)
+    if coseismic_dt is not None:
+        coseismic_run(base, seed=seed, dt=coseismic_dt)
     rng = np.random.default_rng(seed + 7)
     for k in range(n_models):
         conv = rng.uniform(2.0, 8.0) * CM_PER_YR
@@ -458,7 +462,7 @@ def config_ensemble_production(n_models=24, seed=DEFAULT_SEED):
         prm = type(base.params).from_buffer_copy(base.params)
         prm.a_vw = prm.b_vw + amb
         yield k, Config(f"production-ensemble-{k}", base.grid, prm, base.mats, base.markers, nod,
-                        base.timestep, base.timesum, base.geom)
+                        base.timestep, base.timesum, base.geom, base.fault, base.rsf)
 
 
 def post_solve_nodal(cfg, seed=DEFAULT_SEED, degenerate=True):
@@ -665,4 +669,23 @@ def coseismic(cfg, seed=DEFAULT_SEED, dt=1e-2):
     r["V"][f] = V
     r["phi"][f] = phi
     cfg.timestep = dt
+    return cfg
+
+
+def coseismic_run(cfg, seed=DEFAULT_SEED, dt=1e-3, scale=1e6):
+    """cfg set up for a run of coseismic steps (returns cfg, changed in place,
+    timesum = 1 s, timestep = dt): nodal velocities and strain rates of the
+    corner flow scaled by `scale` (1e6: a few cm/yr become a few mm/s, and
+    calc_timestep's next step a fraction of a second), nodal
+    slip rates RK_sliprs_final log-uniform on [1e-3, 10] m/s with
+    RK_sliprs_solved within a factor e^0.5 of them, and the fault markers of
+    coseismic(cfg, seed, dt)."""
+    for k in ("vx", "vy", "mvx", "mvy", "exx", "exy", "esp", "exxpl"):
+        cfg.nodal[k] = cfg.nodal[k] * scale
+    rng = np.random.default_rng(seed + 17)
+    shape = cfg.nodal["RK_sliprs_final"].shape
+    cfg.nodal["RK_sliprs_final"] = 10.0 ** rng.uniform(-3.0, 1.0, shape)
+    cfg.nodal["RK_sliprs_solved"] = cfg.nodal["RK_sliprs_final"] * np.exp(rng.uniform(-0.5, 0.5, shape))
+    coseismic(cfg, seed=seed, dt=dt)
+    cfg.timesum = 1.0
     return cfg
