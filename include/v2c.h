@@ -251,6 +251,22 @@ int v2c_ductilevisc_batch(v2c_ctx *ctx, long n, const char *type,
                           const double *tk, const double *pb,
                           double *nu_out, int *iters);
 
+/* v2c_ductilevisc_batch without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  The
+ * inputs and results are staged in a static device array of the library
+ * (45 bytes per input, 11.25 MiB per device, held until the process ends),
+ * so n is at most V2C_DUCTILEVISC_NOALLOC_MAX (V2C_EINVAL with a message
+ * above it).  Every call on one device shares that device's array: calls
+ * that overlap on one device (from several threads or contexts) race.
+ * Results equal v2c_ductilevisc_batch's. */
+#define V2C_DUCTILEVISC_NOALLOC_MAX (1L << 18)
+int v2c_ductilevisc_batch_noalloc(v2c_ctx *ctx, long n, const char *type,
+                                  const double *sxy, const double *sxx,
+                                  const double *exy, const double *exx,
+                                  const double *tk, const double *pb,
+                                  double *nu_out, int *iters);
+
 /* ---- post-solve nodal stencils (device-resident nodal fields) ------------
  * The reference's per-node routines between the Stokes solve and the next
  * marker2nodes, on the context's nodal fields.  Each writes exactly the

@@ -235,6 +235,9 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                        ("v2c_ductilevisc_batch", [C.c_long, C.c_char_p, P(C.c_double), P(C.c_double),
                                                   P(C.c_double), P(C.c_double), P(C.c_double),
                                                   P(C.c_double), P(C.c_double), P(C.c_int)]),
+                       ("v2c_ductilevisc_batch_noalloc", [C.c_long, C.c_char_p, P(C.c_double), P(C.c_double),
+                                                          P(C.c_double), P(C.c_double), P(C.c_double),
+                                                          P(C.c_double), P(C.c_double), P(C.c_int)]),
                        ("v2c_enable_timing", [C.c_int]),
                        ("v2c_set_friction_fields", [C.c_int]),
                        ("v2c_copy_calibration", [C.c_long, P(C.c_double)]),
@@ -404,15 +407,18 @@ class Context:
                     "compute_after_solve")
         return d.value, nerr.value
 
-    def ductilevisc_batch(self, t, sxy, sxx, exy, exx, tk, pb):
+    def ductilevisc_batch(self, t, sxy, sxx, exy, exx, tk, pb, noalloc=False):
+        """noalloc: v2c_ductilevisc_batch_noalloc (static staging, at most
+        NOALLOC_MAX inputs)."""
         n = len(t)
         t8 = np.ascontiguousarray(t, dtype=np.int8)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (sxy, sxx, exy, exx, tk, pb)]
         out = np.empty(n)
         it = np.empty(n, dtype=np.int32)
-        self._check(self.lib.v2c_ductilevisc_batch(
-            self.h, n, C.cast(t8.ctypes.data, C.c_char_p), *[_dptr(a) for a in arrs], _dptr(out),
-            it.ctypes.data_as(C.POINTER(C.c_int))), "ductilevisc_batch")
+        fn = self.lib.v2c_ductilevisc_batch_noalloc if noalloc else self.lib.v2c_ductilevisc_batch
+        self._check(fn(self.h, n, C.cast(t8.ctypes.data, C.c_char_p), *[_dptr(a) for a in arrs], _dptr(out),
+                       it.ctypes.data_as(C.POINTER(C.c_int))),
+                    "ductilevisc_batch_noalloc" if noalloc else "ductilevisc_batch")
         return out, it
 
     def enable_timing(self, on=True):
@@ -484,6 +490,7 @@ class TileShape(C.Structure):
 
 
 ENS_CLASSES = 3
+NOALLOC_MAX = 1 << 18          # V2C_DUCTILEVISC_NOALLOC_MAX
 
 
 def _bind_ens(lib):

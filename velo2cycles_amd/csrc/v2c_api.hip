@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -19,6 +20,8 @@ struct TimedLaunch { int kernel; hipEvent_t start, stop; };
 struct v2c_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // device address of this device's `scratch` array (noalloc staging)
+    unsigned char *scratch = nullptr;
     // marker2nodes runs its per-marker property pass here, beside the
     // re-ordering on `stream` (default priority, like `stream`)
     hipStream_t side = nullptr;
@@ -346,6 +349,38 @@ const char *v2c_nodal_name(int field)
 
 const char *v2c_last_error(const v2c_ctx *c) { return c ? c->err : "null context"; }
 
+// ---- static staging of v2c_ductilevisc_batch_noalloc ------------------------
+// Five double columns (sxy, sxx, tk, pb, result), the iteration counts and
+// the types, each column V2C_DUCTILEVISC_NOALLOC_MAX entries long: 45 bytes
+// per input, 11.25 MiB per device, held until the process ends.  In host
+// code `scratch` names a host-side stand-in, not device memory: the device
+// address of each device's copy comes from hipGetSymbolAddress, taken once
+// per device at context creation (scratch_address) and kept in
+// g_scratch[device]; only that address reaches kernels and copies.
+constexpr size_t NOALLOC_COL = (size_t)V2C_DUCTILEVISC_NOALLOC_MAX * sizeof(double);
+constexpr size_t NOALLOC_BYTES = 5 * NOALLOC_COL + (size_t)V2C_DUCTILEVISC_NOALLOC_MAX * (sizeof(int) + 1);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device`, which must be current.
+static hipError_t scratch_address(int device, unsigned char **out)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch[device] = (unsigned char *)p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
 static void free_all(v2c_ctx *c)
 {
     void *ptrs[] = {c->d_gx, c->d_gy, c->d_bx, c->d_by, c->d_cellx, c->d_celly, c->d_mat, c->d_mk, c->d_t, c->d_key, c->d_skey, c->d_iota,
@@ -404,6 +439,7 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
         }                                                                             \
     } while (0)
     CK(hipSetDevice(device));
+    CK(scratch_address(device, &c->scratch));
     CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     CK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -839,20 +875,12 @@ int v2c_compute_after_solve(v2c_ctx *c, double dt, int rk, double *deltas, long 
     return 0;
 }
 
-int v2c_ductilevisc_batch(v2c_ctx *c, long n, const char *type, const double *sxy, const double *sxx,
-                          const double *exy, const double *exx, const double *tk, const double *pb,
-                          double *nu_out, int *iters)
+// Inputs to the device, k_ductile_batch, results back, on the context's
+// stream; d_* are device columns of at least n entries.
+static hipError_t ductile_staged(v2c_ctx *c, long n, const char *type, const double *sxy, const double *sxx,
+                                 const double *tk, const double *pb, double *nu_out, int *iters, double *d_sxy,
+                                 double *d_sxx, double *d_tk, double *d_pb, double *d_out, int *d_it, int8_t *d_t)
 {
-    (void)exy; (void)exx;   // accepted but unused, as in density_ductilevisc_calc.c:114
-    if (!c || n < 0 || !type || !sxy || !sxx || !tk || !pb || !nu_out) return V2C_EINVAL;
-    if (n == 0) return 0;
-    HIPCHK(c, hipSetDevice(c->device));
-    char *buf = nullptr;
-    const size_t bytes = (size_t)n * (5 * sizeof(double) + sizeof(int) + 1);
-    HIPCHK(c, hipMalloc(&buf, bytes));
-    double *d_sxy = (double *)buf, *d_sxx = d_sxy + n, *d_tk = d_sxx + n, *d_pb = d_tk + n, *d_out = d_pb + n;
-    int *d_it = (int *)(d_out + n);
-    int8_t *d_t = (int8_t *)(d_it + n);
     hipError_t e = hipMemcpyAsync(d_sxy, sxy, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_sxx, sxx, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_tk, tk, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -868,8 +896,54 @@ int v2c_ductilevisc_batch(v2c_ctx *c, long n, const char *type, const double *sx
     if (e == hipSuccess && iters) e = hipMemcpyAsync(iters, d_it, n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipStreamSynchronize(c->stream);
+    return e;
+}
+
+int v2c_ductilevisc_batch(v2c_ctx *c, long n, const char *type, const double *sxy, const double *sxx,
+                          const double *exy, const double *exx, const double *tk, const double *pb,
+                          double *nu_out, int *iters)
+{
+    (void)exy; (void)exx;   // accepted but unused, as in density_ductilevisc_calc.c:114
+    if (!c || n < 0 || !type || !sxy || !sxx || !tk || !pb || !nu_out) return V2C_EINVAL;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    char *buf = nullptr;
+    const size_t bytes = (size_t)n * (5 * sizeof(double) + sizeof(int) + 1);
+    HIPCHK(c, hipMalloc(&buf, bytes));
+    double *d_sxy = (double *)buf, *d_sxx = d_sxy + n, *d_tk = d_sxx + n, *d_pb = d_tk + n, *d_out = d_pb + n;
+    int *d_it = (int *)(d_out + n);
+    int8_t *d_t = (int8_t *)(d_it + n);
+    const hipError_t e = ductile_staged(c, n, type, sxy, sxx, tk, pb, nu_out, iters, d_sxy, d_sxx, d_tk, d_pb,
+                                        d_out, d_it, d_t);
     (void)hipFree(buf);
     if (e != hipSuccess) return set_err(c, V2C_EHIP, "ductilevisc_batch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int v2c_ductilevisc_batch_noalloc(v2c_ctx *c, long n, const char *type, const double *sxy, const double *sxx,
+                                  const double *exy, const double *exx, const double *tk, const double *pb,
+                                  double *nu_out, int *iters)
+{
+    (void)exy; (void)exx;
+    if (!c || n < 0 || !type || !sxy || !sxx || !tk || !pb || !nu_out) return V2C_EINVAL;
+    if (n > V2C_DUCTILEVISC_NOALLOC_MAX)
+        return set_err(c, V2C_EINVAL,
+                       "ductilevisc_batch_noalloc: n = %ld exceeds the static staging limit of %ld inputs "
+                       "(V2C_DUCTILEVISC_NOALLOC_MAX); use v2c_ductilevisc_batch or split the batch",
+                       n, (long)V2C_DUCTILEVISC_NOALLOC_MAX);
+    if (n == 0) return 0;
+    if (!c->scratch) return set_err(c, V2C_EHIP, "ductilevisc_batch_noalloc: no staging address for device %d",
+                                    c->device);
+    HIPCHK(c, hipSetDevice(c->device));
+    // fixed column layout: every column starts on a 256-byte boundary
+    unsigned char *b = c->scratch;
+    double *d_sxy = (double *)b, *d_sxx = (double *)(b + NOALLOC_COL), *d_tk = (double *)(b + 2 * NOALLOC_COL);
+    double *d_pb = (double *)(b + 3 * NOALLOC_COL), *d_out = (double *)(b + 4 * NOALLOC_COL);
+    int *d_it = (int *)(b + 5 * NOALLOC_COL);
+    int8_t *d_t = (int8_t *)(b + 5 * NOALLOC_COL + (size_t)V2C_DUCTILEVISC_NOALLOC_MAX * sizeof(int));
+    const hipError_t e = ductile_staged(c, n, type, sxy, sxx, tk, pb, nu_out, iters, d_sxy, d_sxx, d_tk, d_pb,
+                                        d_out, d_it, d_t);
+    if (e != hipSuccess) return set_err(c, V2C_EHIP, "ductilevisc_batch_noalloc: %s", hipGetErrorString(e));
     return 0;
 }
 
