@@ -211,6 +211,15 @@ int v2c_order_get(v2c_ctx *ctx, unsigned int *order, unsigned int *skey, long n)
 int v2c_nodal_set(v2c_ctx *ctx, int field, const double *host);
 int v2c_nodal_get(v2c_ctx *ctx, int field, double *host);
 
+/* Several nodal fields per transfer: fields[i] (V2C_N_*) from / to host[i]
+ * (nodenum doubles each).  The library packs them into one pinned staging
+ * buffer and moves them in one copy per direction (24 fields per copy; the
+ * buffers, pinned host and device, are allocated at the first call).  No
+ * reference counterpart: the reference keeps its nodal arrays in host
+ * memory; the drop-in binding uses these for its residency hand-overs. */
+int v2c_nodal_set_many(v2c_ctx *ctx, const int *fields, const double *const *host, int n);
+int v2c_nodal_get_many(v2c_ctx *ctx, const int *fields, double *const *host, int n);
+
 /* ---- hot-path entry points ---------------------------------------------- */
 
 /* marker2nodes() (interpolation.c:55-916, called from i2evps.c:89):

@@ -12,6 +12,7 @@
  * IN sets reference globals by name; the ops run in order; OUT receives every
  * global of the table after the last op.  Ops:
  *   m2n | n2m | move | before:<dt>:<rk> | after:<dt>:<rk> | ductile | allinteri
+ *   hostsolve | hostout (tests/native/host_stand_in.h; hostout's sum in drv_ret[3])
  *   stress:<dt>:<rk> | c2b:<dt> | c2bold:<dt> | b2c | copy:<rk> | tstep:<oldornew>
  *   bcvxvy | bcp | bctk | save:<cycle>:<path> | load:<path>
  * (tstep leaves the returned time step in drv_ret[2]; save and load are the
@@ -87,6 +88,9 @@
 #include "../velo2cycles_amd/csrc/dropin/v2c_dropin.c"
 #endif
 
+/* host stand-ins shared with the binding's test host */
+#include "../tests/native/host_stand_in.h"
+
 /* driver-local arrays for the pure-function ops */
 #define DRV_MAX 2000000
 static double dv_sxy[DRV_MAX], dv_sxx[DRV_MAX], dv_tk[DRV_MAX], dv_pb[DRV_MAX], dv_out[DRV_MAX];
@@ -95,6 +99,9 @@ static double ai_x[DRV_MAX], ai_y[DRV_MAX], ai_vx[DRV_MAX], ai_vy[DRV_MAX], ai_e
 static long drv_n;
 static double drv_ret[8];
 static double drv_time;   /* wall seconds of the op sequence */
+/* wall seconds per routine: marker2nodes, compute_before_solve,
+ * compute_after_solve, nodes2markers, moveandrotatemarkers */
+static double drv_optime[5];
 static double RK_sliprs0[MAXNOD], RK_sliprs1[MAXNOD], RK_phidts0[MAXNOD], RK_phidts1[MAXNOD];
 
 typedef struct { const char *name; char kind; void *ptr; long cap; } entry;
@@ -146,7 +153,7 @@ static entry table[] = {
     D(marke0), D(marke1), D(markf0), D(markf1), D(markv),
     /* pure-function op arrays */
     D(dv_sxy), D(dv_sxx), D(dv_tk), D(dv_pb), D(dv_out), {"dv_type", 'c', (void *)dv_type, DRV_MAX},
-    D(ai_x), D(ai_y), D(ai_vx), D(ai_vy), D(ai_esp), D(ai_ee), L(drv_n), D(drv_ret), S(drv_time),
+    D(ai_x), D(ai_y), D(ai_vx), D(ai_vy), D(ai_esp), D(ai_ee), L(drv_n), D(drv_ret), S(drv_time), D(drv_optime),
 };
 #define NTAB ((long)(sizeof table / sizeof table[0]))
 
@@ -233,11 +240,16 @@ int main(int argc, char **argv)
     for (int a = 3; a < argc; a++) {
         const char *op = argv[a];
         double dt; int rk;
-        if (!strcmp(op, "m2n")) marker2nodes();
-        else if (!strcmp(op, "n2m")) nodes2markers();
-        else if (!strcmp(op, "move")) moveandrotatemarkers();
-        else if (sscanf(op, "before:%lf:%d", &dt, &rk) == 2) drv_ret[0] = compute_before_solve(dt, rk);
-        else if (sscanf(op, "after:%lf:%d", &dt, &rk) == 2) drv_ret[1] = compute_after_solve(dt, rk);
+        struct timespec o0, o1;
+        int slot = -1;
+        clock_gettime(CLOCK_MONOTONIC, &o0);
+        if (!strcmp(op, "m2n")) { marker2nodes(); slot = 0; }
+        else if (!strcmp(op, "n2m")) { nodes2markers(); slot = 3; }
+        else if (!strcmp(op, "move")) { moveandrotatemarkers(); slot = 4; }
+        else if (sscanf(op, "before:%lf:%d", &dt, &rk) == 2) { drv_ret[0] = compute_before_solve(dt, rk); slot = 1; }
+        else if (sscanf(op, "after:%lf:%d", &dt, &rk) == 2) { drv_ret[1] = compute_after_solve(dt, rk); slot = 2; }
+        else if (!strcmp(op, "hostsolve")) stand_in_stokes();
+        else if (!strcmp(op, "hostout")) drv_ret[3] = stand_in_output();
         else if (!strcmp(op, "ductile")) {
             for (long i = 0; i < drv_n; i++)
                 dv_out[i] = ductilevisc((int)dv_type[i], dv_sxy[i], dv_sxx[i], 0, 0, dv_tk[i], dv_pb[i]);
@@ -265,6 +277,10 @@ int main(int argc, char **argv)
         else if (sscanf(op, "heat:%lf", &dt) == 1) v2c_dropin_temperature_markers(dt);
 #endif
         else { fprintf(stderr, "ref_driver: unknown op %s\n", op); return 4; }
+        if (slot >= 0) {
+            clock_gettime(CLOCK_MONOTONIC, &o1);
+            drv_optime[slot] += (double)(o1.tv_sec - o0.tv_sec) + 1e-9 * (double)(o1.tv_nsec - o0.tv_nsec);
+        }
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);
     drv_time = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);

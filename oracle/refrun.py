@@ -79,6 +79,11 @@ def _read(path):
     return out
 
 
+def read_records(path):
+    """The records of a driver input or output file (name -> array)."""
+    return _read(path)
+
+
 def write_input(path, cfg, ops=None, markers=None, nodal=None, timesum=None, timestep=None, extra=None):
     """Write cfg (+ overrides and `extra` records) as a driver input file."""
     m = markers or cfg.markers

@@ -7,7 +7,8 @@ reference's own routines (oracle/_ref/ref_driver, compiled in place from
 Each fixture holds the exact driver input file (`in_bin`: the record stream
 that tests/native/dropin_harness reads as well), the op list, and the
 reference's outputs after those ops: every marker and nodal field, the
-routines' return values (`drv_ret`), `errcheck` and calc_timestep's scalars.
+routines' return values (`drv_ret`; [3] the host output stand-in's sum),
+`errcheck` and calc_timestep's scalars.
 The drop-in tests run the same input through the reference-side binding and
 libv2c on the GPU and compare with these outputs; neither the reference nor
 this generator is needed there.
@@ -67,6 +68,38 @@ def bc_cases():
         yield f"dropin_{name}", cfg, ["bcvxvy", "bcp", "bctk"], bond_extra(bc)
 
 
+def loop_cases():
+    """The reference loop around the host Stokes solve (i2evps.c:89-113 with
+    solve_momentum_continuity, momentum_continuity_iterate.c:44-129): the
+    binding's routines with the host stand-ins (tests/native/host_stand_in.h)
+    where the reference runs host code, so the drop-in test exercises the
+    binding's residency hand-overs.  One Picard iteration; two (rk 0, 1);
+    rkmod 3 with four stages (dt / 2 for the first two)."""
+    def iteration(dt, rk):
+        return [f"before:{dt}:{rk}", "b2c", "hostsolve", f"stress:{dt}:{rk}", f"c2b:{dt}", f"after:{dt}:{rk}",
+                "tstep:1"]
+    head = ["m2n", f"c2bold:{DT}"]
+    tail = ["hostout", "n2m", "move"]
+    specs = (("rk1", (41, 21, 40e3, 20e3, 3, False), {}, iteration(DT, 0) + ["copy:1"]),
+             ("rk2", (61, 31, 60e3, 30e3, 3, True), {}, iteration(DT, 0) + iteration(DT, 1) + ["copy:2"]),
+             ("rkmod3", (41, 21, 40e3, 20e3, 3, True), dict(rkmod=3),
+              iteration(DT / 2, 0) + iteration(DT / 2, 1) + iteration(DT, 2) + iteration(DT, 3) + ["copy:4"]))
+    for tag, (nx, ny, xs, ys, per, ref), flags, body in specs:
+        cfg = synth.config_grid(nx, ny, xs, ys, per, refine=ref, fault_frac=0.02)
+        cfg.timesum = DT
+        # viscosity cap 3e26 rather than the table's 1e26: calc_timestep
+        # branches on nu_m2c > 1e26 (calc_RSF.c:390, 417), and a node whose
+        # markers all sit at a 1e26 cap is then decided by the last bit of
+        # marker2nodes' weighted average, which no reordered sum reproduces
+        for t in range(256):
+            cfg.mats.n1[t] = 3e26
+        for k, v in flags.items():
+            setattr(cfg.params, k, v)
+        extra = bond_extra(synth.boundary_equations(cfg, 5))
+        extra["maxtmstep"] = ("d", np.array([DT]))
+        yield f"dropin_loop_{tag}", cfg, head + body + tail, extra
+
+
 def write_case(name, cfg, ops, extra):
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
@@ -75,7 +108,7 @@ def write_case(name, cfg, ops, extra):
         raw = refrun.run_file(refrun.DRIVER, fin, ops)
     mk, nd = refrun.split_output(raw, cfg.n, cfg.grid.nodenum)
     d = {"in_bin": in_bin, "ops": np.array(json.dumps(list(ops))), "n": np.array(cfg.n),
-         "nodenum": np.array(cfg.grid.nodenum), "drv_ret": raw["drv_ret"][:3],
+         "nodenum": np.array(cfg.grid.nodenum), "drv_ret": raw["drv_ret"][:4],
          "errcheck": raw["errcheck"]}
     for f in MARKER_FIELDS:
         d["out_m_" + f] = mk[f]
@@ -91,7 +124,7 @@ def write_case(name, cfg, ops, extra):
 
 def main():
     assert refrun.available(), "needs the reference sources and oracle/_ref/ref_driver"
-    for gen in (step_cases, post_cases, bc_cases):
+    for gen in (step_cases, post_cases, bc_cases, loop_cases):
         for name, cfg, ops, extra in gen():
             write_case(name, cfg, ops, extra)
 

@@ -17,7 +17,9 @@
  *   int64 count; payload
  * Ops: m2n | n2m | move | before:<dt>:<rk> | after:<dt>:<rk> |
  *      stress:<dt>:<rk> | c2b:<dt> | c2bold:<dt> | b2c | copy:<rk> |
- *      tstep:<oldornew> | bcvxvy | bcp | bctk | heat:<heatdif>
+ *      tstep:<oldornew> | bcvxvy | bcp | bctk | heat:<heatdif> | hostsolve | hostout
+ * (host_stand_in.h; hostsolve between the binding's Stokes hand-overs, hostout after
+ * its output hand-over, its sum in drv_ret[3])
  * before / after / tstep leave their return value in drv_ret[0] / [1] / [2].
  */
 #include <stdio.h>
@@ -79,6 +81,11 @@ int v2c_dropin_attach(long capacity, int device);
 void v2c_dropin_markers_to_host(void);
 void v2c_dropin_detach(void);
 void v2c_dropin_temperature_markers(double heatdif);
+void v2c_dropin_before_stokes(void);
+void v2c_dropin_after_stokes(void);
+void v2c_dropin_before_output(void);
+void v2c_dropin_sync_to_host(void);
+
 void marker2nodes(void);
 double nodes2markers(void);
 void moveandrotatemarkers(void);
@@ -97,6 +104,8 @@ void tkrecalc(void);
 /* host-side columns of RK_sliprs / RK_phidts, as the input file names them */
 static double RK_sliprs0[H_NOD], RK_sliprs1[H_NOD], RK_phidts0[H_NOD], RK_phidts1[H_NOD];
 static double drv_ret[8];
+
+#include "host_stand_in.h"
 
 typedef struct { const char *name; char kind; void *ptr; long cap; } entry;
 #define D(a) {#a, 'd', (void *)(a), (long)(sizeof(a) / sizeof(double))}
@@ -247,8 +256,17 @@ int main(int argc, char **argv)
         else if (!strcmp(op, "bcp")) P_boundarycorrection();
         else if (!strcmp(op, "bctk")) tkrecalc();
         else if (sscanf(op, "heat:%lf", &dt) == 1) v2c_dropin_temperature_markers(dt);
+        else if (!strcmp(op, "hostsolve")) {
+            v2c_dropin_before_stokes();
+            stand_in_stokes();
+            v2c_dropin_after_stokes();
+        } else if (!strcmp(op, "hostout")) {
+            v2c_dropin_before_output();
+            drv_ret[3] = stand_in_output();
+        }
         else { fprintf(stderr, "dropin_harness: unknown op %s\n", op); v2c_dropin_detach(); return 4; }
     }
+    v2c_dropin_sync_to_host();
     v2c_dropin_markers_to_host();
     v2c_dropin_detach();
     if (errcheck) fprintf(stderr, "dropin_harness: errcheck set by the binding\n");

@@ -19,7 +19,7 @@ from velo2cycles_amd import synth
 from velo2cycles_amd.abi import NODAL_FIELDS, Context, Ensemble, MarkerArrays, TimestepParams
 from velo2cycles_amd.warmstart import WarmStart, generate, init_params, upload_cloud
 from oracle.pyoracle import MX_FIELDS, Oracle, local_update
-from tests.util import compare, rel_errors
+from tests.util import check_deltavs, compare, rel_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -125,31 +125,6 @@ def check_local(got, mx, fault, what):
         assert nm == 0 and err <= 1e-10, (what, f, err)
 
 
-def check_deltavs(got, ora, slip_dev, slip_ora, dt, what):
-    """copyarrays' deltaVs = (log(solved) - log(final)) / dt subtracts two
-    nearly equal logarithms, so last-bit differences of its inputs show
-    undamped (one bit of a logarithm is |log V| eps / dt, 1.3e-12 of the
-    field's rms here).  Allowed per node: the first-order effect of the
-    inputs' relative differences, (r_solved + r_final) / dt, the two slip
-    rates being the device's and the oracle's sliprs after this step and the
-    one before, plus two last bits of each logarithm, 8 (|log V| + 1) eps / dt,
-    plus 1e-15 relative for the step."""
-    g, o = got["deltaVs"], ora["deltaVs"]
-
-    def rel(a, b):
-        return np.abs(a - b) / np.where(np.abs(b) > 0, np.abs(b), 1.0)
-
-    r = rel(slip_dev[-1], slip_ora[-1]) + rel(slip_dev[-2], slip_ora[-2])
-    v = slip_ora[-1]
-    lv = np.abs(np.log(np.where(v > 0, v, 1.0)))
-    allow = (r + 8.0 * np.finfo(np.float64).eps * (lv + 1.0)) / dt + 1e-15 * np.abs(o)
-    both_nan = np.isnan(g) & np.isnan(o)
-    ok = both_nan | (np.abs(g - o) <= allow)
-    worst = np.max(np.where(both_nan, 0.0, np.abs(g - o) / allow))
-    assert ok.all(), (what, int((~ok).sum()), float(worst))
-    assert np.max(r) <= 1e-12, (what, float(np.max(r)))
-
-
 def same_model(ca, cb, n, what):
     """Two contexts hold bit-identical markers, nodal fields and local outputs."""
     ma, mb = MarkerArrays(n), MarkerArrays(n)
@@ -203,7 +178,11 @@ def test_combined_small_models_against_oracle(gpu, net):
         assert DT_MIN <= min(dts) and max(dts) < DT_MAX, dts
         bad = compare(mb, o.markers, nb, o.nodal, skip=("deltaVs",))
         assert not bad, f"model {k}:\n" + "\n".join(bad)
-        check_deltavs(nb, o.nodal, slip[k], slip_ora, dts[steps - 1], f"model {k}")
+        bad = check_deltavs(nb["deltaVs"], o.nodal["deltaVs"], (slip[k][-1], slip_ora[-1]),
+                            (slip[k][-2], slip_ora[-2]), dts[steps - 1])
+        assert not bad, (k, bad)
+        r = rel_errors(slip[k][-1], slip_ora[-1])[0]
+        assert r <= 1e-12, (k, r)
         fault = cfg.rsf["fault"].astype(bool)
         check_local(lb, mx, fault, f"model {k}")
         st = eb.local_stats(k)

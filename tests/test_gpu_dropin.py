@@ -7,7 +7,10 @@ writes the globals back.  The binding forwards every call to libv2c on the GPU.
 Each case replays a fixture that the reference's own routines wrote from the
 same input file (tests/golden/dropin_*.npz, made by
 tests/golden/make_dropin_golden.py in a container holding the reference
-sources); nothing built from the reference runs here.  The marker part of the
+sources); nothing built from the reference runs here.  The loop cases run
+host stand-ins (tests/native/host_stand_in.h) where the reference runs host
+code, between the binding's hand-overs, so the binding's residency tracking
+(what crosses PCIe when) is checked against the reference's plain globals.  The marker part of the
 heat step has no stand-alone reference routine and is compared with the
 oracle.  Bar as tests/util.py: positions and types bit-exact, FP64 within
 1e-12 scaled."""
@@ -23,7 +26,7 @@ import pytest
 from oracle import refrun
 from velo2cycles_amd import synth
 from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS, MarkerArrays
-from tests.util import compare, run_oracle
+from tests.util import check_deltavs, compare, run_oracle
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
@@ -55,7 +58,7 @@ def _close(a, b, rel=1e-12):
 
 
 def test_fixtures_present():
-    assert len(FIXTURES) == 12, FIXTURES
+    assert len(FIXTURES) == 15, FIXTURES
 
 
 @pytest.mark.parametrize("path", FIXTURES, ids=lambda p: os.path.basename(p)[7:-4])
@@ -74,8 +77,22 @@ def test_binding_matches_reference_fixture(gpu, harness, path):
     rn = {k: z["out_n_" + k] for k in NODAL_FIELDS}
     with_markers = n > 0 and any(o in ("m2n", "n2m", "move") for o in ops)
     skip = N2M_SCRATCH if "n2m" in ops else ()
+    copies = [o for o in ops if o.startswith("copy:")]
+    if copies:
+        skip = skip + ("deltaVs",)
     bad = compare(MarkerArrays(n, dm) if with_markers else None, MarkerArrays(n, rm) if with_markers else None,
                   dn, rn, skip=skip)
+    if copies:
+        # the one copyarrays of the case reads the input's RK_sliprs_final
+        # and the RK_sliprs_solved it leaves
+        assert len(copies) == 1
+        with tempfile.TemporaryDirectory() as td:
+            fin = os.path.join(td, "in.bin")
+            z["in_bin"].tofile(fin)
+            inp = refrun.read_records(fin)
+        final = inp["RK_sliprs_final"][:nn]
+        bad += check_deltavs(dn["deltaVs"], rn["deltaVs"], (dn["RK_sliprs_solved"], rn["RK_sliprs_solved"]),
+                             (final, final), float(inp["timestep"][0]))
     assert not bad, "\n".join(bad)
     if any(o.startswith("after:") for o in ops):
         assert _close(raw["drv_ret"][1], z["drv_ret"][1])      # compute_after_solve's misfit
@@ -83,6 +100,8 @@ def test_binding_matches_reference_fixture(gpu, harness, path):
         assert _close(raw["drv_ret"][2], z["drv_ret"][2])      # calc_timestep's return
         for k in SCALARS:
             assert _close(raw[k][0], z["s_" + k][0]), k
+    if "hostout" in ops:
+        assert _close(raw["drv_ret"][3], z["drv_ret"][3])      # what host output code read
     assert raw["errcheck"][0] == z["errcheck"][0]
 
 

@@ -354,3 +354,27 @@ def test_async_and_timed_runs_match_sync(gpu):
     ms_props, n_props = ca.kernel_time("m2n_props")
     assert n_props == 2 and ms_props > 0
     ca.close()
+
+
+def test_many_field_transfers_roundtrip(gpu):
+    """v2c_nodal_set_many / get_many (staged, 24 fields per copy): 30 fields
+    in, read back one by one and all at once, bit for bit; fields not named
+    are left alone."""
+    from velo2cycles_amd.abi import Context
+    cfg = synth.config_grid(61, 31, 60e3, 30e3, 2, seed=3)
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=1, lib=gpu)
+    nn = cfg.grid.nodenum
+    rng = np.random.default_rng(3)
+    names = list(NODAL_FIELDS[:30])
+    untouched = NODAL_FIELDS[30]
+    ctx.set_nodal(untouched, np.full(nn, 7.0))
+    vals = {k: rng.standard_normal(nn) for k in names}
+    print("case many-field transfers", len(names), flush=True)
+    ctx.set_nodal_many(vals)
+    for k in names:
+        assert np.array_equal(ctx.get_nodal(k), vals[k]), k
+    got = ctx.get_nodal_many(names[::-1])
+    for k in names:
+        assert np.array_equal(got[k], vals[k]), k
+    assert np.array_equal(ctx.get_nodal(untouched), np.full(nn, 7.0))
+    ctx.close()

@@ -216,3 +216,30 @@ def load_post_fixture(path):
         scal["dt"] = float(z[pre + "ret"][2])
         runs[tag] = (ops, out, scal)
     return cfg, runs, float(z["maxtmstep"])
+
+
+def check_deltavs(got, ora, solved, final, dt):
+    """Failures of copyarrays' deltaVs = (log(solved) - log(final)) / dt.
+    It subtracts two nearly equal logarithms, so last-bit differences of its
+    inputs show undamped (one bit of a logarithm is |log V| eps / dt).
+    Allowed per node: the first-order effect of the inputs' relative
+    differences, (r_solved + r_final) / dt, with solved = (device, oracle)
+    and final = (device, oracle) the two slip-rate arrays copyarrays read,
+    plus two last bits of each logarithm, 8 (|log V| + 1) eps / dt, plus
+    1e-15 relative for the step; equal values (infinities of zero slip
+    rates among them) and both NaN (negative ones) pass."""
+    def rel(a, b):
+        return np.abs(a - b) / np.where(np.abs(b) > 0, np.abs(b), 1.0)
+
+    r = rel(solved[0], solved[1]) + rel(final[0], final[1])
+    v = np.asarray(solved[1], dtype=np.float64)
+    lv = np.abs(np.log(np.where(v > 0, v, 1.0)))
+    allow = (r + 8.0 * np.finfo(np.float64).eps * (lv + 1.0)) / dt + 1e-15 * np.abs(ora)
+    both_nan = np.isnan(got) & np.isnan(ora)
+    with np.errstate(invalid="ignore"):
+        ok = both_nan | (got == ora) | (np.abs(got - ora) <= allow)
+    if ok.all():
+        return []
+    with np.errstate(invalid="ignore"):
+        worst = float(np.nanmax(np.where(ok, 0.0, np.abs(got - ora) / allow)))
+    return [f"nodal deltaVs: {int((~ok).sum())} nodes beyond the conditioning bound (worst {worst:.3g}x)"]

@@ -128,6 +128,9 @@ def default_params(xsize: float = 0.0, xnumx: int = 2) -> Params:
     return p
 
 
+P_DOUBLE = C.POINTER(C.c_double)
+
+
 def _dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
@@ -228,6 +231,8 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                        ("v2c_markers_download", [P(Markers), C.c_long]),
                        ("v2c_nodal_set", [C.c_int, P(C.c_double)]),
                        ("v2c_nodal_get", [C.c_int, P(C.c_double)]),
+                       ("v2c_nodal_set_many", [P(C.c_int), P(P(C.c_double)), C.c_int]),
+                       ("v2c_nodal_get_many", [P(C.c_int), P(P(C.c_double)), C.c_int]),
                        ("v2c_marker2nodes", []), ("v2c_nodes2markers", []),
                        ("v2c_moveandrotatemarkers", []), ("v2c_nodes2markers_advect", []),
                        ("v2c_compute_before_solve", [C.c_double, C.c_int, P(C.c_long)]),
@@ -325,6 +330,25 @@ class Context:
     def get_nodal(self, name: str) -> np.ndarray:
         out = np.empty(self.nodenum)
         self._check(self.lib.v2c_nodal_get(self.h, FIELD_ID[name], _dptr(out)), f"nodal_get {name}")
+        return out
+
+    def set_nodal_many(self, nod: dict):
+        """Several fields in one staged transfer (v2c_nodal_set_many)."""
+        names = list(nod)
+        arrs = [np.ascontiguousarray(nod[k], dtype=np.float64) for k in names]
+        for k, a in zip(names, arrs):
+            assert a.size == self.nodenum, (k, a.size)
+        ids = (C.c_int * len(names))(*[FIELD_ID[k] for k in names])
+        ptrs = (P_DOUBLE * len(names))(*[_dptr(a) for a in arrs])
+        self._check(self.lib.v2c_nodal_set_many(self.h, ids, ptrs, len(names)), "nodal_set_many")
+
+    def get_nodal_many(self, names, out: dict | None = None) -> dict:
+        """Several fields in one staged transfer (v2c_nodal_get_many); `out`
+        may hold the destination arrays."""
+        out = out if out is not None else {k: np.empty(self.nodenum) for k in names}
+        ids = (C.c_int * len(names))(*[FIELD_ID[k] for k in names])
+        ptrs = (P_DOUBLE * len(names))(*[_dptr(out[k]) for k in names])
+        self._check(self.lib.v2c_nodal_get_many(self.h, ids, ptrs, len(names)), "nodal_get_many")
         return out
 
     def set_all_nodal(self, nod: dict):

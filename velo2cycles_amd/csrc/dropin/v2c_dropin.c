@@ -22,10 +22,33 @@
  * the device context.  The device owns the marker cloud between
  * v2c_dropin_attach() (after load_model/extra_init) and
  * v2c_dropin_markers_to_host() (before save_model and solve_temperature,
- * which read markers on the host).  Nodal arrays cross PCIe on every call:
- * inputs are sent, outputs fetched, exactly the arrays each routine reads and
- * writes.  The boundary-condition tail of marker2nodes
- * (interpolation.c:918-925) runs on the device inside marker2nodes().
+ * which read markers on the host).  The boundary-condition tail of
+ * marker2nodes (interpolation.c:918-925) runs on the device inside
+ * marker2nodes().
+ *
+ * Nodal arrays are resident on both sides, and the binding tracks which side
+ * holds the newer copy of each.  A routine sends, in one staged transfer,
+ * only the arrays it reads that host code changed since they last crossed,
+ * and leaves its results on the device.  Host code that reads nodal arrays
+ * asks for them first, and host code that writes them says so afterwards:
+ *
+ *   v2c_dropin_before_stokes()   before preparematrix
+ *                                (momentum_continuity_iterate.c:76): brings
+ *                                back what the Stokes assembly, solve, reload
+ *                                and checkerror read
+ *   v2c_dropin_after_stokes()    after reloadVxVyP (:83): pr, vx, vy changed
+ *   v2c_dropin_before_output()   before the sticky-air diagnostics (:129-170)
+ *                                and postproc (i2evps.c:131): the arrays they
+ *                                read each step
+ *   v2c_dropin_host_reads(f, n) / v2c_dropin_host_wrote(f, n)
+ *                                any other host code, by V2C_N_* field ids
+ *   v2c_dropin_sync_to_host()    every array the device changed (before the
+ *                                heat solve, hdf5 output, save_model)
+ *   v2c_dropin_host_wrote_all()  after host code that may change any array
+ *
+ * A step of the reference loop then moves pr, vx, vy to the device and the
+ * 18 arrays the host reads back, instead of every array each routine
+ * touches.
  *
  * solve_temperature (temperature_iterate.c:2-419) keeps its heat solve on
  * the host; its marker part (:181-419) is replaced by a call to
@@ -117,17 +140,120 @@ static void rk_columns(int to_device)
     free(tmp);
 }
 
-static void send(const int *f, int n)
-{
-    for (int i = 0; i < n; i++) die_on(v2c_nodal_set(g_ctx, f[i], nodal_ptr(f[i])), "nodal_set");
-}
-
-static void fetch(const int *f, int n)
-{
-    for (int i = 0; i < n; i++) die_on(v2c_nodal_get(g_ctx, f[i], nodal_ptr(f[i])), "nodal_get");
-}
-
 #define NOF(a) ((int)(sizeof(a) / sizeof((a)[0])))
+
+/* ---- residency of the nodal arrays ---------------------------------------
+ * g_dev_newer[f]: a device routine wrote array f after its host copy was
+ * last brought up to date; g_host_newer[f]: host code wrote it (or it has
+ * never been sent) after the device copy.  The RK_sliprs / RK_phidts
+ * columns (compute_before_solve with rkmod 3) have one pair of flags. */
+static unsigned char g_dev_newer[V2C_NF], g_host_newer[V2C_NF];
+static int g_rk_dev_newer, g_rk_host_newer;
+
+/* Send the host-newer arrays among f[0..n) in one staged transfer. */
+static void send_newer(const int *f, int n)
+{
+    int ids[V2C_NF];
+    const double *src[V2C_NF];
+    int k = 0;
+    for (int i = 0; i < n; i++)
+        if (g_host_newer[f[i]]) {
+            g_host_newer[f[i]] = 0;      /* also drops repeats within f */
+            ids[k] = f[i];
+            src[k++] = nodal_ptr(f[i]);
+        }
+    if (k) die_on(v2c_nodal_set_many(g_ctx, ids, src, k), "nodal_set_many");
+}
+
+/* Bring the device-newer arrays among f[0..n) to the host arrays. */
+static void fetch_newer(const int *f, int n)
+{
+    int ids[V2C_NF];
+    double *dst[V2C_NF];
+    int k = 0;
+    for (int i = 0; i < n; i++)
+        if (g_dev_newer[f[i]]) {
+            g_dev_newer[f[i]] = 0;
+            ids[k] = f[i];
+            dst[k++] = nodal_ptr(f[i]);
+        }
+    if (k) die_on(v2c_nodal_get_many(g_ctx, ids, dst, k), "nodal_get_many");
+}
+
+/* A routine's outputs: the device copy is now the newest, also of arrays
+ * the host never sent (a routine that writes an array only in part lists it
+ * among its inputs too, so the host's values went first). */
+static void device_wrote(const int *f, int n)
+{
+    for (int i = 0; i < n; i++) {
+        g_dev_newer[f[i]] = 1;
+        g_host_newer[f[i]] = 0;
+    }
+}
+
+void v2c_dropin_host_reads(const int *fields, int n)
+{
+    for (int i = 0; i < n; i++)
+        if (fields[i] < 0 || fields[i] >= V2C_NF || !nodal_ptr(fields[i])) {
+            die_on(V2C_EINVAL, "host_reads");
+            return;
+        }
+    fetch_newer(fields, n);
+}
+
+void v2c_dropin_host_wrote(const int *fields, int n)
+{
+    for (int i = 0; i < n; i++) {
+        if (fields[i] < 0 || fields[i] >= V2C_NF || !nodal_ptr(fields[i])) {
+            die_on(V2C_EINVAL, "host_wrote");
+            return;
+        }
+        /* a partial host write over a device-newer array would lose the
+         * device's values: such host code must call host_reads first */
+        if (g_dev_newer[fields[i]]) die_on(V2C_EINVAL, "host_wrote: array not brought back first");
+        g_host_newer[fields[i]] = 1;
+    }
+}
+
+void v2c_dropin_sync_to_host(void)
+{
+    int all[V2C_NF], k = 0;
+    for (int f = 0; f < V2C_NF; f++)
+        if (nodal_ptr(f)) all[k++] = f;
+    fetch_newer(all, k);
+    if (g_rk_dev_newer) {
+        rk_columns(0);
+        g_rk_dev_newer = 0;
+    }
+}
+
+void v2c_dropin_host_wrote_all(void)
+{
+    v2c_dropin_sync_to_host();
+    for (int f = 0; f < V2C_NF; f++)
+        if (nodal_ptr(f)) g_host_newer[f] = 1;
+    g_rk_host_newer = 1;
+}
+
+/* What the host Stokes step reads: preparematrix and its stencils
+ * (momentum_continuity_addmatrix_checkerror.c:22-213, 380-1000: ro, nu, nd,
+ * gg, gd, sxxe, sxye, sppe, mvx, mvy, mrx, mry, bm, pr, vx, vy, sxx, sxy),
+ * reloadVxVyP (momentum_continuity_iterate.c:177-210) and checkerror
+ * (:214-378: pr, vx, vy); reloadVxVyP writes pr, vx, vy. */
+static const int STOKES_READS[] = {V2C_N_ro, V2C_N_nu, V2C_N_nd, V2C_N_gg, V2C_N_gd, V2C_N_sxxe,
+                                   V2C_N_sxye, V2C_N_sppe, V2C_N_mvx, V2C_N_mvy, V2C_N_mrx, V2C_N_mry,
+                                   V2C_N_bm, V2C_N_pr, V2C_N_vx, V2C_N_vy, V2C_N_sxx, V2C_N_sxy};
+static const int STOKES_WRITES[] = {V2C_N_pr, V2C_N_vx, V2C_N_vy};
+void v2c_dropin_before_stokes(void) { fetch_newer(STOKES_READS, NOF(STOKES_READS)); }
+void v2c_dropin_after_stokes(void) { v2c_dropin_host_wrote(STOKES_WRITES, NOF(STOKES_WRITES)); }
+
+/* What the per-step diagnostics after copyarrays
+ * (momentum_continuity_iterate.c:129-170: sxxs, sxy, nu) and postproc
+ * (output_RSF.c:2-72: nu, sliprs, phis_it, syields) read. */
+static const int OUTPUT_READS[] = {V2C_N_sxxs, V2C_N_sxy, V2C_N_nu, V2C_N_sliprs, V2C_N_phis_it,
+                                   V2C_N_syields};
+void v2c_dropin_before_output(void) { fetch_newer(OUTPUT_READS, NOF(OUTPUT_READS)); }
+
 
 static v2c_markers marker_view(void)
 {
@@ -167,6 +293,13 @@ int v2c_dropin_attach(long capacity, int device)
     }
     int rc = v2c_create(&g_ctx, &g, &p, &mat, capacity, device);
     if (rc) return rc;
+    /* the device holds no nodal array yet */
+    for (int f = 0; f < V2C_NF; f++) {
+        g_dev_newer[f] = 0;
+        g_host_newer[f] = nodal_ptr(f) != NULL;
+    }
+    g_rk_dev_newer = 0;
+    g_rk_host_newer = 1;
     /* boundary equations as load_model left them (load_save.c:279-304) */
     if (bondnum > 1) {
         rc = v2c_set_boundary(g_ctx, bondm, &bondn[0][0], &bondv[0][0], bondnum);
@@ -188,8 +321,10 @@ void v2c_dropin_markers_to_device(void)
     die_on(v2c_markers_upload(g_ctx, &m, marknum), "markers_upload");
 }
 
+/* Brings every device-newer nodal array back first. */
 void v2c_dropin_detach(void)
 {
+    if (g_ctx) v2c_dropin_sync_to_host();
     v2c_destroy(g_ctx);
     g_ctx = NULL;
 }
@@ -212,7 +347,7 @@ void marker2nodes(void)
                               V2C_N_pen, V2C_N_nu_m2c, V2C_N_nd_m2c, V2C_N_mvxn, V2C_N_mvyn,
                               V2C_N_mrx0, V2C_N_mry0};
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, (int)(sizeof in / sizeof in[0]));
+    send_newer(in, NOF(in));
     die_on(v2c_marker2nodes(g_ctx), "marker2nodes");
     /* the routine's boundary-condition tail (interpolation.c:918-925) */
     if (bondnum > 1) {
@@ -220,7 +355,7 @@ void marker2nodes(void)
         die_on(v2c_P_boundarycorrection(g_ctx), "P_boundarycorrection");
         if (tempmod) die_on(v2c_tkrecalc(g_ctx), "tkrecalc");
     }
-    fetch(out, (int)(sizeof out / sizeof out[0]));
+    device_wrote(out, NOF(out));
 }
 
 double nodes2markers(void)
@@ -233,9 +368,9 @@ double nodes2markers(void)
     static const int out[] = {V2C_N_phie, V2C_N_sxyee, V2C_N_exyee, V2C_N_pee, V2C_N_sppee,
                               V2C_N_sxxee, V2C_N_exxee, V2C_N_mvxe, V2C_N_mvye};
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, (int)(sizeof in / sizeof in[0]));
+    send_newer(in, NOF(in));
     die_on(v2c_nodes2markers(g_ctx), "nodes2markers");
-    fetch(out, (int)(sizeof out / sizeof out[0]));
+    device_wrote(out, NOF(out));
     return 0;
 }
 
@@ -243,7 +378,7 @@ void moveandrotatemarkers(void)
 {
     static const int in[] = {V2C_N_vx, V2C_N_vy, V2C_N_exx, V2C_N_exy, V2C_N_esp};
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, (int)(sizeof in / sizeof in[0]));
+    send_newer(in, NOF(in));
     die_on(v2c_moveandrotatemarkers(g_ctx), "moveandrotatemarkers");
 }
 
@@ -258,11 +393,14 @@ double compute_before_solve(double dt, int rk)
                               V2C_N_sliprs};
     long nerr = 0;
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, (int)(sizeof in / sizeof in[0]));
-    if (rkmod == 3) rk_columns(1);
+    send_newer(in, NOF(in));
+    if (rkmod == 3 && g_rk_host_newer) {
+        rk_columns(1);
+        g_rk_host_newer = 0;
+    }
     die_on(v2c_compute_before_solve(g_ctx, dt, rk, &nerr), "compute_before_solve");
-    fetch(out, (int)(sizeof out / sizeof out[0]));
-    if (rkmod == 3) rk_columns(0);
+    device_wrote(out, NOF(out));
+    if (rkmod == 3) g_rk_dev_newer = 1;
     if (nerr) errcheck = 1;
     return 0;
 }
@@ -277,9 +415,9 @@ double compute_after_solve(double dt, int rk)
     double deltas = 0;
     long nerr = 0;
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, (int)(sizeof in / sizeof in[0]));
+    send_newer(in, NOF(in));
     die_on(v2c_compute_after_solve(g_ctx, dt, rk, &deltas, &nerr), "compute_after_solve");
-    fetch(out, (int)(sizeof out / sizeof out[0]));
+    device_wrote(out, NOF(out));
     if (nerr) errcheck = 1;
     return deltas;
 }
@@ -297,9 +435,9 @@ int recompute_stress_strainrate(double dt, int rk)
                               V2C_N_deltasxy};
     v2c_stress_stats st;
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, NOF(in));
+    send_newer(in, NOF(in));
     die_on(v2c_recompute_stress_strainrate(g_ctx, dt, rk, &st), "recompute_stress_strainrate");
-    fetch(out, NOF(out));
+    device_wrote(out, NOF(out));
     maxdphi = st.maxdphi;
     maxnvp = st.maxnvp;
     printf("VISKOS: min = %e max = %e \n", st.minnu, st.maxnu);
@@ -313,9 +451,9 @@ double interpolate_center2basic(double dt)
     static const int in[] = {V2C_N_exx, V2C_N_sxx, V2C_N_sxxe, V2C_N_gd, V2C_N_nd_m2c, V2C_N_pr,
                              V2C_N_exxs, V2C_N_exxpl, V2C_N_sxxs, V2C_N_prs, V2C_N_exxel};
     static const int out[] = {V2C_N_exxs, V2C_N_exxpl, V2C_N_sxxs, V2C_N_prs, V2C_N_exxel};
-    send(in, NOF(in));
+    send_newer(in, NOF(in));
     die_on(v2c_interpolate_center2basic(g_ctx, dt), "interpolate_center2basic");
-    fetch(out, NOF(out));
+    device_wrote(out, NOF(out));
     return 0;
 }
 
@@ -324,9 +462,9 @@ double interpolate_center2basic_old(double dt)
     static const int in[] = {V2C_N_exxe, V2C_N_sxxe, V2C_N_sppe, V2C_N_exxes, V2C_N_sxxes, V2C_N_sppes};
     static const int out[] = {V2C_N_exxes, V2C_N_sxxes, V2C_N_sppes};
     (void)dt;   /* unused by the reference routine too */
-    send(in, NOF(in));
+    send_newer(in, NOF(in));
     die_on(v2c_interpolate_center2basic_old(g_ctx), "interpolate_center2basic_old");
-    fetch(out, NOF(out));
+    device_wrote(out, NOF(out));
     return 0;
 }
 
@@ -334,9 +472,9 @@ double interpolate_basic2center(void)
 {
     static const int in[] = {V2C_N_nu, V2C_N_nd};
     static const int out[] = {V2C_N_nd};
-    send(in, NOF(in));
+    send_newer(in, NOF(in));
     die_on(v2c_interpolate_basic2center(g_ctx), "interpolate_basic2center");
-    fetch(out, NOF(out));
+    device_wrote(out, NOF(out));
     return 0;
 }
 
@@ -346,9 +484,9 @@ void copyarrays(int rk)
                              V2C_N_RK_phidts_final, V2C_N_deltaVs, V2C_N_sliprs};
     static const int out[] = {V2C_N_deltaVs, V2C_N_sliprs, V2C_N_RK_sliprs_final, V2C_N_RK_phidts_final};
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, NOF(in));
+    send_newer(in, NOF(in));
     die_on(v2c_copyarrays(g_ctx, rk), "copyarrays");
-    fetch(out, NOF(out));
+    device_wrote(out, NOF(out));
 }
 
 /* The sticky-air diagnostics of the reference routine (calc_RSF.c:506-551)
@@ -364,7 +502,7 @@ double calc_timestep(int oldornew)
     v2c_timestep_params_default(&p);
     p.maxtmstep = maxtmstep; p.tcnvp = tcnvp; p.tcd = tcd; p.tcphih = tcphih; p.dtfactor = dtfactor;
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, NOF(in));
+    send_newer(in, NOF(in));
     const int rc = v2c_calc_timestep(g_ctx, oldornew, &p, &r);
     if (rc) {
         /* the reference exits on chi == 0 (calc_RSF.c:478) */
@@ -384,25 +522,25 @@ double calc_timestep(int oldornew)
 void vxvy_boundarycorrection(void)
 {
     static const int io[] = {V2C_N_mvx, V2C_N_mvy};
-    send(io, NOF(io));
+    send_newer(io, NOF(io));
     die_on(v2c_vxvy_boundarycorrection(g_ctx), "vxvy_boundarycorrection");
-    fetch(io, NOF(io));
+    device_wrote(io, NOF(io));
 }
 
 void P_boundarycorrection(void)
 {
     static const int io[] = {V2C_N_sppe};
-    send(io, NOF(io));
+    send_newer(io, NOF(io));
     die_on(v2c_P_boundarycorrection(g_ctx), "P_boundarycorrection");
-    fetch(io, NOF(io));
+    device_wrote(io, NOF(io));
 }
 
 void tkrecalc(void)
 {
     static const int io[] = {V2C_N_tk};
-    send(io, NOF(io));
+    send_newer(io, NOF(io));
     die_on(v2c_tkrecalc(g_ctx), "tkrecalc");
-    fetch(io, NOF(io));
+    device_wrote(io, NOF(io));
 }
 
 /* ---- the marker part of solve_temperature -------------------------------- */
@@ -414,7 +552,9 @@ void v2c_dropin_temperature_markers(double heatdif)
     static const int in[] = {V2C_N_tk, V2C_N_tk2, V2C_N_kt, V2C_N_cp, V2C_N_ro};
     static const int out[] = {V2C_N_tol0, V2C_N_tol1};
     v2c_set_time(g_ctx, timesum, timestep);
-    send(in, NOF(in));
+    send_newer(in, NOF(in));
     die_on(v2c_temperature_markers(g_ctx, heatdif), "temperature_markers");
-    fetch(out, NOF(out));
+    /* the rest of the heat step reads them on the host */
+    device_wrote(out, NOF(out));
+    fetch_newer(out, NOF(out));
 }

@@ -7,6 +7,7 @@
 #include <string.h>
 #include <algorithm>
 #include <mutex>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -22,6 +23,9 @@ struct v2c_ctx {
     hipStream_t stream = nullptr;
     // device address of this device's `scratch` array (noalloc staging)
     unsigned char *scratch = nullptr;
+    // staging of the many-field nodal transfers (first use): pinned host and
+    // device buffers of STAGE_FIELDS fields each
+    double *h_stage = nullptr, *d_stage = nullptr;
     // marker2nodes runs its per-marker property pass here, beside the
     // re-ordering on `stream` (default priority, like `stream`)
     hipStream_t side = nullptr;
@@ -394,6 +398,8 @@ static void free_all(v2c_ctx *c)
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     if (c->h_nsel) (void)hipHostFree(c->h_nsel);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->d_stage) (void)hipFree(c->d_stage);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -673,6 +679,126 @@ int v2c_nodal_get(v2c_ctx *c, int field, double *host)
     return 0;
 }
 
+// ---- many nodal fields per transfer ----------------------------------------
+// The host arrays are packed into one pinned buffer by a few host threads,
+// cross PCIe in one copy, and a kernel scatters the rows into the nodal
+// fields (and the reverse for reads), STAGE_FIELDS fields at a time.
+constexpr int STAGE_FIELDS = 24;
+struct FieldIds { int f[STAGE_FIELDS]; };
+
+__global__ void k_stage_to_nodal(double *__restrict__ nod, const double *__restrict__ stage, FieldIds ids, int k,
+                                 long nodenum)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (r < k && i < nodenum) nod[(size_t)ids.f[r] * nodenum + i] = stage[(size_t)r * nodenum + i];
+}
+
+__global__ void k_nodal_to_stage(const double *__restrict__ nod, double *__restrict__ stage, FieldIds ids, int k,
+                                 long nodenum)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (r < k && i < nodenum) stage[(size_t)r * nodenum + i] = nod[(size_t)ids.f[r] * nodenum + i];
+}
+
+static int stage_ready(v2c_ctx *c)
+{
+    if (c->h_stage) return 0;
+    const size_t bytes = sizeof(double) * STAGE_FIELDS * c->nodenum;
+    HIPCHK(c, hipHostMalloc((void **)&c->h_stage, bytes, hipHostMallocDefault));
+    const hipError_t e = hipMalloc(&c->d_stage, bytes);
+    if (e != hipSuccess) {
+        (void)hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        return set_err(c, e == hipErrorOutOfMemory ? V2C_ENOMEM : V2C_EHIP, "staging: %s", hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// Row copies between caller arrays and the pinned buffer on up to 8 threads.
+static void stage_copy(double *const *dst, const double *const *src, int k, long nodenum)
+{
+    const size_t bytes = sizeof(double) * nodenum;
+    const int nt = std::max(1, std::min({k, 8, (int)std::thread::hardware_concurrency()}));
+    if (nt == 1) {
+        for (int r = 0; r < k; r++) memcpy(dst[r], src[r], bytes);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+        pool.emplace_back([=] {
+            for (int r = t; r < k; r += nt) memcpy(dst[r], src[r], bytes);
+        });
+    for (auto &th : pool) th.join();
+}
+
+static int check_fields(const int *fields, int n)
+{
+    if (!fields || n < 0) return V2C_EINVAL;
+    for (int i = 0; i < n; i++)
+        if (fields[i] < 0 || fields[i] >= V2C_NF) return V2C_EINVAL;
+    return 0;
+}
+
+int v2c_nodal_set_many(v2c_ctx *c, const int *fields, const double *const *host, int n)
+{
+    if (!c || !host || check_fields(fields, n)) return V2C_EINVAL;
+    for (int i = 0; i < n; i++) if (!host[i]) return V2C_EINVAL;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = stage_ready(c);
+    if (rc) return rc;
+    const long nn = c->nodenum;
+    for (int b = 0; b < n; b += STAGE_FIELDS) {
+        const int k = std::min(STAGE_FIELDS, n - b);
+        // the previous chunk's copy out of the pinned buffer has finished
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double *rows[STAGE_FIELDS];
+        FieldIds ids{};
+        for (int r = 0; r < k; r++) {
+            rows[r] = c->h_stage + (size_t)r * nn;
+            ids.f[r] = fields[b + r];
+        }
+        stage_copy(rows, host + b, k, nn);
+        HIPCHK(c, hipMemcpyAsync(c->d_stage, c->h_stage, sizeof(double) * k * nn, hipMemcpyHostToDevice,
+                                 c->stream));
+        hipLaunchKernelGGL(k_stage_to_nodal, dim3((unsigned)blocks_for(nn, 256), (unsigned)k), dim3(256), 0,
+                           c->stream, c->d_nod, c->d_stage, ids, k, nn);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int v2c_nodal_get_many(v2c_ctx *c, const int *fields, double *const *host, int n)
+{
+    if (!c || !host || check_fields(fields, n)) return V2C_EINVAL;
+    for (int i = 0; i < n; i++) if (!host[i]) return V2C_EINVAL;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = stage_ready(c);
+    if (rc) return rc;
+    const long nn = c->nodenum;
+    for (int b = 0; b < n; b += STAGE_FIELDS) {
+        const int k = std::min(STAGE_FIELDS, n - b);
+        FieldIds ids{};
+        const double *rows[STAGE_FIELDS];
+        for (int r = 0; r < k; r++) {
+            rows[r] = c->h_stage + (size_t)r * nn;
+            ids.f[r] = fields[b + r];
+        }
+        hipLaunchKernelGGL(k_nodal_to_stage, dim3((unsigned)blocks_for(nn, 256), (unsigned)k), dim3(256), 0,
+                           c->stream, c->d_nod, c->d_stage, ids, k, nn);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(c->h_stage, c->d_stage, sizeof(double) * k * nn, hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        stage_copy(host + b, rows, k, nn);
+    }
+    return 0;
+}
+
 int v2c_marker2nodes(v2c_ctx *c)
 {
     if (!c) return V2C_EINVAL;
@@ -745,24 +871,39 @@ static int n2m_impl(v2c_ctx *c, bool advect)
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
     }
+    // on an error after the fork the main stream is still ordered after
+    // whatever reached the side stream (it writes the antidiffusion sums)
+    auto join_side = [&](int code) {
+        if (side) {
+            (void)hipEventRecord(c->ev_join, c->side);
+            (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+        }
+        return code;
+    };
     if (c->prm.n2m_phantom && anti) {
         rc = launch(c, V2C_K_N2M_NODES, [&] {
             hipLaunchKernelGGL(k_n2m_phantom, dim3(1), dim3(1), 0, st, a);
         }, st);
-        if (rc) return rc;
+        if (rc) return join_side(rc);
     }
     if (anti) {
         const int nwg = (int)((c->ncell + CPW - 1) / CPW);
         rc = launch(c, V2C_K_N2M_CELLS, [&] {
             hipLaunchKernelGGL(k_n2m_scatter, dim3(nwg), dim3(WAVE), 0, st, a, nwg);
         }, st);
-        if (rc) return rc;
+        if (rc) return join_side(rc);
         rc = launch(c, V2C_K_N2M_NODES, [&] {
             hipLaunchKernelGGL(k_n2m_nodes, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, st, a);
         }, st);
-        if (rc) return rc;
+        if (rc) return join_side(rc);
     }
-    if (side) HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+    if (side) {
+        const hipError_t je = hipEventRecord(c->ev_join, c->side);
+        if (je != hipSuccess) {
+            (void)hipStreamSynchronize(c->side);
+            return set_err(c, V2C_EHIP, "hipEventRecord: %s", hipGetErrorString(je));
+        }
+    }
     if (c->n > 0) {
         const int nb = (int)blocks_for(c->n, 256);
         // With antidiffusion and advection the fast form takes every marker

@@ -449,6 +449,7 @@ struct v2c_ens {
     int local_warm = -1;
     double local_tol = 1e-10;
     hipEvent_t ev_lfork = nullptr, ev_ljoin = nullptr;
+    hipEvent_t ev_pre = nullptr;             // property pass after queued work
     int adaptive = 0;
     TsArgs ts{};
     double dt_min = 0, dt_max = 0;
@@ -807,6 +808,14 @@ static int ens_props(v2c_ens *e)
 {
     if (!e->n_mark) return 0;
     const size_t nm = e->models.size();
+    // ordered after all work queued so far on the ensemble stream and on
+    // every model's stream (the events are re-recorded by the ordering)
+    ENSCHK(e, hipEventRecord(e->ev_pre, e->stream));
+    ENSCHK(e, hipStreamWaitEvent(e->side, e->ev_pre, 0));
+    for (size_t k = 0; k < nm; k++) {
+        ENSCHK(e, hipEventRecord(e->ordered[k], e->models[k]->stream));
+        ENSCHK(e, hipStreamWaitEvent(e->side, e->ordered[k], 0));
+    }
     for (size_t k = 0; k < nm; k++) e->h_args_p[k] = make_args(e->models[k]);
     ENSCHK(e, hipMemcpyAsync(e->d_args_p, e->h_args_p, sizeof(KArgs) * nm, hipMemcpyHostToDevice, e->side));
     hipLaunchKernelGGL(k_ens_m2n_props, dim3((unsigned)e->n_mark), dim3(ENS_BLOCK), 0, e->side, e->d_args_p,
@@ -938,7 +947,8 @@ int v2c_ens_create(v2c_ens **out, v2c_ctx *const *models, int nmodels)
     if (hipMalloc(&e->d_defer, sizeof(uint64_t) * std::max(capsum, 1L)) != hipSuccess) return fail(V2C_ENOMEM);
     e->defer_cap = capsum;
     if (hipEventCreateWithFlags(&e->ev_lfork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->ev_ljoin, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&e->ev_ljoin, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_pre, hipEventDisableTiming) != hipSuccess)
         return fail(V2C_EHIP);
     e->plan_n.assign(nmodels, -1);
     e->plan_nf.assign(nmodels, -1);
@@ -963,6 +973,7 @@ void v2c_ens_destroy(v2c_ens *e)
     if (e->h_tsout) (void)hipHostFree(e->h_tsout);
     if (e->ev_lfork) (void)hipEventDestroy(e->ev_lfork);
     if (e->ev_ljoin) (void)hipEventDestroy(e->ev_ljoin);
+    if (e->ev_pre) (void)hipEventDestroy(e->ev_pre);
     if (e->h_args) (void)hipHostFree(e->h_args);
     if (e->h_args_p) (void)hipHostFree(e->h_args_p);
     if (e->ev_props) (void)hipEventDestroy(e->ev_props);
