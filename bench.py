@@ -29,6 +29,12 @@ from velo2cycles_amd.abi import KERNELS, Context, load_lib  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+
+def log(msg):
+    """Progress of the phases on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 # Algorithmic bytes (FP64 = 8 B, type = 1 B): what each kernel must read and
 # write for its function, excluding the library's own intermediates (sort
 # order, keys, per-cell partials, per-marker property hand-over).
@@ -46,16 +52,29 @@ ALG = {
     "n2m_cells": (17, 9 * 8),
     "n2m_update": (25 + 88, 19 * 8),
 }
-# Whole step: marker2nodes 161 B and nodes2markers + advection 113 B per
-# marker; ~360 B per node over all passes (each nodal array touched once).
-B_STEP_MARKER = 161 + 113
+# Whole step, the path's byte model: 256 B per marker (marker2nodes 116 R +
+# 16 W, nodes2markers + advection 28 R + 96 W), 80 B per fault marker (its
+# rate-and-state parameters read in each pass) and 360 B per node.  The sum
+# of the per-kernel figures above (161 + 113 B per marker) is the same path
+# counted field by field for this build's kernels; both are reported.
+B_STEP_MARKER = 256
+B_STEP_FAULT = 80
 B_STEP_NODE = 360
+B_STEP_MARKER_KERNELS = 161 + 113
 
-# Nodal fields the reference-side binding moves per step when the host keeps
-# the nodal arrays (marker2nodes 24 in / 43 out, nodes2markers 28 / 9,
+# Nodal arrays the reference-side binding moves per step of the reference loop
+# (one Picard iteration): the host Stokes solve's results to the device, and
+# back to the host what the Stokes assembly reads of the device's outputs and
+# what the per-step diagnostics and postproc read (v2c_dropin.c's
+# STOKES_READS / OUTPUT_READS, those a device routine wrote).
+PCIE_H2D = ("pr", "vx", "vy")
+PCIE_D2H = ("ro", "nu", "nd", "gg", "gd", "sxxe", "sxye", "sppe", "mvx", "mvy", "mrx", "mry", "bm",
+            "sxxs", "sxy", "sliprs", "phis_it", "syields")
+# Before residency tracking: every array each routine reads, sent, and every
+# array it writes, fetched (marker2nodes 24 / 43, nodes2markers 28 / 9,
 # moveandrotatemarkers 5 / 0, before solve 22 / 6, after solve 19 / 3).
-PCIE_H2D_FIELDS = 24 + 28 + 5 + 22 + 19
-PCIE_D2H_FIELDS = 43 + 9 + 6 + 3
+PCIE_H2D_EVERY_CALL = 24 + 28 + 5 + 22 + 19
+PCIE_D2H_EVERY_CALL = 43 + 9 + 6 + 3
 
 # FP64 vector issue: one 64-lane wavefront instruction per SIMD every 4
 # cycles (16 FP64 lanes per SIMD); 256 CUs x 4 SIMDs at 2.4 GHz.  Quarter-rate
@@ -64,79 +83,81 @@ SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
 
 
 def cpu_baseline(cfg, sample: int, steps: int = 2):
-    """The reference's own OpenMP routines (oracle/_ref/ref_driver, built from
-    its sources) timed on this host's cores for `steps` full steps on the
-    largest window of the production grid its capacity macros allow
-    (synth.config_production_window: 250 x 120 nodes of the 500 m refined
-    part, 36 markers per cell, 1.07 M markers), with OMP_NUM_THREADS threads
-    and with one; plus the FP64 oracle on a sample of the production cloud
-    itself.  Without the built driver, the oracle figure alone ("port")."""
-    port = cpu_port(cfg, sample, steps)
-    from oracle import refrun
-    if not refrun.built(refrun.DRIVER):
-        return port
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    win = synth.config_production_window()
-    dt = cfg.timestep
-    ops = ["m2n", f"before:{dt}:0", f"after:{dt}:0", "n2m", "move"] * steps
-    rates = {}
-    for th in sorted({threads, 1}):
-        _, _, raw = refrun.run(win, ops, timesum=dt, threads=th)
-        rates[th] = win.n * steps / float(raw["drv_time"][0])
-    return {"value": rates[threads], "unit": "marker-steps/s", "cores": threads, "kind": "reference",
-            "sample": f"reference routines (marker2nodes, compute_before_solve, compute_after_solve, "
-                      f"nodes2markers, moveandrotatemarkers; OpenMP, {threads} threads) on "
-                      f"{win.grid.xnumx}x{win.grid.ynumy} nodes of the production grid's 500 m part with "
-                      f"{win.n:,} markers (the reference's capacity), {steps} steps, op time only",
-            "value_1thread": rates[1], "port_1thread": port}
+    """The FP64 oracle timed on this host's cores ("port"): full steps on all
+    usable physical cores (lscpu's count, capped by this job's CPU share)
+    over the whole cloud, and on one core over its first `sample` markers.
+    The reference's own routines are never run here: their rate comes from
+    oracle/reference_timing.json, measured where the reference is built
+    (oracle/time_reference.py: the reference and the oracle on the same
+    1.07 M-marker production window), which also gives the oracle/reference
+    rate ratio that calibrates the oracle figures."""
+    from oracle import cpu_rate
+    cores = cpu_rate.usable_threads()
+    th = cores["threads"]
+    multi = cpu_rate.step_rate(cfg, cfg.n, steps, th)
+    one = cpu_rate.step_rate(cfg, sample, steps, 1)
+    out = {"value": multi["rate"], "unit": "marker-steps/s", "cores": th, "kind": "port",
+           "sample": (f"oracle (FP64 C restatement), {steps} full steps of all {cfg.n:,} markers of the "
+                      f"same cloud with the full {cfg.grid.nodenum:,}-node grid on {th} threads, one "
+                      f"marker slice each ({multi['seconds']:.1f} s wall; each thread also runs the grid's "
+                      f"node loops); one core: the first {one['markers']:,} markers ({one['seconds']:.1f} s)"),
+           "value_1core": one["rate"], "host_cpu": cpu_rate.cpu_model(), **{k: v for k, v in cores.items()
+                                                                              if k != "threads"}}
+    path = os.path.join(REPO, "oracle", "reference_timing.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            ref = json.load(f)
+        r_all = ref["oracle_over_reference"][str(ref["threads"])]
+        out["reference_elsewhere"] = {
+            "what": "the reference's routines (its OpenMP build) timed where it is built; never run here",
+            "workload": ref["workload"], "host_cpu": ref["host_cpu"], "threads": ref["threads"],
+            "rate": ref["reference"][str(ref["threads"])], "rate_1thread": ref["reference"]["1"],
+            "oracle_over_reference": ref["oracle_over_reference"]}
+        out["reference_equivalent_here"] = {"value": multi["rate"] / r_all, "value_1core": one["rate"] /
+                                            ref["oracle_over_reference"]["1"],
+                                            "how": "this host's oracle rates divided by the oracle/reference "
+                                                   "ratios measured beside the reference (estimate)"}
+    return out
 
 
-def cpu_port(cfg, sample: int, steps: int = 2):
-    """FP64 oracle, one thread, full step on the first `sample` markers of the
-    same cloud (x-major cell order) with the full nodal grid."""
-    from oracle.pyoracle import Oracle
-    from velo2cycles_amd.abi import MarkerArrays
-    fields = {f: cfg.markers[f][:sample] for f in cfg.markers.arrays}
-    fields["t"] = cfg.markers.t[:sample]
-    sub = synth.Config(cfg.name, cfg.grid, cfg.params, cfg.mats, MarkerArrays(sample, fields),
-                       cfg.nodal, cfg.timestep, cfg.timesum)
-    o = Oracle(sub)
-    dt = cfg.timestep
-    t0 = time.perf_counter()
-    for s in range(steps):
-        o.set_time(cfg.timesum + s * dt, dt)
-        o.marker2nodes()
-        o.compute_before_solve(dt, 0)
-        o.compute_after_solve(dt, 0)
-        o.nodes2markers()
-        o.moveandrotatemarkers()
-    el = time.perf_counter() - t0
-    return {"value": sample * steps / el, "unit": "marker-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (FP64 C restatement, 1 thread) on the first {sample:,} markers of the "
-                      f"same cloud with the full {cfg.grid.nodenum:,}-node grid, {steps} full steps, "
-                      f"{el:.1f} s"}
-
-
-def pcie_inclusive(ctx, cfg, ms_step):
-    """Step time if the host keeps the nodal arrays (reference-side binding):
-    the device step plus the per-step nodal transfers, each field timed as a
-    synchronous copy of the reference's pageable host array."""
-    names = list(cfg.nodal)
-    host = {k: np.ascontiguousarray(cfg.nodal[k]) for k in names}
+def pcie_inclusive(ctx, cfg, ms_step, reps: int = 5):
+    """Step time with the host keeping its nodal arrays (the reference-side
+    binding): the device step plus the binding's per-step hand-overs, timed
+    here as the staged many-field transfers it makes (v2c_nodal_set_many /
+    get_many) between ordinary pageable host arrays; best of `reps`.  The
+    figure of the binding without residency tracking (every array a routine
+    touches, one synchronous pageable copy each) is given beside it."""
+    nn = cfg.grid.nodenum
+    host = {k: np.ascontiguousarray(cfg.nodal.get(k, np.zeros(nn)), dtype=np.float64).copy()
+            for k in set(PCIE_H2D) | set(PCIE_D2H)}
     ctx.synchronize()
+    t_h2d = t_d2h = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.set_nodal_many({k: host[k] for k in PCIE_H2D})
+        t_h2d = min(t_h2d, time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        ctx.get_nodal_many(PCIE_D2H, out=host)
+        t_d2h = min(t_d2h, time.perf_counter() - t0)
+    ms = ms_step + (t_h2d + t_d2h) * 1e3
+    # per-field synchronous pageable copies, as the binding made them before
+    names = list(cfg.nodal)
     t0 = time.perf_counter()
     for k in names:
-        ctx.set_nodal(k, host[k])
-    t_h2d = (time.perf_counter() - t0) / len(names)
+        ctx.set_nodal(k, np.ascontiguousarray(cfg.nodal[k]))
+    f_h2d = (time.perf_counter() - t0) / len(names)
     t0 = time.perf_counter()
     for k in names:
         ctx.get_nodal(k)
-    t_d2h = (time.perf_counter() - t0) / len(names)
-    ms = ms_step + (PCIE_H2D_FIELDS * t_h2d + PCIE_D2H_FIELDS * t_d2h) * 1e3
+    f_d2h = (time.perf_counter() - t0) / len(names)
+    ms_every = ms_step + (PCIE_H2D_EVERY_CALL * f_h2d + PCIE_D2H_EVERY_CALL * f_d2h) * 1e3
     return {"ms_per_step": ms, "value": cfg.n / (ms * 1e-3), "unit": "marker-steps/s",
-            "h2d_fields": PCIE_H2D_FIELDS, "d2h_fields": PCIE_D2H_FIELDS,
-            "ms_per_field_h2d": t_h2d * 1e3, "ms_per_field_d2h": t_d2h * 1e3,
-            "field_mb": cfg.grid.nodenum * 8 / 1e6}
+            "h2d_fields": len(PCIE_H2D), "d2h_fields": len(PCIE_D2H), "ms_h2d": t_h2d * 1e3,
+            "ms_d2h": t_d2h * 1e3, "field_mb": nn * 8 / 1e6,
+            "h2d_gbs": len(PCIE_H2D) * nn * 8 / t_h2d / 1e9, "d2h_gbs": len(PCIE_D2H) * nn * 8 / t_d2h / 1e9,
+            "every_call_binding": {"ms_per_step": ms_every, "h2d_fields": PCIE_H2D_EVERY_CALL,
+                                   "d2h_fields": PCIE_D2H_EVERY_CALL, "ms_per_field_h2d": f_h2d * 1e3,
+                                   "ms_per_field_d2h": f_d2h * 1e3}}
 
 
 # Algorithmic bytes of the routines after the solves, per node (and per
@@ -333,30 +354,73 @@ def device_memory_used():
         return None
 
 
+COSEISMIC_DT = 1e-3
+
+
+def coseismic_network(lib):
+    """Warm-start network trained on plain coseismic solves of eight small
+    clouds (101x51 nodes, 16 markers per cell, 3 % fault markers in the
+    coseismic regime of synth.coseismic_run)."""
+    from velo2cycles_amd.warmstart import WarmStart, generate, init_params
+
+    def maker(seed):
+        return synth.coseismic_run(synth.config_grid(101, 51, 100e3, 50e3, 4, seed=seed, fault_frac=0.03),
+                                   seed=seed, dt=COSEISMIC_DT)
+    cfg = maker(101)
+    ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=lib)
+    X, M = generate(ctx, range(101, 109), COSEISMIC_DT, maker=maker, keep_other=0.05)
+    ws = WarmStart(ctx)
+    ws.fit_norm(X, M)
+    ws.load(init_params(0))
+    ws.train(X, M, epochs=8, batch=4096, lr=2e-3)
+    net = (ws.fetch().copy(), ws.norm.copy())
+    ctx.close()
+    return net
+
+
 def ensemble_bench(args):
     """Batched ensemble of 64 uneven models: one step = marker2nodes,
     before/after solve, nodes2markers + advection of every model.  Times the
     graph-replayed batched step with default tile shapes, after autotuning, and
-    the same step as per-model launches."""
-    from velo2cycles_amd.abi import Ensemble
+    the same step as per-model launches.  --warm (with --production-models):
+    the combined workload, every model's cloud in the coseismic regime, the
+    per-marker local update with the network's slip-rate guess inside the
+    solve (residual 1e-10) and adaptive steps from each model's
+    calc_timestep (1e-3 s .. 1 yr) added to every step."""
+    from velo2cycles_amd.abi import Ensemble, TimestepParams
+    from velo2cycles_amd.warmstart import WarmStart, upload_cloud
     lib = load_lib()
     ctxs, n_total, nodes_total, sizes = [], 0, 0, []
     t0 = time.perf_counter()
     prod = args.production_models > 0
-    models = (synth.config_ensemble_production(args.production_models) if prod
+    warm = args.warm and prod
+    net = coseismic_network(lib) if warm else None
+    models = (synth.config_ensemble_production(args.production_models,
+                                               coseismic_dt=COSEISMIC_DT if warm else None) if prod
               else synth.config_ensemble(args.models, small=args.small_models))
+    n_fault = 0
     for k, cfg in models:
         ctx = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=lib)
-        ctx.set_time(cfg.timestep, cfg.timestep)
         ctx.set_all_nodal(cfg.nodal)
-        ctx.upload_markers(cfg.markers)
+        if warm:
+            upload_cloud(ctx, cfg)
+            WarmStart(ctx).load(*net)
+            n_fault += int(cfg.rsf["fault"].sum())
+        else:
+            ctx.set_time(cfg.timestep, cfg.timestep)
+            ctx.upload_markers(cfg.markers)
         ctxs.append(ctx)
         n_total += cfg.n
         nodes_total += cfg.grid.nodenum
         sizes.append(cfg.n)
         del cfg
     t_gen = time.perf_counter() - t0
+    log(f"{len(ctxs)} models set up, {n_total} markers ({t_gen:.0f} s)")
     ens = Ensemble(ctxs)
+    tsp = TimestepParams.default(maxtmstep=synth.YEAR)
+    if warm:
+        ens.set_local(warm=2, tol=1e-10)
+        ens.set_adaptive(tsp, 0, COSEISMIC_DT, synth.YEAR)
 
     def timed(fn, steps):
         for c in ctxs:
@@ -372,13 +436,37 @@ def ensemble_bench(args):
     for _ in range(args.warmup):
         ens.step(graph=True)
     t_default = timed(lambda: ens.step(graph=True), args.steps)
+    log(f"batched graph step {t_default * 1e3:.1f} ms")
     t_looped = timed(ens.step_looped, args.steps)
+    log(f"looped step {t_looped * 1e3:.1f} ms")
     if args.no_tune:
         t_graph_only, t_tuned = None, t_default
     else:
         t_graph_only = ens.autotune(reps=2)
         t_tuned = timed(lambda: ens.step(graph=True), args.steps)
+        log(f"tuned batched step {t_tuned * 1e3:.1f} ms")
     shapes = {str(c): ens.get_shape(c) for c in range(3)}
+    extra = {}
+    if warm:
+        times = ens.times()
+        st = [ens.local_stats(k) for k in range(len(ctxs))]
+        fm = sum(x["fault_markers"] for x in st)
+        extra["combined"] = {
+            "fault_markers": n_fault, "residual": 1e-10, "dt_range_s": [COSEISMIC_DT, synth.YEAR],
+            "last_dt_s": {"min": float(times[:, 1].min()), "max": float(times[:, 1].max())},
+            "rsf_iters_per_fault_marker": sum(x["rsf_iters"] for x in st) / max(fm, 1),
+            "ductile_iters_per_marker": sum(x["ductile_iters"] for x in st) / max(sum(x["ductile_markers"]
+                                                                                   for x in st), 1),
+            "nonfinite": sum(x["nonfinite"] for x in st), "fallbacks": sum(x["fallbacks"] for x in st)}
+        # the same batched step with the plain slip-rate start (no network)
+        ens.set_local(warm=0, tol=1e-10)
+        t_plain = timed(lambda: ens.step(graph=True), args.steps)
+        log(f"plain-start batched step {t_plain * 1e3:.1f} ms")
+        st = [ens.local_stats(k) for k in range(len(ctxs))]
+        extra["combined"]["plain_start_batched_ms"] = t_plain * 1e3
+        extra["combined"]["plain_rsf_iters_per_fault_marker"] = (sum(x["rsf_iters"] for x in st) /
+                                                                 max(sum(x["fault_markers"] for x in st), 1))
+        extra["combined"]["warm_over_plain_wall"] = t_plain / min(t_default, t_tuned)
     ens.close()
     for c in ctxs:
         c.close()
@@ -389,19 +477,24 @@ def ensemble_bench(args):
            "data": ("synthetic (one seeded production cloud, per-model convergence rate and a-b, "
                     "analytic corner-flow nodal fields)" if prod else
                     "synthetic (seeded uneven models, analytic corner-flow nodal fields)"),
-           "config": {"workload": (f"ensemble-{len(ctxs)}-production" if prod else
+           "config": {"workload": (f"ensemble-{len(ctxs)}-production" + ("-coseismic-warm" if warm else "")
+                                   if prod else
                                    f"ensemble-{args.models}" + ("-small" if args.small_models else "")),
                       "models": len(ctxs), "markers": n_total,
                       "nodes": nodes_total, "markers_min": min(sizes), "markers_max": max(sizes),
                       "grids": ("1501x301 refined to 500 m, 36 markers per cell, one cloud, 2-8 cm/yr"
                                 if prod else
                                 ("41x21 .. 161x61" if args.small_models else "201x51 .. 801x201")
-                                + " nodes, 4-36 markers per cell"), "dt_s": synth.YEAR},
+                                + " nodes, 4-36 markers per cell"),
+                      "dt_s": "adaptive 1e-3 s .. 1 yr" if warm else synth.YEAR,
+                      "step": ("marker2nodes, before/after solve, local update (network warm start), "
+                               "nodes2markers + advection, copyarrays, calc_timestep" if warm else
+                               "marker2nodes, before/after solve, nodes2markers + advection")},
            "device_memory_used_gb": None if resident is None else round(resident / 1e9, 1),
            "batched_graph_default_ms": t_default * 1e3, "batched_graph_tuned_ms": t_tuned * 1e3,
            "looped_single_model_ms": t_looped * 1e3,
            "tuned_graph_only_ms": t_graph_only, "speedup_vs_looped": t_looped / best,
-           "tuned_shapes": shapes, "setup_s": round(t_gen, 1)}
+           "tuned_shapes": shapes, "setup_s": round(t_gen, 1), **extra}
     print(json.dumps(out), flush=True)
 
 
@@ -422,6 +515,9 @@ def main():
     ap.add_argument("--small-models", action="store_true", help="ensemble: 41x21 .. 161x61-node models")
     ap.add_argument("--production-models", type=int, default=0,
                     help="ensemble: this many production-size models (16.2 M markers each) instead")
+    ap.add_argument("--warm", action="store_true",
+                    help="ensemble with --production-models: the combined coseismic workload (local update "
+                         "with the network warm start, adaptive steps)")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -468,6 +564,7 @@ def main():
         ts += dt
     ctx.synchronize()
     el = time.perf_counter() - t0
+    log(f"{args.steps} steps: {el / args.steps * 1e3:.3f} ms per step")
     ctx.enable_timing(False)
     kern = {}
     for k in KERNELS:
@@ -476,7 +573,9 @@ def main():
             kern[k] = {"ms_total": ms, "launches": nl, "ms_per_step": ms / args.steps}
     n, nn = cfg.n, cfg.grid.nodenum
     ms_step = el / args.steps * 1e3
-    bytes_step = n * B_STEP_MARKER + nn * B_STEP_NODE
+    f_fault = float(np.mean(cfg.rsf["fault"])) if cfg.rsf is not None else 0.0
+    bytes_step = n * (B_STEP_MARKER + B_STEP_FAULT * f_fault) + nn * B_STEP_NODE
+    bytes_step_kernels = n * B_STEP_MARKER_KERNELS + nn * B_STEP_NODE
     dom = max((k for k in ALG if k in kern), key=lambda k: kern[k]["ms_total"])
     per_launch_bytes = n * ALG[dom][0] + nn * ALG[dom][1]
     avg_ms = kern[dom]["ms_total"] / max(kern[dom]["launches"], 1)
@@ -490,6 +589,14 @@ def main():
         ent = tj.get(cfg.name, {}).get(dom)
         if ent:
             traffic = ent["hbm_bytes_per_launch"]
+    # what limits the dominant kernel: its FP64 VALU issue bound over its time
+    # against its counted HBM traffic over its time at peak (both counter
+    # figures from the committed profiles of this workload)
+    valu_frac = valu[dom]["frac"] if valu and dom in valu else None
+    hbm_frac_traffic = traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None
+    limiter = None
+    if valu_frac is not None and hbm_frac_traffic is not None:
+        limiter = "fp64-valu" if valu_frac > hbm_frac_traffic else "hbm"
     out = {
         "metric": "marker-steps/sec", "value": n * args.steps / el, "unit": "marker-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -499,9 +606,18 @@ def main():
                    if args.config == "production" else 16, "dt_s": dt, "grid": "1501x301 refined to 500 m"
                    if args.config == "production" else "401x101 uniform"},
         "hbm_frac_step": bytes_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "bytes_step": bytes_step, "fault_fraction": f_fault,
+        "hbm_frac_step_kernel_bytes": bytes_step_kernels / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        # bound: the roofline `achieved` is priced against (algorithmic bytes
+        # over HBM peak; the path has no MFMA work); limiter: what the counter
+        # evidence says limits the kernel
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms},
+                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms,
+                     "limiter": limiter, "valu_issue_frac": valu_frac, "hbm_traffic_frac": hbm_frac_traffic,
+                     "counter_source": "committed rocprofv3 --pmc runs of this workload: profiles/traffic.json "
+                                       "(FETCH_SIZE, WRITE_SIZE, copy-calibrated) and profiles/valu.json "
+                                       "(SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F64); commands in profiles/*.txt"},
         "kernels_ms_per_step": {k: v["ms_per_step"] for k, v in kern.items()},
         # m2n_props runs on a second stream beside the re-ordering ("sort"):
         # their event times include each other's share of the GPU
@@ -512,11 +628,14 @@ def main():
         out["valu_issue"] = valu
     if not args.no_next:
         ctx.set_async(False)
+        log("post-solve routines")
         out["next_rows"] = next_rows(ctx, cfg)
     ctx.close()
     if not args.no_warm:
+        log("warm-start evaluation")
         out["warm_start"] = warm_start_eval(lib, synth.YEAR)
     if not args.no_cpu:
+        log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(cfg, min(args.cpu_sample, cfg.n))
     print(json.dumps(out), flush=True)
 

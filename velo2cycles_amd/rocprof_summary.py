@@ -24,8 +24,11 @@ import re
 import sqlite3
 
 # library kernel -> the name bench.py reports it under
+# (the fused update's two kernels, one dispatch each per step, are one bench
+# entry: their per-dispatch figures add)
 BENCH_NAME = {"k_m2n_props": "m2n_props", "k_m2n_cells": "m2n_cells", "k_n2m_scatter": "n2m_cells",
-              "k_n2m_update<true>": "n2m_update", "k_m2n_nodes": "m2n_nodes", "k_n2m_nodes": "n2m_nodes"}
+              "k_n2m_update<true>": "n2m_update", "k_n2m_update_fast": "n2m_update",
+              "k_n2m_update_list": "n2m_update", "k_m2n_nodes": "m2n_nodes", "k_n2m_nodes": "n2m_nodes"}
 
 
 def short(name: str) -> str:
@@ -81,7 +84,12 @@ def traffic(fetch_db: str, write_db: str, copy_bytes: int, workload: str) -> dic
         if k in f and k in w:
             rd = f[k]["FETCH_SIZE"] * 1024.0 * cf
             wr = w[k]["WRITE_SIZE"] * 1024.0 * cw
-            out[name] = {"hbm_read_bytes": rd, "hbm_write_bytes": wr, "hbm_bytes_per_launch": rd + wr}
+            e = out.setdefault(name, {"hbm_read_bytes": 0.0, "hbm_write_bytes": 0.0, "hbm_bytes_per_launch": 0.0,
+                                      "kernels": []})
+            e["hbm_read_bytes"] += rd
+            e["hbm_write_bytes"] += wr
+            e["hbm_bytes_per_launch"] += rd + wr
+            e["kernels"].append(k)
     return {workload: out}
 
 
@@ -90,8 +98,10 @@ def valu(db: str, workload: str) -> dict:
     out = {}
     for k, name in BENCH_NAME.items():
         if k in t and "SQ_INSTS_VALU" in t[k]:
-            out[name] = {"valu_per_launch": t[k]["SQ_INSTS_VALU"],
-                         "trans_f64_per_launch": t[k].get("SQ_INSTS_VALU_TRANS_F64", 0.0)}
+            e = out.setdefault(name, {"valu_per_launch": 0.0, "trans_f64_per_launch": 0.0, "kernels": []})
+            e["valu_per_launch"] += t[k]["SQ_INSTS_VALU"]
+            e["trans_f64_per_launch"] += t[k].get("SQ_INSTS_VALU_TRANS_F64", 0.0)
+            e["kernels"].append(k)
     return {workload: out}
 
 
