@@ -439,6 +439,16 @@ def ensemble_bench(args):
     log(f"batched graph step {t_default * 1e3:.1f} ms")
     t_looped = timed(ens.step_looped, args.steps)
     log(f"looped step {t_looped * 1e3:.1f} ms")
+    # the same graph as 2..4 concurrent branches of model groups
+    t_branch = {1: t_default}
+    for b in range(2, min(4, len(ctxs)) + 1):
+        ens.set_branches(b)
+        ens.step(graph=True)
+        t_branch[b] = timed(lambda: ens.step(graph=True), args.steps)
+        log(f"{b} branches: {t_branch[b] * 1e3:.1f} ms")
+    best_b = min(t_branch, key=t_branch.get)
+    ens.set_branches(best_b)
+    t_default = t_branch[best_b]
     if args.no_tune:
         t_graph_only, t_tuned = None, t_default
     else:
@@ -446,6 +456,7 @@ def ensemble_bench(args):
         t_tuned = timed(lambda: ens.step(graph=True), args.steps)
         log(f"tuned batched step {t_tuned * 1e3:.1f} ms")
     shapes = {str(c): ens.get_shape(c) for c in range(3)}
+    ens_branches = ens.branches()
     extra = {}
     if warm:
         times = ens.times()
@@ -493,6 +504,7 @@ def ensemble_bench(args):
            "device_memory_used_gb": None if resident is None else round(resident / 1e9, 1),
            "batched_graph_default_ms": t_default * 1e3, "batched_graph_tuned_ms": t_tuned * 1e3,
            "looped_single_model_ms": t_looped * 1e3,
+           "branches_ms": {str(b): v * 1e3 for b, v in t_branch.items()}, "branches": ens_branches,
            "tuned_graph_only_ms": t_graph_only, "speedup_vs_looped": t_looped / best,
            "tuned_shapes": shapes, "setup_s": round(t_gen, 1), **extra}
     print(json.dumps(out), flush=True)

@@ -554,6 +554,13 @@ int v2c_ens_set_shape(v2c_ens *ens, int size_class, const v2c_tile_shape *shape)
 int v2c_ens_get_shape(const v2c_ens *ens, int size_class, v2c_tile_shape *shape);
 /* persistent scatter grid = 4 x blocks_per_cu x compute units wavefronts */
 int v2c_ens_set_blocks_per_cu(v2c_ens *ens, int blocks);
+/* Run the step as `branches` (1..4) concurrent branches of the graph, each a
+ * contiguous group of models of about equal marker count with its own stage
+ * sequence on its own stream, so that stages of different groups overlap.
+ * Results are the same whatever the count.  v2c_ens_get_branches returns the
+ * count in use (at most the number of models). */
+int v2c_ens_set_branches(v2c_ens *ens, int branches);
+int v2c_ens_get_branches(const v2c_ens *ens);
 /* one batched step; use_graph: replay the captured stage graph (every model's
  * property pass runs beside the batched re-ordering, before the graph) */
 int v2c_ens_step(v2c_ens *ens, int use_graph);
@@ -579,9 +586,10 @@ int v2c_ens_times(v2c_ens *ens, double *timesum_timestep);
 /* compute_after_solve misfit of every model's last batched step:
  * rms_count[2*k] = RMS, rms_count[2*k+1] = node count */
 int v2c_ens_rms(v2c_ens *ens, double *rms_count);
-/* time tile shapes per size class and the grid size on the ensemble itself
- * (coordinate descent, reps timed steps each, the graph's time); keeps the
- * fastest.  Advances the models by the steps it runs. */
+/* time tile shapes per size class, the grid size and the branch count on
+ * the ensemble itself (coordinate descent, reps timed steps each, the
+ * graph's time); keeps the fastest.  Advances the models by the steps it
+ * runs. */
 int v2c_ens_autotune(v2c_ens *ens, int reps, double *best_ms);
 
 #ifdef __cplusplus

@@ -146,7 +146,8 @@ def same_model(ca, cb, n, what):
 def test_combined_small_models_against_oracle(gpu, net):
     """Four reduced-size models (61x31 .. 101x51 nodes, uniform and refined,
     3 % fault markers, one with stoksmod 2 and one with markmod 1), three
-    combined steps: batched == per-model bit for bit; against the oracle,
+    combined steps: batched (two graph branches) == per-model bit for bit;
+    against the oracle,
     every marker and nodal field, the local update's outputs and the time
     step sequence."""
     steps = 3
@@ -154,6 +155,7 @@ def test_combined_small_models_against_oracle(gpu, net):
     batch = [make_ctx(gpu, c, net) for c in cfgs]
     looped = [make_ctx(gpu, c, net) for c in cfgs]
     eb, el = Ensemble(batch), Ensemble(looped)
+    eb.set_branches(2)
     for e in (eb, el):
         e.set_local(warm=2, tol=TOL)
         e.set_adaptive(ts_params(), 0, DT_MIN, DT_MAX)

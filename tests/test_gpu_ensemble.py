@@ -55,12 +55,17 @@ def _assert_same(a, b, what):
         assert np.array_equal(na[k], nb[k], equal_nan=True), (what, k)
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_ensemble_equals_single_models(gpu, graph):
+@pytest.mark.parametrize("graph,branches", [(True, 1), (False, 1), (True, 2), (True, 3), (False, 4)])
+def test_ensemble_equals_single_models(gpu, graph, branches):
+    """Batched step (graph replay or direct launches; the models split into
+    1..4 concurrent branches) == per-model entry points, bit for bit."""
     cfgs = _models()
     singles = [_ctx(gpu, c) for c in cfgs]
     batch = [_ctx(gpu, c) for c in cfgs]
     ens = Ensemble(batch)
+    ens.set_branches(branches)
+    assert ens.branches() == min(branches, len(cfgs))
+    print("case ensemble graph", graph, "branches", branches, flush=True)
     for step in range(2):
         for ctx, cfg in zip(singles, cfgs):
             _single_step(ctx, cfg, cfg.timesum + step * cfg.timestep)

@@ -529,6 +529,7 @@ def _bind_ens(lib):
     for name, args in [("v2c_ens_size_class", [C.c_int]), ("v2c_ens_set_shape", [C.c_int, P(TileShape)]),
                        ("v2c_ens_get_shape", [C.c_int, P(TileShape)]), ("v2c_ens_set_blocks_per_cu", [C.c_int]),
                        ("v2c_ens_step", [C.c_int]), ("v2c_ens_step_looped", []), ("v2c_ens_order", []),
+                       ("v2c_ens_set_branches", [C.c_int]),
                        ("v2c_ens_rms", [P(C.c_double)]),
                        ("v2c_ens_set_local", [C.c_int, C.c_double]),
                        ("v2c_ens_local_stats", [C.c_int, P(LUStats)]),
@@ -538,6 +539,8 @@ def _bind_ens(lib):
         fn = getattr(lib, name)
         fn.argtypes = [C.c_void_p] + args
         fn.restype = C.c_int
+    lib.v2c_ens_get_branches.argtypes = [C.c_void_p]
+    lib.v2c_ens_get_branches.restype = C.c_int
     lib._ens_bound = True
 
 
@@ -580,6 +583,13 @@ class Ensemble:
         s = TileShape()
         self._check(self.lib.v2c_ens_get_shape(self.h, cls, C.byref(s)), "ens_get_shape")
         return (s.cells_per_group, s.markers_per_tile, s.nodes_per_tile)
+
+    def set_branches(self, b):
+        """Run the step as b concurrent graph branches of model groups."""
+        self._check(self.lib.v2c_ens_set_branches(self.h, b), "ens_set_branches")
+
+    def branches(self):
+        return self.lib.v2c_ens_get_branches(self.h)
 
     def set_blocks_per_cu(self, b):
         self._check(self.lib.v2c_ens_set_blocks_per_cu(self.h, b), "ens_set_blocks_per_cu")

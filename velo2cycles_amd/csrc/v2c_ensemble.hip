@@ -108,11 +108,11 @@ __global__ __launch_bounds__(RED_BLOCK) void k_ens_after(const KArgs *__restrict
 
 // One workgroup per model: fixed-order sum of the model's partials.
 __global__ __launch_bounds__(RED_BLOCK) void k_ens_after_final(double *const *__restrict__ red,
-                                                               const int *__restrict__ nblocks, double *out)
+                                                               const int *__restrict__ nblocks, double *out, int k0)
 {
     __shared__ double ssum[RED_BLOCK];
     __shared__ double scnt[RED_BLOCK];
-    const int mod = blockIdx.x;
+    const int mod = k0 + (int)blockIdx.x;
     const double *r = red[mod];
     double s = 0, c = 0;
     for (int b = threadIdx.x; b < nblocks[mod]; b += RED_BLOCK) { s += r[2 * b]; c += r[2 * b + 1]; }
@@ -129,10 +129,10 @@ __global__ __launch_bounds__(RED_BLOCK) void k_ens_after_final(double *const *__
     }
 }
 
-__global__ void k_ens_phantom(const KArgs *__restrict__ A, int nmodels)
+__global__ void k_ens_phantom(const KArgs *__restrict__ A, int k0, int k1)
 {
-    const int mod = blockIdx.x * blockDim.x + threadIdx.x;
-    if (mod < nmodels && A[mod].p.antidiffinter == 1 && A[mod].p.n2m_phantom) n2m_phantom_one(model_args(A, mod));
+    const int mod = k0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (mod < k1 && A[mod].p.antidiffinter == 1 && A[mod].p.n2m_phantom) n2m_phantom_one(model_args(A, mod));
 }
 
 __global__ __launch_bounds__(WAVE) void k_ens_n2m_scatter(const KArgs *__restrict__ A,
@@ -196,9 +196,9 @@ __global__ __launch_bounds__(256) void k_ens_n2m_update_list(const KArgs *__rest
 }
 
 // ---- per-marker local update of every model (v2c_local.hip) ----------------
-__global__ void k_ens_lu_clear(const KArgs *__restrict__ A)
+__global__ void k_ens_lu_clear(const KArgs *__restrict__ A, int k0)
 {
-    unsigned long long *st = model_args(A, blockIdx.x).lu_stats;
+    unsigned long long *st = model_args(A, k0 + (int)blockIdx.x).lu_stats;
     for (int i = threadIdx.x; i < LS_COUNT; i += blockDim.x) st[i] = 0;
 }
 
@@ -211,7 +211,9 @@ __global__ __launch_bounds__(LU_BLOCK) void k_ens_local_ductile(const KArgs *__r
     const EnsTile T = tiles[blockIdx.x];
     const KArgs &a = model_args(A, T.model);
     const long m = T.begin + threadIdx.x;
-    if (m < T.end) lu_ductile_one(a, m, warm == 1, tol, sh);
+    LuAcc acc;
+    if (m < T.end) lu_ductile_one(a, m, warm == 1, tol, acc);
+    lu_wave_add(acc, sh);
     lu_flush(sh, a.lu_stats);
 }
 
@@ -258,10 +260,10 @@ __global__ __launch_bounds__(ENS_BLOCK) void k_ens_copyarrays(const KArgs *__res
     ens_item(A, tiles, [&](const KArgs &a, long m3) { copyarrays_one(a, m3); });
 }
 
-__global__ void k_ens_ts_clear(const EnsTs *__restrict__ S, int nm)
+__global__ void k_ens_ts_clear(const EnsTs *__restrict__ S, int k0, int k1)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nm) S[k].err[0] = 0;
+    const int k = k0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < k1) S[k].err[0] = 0;
 }
 
 // Node entries start at multiples of POST_BLOCK: entry = block begin / POST_BLOCK.
@@ -275,9 +277,9 @@ __global__ __launch_bounds__(POST_BLOCK) void k_ens_ts_lambda(const KArgs *__res
     ts_lambda_block(model_args(A, E.model), T, E.begin / POST_BLOCK, s.bset, s.bval, s.err);
 }
 
-__global__ __launch_bounds__(FIN_BLOCK) void k_ens_ts_carry(const EnsTs *__restrict__ S, TsArgs T)
+__global__ __launch_bounds__(FIN_BLOCK) void k_ens_ts_carry(const EnsTs *__restrict__ S, TsArgs T, int k0)
 {
-    const EnsTs &s = S[blockIdx.x];
+    const EnsTs &s = S[k0 + (int)blockIdx.x];
     ts_carry_block(s.bset, s.bval, s.nblocks, T.lambda0, s.carry);
 }
 
@@ -290,9 +292,9 @@ __global__ __launch_bounds__(POST_BLOCK) void k_ens_ts_node(const KArgs *__restr
     ts_node_block(model_args(A, E.model), T, E.begin / POST_BLOCK, s.carry, s.part);
 }
 
-__global__ __launch_bounds__(FIN_BLOCK) void k_ens_ts_final(const EnsTs *__restrict__ S, TsArgs T)
+__global__ __launch_bounds__(FIN_BLOCK) void k_ens_ts_final(const EnsTs *__restrict__ S, TsArgs T, int k0)
 {
-    const EnsTs &s = S[blockIdx.x];
+    const EnsTs &s = S[k0 + (int)blockIdx.x];
     ts_final_block(s.part, s.nblocks, T, s.out);
 }
 
@@ -416,6 +418,17 @@ struct EnsPlan {
     std::vector<EnsTile> markers, cells, nodes, otiles, faults;
 };
 
+// A branch of the step graph: the models [k0, k1) and their entries in the
+// marker, cell, node and fault lists, which hold the branches one after the
+// other; defer_off is the branch's part of the deferred-update list.
+constexpr int ENS_MAXBR = 4;
+struct EnsBranch {
+    int k0 = 0, k1 = 0;
+    int mark_off = 0, n_mark = 0, cell_off = 0, n_cell = 0, node_off = 0, n_node = 0, fault_off = 0,
+        n_fault = 0;
+    long defer_off = 0;
+};
+
 struct v2c_ens {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -445,6 +458,14 @@ struct v2c_ens {
     EnsTile *t_fault = nullptr;              // fault-list entries (local update)
     uint64_t *d_defer = nullptr;             // (model, marker) the fast fused update hands on
     long defer_cap = 0;
+    // branches: contiguous groups of models whose stage sequences run on
+    // their own streams inside the one graph, so that different stages of
+    // different groups overlap (a VALU-bound stage of one group beside a
+    // memory-bound stage of another); flags hold 4 ints per branch
+    int nbranch = 1, nbranch_req = 1;
+    EnsBranch br[ENS_MAXBR];
+    hipStream_t bstream[ENS_MAXBR] = {};
+    hipEvent_t bfork = nullptr, bjoin[ENS_MAXBR] = {};
     // optional stages: the local update (warm mode, tolerance) and adaptive steps
     int local_warm = -1;
     double local_tol = 1e-10;
@@ -543,29 +564,70 @@ static void ens_cut_runs(std::vector<std::vector<EnsTile>> &runs, int mod, long 
     }
 }
 
-static EnsPlan ens_plan(const v2c_ens *e)
+// Contiguous model ranges of about equal marker counts, one per branch.
+static void ens_branches(v2c_ens *e)
+{
+    const int nm = (int)e->models.size();
+    const int nb = std::max(1, std::min(e->nbranch_req, nm));
+    long total = 0;
+    for (auto *c : e->models) total += c->n;
+    int k = 0;
+    long acc = 0;
+    for (int b = 0; b < nb; b++) {
+        e->br[b] = EnsBranch{};
+        e->br[b].k0 = k;
+        const long target = total * (b + 1) / nb;
+        // at least one model per branch, and leave one for each later branch
+        do {
+            acc += e->models[k]->n;
+            k++;
+        } while (k < nm - (nb - 1 - b) && acc < target);
+        if (b == nb - 1) k = nm;
+        e->br[b].k1 = k;
+    }
+    e->nbranch = nb;
+}
+
+static EnsPlan ens_plan(v2c_ens *e)
 {
     EnsPlan p;
-    std::vector<std::vector<EnsTile>> mruns, nruns;
+    long defer = 0;
+    for (int b = 0; b < e->nbranch; b++) {
+        EnsBranch &B = e->br[b];
+        std::vector<std::vector<EnsTile>> mruns, nruns;
+        B.mark_off = (int)p.markers.size();
+        B.cell_off = (int)p.cells.size();
+        B.node_off = (int)p.nodes.size();
+        B.fault_off = (int)p.faults.size();
+        B.defer_off = defer;
+        for (int k = B.k0; k < B.k1; k++) {
+            const v2c_ctx *c = e->models[k];
+            const v2c_tile_shape &s = e->shape[e->size_class[k]];
+            ens_cut_runs(mruns, k, c->n, s.markers_per_tile / ENS_BLOCK);
+            ens_cut_runs(nruns, k, c->nodenum, s.nodes_per_tile / ENS_BLOCK);
+            for (long q = 0; q < c->ncell; q += s.cells_per_group)
+                p.cells.push_back(EnsTile{k, 0, q, std::min(c->ncell, q + (long)s.cells_per_group)});
+            for (long q = 0; q < c->nfault; q += LU_BLOCK)
+                p.faults.push_back(EnsTile{k, 0, q, std::min(c->nfault, q + (long)LU_BLOCK)});
+            defer += c->cap;
+        }
+        ens_item_list(p.markers, mruns);
+        ens_item_list(p.nodes, nruns);
+        B.n_mark = (int)p.markers.size() - B.mark_off;
+        B.n_cell = (int)p.cells.size() - B.cell_off;
+        B.n_node = (int)p.nodes.size() - B.node_off;
+        B.n_fault = (int)p.faults.size() - B.fault_off;
+    }
     for (size_t k = 0; k < e->models.size(); k++) {
         const v2c_ctx *c = e->models[k];
-        const v2c_tile_shape &s = e->shape[e->size_class[k]];
-        const int mod = (int)k;
-        ens_cut_runs(mruns, mod, c->n, s.markers_per_tile / ENS_BLOCK);
-        ens_cut_runs(nruns, mod, c->nodenum, s.nodes_per_tile / ENS_BLOCK);
-        for (long b = 0; b < c->ncell; b += s.cells_per_group)
-            p.cells.push_back(EnsTile{mod, 0, b, std::min(c->ncell, b + (long)s.cells_per_group)});
-        for (long t = 0; t < (c->n + ORD_TILE - 1) / ORD_TILE; t++) p.otiles.push_back(EnsTile{mod, 0, t, t + 1});
-        for (long b = 0; b < c->nfault; b += LU_BLOCK)
-            p.faults.push_back(EnsTile{mod, 0, b, std::min(c->nfault, b + (long)LU_BLOCK)});
+        for (long t = 0; t < (c->n + ORD_TILE - 1) / ORD_TILE; t++) p.otiles.push_back(EnsTile{(int)k, 0, t, t + 1});
     }
-    ens_item_list(p.markers, mruns);
-    ens_item_list(p.nodes, nruns);
     return p;
 }
 
 static int ens_upload_plan(v2c_ens *e)
 {
+    ens_branches(e);
     const EnsPlan p = ens_plan(e);
     const long total = (long)(p.markers.size() + p.cells.size() + p.nodes.size() + p.otiles.size() +
                               p.faults.size());
@@ -610,73 +672,102 @@ static unsigned ens_grid(const v2c_ens *e, int ntiles)
     return (unsigned)std::max(1L, std::min((long)ntiles, g));
 }
 
-// The graph-captured stage sequence of one step.
-static void ens_stages(v2c_ens *e)
+// The stage sequence of branch b on stream st.  side: the slip-rate solves
+// may fork to the side stream (one branch only); otherwise they run before
+// the ductile solves on st.
+static void ens_branch_stages(v2c_ens *e, int b, hipStream_t st, bool side)
 {
-    const int nm = (int)e->models.size();
-    const dim3 mg((unsigned)e->n_mark), ng((unsigned)e->n_node);
-    hipLaunchKernelGGL(k_ens_clear, dim3(1), dim3(64), 0, e->stream, e->d_flags, 4);
-    // (the property pass ran on the side stream: ens_props)
-    if (e->n_cell)
-        hipLaunchKernelGGL(k_ens_m2n_cells, dim3(ens_grid(e, e->n_cell)), dim3(WAVE), 0, e->stream, e->d_args,
-                           e->t_cell, e->n_cell);
-    if (e->n_node) {
-        hipLaunchKernelGGL(k_ens_m2n_nodes, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
+    const EnsBranch &B = e->br[b];
+    const int k0 = B.k0, nmb = B.k1 - B.k0;
+    const EnsTile *t_mark = e->t_mark + B.mark_off, *t_cell = e->t_cell + B.cell_off;
+    const EnsTile *t_node = e->t_node + B.node_off, *t_fault = e->t_fault + B.fault_off;
+    int *flags = e->d_flags + 4 * b;
+    uint64_t *defer = e->d_defer + B.defer_off;
+    const dim3 mg((unsigned)B.n_mark), ng((unsigned)B.n_node);
+    auto grid = [&](int ntiles) {
+        const long g = std::max(1L, (long)e->ncu * e->blocks_per_cu * 4 / e->nbranch);
+        return (unsigned)std::max(1L, std::min((long)ntiles, g));
+    };
+    hipLaunchKernelGGL(k_ens_clear, dim3(1), dim3(64), 0, st, flags, 4);
+    // (the property pass ran on the side stream beside the re-ordering:
+    // ens_props; inside the branches it measured slower at 24 production
+    // models, where it would no longer overlap the re-ordering)
+    if (B.n_cell)
+        hipLaunchKernelGGL(k_ens_m2n_cells, dim3(grid(B.n_cell)), dim3(WAVE), 0, st, e->d_args, t_cell, B.n_cell);
+    if (B.n_node) {
+        hipLaunchKernelGGL(k_ens_m2n_nodes, ng, dim3(ENS_BLOCK), 0, st, e->d_args, t_node);
         bool dphis = false;
-        for (auto *c : e->models) dphis = dphis || c->prm.Dphis > 0;
-        if (dphis) hipLaunchKernelGGL(k_ens_phis_bc, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
-        hipLaunchKernelGGL(k_ens_before, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
-        hipLaunchKernelGGL(k_ens_after, ng, dim3(RED_BLOCK), 0, e->stream, e->d_args, e->d_red, e->t_node);
+        for (int k = B.k0; k < B.k1; k++) dphis = dphis || e->models[k]->prm.Dphis > 0;
+        if (dphis) hipLaunchKernelGGL(k_ens_phis_bc, ng, dim3(ENS_BLOCK), 0, st, e->d_args, t_node);
+        hipLaunchKernelGGL(k_ens_before, ng, dim3(ENS_BLOCK), 0, st, e->d_args, t_node);
+        hipLaunchKernelGGL(k_ens_after, ng, dim3(RED_BLOCK), 0, st, e->d_args, e->d_red, t_node);
     }
-    hipLaunchKernelGGL(k_ens_after_final, dim3(nm), dim3(RED_BLOCK), 0, e->stream, e->d_red, e->d_redn, e->d_rms);
+    hipLaunchKernelGGL(k_ens_after_final, dim3(nmb), dim3(RED_BLOCK), 0, st, e->d_red, e->d_redn, e->d_rms, k0);
     if (e->local_warm >= 0) {
         // local update of every model: the slip-rate solves (fault-list
-        // entries) on the side stream beside the ductile solves (marker
-        // entries), then the visco-plastic viscosity that needs both
-        hipLaunchKernelGGL(k_ens_lu_clear, dim3(nm), dim3(64), 0, e->stream, e->d_args);
-        if (e->n_fault) {
-            (void)hipEventRecord(e->ev_lfork, e->stream);
+        // entries), beside the ductile solves (marker entries) on the side
+        // stream when there is one branch, then the visco-plastic viscosity
+        // that needs both
+        hipLaunchKernelGGL(k_ens_lu_clear, dim3(nmb), dim3(64), 0, st, e->d_args, k0);
+        hipStream_t rs = st;
+        if (B.n_fault && side) {
+            (void)hipEventRecord(e->ev_lfork, st);
             (void)hipStreamWaitEvent(e->side, e->ev_lfork, 0);
-            const dim3 fg((unsigned)e->n_fault);
+            rs = e->side;
+        }
+        if (B.n_fault) {
+            const dim3 fg((unsigned)B.n_fault);
             if (e->local_warm == 2)
-                hipLaunchKernelGGL(k_ens_local_rsf<2>, fg, dim3(LU_BLOCK), 0, e->side, e->d_args, e->t_fault,
-                                   e->local_tol);
+                hipLaunchKernelGGL(k_ens_local_rsf<2>, fg, dim3(LU_BLOCK), 0, rs, e->d_args, t_fault, e->local_tol);
             else if (e->local_warm == 1)
-                hipLaunchKernelGGL(k_ens_local_rsf<1>, fg, dim3(LU_BLOCK), 0, e->side, e->d_args, e->t_fault,
-                                   e->local_tol);
+                hipLaunchKernelGGL(k_ens_local_rsf<1>, fg, dim3(LU_BLOCK), 0, rs, e->d_args, t_fault, e->local_tol);
             else
-                hipLaunchKernelGGL(k_ens_local_rsf<0>, fg, dim3(LU_BLOCK), 0, e->side, e->d_args, e->t_fault,
-                                   e->local_tol);
-            (void)hipEventRecord(e->ev_ljoin, e->side);
+                hipLaunchKernelGGL(k_ens_local_rsf<0>, fg, dim3(LU_BLOCK), 0, rs, e->d_args, t_fault, e->local_tol);
+            if (side) (void)hipEventRecord(e->ev_ljoin, e->side);
         }
-        if (e->n_mark)
-            hipLaunchKernelGGL(k_ens_local_ductile, mg, dim3(LU_BLOCK), 0, e->stream, e->d_args, e->t_mark,
-                               e->local_warm, e->local_tol);
-        if (e->n_fault) {
-            (void)hipStreamWaitEvent(e->stream, e->ev_ljoin, 0);
-            hipLaunchKernelGGL(k_ens_local_etavp, dim3((unsigned)e->n_fault), dim3(LU_BLOCK), 0, e->stream,
-                               e->d_args, e->t_fault);
+        if (B.n_mark)
+            hipLaunchKernelGGL(k_ens_local_ductile, mg, dim3(LU_BLOCK), 0, st, e->d_args, t_mark, e->local_warm,
+                               e->local_tol);
+        if (B.n_fault) {
+            if (side) (void)hipStreamWaitEvent(st, e->ev_ljoin, 0);
+            hipLaunchKernelGGL(k_ens_local_etavp, dim3((unsigned)B.n_fault), dim3(LU_BLOCK), 0, st, e->d_args,
+                               t_fault);
         }
     }
-    hipLaunchKernelGGL(k_ens_phantom, dim3((nm + 63) / 64), dim3(64), 0, e->stream, e->d_args, nm);
-    if (e->n_cell)
-        hipLaunchKernelGGL(k_ens_n2m_scatter, dim3(ens_grid(e, e->n_cell)), dim3(WAVE), 0, e->stream, e->d_args,
-                           e->t_cell, e->n_cell);
-    if (e->n_node) hipLaunchKernelGGL(k_ens_n2m_nodes, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
-    if (e->n_mark) {
-        hipLaunchKernelGGL(k_ens_n2m_update_fast, mg, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_mark,
-                           e->d_defer, e->d_flags + 1);
-        hipLaunchKernelGGL(k_ens_n2m_update_list, dim3(N2M_LIST_BLOCKS), dim3(256), 0, e->stream, e->d_args,
-                           e->d_defer, e->d_flags + 1, e->d_flags);
+    hipLaunchKernelGGL(k_ens_phantom, dim3((nmb + 63) / 64), dim3(64), 0, st, e->d_args, B.k0, B.k1);
+    if (B.n_cell)
+        hipLaunchKernelGGL(k_ens_n2m_scatter, dim3(grid(B.n_cell)), dim3(WAVE), 0, st, e->d_args, t_cell, B.n_cell);
+    if (B.n_node) hipLaunchKernelGGL(k_ens_n2m_nodes, ng, dim3(ENS_BLOCK), 0, st, e->d_args, t_node);
+    if (B.n_mark) {
+        hipLaunchKernelGGL(k_ens_n2m_update_fast, mg, dim3(ENS_BLOCK), 0, st, e->d_args, t_mark, defer, flags + 1);
+        hipLaunchKernelGGL(k_ens_n2m_update_list, dim3(N2M_LIST_BLOCKS), dim3(256), 0, st, e->d_args, defer,
+                           flags + 1, flags);
     }
-    if (e->adaptive && e->n_node) {
+    if (e->adaptive && B.n_node) {
         // the next time step of every model: copyarrays, then calc_timestep
-        hipLaunchKernelGGL(k_ens_copyarrays, ng, dim3(ENS_BLOCK), 0, e->stream, e->d_args, e->t_node);
-        hipLaunchKernelGGL(k_ens_ts_clear, dim3((nm + 63) / 64), dim3(64), 0, e->stream, e->d_ts, nm);
-        hipLaunchKernelGGL(k_ens_ts_lambda, ng, dim3(POST_BLOCK), 0, e->stream, e->d_args, e->t_node, e->d_ts, e->ts);
-        hipLaunchKernelGGL(k_ens_ts_carry, dim3(nm), dim3(FIN_BLOCK), 0, e->stream, e->d_ts, e->ts);
-        hipLaunchKernelGGL(k_ens_ts_node, ng, dim3(POST_BLOCK), 0, e->stream, e->d_args, e->t_node, e->d_ts, e->ts);
-        hipLaunchKernelGGL(k_ens_ts_final, dim3(nm), dim3(FIN_BLOCK), 0, e->stream, e->d_ts, e->ts);
+        hipLaunchKernelGGL(k_ens_copyarrays, ng, dim3(ENS_BLOCK), 0, st, e->d_args, t_node);
+        hipLaunchKernelGGL(k_ens_ts_clear, dim3((nmb + 63) / 64), dim3(64), 0, st, e->d_ts, B.k0, B.k1);
+        hipLaunchKernelGGL(k_ens_ts_lambda, ng, dim3(POST_BLOCK), 0, st, e->d_args, t_node, e->d_ts, e->ts);
+        hipLaunchKernelGGL(k_ens_ts_carry, dim3(nmb), dim3(FIN_BLOCK), 0, st, e->d_ts, e->ts, k0);
+        hipLaunchKernelGGL(k_ens_ts_node, ng, dim3(POST_BLOCK), 0, st, e->d_args, t_node, e->d_ts, e->ts);
+        hipLaunchKernelGGL(k_ens_ts_final, dim3(nmb), dim3(FIN_BLOCK), 0, st, e->d_ts, e->ts, k0);
+    }
+}
+
+// The graph-captured stage sequence of one step: every branch on its own
+// stream, forked from and joined back to the ensemble stream.
+static void ens_stages(v2c_ens *e)
+{
+    if (e->nbranch == 1) {
+        ens_branch_stages(e, 0, e->stream, true);
+        return;
+    }
+    (void)hipEventRecord(e->bfork, e->stream);
+    for (int b = 1; b < e->nbranch; b++) (void)hipStreamWaitEvent(e->bstream[b], e->bfork, 0);
+    for (int b = 0; b < e->nbranch; b++) ens_branch_stages(e, b, b ? e->bstream[b] : e->stream, false);
+    for (int b = 1; b < e->nbranch; b++) {
+        (void)hipEventRecord(e->bjoin[b], e->bstream[b]);
+        (void)hipStreamWaitEvent(e->stream, e->bjoin[b], 0);
     }
 }
 
@@ -853,7 +944,8 @@ static int ens_finish_step(v2c_ens *e)
 // calc_timestep results and error flags of every model.
 static int ens_readback(v2c_ens *e, int *bad)
 {
-    ENSCHK(e, hipMemcpyAsync(bad, e->d_flags, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    int fl[4 * ENS_MAXBR];
+    ENSCHK(e, hipMemcpyAsync(fl, e->d_flags, sizeof(int) * 4 * e->nbranch, hipMemcpyDeviceToHost, e->stream));
     const size_t nm = e->models.size();
     std::vector<int> err(nm, 0);
     if (e->adaptive) {
@@ -861,6 +953,8 @@ static int ens_readback(v2c_ens *e, int *bad)
         ENSCHK(e, hipMemcpyAsync(err.data(), e->d_tserr, sizeof(int) * nm, hipMemcpyDeviceToHost, e->stream));
     }
     ENSCHK(e, hipStreamSynchronize(e->stream));
+    *bad = 0;
+    for (int b = 0; b < e->nbranch; b++) *bad += fl[4 * b];
     for (size_t k = 0; k < nm; k++)
         if (err[k])
             return ens_err(e, V2C_EUNSUPPORTED, "model %zu: calc_timestep: %s", k,
@@ -915,7 +1009,12 @@ int v2c_ens_create(v2c_ens **out, v2c_ctx *const *models, int nmodels)
     if (hipMalloc(&e->d_red, sizeof(double *) * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
     if (hipMalloc(&e->d_redn, sizeof(int) * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
     if (hipMalloc(&e->d_rms, sizeof(double) * 2 * nmodels) != hipSuccess) return fail(V2C_ENOMEM);
-    if (hipMalloc(&e->d_flags, sizeof(int) * 4) != hipSuccess) return fail(V2C_ENOMEM);
+    if (hipMalloc(&e->d_flags, sizeof(int) * 4 * ENS_MAXBR) != hipSuccess) return fail(V2C_ENOMEM);
+    if (hipEventCreateWithFlags(&e->bfork, hipEventDisableTiming) != hipSuccess) return fail(V2C_EHIP);
+    for (int b = 1; b < ENS_MAXBR; b++)
+        if (hipStreamCreateWithFlags(&e->bstream[b], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->bjoin[b], hipEventDisableTiming) != hipSuccess)
+            return fail(V2C_EHIP);
     if (hipMemcpy(e->d_red, reds.data(), sizeof(double *) * nmodels, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(e->d_redn, nred.data(), sizeof(int) * nmodels, hipMemcpyHostToDevice) != hipSuccess)
         return fail(V2C_EHIP);
@@ -974,6 +1073,14 @@ void v2c_ens_destroy(v2c_ens *e)
     if (e->ev_lfork) (void)hipEventDestroy(e->ev_lfork);
     if (e->ev_ljoin) (void)hipEventDestroy(e->ev_ljoin);
     if (e->ev_pre) (void)hipEventDestroy(e->ev_pre);
+    if (e->bfork) (void)hipEventDestroy(e->bfork);
+    for (int b = 1; b < ENS_MAXBR; b++) {
+        if (e->bstream[b]) {
+            (void)hipStreamSynchronize(e->bstream[b]);
+            (void)hipStreamDestroy(e->bstream[b]);
+        }
+        if (e->bjoin[b]) (void)hipEventDestroy(e->bjoin[b]);
+    }
     if (e->h_args) (void)hipHostFree(e->h_args);
     if (e->h_args_p) (void)hipHostFree(e->h_args_p);
     if (e->ev_props) (void)hipEventDestroy(e->ev_props);
@@ -1020,6 +1127,16 @@ int v2c_ens_set_blocks_per_cu(v2c_ens *e, int blocks)
     e->graph_valid = false;
     return 0;
 }
+
+int v2c_ens_set_branches(v2c_ens *e, int branches)
+{
+    if (!e || branches < 1 || branches > ENS_MAXBR) return V2C_EINVAL;
+    ENSCHK(e, hipSetDevice(e->device));
+    e->nbranch_req = branches;
+    return ens_upload_plan(e);
+}
+
+int v2c_ens_get_branches(const v2c_ens *e) { return e ? e->nbranch : V2C_EINVAL; }
 
 int v2c_ens_step(v2c_ens *e, int use_graph)
 {
@@ -1271,6 +1388,16 @@ int v2c_ens_autotune(v2c_ens *e, int reps, double *best_ms)
         if (!rc && ms < best) { best = ms; keepb = bpc[q]; }
     }
     if (!rc) v2c_ens_set_blocks_per_cu(e, keepb);
+    // concurrent branches of model groups
+    int keepbr = e->nbranch_req;
+    for (int b = 1; b <= ENS_MAXBR && !rc; b++) {
+        if (b == keepbr || b > (int)e->models.size()) continue;
+        rc = v2c_ens_set_branches(e, b);
+        double ms = 0;
+        if (!rc) rc = measure(ms);
+        if (!rc && ms < best) { best = ms; keepbr = b; }
+    }
+    if (!rc) rc = v2c_ens_set_branches(e, keepbr);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (best_ms) *best_ms = best;

@@ -15,7 +15,8 @@
 //              slip-rate kernel (BF16 MFMA, one wavefront per 64 fault
 //              markers); the ductile solve starts from its default, where it
 //              converges in about one iteration.
-// Statistics go to lu_stats with integer atomics (order independent).
+// Statistics go to lu_stats with integer atomics (order independent), summed
+// per wavefront first.
 #pragma once
 #include "v2c_mlp.hip"
 
@@ -43,8 +44,73 @@ __device__ __forceinline__ void lu_flush(const unsigned long long *sh, unsigned 
     }
 }
 
+// One marker's contribution to the statistics (hist < 0: none).
+struct LuAcc {
+    int n = 0, duct = 0, duct_it = 0, duct_ev = 0, duct_hist = -1;
+    int fault = 0, rsf_it = 0, rsf_ev = 0, rsf_hist = -1;
+    int fallback = 0, nonfinite = 0;
+};
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = WAVE / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+// Histogram bins of a wavefront: one LDS add per distinct bin.
+__device__ __forceinline__ void wave_hist(int bin, unsigned long long *sh, int base, int lane)
+{
+    unsigned long long pending = __ballot(bin >= 0);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int b = __shfl(bin, leader, WAVE);
+        const unsigned long long same = __ballot(bin == b);
+        if (lane == leader) atomicAdd(&sh[base + b], (unsigned long long)__popcll(same));
+        pending &= ~same;
+    }
+}
+
+// Every lane of the wavefront calls this with its marker's contribution
+// (zeros for lanes without one); lane 0 adds the wavefront's totals to the
+// workgroup's counters, so the LDS sees a handful of atomics per wavefront
+// instead of several per marker.
+__device__ __forceinline__ void lu_wave_add(const LuAcc &c, unsigned long long *sh)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int n = (int)__popcll(__ballot(c.n != 0)), duct = (int)__popcll(__ballot(c.duct != 0));
+    const int fault = (int)__popcll(__ballot(c.fault != 0));
+    const int fb = (int)__popcll(__ballot(c.fallback != 0)), nf = (int)__popcll(__ballot(c.nonfinite != 0));
+    const int dit = wave_sum(c.duct_it), dev = wave_sum(c.duct_ev), dmax = wave_max(c.duct_it);
+    const int rit = wave_sum(c.rsf_it), rev = wave_sum(c.rsf_ev), rmax = wave_max(c.rsf_it);
+    wave_hist(c.duct_hist, sh, LS_HIST_D, lane);
+    wave_hist(c.rsf_hist, sh, LS_HIST_R, lane);
+    if (lane == 0) {
+        if (n) atomicAdd(&sh[LS_N], (unsigned long long)n);
+        if (duct) {
+            atomicAdd(&sh[LS_DUCT], (unsigned long long)duct);
+            atomicAdd(&sh[LS_DUCT_IT], (unsigned long long)dit);
+            atomicAdd(&sh[LS_DUCT_EV], (unsigned long long)dev);
+            atomicMax(&sh[LS_DUCT_MAX], (unsigned long long)dmax);
+        }
+        if (fault) {
+            atomicAdd(&sh[LS_FAULT], (unsigned long long)fault);
+            atomicAdd(&sh[LS_RSF_IT], (unsigned long long)rit);
+            atomicAdd(&sh[LS_RSF_EV], (unsigned long long)rev);
+            atomicMax(&sh[LS_RSF_MAX], (unsigned long long)rmax);
+        }
+        if (fb) atomicAdd(&sh[LS_FALLBACK], (unsigned long long)fb);
+        if (nf) atomicAdd(&sh[LS_NONFINITE], (unsigned long long)nf);
+    }
+}
+
 // Ductile solve of marker m (all markers).
-__device__ __forceinline__ void lu_ductile_one(const KArgs &a, long m, bool warm, double tol, unsigned long long *sh)
+__device__ __forceinline__ void lu_ductile_one(const KArgs &a, long m, bool warm, double tol, LuAcc &acc)
 {
     const int t = a.t[m];
     const double sxx = MK(a, MF_XX, m), sxy = MK(a, MF_XY, m);
@@ -55,23 +121,22 @@ __device__ __forceinline__ void lu_ductile_one(const KArgs &a, long m, bool warm
                                       &fb, &root_eta);
     MXF(a, MX_ETA, m) = eta;
     MXF(a, MX_ROOT_ETA, m) = root_eta;
-    atomicAdd(&sh[LS_N], 1ull);
+    acc.n = 1;
     if (power_law_type(a.mat, t) && t != 1) {
-        atomicAdd(&sh[LS_DUCT], 1ull);
-        atomicAdd(&sh[LS_DUCT_IT], (unsigned long long)it);
-        atomicAdd(&sh[LS_DUCT_EV], (unsigned long long)ev);
-        atomicMax(&sh[LS_DUCT_MAX], (unsigned long long)it);
-        atomicAdd(&sh[LS_HIST_D + (it < LU_HIST ? it : LU_HIST - 1)], 1ull);
+        acc.duct = 1;
+        acc.duct_it = it;
+        acc.duct_ev = ev;
+        acc.duct_hist = it < LU_HIST ? it : LU_HIST - 1;
     }
-    if (fb) atomicAdd(&sh[LS_FALLBACK], 1ull);
-    if (!isfinite(eta)) atomicAdd(&sh[LS_NONFINITE], 1ull);
+    if (fb) acc.fallback = 1;
+    if (!isfinite(eta)) acc.nonfinite = 1;
 }
 
 // Slip-rate solve of fault marker m from the guess guessV (0: default start).
 // It does not read the ductile solve's result, so the two run side by side;
 // k_local_etavp combines them.
 __device__ __forceinline__ void lu_rsf_one(const KArgs &a, long m, double dt, double guessV, double tol,
-                                           unsigned long long *sh)
+                                           LuAcc &acc)
 {
     const double sxx = MK(a, MF_XX, m), sxy = MK(a, MF_XY, m), p = MK(a, MF_P, m);
     const double exx = MK(a, MF_EXX, m), exy = MK(a, MF_EXY, m);
@@ -89,13 +154,12 @@ __device__ __forceinline__ void lu_rsf_one(const KArgs &a, long m, double dt, do
     MXF(a, MX_V, m) = V;
     MXF(a, MX_PHI, m) = phin;
     MXF(a, MX_SY, m) = sig;
-    atomicAdd(&sh[LS_FAULT], 1ull);
-    atomicAdd(&sh[LS_RSF_IT], (unsigned long long)rit);
-    atomicAdd(&sh[LS_RSF_EV], (unsigned long long)rev);
-    atomicMax(&sh[LS_RSF_MAX], (unsigned long long)rit);
-    atomicAdd(&sh[LS_HIST_R + (rit < LU_HIST ? rit : LU_HIST - 1)], 1ull);
-    if (rfb) atomicAdd(&sh[LS_FALLBACK], 1ull);
-    if (!isfinite(V) || !isfinite(phin)) atomicAdd(&sh[LS_NONFINITE], 1ull);
+    acc.fault = 1;
+    acc.rsf_it = rit;
+    acc.rsf_ev = rev;
+    acc.rsf_hist = rit < LU_HIST ? rit : LU_HIST - 1;
+    if (rfb) acc.fallback = 1;
+    if (!isfinite(V) || !isfinite(phin)) acc.nonfinite = 1;
 }
 
 __global__ __launch_bounds__(LU_BLOCK) void k_local_ductile(KArgs a, int warm, double tol)
@@ -103,7 +167,9 @@ __global__ __launch_bounds__(LU_BLOCK) void k_local_ductile(KArgs a, int warm, d
     __shared__ unsigned long long sh[LS_COUNT];
     lu_clear(sh);
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < a.n) lu_ductile_one(a, m, warm == 1, tol, sh);
+    LuAcc acc;
+    if (m < a.n) lu_ductile_one(a, m, warm == 1, tol, acc);
+    lu_wave_add(acc, sh);
     lu_flush(sh, a.lu_stats);
 }
 
@@ -121,6 +187,7 @@ template <int WARM>
 __device__ __forceinline__ void lu_rsf_block(const KArgs &a, long i, long iend, double dt, double tol,
                                              unsigned long long *sh, LuRsfLds *L)
 {
+    LuAcc acc;
     const bool live = i < iend;
     const long m = live ? (long)a.fault_list[i] : 0;
     double guess = 0.0;
@@ -143,7 +210,8 @@ __device__ __forceinline__ void lu_rsf_block(const KArgs &a, long i, long iend, 
             guess = ft.vhi * pow(10.0, r1);
         }
     }
-    if (live) lu_rsf_one(a, m, dt, guess, tol, sh);
+    if (live) lu_rsf_one(a, m, dt, guess, tol, acc);
+    lu_wave_add(acc, sh);
 }
 
 template <int WARM>
