@@ -690,8 +690,9 @@ static void ens_branch_stages(v2c_ens *e, int b, hipStream_t st, bool side)
     };
     hipLaunchKernelGGL(k_ens_clear, dim3(1), dim3(64), 0, st, flags, 4);
     // (the property pass ran on the side stream beside the re-ordering:
-    // ens_props; inside the branches it measured slower at 24 production
-    // models, where it would no longer overlap the re-ordering)
+    // ens_props.  Inside the branches, all of them or all but the first,
+    // it measured 2-6 % slower at 24 production models, where it no longer
+    // overlaps the re-ordering)
     if (B.n_cell)
         hipLaunchKernelGGL(k_ens_m2n_cells, dim3(grid(B.n_cell)), dim3(WAVE), 0, st, e->d_args, t_cell, B.n_cell);
     if (B.n_node) {
@@ -899,11 +900,15 @@ static int ens_props(v2c_ens *e)
 {
     if (!e->n_mark) return 0;
     const size_t nm = e->models.size();
-    // ordered after all work queued so far on the ensemble stream and on
-    // every model's stream (the events are re-recorded by the ordering)
+    // ordered after all work queued so far on the ensemble stream and on the
+    // streams of models in asynchronous mode (the only ones a caller can have
+    // left work on: every other entry point returns with its stream drained,
+    // and the previous step's per-model ordering work was joined into the
+    // ensemble stream); the events are re-recorded by the ordering
     ENSCHK(e, hipEventRecord(e->ev_pre, e->stream));
     ENSCHK(e, hipStreamWaitEvent(e->side, e->ev_pre, 0));
     for (size_t k = 0; k < nm; k++) {
+        if (!e->models[k]->async_mode) continue;
         ENSCHK(e, hipEventRecord(e->ordered[k], e->models[k]->stream));
         ENSCHK(e, hipStreamWaitEvent(e->side, e->ordered[k], 0));
     }
