@@ -529,7 +529,12 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
         memcpy(dm.kt, mat->kt, sizeof dm.kt); memcpy(dm.kf, mat->kf, sizeof dm.kf);
         memcpy(dm.kp, mat->kp, sizeof dm.kp); memcpy(dm.ht, mat->ht, sizeof dm.ht);
         memcpy(dm.im, mat->im, sizeof dm.im);
-        for (int t = 0; t < V2C_MAX_TYPES; t++) dm.ssn1[t] = pow(mat->ss[t], mat->mm[t] - 1.0);
+        for (int t = 0; t < V2C_MAX_TYPES; t++) {
+            dm.ssn1[t] = pow(mat->ss[t], mat->mm[t] - 1.0);
+            dm.k1[t] = mat->nu[t] / dm.ssn1[t];
+            dm.rn[t] = 1.0 / mat->mm[t];
+            dm.ex1[t] = (mat->mm[t] - 1) / mat->mm[t];
+        }
         CK(hipMemcpyAsync(c->d_mat, &dm, sizeof dm, hipMemcpyHostToDevice, c->stream));
         CK(hipStreamSynchronize(c->stream));
     }

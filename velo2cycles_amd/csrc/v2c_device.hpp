@@ -39,6 +39,9 @@ struct DevMat {
     double ro[V2C_MAX_TYPES], bb[V2C_MAX_TYPES], aa[V2C_MAX_TYPES], cp[V2C_MAX_TYPES];
     double kt[V2C_MAX_TYPES], kf[V2C_MAX_TYPES], kp[V2C_MAX_TYPES], ht[V2C_MAX_TYPES];
     double ssn1[V2C_MAX_TYPES];     // ss^(mm-1), computed once on the host
+    // per-type quotients of the viscosity laws, computed once on the host
+    // (IEEE division there and here round alike): nu/ss^(mm-1), 1/mm, (mm-1)/mm
+    double k1[V2C_MAX_TYPES], rn[V2C_MAX_TYPES], ex1[V2C_MAX_TYPES];
     int im[V2C_MAX_TYPES];
 };
 
@@ -387,14 +390,14 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
     double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * mtk);
     if (e1 > 150.0) e1 = 150.0;
     e1 = exp(e1);
-    double k1 = M->nu[t] / M->ssn1[t];
+    double k1 = M->k1[t];
     double inv_newt = 1.0 / (0.5 * k1 * e1);
     double inv_powl = 0;
     if (sigin > 0) {
         double eta_disl = 0.5 * sigin * M->nu[t] * e1 / pow(sigin, n);
         double eta_new = eta_disl;
         double eta_diff = 0.5 * k1 * e1;
-        double coeff = pow(eta_disl, 1.0 / n);
+        double coeff = pow(eta_disl, M->rn[t]);
         double gdt = M->gg[t] * a.timestep;
         double mu_prev = 0, mu;
         // Loop-invariant pieces of the iteration, and two exact rewrites:
@@ -402,7 +405,7 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
         // exp(2/(eta_d+eta) * ln eta_d) with ln eta_d hoisted.  The stop test
         // |log10(mu_k/mu_{k-1})| < 1e-6 is applied as the equivalent bounds on
         // the ratio (10^-1e-6, 10^1e-6).
-        const double ex1 = (n - 1) / n, c1 = (n - 1) / (n * eta_diff);
+        const double ex1 = M->ex1[t], c1 = (n - 1) / (n * eta_diff);
         const double inv_ed = 1.0 / eta_diff, ln_ed = log(eta_diff);
         const double r_lo = 0.999997697417558, r_hi = 1.000002302587744;
         int k = 0;
@@ -411,7 +414,9 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
             k++;
             const double eta_old = eta_new;
             mu = 1.0 / (inv_ed + 1.0 / eta_new);
-            const double pref = (eta_diff + eta_old) / eta_diff;
+            // (eta_diff + eta_old) / eta_diff as the exact division by the
+            // correctly rounded reciprocal already at hand
+            const double pref = div_rcp(eta_diff + eta_old, eta_diff, inv_ed);
             const double gm = gdt + mu;
             const double X = pref * gm;
             const double f1 = exp(ex1 * log(X));
@@ -728,13 +733,13 @@ __device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double s
     double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * mtk);
     if (e1 > 150.0) e1 = 150.0;
     e1 = exp(e1);
-    const double k1 = M->nu[t] / M->ssn1[t];
+    const double k1 = M->k1[t];
     const double eta_diff = 0.5 * k1 * e1;
     if (!(sigin > 0)) return clamp_visc(a, t, eta_diff);
     const double eta_disl0 = 0.5 * sigin * M->nu[t] * e1 / pow(sigin, n);
-    const double coeff = pow(eta_disl0, 1.0 / n);
+    const double coeff = pow(eta_disl0, M->rn[t]);
     const double gdt = M->gg[t] * a.timestep;
-    const double p1 = (n - 1) / n, slope = gdt / eta_diff + 1;
+    const double p1 = M->ex1[t], slope = gdt / eta_diff + 1;
     auto resid = [&](double eta, double &f, double &X, double &Xp) {
         X = (eta_diff + eta) / eta_diff * gdt + eta;
         Xp = pow(X, p1);
