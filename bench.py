@@ -291,7 +291,7 @@ def _warm_regime(lib, dt: float, maker, label: str, seeds=(11, 12, 13, 14), keep
     loss = ws.train(X, M, epochs=epochs, batch=4096, lr=2e-3)
     t_train = time.perf_counter() - t0
     modes = ("plain", "warm", "warm_rsf")
-    res = evaluate(ctx, cfg, ws, dt, tol=1e-10, repeats=3, modes=modes)
+    res = evaluate(ctx, cfg, ws, dt, tol=1e-10, repeats=7, modes=modes)
     ctx.close()
     out = {"workload": label, "dt_s": dt, "residual": 1e-10, "training_rows": int(len(X)),
            "slip_rate_rows": int((M[:, 1] > 0).sum()), "epochs": epochs,

@@ -229,10 +229,11 @@ def evaluate(ctx, cfg, ws: WarmStart, dt, tol=1e-10, repeats=3, modes=("plain", 
     fork to join (ms_wall), the separate inference (ms_mlp), and
     ms_total = ms_wall + ms_mlp."""
     res = {}
-    for mode in modes:
-        outputs, warm = MODES[mode]
-        best = None
-        for _ in range(repeats):
+    # repeats interleaved across the modes, so that drifts of the machine's
+    # clock or load fall on every mode alike; the best of each mode is kept
+    for r in range(repeats):
+        for mode in modes:
+            outputs, warm = MODES[mode]
             upload_cloud(ctx, cfg)
             ctx.enable_timing(True)
             ctx.reset_timing()
@@ -245,9 +246,9 @@ def evaluate(ctx, cfg, ws: WarmStart, dt, tol=1e-10, repeats=3, modes=("plain", 
             ms_wall, _ = ctx.kernel_time("local_wall")
             ctx.enable_timing(False)
             tot = ms_wall + ms_mlp
+            best = res.get(mode)
             if best is None or tot < best["ms_total"]:
-                best = dict(stats=st, ms_ductile=ms_d, ms_rsf=ms_r, ms_mlp=ms_mlp, ms_wall=ms_wall, ms_total=tot)
-        out = ctx.local_download(cfg.n)
-        best["out"] = out
-        res[mode] = best
+                res[mode] = dict(stats=st, ms_ductile=ms_d, ms_rsf=ms_r, ms_mlp=ms_mlp, ms_wall=ms_wall, ms_total=tot)
+            if r == repeats - 1:
+                res[mode]["out"] = ctx.local_download(cfg.n)
     return res
