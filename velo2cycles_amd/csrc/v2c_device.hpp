@@ -13,6 +13,7 @@
 #include <stdint.h>
 #include "../../include/v2c.h"
 #include "v2c_search.hpp"
+#include "v2c_vmath.hpp"
 
 namespace v2c {
 
@@ -365,10 +366,71 @@ __device__ __forceinline__ double clamp_visc(const KArgs &a, int t, double v)
     return v;
 }
 
+// The power-law part of ductilevisc (density_ductilevisc_calc.c:157-203):
+// 1/eta of the quasi-Newton's last iterate, its iteration count in *iters.
+// Fast: the loop's divisions, logarithms, exponentials and powers through
+// quot, log_pos, exp_red and pow_pos (v2c_vmath.hpp, within about 2 ulp); otherwise IEEE division and
+// the library log and pow.
+template <bool Fast>
+__device__ __forceinline__ double ductile_iterate(const KArgs &a, int t, double sigin, double e1, int *iters)
+{
+    const DevMat *M = a.mat;
+    auto dv = [](double p, double q) { return Fast ? quot(p, q) : p / q; };
+    auto ln = [](double x) { return Fast ? log_pos(x) : log(x); };
+    auto ex = [](double x) { return Fast ? exp_red(x) : exp(x); };
+    auto pw = [](double x, double y) { return Fast ? pow_pos(x, y) : pow(x, y); };
+    const double n = M->mm[t];
+    const double k1 = M->k1[t];
+    double eta_disl = 0.5 * sigin * M->nu[t] * e1 / pw(sigin, n);
+    double eta_new = eta_disl;
+    double eta_diff = 0.5 * k1 * e1;
+    double coeff = pw(eta_disl, M->rn[t]);
+    double gdt = M->gg[t] * a.timestep;
+    double mu_prev = 0, mu;
+    // Loop-invariant pieces of the iteration, and two exact rewrites:
+    // X^(-1/n) = X^((n-1)/n) / X, and eta_d^(2/(eta_d+eta)) evaluated as
+    // exp(2/(eta_d+eta) * ln eta_d) with ln eta_d hoisted.  The stop test
+    // |log10(mu_k/mu_{k-1})| < 1e-6 is applied as the equivalent bounds on
+    // the ratio (10^-1e-6, 10^1e-6).
+    const double ex1 = M->ex1[t], c1 = (n - 1) / (n * eta_diff);
+    const double inv_ed = 1.0 / eta_diff, ln_ed = ln(eta_diff);
+    const double r_lo = 0.999997697417558, r_hi = 1.000002302587744;
+    int k = 0;
+    for (;;) {
+        if (k >= 50) break;
+        k++;
+        const double eta_old = eta_new;
+        mu = dv(1.0, inv_ed + dv(1.0, eta_new));
+        // (eta_diff + eta_old) / eta_diff as the exact division by the
+        // correctly rounded reciprocal already at hand
+        const double pref = div_rcp(eta_diff + eta_old, eta_diff, inv_ed);
+        const double gm = gdt + mu;
+        const double X = pref * gm;
+        const double f1 = ex(ex1 * ln(X));
+        const double f2 = eta_old - coeff * f1;
+        // eta_d^(2/(eta_d+eta)): exactly 1.0 once the exponent's
+        // magnitude is below 2^-54, which holds for any geological eta_d
+        const double sum = eta_diff + eta_old;
+        const double t3 = fabs(ln_ed) < 0x1p-56 * sum ? 1.0 : exp(2 / sum * ln_ed);
+        const double df = 1 - c1 * coeff * dv(f1, X) * (gm + t3);
+        eta_new = eta_old - dv(f2, df);
+        if (k > 3) {
+            const double r = dv(mu, mu_prev);
+            if (r > r_lo && r < r_hi) break;
+        }
+        mu_prev = mu;
+    }
+    *iters = k;
+    return 1 / eta_new;
+}
+
 // ductilevisc (density_ductilevisc_calc.c:29-223): sticky air, Newtonian, or
 // diffusion + dislocation creep with the reference's quasi-Newton iteration
 // (:157-203, stop on |log10(mu_k/mu_{k-1})| < 1e-6 after >3 iterations, at
-// most 50).  The uninitialised branch (:55-65) returns NaN.
+// most 50).  The uninitialised branch (:55-65) returns NaN.  The iteration
+// runs in the fast arithmetic; an iterate that turns NaN there (operands
+// outside the fast forms' range) is recomputed with IEEE division and the
+// library functions.
 __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy, double sxx,
                                               double mtk, double mpb, int *iters)
 {
@@ -384,7 +446,6 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
     }
     if (M->dh[t] == 0 && M->dv[t] == 0 && (M->ss[t] == 0 || M->mm[t] == 1.0)) return M->nu[t];
     if (!power_law_type(M, t)) return __builtin_nan("");
-    const double n = M->mm[t];
     // pow(s, 0.5) of a non-negative s is sqrt(s) up to pow's last-ulp error
     double sigin = sqrt(sxx * sxx + sxy * sxy);
     double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * mtk);
@@ -394,47 +455,8 @@ __device__ __forceinline__ double ductilevisc(const KArgs &a, int t, double sxy,
     double inv_newt = 1.0 / (0.5 * k1 * e1);
     double inv_powl = 0;
     if (sigin > 0) {
-        double eta_disl = 0.5 * sigin * M->nu[t] * e1 / pow(sigin, n);
-        double eta_new = eta_disl;
-        double eta_diff = 0.5 * k1 * e1;
-        double coeff = pow(eta_disl, M->rn[t]);
-        double gdt = M->gg[t] * a.timestep;
-        double mu_prev = 0, mu;
-        // Loop-invariant pieces of the iteration, and two exact rewrites:
-        // X^(-1/n) = X^((n-1)/n) / X, and eta_d^(2/(eta_d+eta)) evaluated as
-        // exp(2/(eta_d+eta) * ln eta_d) with ln eta_d hoisted.  The stop test
-        // |log10(mu_k/mu_{k-1})| < 1e-6 is applied as the equivalent bounds on
-        // the ratio (10^-1e-6, 10^1e-6).
-        const double ex1 = M->ex1[t], c1 = (n - 1) / (n * eta_diff);
-        const double inv_ed = 1.0 / eta_diff, ln_ed = log(eta_diff);
-        const double r_lo = 0.999997697417558, r_hi = 1.000002302587744;
-        int k = 0;
-        for (;;) {
-            if (k >= 50) break;
-            k++;
-            const double eta_old = eta_new;
-            mu = 1.0 / (inv_ed + 1.0 / eta_new);
-            // (eta_diff + eta_old) / eta_diff as the exact division by the
-            // correctly rounded reciprocal already at hand
-            const double pref = div_rcp(eta_diff + eta_old, eta_diff, inv_ed);
-            const double gm = gdt + mu;
-            const double X = pref * gm;
-            const double f1 = exp(ex1 * log(X));
-            const double f2 = eta_old - coeff * f1;
-            // eta_d^(2/(eta_d+eta)): exactly 1.0 once the exponent's
-            // magnitude is below 2^-54, which holds for any geological eta_d
-            const double sum = eta_diff + eta_old;
-            const double t3 = fabs(ln_ed) < 0x1p-56 * sum ? 1.0 : exp(2 / sum * ln_ed);
-            const double df = 1 - c1 * coeff * (f1 / X) * (gm + t3);
-            eta_new = eta_old - f2 / df;
-            if (k > 3) {
-                const double r = mu / mu_prev;
-                if (r > r_lo && r < r_hi) break;
-            }
-            mu_prev = mu;
-        }
-        *iters = k;
-        inv_powl = 1 / eta_new;
+        inv_powl = ductile_iterate<true>(a, t, sigin, e1, iters);
+        if (__builtin_expect(__builtin_isnan(inv_powl), 0)) inv_powl = ductile_iterate<false>(a, t, sigin, e1, iters);
     }
     return clamp_visc(a, t, 1.0 / (inv_newt + inv_powl));
 }
@@ -715,10 +737,12 @@ __device__ __forceinline__ void rk_eval(const KArgs &a, const RkCache &c, double
 // is evaluated at both starts and Newton starts from the smaller one; a guess
 // whose residual grows after its first step falls back to eta_disl0
 // (*fallback = 1).  *iters counts Newton updates, *evals residual evaluations.
-__device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double sxy, double sxx,
-                                                 double mtk, double mpb, double guess, double tol,
-                                                 int *iters, int *evals, int *fallback, double *root)
+template <bool Fast>
+__device__ __forceinline__ double ductile_newton_impl(const KArgs &a, int t, double sxy, double sxx,
+                                                      double mtk, double mpb, double guess, double tol,
+                                                      int *iters, int *evals, int *fallback, double *root)
 {
+    auto pw = [](double x, double y) { return Fast ? pow_pos(x, y) : pow(x, y); };
     const DevMat *M = a.mat;
     *iters = 0;
     *evals = 0;
@@ -736,13 +760,13 @@ __device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double s
     const double k1 = M->k1[t];
     const double eta_diff = 0.5 * k1 * e1;
     if (!(sigin > 0)) return clamp_visc(a, t, eta_diff);
-    const double eta_disl0 = 0.5 * sigin * M->nu[t] * e1 / pow(sigin, n);
-    const double coeff = pow(eta_disl0, M->rn[t]);
+    const double eta_disl0 = 0.5 * sigin * M->nu[t] * e1 / pw(sigin, n);
+    const double coeff = pw(eta_disl0, M->rn[t]);
     const double gdt = M->gg[t] * a.timestep;
     const double p1 = M->ex1[t], slope = gdt / eta_diff + 1;
     auto resid = [&](double eta, double &f, double &X, double &Xp) {
         X = (eta_diff + eta) / eta_diff * gdt + eta;
-        Xp = pow(X, p1);
+        Xp = pw(X, p1);
         f = eta - coeff * Xp;
         return fabs(f) / eta;
     };
@@ -780,6 +804,17 @@ __device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double s
     *evals = ev;
     *root = eta;
     return clamp_visc(a, t, 1.0 / (1.0 / eta_diff + 1.0 / eta));
+}
+
+// The solve in the fast arithmetic (pow_pos, within a few 1e-15); a result
+// that turns NaN there is recomputed with the library pow.
+__device__ __forceinline__ double ductile_newton(const KArgs &a, int t, double sxy, double sxx,
+                                                 double mtk, double mpb, double guess, double tol,
+                                                 int *iters, int *evals, int *fallback, double *root)
+{
+    const double v = ductile_newton_impl<true>(a, t, sxy, sxx, mtk, mpb, guess, tol, iters, evals, fallback, root);
+    if (__builtin_expect(!__builtin_isnan(v), 1)) return v;
+    return ductile_newton_impl<false>(a, t, sxy, sxx, mtk, mpb, guess, tol, iters, evals, fallback, root);
 }
 
 // Per-marker RSF slip rate with radiation damping at the aged state
