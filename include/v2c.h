@@ -519,6 +519,11 @@ enum v2c_kernel_id {
     V2C_K_COUNT
 };
 int v2c_enable_timing(v2c_ctx *ctx, int on);
+/* Measurement aid (no reference counterpart): while timing is on, record
+ * events only around the kernels whose bit (1u << V2C_K_*) is set (default:
+ * all).  An event pair costs a few microseconds of stream time, so a timed
+ * run that wants the whole step's duration undisturbed times one kernel. */
+int v2c_timing_select(v2c_ctx *ctx, unsigned int kernel_mask);
 /* Measurement aid (no reference counterpart): time a plain device copy of
  * `bytes` (reads and writes `bytes` each), used to calibrate HBM counters. */
 int v2c_copy_calibration(v2c_ctx *ctx, long bytes, double *ms);

@@ -244,6 +244,7 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                                                           P(C.c_double), P(C.c_double), P(C.c_double),
                                                           P(C.c_double), P(C.c_double), P(C.c_int)]),
                        ("v2c_enable_timing", [C.c_int]),
+                       ("v2c_timing_select", [C.c_uint]),
                        ("v2c_set_friction_fields", [C.c_int]),
                        ("v2c_copy_calibration", [C.c_long, P(C.c_double)]),
                        ("v2c_order_stats", [P(C.c_long), P(C.c_long), P(C.c_long)]),
@@ -447,6 +448,12 @@ class Context:
 
     def enable_timing(self, on=True):
         self._check(self.lib.v2c_enable_timing(self.h, int(on)), "enable_timing")
+
+    def timing_select(self, names=None):
+        """Time only the named kernels (KERNELS entries) while timing is on;
+        None restores all."""
+        mask = 0xffffffff if names is None else sum(1 << KERNELS.index(k) for k in names)
+        self._check(self.lib.v2c_timing_select(self.h, mask), "timing_select")
 
     def reset_timing(self):
         self._check(self.lib.v2c_reset_timing(self.h), "reset_timing")

@@ -54,6 +54,8 @@ struct v2c_ctx {
     uint32_t *d_key = nullptr, *d_skey = nullptr, *d_iota = nullptr, *d_order = nullptr, *d_cs = nullptr;
     double *d_nod = nullptr, *d_part = nullptr, *d_red = nullptr, *d_prop = nullptr;
     int *d_counters = nullptr;
+    int before_slot = 0;                // compute_before_solve's error-count slot (0 or BEFORE_SLOTS)
+    unsigned timing_mask = ~0u;         // kernels (1 << V2C_K_*) timed while timing is on
     // incremental re-ordering buffers
     uint32_t *d_order2 = nullptr, *d_skey2 = nullptr, *d_tile_count = nullptr, *d_tile_off = nullptr;
     unsigned long long *d_mask = nullptr;
@@ -127,7 +129,8 @@ static int launch(v2c_ctx *c, int kid, F &&fn, hipStream_t st = nullptr)
 {
     if (!st) st = c->stream;
     TimedLaunch tl{kid, nullptr, nullptr};
-    if (c->timing) {
+    const bool timed = c->timing && ((c->timing_mask >> kid) & 1u);
+    if (timed) {
         tl.start = take_event(c);
         tl.stop = take_event(c);
         if (tl.start) (void)hipEventRecord(tl.start, st);
@@ -137,7 +140,7 @@ static int launch(v2c_ctx *c, int kid, F &&fn, hipStream_t st = nullptr)
     else fn();
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return set_err(c, V2C_EHIP, "kernel %d launch failed: %s", kid, hipGetErrorString(e));
-    if (c->timing && tl.start && tl.stop) {
+    if (timed && tl.start && tl.stop) {
         (void)hipEventRecord(tl.stop, st);
         c->pending.push_back(tl);
     }
@@ -207,7 +210,7 @@ static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 
 static int cell_starts(v2c_ctx *c)
 {
     return launch(c, V2C_K_SORT, [&] {
-        hipLaunchKernelGGL(k_sub_start, dim3(blocks_for(c->n + 1, 256)), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(k_sub_start, dim3(blocks_for(c->n + 1, ORD_TILE)), dim3(ORD_BLOCK), 0, c->stream,
                            c->d_skey, c->n, 4 * c->ncell, c->d_cs);
     });
 }
@@ -287,8 +290,8 @@ static int order_finish(v2c_ctx *c)
         hipLaunchKernelGGL(k_order_stayers, dim3((unsigned)ntiles), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
                            c->d_skey, n, c->d_mask, c->d_tile_off, c->d_k64_out, nmov, c->d_order2, c->d_skey2);
         hipLaunchKernelGGL(k_order_place_movers, dim3(blocks_for(nmov, 256)), dim3(256), 0, c->stream,
-                           c->d_k64_out, nmov, c->d_order, c->d_skey, n, c->d_mask, c->d_tile_off,
-                           c->d_order2, c->d_skey2);
+                           c->d_k64_out, nmov, c->d_order, c->d_skey, n, c->d_cs, 4 * c->ncell, c->d_mask,
+                           c->d_tile_off, c->d_order2, c->d_skey2);
         return hipSuccess;
     });
     if (rc) return rc;
@@ -864,7 +867,8 @@ static int n2m_impl(v2c_ctx *c, bool advect)
         if (rc) return rc;
     }
     KArgs a = make_args(c);
-    if (advect) HIPCHK(c, hipMemsetAsync(c->d_counters + 1, 0, sizeof(int), c->stream));
+    // with antidiffusion the scatter clears the update's counters itself
+    if (advect && !anti) HIPCHK(c, hipMemsetAsync(c->d_counters + 1, 0, sizeof(int), c->stream));
     // The antidiffusion scatter reads marker positions and types and the
     // solve's nodal fields; its node pass writes the sums and the *ee fields.
     // The marker update reads neither and, without advection, writes no
@@ -885,16 +889,11 @@ static int n2m_impl(v2c_ctx *c, bool advect)
         }
         return code;
     };
-    if (c->prm.n2m_phantom && anti) {
-        rc = launch(c, V2C_K_N2M_NODES, [&] {
-            hipLaunchKernelGGL(k_n2m_phantom, dim3(1), dim3(1), 0, st, a);
-        }, st);
-        if (rc) return join_side(rc);
-    }
     if (anti) {
         const int nwg = (int)((c->ncell + CPW - 1) / CPW);
+        const int pre = (c->prm.n2m_phantom ? N2M_PRE_PHANTOM : 0) | (advect ? N2M_PRE_CLEAR : 0);
         rc = launch(c, V2C_K_N2M_CELLS, [&] {
-            hipLaunchKernelGGL(k_n2m_scatter, dim3(nwg), dim3(WAVE), 0, st, a, nwg);
+            hipLaunchKernelGGL(k_n2m_scatter, dim3(nwg), dim3(WAVE), 0, st, a, nwg, pre);
         }, st);
         if (rc) return join_side(rc);
         rc = launch(c, V2C_K_N2M_NODES, [&] {
@@ -914,11 +913,8 @@ static int n2m_impl(v2c_ctx *c, bool advect)
         // With antidiffusion and advection the fast form takes every marker
         // whose RK stages stay in one stencil configuration and lists the
         // others for the general form.
+        // (the scatter cleared its deferred-marker count)
         const bool fast = advect && anti;
-        if (fast) {
-            const hipError_t e = hipMemsetAsync(c->d_counters + 2, 0, sizeof(int), c->stream);
-            if (e != hipSuccess) rc = set_err(c, V2C_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
-        }
         if (!rc)
             rc = launch(c, V2C_K_N2M_UPDATE, [&] {
                 if (fast) {
@@ -980,16 +976,19 @@ int v2c_compute_before_solve(v2c_ctx *c, double dt, int rk, long *nerr)
     if (!c) return V2C_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     KArgs a = make_args(c);
-    HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(int), c->stream));
+    // the error count alternates between two counter slots; the kernel clears
+    // the other one for the next call
+    const int slot = c->before_slot;
+    c->before_slot ^= BEFORE_SLOTS;
     int rc = launch(c, V2C_K_BEFORE, [&] {
-        hipLaunchKernelGGL(k_before, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, c->stream, a, dt, rk);
+        hipLaunchKernelGGL(k_before, dim3(blocks_for(c->nodenum, 256)), dim3(256), 0, c->stream, a, dt, rk, slot);
     });
     if (rc) return rc;
     rc = finish(c);
     if (rc) return rc;
     if (nerr) {
         int v = 0;
-        rc = read_counter(c, 0, &v);
+        rc = read_counter(c, slot, &v);
         if (rc) return rc;
         *nerr = v;
     }
@@ -1471,6 +1470,13 @@ int v2c_enable_timing(v2c_ctx *c, int on)
 {
     if (!c) return V2C_EINVAL;
     c->timing = on ? 1 : 0;
+    return 0;
+}
+
+int v2c_timing_select(v2c_ctx *c, unsigned int kernel_mask)
+{
+    if (!c) return V2C_EINVAL;
+    c->timing_mask = kernel_mask;
     return 0;
 }
 

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(ENS_BLOCK) void k_ens_phis_bc(const KArgs *__restri
 __global__ __launch_bounds__(ENS_BLOCK) void k_ens_before(const KArgs *__restrict__ A,
                                                           const EnsTile *__restrict__ tiles)
 {
-    ens_item(A, tiles, [&](const KArgs &a, long m3) { before_one(a, m3, a.timestep, 0); });
+    ens_item(A, tiles, [&](const KArgs &a, long m3) { before_one(a, m3, a.timestep, 0, 0); });
 }
 
 // Node entries are whole RED_BLOCK chunks of one model (they start at
@@ -389,8 +389,8 @@ __global__ void k_eo_place(const EnsOrd *__restrict__ O, const uint64_t *__restr
     const uint64_t v = mov[r];
     const int k = (int)(v >> shift);
     const EnsOrd &o = O[k];
-    order_place_one(v & ((1ull << shift) - 1ull), r - base[k], o.order, o.skey, o.n, o.mask, o.tile_off, o.order2,
-                    o.skey2);
+    order_place_one(v & ((1ull << shift) - 1ull), r - base[k], o.order, o.skey, o.n, o.sub_start, o.nsub, o.mask,
+                    o.tile_off, o.order2, o.skey2);
 }
 
 // Sub-cell starts of the new order over the marker entries (sorted

@@ -77,12 +77,6 @@ __device__ __forceinline__ void sub_start_at(const uint32_t *__restrict__ skey, 
     for (long k = prev + 1; k <= cur && k <= nsub; k++) ss[k] = (uint32_t)i;
 }
 
-__global__ void k_sub_start(const uint32_t *__restrict__ skey, long n, long nsub, uint32_t *ss)
-{
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= n) sub_start_at(skey, n, nsub, ss, i);
-}
-
 // ---------------------------------------------------------------------------
 // Incremental re-ordering after advection.  The order is always the (key, id)
 // lexicographic order (a stable radix sort of keys over ids produces exactly
@@ -106,6 +100,37 @@ constexpr int ORD_MASKS = ORD_TILE / WAVE;        // masks per tile
 
 __device__ __forceinline__ uint64_t kid(uint32_t k, uint32_t id) { return ((uint64_t)k << 32) | id; }
 
+// Sub-cell starts over positions [t0, t0 + ORD_TILE) (and n, in the last
+// tile), ORD_ITEMS positions per thread with every key load issued before
+// the first store: a workgroup's waves stay resident for a few memory
+// latencies rather than one per position.
+__global__ __launch_bounds__(ORD_BLOCK) void k_sub_start(const uint32_t *__restrict__ skey, long n, long nsub,
+                                                         uint32_t *ss)
+{
+    const long t0 = (long)blockIdx.x * ORD_TILE;
+    if (n == 0) {
+        for (long q = threadIdx.x; q <= nsub; q += ORD_BLOCK) ss[q] = 0u;
+        return;
+    }
+    // unconditional loads at clamped positions, the tile's out-of-range
+    // cases resolved after all of them are in flight
+    uint32_t rc[ORD_ITEMS], rp[ORD_ITEMS];
+#pragma unroll
+    for (int k = 0; k < ORD_ITEMS; k++) {
+        const long i = t0 + (long)k * ORD_BLOCK + threadIdx.x;
+        rc[k] = skey[i < n ? i : n - 1];
+        rp[k] = skey[i <= 0 ? 0 : (i <= n ? i - 1 : n - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < ORD_ITEMS; k++) {
+        const long i = t0 + (long)k * ORD_BLOCK + threadIdx.x;
+        if (i > n) continue;
+        const long cur = i < n ? (long)rc[k] : nsub;
+        const long prev = i > 0 ? (long)rp[k] : -1;
+        for (long q = prev + 1; q <= cur && q <= nsub; q++) ss[q] = (uint32_t)i;
+    }
+}
+
 // Pass 1: mover masks and mover count of one tile.
 __device__ __forceinline__ void order_flags_tile(const uint32_t *__restrict__ order, const uint32_t *__restrict__ key,
                                                  const uint32_t *__restrict__ skey, long n,
@@ -115,10 +140,22 @@ __device__ __forceinline__ void order_flags_tile(const uint32_t *__restrict__ or
     __shared__ uint32_t wsum[ORD_BLOCK / WAVE];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const long t0 = tile * ORD_TILE;
-    uint32_t cnt = 0;
+    // all loads of the tile first (ids, then their keys), so a wave waits
+    // two memory latencies rather than two per position
+    uint32_t id[ORD_ITEMS], sk[ORD_ITEMS], kk[ORD_ITEMS];
+#pragma unroll
     for (int k = 0; k < ORD_ITEMS; k++) {
         const long i = t0 + (long)k * ORD_BLOCK + threadIdx.x;
-        const bool mv = i < n && key[order[i]] != skey[i];
+        id[k] = i < n ? order[i] : 0u;
+        sk[k] = i < n ? skey[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < ORD_ITEMS; k++) kk[k] = t0 + (long)k * ORD_BLOCK + threadIdx.x < n ? key[id[k]] : 0u;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < ORD_ITEMS; k++) {
+        const long i = t0 + (long)k * ORD_BLOCK + threadIdx.x;
+        const bool mv = i < n && kk[k] != sk[k];
         const unsigned long long b = __ballot(mv);
         if (lane == 0 && t0 + (long)k * ORD_BLOCK + wv * WAVE < n) mask[(t0 + (long)k * ORD_BLOCK) / WAVE + wv] = b;
         cnt += (uint32_t)__popcll(b);
@@ -234,9 +271,24 @@ __device__ __forceinline__ long lower_bound64(const uint64_t *__restrict__ a, lo
     return lo;
 }
 
-// Pass 5: the tile's stayers to their new positions.  The movers that can
-// precede any stayer of the tile after the first ones are confined to
-// [lo, hi), found once per tile; each stayer counts within that window.
+// Pass 5: the tile's stayers to their new positions.  The tile's masks and
+// their prefix counts go to LDS once; the movers that can precede any of the
+// tile's stayers are confined to mov[lo, hi), found once per tile and staged
+// in LDS when there are at most ORD_WIN of them; every id and key of the
+// tile is loaded before the first placement.
+constexpr int ORD_WIN = 256;
+static_assert(ORD_MASKS == WAVE, "one wave scans a tile's masks");
+
+__device__ __forceinline__ long lower_bound_lds(const uint64_t *a, long m, uint64_t c)
+{
+    long lo = 0, hi = m;
+    while (lo < hi) {
+        const long mid = (lo + hi) >> 1;
+        if (a[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ void order_stayers_tile(const uint32_t *__restrict__ order,
                                                    const uint32_t *__restrict__ skey, long n,
                                                    const unsigned long long *__restrict__ mask,
@@ -246,7 +298,9 @@ __device__ __forceinline__ void order_stayers_tile(const uint32_t *__restrict__ 
                                                    uint32_t *__restrict__ skey_out, long tile)
 {
     __shared__ long win[2];
-    __shared__ uint32_t wpre[ORD_BLOCK / WAVE + 1];
+    __shared__ unsigned long long msk[ORD_MASKS];
+    __shared__ uint32_t mpre[ORD_MASKS];
+    __shared__ uint64_t wmov[ORD_WIN];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const long t0 = tile * ORD_TILE;
     const long t1 = t0 + ORD_TILE < n ? t0 + ORD_TILE : n;
@@ -256,39 +310,44 @@ __device__ __forceinline__ void order_stayers_tile(const uint32_t *__restrict__ 
         win[1] = lower_bound64(mov, win[0], nmov, (kid(skey[t1 - 1], 0xffffffffu) | hib) + 1ull);
         if (skey[t1 - 1] == 0xffffffffu) win[1] = nmov;
     }
-    uint32_t moved = off[tile];                     // movers before the current chunk
+    if (wv == 0) {                                  // the tile's masks, exclusive popcount prefix
+        const unsigned long long b = t0 + (long)lane * WAVE < n ? mask[t0 / WAVE + lane] : 0ull;
+        const uint32_t own = (uint32_t)__popcll(b);
+        uint32_t c = own;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const uint32_t v = __shfl_up(c, d);
+            if (lane >= d) c += v;
+        }
+        msk[lane] = b;
+        mpre[lane] = c - own;
+    }
+    uint32_t id[ORD_ITEMS], sk[ORD_ITEMS];
+#pragma unroll
+    for (int k = 0; k < ORD_ITEMS; k++) {
+        const long i = t0 + (long)k * ORD_BLOCK + threadIdx.x;
+        id[k] = i < n ? order[i] : 0u;
+        sk[k] = i < n ? skey[i] : 0u;
+    }
     __syncthreads();
     const long lo = win[0], hi = win[1];
+    const bool staged = hi - lo <= ORD_WIN;
+    if (staged)
+        for (long j = lo + threadIdx.x; j < hi; j += ORD_BLOCK) wmov[j - lo] = mov[j];
+    __syncthreads();
+    const uint32_t moved = off[tile];               // movers before the tile
+#pragma unroll
     for (int k = 0; k < ORD_ITEMS; k++) {
-        const long c0 = t0 + (long)k * ORD_BLOCK;
-        if (c0 >= n) break;
-        const long i = c0 + threadIdx.x;
-        const unsigned long long b = c0 + wv * WAVE < n ? mask[c0 / WAVE + wv] : 0ull;
-        if (lane == 0) wpre[wv] = (uint32_t)__popcll(b);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t r = 0;
-            for (int w = 0; w < ORD_BLOCK / WAVE; w++) { const uint32_t t = wpre[w]; wpre[w] = r; r += t; }
-            wpre[ORD_BLOCK / WAVE] = r;
-        }
-        __syncthreads();
+        const long i = t0 + (long)k * ORD_BLOCK + threadIdx.x;
+        const int q = k * (ORD_BLOCK / WAVE) + wv;
+        const unsigned long long b = msk[q];
         if (i < n && !((b >> lane) & 1ull)) {
-            const uint32_t id = order[i], k32 = skey[i];
-            const uint64_t c = kid(k32, id) | hib;
-            const uint32_t mb = moved + wpre[wv] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
-            long before;
-            if (hi - lo <= 8) {
-                before = lo;
-                for (long j = lo; j < hi; j++) before += mov[j] < c ? 1 : 0;
-            } else {
-                before = lower_bound64(mov, lo, hi, c);
-            }
+            const uint64_t c = kid(sk[k], id[k]) | hib;
+            const uint32_t mb = moved + mpre[q] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+            const long before = staged ? lo + lower_bound_lds(wmov, hi - lo, c) : lower_bound64(mov, lo, hi, c);
             const long pos = (i - (long)mb) + before;
-            order_out[pos] = id;
-            skey_out[pos] = k32;
+            order_out[pos] = id[k];
+            skey_out[pos] = sk[k];
         }
-        moved += wpre[ORD_BLOCK / WAVE];
-        __syncthreads();
     }
 }
 
@@ -303,17 +362,22 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_stayers(const uint32_t *__r
     order_stayers_tile(order, skey, n, mask, off, mov, nmov, 0ull, order_out, skey_out, blockIdx.x);
 }
 
-// Pass 6: mover r (value c, high bits removed by `low`) to its new position.
-// Stayers before a mover = entries of the old order below it (binary search;
-// the old order is sorted by the keys it was built with, and stayers kept
-// theirs) minus the movers among them.
+// Pass 6: mover r (value c, high bits removed) to its new position.  Stayers
+// before a mover = entries of the old order below it minus the movers among
+// them.  The old order is sorted by the keys it was built with (stayers kept
+// theirs), and ss holds its sub-cell starts: the entries below c are those of
+// the sub-cells before c's and those of c's own sub-cell with a smaller id,
+// found by a search confined to that sub-cell.
 __device__ __forceinline__ void order_place_one(uint64_t c, long r, const uint32_t *__restrict__ order,
                                                 const uint32_t *__restrict__ skey, long n,
+                                                const uint32_t *__restrict__ ss, long nsub,
                                                 const unsigned long long *__restrict__ mask,
                                                 const uint32_t *__restrict__ off, uint32_t *__restrict__ order_out,
                                                 uint32_t *__restrict__ skey_out)
 {
-    long lo = 0, hi = n;
+    const uint32_t kc = (uint32_t)(c >> 32);
+    long lo = kc < (uint64_t)nsub ? (long)ss[kc] : (long)ss[nsub];
+    long hi = kc < (uint64_t)nsub ? (long)ss[kc + 1] : n;
     while (lo < hi) {
         const long mid = (lo + hi) >> 1;
         if (kid(skey[mid], order[mid]) < c) lo = mid + 1; else hi = mid;
@@ -326,12 +390,13 @@ __device__ __forceinline__ void order_place_one(uint64_t c, long r, const uint32
 
 __global__ void k_order_place_movers(const uint64_t *__restrict__ mov, long nmov,
                                      const uint32_t *__restrict__ order, const uint32_t *__restrict__ skey, long n,
+                                     const uint32_t *__restrict__ ss, long nsub,
                                      const unsigned long long *__restrict__ mask,
                                      const uint32_t *__restrict__ off, uint32_t *__restrict__ order_out,
                                      uint32_t *__restrict__ skey_out)
 {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nmov) order_place_one(mov[r], r, order, skey, n, mask, off, order_out, skey_out);
+    if (r < nmov) order_place_one(mov[r], r, order, skey, n, ss, nsub, mask, off, order_out, skey_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1125,12 +1190,6 @@ __device__ __forceinline__ void n2m_scatter_group(const KArgs &a, long c0, long 
     zero_empty<NRec>(a, c0, c1, lane);
 }
 
-__global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg)
-{
-    __shared__ N2mLds L;
-    const long c0 = (long)xcd_remap(blockIdx.x, nwg) * CPW;
-    n2m_scatter_group(a, c0, c0 + CPW < a.ncell ? c0 + CPW : a.ncell, threadIdx.x, L);
-}
 
 // nodes2markers marker update (interpolation.c:943-1011), fused with
 // moveandrotatemarkers (moveandrotatemarkers.c:1-138) when ADVECT, in physical
@@ -1318,6 +1377,24 @@ __global__ void k_n2m_phantom(KArgs a)
     if (threadIdx.x == 0 && blockIdx.x == 0 && a.p.antidiffinter == 1) n2m_phantom_one(a);
 }
 
+// The antidiffusion scatter.  Its first workgroup also does what would
+// otherwise be separate one-thread launches before it: with N2M_PRE_PHANTOM
+// the phantom marker's contribution (which goes straight to the nodal sums
+// the node pass completes, not to the records this kernel writes), with
+// N2M_PRE_CLEAR the clearing of the update's counters (unsupported markers,
+// deferred markers), which no kernel before the update reads.
+enum { N2M_PRE_PHANTOM = 1, N2M_PRE_CLEAR = 2 };
+__global__ __launch_bounds__(WAVE) void k_n2m_scatter(KArgs a, int nwg, int pre)
+{
+    __shared__ N2mLds L;
+    if (pre && blockIdx.x == 0 && threadIdx.x == 0) {
+        if (pre & N2M_PRE_PHANTOM) n2m_phantom_one(a);
+        if (pre & N2M_PRE_CLEAR) { a.counters[1] = 0; a.counters[2] = 0; }
+    }
+    const long c0 = (long)xcd_remap(blockIdx.x, nwg) * CPW;
+    n2m_scatter_group(a, c0, c0 + CPW < a.ncell ? c0 + CPW : a.ncell, threadIdx.x, L);
+}
+
 // nodes2markers node pass: add the cell partials to the persistent sums and
 // normalise (interpolation.c:1014-1046).
 __device__ __forceinline__ void n2m_nodes_one(const KArgs &a, long m3)
@@ -1419,7 +1496,7 @@ __global__ __launch_bounds__(256) void k_advect(KArgs a)
 // ---------------------------------------------------------------------------
 // compute_before_solve / calc_vpvisc (plastic.c:2-56, calc_RSF.c:33-169)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void before_one(const KArgs &a, long m3, double dt, int rk)
+__device__ __forceinline__ void before_one(const KArgs &a, long m3, double dt, int rk, int slot)
 {
     const int ynumy = a.g.ynumy;
     const int m2 = (int)(m3 % ynumy), m1 = (int)(m3 / ynumy);
@@ -1484,13 +1561,18 @@ __device__ __forceinline__ void before_one(const KArgs &a, long m3, double dt, i
     }
     NOD(a, V2C_N_nu)[m3] = nu;
     if (isnan(nu) || nu <= 0) err = 1;
-    if (err) atomicAdd(&a.counters[0], 1);
+    if (err) atomicAdd(&a.counters[slot], 1);
 }
 
-__global__ __launch_bounds__(256) void k_before(KArgs a, double dt, int rk)
+// The error count goes to counters[slot]; the other slot of the pair
+// (slot ^ BEFORE_SLOTS), the next call's, is cleared here, so no separate
+// clearing launch precedes the kernel.
+constexpr int BEFORE_SLOTS = 4;
+__global__ __launch_bounds__(256) void k_before(KArgs a, double dt, int rk, int slot)
 {
     const long m3 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m3 < a.nodenum) before_one(a, m3, dt, rk);
+    if (m3 == 0) a.counters[slot ^ BEFORE_SLOTS] = 0;
+    if (m3 < a.nodenum) before_one(a, m3, dt, rk, slot);
 }
 
 // compute_after_solve / calc_sliprate_after_solve (plastic.c:61-129,
