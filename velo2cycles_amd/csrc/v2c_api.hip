@@ -55,6 +55,7 @@ struct v2c_ctx {
     double *d_nod = nullptr, *d_part = nullptr, *d_red = nullptr, *d_prop = nullptr;
     int *d_counters = nullptr;
     int before_slot = 0;                // compute_before_solve's error-count slot (0 or BEFORE_SLOTS)
+    long ord_max = 512;                 // workgroup cap of the re-ordering passes (0: one per tile)
     unsigned timing_mask = ~0u;         // kernels (1 << V2C_K_*) timed while timing is on
     // incremental re-ordering buffers
     uint32_t *d_order2 = nullptr, *d_skey2 = nullptr, *d_tile_count = nullptr, *d_tile_off = nullptr;
@@ -207,10 +208,20 @@ static int finish(v2c_ctx *c)
 
 static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Workgroups of the re-ordering passes (each loops over its share of the
+// tiles): at most ord_max, so that beside the property pass they hold fewer
+// of the CUs' wave slots (production step: 3.08 ms with one workgroup per
+// tile, 3.05 with 256 or 512, 3.12 with 128 and 3.52 with 64, where the
+// re-ordering outlasts the property pass).
+static inline unsigned ord_grid(const v2c_ctx *c, long ntiles)
+{
+    return (unsigned)(c->ord_max > 0 && ntiles > c->ord_max ? c->ord_max : ntiles);
+}
+
 static int cell_starts(v2c_ctx *c)
 {
     return launch(c, V2C_K_SORT, [&] {
-        hipLaunchKernelGGL(k_sub_start, dim3(blocks_for(c->n + 1, ORD_TILE)), dim3(ORD_BLOCK), 0, c->stream,
+        hipLaunchKernelGGL(k_sub_start, dim3(ord_grid(c, blocks_for(c->n + 1, ORD_TILE))), dim3(ORD_BLOCK), 0, c->stream,
                            c->d_skey, c->n, 4 * c->ncell, c->d_cs);
     });
 }
@@ -253,7 +264,7 @@ static int order_begin(v2c_ctx *c, bool *pending)
     const long n = c->n;
     const long ntiles = (n + ORD_TILE - 1) / ORD_TILE;
     int rc = launch(c, V2C_K_SORT, [&] {
-        hipLaunchKernelGGL(k_order_flags, dim3((unsigned)ntiles), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
+        hipLaunchKernelGGL(k_order_flags, dim3(ord_grid(c, ntiles)), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
                            c->d_key, c->d_skey, n, c->d_mask, c->d_tile_count);
         hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, c->stream, c->d_tile_count, ntiles,
                            c->d_tile_off, c->d_nsel);
@@ -282,12 +293,12 @@ static int order_finish(v2c_ctx *c)
         return 0;
     }
     rc = launch(c, V2C_K_SORT, [&] {
-        hipLaunchKernelGGL(k_order_movers, dim3((unsigned)ntiles), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
+        hipLaunchKernelGGL(k_order_movers, dim3(ord_grid(c, ntiles)), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
                            c->d_key, n, c->d_mask, c->d_tile_off, c->d_k64_in);
         hipError_t e = hipcub::DeviceRadixSort::SortKeys(c->d_sort_tmp, c->sort_tmp_bytes, c->d_k64_in,
                                                          c->d_k64_out, (int)nmov, 0, 32 + c->key_bits, c->stream);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_order_stayers, dim3((unsigned)ntiles), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
+        hipLaunchKernelGGL(k_order_stayers, dim3(ord_grid(c, ntiles)), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
                            c->d_skey, n, c->d_mask, c->d_tile_off, c->d_k64_out, nmov, c->d_order2, c->d_skey2);
         hipLaunchKernelGGL(k_order_place_movers, dim3(blocks_for(nmov, 256)), dim3(256), 0, c->stream,
                            c->d_k64_out, nmov, c->d_order, c->d_skey, n, c->d_cs, 4 * c->ncell, c->d_mask,

@@ -107,11 +107,14 @@ __device__ __forceinline__ uint64_t kid(uint32_t k, uint32_t id) { return ((uint
 __global__ __launch_bounds__(ORD_BLOCK) void k_sub_start(const uint32_t *__restrict__ skey, long n, long nsub,
                                                          uint32_t *ss)
 {
-    const long t0 = (long)blockIdx.x * ORD_TILE;
     if (n == 0) {
-        for (long q = threadIdx.x; q <= nsub; q += ORD_BLOCK) ss[q] = 0u;
+        if (blockIdx.x == 0)
+            for (long q = threadIdx.x; q <= nsub; q += ORD_BLOCK) ss[q] = 0u;
         return;
     }
+    const long ntiles = (n + 1 + ORD_TILE - 1) / ORD_TILE;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long t0 = tile * ORD_TILE;
     // unconditional loads at clamped positions, the tile's out-of-range
     // cases resolved after all of them are in flight
     uint32_t rc[ORD_ITEMS], rp[ORD_ITEMS];
@@ -128,6 +131,7 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_sub_start(const uint32_t *__restr
         const long cur = i < n ? (long)rc[k] : nsub;
         const long prev = i > 0 ? (long)rp[k] : -1;
         for (long q = prev + 1; q <= cur && q <= nsub; q++) ss[q] = (uint32_t)i;
+    }
     }
 }
 
@@ -175,7 +179,11 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_flags(const uint32_t *__res
                                                            unsigned long long *__restrict__ mask,
                                                            uint32_t *__restrict__ tile_count)
 {
-    order_flags_tile(order, key, skey, n, mask, tile_count, blockIdx.x);
+    const long ntiles = (n + ORD_TILE - 1) / ORD_TILE;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();
+        order_flags_tile(order, key, skey, n, mask, tile_count, tile);
+    }
 }
 
 // Pass 2: exclusive scan of the tile counts (one 1024-thread workgroup);
@@ -259,7 +267,11 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_movers(const uint32_t *__re
                                                             const uint32_t *__restrict__ off,
                                                             uint64_t *__restrict__ mov)
 {
-    order_movers_tile(order, key, n, mask, off, mov, 0ull, blockIdx.x);
+    const long ntiles = (n + ORD_TILE - 1) / ORD_TILE;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();
+        order_movers_tile(order, key, n, mask, off, mov, 0ull, tile);
+    }
 }
 
 __device__ __forceinline__ long lower_bound64(const uint64_t *__restrict__ a, long lo, long hi, uint64_t c)
@@ -359,7 +371,11 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_stayers(const uint32_t *__r
                                                              uint32_t *__restrict__ order_out,
                                                              uint32_t *__restrict__ skey_out)
 {
-    order_stayers_tile(order, skey, n, mask, off, mov, nmov, 0ull, order_out, skey_out, blockIdx.x);
+    const long ntiles = (n + ORD_TILE - 1) / ORD_TILE;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();
+        order_stayers_tile(order, skey, n, mask, off, mov, nmov, 0ull, order_out, skey_out, tile);
+    }
 }
 
 // Pass 6: mover r (value c, high bits removed) to its new position.  Stayers

@@ -9,6 +9,14 @@
         HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
         scaled by the factor that makes the plain copy kernel k_copy_calib (N bytes
         read and N bytes written) read exactly N bytes
+    python -m velo2cycles_amd.rocprof_summary requests --reads DB --write DB --copy-bytes N
+                                                        --workload NAME [--out JSON]
+        HBM bytes per launch from the L2-to-memory request counts by size:
+        reads 128 TCC_EA0_RDREQ_128B + 32 TCC_EA0_RDREQ_32B + 64 x the rest of
+        TCC_EA0_RDREQ, writes WRITE_SIZE; the plain copy kernel's figures give
+        the check factors (1.0 when every request is counted at its size).
+        (FETCH_SIZE counts a request as 128 B only through TCC_BUBBLE, which on
+        gfx950 misses the 128-byte reads: a streaming copy reads as half.)
     python -m velo2cycles_amd.rocprof_summary valu DB --workload NAME [--out JSON]
         VALU and f64-transcendental wavefront instructions per launch
         (SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F64)
@@ -93,6 +101,39 @@ def traffic(fetch_db: str, write_db: str, copy_bytes: int, workload: str) -> dic
     return {workload: out}
 
 
+def _req_bytes(c: dict) -> float:
+    r, r32, r128 = c["TCC_EA0_RDREQ_sum"], c["TCC_EA0_RDREQ_32B_sum"], c["TCC_EA0_RDREQ_128B_sum"]
+    return 128.0 * r128 + 32.0 * r32 + 64.0 * (r - r128 - r32)
+
+
+def requests(reads_db: str, write_db: str, copy_bytes: int, workload: str) -> dict:
+    f, w = pmc_table(reads_db), pmc_table(write_db)
+    cf = copy_bytes / _req_bytes(f["k_copy_calib"])
+    cw = copy_bytes / (w["k_copy_calib"]["WRITE_SIZE"] * 1024.0)
+    cb = f["k_copy_calib"]
+    fetch_size_formula = (cb["TCC_BUBBLE_sum"] * 128.0 + (cb["TCC_EA0_RDREQ_sum"] - cb["TCC_BUBBLE_sum"]
+                          - cb["TCC_EA0_RDREQ_32B_sum"]) * 64.0 + cb["TCC_EA0_RDREQ_32B_sum"] * 32.0)
+    out = {"_calibration": {"copy_bytes": copy_bytes, "read_factor": cf, "write_factor": cw,
+                            "copy_read_by_fetch_size_formula": fetch_size_formula,
+                            "method": "reads from TCC_EA0_RDREQ by request size, writes WRITE_SIZE"}}
+    per_kernel = {}
+    for k in f:
+        if k in w and "TCC_EA0_RDREQ_sum" in f[k]:
+            per_kernel[k] = {"read_bytes": _req_bytes(f[k]) * cf,
+                             "write_bytes": w[k]["WRITE_SIZE"] * 1024.0 * cw}
+    for k, name in BENCH_NAME.items():
+        if k in per_kernel:
+            rd, wr = per_kernel[k]["read_bytes"], per_kernel[k]["write_bytes"]
+            e = out.setdefault(name, {"hbm_read_bytes": 0.0, "hbm_write_bytes": 0.0, "hbm_bytes_per_launch": 0.0,
+                                      "kernels": []})
+            e["hbm_read_bytes"] += rd
+            e["hbm_write_bytes"] += wr
+            e["hbm_bytes_per_launch"] += rd + wr
+            e["kernels"].append(k)
+    out["_all_kernels"] = per_kernel
+    return {workload: out}
+
+
 def valu(db: str, workload: str) -> dict:
     t = pmc_table(db)
     out = {}
@@ -118,6 +159,12 @@ def main():
     a.add_argument("--copy-bytes", type=int, default=1 << 30)
     a.add_argument("--workload", required=True)
     a.add_argument("--out")
+    a = sub.add_parser("requests")
+    a.add_argument("--reads", required=True)
+    a.add_argument("--write", required=True)
+    a.add_argument("--copy-bytes", type=int, default=1 << 30)
+    a.add_argument("--workload", required=True)
+    a.add_argument("--out")
     a = sub.add_parser("valu")
     a.add_argument("db")
     a.add_argument("--workload", required=True)
@@ -128,8 +175,12 @@ def main():
     elif args.cmd == "pmc":
         print(pmc(args.db))
     else:
-        res = (traffic(args.fetch, args.write, args.copy_bytes, args.workload) if args.cmd == "traffic"
-               else valu(args.db, args.workload))
+        if args.cmd == "traffic":
+            res = traffic(args.fetch, args.write, args.copy_bytes, args.workload)
+        elif args.cmd == "requests":
+            res = requests(args.reads, args.write, args.copy_bytes, args.workload)
+        else:
+            res = valu(args.db, args.workload)
         text = json.dumps(res, indent=1)
         if args.out:
             open(args.out, "w").write(text + "\n")
