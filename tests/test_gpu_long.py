@@ -36,3 +36,58 @@ def test_hundred_steps_small_config(gpu):
     assert not bad, "\n".join(bad)
     moved = np.sum(gm["x"] != cfg.markers["x"])
     assert moved > 0.9 * cfg.n
+
+
+@pytest.mark.timeout(900)
+def test_hundred_steps_production_size(gpu):
+    """The production workload's own run: 1501x301 nodes refined to 500 m, 36
+    markers per cell (16.2 M markers), 100 steps at dt = 1 yr.  The oracle
+    needs about 11 s per full step here, so the check is on properties that
+    hold at any size: two contexts give the same bits after 100 steps (marker
+    and nodal fields); the cell order kept up to date incrementally over the
+    100 advections is the (key, id) order of the keys the oracle's bisection
+    gives for the final positions; every field stays finite.  The oracle's
+    parity at this size is checked for one step in
+    test_gpu_parity.py::test_production_full_step_parity, and over 100 steps
+    on the 640 k-marker cloud above."""
+    from velo2cycles_amd.abi import Context, MarkerArrays
+    from tests.test_gpu_parity import _oracle_keys
+    cfg = synth.config_production()
+    dt = cfg.timestep
+    cfg.timesum = dt
+    print("case production 100 steps", cfg.n, flush=True)
+    ctxs = []
+    for _ in range(2):
+        c = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
+        c.set_time(cfg.timesum, dt)
+        c.set_all_nodal(cfg.nodal)
+        c.upload_markers(cfg.markers)
+        ctxs.append(c)
+    for s in range(100):
+        for c in ctxs:
+            c.set_time(cfg.timesum + s * dt, dt)
+            c.marker2nodes()
+            c.compute_before_solve(dt, 0)
+            c.compute_after_solve(dt, 0)
+            c.nodes2markers_advect()
+    res = []
+    for c in ctxs:
+        m = MarkerArrays(cfg.n)
+        c.download_markers(m)
+        res.append((m, c.get_all_nodal()))
+    (m0, n0), (m1, n1) = res
+    for f in MARKER_FIELDS:
+        assert np.array_equal(m0[f], m1[f], equal_nan=True), f
+        assert np.all(np.isfinite(m0[f])), f
+    assert np.array_equal(m0.t, m1.t)
+    for k in NODAL_FIELDS:
+        assert np.array_equal(n0[k], n1[k], equal_nan=True), k
+    moved = np.sum(m0["x"] != cfg.markers["x"])
+    assert moved > 0.9 * cfg.n
+    order, skey = ctxs[0].order_get()
+    ref = _oracle_keys(cfg, m0["x"], m0["y"])
+    want = np.lexsort((np.arange(cfg.n), ref))
+    assert np.array_equal(order.astype(np.int64), want)
+    assert np.array_equal(skey.astype(np.int64), ref[want])
+    for c in ctxs:
+        c.close()
