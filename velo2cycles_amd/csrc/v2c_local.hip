@@ -198,7 +198,7 @@ __device__ __forceinline__ void lu_rsf_block(const KArgs &a, long i, long iend, 
         MlpFeat ft;
         ft.rsf = false;
         ft.vhi = 0;
-        if (live) mlp_features(a, m, dt, ft);
+        if (live) mlp_features<false>(a, m, dt, ft);
         mlp_stage_weights(a.mlp, L->w, threadIdx.x, LU_BLOCK);
         mlp_load_row(ft, a.mlp_norm, live, L->x[w][lane]);
         __syncthreads();
@@ -207,7 +207,7 @@ __device__ __forceinline__ void lu_rsf_block(const KArgs &a, long i, long iend, 
         (void)y0;
         if (ft.rsf) {
             const double r1 = y1 * a.mlp_norm[2 * MLP_IN + 3] + a.mlp_norm[2 * MLP_IN + 1];
-            guess = ft.vhi * pow(10.0, r1);
+            guess = ft.vhi * feat_exp(r1 * LN10);
         }
     }
     if (live) lu_rsf_one(a, m, dt, guess, tol, acc);

@@ -118,6 +118,25 @@ struct MlpFeat {
     double eta0, vhi;         // default starts
 };
 
+// Logarithm and exponential of the features: the FP64 forms of the viscosity
+// solves (a few ulp, far below what BF16 inputs resolve), the library where
+// those forms' range ends.  Training rows and inference share them.
+__device__ __forceinline__ double feat_log(double x)
+{
+    const double r = log_pos(x);
+    return r == r ? r : log(x);
+}
+__device__ __forceinline__ double feat_exp(double x)
+{
+    const double r = exp_red(x);
+    return r == r ? r : exp(x);
+}
+constexpr double INV_LN10 = 0.43429448190325182765, LN10 = 2.30258509299404568402;
+
+// DUCT = false: the ductile default start is not needed (the slip-rate solve
+// asks for the slip-rate guess only) and is left 0; the features do not
+// depend on it.
+template <bool DUCT = true>
 __device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, MlpFeat &o)
 {
     const int t = a.t[m];
@@ -131,7 +150,7 @@ __device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, 
     o.rsf = false;
     o.eta0 = 0;
     o.vhi = 0;
-    if (power_law_type(M, t) && t != 1 && sig > 0) {
+    if (DUCT && power_law_type(M, t) && t != 1 && sig > 0) {
         const double n = M->mm[t], mpb = 1e-5 * p;
         double e1 = (M->dh[t] + M->dv[t] * mpb) / (8.314 * tk);
         if (e1 > 150.0) e1 = 150.0;
@@ -145,25 +164,28 @@ __device__ __forceinline__ void mlp_features(const KArgs &a, long m, double dt, 
         const double mu0 = MXF(a, MX_MU0, m), phi = MXF(a, MX_PHI, m), Vp = MXF(a, MX_V, m);
         const double nP = a.p.pmode == 0 ? a.p.nstress : p;
         const double eta_rad = a.p.inertyn == 0 ? a.p.rad_damping : 0.0;
+        const double ephi = feat_exp(phi);
         double phin;
-        if (Vp * dt / rL <= 1e-6) phin = log(exp(phi) * (1 - Vp * dt / rL) + rV0 * dt / rL);
-        else phin = log(rV0 / Vp + (exp(phi) - rV0 / Vp) * exp(-Vp * dt / rL));
+        if (Vp * dt / rL <= 1e-6) phin = feat_log(ephi * (1 - Vp * dt / rL) + rV0 * dt / rL);
+        else phin = feat_log(rV0 / Vp + (ephi - rV0 / Vp) * feat_exp(-Vp * dt / rL));
         if (sig > 0 && eta_rad > 0) {
-            double vhi = 2 * rV0 * sinh(sig / (ra * nP)) * exp(-(mu0 + rb * phin) / ra);
+            const double x = sig / (ra * nP);
+            const double ex = feat_exp(x);
+            double vhi = 2 * rV0 * (0.5 * (ex - 1.0 / ex)) * feat_exp(-(mu0 + rb * phin) / ra);
             if (sig / eta_rad < vhi) vhi = sig / eta_rad;
             o.vhi = vhi;
             o.rsf = vhi > 0 && isfinite(vhi);
-            if (o.rsf) vhi_l10 = log10(vhi);
+            if (o.rsf) vhi_l10 = feat_log(vhi) * INV_LN10;
         }
-        theta_l10 = (phi + log(rL / rV0)) / 2.302585092994046;   // theta = L e^phi / V0
+        theta_l10 = (phi + feat_log(rL / rV0)) * INV_LN10;   // theta = L e^phi / V0
         amb = ra - rb;
-        L_l10 = log10(rL);
-        vp_l10 = log10(Vp > 1e-40 ? Vp : 1e-40);
+        L_l10 = feat_log(rL) * INV_LN10;
+        vp_l10 = feat_log(Vp > 1e-40 ? Vp : 1e-40) * INV_LN10;
     }
-    o.f[0] = (float)log10(eii > 1e-30 ? eii : 1e-30);
+    o.f[0] = (float)(feat_log(eii > 1e-30 ? eii : 1e-30) * INV_LN10);
     o.f[1] = (float)tk;
     o.f[2] = (float)p;
-    o.f[3] = (float)log10(sig > 1e-30 ? sig : 1e-30);
+    o.f[3] = (float)(feat_log(sig > 1e-30 ? sig : 1e-30) * INV_LN10);
     o.f[4] = (float)theta_l10;
     o.f[5] = (float)amb;
     o.f[6] = (float)vhi_l10;
