@@ -46,7 +46,8 @@ def test_hundred_steps_production_size(gpu):
     hold at any size: two contexts give the same bits after 100 steps (marker
     and nodal fields); the cell order kept up to date incrementally over the
     100 advections is the (key, id) order of the keys the oracle's bisection
-    gives for the final positions; every field stays finite.  The oracle's
+    gives for the final positions; marker2nodes through that order gives
+    the bits it gives through a fresh full sort; every field stays finite.  The oracle's
     parity at this size is checked for one step in
     test_gpu_parity.py::test_production_full_step_parity, and over 100 steps
     on the 640 k-marker cloud above."""
@@ -84,6 +85,21 @@ def test_hundred_steps_production_size(gpu):
         assert np.array_equal(n0[k], n1[k], equal_nan=True), k
     moved = np.sum(m0["x"] != cfg.markers["x"])
     assert moved > 0.9 * cfg.n
+    # marker2nodes once more: through the order kept incrementally over the 100
+    # advections (its sub-cell starts scanned from maintained counts) and
+    # through a fresh context's full sort of the same markers, the same bits
+    t_end = cfg.timesum + 100 * dt
+    fresh = Context(cfg.grid, cfg.params, cfg.mats, capacity=cfg.n, lib=gpu)
+    fresh.set_time(t_end, dt)
+    fresh.set_all_nodal(n0)
+    fresh.upload_markers(m0)
+    for c in (ctxs[0], fresh):
+        c.set_time(t_end, dt)
+        c.marker2nodes()
+    na, nb = ctxs[0].get_all_nodal(), fresh.get_all_nodal()
+    for k in NODAL_FIELDS:
+        assert np.array_equal(na[k], nb[k], equal_nan=True), k
+    fresh.close()
     order, skey = ctxs[0].order_get()
     ref = _oracle_keys(cfg, m0["x"], m0["y"])
     want = np.lexsort((np.arange(cfg.n), ref))

@@ -52,10 +52,17 @@ struct v2c_ctx {
     double *d_mk = nullptr;
     int8_t *d_t = nullptr;
     uint32_t *d_key = nullptr, *d_skey = nullptr, *d_iota = nullptr, *d_order = nullptr, *d_cs = nullptr;
+    // markers per sub-cell of the current order (4 ncell + 1 entries), valid
+    // when cnt_valid; the incremental re-ordering keeps them and scans them
+    // into the sub-cell starts
+    uint32_t *d_cnt = nullptr;
+    bool cnt_valid = false;
+    void *d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
     double *d_nod = nullptr, *d_part = nullptr, *d_red = nullptr, *d_prop = nullptr;
     int *d_counters = nullptr;
     int before_slot = 0;                // compute_before_solve's error-count slot (0 or BEFORE_SLOTS)
-    long ord_max = 512;                 // workgroup cap of the re-ordering passes (0: one per tile)
+    long ord_max = 256;                 // workgroup cap of the re-ordering passes (0: one per tile)
     unsigned timing_mask = ~0u;         // kernels (1 << V2C_K_*) timed while timing is on
     // incremental re-ordering buffers
     uint32_t *d_order2 = nullptr, *d_skey2 = nullptr, *d_tile_count = nullptr, *d_tile_off = nullptr;
@@ -212,18 +219,33 @@ static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 
 // tiles): at most ord_max, so that beside the property pass they hold fewer
 // of the CUs' wave slots (production step: 3.08 ms with one workgroup per
 // tile, 3.05 with 256 or 512, 3.12 with 128 and 3.52 with 64, where the
-// re-ordering outlasts the property pass).
+// re-ordering outlasts the property pass; with the sub-cell starts scanned
+// from counts: 3.002 ms with 512, 2.996 with 256, 3.042 with 128).
 static inline unsigned ord_grid(const v2c_ctx *c, long ntiles)
 {
     return (unsigned)(c->ord_max > 0 && ntiles > c->ord_max ? c->ord_max : ntiles);
 }
 
+// Sub-cell starts of the current order: from the keys in a pass over the
+// order (and the per-sub-cell counts from them), or, where the counts are
+// valid (kept by the incremental re-ordering), as their exclusive scan.
 static int cell_starts(v2c_ctx *c)
 {
-    return launch(c, V2C_K_SORT, [&] {
+    const long nsub = 4 * c->ncell;
+    if (c->cnt_valid)
+        return launch(c, V2C_K_SORT, [&] {
+            return hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, c->scan_tmp_bytes, c->d_cnt, c->d_cs,
+                                                    (int)(nsub + 1), c->stream);
+        });
+    int rc = launch(c, V2C_K_SORT, [&] {
         hipLaunchKernelGGL(k_sub_start, dim3(ord_grid(c, blocks_for(c->n + 1, ORD_TILE))), dim3(ORD_BLOCK), 0, c->stream,
-                           c->d_skey, c->n, 4 * c->ncell, c->d_cs);
+                           c->d_skey, c->n, nsub, c->d_cs);
+        hipLaunchKernelGGL(k_sub_count, dim3(blocks_for(nsub + 1, 256)), dim3(256), 0, c->stream, c->d_cs, nsub,
+                           c->d_cnt);
     });
+    if (rc) return rc;
+    c->cnt_valid = true;
+    return 0;
 }
 
 // Full stable radix sort of (sub-cell key, marker id) -> order[], skey[].
@@ -239,6 +261,7 @@ static int full_sort(v2c_ctx *c)
     }
     c->n_full_sorts++;
     c->ever_sorted = true;
+    c->cnt_valid = false;
     return cell_starts(c);
 }
 
@@ -292,9 +315,11 @@ static int order_finish(v2c_ctx *c)
         c->order_valid = true;
         return 0;
     }
+    // the counts follow the movers only if they describe the old order
+    uint32_t *cnt = c->cnt_valid ? c->d_cnt : nullptr;
     rc = launch(c, V2C_K_SORT, [&] {
         hipLaunchKernelGGL(k_order_movers, dim3(ord_grid(c, ntiles)), dim3(ORD_BLOCK), 0, c->stream, c->d_order,
-                           c->d_key, n, c->d_mask, c->d_tile_off, c->d_k64_in);
+                           c->d_key, n, c->d_mask, c->d_tile_off, c->d_k64_in, c->d_skey, cnt);
         hipError_t e = hipcub::DeviceRadixSort::SortKeys(c->d_sort_tmp, c->sort_tmp_bytes, c->d_k64_in,
                                                          c->d_k64_out, (int)nmov, 0, 32 + c->key_bits, c->stream);
         if (e != hipSuccess) return e;
@@ -406,7 +431,8 @@ static void free_all(v2c_ctx *c)
                     c->d_order2, c->d_skey2, c->d_tile_count, c->d_tile_off, c->d_mask,
                     c->d_k64_in, c->d_k64_out, c->d_nsel, c->d_mx, c->d_fault, c->d_lu, c->d_mlp,
                     c->d_mlp_m, c->d_mlp_v, c->d_mlp_norm, c->d_post, c->d_bondm, c->d_bondn,
-                    c->d_bondv, c->d_bcx, c->d_bcflag, c->d_defer, c->d_fclass, c->d_flist};
+                    c->d_bondv, c->d_bcx, c->d_bcflag, c->d_defer, c->d_fclass, c->d_flist, c->d_cnt,
+                    c->d_scan_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &tl : c->pending) { (void)hipEventDestroy(tl.start); (void)hipEventDestroy(tl.stop); }
@@ -474,6 +500,7 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
     CK(hipMalloc(&c->d_iota, sizeof(uint32_t) * c->cap));
     CK(hipMalloc(&c->d_order, sizeof(uint32_t) * c->cap));
     CK(hipMalloc(&c->d_cs, sizeof(uint32_t) * (4 * c->ncell + 1)));
+    CK(hipMalloc(&c->d_cnt, sizeof(uint32_t) * (4 * c->ncell + 1)));
     CK(hipMalloc(&c->d_nod, sizeof(double) * V2C_NF * c->nodenum));
     CK(hipMalloc(&c->d_part, sizeof(double) * 4 * M_NF * c->nodenum));
     CK(hipMalloc(&c->d_prop, sizeof(double) * NPROP * c->cap));
@@ -561,8 +588,11 @@ int v2c_create(v2c_ctx **out, const v2c_grid *g, const v2c_params *prm, const v2
         CK(hipcub::DeviceRadixSort::SortKeys(nullptr, b3, c->d_k64_in, c->d_k64_out, (int)c->cap, 0,
                                              32 + c->key_bits, c->stream));
         c->sort_tmp_bytes = std::max(b1, b3);
+        CK(hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_cnt, c->d_cs, (int)(4 * c->ncell + 1),
+                                            c->stream));
     }
     CK(hipMalloc(&c->d_sort_tmp, c->sort_tmp_bytes > 0 ? c->sort_tmp_bytes : 16));
+    CK(hipMalloc(&c->d_scan_tmp, c->scan_tmp_bytes > 0 ? c->scan_tmp_bytes : 16));
     CK(hipStreamSynchronize(c->stream));
 #undef CK
     *out = c;

@@ -874,6 +874,7 @@ static int ens_order_batched(v2c_ens *e)
             std::swap(c->d_skey, c->d_skey2);
             c->n_incremental++;
             c->order_valid = true;
+            c->cnt_valid = false;       // the batched passes recompute the starts, not the counts
         } else if (nmov == 0) {
             c->order_valid = true;
         } else {

@@ -135,6 +135,15 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_sub_start(const uint32_t *__restr
     }
 }
 
+// Markers per sub-cell from the sub-cell starts: cnt[k] = ss[k+1] - ss[k],
+// cnt[nsub] = 0 (so that an exclusive scan of cnt gives ss back).
+__global__ void k_sub_count(const uint32_t *__restrict__ ss, long nsub, uint32_t *__restrict__ cnt)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nsub) cnt[k] = ss[k + 1] - ss[k];
+    else if (k == nsub) cnt[k] = 0u;
+}
+
 // Pass 1: mover masks and mover count of one tile.
 __device__ __forceinline__ void order_flags_tile(const uint32_t *__restrict__ order, const uint32_t *__restrict__ key,
                                                  const uint32_t *__restrict__ skey, long n,
@@ -232,7 +241,9 @@ __device__ __forceinline__ void order_movers_tile(const uint32_t *__restrict__ o
                                                   const uint32_t *__restrict__ key, long n,
                                                   const unsigned long long *__restrict__ mask,
                                                   const uint32_t *__restrict__ off, uint64_t *__restrict__ mov,
-                                                  uint64_t hi, long tile)
+                                                  uint64_t hi, long tile,
+                                                  const uint32_t *__restrict__ skey = nullptr,
+                                                  uint32_t *__restrict__ cnt = nullptr)
 {
     __shared__ uint32_t wpre[ORD_BLOCK / WAVE + 1];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
@@ -254,7 +265,15 @@ __device__ __forceinline__ void order_movers_tile(const uint32_t *__restrict__ o
         __syncthreads();
         if ((b >> lane) & 1ull) {
             const uint32_t id = order[i];
-            mov[base + wpre[wv] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = kid(key[id], id) | hi;
+            const uint32_t k = key[id];
+            mov[base + wpre[wv] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = kid(k, id) | hi;
+            if (cnt) {
+                // the sub-cell counts of the new order: one marker fewer in its
+                // old sub-cell, one more in its new one (integer atomics: the
+                // totals do not depend on their order)
+                atomicSub(&cnt[skey[i]], 1u);
+                atomicAdd(&cnt[k], 1u);
+            }
         }
         base += wpre[ORD_BLOCK / WAVE];
         __syncthreads();
@@ -265,12 +284,13 @@ __global__ __launch_bounds__(ORD_BLOCK) void k_order_movers(const uint32_t *__re
                                                             const uint32_t *__restrict__ key, long n,
                                                             const unsigned long long *__restrict__ mask,
                                                             const uint32_t *__restrict__ off,
-                                                            uint64_t *__restrict__ mov)
+                                                            uint64_t *__restrict__ mov,
+                                                            const uint32_t *__restrict__ skey, uint32_t *__restrict__ cnt)
 {
     const long ntiles = (n + ORD_TILE - 1) / ORD_TILE;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         __syncthreads();
-        order_movers_tile(order, key, n, mask, off, mov, 0ull, tile);
+        order_movers_tile(order, key, n, mask, off, mov, 0ull, tile, skey, cnt);
     }
 }
 
