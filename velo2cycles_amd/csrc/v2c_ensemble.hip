@@ -59,11 +59,39 @@ __global__ __launch_bounds__(ENS_BLOCK) void k_ens_m2n_props(const KArgs *__rest
     ens_item(A, tiles, [&](const KArgs &a, long m) { m2n_props_one(a, m); });
 }
 
+// The persistent scatter grids deal the cell-group list to the XCDs in chunks
+// of ENS_CHUNK consecutive groups, round-robin (chunk q to XCD q % 8), and the
+// workgroups of an XCD (workgroup b runs on XCD b % 8) stride over that XCD's
+// chunks: neighbouring groups, which share marker cache lines and node
+// records, mostly meet in one L2, while every XCD gets chunks of every part
+// of the list (models of all sizes).  With a grid that is not a multiple of 8
+// the list is simply strided.
+constexpr int ENS_CHUNK = 32;
+struct EnsBand {
+    int x, r, step;
+    bool chunked;
+};
+__device__ __forceinline__ EnsBand ens_band(int ntiles)
+{
+    const int G = gridDim.x;
+    if (G % 8 != 0 || ntiles < G) return EnsBand{0, (int)blockIdx.x, G, false};
+    return EnsBand{(int)blockIdx.x % 8, (int)blockIdx.x / 8, G / 8, true};
+}
+// The j-th group this workgroup takes (j = r, r + step, ...), or past the end.
+__device__ __forceinline__ int ens_band_tile(const EnsBand &bd, int j)
+{
+    if (!bd.chunked) return j;
+    return ((j / ENS_CHUNK) * 8 + bd.x) * ENS_CHUNK + j % ENS_CHUNK;
+}
+
 __global__ __launch_bounds__(WAVE) void k_ens_m2n_cells(const KArgs *__restrict__ A,
                                                         const EnsTile *__restrict__ tiles, int ntiles)
 {
     __shared__ M2nLds L;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const EnsBand bd = ens_band(ntiles);
+    for (int j = bd.r;; j += bd.step) {
+        const int t = ens_band_tile(bd, j);
+        if (t >= ntiles) break;
         const EnsTile T = tiles[t];
         m2n_cells_group(model_args(A, T.model), T.begin, T.end, threadIdx.x, L);
         __syncthreads();
@@ -139,7 +167,10 @@ __global__ __launch_bounds__(WAVE) void k_ens_n2m_scatter(const KArgs *__restric
                                                           const EnsTile *__restrict__ tiles, int ntiles)
 {
     __shared__ N2mLds L;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const EnsBand bd = ens_band(ntiles);
+    for (int j = bd.r;; j += bd.step) {
+        const int t = ens_band_tile(bd, j);
+        if (t >= ntiles) break;
         const EnsTile T = tiles[t];
         const KArgs &a = model_args(A, T.model);
         if (a.p.antidiffinter == 1) n2m_scatter_group(a, T.begin, T.end, threadIdx.x, L);
@@ -685,7 +716,8 @@ static void ens_branch_stages(v2c_ens *e, int b, hipStream_t st, bool side)
     uint64_t *defer = e->d_defer + B.defer_off;
     const dim3 mg((unsigned)B.n_mark), ng((unsigned)B.n_node);
     auto grid = [&](int ntiles) {
-        const long g = std::max(1L, (long)e->ncu * e->blocks_per_cu * 4 / e->nbranch);
+        // a multiple of 8 where it can be (ens_band's XCD bands)
+        const long g = std::max(8L, ((long)e->ncu * e->blocks_per_cu * 4 / e->nbranch) / 8 * 8);
         return (unsigned)std::max(1L, std::min((long)ntiles, g));
     };
     hipLaunchKernelGGL(k_ens_clear, dim3(1), dim3(64), 0, st, flags, 4);
