@@ -144,3 +144,35 @@ def test_ensemble_batched_reordering_matches_single_models(gpu):
     ens.close()
     for c in singles + batch:
         c.close()
+
+
+def test_batched_and_single_steps_interleave(gpu):
+    """Contexts stepped alternately by the batched ensemble and by their own
+    entry points (the batched re-ordering recomputes the sub-cell starts from
+    the keys; a single-model re-ordering after it rebuilds the per-sub-cell
+    counts it keeps, and scans them on the steps after) give, step for step,
+    the bits of contexts stepped by their entry points alone."""
+    cfgs = _models()
+    for c, years in zip(cfgs, (1.0, 10.0, 100.0, 1.0)):
+        c.timestep = years * DT
+    singles = [_ctx(gpu, c) for c in cfgs]
+    mixed = [_ctx(gpu, c) for c in cfgs]
+    ens = Ensemble(mixed)
+    for step, how in enumerate(("single", "batched", "single", "single", "batched", "batched", "single",
+                                "single")):
+        for ctx, cfg in zip(singles, cfgs):
+            _single_step(ctx, cfg, cfg.timesum + step * cfg.timestep)
+        if how == "batched":
+            for ctx, cfg in zip(mixed, cfgs):
+                ctx.set_time(cfg.timesum + step * cfg.timestep, cfg.timestep)
+            ens.step(graph=True)
+        else:
+            for ctx, cfg in zip(mixed, cfgs):
+                _single_step(ctx, cfg, cfg.timesum + step * cfg.timestep)
+        for k, cfg in enumerate(cfgs):
+            _assert_same(_state(mixed[k], cfg), _state(singles[k], cfg), f"step {step} ({how}) model {k}")
+    stats = [c.order_stats() for c in mixed]
+    assert any(s["incremental"] > 1 for s in stats), stats
+    ens.close()
+    for c in singles + mixed:
+        c.close()
