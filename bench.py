@@ -69,7 +69,7 @@ B_STEP_MARKER_KERNELS = 161 + 113
 # STOKES_READS / OUTPUT_READS, those a device routine wrote).
 PCIE_H2D = ("pr", "vx", "vy")
 PCIE_D2H = ("ro", "nu", "nd", "gg", "gd", "sxxe", "sxye", "sppe", "mvx", "mvy", "mrx", "mry", "bm",
-            "sxxs", "sxy", "sliprs", "phis_it", "syields")
+            "sxxs", "sxy", "sliprs", "phis_it", "syields", "sticky01")
 # Before residency tracking: every array each routine reads, sent, and every
 # array it writes, fetched (marker2nodes 24 / 43, nodes2markers 28 / 9,
 # moveandrotatemarkers 5 / 0, before solve 22 / 6, after solve 19 / 3).
@@ -189,6 +189,10 @@ NEXT_ALG = {
     "vxvy_boundarycorrection": (0, 2 * 8 * 3),
     "P_boundarycorrection": (0, 8 * 3),
     "tkrecalc": (0, 8 * 3),
+    # reads every marker type and the positions of the markers of types 1
+    # and 2 (added per call below), writes sticky01
+    "sticky_marks": (1, 8),
+    "calc_sticky_stats": (0, 4 * 8),
 }
 
 
@@ -220,7 +224,10 @@ def next_rows(ctx, cfg, reps: int = 20):
         "vxvy_boundarycorrection": ctx.vxvy_boundarycorrection,
         "P_boundarycorrection": ctx.P_boundarycorrection,
         "tkrecalc": ctx.tkrecalc,
+        "sticky_marks": ctx.sticky_marks,
+        "calc_sticky_stats": ctx.sticky_stats,
     }
+    n12 = int(np.count_nonzero((cfg.markers.t == 1) | (cfg.markers.t == 2)))
     out = {}
     for name, fn in calls.items():
         fn()                                   # warm
@@ -236,7 +243,8 @@ def next_rows(ctx, cfg, reps: int = 20):
         ms, _ = ctx.kernel_time("post")
         ms /= reps
         per_m, per_n = NEXT_ALG[name]
-        alg = n * per_m + nn * per_n + (rows * 56 if name.endswith(("correction", "recalc")) else 0)
+        alg = n * per_m + nn * per_n + (rows * 56 if name.endswith(("correction", "recalc")) else 0) + \
+            (n12 * 16 if name == "sticky_marks" else 0)
         out[name] = {"kernel_ms": ms, "wall_ms": wall, "alg_bytes": alg,
                      "achieved_gbs": alg / (ms * 1e-3) / 1e9 if ms > 0 else None}
     d = synth.prn_model(cfg, 5, bonds=False)

@@ -160,6 +160,8 @@ enum v2c_nodal_field {
     /* heat solve: temperature before it (temperature_iterate.c:58-62) and
      * the marker-temperature averages (temperature_iterate.c:183-184,353-364) */
     V2C_N_tk2, V2C_N_tol0, V2C_N_tol1,
+    /* sticky-air marks of marker2nodes (interpolation.c:121, 861-916) */
+    V2C_N_sticky01,
     V2C_NF
 };
 
@@ -222,11 +224,26 @@ int v2c_nodal_get_many(v2c_ctx *ctx, const int *fields, double *const *host, int
 
 /* ---- hot-path entry points ---------------------------------------------- */
 
-/* marker2nodes() (interpolation.c:55-916, called from i2evps.c:89):
+/* marker2nodes() (interpolation.c:55-859, called from i2evps.c:89):
  * marker->node scatter with interpolationmod=2 weights, normalisation,
- * antidiffusion and nodal friction parameters.  The boundary-condition tail
- * (interpolation.c:918-925) stays on the host. */
+ * antidiffusion and nodal friction parameters.  The routine's sticky-air
+ * marks (:861-916) are v2c_sticky_marks and its boundary-condition tail
+ * (:918-925) the three boundary routines below; the reference-side binding
+ * calls both after this one. */
 int v2c_marker2nodes(v2c_ctx *ctx);
+
+/* The sticky-air marks at the end of marker2nodes (interpolation.c:861-916;
+ * the field is cleared at :121), in the nodal field V2C_N_sticky01: every
+ * node is 0 except those a marker of type 1 or 2 marks with its type.  A
+ * marker's cell is nx = nxsearch(x), ny = nysearch(y); its quadrant, from
+ * celdx and celdy against 0.5, picks node m3, m3+1, m3+ynumy or m3+ynumy+1
+ * with the reference's m3 = nx*ynumy+ny+1 (one node below the cell's
+ * upper-left node); a marker on a cell midline marks nothing.  Indices past
+ * the end of a column wrap into the next column, as in the reference; those
+ * at or beyond nodenum, which no reference loop reads, are dropped.  Where
+ * several markers mark one node, the one with the highest index wins, as in
+ * the reference's serial loop. */
+int v2c_sticky_marks(v2c_ctx *ctx);
 
 /* nodes2markers() (interpolation.c:929-1048, called from i2evps.c:108). */
 int v2c_nodes2markers(v2c_ctx *ctx);
@@ -315,8 +332,9 @@ int v2c_interpolate_basic2center(v2c_ctx *ctx);
 int v2c_copyarrays(v2c_ctx *ctx, int rk);
 
 /* calc_timestep(oldornew) (calc_RSF.c:352-572) without its sticky-air
- * diagnostics: the adaptive time step from the visco-plastic, displacement
- * and rate-and-state limits over interior nodes.  Every running minimum and
+ * statistics (v2c_calc_sticky_stats): the adaptive time step from the
+ * visco-plastic, displacement and rate-and-state limits over interior
+ * nodes.  Every running minimum and
  * maximum follows the reference's sequential MINV/MAXV semantics exactly
  * (including NaN and the reset of dtphi/dtL to maxtmstep at nodes with
  * bs <= 0), whatever the reduction order on the device.  Where the reference
@@ -338,6 +356,27 @@ typedef struct {
 /* reference defaults (input_RSF.c:17-20), maxtmstep = 1 yr, lambda0 = NaN */
 void v2c_timestep_params_default(v2c_timestep_params *p);
 int v2c_calc_timestep(v2c_ctx *ctx, int oldornew, const v2c_timestep_params *p, v2c_timestep_result *out);
+
+/* The sticky-air statistics of calc_timestep (calc_RSF.c:364-374, 505-550;
+ * the reference globals of headevps.c:361-370 without the std_* members,
+ * which host code after copyarrays computes, momentum_continuity_iterate.c:
+ * 129-158), over the interior nodes 1 <= m1 < xnumx-1, 1 <= m2 < ynumy-1,
+ * from sxxs, sxy, nu and sticky01, with sigin = (sxxs^2 + sxy^2)^0.5.  As in
+ * the reference, the "_1" members gather the nodes with sticky01 == 2 and the
+ * "_0" members those with sticky01 == 1; P1 is node (50, 50) and P2 node
+ * (500, 50), set where that node has sticky01 == 1.  Counts, extrema and the
+ * P values follow the reference's serial loop exactly (MAXV/MINV, NaN
+ * included, from 0 and 1e30); the sums are added in a fixed order
+ * (deterministic, within 1e-12 relative of the serial sums). */
+typedef struct {
+    int nr_1, nr_0;
+    double max_sigin_1, max_sigin_0, min_sigin_1, min_sigin_0;
+    double sum_sigin_1, sum2_sigin_1, sum_sigin_0, sum2_sigin_0;
+    double max_visc_1, max_visc_0, min_visc_1, min_visc_0;
+    double sum_visc_1, sum2_visc_1, sum_visc_0, sum2_visc_0;
+    double sigin_P1, sigin_P2, visco_P1, visco_P2;
+} v2c_sticky_stats;
+int v2c_calc_sticky_stats(v2c_ctx *ctx, v2c_sticky_stats *out);
 
 /* ---- boundary-condition tail of marker2nodes and the heat solve -----------
  * Boundary equations in the reference's layout (headevps.c:197-202, read by

@@ -11,8 +11,8 @@ import subprocess
 
 import numpy as np
 
-from velo2cycles_amd.abi import (NF, NODAL_FIELDS, Grid, Markers, Materials, Params, TimestepParams,
-                                 TimestepResult, _dptr)
+from velo2cycles_amd.abi import (NF, NODAL_FIELDS, Grid, Markers, Materials, Params, StickyStats,
+                                 TimestepParams, TimestepResult, _dptr)
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(ORACLE_DIR, "liboracle.so")
@@ -77,6 +77,10 @@ def lib():
         L.or_copyarrays.restype = None
         L.or_calc_timestep.argtypes = [P(OrState), C.c_int, P(TimestepParams), P(TimestepResult)]
         L.or_calc_timestep.restype = C.c_int
+        L.or_sticky_marks.argtypes = [P(OrState)]
+        L.or_sticky_marks.restype = None
+        L.or_calc_sticky_stats.argtypes = [P(OrState), P(StickyStats)]
+        L.or_calc_sticky_stats.restype = None
         L.or_temperature_markers.argtypes = [P(OrState), C.c_double]
         L.or_temperature_markers.restype = None
         for name in ("or_vxvy_boundarycorrection", "or_P_boundarycorrection", "or_tkrecalc"):
@@ -153,6 +157,14 @@ class Oracle:
         r = TimestepResult()
         rc = lib().or_calc_timestep(C.byref(self.st), oldornew, C.byref(p), C.byref(r))
         return rc, r.as_dict()
+
+    def sticky_marks(self):
+        lib().or_sticky_marks(C.byref(self.st))
+
+    def sticky_stats(self) -> dict:
+        r = StickyStats()
+        lib().or_calc_sticky_stats(C.byref(self.st), C.byref(r))
+        return r.as_dict()
 
     def boundary(self, which, bondm, bondn, bondv):
         """which: 'vxvy', 'P' or 'tk'; returns the oracle's code (0, or -1 / -2

@@ -141,6 +141,11 @@ static entry table[] = {
     D(RK_sliprs_solved), D(RK_phidts_solved), D(RK_deltas),
     D(RK_sliprs0), D(RK_sliprs1), D(RK_phidts0), D(RK_phidts1),
     D(deltasxx), D(deltasxy), D(deltaVs), D(exxs), D(exxes), D(sxxes), D(prs), D(tk2), D(tol0), D(tol1),
+    /* sticky-air marks and statistics (headevps.c:34, 361-370) */
+    D(sticky01), I(nr_1), I(nr_0), S(max_sigin_1), S(max_sigin_0), S(min_sigin_1), S(min_sigin_0),
+    S(sum_sigin_1), S(sum2_sigin_1), S(sum_sigin_0), S(sum2_sigin_0), S(max_visc_1), S(max_visc_0),
+    S(min_visc_1), S(min_visc_0), S(sum_visc_1), S(sum2_visc_1), S(sum_visc_0), S(sum2_visc_0),
+    S(sigin_P1), S(sigin_P2), S(visco_P1), S(visco_P2),
     /* post-solve scalars (headevps.c:120-122, 181, 339, 345-346) */
     S(maxvel), S(maxdphi), S(maxnvp), S(dtL), S(dtphi), S(dtd), S(dtnvp),
     S(maxtmstep), S(tcnvp), S(tcd), S(tcphih), S(dtfactor),

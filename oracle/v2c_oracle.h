@@ -70,6 +70,9 @@ void or_interpolate_basic2center(or_state *s);
 void or_copyarrays(or_state *s, int rk);
 /* 0, or -1 where chi == 0 (the reference exits), -2 for an unknown pmode */
 int or_calc_timestep(or_state *s, int oldornew, const v2c_timestep_params *p, v2c_timestep_result *r);
+/* sticky-air marks of marker2nodes and statistics of calc_timestep */
+void or_sticky_marks(or_state *s);
+void or_calc_sticky_stats(or_state *s, v2c_sticky_stats *o);
 /* marker temperatures after the heat solve */
 void or_temperature_markers(or_state *s, double heatdif);
 /* boundary-condition tail: 0, or -1 / -2 where the reference exits */

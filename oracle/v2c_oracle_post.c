@@ -11,9 +11,13 @@
  *   interpolate_center2basic     interpolation.c:1600-1637
  *   interpolate_basic2center     interpolation.c:1640-1685
  *   copyarrays                   momentum_continuity_iterate.c:265-283
- *   calc_timestep                calc_RSF.c:352-572 (sticky-air diagnostics
- *                                and printf omitted; chi == 0 returns -1
- *                                where the reference exits)
+ *   calc_timestep                calc_RSF.c:352-572 (sticky-air statistics
+ *                                separate, printf omitted; chi == 0 returns
+ *                                -1 where the reference exits)
+ *   sticky-air marks             interpolation.c:121, 861-916 (the end of
+ *                                marker2nodes)
+ *   sticky-air statistics        calc_RSF.c:364-374, 505-550 (inside
+ *                                calc_timestep's node loop)
  *   marker temperatures          temperature_iterate.c:181-419 (the part of
  *                                solve_temperature after the solve)
  */
@@ -537,4 +541,79 @@ int or_tkrecalc(or_state *s, const long *bondm, const long *bondn, const double 
     }
     band_free(&B);
     return rc;
+}
+
+/* interpolation.c:121 clears the field over every node; :866-916 then marks
+ * in marker order, each write replacing the last. */
+void or_sticky_marks(or_state *s)
+{
+    const long ny = s->grid.ynumy, nodenum = (long)s->grid.xnumx * ny;
+    const double *gx = s->grid.gx, *gy = s->grid.gy;
+    double *st = N(sticky01);
+    for (long m3 = 0; m3 < nodenum; m3++) st[m3] = 0;
+    for (long mm1 = 0; mm1 < s->n; mm1++) {
+        const int t = s->m.t[mm1];
+        if (t != 1 && t != 2) continue;
+        const long nx = or_nxsearch(&s->grid, s->m.x[mm1]);
+        const long nyc = or_nysearch(&s->grid, s->m.y[mm1]);
+        const long m3 = nx * ny + nyc + 1;
+        const double celdx = (s->m.x[mm1] - gx[nx]) / (gx[nx + 1] - gx[nx]);
+        const double celdy = (s->m.y[mm1] - gy[nyc]) / (gy[nyc + 1] - gy[nyc]);
+        long k = -1;
+        if (celdx < 0.5 && celdy < 0.5) k = m3;
+        else if (celdx < 0.5 && celdy > 0.5) k = m3 + 1;
+        else if (celdx > 0.5 && celdy < 0.5) k = m3 + ny;
+        else if (celdx > 0.5 && celdy > 0.5) k = m3 + ny + 1;
+        /* indices at or past nodenum land beyond every array the reference
+         * reads (its arrays are MAXNOD long) */
+        if (k >= 0 && k < nodenum) st[k] = t;
+    }
+}
+
+/* calc_RSF.c:364-374 (initial values) and :505-550 (per interior node, in
+ * node order); the std members are computed by host code elsewhere. */
+void or_calc_sticky_stats(or_state *s, v2c_sticky_stats *o)
+{
+    const long ny = s->grid.ynumy, nx = s->grid.xnumx, nodenum = nx * ny;
+    const double *sxxs = N(sxxs), *sxy = N(sxy), *nu = N(nu), *st = N(sticky01);
+    o->nr_1 = o->nr_0 = 0;
+    o->max_sigin_1 = o->max_sigin_0 = 0.0;
+    o->min_sigin_1 = o->min_sigin_0 = 1e30;
+    o->sum_sigin_1 = o->sum2_sigin_1 = 0.0;
+    o->sum_sigin_0 = o->sum2_sigin_0 = 0.0;
+    o->max_visc_1 = o->max_visc_0 = 0.0;
+    o->min_visc_1 = o->min_visc_0 = 1e30;
+    o->sum_visc_1 = o->sum2_visc_1 = 0.0;
+    o->sum_visc_0 = o->sum2_visc_0 = 0.0;
+    o->sigin_P1 = o->sigin_P2 = 0.0;
+    o->visco_P1 = o->visco_P2 = 0.0;
+    for (long m3 = 0; m3 < nodenum; m3++) {
+        const long m2 = m3 % ny, m1 = (m3 - m2) / ny;
+        if (!(m1 >= 1 && m2 >= 1 && m1 < nx - 1 && m2 < ny - 1)) continue;
+        const double sigin = pow(sxxs[m3] * sxxs[m3] + sxy[m3] * sxy[m3], 0.5);
+        if (st[m3] == 1 && m1 == 50 && m2 == 50) { o->sigin_P1 = sigin; o->visco_P1 = nu[m3]; }
+        if (st[m3] == 1 && m1 == 500 && m2 == 50) { o->sigin_P2 = sigin; o->visco_P2 = nu[m3]; }
+        if (st[m3] == 2) {
+            o->nr_1 = o->nr_1 + 1;
+            o->max_sigin_1 = maxv(o->max_sigin_1, sigin);
+            o->min_sigin_1 = minv(o->min_sigin_1, sigin);
+            o->sum_sigin_1 += sigin;
+            o->sum2_sigin_1 += pow(sigin, 2);
+            o->max_visc_1 = maxv(o->max_visc_1, nu[m3]);
+            o->min_visc_1 = minv(o->min_visc_1, nu[m3]);
+            o->sum_visc_1 += log10(nu[m3]);
+            o->sum2_visc_1 += pow(log10(nu[m3]), 2);
+        }
+        if (st[m3] == 1) {
+            o->nr_0 = o->nr_0 + 1;
+            o->max_sigin_0 = maxv(o->max_sigin_0, sigin);
+            o->min_sigin_0 = minv(o->min_sigin_0, sigin);
+            o->sum_sigin_0 += sigin;
+            o->sum2_sigin_0 += pow(sigin, 2);
+            o->max_visc_0 = maxv(o->max_visc_0, nu[m3]);
+            o->min_visc_0 = minv(o->min_visc_0, nu[m3]);
+            o->sum_visc_0 += log10(nu[m3]);
+            o->sum2_visc_0 += pow(log10(nu[m3]), 2);
+        }
+    }
 }

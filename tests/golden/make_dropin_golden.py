@@ -8,7 +8,7 @@ Each fixture holds the exact driver input file (`in_bin`: the record stream
 that tests/native/dropin_harness reads as well), the op list, and the
 reference's outputs after those ops: every marker and nodal field, the
 routines' return values (`drv_ret`; [3] the host output stand-in's sum),
-`errcheck` and calc_timestep's scalars.
+`errcheck` and calc_timestep's scalars (its sticky-air statistics among them).
 The drop-in tests run the same input through the reference-side binding and
 libv2c on the GPU and compare with these outputs; neither the reference nor
 this generator is needed there.
@@ -32,7 +32,12 @@ from oracle import refrun  # noqa: E402
 
 DT = synth.YEAR
 STEP = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m", "move"]
-SCALARS = ("maxvel", "maxdphi", "maxnvp", "dtL", "dtphi", "dtd", "dtnvp")
+SCALARS = ("maxvel", "maxdphi", "maxnvp", "dtL", "dtphi", "dtd", "dtnvp",
+           # calc_timestep's sticky-air statistics
+           "nr_1", "nr_0", "max_sigin_1", "max_sigin_0", "min_sigin_1", "min_sigin_0", "sum_sigin_1",
+           "sum2_sigin_1", "sum_sigin_0", "sum2_sigin_0", "max_visc_1", "max_visc_0", "min_visc_1",
+           "min_visc_0", "sum_visc_1", "sum2_visc_1", "sum_visc_0", "sum2_visc_0", "sigin_P1", "sigin_P2",
+           "visco_P1", "visco_P2")
 
 
 def bond_extra(bc):

@@ -6,6 +6,8 @@ hot-path routines, compiled in place from /root/reference/src
     python tests/golden/make_golden.py post       # post-solve nodal routines
     python tests/golden/make_golden.py bc         # boundary-condition tail
     python tests/golden/make_golden.py prn        # a .prn written by save_model
+    python tests/golden/make_golden.py sticky     # sticky-air marks and statistics
+    python tests/golden/make_golden.py addmarks   # add the marks to older step fixtures
 
 Each fixture holds its complete input (grid, flags, rock table, markers, nodal
 arrays) and the reference's output after the listed ops, so the tests never
@@ -29,6 +31,12 @@ from oracle import refrun  # noqa: E402
 
 DT = synth.YEAR
 MAT = "nu dh dv ss mm gg n0 n1 ro bb aa cp kt kf kp ht".split()
+# calc_timestep's scalars: the time-step limits and the sticky-air statistics
+TSTEP_SCALARS = ("maxvel", "maxdphi", "maxnvp", "dtL", "dtphi", "dtd", "dtnvp")
+STICKY_SCALARS = ("nr_1", "nr_0", "max_sigin_1", "max_sigin_0", "min_sigin_1", "min_sigin_0", "sum_sigin_1",
+                  "sum2_sigin_1", "sum_sigin_0", "sum2_sigin_0", "max_visc_1", "max_visc_0", "min_visc_1",
+                  "min_visc_0", "sum_visc_1", "sum2_visc_1", "sum_visc_0", "sum2_visc_0", "sigin_P1",
+                  "sigin_P2", "visco_P1", "visco_P2")
 
 
 def pack(cfg, ops, out_markers, out_nodal, raw, extra=None):
@@ -74,6 +82,8 @@ def post_cases():
         cfg.params.pmode = pmode
         cfg.params.RSFmode = rsf
         cfg.nodal = synth.post_solve_nodal(cfg, seed=seed + 7)
+        # sticky-air marks for calc_timestep's statistics
+        cfg.nodal["sticky01"] = np.random.default_rng(seed + 11).integers(0, 3, cfg.grid.nodenum).astype(float)
         runs = {"ops": post}
         for oo in (0, 1):
             runs[f"tstep{oo}"] = [f"tstep:{oo}"]
@@ -92,7 +102,7 @@ def post_cases():
                 if not np.array_equal(nd[k], cfg.nodal[k], equal_nan=True):
                     d[pre + "out_n_" + k] = nd[k]
             d[pre + "ret"] = raw["drv_ret"][:3]
-            for k in ("maxvel", "maxdphi", "maxnvp", "dtL", "dtphi", "dtd", "dtnvp"):
+            for k in TSTEP_SCALARS + (STICKY_SCALARS if tag != "ops" else ()):
                 d[pre + "g_" + k] = raw[k]
         d["maxtmstep"] = np.array(DT)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
@@ -125,6 +135,105 @@ def bc_cases():
         print("wrote", name, nn, "nodes", len(bn), "equations")
 
 
+def sticky_cases():
+    """sticky.npz: marker2nodes, compute_before_solve and calc_timestep(0)
+    on a 53 x 53 refined grid (node (50, 50) is interior, so sigin_P1 and
+    visco_P1 are set), 2 x 2 markers per cell of types 1, 2, 4, 7, 8, plus
+    markers that mark one node several times (the last one wins), markers on
+    cell midlines (no mark), in the last cell row (the mark wraps into the
+    next column) and the last corner cell (the mark runs past nodenum),
+    outside the grid and at NaN.  Keeps the input, the marks, nu (which the
+    statistics read) and the statistics."""
+    cfg = synth.config_grid(53, 53, 52e3, 52e3, 2, refine=True, seed=77)
+    cfg.params.RSFmode = 0
+    g = cfg.grid
+    rng = np.random.default_rng(77)
+    base = cfg.markers
+    n0 = base.n
+    extra_n = 600
+    fields = {f: np.concatenate([base[f], base[f][rng.integers(0, n0, extra_n)]]) for f in MARKER_FIELDS}
+    t = np.concatenate([base.t, np.zeros(extra_n, np.int8)])
+    t[:] = rng.choice(np.array([1, 2, 4, 7, 8], np.int8), t.size, p=[0.35, 0.35, 0.1, 0.1, 0.1])
+    x, y = fields["x"], fields["y"]
+    j = n0
+    # repeated marks: copies of earlier markers, jittered within their quadrant
+    x[j:j + 300] += rng.uniform(-1.0, 1.0, 300)
+    y[j:j + 300] += rng.uniform(-1.0, 1.0, 300)
+    j += 300
+    # cell midlines in x, in y, and both
+    for k in range(60):
+        cx, cy = rng.integers(0, g.xnumx - 1), rng.integers(0, g.ynumy - 1)
+        mx, my = (g.gx[cx] + g.gx[cx + 1]) / 2, (g.gy[cy] + g.gy[cy + 1]) / 2
+        x[j], y[j] = (mx, y[j]) if k % 3 == 0 else (x[j], my) if k % 3 == 1 else (mx, my)
+        j += 1
+    # the last cell row (celdy > 0.5) and the last corner cell
+    for k in range(80):
+        cx = rng.integers(0, g.xnumx - 1) if k < 60 else g.xnumx - 2
+        cy = g.ynumy - 2
+        x[j] = g.gx[cx] + rng.uniform(0.05, 0.95) * (g.gx[cx + 1] - g.gx[cx])
+        y[j] = g.gy[cy] + rng.uniform(0.6, 0.95) * (g.gy[cy + 1] - g.gy[cy])
+        j += 1
+    # outside the grid and NaN
+    for k in range(40):
+        if k % 4 == 0:
+            x[j] = g.gx[0] - rng.uniform(1.0, 5e3)
+        elif k % 4 == 1:
+            x[j] = g.gx[-1] + rng.uniform(1.0, 5e3)
+        elif k % 4 == 2:
+            y[j] = g.gy[0] - rng.uniform(1.0, 5e3)
+        else:
+            x[j] = np.nan
+        j += 1
+    # node (50, 50) marked 1 by the last marker of the cloud
+    x[-1] = g.gx[50] + 0.25 * (g.gx[51] - g.gx[50])
+    y[-1] = g.gy[49] + 0.25 * (g.gy[50] - g.gy[49])
+    t[-1] = 1
+    fields["t"] = t
+    cfg.markers = synth.MarkerArrays(t.size, fields)
+    nn = g.nodenum
+    cfg.nodal["sxxs"] = rng.normal(0, 2e7, nn)
+    cfg.nodal["sxy"] = rng.normal(0, 2e7, nn)
+    # compute_before_solve sets the nu the statistics read (marker2nodes
+    # clears it, interpolation.c:72)
+    ops = ["m2n", f"before:{DT}:0", "tstep:0"]
+    mk, nd, raw = refrun.run(cfg, ops, extra={"maxtmstep": ("d", np.array([DT]))})
+    d = {k: v for k, v in pack(cfg, ops, mk, nd, raw).items() if not k.startswith(("out_m_", "out_n_"))}
+    d = {k: v for k, v in d.items() if not (k.startswith("in_n_") and not np.any(v != 0))}
+    d["out_n_sticky01"] = nd["sticky01"]
+    d["out_n_nu"] = nd["nu"]
+    for k in STICKY_SCALARS:
+        d["g_" + k] = raw[k]
+    np.savez_compressed(os.path.join(HERE, "sticky.npz"), **d)
+    print("wrote sticky", cfg.n, "markers", nn, "nodes", int(np.sum(nd["sticky01"] != 0)), "marked")
+
+
+def add_marks():
+    """Add out_n_sticky01 to the step fixtures written before the marks were
+    recorded: the reference re-runs each fixture's own stored input and ops,
+    every stored output must come back bit for bit, and the marks its
+    marker2nodes leaves are added."""
+    from tests.util import load_fixture
+    for name in ("edge_step_t0", "edge_step_t1", "edge_euler_jaumann_densimod", "edge_dp_noantidiff",
+                 "edge_rk3_powerlaw"):
+        path = os.path.join(HERE, name + ".npz")
+        cfg, ops, _, z = load_fixture(path)
+        d = {k: z[k] for k in z.files}
+        if "out_n_sticky01" in d:
+            continue
+        mk, nd, raw = refrun.run(cfg, ops)
+        for k in d:
+            if k.startswith("out_m_"):
+                got = mk["t"] if k == "out_m_t" else mk[k[6:]]
+            elif k.startswith("out_n_"):
+                got = nd[k[6:]]
+            else:
+                continue
+            assert np.array_equal(got, d[k], equal_nan=True), f"{name}: {k} does not reproduce"
+        d["out_n_sticky01"] = nd["sticky01"]
+        np.savez_compressed(path, **d)
+        print("added marks to", name, int(np.sum(nd["sticky01"] != 0)), "marked")
+
+
 def prn_cases():
     """prn_small.prn: the reference's save_model output for a seeded model
     (synth.prn_model of a 9 x 7 grid, 2 x 2 markers per cell, boundary
@@ -153,6 +262,12 @@ def main():
         return
     if sys.argv[1:] == ["bc"]:
         bc_cases()
+        return
+    if sys.argv[1:] == ["sticky"]:
+        sticky_cases()
+        return
+    if sys.argv[1:] == ["addmarks"]:
+        add_marks()
         return
     step = ["m2n", f"before:{DT}:0", f"after:{DT}:0", "n2m", "move"]
     cfg = synth.config_edge()

@@ -72,6 +72,11 @@ double RK_sliprs_final[H_NOD], RK_phidts_final[H_NOD], RK_sliprs_solved[H_NOD], 
 double RK_deltas[H_NOD];
 double deltasxx[H_NOD], deltasxy[H_NOD], deltaVs[H_NOD], exxs[H_NOD], exxes[H_NOD], sxxes[H_NOD];
 double prs[H_NOD], tk2[H_NOD], tol0[H_NOD], tol1[H_NOD];
+double sticky01[H_NOD];
+int nr_1, nr_0;
+double max_sigin_1, max_sigin_0, min_sigin_1, min_sigin_0, sum_sigin_1, sum2_sigin_1, sum_sigin_0;
+double sum2_sigin_0, max_visc_1, max_visc_0, min_visc_1, min_visc_0, sum_visc_1, sum2_visc_1;
+double sum_visc_0, sum2_visc_0, sigin_P1, sigin_P2, visco_P1, visco_P2;
 long int bondm[4 * H_NOD], bondn[H_BON][3], bondnum;
 double bondv[H_BON][4];
 double maxvel, maxdphi, maxnvp, dtL, dtphi, dtd, dtnvp, maxtmstep, tcnvp, tcd, tcphih, dtfactor;
@@ -139,6 +144,11 @@ static const entry table[] = {
     D(RK_sliprs_solved), D(RK_phidts_solved), D(RK_deltas),
     D(RK_sliprs0), D(RK_sliprs1), D(RK_phidts0), D(RK_phidts1),
     D(deltasxx), D(deltasxy), D(deltaVs), D(exxs), D(exxes), D(sxxes), D(prs), D(tk2), D(tol0), D(tol1),
+    /* sticky-air marks and statistics (headevps.c:34, 361-370) */
+    D(sticky01), I(nr_1), I(nr_0), S(max_sigin_1), S(max_sigin_0), S(min_sigin_1), S(min_sigin_0),
+    S(sum_sigin_1), S(sum2_sigin_1), S(sum_sigin_0), S(sum2_sigin_0), S(max_visc_1), S(max_visc_0),
+    S(min_visc_1), S(min_visc_0), S(sum_visc_1), S(sum2_visc_1), S(sum_visc_0), S(sum2_visc_0),
+    S(sigin_P1), S(sigin_P2), S(visco_P1), S(visco_P2),
     S(maxvel), S(maxdphi), S(maxnvp), S(dtL), S(dtphi), S(dtd), S(dtnvp),
     S(maxtmstep), S(tcnvp), S(tcd), S(tcphih), S(dtfactor),
     {"bondm", 'l', (void *)bondm, 4 * H_NOD}, {"bondn", 'l', (void *)bondn, 3 * H_BON},

@@ -35,7 +35,12 @@ FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "dropin_*.npz")))
 
 pytestmark = [pytest.mark.gpu]
 
-SCALARS = ("maxvel", "maxdphi", "maxnvp", "dtL", "dtphi", "dtd", "dtnvp")
+SCALARS = ("maxvel", "maxdphi", "maxnvp", "dtL", "dtphi", "dtd", "dtnvp",
+           # calc_timestep's sticky-air statistics (counts exact: _close of ints)
+           "nr_1", "nr_0", "max_sigin_1", "max_sigin_0", "min_sigin_1", "min_sigin_0", "sum_sigin_1",
+           "sum2_sigin_1", "sum_sigin_0", "sum2_sigin_0", "max_visc_1", "max_visc_0", "min_visc_1",
+           "min_visc_0", "sum_visc_1", "sum2_visc_1", "sum_visc_0", "sum2_visc_0", "sigin_P1", "sigin_P2",
+           "visco_P1", "visco_P2")
 # nodes2markers' per-node sums (interpolation.c:934-1048, mrx0 / mry0 among
 # them): marker2nodes clears them (interpolation.c:76-138, 793) and
 # nodes2markers accumulates into them and reads them; nothing else does, so

@@ -15,7 +15,7 @@ import pytest
 from velo2cycles_amd import synth
 from velo2cycles_amd.abi import NODAL_FIELDS, Context, TimestepParams, V2CError
 from oracle.pyoracle import Oracle
-from tests.util import compare, load_post_fixture, run_gpu, run_oracle
+from tests.util import compare, load_post_fixture, run_gpu, run_oracle, sticky_mismatches
 
 pytestmark = pytest.mark.gpu
 DT = synth.YEAR
@@ -47,6 +47,9 @@ def test_post_routines_match_reference_golden(gpu, path):
         _, _, s = runs[tag]
         for k in TS_KEYS:
             assert _close(r[k], s[k]), (tag, k, r[k], s[k])
+        # the sticky-air statistics of the same calc_timestep
+        bad = sticky_mismatches(ctx.sticky_stats(), s)
+        assert not bad, (tag, bad)
     ctx.close()
 
 

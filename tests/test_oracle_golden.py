@@ -9,7 +9,7 @@ import pytest
 
 from velo2cycles_amd.abi import MARKER_FIELDS, NODAL_FIELDS
 from oracle.pyoracle import Oracle
-from tests.util import load_fixture, run_oracle
+from tests.util import STICKY_KEYS, load_fixture, run_oracle
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 STEP_FIXTURES = sorted(p for p in glob.glob(os.path.join(GOLDEN, "edge_*.npz")))
@@ -65,6 +65,28 @@ def test_oracle_post_routines_match_reference_bitwise(path):
         assert r["dt"] == scal["dt"], tag
         for k in ("dtnvp", "dtd", "dtphi", "dtL", "maxvel"):
             assert r[k] == scal[k], (tag, k)
+        # the sticky-air statistics of the same calc_timestep (sticky01 is an
+        # input of these fixtures)
+        st = Oracle(cfg).sticky_stats()
+        for k in STICKY_KEYS:
+            assert st[k] == scal[k] or (np.isnan(st[k]) and np.isnan(scal[k])), (tag, k, st[k], scal[k])
+        assert st["nr_1"] + st["nr_0"] > 0
+
+
+def test_oracle_sticky_marks_and_stats_match_reference_bitwise():
+    """marker2nodes' sticky-air marks and calc_timestep's statistics
+    (interpolation.c:861-916, calc_RSF.c:364-374, 505-550) on the fixture's
+    repeated marks, midline markers, wrapped and dropped indices, markers
+    outside the grid and at NaN: marks, nu and every statistic bit for bit."""
+    cfg, ops, exp, z = load_fixture(os.path.join(GOLDEN, "sticky.npz"))
+    _, on, ret = run_oracle(cfg, ops)
+    assert np.array_equal(on["sticky01"], exp["nodal"]["sticky01"])
+    assert np.array_equal(on["nu"], exp["nodal"]["nu"], equal_nan=True)
+    st = ret["sticky"]
+    for k in STICKY_KEYS:
+        assert st[k] == float(z["g_" + k][0]), k
+    assert st["nr_1"] > 0 and st["nr_0"] > 0 and st["sigin_P1"] != 0 and st["visco_P1"] != 0
+    assert np.all(np.isfinite([st[k] for k in STICKY_KEYS]))
 
 
 BC_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "bc_*.npz")))

@@ -16,6 +16,33 @@ REL = 1e-12
 EXACT_MARKER = ("x", "y")
 
 
+# calc_timestep's sticky-air statistics (v2c_sticky_stats): counts, extrema
+# and P values, then the sums
+STICKY_EXACT = ("nr_1", "nr_0")
+STICKY_KEYS = ("nr_1", "nr_0", "max_sigin_1", "max_sigin_0", "min_sigin_1", "min_sigin_0", "sum_sigin_1",
+               "sum2_sigin_1", "sum_sigin_0", "sum2_sigin_0", "max_visc_1", "max_visc_0", "min_visc_1",
+               "min_visc_0", "sum_visc_1", "sum2_visc_1", "sum_visc_0", "sum2_visc_0", "sigin_P1", "sigin_P2",
+               "visco_P1", "visco_P2")
+
+
+def sticky_mismatches(got: dict, exp: dict, rel: float = 1e-12) -> list:
+    """Statistics that differ: counts exactly, the rest within rel (the
+    device's square root, log10 and summation order against the serial
+    loop's pow and order), NaN only against NaN."""
+    bad = []
+    for k in STICKY_KEYS:
+        a, b = float(got[k]), float(exp[k])
+        if k in STICKY_EXACT:
+            ok = a == b
+        elif np.isnan(a) or np.isnan(b):
+            ok = np.isnan(a) and np.isnan(b)
+        else:
+            ok = a == b or abs(a - b) <= rel * max(abs(a), abs(b))
+        if not ok:
+            bad.append(f"{k}: {a!r} vs {b!r}")
+    return bad
+
+
 def rel_errors(a: np.ndarray, b: np.ndarray) -> tuple[float, float, int]:
     """(max scaled error, rms scaled error, #nan mismatches) of a against b."""
     a = np.asarray(a, dtype=np.float64)
@@ -75,7 +102,9 @@ def run_oracle(cfg, ops, markers=None, nodal=None):
     for op in ops:
         name, *args = op.split(":")
         if name == "m2n":
+            # the reference's marker2nodes ends with the sticky-air marks
             o.marker2nodes()
+            o.sticky_marks()
         elif name == "n2m":
             o.nodes2markers()
         elif name == "move":
@@ -103,6 +132,7 @@ def run_oracle(cfg, ops, markers=None, nodal=None):
             rc, r = o.calc_timestep(int(args[0]))
             assert rc == 0, rc
             ret["tstep"] = r
+            ret["sticky"] = o.sticky_stats()
         elif name == "heat":
             o.temperature_markers(float(args[0]))
         else:
@@ -121,6 +151,7 @@ def run_gpu(lib, cfg, ops, markers=None, nodal=None, ctx=None):
         name, *args = op.split(":")
         if name == "m2n":
             ctx.marker2nodes()
+            ctx.sticky_marks()
         elif name == "n2m":
             ctx.nodes2markers()
         elif name == "move":
@@ -145,6 +176,7 @@ def run_gpu(lib, cfg, ops, markers=None, nodal=None, ctx=None):
             ctx.copyarrays(int(args[0]))
         elif name == "tstep":
             ret["tstep"] = ctx.calc_timestep(int(args[0]))
+            ret["sticky"] = ctx.sticky_stats()
         elif name == "heat":
             ctx.temperature_markers(float(args[0]))
         else:
@@ -174,10 +206,14 @@ def load_fixture(path):
         fields = {f: z["in_m_" + f] for f in MARKER_FIELDS}
         fields["t"] = z["in_m_t"]
         markers = MarkerArrays(len(z["in_m_x"]), fields)
-        ef = {f: z["out_m_" + f] for f in MARKER_FIELDS}
-        ef["t"] = z["out_m_t"]
-        out["markers"] = MarkerArrays(len(z["in_m_x"]), ef)
-        out["nodal"] = {k: _field(z, "out_n_" + k, grid.nodenum) for k in NODAL_FIELDS}
+        if "out_m_x" in z.files:
+            ef = {f: z["out_m_" + f] for f in MARKER_FIELDS}
+            ef["t"] = z["out_m_t"]
+            out["markers"] = MarkerArrays(len(z["in_m_x"]), ef)
+            out["nodal"] = {k: _field(z, "out_n_" + k, grid.nodenum) for k in NODAL_FIELDS}
+        else:
+            # a fixture that keeps some outputs only
+            out["nodal"] = {k[6:]: z[k] for k in z.files if k.startswith("out_n_")}
     else:
         markers = MarkerArrays(0)
     nodal = {k: np.array(_field(z, "in_n_" + k, grid.nodenum)) for k in NODAL_FIELDS}

@@ -32,7 +32,8 @@ NODAL_FIELDS = (
     "pes phis_it syields nuvp_it sliprs RK_sliprs_final RK_phidts_final RK_sliprs_solved "
     "RK_phidts_solved RK_deltas RK_sliprs0 RK_sliprs1 RK_phidts0 RK_phidts1 "
     "deltasxx deltasxy deltaVs exxs exxes sxxes prs "
-    "tk2 tol0 tol1"
+    "tk2 tol0 tol1 "
+    "sticky01"
 ).split()
 NF = len(NODAL_FIELDS)
 FIELD_ID = {n: i for i, n in enumerate(NODAL_FIELDS)}
@@ -211,6 +212,19 @@ class TimestepResult(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class StickyStats(C.Structure):
+    """v2c_sticky_stats: calc_timestep's sticky-air statistics (calc_RSF.c:364-374, 505-550)."""
+    _fields_ = [("nr_1", C.c_int), ("nr_0", C.c_int)] + [
+        (n, C.c_double) for n in ("max_sigin_1", "max_sigin_0", "min_sigin_1", "min_sigin_0",
+                                  "sum_sigin_1", "sum2_sigin_1", "sum_sigin_0", "sum2_sigin_0",
+                                  "max_visc_1", "max_visc_0", "min_visc_1", "min_visc_0",
+                                  "sum_visc_1", "sum2_visc_1", "sum_visc_0", "sum2_visc_0",
+                                  "sigin_P1", "sigin_P2", "visco_P1", "visco_P2")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 def load_lib(path: str = LIB_PATH) -> C.CDLL:
     if not os.path.exists(path):
         raise RuntimeError(f"libv2c.so not built at {path}: run __graft_entry__.build()")
@@ -262,6 +276,7 @@ def load_lib(path: str = LIB_PATH) -> C.CDLL:
                        ("v2c_set_boundary", [P(C.c_long), P(C.c_long), P(C.c_double), C.c_long]),
                        ("v2c_vxvy_boundarycorrection", []), ("v2c_P_boundarycorrection", []), ("v2c_tkrecalc", []),
                        ("v2c_calc_timestep", [C.c_int, P(TimestepParams), P(TimestepResult)]),
+                       ("v2c_sticky_marks", []), ("v2c_calc_sticky_stats", [P(StickyStats)]),
                        ("v2c_reset_timing", [])]:
         fn = getattr(lib, name)
         fn.argtypes = [C.c_void_p] + args
@@ -408,6 +423,14 @@ class Context:
         p = params or TimestepParams.default()
         r = TimestepResult()
         self._check(self.lib.v2c_calc_timestep(self.h, oldornew, C.byref(p), C.byref(r)), "calc_timestep")
+        return r.as_dict()
+
+    def sticky_marks(self):
+        self._check(self.lib.v2c_sticky_marks(self.h), "sticky_marks")
+
+    def sticky_stats(self) -> dict:
+        r = StickyStats()
+        self._check(self.lib.v2c_calc_sticky_stats(self.h, C.byref(r)), "calc_sticky_stats")
         return r.as_dict()
 
     def nodes2markers(self):

@@ -14,6 +14,7 @@
  *   interpolate_basic2center()      momentum_continuity_iterate.c:71
  *   copyarrays()                    momentum_continuity_iterate.c:128
  *   calc_timestep()                 momentum_continuity_iterate.c:44, 92
+ *                                   (with its sticky-air statistics)
  *   vxvy_boundarycorrection(), P_boundarycorrection(), tkrecalc()
  *                                   interpolation.c:919-924, temperature_iterate.c
  *
@@ -88,6 +89,11 @@ extern double exxsum[], wtvxsum[], vxsum[], wtvysum[], vysum[], mrx0[], mry0[];
 extern double pes[], phis_it[], syields[], nuvp_it[], sliprs[], RK_sliprs_final[], RK_phidts_final[];
 extern double RK_sliprs_solved[], RK_phidts_solved[], RK_deltas[];
 extern double deltasxx[], deltasxy[], deltaVs[], exxs[], exxes[], sxxes[], prs[], tk2[], tol0[], tol1[];
+extern double sticky01[];
+extern int nr_1, nr_0;
+extern double max_sigin_1, max_sigin_0, min_sigin_1, min_sigin_0, sum_sigin_1, sum2_sigin_1, sum_sigin_0;
+extern double sum2_sigin_0, max_visc_1, max_visc_0, min_visc_1, min_visc_0, sum_visc_1, sum2_visc_1;
+extern double sum_visc_0, sum2_visc_0, sigin_P1, sigin_P2, visco_P1, visco_P2;
 extern long int bondm[], bondn[][3], bondnum;
 extern double bondv[][4];
 extern double maxvel, maxdphi, maxnvp, dtL, dtphi, dtd, dtnvp, maxtmstep, tcnvp, tcd, tcphih, dtfactor;
@@ -106,7 +112,7 @@ static double *nodal_ptr(int f)
         wtvysum, vysum, mrx0, mry0,
         pes, phis_it, syields, nuvp_it, sliprs, RK_sliprs_final, RK_phidts_final, RK_sliprs_solved,
         RK_phidts_solved, RK_deltas, NULL, NULL, NULL, NULL,
-        deltasxx, deltasxy, deltaVs, exxs, exxes, sxxes, prs, tk2, tol0, tol1};
+        deltasxx, deltasxy, deltaVs, exxs, exxes, sxxes, prs, tk2, tol0, tol1, sticky01};
     typedef char one_pointer_per_field[sizeof tab / sizeof tab[0] == V2C_NF ? 1 : -1];
     (void)sizeof(one_pointer_per_field);
     return f >= 0 && f < V2C_NF ? tab[f] : NULL;
@@ -248,10 +254,10 @@ void v2c_dropin_before_stokes(void) { fetch_newer(STOKES_READS, NOF(STOKES_READS
 void v2c_dropin_after_stokes(void) { v2c_dropin_host_wrote(STOKES_WRITES, NOF(STOKES_WRITES)); }
 
 /* What the per-step diagnostics after copyarrays
- * (momentum_continuity_iterate.c:129-170: sxxs, sxy, nu) and postproc
- * (output_RSF.c:2-72: nu, sliprs, phis_it, syields) read. */
+ * (momentum_continuity_iterate.c:129-170: sxxs, sxy, nu, sticky01) and
+ * postproc (output_RSF.c:2-72: nu, sliprs, phis_it, syields) read. */
 static const int OUTPUT_READS[] = {V2C_N_sxxs, V2C_N_sxy, V2C_N_nu, V2C_N_sliprs, V2C_N_phis_it,
-                                   V2C_N_syields};
+                                   V2C_N_syields, V2C_N_sticky01};
 void v2c_dropin_before_output(void) { fetch_newer(OUTPUT_READS, NOF(OUTPUT_READS)); }
 
 
@@ -345,10 +351,12 @@ void marker2nodes(void)
                               V2C_N_cohesion, V2C_N_dro, V2C_N_drp, V2C_N_cp, V2C_N_kt, V2C_N_ht,
                               V2C_N_tk, V2C_N_mrx, V2C_N_mry, V2C_N_mvx, V2C_N_mvy, V2C_N_pe,
                               V2C_N_pen, V2C_N_nu_m2c, V2C_N_nd_m2c, V2C_N_mvxn, V2C_N_mvyn,
-                              V2C_N_mrx0, V2C_N_mry0};
+                              V2C_N_mrx0, V2C_N_mry0, V2C_N_sticky01};
     v2c_set_time(g_ctx, timesum, timestep);
     send_newer(in, NOF(in));
     die_on(v2c_marker2nodes(g_ctx), "marker2nodes");
+    /* the routine's sticky-air marks (interpolation.c:861-916) */
+    die_on(v2c_sticky_marks(g_ctx), "sticky_marks");
     /* the routine's boundary-condition tail (interpolation.c:918-925) */
     if (bondnum > 1) {
         die_on(v2c_vxvy_boundarycorrection(g_ctx), "vxvy_boundarycorrection");
@@ -489,14 +497,16 @@ void copyarrays(int rk)
     device_wrote(out, NOF(out));
 }
 
-/* The sticky-air diagnostics of the reference routine (calc_RSF.c:506-551)
- * only print; they are not computed here. */
+/* The time step and, into the reference's globals, the sticky-air
+ * statistics (calc_RSF.c:364-374, 505-550) that host code after copyarrays
+ * prints (momentum_continuity_iterate.c:129-170). */
 double calc_timestep(int oldornew)
 {
     static const int in[] = {V2C_N_sppes, V2C_N_prs, V2C_N_bm, V2C_N_gg, V2C_N_Ls, V2C_N_as, V2C_N_bs,
                              V2C_N_V0s, V2C_N_RK_sliprs_final, V2C_N_RK_sliprs_solved, V2C_N_nu_m2c,
                              V2C_N_sxxes, V2C_N_sxye, V2C_N_nu, V2C_N_vx, V2C_N_mvx, V2C_N_vy, V2C_N_mvy,
-                             V2C_N_phis, V2C_N_phis_it};
+                             V2C_N_phis, V2C_N_phis_it, V2C_N_sxxs, V2C_N_sxy, V2C_N_sticky01};
+    v2c_sticky_stats st;
     v2c_timestep_params p;
     v2c_timestep_result r;
     v2c_timestep_params_default(&p);
@@ -510,6 +520,17 @@ double calc_timestep(int oldornew)
         exit(0);
     }
     maxvel = r.maxvel; dtnvp = r.dtnvp; dtd = r.dtd; dtphi = r.dtphi; dtL = r.dtL;
+    die_on(v2c_calc_sticky_stats(g_ctx, &st), "calc_sticky_stats");
+    nr_1 = st.nr_1; nr_0 = st.nr_0;
+    max_sigin_1 = st.max_sigin_1; max_sigin_0 = st.max_sigin_0;
+    min_sigin_1 = st.min_sigin_1; min_sigin_0 = st.min_sigin_0;
+    sum_sigin_1 = st.sum_sigin_1; sum2_sigin_1 = st.sum2_sigin_1;
+    sum_sigin_0 = st.sum_sigin_0; sum2_sigin_0 = st.sum2_sigin_0;
+    max_visc_1 = st.max_visc_1; max_visc_0 = st.max_visc_0;
+    min_visc_1 = st.min_visc_1; min_visc_0 = st.min_visc_0;
+    sum_visc_1 = st.sum_visc_1; sum2_visc_1 = st.sum2_visc_1;
+    sum_visc_0 = st.sum_visc_0; sum2_visc_0 = st.sum2_visc_0;
+    sigin_P1 = st.sigin_P1; sigin_P2 = st.sigin_P2; visco_P1 = st.visco_P1; visco_P2 = st.visco_P2;
     printf("===================================\n");
     printf(">>> dt_f =%e s, Max Slip-Vel=%e m/s, Max Vx=%e m/s", r.dt, r.maxvel, r.maxVx);
     printf("\n>>> dtnvp=%e s, dtd =%e s, dtphi =%e s, dtL=%e s \n", r.dtnvp, r.dtd, r.dtphi, r.dtL);

@@ -383,7 +383,7 @@ static const char *const kNodalNames[V2C_NF] = {
     "RK_sliprs_solved", "RK_phidts_solved", "RK_deltas", "RK_sliprs0", "RK_sliprs1",
     "RK_phidts0", "RK_phidts1",
     "deltasxx", "deltasxy", "deltaVs", "exxs", "exxes", "sxxes", "prs",
-    "tk2", "tol0", "tol1"};
+    "tk2", "tol0", "tol1", "sticky01"};
 
 const char *v2c_nodal_name(int field)
 {

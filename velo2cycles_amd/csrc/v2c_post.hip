@@ -1,7 +1,8 @@
 // v2c_post.hip -- the reference's per-node routines between the Stokes solve
 // and the next marker2nodes, on the device-resident nodal fields:
 // recompute_stress_strainrate, interpolate_center2basic(_old),
-// interpolate_basic2center, copyarrays and calc_timestep.
+// interpolate_basic2center, copyarrays and calc_timestep, and the
+// sticky-air marks of marker2nodes and statistics of calc_timestep.
 //
 // Every node is one thread in node order (m3 = m1*ynumy + m2).  No kernel
 // writes a field it reads at another node, so each routine is one launch.
@@ -532,6 +533,158 @@ __global__ void k_post_clear(int *p, int n)
     if (i < n) p[i] = 0;
 }
 
+// ---- sticky-air marks (interpolation.c:861-916) ----------------------------------
+// The reference's serial loop leaves at each node the type of the last marker
+// (highest index) that marks it; win[node] = 1 + that index, from an atomic
+// maximum over the markers, gives the same node values in any order.
+__global__ __launch_bounds__(POST_BLOCK) void k_sticky_clear(uint32_t *__restrict__ win, long n)
+{
+    const long i = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (i < n) win[i] = 0u;
+}
+
+__global__ __launch_bounds__(POST_BLOCK) void k_sticky_marks(KArgs a, uint32_t *__restrict__ win)
+{
+    const long i = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const int t = a.t[i];
+    if (t != 1 && t != 2) return;
+    const double x = MK(a, MF_X, i), y = MK(a, MF_Y, i);
+    const Axis4 X = find_x(a.g, x), Y = find_y(a.g, y);
+    const double celdx = (x - X.g[1]) / (X.g[2] - X.g[1]);
+    const double celdy = (y - Y.g[1]) / (Y.g[2] - Y.g[1]);
+    const long ny = a.g.ynumy;
+    long m3 = (long)X.s * ny + Y.s + 1;
+    if (celdx < 0.5 && celdy < 0.5) {
+    } else if (celdx < 0.5 && celdy > 0.5) {
+        m3 += 1;
+    } else if (celdx > 0.5 && celdy < 0.5) {
+        m3 += ny;
+    } else if (celdx > 0.5 && celdy > 0.5) {
+        m3 += ny + 1;
+    } else {
+        return;
+    }
+    if (m3 < a.nodenum) atomicMax(&win[m3], (uint32_t)(i + 1));
+}
+
+__global__ __launch_bounds__(POST_BLOCK) void k_sticky_nodes(KArgs a, const uint32_t *__restrict__ win)
+{
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (m3 >= a.nodenum) return;
+    const uint32_t w = win[m3];
+    NF_(sticky01)[m3] = w ? (double)a.t[w - 1] : 0.0;
+}
+
+// ---- calc_timestep's sticky-air statistics (calc_RSF.c:364-374, 505-550) ----------
+// Each thread folds a run of STK_RUN nodes in node order; the runs are then
+// composed in order.  Counts and sums add; extrema compose as running
+// MAXV/MINV folds and the P1/P2 assignments as constants, so those come out
+// as the serial loop's.  Set g = 0 gathers sticky01 == 1 (the reference's
+// "_0" members), g = 1 sticky01 == 2 ("_1").
+constexpr int STK_RUN = 64;
+constexpr int STK_FIN = 64;
+
+struct StickyAcc {
+    int nr[2];
+    // f[4g + 0..3]: max sigin, min sigin, max nu, min nu of set g;
+    // f[8..11]: sigin_P1, visco_P1, sigin_P2, visco_P2 (assignments)
+    Fold f[12];
+    // s[4g + 0..3]: sum sigin, sum sigin^2, sum log10 nu, sum (log10 nu)^2
+    double s[8];
+};
+
+__device__ __forceinline__ void sticky_clear(StickyAcc &A)
+{
+    A.nr[0] = A.nr[1] = 0;
+    for (int k = 0; k < 12; k++) A.f[k] = Fold{0, 0.0};
+    for (int k = 0; k < 8; k++) A.s[k] = 0.0;
+}
+
+// A <- B after A
+__device__ __forceinline__ void sticky_then(StickyAcc &A, const StickyAcc &B)
+{
+    A.nr[0] += B.nr[0];
+    A.nr[1] += B.nr[1];
+    for (int k = 0; k < 12; k++)
+        A.f[k] = (k < 8 && (k & 1) == 0) ? fold_then<true>(A.f[k], B.f[k]) : fold_then<false>(A.f[k], B.f[k]);
+    for (int k = 0; k < 8; k++) A.s[k] += B.s[k];
+}
+
+__global__ __launch_bounds__(POST_BLOCK) void k_sticky_part(KArgs a, StickyAcc *__restrict__ part, long np)
+{
+    const long r = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (r >= np) return;
+    StickyAcc A;
+    sticky_clear(A);
+    const long ny = a.g.ynumy, e = min((r + 1) * STK_RUN, a.nodenum);
+    const double *sxxs = NF_(sxxs), *sxy = NF_(sxy), *nu = NF_(nu), *st = NF_(sticky01);
+    for (long m3 = r * STK_RUN; m3 < e; m3++) {
+        const long m2 = m3 % ny, m1 = m3 / ny;
+        if (!(m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1)) continue;
+        const double sig = sqrt(sxxs[m3] * sxxs[m3] + sxy[m3] * sxy[m3]);
+        const double v = nu[m3], mark = st[m3];
+        if (mark == 1 && m1 == 50 && m2 == 50) { A.f[8] = Fold{2, sig}; A.f[9] = Fold{2, v}; }
+        if (mark == 1 && m1 == 500 && m2 == 50) { A.f[10] = Fold{2, sig}; A.f[11] = Fold{2, v}; }
+        const int g = mark == 2 ? 1 : mark == 1 ? 0 : -1;
+        if (g < 0) continue;
+        A.nr[g]++;
+        A.f[4 * g + 0] = fold_then<true>(A.f[4 * g + 0], fold_of<true>(sig));
+        A.f[4 * g + 1] = fold_then<false>(A.f[4 * g + 1], fold_of<false>(sig));
+        A.f[4 * g + 2] = fold_then<true>(A.f[4 * g + 2], fold_of<true>(v));
+        A.f[4 * g + 3] = fold_then<false>(A.f[4 * g + 3], fold_of<false>(v));
+        const double l = log10(v);
+        A.s[4 * g + 0] += sig;
+        A.s[4 * g + 1] += sig * sig;
+        A.s[4 * g + 2] += l;
+        A.s[4 * g + 3] += l * l;
+    }
+    part[r] = A;
+}
+
+__global__ __launch_bounds__(STK_FIN) void k_sticky_final(const StickyAcc *__restrict__ part, long np,
+                                                          v2c_sticky_stats *__restrict__ out)
+{
+    __shared__ StickyAcc S[STK_FIN];
+    const int t = threadIdx.x;
+    const long per = (np + STK_FIN - 1) / STK_FIN, b0 = t * per, b1 = min(b0 + per, np);
+    StickyAcc A;
+    sticky_clear(A);
+    for (long b = b0; b < b1; b++) sticky_then(A, part[b]);
+    S[t] = A;
+    __syncthreads();
+    for (int s = 1; s < STK_FIN; s <<= 1) {
+        if ((t & (2 * s - 1)) == 0) sticky_then(S[t], S[t + s]);
+        __syncthreads();
+    }
+    if (t != 0) return;
+    const StickyAcc &R = S[0];
+    v2c_sticky_stats o;
+    o.nr_1 = R.nr[1];
+    o.nr_0 = R.nr[0];
+    o.max_sigin_1 = fold_apply<true>(R.f[4], 0.0);
+    o.min_sigin_1 = fold_apply<false>(R.f[5], 1e30);
+    o.max_visc_1 = fold_apply<true>(R.f[6], 0.0);
+    o.min_visc_1 = fold_apply<false>(R.f[7], 1e30);
+    o.max_sigin_0 = fold_apply<true>(R.f[0], 0.0);
+    o.min_sigin_0 = fold_apply<false>(R.f[1], 1e30);
+    o.max_visc_0 = fold_apply<true>(R.f[2], 0.0);
+    o.min_visc_0 = fold_apply<false>(R.f[3], 1e30);
+    o.sum_sigin_1 = R.s[4];
+    o.sum2_sigin_1 = R.s[5];
+    o.sum_visc_1 = R.s[6];
+    o.sum2_visc_1 = R.s[7];
+    o.sum_sigin_0 = R.s[0];
+    o.sum2_sigin_0 = R.s[1];
+    o.sum_visc_0 = R.s[2];
+    o.sum2_visc_0 = R.s[3];
+    o.sigin_P1 = fold_apply<true>(R.f[8], 0.0);
+    o.visco_P1 = fold_apply<true>(R.f[9], 0.0);
+    o.sigin_P2 = fold_apply<true>(R.f[10], 0.0);
+    o.visco_P2 = fold_apply<true>(R.f[11], 0.0);
+    *out = o;
+}
+
 #undef NF_
 
 }  // namespace v2c
@@ -657,6 +810,47 @@ int v2c_calc_timestep(v2c_ctx *c, int oldornew, const v2c_timestep_params *p, v2
     r->dtL = h[4];
     r->maxvel = h[5];
     r->maxVx = h[6];
+    return 0;
+}
+
+int v2c_sticky_marks(v2c_ctx *c)
+{
+    if (!c) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = post_scratch(c, sizeof(uint32_t) * (size_t)c->nodenum);
+    if (rc) return rc;
+    uint32_t *win = (uint32_t *)c->d_post;
+    const KArgs a = make_args(c);
+    const unsigned nbn = blocks_for(c->nodenum, POST_BLOCK);
+    rc = launch(c, V2C_K_POST, [&] {
+        hipLaunchKernelGGL(k_sticky_clear, dim3(nbn), dim3(POST_BLOCK), 0, c->stream, win, c->nodenum);
+        if (c->n > 0)
+            hipLaunchKernelGGL(k_sticky_marks, dim3(blocks_for(c->n, POST_BLOCK)), dim3(POST_BLOCK), 0, c->stream,
+                               a, win);
+        hipLaunchKernelGGL(k_sticky_nodes, dim3(nbn), dim3(POST_BLOCK), 0, c->stream, a, win);
+    });
+    if (rc) return rc;
+    return finish(c);
+}
+
+int v2c_calc_sticky_stats(v2c_ctx *c, v2c_sticky_stats *out)
+{
+    if (!c || !out) return V2C_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long np = (c->nodenum + STK_RUN - 1) / STK_RUN;
+    int rc = post_scratch(c, sizeof(StickyAcc) * (size_t)np + sizeof(v2c_sticky_stats));
+    if (rc) return rc;
+    StickyAcc *part = (StickyAcc *)c->d_post;
+    v2c_sticky_stats *dout = (v2c_sticky_stats *)(part + np);
+    const KArgs a = make_args(c);
+    rc = launch(c, V2C_K_POST, [&] {
+        hipLaunchKernelGGL(k_sticky_part, dim3(blocks_for(np, POST_BLOCK)), dim3(POST_BLOCK), 0, c->stream, a,
+                           part, np);
+        hipLaunchKernelGGL(k_sticky_final, dim3(1), dim3(STK_FIN), 0, c->stream, part, np, dout);
+    });
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, dout, sizeof *out, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
