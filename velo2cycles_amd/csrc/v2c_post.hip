@@ -543,12 +543,13 @@ __global__ __launch_bounds__(POST_BLOCK) void k_sticky_clear(uint32_t *__restric
     if (i < n) win[i] = 0u;
 }
 
-__global__ __launch_bounds__(POST_BLOCK) void k_sticky_marks(KArgs a, uint32_t *__restrict__ win)
+// STK_MPT markers per thread: their types in one 8-byte load.
+constexpr int STK_MPT = 8;
+
+// The node that marker i marks, or -1 (a cell midline, or an index at or
+// beyond nodenum).
+__device__ __forceinline__ long sticky_node(const KArgs &a, long i)
 {
-    const long i = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
-    if (i >= a.n) return;
-    const int t = a.t[i];
-    if (t != 1 && t != 2) return;
     const double x = MK(a, MF_X, i), y = MK(a, MF_Y, i);
     const Axis4 X = find_x(a.g, x), Y = find_y(a.g, y);
     const double celdx = (x - X.g[1]) / (X.g[2] - X.g[1]);
@@ -563,9 +564,39 @@ __global__ __launch_bounds__(POST_BLOCK) void k_sticky_marks(KArgs a, uint32_t *
     } else if (celdx > 0.5 && celdy > 0.5) {
         m3 += ny + 1;
     } else {
-        return;
+        return -1;
     }
-    if (m3 < a.nodenum) atomicMax(&win[m3], (uint32_t)(i + 1));
+    return m3 < a.nodenum ? m3 : -1;
+}
+
+// A thread's markers are consecutive, and neighbours often mark one node: one
+// atomic per run of markers marking the same node, with the run's last
+// (highest) index.
+__global__ __launch_bounds__(POST_BLOCK) void k_sticky_marks(KArgs a, uint32_t *__restrict__ win)
+{
+    const long i0 = ((long)blockIdx.x * POST_BLOCK + threadIdx.x) * STK_MPT;
+    if (i0 >= a.n) return;
+    int8_t tt[STK_MPT];
+    if (i0 + STK_MPT <= a.n) {
+        const uint2 w = *reinterpret_cast<const uint2 *>(a.t + i0);   // i0 is a multiple of 8
+        __builtin_memcpy(tt, &w, sizeof tt);
+    } else {
+        for (int k = 0; k < STK_MPT; k++) tt[k] = i0 + k < a.n ? a.t[i0 + k] : (int8_t)0;
+    }
+    long cur = -1;
+    uint32_t top = 0;
+#pragma unroll
+    for (int k = 0; k < STK_MPT; k++) {
+        if (tt[k] != 1 && tt[k] != 2) continue;
+        const long m3 = sticky_node(a, i0 + k);
+        if (m3 < 0) continue;
+        if (m3 != cur) {
+            if (cur >= 0) atomicMax(&win[cur], top);
+            cur = m3;
+        }
+        top = (uint32_t)(i0 + k + 1);
+    }
+    if (cur >= 0) atomicMax(&win[cur], top);
 }
 
 __global__ __launch_bounds__(POST_BLOCK) void k_sticky_nodes(KArgs a, const uint32_t *__restrict__ win)
@@ -577,13 +608,13 @@ __global__ __launch_bounds__(POST_BLOCK) void k_sticky_nodes(KArgs a, const uint
 }
 
 // ---- calc_timestep's sticky-air statistics (calc_RSF.c:364-374, 505-550) ----------
-// Each thread folds a run of STK_RUN nodes in node order; the runs are then
-// composed in order.  Counts and sums add; extrema compose as running
-// MAXV/MINV folds and the P1/P2 assignments as constants, so those come out
-// as the serial loop's.  Set g = 0 gathers sticky01 == 1 (the reference's
-// "_0" members), g = 1 sticky01 == 2 ("_1").
-constexpr int STK_RUN = 64;
-constexpr int STK_FIN = 64;
+// One node per thread; the per-node contributions are composed in node order:
+// over the lanes of a wavefront (a shuffle tree), over the wavefronts of a
+// block, then over the blocks.  Counts and sums add; extrema compose as
+// running MAXV/MINV folds and the P1/P2 assignments as constants, so those
+// come out as the serial loop's.  Set g = 0 gathers sticky01 == 1 (the
+// reference's "_0" members), g = 1 sticky01 == 2 ("_1").
+constexpr int STK_FIN = 1024;
 
 struct StickyAcc {
     int nr[2];
@@ -597,7 +628,9 @@ struct StickyAcc {
 __device__ __forceinline__ void sticky_clear(StickyAcc &A)
 {
     A.nr[0] = A.nr[1] = 0;
+#pragma unroll
     for (int k = 0; k < 12; k++) A.f[k] = Fold{0, 0.0};
+#pragma unroll
     for (int k = 0; k < 8; k++) A.s[k] = 0.0;
 }
 
@@ -606,59 +639,89 @@ __device__ __forceinline__ void sticky_then(StickyAcc &A, const StickyAcc &B)
 {
     A.nr[0] += B.nr[0];
     A.nr[1] += B.nr[1];
+#pragma unroll
     for (int k = 0; k < 12; k++)
         A.f[k] = (k < 8 && (k & 1) == 0) ? fold_then<true>(A.f[k], B.f[k]) : fold_then<false>(A.f[k], B.f[k]);
+#pragma unroll
     for (int k = 0; k < 8; k++) A.s[k] += B.s[k];
 }
 
-__global__ __launch_bounds__(POST_BLOCK) void k_sticky_part(KArgs a, StickyAcc *__restrict__ part, long np)
+// Lane 0 ends with the composition of the wavefront's lanes in lane order.
+__device__ __forceinline__ void sticky_wave(StickyAcc &A, int lane)
 {
-    const long r = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
-    if (r >= np) return;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        StickyAcc B;
+        B.nr[0] = __shfl_down(A.nr[0], d);
+        B.nr[1] = __shfl_down(A.nr[1], d);
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            B.f[k].kind = __shfl_down(A.f[k].kind, d);
+            B.f[k].v = __shfl_down(A.f[k].v, d);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) B.s[k] = __shfl_down(A.s[k], d);
+        if ((lane & (2 * d - 1)) == 0) sticky_then(A, B);
+    }
+}
+
+// One node of set G (constant indices keep the accumulator in registers).
+template <int G>
+__device__ __forceinline__ void sticky_node_in(StickyAcc &A, double sig, double v)
+{
+    A.nr[G] = 1;
+    A.f[4 * G + 0] = fold_of<true>(sig);
+    A.f[4 * G + 1] = fold_of<false>(sig);
+    A.f[4 * G + 2] = fold_of<true>(v);
+    A.f[4 * G + 3] = fold_of<false>(v);
+    const double l = log10(v);
+    A.s[4 * G + 0] = sig;
+    A.s[4 * G + 1] = sig * sig;
+    A.s[4 * G + 2] = l;
+    A.s[4 * G + 3] = l * l;
+}
+
+__global__ __launch_bounds__(POST_BLOCK) void k_sticky_part(KArgs a, StickyAcc *__restrict__ part)
+{
+    __shared__ StickyAcc W[POST_BLOCK / WAVE];
+    const long m3 = (long)blockIdx.x * POST_BLOCK + threadIdx.x;
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
     StickyAcc A;
     sticky_clear(A);
-    const long ny = a.g.ynumy, e = min((r + 1) * STK_RUN, a.nodenum);
-    const double *sxxs = NF_(sxxs), *sxy = NF_(sxy), *nu = NF_(nu), *st = NF_(sticky01);
-    for (long m3 = r * STK_RUN; m3 < e; m3++) {
-        const long m2 = m3 % ny, m1 = m3 / ny;
-        if (!(m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1)) continue;
-        const double sig = sqrt(sxxs[m3] * sxxs[m3] + sxy[m3] * sxy[m3]);
-        const double v = nu[m3], mark = st[m3];
+    const long ny = a.g.ynumy;
+    const long m2 = m3 % ny, m1 = m3 / ny;
+    if (m3 < a.nodenum && m1 >= 1 && m2 >= 1 && m1 < a.g.xnumx - 1 && m2 < ny - 1) {
+        const double sx = NF_(sxxs)[m3], sy = NF_(sxy)[m3];
+        const double sig = sqrt(sx * sx + sy * sy);
+        const double v = NF_(nu)[m3], mark = NF_(sticky01)[m3];
         if (mark == 1 && m1 == 50 && m2 == 50) { A.f[8] = Fold{2, sig}; A.f[9] = Fold{2, v}; }
         if (mark == 1 && m1 == 500 && m2 == 50) { A.f[10] = Fold{2, sig}; A.f[11] = Fold{2, v}; }
-        const int g = mark == 2 ? 1 : mark == 1 ? 0 : -1;
-        if (g < 0) continue;
-        A.nr[g]++;
-        A.f[4 * g + 0] = fold_then<true>(A.f[4 * g + 0], fold_of<true>(sig));
-        A.f[4 * g + 1] = fold_then<false>(A.f[4 * g + 1], fold_of<false>(sig));
-        A.f[4 * g + 2] = fold_then<true>(A.f[4 * g + 2], fold_of<true>(v));
-        A.f[4 * g + 3] = fold_then<false>(A.f[4 * g + 3], fold_of<false>(v));
-        const double l = log10(v);
-        A.s[4 * g + 0] += sig;
-        A.s[4 * g + 1] += sig * sig;
-        A.s[4 * g + 2] += l;
-        A.s[4 * g + 3] += l * l;
+        if (mark == 2) sticky_node_in<1>(A, sig, v);
+        else if (mark == 1) sticky_node_in<0>(A, sig, v);
     }
-    part[r] = A;
+    sticky_wave(A, lane);
+    if (lane == 0) W[wv] = A;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    StickyAcc R = W[0];
+    for (int w = 1; w < POST_BLOCK / WAVE; w++) sticky_then(R, W[w]);
+    part[blockIdx.x] = R;
 }
 
 __global__ __launch_bounds__(STK_FIN) void k_sticky_final(const StickyAcc *__restrict__ part, long np,
                                                           v2c_sticky_stats *__restrict__ out)
 {
-    __shared__ StickyAcc S[STK_FIN];
-    const int t = threadIdx.x;
+    __shared__ StickyAcc W[STK_FIN / WAVE];
+    const int t = threadIdx.x, lane = t % WAVE;
     const long per = (np + STK_FIN - 1) / STK_FIN, b0 = t * per, b1 = min(b0 + per, np);
     StickyAcc A;
     sticky_clear(A);
     for (long b = b0; b < b1; b++) sticky_then(A, part[b]);
-    S[t] = A;
+    sticky_wave(A, lane);
+    if (lane == 0) W[t / WAVE] = A;
     __syncthreads();
-    for (int s = 1; s < STK_FIN; s <<= 1) {
-        if ((t & (2 * s - 1)) == 0) sticky_then(S[t], S[t + s]);
-        __syncthreads();
-    }
     if (t != 0) return;
-    const StickyAcc &R = S[0];
+    StickyAcc R = W[0];
+    for (int w = 1; w < STK_FIN / WAVE; w++) sticky_then(R, W[w]);
     v2c_sticky_stats o;
     o.nr_1 = R.nr[1];
     o.nr_0 = R.nr[0];
@@ -825,8 +888,8 @@ int v2c_sticky_marks(v2c_ctx *c)
     rc = launch(c, V2C_K_POST, [&] {
         hipLaunchKernelGGL(k_sticky_clear, dim3(nbn), dim3(POST_BLOCK), 0, c->stream, win, c->nodenum);
         if (c->n > 0)
-            hipLaunchKernelGGL(k_sticky_marks, dim3(blocks_for(c->n, POST_BLOCK)), dim3(POST_BLOCK), 0, c->stream,
-                               a, win);
+            hipLaunchKernelGGL(k_sticky_marks, dim3(blocks_for((c->n + STK_MPT - 1) / STK_MPT, POST_BLOCK)),
+                               dim3(POST_BLOCK), 0, c->stream, a, win);
         hipLaunchKernelGGL(k_sticky_nodes, dim3(nbn), dim3(POST_BLOCK), 0, c->stream, a, win);
     });
     if (rc) return rc;
@@ -837,15 +900,14 @@ int v2c_calc_sticky_stats(v2c_ctx *c, v2c_sticky_stats *out)
 {
     if (!c || !out) return V2C_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
-    const long np = (c->nodenum + STK_RUN - 1) / STK_RUN;
+    const long np = blocks_for(c->nodenum, POST_BLOCK);
     int rc = post_scratch(c, sizeof(StickyAcc) * (size_t)np + sizeof(v2c_sticky_stats));
     if (rc) return rc;
     StickyAcc *part = (StickyAcc *)c->d_post;
     v2c_sticky_stats *dout = (v2c_sticky_stats *)(part + np);
     const KArgs a = make_args(c);
     rc = launch(c, V2C_K_POST, [&] {
-        hipLaunchKernelGGL(k_sticky_part, dim3(blocks_for(np, POST_BLOCK)), dim3(POST_BLOCK), 0, c->stream, a,
-                           part, np);
+        hipLaunchKernelGGL(k_sticky_part, dim3((unsigned)np), dim3(POST_BLOCK), 0, c->stream, a, part);
         hipLaunchKernelGGL(k_sticky_final, dim3(1), dim3(STK_FIN), 0, c->stream, part, np, dout);
     });
     if (rc) return rc;
