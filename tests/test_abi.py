@@ -56,6 +56,24 @@ def test_struct_layout_matches_c():
                      ctypes.sizeof(abi.Materials), ctypes.sizeof(abi.Markers)]
 
 
+def test_result_structs_match_c():
+    """sizeof and field offsets of the result structs (v2c_sticky_stats,
+    v2c_timestep_params / _result) equal the ctypes mirrors."""
+    names = [n for n, _ in abi.StickyStats._fields_]
+    offs = ",".join(f"offsetof(v2c_sticky_stats,{n})" for n in names)
+    src = ('#include "v2c.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+           'int main(void){size_t o[]={' + offs + '};'
+           'printf("%zu %zu %zu",sizeof(v2c_sticky_stats),sizeof(v2c_timestep_params),sizeof(v2c_timestep_result));'
+           'for(unsigned i=0;i<sizeof o/sizeof o[0];i++)printf(" %zu",o[i]);printf("\\n");return 0;}')
+    exe = "/tmp/v2c_result_sizes"
+    subprocess.run(["gcc", "-x", "c", "-I", os.path.dirname(HEADER), "-o", exe, "-"], input=src,
+                   text=True, check=True)
+    got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
+    assert got[:3] == [ctypes.sizeof(abi.StickyStats), ctypes.sizeof(abi.TimestepParams),
+                       ctypes.sizeof(abi.TimestepResult)]
+    assert got[3:] == [getattr(abi.StickyStats, n).offset for n in names]
+
+
 def test_prn_struct_matches_c():
     """The ctypes mirror of v2c_prn has the C struct's size and field offsets."""
     from velo2cycles_amd.prn import Prn
