@@ -11,7 +11,11 @@ hot-path routines, compiled in place from /root/reference/src
 
 Each fixture holds its complete input (grid, flags, rock table, markers, nodal
 arrays) and the reference's output after the listed ops, so the tests never
-need the reference or the generator that produced the inputs.
+need the reference or the generator that produced the inputs.  The committed
+edge_*.npz step fixtures hold inputs from an earlier version of
+velo2cycles_amd/synth.py; `addmarks` re-runs the reference on those stored
+inputs (so the default run would write equally valid fixtures from today's
+synthetic inputs rather than reproduce those files).
 """
 from __future__ import annotations
 
